@@ -1,0 +1,176 @@
+"""Benchmark: Recon 1920x1080 YUV420P10 -> YUV444P10, algo 2, bp 0.5.
+
+One step = one pass of the Recon hot path over a batch of synthetic frames
+that are already resident in HBM (default 240 frames, one scene per frame).
+Prints ONE JSON line (rank 0).  The CPU baseline is the oracle (scalar C
+restatement of the reference) timed on a bounded sample of the same
+workload on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "vapoursynth-lgcr_amd"))
+sys.path.insert(0, ROOT)
+
+METRIC = "Recon 1080p420 10-bit fps, HBM GB/s (% peak); serving p50/p99 frame latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(w, h, cw, ch, bps):
+    """Compulsory plane I/O per frame: Y, U, V read once; U, V 4:4:4 written
+    once; same-size Recon shares Y (not written)."""
+    return bps * (w * h + 2 * cw * ch + 2 * w * h)
+
+
+def cpu_baseline(frames, params_kw, bits, seconds, threads):
+    """Oracle frames/s on `threads` host threads for about `seconds`."""
+    import threading
+    import oracle
+    w = frames[0][0].shape[1]
+    h = frames[0][0].shape[0]
+    done = [0] * threads
+    stop = time.perf_counter() + seconds
+
+    def work(t):
+        p = oracle.make_params(w, h, **params_kw)
+        k = t
+        while time.perf_counter() < stop:
+            y, u, v = frames[k % len(frames)]
+            oracle.recon_frame(p, y, u, v, bits=bits)
+            done[t] += 1
+            k += threads
+
+    t0 = time.perf_counter()
+    ts = [threading.Thread(target=work, args=(i,)) for i in range(threads)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    el = time.perf_counter() - t0
+    return sum(done) / el, sum(done), el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--frames", type=int, default=240)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--bits", type=int, default=10)
+    ap.add_argument("--algo", type=int, default=2)
+    ap.add_argument("--bp", type=float, default=0.5)
+    ap.add_argument("--strength", type=float, default=0.8)
+    ap.add_argument("--inflight", type=int, default=4)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(Recon is data-parallel over frames: see the sharding notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from lgcr_amd import abi, synth
+
+    W, H, bits = args.width, args.height, args.bits
+    cw, ch = W // 2, H // 2
+    bps = 1 if bits <= 8 else 2
+    frames = synth.clip(args.frames, W, H, bits=bits, seed=0, scene_per_frame=True,
+                        threads=min(16, os.cpu_count() or 1))
+    tdt = torch.uint8 if bps == 1 else torch.int16
+
+    def to_dev(a):
+        return torch.from_numpy(a.view(np.int16) if bps == 2 else a).to("cuda")
+
+    dev_in = [[to_dev(p) for p in f] for f in frames]
+    dev_out = [[torch.empty((H, W), dtype=tdt, device="cuda") for _ in range(2)] for _ in frames]
+    fin = [[(t.data_ptr(), t.stride(0) * bps) for t in f] for f in dev_in]
+    fout = [[(0, 0)] + [(t.data_ptr(), t.stride(0) * bps) for t in o] for o in dev_out]
+    torch.cuda.synchronize()
+
+    dev = abi.Device(0)
+    params = abi.default_params(algo=args.algo, bp=args.bp, strength=args.strength)
+    clip = abi.Clip(dev, params, abi.make_format(np.uint8 if bps == 1 else np.uint16, W, H, bits=bits),
+                    max_inflight=args.inflight)
+    for _ in range(args.warmup):
+        clip.recon_batch(fin, fout)
+    torch.cuda.synchronize()
+
+    clip.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        clip.recon_batch(fin, fout)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    stages = clip.stage_times()
+    clip.set_timing(False)
+
+    elapsed = t1 - t0
+    n = args.frames * args.steps
+    fps = n / elapsed
+    per_frame_bytes = algorithmic_bytes(W, H, cw, ch, bps)
+    dominant = max(stages.items(), key=lambda kv: kv[1][0])
+    dom_name, (dom_us, dom_n) = dominant
+    dom_avg_us = dom_us / max(1, dom_n)
+    achieved = per_frame_bytes / (dom_avg_us * 1e-6) / 1e9
+    stage_avg = {k: round(v[0] / max(1, v[1]), 2) for k, v in stages.items()}
+
+    result = {
+        "metric": METRIC,
+        "value": round(fps, 2),
+        "unit": "fps",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1000.0, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 accumulators)",
+        "data": "synthetic (polygon scenes, seed 0, one scene per frame)",
+        "config": {"workload": f"Recon {W}x{H} YUV420P{bits} -> YUV444P{bits}, algo {args.algo}, "
+                               f"lanczos3, strength {args.strength}, sparse 1, bp {args.bp}",
+                   "frames_per_step": args.frames, "inflight_workspaces": args.inflight},
+        "end_to_end_hbm_gbs": round(per_frame_bytes * fps / 1e9, 2),
+        "end_to_end_hbm_frac": round(per_frame_bytes * fps / 1e9 / HBM_PEAK_GBS, 5),
+        "stage_avg_us": stage_avg,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom_name,
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": None,
+            "bytes_per_launch": per_frame_bytes,
+            "avg_launch_us": round(dom_avg_us, 2),
+        },
+    }
+    clip.close()
+    dev.close()
+
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        sample = frames[: max(1, min(len(frames), 2 * threads))]
+        kw = {"algo": args.algo, "bp": args.bp, "strength": args.strength}
+        v, nf, el = cpu_baseline(sample, kw, bits, args.cpu_seconds, threads)
+        result["cpu_baseline"] = {
+            "value": round(v, 3), "unit": "fps", "cores": threads, "kind": "port",
+            "sample": f"{nf} frames of the same workload in {el:.1f} s on {threads} threads "
+                      f"(oracle C restatement, scalar, one frame per thread)",
+        }
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

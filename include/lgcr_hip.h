@@ -1,0 +1,198 @@
+/*
+ * lgcr_hip.h — C ABI of the MI355X (gfx950) LGCR chroma-reconstruction
+ * backend.  Plain C types only: pointers, sizes and POD structs.
+ *
+ * The reference has no device boundary: its VapourSynth glue calls the C++
+ * stage functions directly (src/plugin.cpp:177-721) through a compile-time
+ * SIMD vocabulary (src/backend.h:13-96), and ships an unused CUDA executor
+ * (src/cuda_backend.h:36-65).  This header replaces both: the plugin glue
+ * (or any other host, e.g. a serving loop) hands whole frames to one device
+ * backend.  Each declaration below cites the reference interface it stands
+ * in for.
+ *
+ * Threading: every entry point is thread-safe.  Concurrent lgcr_recon_frame
+ * calls on one clip (VapourSynth fmParallel, plugin.cpp:1157) run on
+ * separate per-call workspaces and HIP streams.
+ */
+#ifndef LGCR_HIP_H
+#define LGCR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LGCR_ABI_VERSION 1
+
+/* Status codes: the reference's StageStatus (src/pipeline.h:390-395) plus
+ * allocation and admission failures. */
+typedef enum lgcr_status {
+    LGCR_OK = 0,
+    LGCR_UNSUPPORTED = 1,       /* StageStatus::Unsupported */
+    LGCR_INVALID_CONTEXT = 2,   /* StageStatus::InvalidContext */
+    LGCR_BACKEND_FAILURE = 3,   /* StageStatus::BackendFailure */
+    LGCR_OUT_OF_MEMORY = 4,
+    LGCR_REJECTED = 5,          /* admission refused (HBM budget) */
+    LGCR_INVALID_ARGUMENT = 6
+} lgcr_status;
+
+/* Resampling kernels, same order as lgcr::Kernel (src/lgcr.h:46). */
+typedef enum lgcr_kernel {
+    LGCR_KERNEL_BILINEAR = 0,
+    LGCR_KERNEL_BICUBIC = 1,
+    LGCR_KERNEL_LANCZOS = 2,
+    LGCR_KERNEL_SPLINE16 = 3,
+    LGCR_KERNEL_SPLINE36 = 4,
+    LGCR_KERNEL_JINC = 5
+} lgcr_kernel;
+
+/* Filter parameters: the fields of LGCRData (src/lgcr.h:128-159) after
+ * argument parsing (src/plugin.cpp:1025-1133 for Recon, 939-976 for TRecon).
+ * Fill with lgcr_params_default() and override. */
+typedef struct lgcr_params {
+    int kernel;               /* lgcr_kernel */
+    double kp1, kp2;          /* lanczos/jinc taps, or bicubic b, c */
+    double support;
+    int radial;               /* 1 for jinc */
+    double strength, sigma, sratio, sdb, reg, stretch, gsigma, rescue;
+    int algo;                 /* 2, 4 or 6 */
+    int ridge, cedge;
+    double ms, qgate;
+    int sparse;
+    double bp;
+    int bm;
+    int trad, tsearch;        /* TRecon */
+    double tsad;
+    double ar_margin;         /* <0 disables the hull clamp */
+    double shift_x, shift_y;  /* default chroma siting, luma units */
+    int loc_set;              /* 1: ignore per-frame _ChromaLocation */
+    int out_w, out_h;         /* output luma size */
+} lgcr_params;
+
+/* Input clip format (VSVideoFormat + constant size, src/plugin.cpp:1012-1023). */
+typedef struct lgcr_format {
+    int sample_type;          /* 0 integer, 1 float (32-bit) */
+    int bits_per_sample;      /* 8..16, or 32 for float */
+    int bytes_per_sample;     /* 1, 2 or 4 */
+    int sub_sampling_w;       /* 0 or 1 */
+    int sub_sampling_h;       /* 0 or 1 */
+    int width, height;        /* luma size */
+} lgcr_format;
+
+typedef enum lgcr_memory { LGCR_MEMORY_HOST = 0, LGCR_MEMORY_DEVICE = 1 } lgcr_memory;
+
+/* One frame: three planes with byte strides, all in host memory (the
+ * VapourSynth frame buffers, borrowed for the call) or all in device memory
+ * (resident frames of a serving loop).  Mirrors PlaneView
+ * (src/pipeline.h:46-62). */
+typedef struct lgcr_frame {
+    void *data[3];
+    ptrdiff_t stride[3];
+    int memory;               /* lgcr_memory */
+} lgcr_frame;
+
+/* Stage ids, same order as lgcr::Stage (src/pipeline.h:98-118). */
+enum {
+    LGCR_STAGE_CONVERT_INPUT = 0, LGCR_STAGE_RESAMPLE_LUMA, LGCR_STAGE_BUILD_GUIDE_MAPS,
+    LGCR_STAGE_BUILD_TRUST_MASK, LGCR_STAGE_BUILD_MUTUAL_GATE, LGCR_STAGE_BUILD_BASE_CHROMA,
+    LGCR_STAGE_BUILD_LGF, LGCR_STAGE_BUILD_AFFINE_MAPS, LGCR_STAGE_BUILD_DETAIL,
+    LGCR_STAGE_APPLY_GUIDED_CORRECTION, LGCR_STAGE_APPLY_SELECTOR, LGCR_STAGE_APPLY_DETAIL_TRANSFER,
+    LGCR_STAGE_APPLY_COLLABORATIVE_FILTER, LGCR_STAGE_DOWNSAMPLE_BASE, LGCR_STAGE_DOWNSAMPLE_GUIDE,
+    LGCR_STAGE_DOWNSAMPLE_CANDIDATE_SCORE, LGCR_STAGE_DOWNSAMPLE_OUTPUT, LGCR_STAGE_BACK_PROJECT,
+    LGCR_STAGE_CONVERT_OUTPUT, LGCR_STAGE_COUNT
+};
+
+/* Per-frame metrics: PipelineMetrics (src/pipeline.h:315-336) as exported by
+ * writeProfile (src/plugin.cpp:126-155).  Stage times are measured with HIP
+ * events on the frame's stream. */
+typedef struct lgcr_stats {
+    double stage_us[LGCR_STAGE_COUNT];
+    uint64_t stage_pixels[LGCR_STAGE_COUNT];
+    double total_us;
+    uint64_t output_pixels;
+    uint64_t active_output_pixels;   /* sparse workset size */
+    uint64_t active_chroma_pixels;
+    uint64_t sparse_total_pixels;
+} lgcr_stats;
+
+typedef struct lgcr_device lgcr_device;
+typedef struct lgcr_clip lgcr_clip;
+
+/* Library / ABI identification. */
+int lgcr_abi_version(void);
+
+/* Thread-local message of the last failing call on this thread
+ * (CudaPipeline::lastError, src/cuda_backend.h:40; cuda_backend.cpp:76-87). */
+const char *lgcr_last_error(void);
+
+/* Defaults of lgcrCreate (src/plugin.cpp:1025-1133): lanczos 3, algo 2,
+ * strength 0.8, sparse 1 ...; out_w/out_h are left 0 (same size). */
+void lgcr_params_default(lgcr_params *p);
+
+/* Open a device context; hbm_budget_bytes caps the clip workspaces admitted
+ * on it (0 = 90 % of free HBM).  Replaces CudaPipeline construction and
+ * available() (src/cuda_backend.h:36-39, cuda_backend.cpp:60-74). */
+lgcr_status lgcr_device_open(int ordinal, uint64_t hbm_budget_bytes, lgcr_device **out);
+void lgcr_device_close(lgcr_device *dev);
+/* Bytes currently reserved by clips on the device. */
+uint64_t lgcr_device_reserved(const lgcr_device *dev);
+
+/* Create a clip: validates params like lgcrCreate (src/plugin.cpp:1012-1142),
+ * builds and uploads the geometry tables (GeometryCache,
+ * src/kernels.cpp:94-131, 411-428) and jinc LUT (plugin.cpp:1071-1076), and
+ * reserves the per-frame device workspace (FrameWorkspacePool,
+ * src/lgcr.h:431-464; CudaPipeline::allocate, cuda_backend.cpp:108-131).
+ * max_inflight bounds the workspaces (0 = 4); admission fails with
+ * LGCR_REJECTED when one workspace does not fit the device budget. */
+lgcr_status lgcr_clip_create(lgcr_device *dev, const lgcr_params *p, const lgcr_format *fmt,
+                             int max_inflight, lgcr_clip **out);
+void lgcr_clip_destroy(lgcr_clip *clip);
+/* Device bytes of one frame workspace of this clip. */
+uint64_t lgcr_clip_workspace_bytes(const lgcr_clip *clip);
+
+/* Reconstruct one frame (lgcrGetFrame's arAllFramesReady branch,
+ * src/plugin.cpp:188-721).  shift_x/shift_y are the frame's chroma siting
+ * after applyFrameChromaSiting (plugin.cpp:157-175); pass NAN to use the
+ * clip defaults.  out->data[0] may be NULL when the output has the input
+ * size: the luma plane is then shared with the source (plugin.cpp:265-269).
+ * Blocks until the output is written.  stats may be NULL. */
+lgcr_status lgcr_recon_frame(lgcr_clip *clip, const lgcr_frame *in, double shift_x,
+                             double shift_y, lgcr_frame *out, lgcr_stats *stats);
+
+/* Reconstruct n device-resident frames back to back on the clip's
+ * workspaces without host synchronisation between frames; returns when all
+ * are written.  The batched form of lgcr_recon_frame for serving loops. */
+lgcr_status lgcr_recon_batch(lgcr_clip *clip, int n, const lgcr_frame *in,
+                             lgcr_frame *out);
+
+/* Per-stage device time accumulated while timing is on: the nanosecond
+ * counters of PipelineMetrics (src/pipeline.h:315-336), here measured with
+ * HIP events recorded around each stage on the workspace stream that runs
+ * it.  lgcr_clip_set_timing resets the totals; lgcr_clip_stage_times copies
+ * LGCR_STAGE_COUNT totals (microseconds) and launch counts. */
+lgcr_status lgcr_clip_set_timing(lgcr_clip *clip, int enable);
+lgcr_status lgcr_clip_stage_times(lgcr_clip *clip, double *us, uint64_t *launches);
+
+/* Intermediates of the last frame processed on workspace 0 of a clip, for
+ * parity checks against the stage functions of src/lgcr.h:555-962.
+ * Copies up to `bytes` into host memory `dst` and stores the full size in
+ * *needed.  Returns LGCR_INVALID_ARGUMENT for an unknown id. */
+enum {
+    LGCR_FETCH_Y = 0, LGCR_FETCH_U, LGCR_FETCH_V,           /* converted source, f32 */
+    LGCR_FETCH_JXX, LGCR_FETCH_JXY, LGCR_FETCH_JYY,         /* structure tensor, f32 */
+    LGCR_FETCH_LC, LGCR_FETCH_MS,                           /* chroma res, f32 */
+    LGCR_FETCH_SEED, LGCR_FETCH_MASK,                       /* u8, luma res */
+    LGCR_FETCH_CHROMA_MASK,                                 /* u8, chroma res */
+    LGCR_FETCH_INDICES,                                     /* u32 active output indices */
+    LGCR_FETCH_PLAIN_U, LGCR_FETCH_PLAIN_V,                 /* plain base, f32 */
+    LGCR_FETCH_OUT_U, LGCR_FETCH_OUT_V,                     /* result before packing, f32 */
+    LGCR_FETCH_COUNT
+};
+lgcr_status lgcr_clip_fetch(lgcr_clip *clip, int what, void *dst, size_t bytes, size_t *needed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

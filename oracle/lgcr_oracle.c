@@ -1,0 +1,1206 @@
+/*
+ * LGCR oracle — TEST INFRASTRUCTURE ONLY (see lgcr_oracle.h).
+ *
+ * Scalar restatement of the reference's Recon path.  Every function names the
+ * reference lines it follows; arithmetic order, float/double promotion and
+ * clamping follow the reference's scalar build (Makefile:48-49: -O3, no
+ * -mavx2/-mfma, so no FMA contraction).  Build with -ffp-contract=off.
+ */
+#define _GNU_SOURCE
+#include "lgcr_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+static inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+static inline float clampf(float v, float lo, float hi) { return v < lo ? lo : (hi < v ? hi : v); }
+static inline double clampd(double v, double lo, double hi) { return v < lo ? lo : (hi < v ? hi : v); }
+static inline float minf(float a, float b) { return b < a ? b : a; }  /* std::min */
+static inline float maxf(float a, float b) { return a < b ? b : a; }  /* std::max */
+static inline int mini(int a, int b) { return b < a ? b : a; }
+static inline int maxi(int a, int b) { return a < b ? b : a; }
+
+/* Plane::at clamped read (lgcr.h:289-293) */
+static inline float at(const float *p, int w, int h, int x, int y) {
+    x = clampi(x, 0, w - 1);
+    y = clampi(y, 0, h - 1);
+    return p[(size_t)y * w + x];
+}
+
+/* ------------------------------------------------------------------------ */
+/* closed-form kernels: lgcr.h:53-117                                        */
+/* ------------------------------------------------------------------------ */
+
+static double sinc_d(double x) {
+    if (fabs(x) < 1e-8)
+        return 1.0;
+    x *= M_PI;
+    return sin(x) / x;
+}
+
+double or_kernel_eval(int k, double x, double p1, double p2) {
+    x = fabs(x);
+    switch (k) {
+    case OR_BILINEAR:
+        return x < 1.0 ? 1.0 - x : 0.0;
+    case OR_BICUBIC: {
+        const double b = p1, c = p2;
+        if (x < 1.0)
+            return ((12 - 9 * b - 6 * c) * x * x * x + (-18 + 12 * b + 6 * c) * x * x + (6 - 2 * b)) / 6.0;
+        if (x < 2.0)
+            return ((-b - 6 * c) * x * x * x + (6 * b + 30 * c) * x * x + (-12 * b - 48 * c) * x + (8 * b + 24 * c)) / 6.0;
+        return 0.0;
+    }
+    case OR_LANCZOS:
+        return x < p1 ? sinc_d(x) * sinc_d(x / p1) : 0.0;
+    case OR_SPLINE16:
+        if (x < 1.0)
+            return ((x - 9.0 / 5.0) * x - 1.0 / 5.0) * x + 1.0;
+        if (x < 2.0)
+            return ((-1.0 / 3.0 * (x - 1.0) + 4.0 / 5.0) * (x - 1.0) - 7.0 / 15.0) * (x - 1.0);
+        return 0.0;
+    case OR_SPLINE36:
+        if (x < 1.0)
+            return ((13.0 / 11.0 * x - 453.0 / 209.0) * x - 3.0 / 209.0) * x + 1.0;
+        if (x < 2.0)
+            return ((-6.0 / 11.0 * (x - 1.0) + 270.0 / 209.0) * (x - 1.0) - 156.0 / 209.0) * (x - 1.0);
+        if (x < 3.0)
+            return ((1.0 / 11.0 * (x - 2.0) - 45.0 / 209.0) * (x - 2.0) + 26.0 / 209.0) * (x - 2.0);
+        return 0.0;
+    default:
+        return 0.0;
+    }
+}
+
+/* jinc-lanczos profile (lgcr.h:98-105).  The reference calls libstdc++'s
+ * std::cyl_bessel_j(1, .); this restatement uses glibc's j1(), an independent
+ * implementation of the same function. */
+double or_jinc_windowed(double x, double taps) {
+    x = fabs(x);
+    if (x >= taps)
+        return 0.0;
+    const double j = (x < 1e-8) ? 1.0 : 2.0 * j1(M_PI * x) / (M_PI * x);
+    return j * sinc_d(x / taps);
+}
+
+double or_kernel_support(int k, double p1) {
+    switch (k) {
+    case OR_BILINEAR: return 1.0;
+    case OR_BICUBIC: return 2.0;
+    case OR_LANCZOS: return p1;
+    case OR_SPLINE16: return 2.0;
+    case OR_SPLINE36: return 3.0;
+    case OR_JINC: return p1;
+    }
+    return 1.0;
+}
+
+/* LUT: 1024 entries per unit distance, support*1024+2 entries (plugin.cpp:1071-1076) */
+void or_jinc_lut(double support, float *lut, int n) {
+    for (int i = 0; i < n; ++i)
+        lut[i] = (float)or_jinc_windowed((double)i / 1024.0, support);
+}
+
+/* lgcr.h:636-641 */
+float or_lut_lookup(const float *lut, int n, float idx) {
+    int i0 = (int)idx;
+    i0 = clampi(i0, 0, n - 2);
+    const float fr = minf(idx - (float)i0, 1.0f);
+    return lut[i0] + (lut[i0 + 1] - lut[i0]) * fr;
+}
+
+/* ------------------------------------------------------------------------ */
+/* weight tables: kernels.cpp:135-167                                        */
+/* ------------------------------------------------------------------------ */
+
+int or_build_weights(int src_n, int dst_n, int k, double kp1, double kp2,
+                     double support, double shift, or_weights *t) {
+    t->n = dst_n;
+    const double scale = (double)src_n / dst_n;
+    const double sup = (scale > 1.0) ? support * scale : support;
+    const double inv_scale = (scale > 1.0) ? 1.0 / scale : 1.0;
+    t->sup = maxi(1, 2 * (int)ceil(sup));
+    t->start = (int *)malloc(sizeof(int) * (size_t)dst_n);
+    t->w = (float *)calloc((size_t)dst_n * t->sup, sizeof(float));
+    if (!t->start || !t->w)
+        return -1;
+    for (int i = 0; i < dst_n; ++i) {
+        const double center = (i + 0.5) * scale - 0.5 + shift;
+        int first = (int)ceil(center - sup);
+        first = clampi(first, -(t->sup) + 1, src_n - 1);
+        if (first + t->sup > src_n)
+            first = maxi(-(t->sup) + 1, src_n - t->sup);
+        t->start[i] = first;
+        double sum = 0.0;
+        for (int j = 0; j < t->sup; ++j) {
+            const double d = (first + j) - center;
+            const double w = or_kernel_eval(k, d * inv_scale, kp1, kp2);
+            t->w[(size_t)i * t->sup + j] = (float)w;
+            sum += w;
+        }
+        if (fabs(sum) > 1e-9)
+            for (int j = 0; j < t->sup; ++j)
+                t->w[(size_t)i * t->sup + j] /= (float)sum;
+    }
+    return 0;
+}
+
+void or_free_weights(or_weights *w) {
+    free(w->start);
+    free(w->w);
+    memset(w, 0, sizeof(*w));
+}
+
+/* maps.cpp:190-209 (the AVX2 phase2x descriptor is not part of the result) */
+void or_bilin_axis(const float *pos, int n, int src_n, int *i0, float *f) {
+    if (src_n == 1) {
+        for (int i = 0; i < n; ++i) { i0[i] = 0; f[i] = 0.0f; }
+        return;
+    }
+    for (int i = 0; i < n; ++i) {
+        const float x = pos[i];
+        int j = (int)floorf(x);
+        float fr = x - (float)j;
+        if (j < 0) { j = 0; fr = 0.0f; }
+        if (j > src_n - 2) { j = src_n - 2; fr = 1.0f; }
+        i0[i] = j;
+        f[i] = fr;
+    }
+}
+
+/* kernels.cpp:337-404 */
+int or_build_axis(int src_luma_n, int dst_luma_n, double r, double sit_shift,
+                  const or_params *p, or_axis *a) {
+    memset(a, 0, sizeof(*a));
+    a->n = dst_luma_n;
+    const double scale_c = ((double)src_luma_n / r) / dst_luma_n;
+    const double widen_c = p->radial ? (scale_c > 1.0 ? scale_c : 1.0) : 1.0;
+    a->sup = maxi(1, 2 * (int)ceil(p->support * widen_c));
+    const size_t nn = (size_t)a->n, ns = nn * a->sup;
+    a->start = (int *)malloc(sizeof(int) * nn);
+    a->pos = (float *)malloc(sizeof(float) * nn);
+    a->lpos = (float *)malloc(sizeof(float) * nn);
+    a->w = (float *)calloc(ns, sizeof(float));
+    a->am = (float *)calloc(ns, sizeof(float));
+    a->wsum = (float *)calloc(nn, sizeof(float));
+    a->wabs = (float *)calloc(nn, sizeof(float));
+    a->active = (int *)calloc(nn, sizeof(int));
+    a->cb_i0 = (int *)malloc(sizeof(int) * nn);
+    a->cb_f = (float *)malloc(sizeof(float) * nn);
+    a->lb_i0 = (int *)malloc(sizeof(int) * nn);
+    a->lb_f = (float *)malloc(sizeof(float) * nn);
+    if (!a->start || !a->pos || !a->lpos || !a->w || !a->am || !a->wsum || !a->wabs ||
+        !a->active || !a->cb_i0 || !a->cb_f || !a->lb_i0 || !a->lb_f)
+        return -1;
+    const int ri = (int)r;
+    const int src_chroma_n = (src_luma_n + ri - 1) / ri;
+    const double scale = (double)src_luma_n / dst_luma_n;
+    for (int i = 0; i < a->n; ++i) {
+        const double lx = (i + 0.5) * scale - 0.5;
+        const double sc = (lx + 0.5 - sit_shift) / r - 0.5;
+        a->pos[i] = (float)sc;
+        a->lpos[i] = (float)lx;
+        int first = (int)ceil(sc - p->support * widen_c);
+        first = clampi(first, 0, maxi(0, src_chroma_n - a->sup));
+        a->start[i] = first;
+        float *wi = a->w + (size_t)i * a->sup;
+        float *mi = a->am + (size_t)i * a->sup;
+        for (int j = 0; j < a->sup; ++j)
+            if (fabs((first + j) - sc) < p->support * widen_c) {
+                mi[j] = 1.0f;
+                ++a->active[i];
+            }
+        if (p->radial)
+            continue;
+        double sum = 0.0;
+        for (int j = 0; j < a->sup; ++j) {
+            const double dist = (first + j) - sc;
+            const double w = or_kernel_eval(p->kernel, dist, p->kp1, p->kp2);
+            wi[j] = (float)w;
+            sum += w;
+        }
+        if (fabs(sum) > 1e-9)
+            for (int j = 0; j < a->sup; ++j)
+                wi[j] /= (float)sum;
+        for (int j = 0; j < a->sup; ++j) {
+            if (mi[j] == 0.0f)
+                continue;
+            a->wsum[i] += wi[j];
+            a->wabs[i] += fabsf(wi[j]);
+        }
+    }
+    or_bilin_axis(a->pos, a->n, src_chroma_n, a->cb_i0, a->cb_f);
+    or_bilin_axis(a->lpos, a->n, src_luma_n, a->lb_i0, a->lb_f);
+    return 0;
+}
+
+void or_free_axis(or_axis *a) {
+    free(a->start); free(a->pos); free(a->lpos); free(a->w); free(a->am);
+    free(a->wsum); free(a->wabs); free(a->active);
+    free(a->cb_i0); free(a->cb_f); free(a->lb_i0); free(a->lb_f);
+    memset(a, 0, sizeof(*a));
+}
+
+/* ------------------------------------------------------------------------ */
+/* conversions: lgcr.h:986-1013                                              */
+/* ------------------------------------------------------------------------ */
+
+void or_u8_to_float(const uint8_t *src, ptrdiff_t stride, float *dst, int w, int h,
+                    double scale, double offset) {
+    for (int y = 0; y < h; ++y) {
+        const uint8_t *s = src + stride * y;
+        for (int x = 0; x < w; ++x)
+            dst[(size_t)y * w + x] = (float)(s[x] * scale + offset);
+    }
+}
+
+void or_u16_to_float(const uint16_t *src, ptrdiff_t stride, float *dst, int w, int h,
+                     double scale, double offset) {
+    for (int y = 0; y < h; ++y) {
+        const uint16_t *s = (const uint16_t *)((const uint8_t *)src + stride * y);
+        for (int x = 0; x < w; ++x)
+            dst[(size_t)y * w + x] = (float)(s[x] * scale + offset);
+    }
+}
+
+void or_float_to_u8(const float *src, uint8_t *dst, ptrdiff_t stride, int w, int h,
+                    double scale, double offset, int lo, int hi) {
+    const float fs = (float)scale, fo = (float)offset;
+    for (int y = 0; y < h; ++y) {
+        uint8_t *d = dst + stride * y;
+        for (int x = 0; x < w; ++x) {
+            const int v = (int)(src[(size_t)y * w + x] * fs + fo + 0.5f);
+            d[x] = (uint8_t)clampi(v, lo, hi);
+        }
+    }
+}
+
+void or_float_to_u16(const float *src, uint16_t *dst, ptrdiff_t stride, int w, int h,
+                     double scale, double offset, int lo, int hi) {
+    const float fs = (float)scale, fo = (float)offset;
+    for (int y = 0; y < h; ++y) {
+        uint16_t *d = (uint16_t *)((uint8_t *)dst + stride * y);
+        for (int x = 0; x < w; ++x) {
+            const int v = (int)(src[(size_t)y * w + x] * fs + fo + 0.5f);
+            d[x] = (uint16_t)clampi(v, lo, hi);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* structure tensor + trust seed: maps.cpp:9-168                             */
+/* ------------------------------------------------------------------------ */
+
+/* sobelPixel (maps.cpp:9-20); the scalar interior loop (maps.cpp:36-59 with
+ * one lane) evaluates the identical expression, so every pixel uses it. */
+static void sobel_px(const float *p, int w, int h, int x, int y, float *gx, float *gy) {
+    const float tl = at(p, w, h, x - 1, y - 1);
+    const float t = at(p, w, h, x, y - 1);
+    const float tr = at(p, w, h, x + 1, y - 1);
+    const float l = at(p, w, h, x - 1, y);
+    const float r = at(p, w, h, x + 1, y);
+    const float bl = at(p, w, h, x - 1, y + 1);
+    const float b = at(p, w, h, x, y + 1);
+    const float br = at(p, w, h, x + 1, y + 1);
+    *gx = (tr + 2.0f * r + br - tl - 2.0f * l - bl) * 0.125f;
+    *gy = (bl + 2.0f * b + br - tl - 2.0f * t - tr) * 0.125f;
+}
+
+void or_sobel_tensor(const float *y, int w, int h, float *jxx, float *jxy, float *jyy,
+                     uint8_t *seed, float threshold) {
+    float *gx = (float *)malloc(sizeof(float) * (size_t)w * h);
+    float *gy = (float *)malloc(sizeof(float) * (size_t)w * h);
+    float *vxx = (float *)malloc(sizeof(float) * (size_t)w);
+    float *vxy = (float *)malloc(sizeof(float) * (size_t)w);
+    float *vyy = (float *)malloc(sizeof(float) * (size_t)w);
+    for (int r = 0; r < h; ++r)
+        for (int x = 0; x < w; ++x)
+            sobel_px(y, w, h, x, r, &gx[(size_t)r * w + x], &gy[(size_t)r * w + x]);
+    const float ninth = 1.0f / 9.0f;
+    for (int r = 0; r < h; ++r) {
+        const float *gx0 = gx + (size_t)maxi(0, r - 1) * w, *gx1 = gx + (size_t)r * w;
+        const float *gx2 = gx + (size_t)mini(h - 1, r + 1) * w;
+        const float *gy0 = gy + (size_t)maxi(0, r - 1) * w, *gy1 = gy + (size_t)r * w;
+        const float *gy2 = gy + (size_t)mini(h - 1, r + 1) * w;
+        for (int x = 0; x < w; ++x) {
+            vxx[x] = gx0[x] * gx0[x] + gx1[x] * gx1[x] + gx2[x] * gx2[x];
+            vxy[x] = gx0[x] * gy0[x] + gx1[x] * gy1[x] + gx2[x] * gy2[x];
+            vyy[x] = gy0[x] * gy0[x] + gy1[x] * gy1[x] + gy2[x] * gy2[x];
+        }
+        float *oxx = jxx + (size_t)r * w, *oxy = jxy + (size_t)r * w, *oyy = jyy + (size_t)r * w;
+        const int r1 = mini(1, w - 1);
+        /* border columns divide by 9, the interior multiplies by float(1/9)
+         * (maps.cpp:125-154) */
+        oxx[0] = (2.0f * vxx[0] + vxx[r1]) / 9.0f;
+        oxy[0] = (2.0f * vxy[0] + vxy[r1]) / 9.0f;
+        oyy[0] = (2.0f * vyy[0] + vyy[r1]) / 9.0f;
+        for (int x = 1; x + 1 <= w - 1; ++x) {
+            oxx[x] = ninth * ((vxx[x - 1] + vxx[x]) + vxx[x + 1]);
+            oxy[x] = ninth * ((vxy[x - 1] + vxy[x]) + vxy[x + 1]);
+            oyy[x] = ninth * ((vyy[x - 1] + vyy[x]) + vyy[x + 1]);
+        }
+        if (w > 1) {
+            oxx[w - 1] = (vxx[w - 2] + 2.0f * vxx[w - 1]) / 9.0f;
+            oxy[w - 1] = (vxy[w - 2] + 2.0f * vxy[w - 1]) / 9.0f;
+            oyy[w - 1] = (vyy[w - 2] + 2.0f * vyy[w - 1]) / 9.0f;
+        }
+        if (seed)
+            for (int x = 0; x < w; ++x)
+                seed[(size_t)r * w + x] = oxx[x] + oyy[x] > threshold;
+    }
+    free(gx); free(gy); free(vxx); free(vxy); free(vyy);
+}
+
+/* footprint luma map (maps.cpp:425-499, scalar tail 487-496) */
+void or_lc_map(const float *y, int w, int h, int cw, int ch, double rw, double rh,
+               double shift_x, double shift_y, float *lc) {
+    int *xb = (int *)malloc(sizeof(int) * (size_t)cw), *xe = (int *)malloc(sizeof(int) * (size_t)cw);
+    int *yb = (int *)malloc(sizeof(int) * (size_t)ch), *ye = (int *)malloc(sizeof(int) * (size_t)ch);
+    for (int cx = 0; cx < cw; ++cx) {
+        const double lx = (cx + 0.5) * rw - 0.5 + shift_x;
+        int first = (int)ceil(lx - 0.5 * rw);
+        int last = (int)ceil(lx + 0.5 * rw);
+        first = clampi(first, 0, w - 1);
+        last = clampi(last, first + 1, w);
+        xb[cx] = first; xe[cx] = last;
+    }
+    for (int cy = 0; cy < ch; ++cy) {
+        const double ly = (cy + 0.5) * rh - 0.5 + shift_y;
+        int first = (int)ceil(ly - 0.5 * rh);
+        int last = (int)ceil(ly + 0.5 * rh);
+        first = clampi(first, 0, h - 1);
+        last = clampi(last, first + 1, h);
+        yb[cy] = first; ye[cy] = last;
+    }
+    for (int cy = 0; cy < ch; ++cy)
+        for (int cx = 0; cx < cw; ++cx) {
+            double sum = 0;
+            for (int j = yb[cy]; j < ye[cy]; ++j)
+                for (int i = xb[cx]; i < xe[cx]; ++i)
+                    sum += y[(size_t)j * w + i];
+            lc[(size_t)cy * cw + cx] = (float)(sum / ((double)(xe[cx] - xb[cx]) * (ye[cy] - yb[cy])));
+        }
+    free(xb); free(xe); free(yb); free(ye);
+}
+
+/* separable binary dilation by running counts (maps.cpp:295-339; same
+ * result as dilateMask, maps.cpp:704-742): a pixel is set when any source
+ * pixel within Chebyshev distance `radius` is set. */
+void or_dilate(const uint8_t *src, int w, int h, int r, uint8_t *dst) {
+    uint8_t *tmp = (uint8_t *)calloc((size_t)w * h, 1);
+    for (int y = 0; y < h; ++y) {
+        const uint8_t *s = src + (size_t)y * w;
+        uint8_t *t = tmp + (size_t)y * w;
+        int active = 0;
+        for (int x = 0; x <= mini(w - 1, r); ++x)
+            active += s[x] != 0;
+        for (int x = 0; x < w; ++x) {
+            t[x] = active != 0;
+            if (x - r >= 0) active -= s[x - r] != 0;
+            if (x + r + 1 < w) active += s[x + r + 1] != 0;
+        }
+    }
+    int *cnt = (int *)calloc((size_t)w, sizeof(int));
+    for (int y = 0; y <= mini(h - 1, r); ++y)
+        for (int x = 0; x < w; ++x)
+            cnt[x] += tmp[(size_t)y * w + x] != 0;
+    for (int y = 0; y < h; ++y) {
+        for (int x = 0; x < w; ++x)
+            dst[(size_t)y * w + x] = cnt[x] != 0;
+        for (int x = 0; x < w; ++x) {
+            if (y - r >= 0) cnt[x] -= tmp[(size_t)(y - r) * w + x] != 0;
+            if (y + r + 1 < h) cnt[x] += tmp[(size_t)(y + r + 1) * w + x] != 0;
+        }
+    }
+    free(cnt);
+    free(tmp);
+}
+
+/* ------------------------------------------------------------------------ */
+/* sparse workset: maps.cpp:342-422                                          */
+/* ------------------------------------------------------------------------ */
+
+int or_build_workset(const uint8_t *mask, int mask_w, int mask_h, int out_w, int out_h,
+                     const or_axis *ax, const or_axis *ay, int cw, int ch,
+                     or_workset *ws) {
+    memset(ws, 0, sizeof(*ws));
+    ws->mask_w = mask_w; ws->mask_h = mask_h;
+    ws->out_w = out_w; ws->out_h = out_h;
+    ws->chroma_w = cw; ws->chroma_h = ch;
+    ws->mask = (uint8_t *)malloc((size_t)mask_w * mask_h);
+    memcpy(ws->mask, mask, (size_t)mask_w * mask_h);
+    ws->index_row_offsets = (size_t *)calloc((size_t)out_h + 1, sizeof(size_t));
+    ws->span_row_offsets = (size_t *)calloc((size_t)out_h + 1, sizeof(size_t));
+    ws->indices = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)out_w * out_h + 1));
+    ws->spans = (int *)malloc(sizeof(int) * ((size_t)out_w * out_h + 2));
+    ws->chroma_mask = (uint8_t *)calloc((size_t)cw * ch, 1);
+    int *mx = (int *)malloc(sizeof(int) * (size_t)out_w);
+    for (int x = 0; x < out_w; ++x)
+        mx[x] = mini(mask_w - 1, (int)((int64_t)x * mask_w / out_w));
+    for (int y = 0; y < out_h; ++y) {
+        ws->span_row_offsets[y] = ws->n_spans;
+        ws->index_row_offsets[y] = ws->n_indices;
+        const int my = mini(mask_h - 1, (int)((int64_t)y * mask_h / out_h));
+        const uint8_t *row = ws->mask + (size_t)my * mask_w;
+        int x = 0;
+        while (x < out_w) {
+            while (x < out_w && row[mx[x]] == 0) ++x;
+            const int begin = x;
+            while (x < out_w && row[mx[x]] != 0) {
+                ws->indices[ws->n_indices++] = (uint32_t)((size_t)y * out_w + x);
+                ++x;
+            }
+            if (begin < x) {
+                ws->spans[2 * ws->n_spans] = begin;
+                ws->spans[2 * ws->n_spans + 1] = x;
+                ws->n_spans++;
+            }
+        }
+    }
+    ws->span_row_offsets[out_h] = ws->n_spans;
+    ws->index_row_offsets[out_h] = ws->n_indices;
+    for (int y = 0; y < out_h; ++y) {
+        const int cy0 = ay->cb_i0[y];
+        const int cy1 = mini(cy0 + 1, ch - 1);
+        uint8_t *a0 = ws->chroma_mask + (size_t)cy0 * cw;
+        uint8_t *a1 = ws->chroma_mask + (size_t)cy1 * cw;
+        for (size_t s = ws->span_row_offsets[y]; s < ws->span_row_offsets[y + 1]; ++s)
+            for (int x = ws->spans[2 * s]; x < ws->spans[2 * s + 1]; ++x) {
+                const int cx0 = ax->cb_i0[x];
+                const int cx1 = mini(cx0 + 1, cw - 1);
+                a0[cx0] = a0[cx1] = 1;
+                a1[cx0] = a1[cx1] = 1;
+            }
+    }
+    for (size_t i = 0; i < (size_t)cw * ch; ++i)
+        ws->active_chroma += ws->chroma_mask[i] != 0;
+    free(mx);
+    return 0;
+}
+
+void or_free_workset(or_workset *ws) {
+    free(ws->mask); free(ws->indices); free(ws->index_row_offsets);
+    free(ws->spans); free(ws->span_row_offsets); free(ws->chroma_mask);
+    memset(ws, 0, sizeof(*ws));
+}
+
+/* lgcr.h:780-785 */
+int or_workset_output_dense(const or_workset *ws) {
+    return ws->n_indices * 2 >= (size_t)ws->out_w * ws->out_h;
+}
+int or_workset_chroma_dense(const or_workset *ws) {
+    return ws->active_chroma * 2 >= (size_t)ws->chroma_w * ws->chroma_h;
+}
+
+/* ------------------------------------------------------------------------ */
+/* mutual-structure gate: maps.cpp:831-948                                   */
+/* ------------------------------------------------------------------------ */
+
+typedef struct { float coherence, nx, ny; } tdir;
+
+/* principalTensorDirection (lgcr.h:516-538) */
+static tdir principal_dir(float jxx, float jxy, float jyy) {
+    tdir d = {0.0f, 1.0f, 0.0f};
+    const float diff = jxx - jyy;
+    const float sum = jxx + jyy;
+    const float cross = 2.0f * jxy;
+    const float disc = sqrtf(diff * diff + cross * cross);
+    d.coherence = disc / (sum + 1e-12f);
+    if (disc <= 1e-20f)
+        return d;
+    float vx, vy;
+    if (diff >= 0.0f) { vx = disc + diff; vy = cross; }
+    else { vx = cross; vy = disc - diff; }
+    const float inv = 1.0f / sqrtf(vx * vx + vy * vy);
+    d.nx = vx * inv;
+    d.ny = vy * inv;
+    return d;
+}
+
+/* SobelRowRing::sampleProjected (maps.cpp:782-791) at a bilinearPoint
+ * (maps.cpp:175-186). */
+static float sample_projected(const float *gx, const float *gy, int w, int h,
+                              double px, double py, float nx, float ny) {
+    const int rx = (int)floor(px), ry = (int)floor(py);
+    const int x0 = clampi(rx, 0, w - 1), x1 = clampi(rx + 1, 0, w - 1);
+    const int y0 = clampi(ry, 0, h - 1), y1 = clampi(ry + 1, 0, h - 1);
+    const float fx = (float)(px - rx), fy = (float)(py - ry);
+    const float *xt = gx + (size_t)y0 * w, *xb = gx + (size_t)y1 * w;
+    const float *yt = gy + (size_t)y0 * w, *yb = gy + (size_t)y1 * w;
+    const float a = xt[x0] * nx + yt[x0] * ny;
+    const float b = xt[x1] * nx + yt[x1] * ny;
+    const float c = xb[x0] * nx + yb[x0] * ny;
+    const float d = xb[x1] * nx + yb[x1] * ny;
+    return a + (b - a) * fx + (c - a) * fy + (a - b - c + d) * fx * fy;
+}
+
+void or_mutual_gate(const float *lc, const float *u, const float *v, int w, int h,
+                    const uint8_t *chroma_mask, int chroma_dense, float *gate) {
+    const int dense = !chroma_mask || chroma_dense;
+    const size_t n = (size_t)w * h;
+    float *g = (float *)malloc(sizeof(float) * n * 6);
+    float *lx = g, *ly = g + n, *ux = g + 2 * n, *uy = g + 3 * n, *vx = g + 4 * n, *vy = g + 5 * n;
+    /* Sobel gradients of the three planes.  The sparse reference computes
+     * them only inside the 4-pixel-dilated mask and zero elsewhere; values
+     * read by active pixels lie inside that halo, so computing them
+     * everywhere gives identical gate values (see or_dilate for the halo). */
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            const size_t i = (size_t)y * w + x;
+            sobel_px(lc, w, h, x, y, &lx[i], &ly[i]);
+            sobel_px(u, w, h, x, y, &ux[i], &uy[i]);
+            sobel_px(v, w, h, x, y, &vx[i], &vy[i]);
+        }
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            float *out = gate + (size_t)y * w + x;
+            if (!dense && chroma_mask[(size_t)y * w + x] == 0) {
+                *out = 0.0f;
+                continue;
+            }
+            float sxx = 0, sxy = 0, syy = 0;
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const float a = at(lx, w, h, x + dx, y + dy);
+                    const float b = at(ly, w, h, x + dx, y + dy);
+                    sxx += a * a; sxy += a * b; syy += b * b;
+                }
+            const float jsum = sxx + syy;
+            if (jsum < 1e-10f) { *out = 0.0f; continue; }
+            const tdir d = principal_dir(sxx, sxy, syy);
+            const float nx = d.nx, ny = d.ny;
+            float sumY = 0, sqY = 0, momY = 0, sumC = 0, sqC = 0, momC = 0;
+            for (int k = -3; k <= 3; ++k) {
+                /* int + float arithmetic happens in float, then widens */
+                const double px = (float)x + (float)k * nx;
+                const double py = (float)y + (float)k * ny;
+                const float gY = fabsf(sample_projected(lx, ly, w, h, px, py, nx, ny));
+                const float au = sample_projected(ux, uy, w, h, px, py, nx, ny);
+                const float av = sample_projected(vx, vy, w, h, px, py, nx, ny);
+                const float gC = sqrtf(au * au + av * av);
+                sumY += gY; sqY += gY * gY; momY += (float)k * gY;
+                sumC += gC; sqC += gC * gC; momC += (float)k * gC;
+            }
+            if (sumY < 1e-9f || sumC < 1e-9f) { *out = 0.0f; continue; }
+            const float partY = sumY * sumY / (sqY + 1e-12f);
+            const float partC = sumC * sumC / (sqC + 1e-12f);
+            const float ratio = partC / (partY + 1e-6f);
+            const float dcent = fabsf(momY / sumY - momC / sumC);
+            const float tw = clampf((1.6f - ratio) / 0.4f, 0.0f, 1.0f);
+            const float tp = clampf((1.0f - dcent) / 0.5f, 0.0f, 1.0f);
+            *out = (tw * tw * (3.0f - 2.0f * tw)) * (tp * tp * (3.0f - 2.0f * tp));
+        }
+    free(g);
+}
+
+/* ------------------------------------------------------------------------ */
+/* bilinear samplers: lgcr.h:467-508                                         */
+/* ------------------------------------------------------------------------ */
+
+static float bilinear(const float *p, int w, int h, double x, double y) {
+    const int x0 = (int)floor(x), y0 = (int)floor(y);
+    const float fx = (float)(x - x0), fy = (float)(y - y0);
+    const float a = at(p, w, h, x0, y0), b = at(p, w, h, x0 + 1, y0);
+    const float c = at(p, w, h, x0, y0 + 1), d = at(p, w, h, x0 + 1, y0 + 1);
+    return a + (b - a) * fx + (c - a) * fy + (a - b - c + d) * fx * fy;
+}
+
+static float bilinear_fast(const float *p, int w, int h, int x0, float fx, int y0, float fy) {
+    if (w == 1) {
+        const float a = p[(size_t)y0 * w];
+        if (h == 1)
+            return a;
+        const float c = p[(size_t)(y0 + 1) * w];
+        return a + (c - a) * fy;
+    }
+    const float *r0 = p + (size_t)y0 * w + x0;
+    if (h == 1)
+        return r0[0] + (r0[1] - r0[0]) * fx;
+    const float *r1 = r0 + w;
+    const float a = r0[0], b = r0[1], c = r1[0], d = r1[1];
+    return a + (b - a) * fx + (c - a) * fy + (a - b - c + d) * fx * fy;
+}
+
+/* ------------------------------------------------------------------------ */
+/* plain chroma                                                              */
+/* ------------------------------------------------------------------------ */
+
+/* plainChromaSeparable, scalar path (recon.cpp:1234-1263 horizontal,
+ * 1389-1437 vertical). */
+static void plain_separable(const or_params *p, const float *u, const float *v, int cw, int ch,
+                            const or_axis *ax, const or_axis *ay, int ow, int oh,
+                            float *out_u, float *out_v) {
+    const size_t hn = (size_t)ow * ch;
+    float *hu = (float *)malloc(sizeof(float) * hn * 6);
+    float *hv = hu + hn, *hmnu = hu + 2 * hn, *hmxu = hu + 3 * hn, *hmnv = hu + 4 * hn, *hmxv = hu + 5 * hn;
+    for (int sy = 0; sy < ch; ++sy) {
+        const float *su = u + (size_t)sy * cw, *sv = v + (size_t)sy * cw;
+        for (int ox = 0; ox < ow; ++ox) {
+            const int start = ax->start[ox];
+            const float *wt = ax->w + (size_t)ox * ax->sup;
+            const float *act = ax->am + (size_t)ox * ax->sup;
+            double vu = 0.0, vv = 0.0;
+            float mnu = 1e30f, mxu = -1e30f, mnv = 1e30f, mxv = -1e30f;
+            for (int i = 0; i < ax->sup; ++i) {
+                if (act[i] == 0.0f)
+                    continue;
+                const int x = clampi(start + i, 0, cw - 1);
+                const float a = su[x], b = sv[x];
+                vu += (double)wt[i] * a;
+                mnu = minf(mnu, a); mxu = maxf(mxu, a);
+                vv += (double)wt[i] * b;
+                mnv = minf(mnv, b); mxv = maxf(mxv, b);
+            }
+            const size_t k = (size_t)sy * ow + ox;
+            hu[k] = (float)vu; hv[k] = (float)vv;
+            hmnu[k] = mnu; hmxu[k] = mxu; hmnv[k] = mnv; hmxv[k] = mxv;
+        }
+    }
+    const int clamp_hull = p->ar_margin >= 0.0;
+    const float margin = (float)(p->ar_margin > 0.0 ? p->ar_margin : 0.0);
+    for (int oy = 0; oy < oh; ++oy) {
+        const int start = ay->start[oy];
+        const float *wt = ay->w + (size_t)oy * ay->sup;
+        const float *act = ay->am + (size_t)oy * ay->sup;
+        for (int ox = 0; ox < ow; ++ox) {
+            double vu = 0.0, vv = 0.0;
+            float mnu = 1e30f, mxu = -1e30f, mnv = 1e30f, mxv = -1e30f;
+            for (int j = 0; j < ay->sup; ++j) {
+                if (act[j] == 0.0f)
+                    continue;
+                const int y = clampi(start + j, 0, ch - 1);
+                const size_t k = (size_t)y * ow + ox;
+                vu += (double)wt[j] * hu[k];
+                mnu = minf(mnu, hmnu[k]); mxu = maxf(mxu, hmxu[k]);
+                vv += (double)wt[j] * hv[k];
+                mnv = minf(mnv, hmnv[k]); mxv = maxf(mxv, hmxv[k]);
+            }
+            const double wsum = (double)ax->wsum[ox] * ay->wsum[oy];
+            const double asum = (double)ax->wabs[ox] * ay->wabs[oy];
+            if (wsum > 1e-6 && wsum >= 0.05 * asum) {
+                vu /= wsum;
+                vv /= wsum;
+            } else {
+                vu = bilinear(u, cw, ch, ax->pos[ox], ay->pos[oy]);
+                vv = bilinear(v, cw, ch, ax->pos[ox], ay->pos[oy]);
+            }
+            if (clamp_hull) {
+                vu = clampd(vu, (double)(mnu - margin), (double)(mxu + margin));
+                vv = clampd(vv, (double)(mnv - margin), (double)(mxv + margin));
+            }
+            out_u[(size_t)oy * ow + ox] = (float)vu;
+            out_v[(size_t)oy * ow + ox] = (float)vv;
+        }
+    }
+    free(hu);
+}
+
+/* plainChromaBilinear (recon.cpp:1031-1149, scalar tail 1128-1132) */
+static void plain_bilinear(const float *u, const float *v, int cw, int ch,
+                           const or_axis *ax, const or_axis *ay, int ow, int oh,
+                           float *out_u, float *out_v) {
+    for (int oy = 0; oy < oh; ++oy)
+        for (int ox = 0; ox < ow; ++ox) {
+            out_u[(size_t)oy * ow + ox] = bilinear_fast(u, cw, ch, ax->cb_i0[ox], ax->cb_f[ox],
+                                                        ay->cb_i0[oy], ay->cb_f[oy]);
+            out_v[(size_t)oy * ow + ox] = bilinear_fast(v, cw, ch, ax->cb_i0[ox], ax->cb_f[ox],
+                                                        ay->cb_i0[oy], ay->cb_f[oy]);
+        }
+}
+
+void or_plain_chroma(const or_params *p, const float *u, const float *v, int cw, int ch,
+                     int sw, int sh, float *out_u, float *out_v) {
+    const double rw = (double)sw / cw, rh = (double)sh / ch;
+    or_axis ax, ay;
+    or_build_axis(sw, p->out_w, rw, p->shift_x, p, &ax);
+    or_build_axis(sh, p->out_h, rh, p->shift_y, p, &ay);
+    if (p->kernel == OR_BILINEAR && !p->radial)
+        plain_bilinear(u, v, cw, ch, &ax, &ay, p->out_w, p->out_h, out_u, out_v);
+    else
+        plain_separable(p, u, v, cw, ch, &ax, &ay, p->out_w, p->out_h, out_u, out_v);
+    or_free_axis(&ax);
+    or_free_axis(&ay);
+}
+
+/* ------------------------------------------------------------------------ */
+/* guided reconstruction: recon.cpp:14-819 (scalar branches)                 */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    float luma, nx, ny, coherence, min_diff, ridge, phase, mfade;
+    int has_dir;
+} guide_px;
+
+static void guide_meta(const or_params *p, const or_guide *g, const or_axis *ax,
+                       const or_axis *ay, int cw, int ch, float sp_out, int ox, int oy,
+                       guide_px *m) {
+    const int bxi = ax->lb_i0[ox], byi = ay->lb_i0[oy];
+    const float bxf = ax->lb_f[ox], byf = ay->lb_f[oy];
+    const int sw = g->sw, sh = g->sh;
+    m->luma = bilinear_fast(g->y, sw, sh, bxi, bxf, byi, byf);
+    const float jxx = bilinear_fast(g->jxx, sw, sh, bxi, bxf, byi, byf);
+    const float jxy = bilinear_fast(g->jxy, sw, sh, bxi, bxf, byi, byf);
+    const float jyy = bilinear_fast(g->jyy, sw, sh, bxi, bxf, byi, byf);
+    const float jsum = jxx + jyy;
+    const tdir d = principal_dir(jxx, jxy, jyy);
+    m->coherence = d.coherence;
+    m->nx = d.nx;
+    m->ny = d.ny;
+    m->has_dir = jsum > 1e-6f;
+    m->min_diff = 0.0f;
+    m->ridge = 0.0f;
+    const float lx = ax->lpos[ox], ly = ay->lpos[oy];
+    if (m->has_dir) {
+        const float ya = bilinear(g->y, sw, sh, lx - sp_out * m->nx, ly - sp_out * m->ny);
+        const float yb = bilinear(g->y, sw, sh, lx + sp_out * m->nx, ly + sp_out * m->ny);
+        m->min_diff = minf(fabsf(ya - m->luma), fabsf(yb - m->luma));
+    }
+    if (p->ridge && m->has_dir) {
+        const float dl = 1.5f * sp_out;
+        const float ym2 = bilinear(g->y, sw, sh, lx - 2 * dl * m->nx, ly - 2 * dl * m->ny);
+        const float ym1 = bilinear(g->y, sw, sh, lx - dl * m->nx, ly - dl * m->ny);
+        const float yp1 = bilinear(g->y, sw, sh, lx + dl * m->nx, ly + dl * m->ny);
+        const float yp2 = bilinear(g->y, sw, sh, lx + 2 * dl * m->nx, ly + 2 * dl * m->ny);
+        const float diff[4] = { ym1 - ym2, m->luma - ym1, yp1 - m->luma, yp2 - yp1 };
+        for (int q = 0; q < 3; ++q) {
+            const float a = diff[q], b = diff[q + 1];
+            if (a > 0.0f && b < 0.0f)
+                m->ridge = maxf(m->ridge, minf(a, -b));
+            if (a < 0.0f && b > 0.0f)
+                m->ridge = maxf(m->ridge, minf(-a, b));
+        }
+    }
+    const float fx = ax->pos[ox] - floorf(ax->pos[ox]);
+    const float fy = ay->pos[oy] - floorf(ay->pos[oy]);
+    const float phx = minf(fx, 1.0f - fx);
+    const float phy = minf(fy, 1.0f - fy);
+    m->phase = (1.0f - 2.0f * phx) * (1.0f - 2.0f * phy);
+    m->mfade = 1.0f;
+    if (p->strength > 0.0 && p->ms > 0.0 && g->ms) {
+        const float gate = bilinear_fast(g->ms, cw, ch, ax->cb_i0[ox], ax->cb_f[ox],
+                                         ay->cb_i0[oy], ay->cb_f[oy]);
+        m->mfade = 1.0f - (float)p->ms * (1.0f - gate);
+    }
+}
+
+static void guided_pixel(const or_params *p, const float *U, const float *V, int cw, int ch,
+                         const or_guide *g, const or_axis *ax, const or_axis *ay,
+                         float sp_out, double rw, double rh, int ox, int oy,
+                         float *out_u, float *out_v) {
+    const int guided = p->strength > 0.0;
+    guide_px meta;
+    memset(&meta, 0, sizeof(meta));
+    meta.mfade = 1.0f;
+    if (guided)
+        guide_meta(p, g, ax, ay, cw, ch, sp_out, ox, oy, &meta);
+    const int tx0 = ax->start[ox], ty0 = ay->start[oy];
+    const float *wxp = ax->w + (size_t)ox * ax->sup;
+    const float *wyp = ay->w + (size_t)oy * ay->sup;
+    const float *amx = ax->am + (size_t)ox * ax->sup;
+    const float *amy = ay->am + (size_t)oy * ay->sup;
+    const float scx = ax->pos[ox], scy = ay->pos[oy];
+    const float L0 = meta.luma;
+    const float nxv = meta.nx, nyv = meta.ny;
+    const float *lc = g ? g->lc : NULL;
+
+    /* pass 1: window luma range and the 2D chroma hull (recon.cpp:304-363) */
+    float max_dl = 0.0f;
+    float mnu = 1e30f, mxu = -1e30f, mnv = 1e30f, mxv = -1e30f;
+    for (int j = 0; j < ay->sup; ++j) {
+        if (amy[j] == 0.0f)
+            continue;
+        const int ty = clampi(ty0 + j, 0, ch - 1);
+        for (int i = 0; i < ax->sup; ++i) {
+            if (amx[i] == 0.0f)
+                continue;
+            const int tx = clampi(tx0 + i, 0, cw - 1);
+            if (guided)
+                max_dl = maxf(max_dl, fabsf(lc[(size_t)ty * cw + tx] - L0));
+            const float a = U[(size_t)ty * cw + tx], b = V[(size_t)ty * cw + tx];
+            mnu = minf(mnu, a); mxu = maxf(mxu, a);
+            mnv = minf(mnv, b); mxv = maxf(mxv, b);
+        }
+    }
+    /* adaptive sigma, ridge fade (recon.cpp:369-376) */
+    const float t2 = clampf(meta.min_diff / (0.1f * max_dl + 1e-6f), 0.0f, 1.0f);
+    const float ss_ramp = t2 * t2 * (3.0f - 2.0f * t2);
+    const float sig_mult = (float)p->sratio + ((float)p->sdb - (float)p->sratio) * ss_ramp;
+    const float sig_loc = maxf((float)p->sigma, sig_mult * max_dl);
+    const float inv_sigma2 = 1.0f / (sig_loc * sig_loc);
+    const float ridge_fade = 1.0f - clampf(2.0f * meta.ridge / (max_dl + 1e-6f), 0.0f, 1.0f);
+    /* chroma-edge fade (recon.cpp:385-407) */
+    float cedge_fade = 1.0f;
+    if (p->cedge && guided && meta.has_dir) {
+        const float rc = maxf(mxu - mnu, mxv - mnv);
+        if (rc > 1e-6f) {
+            float gmax = 0.0f;
+            const float anx = fabsf(nxv), any = fabsf(nyv);
+            for (int j = 0; j < ay->sup; ++j) {
+                const int ty = clampi(ty0 + j, 0, ch - 1);
+                for (int i = 0; i < ax->sup; ++i) {
+                    const int tx = clampi(tx0 + i, 0, cw - 1);
+                    const int tx1 = mini(tx + 1, cw - 1);
+                    const int ty1 = mini(ty + 1, ch - 1);
+                    const float gu = maxf(fabsf(at(U, cw, ch, tx1, ty) - at(U, cw, ch, tx, ty)) * anx,
+                                          fabsf(at(U, cw, ch, tx, ty1) - at(U, cw, ch, tx, ty)) * any);
+                    const float gv = maxf(fabsf(at(V, cw, ch, tx1, ty) - at(V, cw, ch, tx, ty)) * anx,
+                                          fabsf(at(V, cw, ch, tx, ty1) - at(V, cw, ch, tx, ty)) * any);
+                    gmax = maxf(gmax, maxf(gu, gv));
+                }
+            }
+            const float wc = rc / (gmax + 1e-6f);
+            cedge_fade = clampf((2.2f - wc) / 0.7f, 0.0f, 1.0f);
+        }
+    }
+    const float guide_fade = ridge_fade * cedge_fade * meta.mfade * (float)p->strength;
+    /* phase-0 rescue with kappa^2 (recon.cpp:425-475) */
+    float rescue = 0.0f;
+    if (guided) {
+        const int ncx = clampi((int)lroundf(scx), 0, cw - 1);
+        const int ncy = clampi((int)lroundf(scy), 0, ch - 1);
+        const float dl0 = at(lc, cw, ch, ncx, ncy) - L0;
+        const float sim0 = 1.0f / (1.0f + dl0 * dl0 * inv_sigma2);
+        const float pg = meta.phase;
+        rescue = (float)p->rescue * guide_fade * pg * pg * (1.0f - sim0);
+        if (rescue > 0.0f) {
+            float kappa;
+            if (!p->radial) {
+                float mxX = 0, ssX = 0, mxY = 0, ssY = 0;
+                for (int i = 0; i < ax->sup; ++i)
+                    if (amx[i] > 0.0f) {
+                        const float w = wxp[i];
+                        mxX = maxf(mxX, fabsf(w));
+                        ssX += w * w;
+                    }
+                for (int j = 0; j < ay->sup; ++j)
+                    if (amy[j] > 0.0f) {
+                        const float w = wyp[j];
+                        mxY = maxf(mxY, fabsf(w));
+                        ssY += w * w;
+                    }
+                kappa = (mxX * mxX * mxY * mxY) / (ssX * ssY + 1e-12f);
+            } else {
+                kappa = 1.0f; /* radial kernels are handled by the resize path */
+            }
+            rescue *= kappa * kappa;
+        }
+    }
+    const float aniso = 1.0f + (float)p->stretch * meta.coherence;
+    const float inv_g2 = 1.0f / (float)(p->gsigma * p->gsigma);
+    const float inv_g2_par = inv_g2 / (aniso * aniso);
+    const float frw = (float)rw, frh = (float)rh;
+
+    /* tap loop (recon.cpp:581-627) */
+    double acc_u = 0.0, acc_v = 0.0, wsum = 0.0, sum_abs = 0.0;
+    for (int j = 0; j < ay->sup; ++j) {
+        if (amy[j] == 0.0f)
+            continue;
+        const int ty = clampi(ty0 + j, 0, ch - 1);
+        const float wy = wyp[j];
+        const float dy = ((float)(ty0 + j) - scy) * frh;
+        for (int i = 0; i < ax->sup; ++i) {
+            if (amx[i] == 0.0f)
+                continue;
+            const int tx = clampi(tx0 + i, 0, cw - 1);
+            const float wx = wxp[i] * wy;
+            float w;
+            if (guided) {
+                const float dx = ((float)(tx0 + i) - scx) * frw;
+                const float dL = lc[(size_t)ty * cw + tx] - L0;
+                const float dp = dx * nxv + dy * nyv;
+                const float dt = -dx * nyv + dy * nxv;
+                const float sim_raw = 1.0f / (1.0f + dL * dL * inv_sigma2);
+                const float sim = 1.0f - guide_fade * (1.0f - sim_raw);
+                const float bump = 1.0f / (1.0f + dp * dp * inv_g2 + dt * dt * inv_g2_par);
+                w = sim * (wx + rescue * bump);
+            } else {
+                w = wx;
+            }
+            acc_u += (double)w * U[(size_t)ty * cw + tx];
+            acc_v += (double)w * V[(size_t)ty * cw + tx];
+            wsum += w;
+            sum_abs += fabsf(w);
+        }
+    }
+    double vu, vv;
+    if (wsum > 1e-6 && wsum >= 0.05 * sum_abs) {
+        vu = acc_u / wsum;
+        vv = acc_v / wsum;
+    } else {
+        vu = bilinear(U, cw, ch, scx, scy);
+        vv = bilinear(V, cw, ch, scx, scy);
+    }
+    if (p->ar_margin >= 0.0) {
+        const float margin = (float)(p->ar_margin > 0.0 ? p->ar_margin : 0.0);
+        vu = clampd(vu, (double)(mnu - margin), (double)(mxu + margin));
+        vv = clampd(vv, (double)(mnv - margin), (double)(mxv + margin));
+    }
+    *out_u = (float)vu;
+    *out_v = (float)vv;
+}
+
+void or_guided(const or_params *p, const float *u, const float *v, int cw, int ch,
+               const or_guide *g, const or_workset *ws, float *out_u, float *out_v) {
+    const int sw = g->sw, sh = g->sh;
+    const double rw = (double)sw / cw, rh = (double)sh / ch;
+    or_axis ax, ay;
+    or_build_axis(sw, p->out_w, rw, p->shift_x, p, &ax);
+    or_build_axis(sh, p->out_h, rh, p->shift_y, p, &ay);
+    const float sp_out = (float)(rw > rh ? rw : rh);
+    const int ow = p->out_w, oh = p->out_h;
+    if (ws && ws->out_w == ow && ws->out_h == oh) {
+        for (size_t k = 0; k < ws->n_indices; ++k) {
+            const int ox = (int)(ws->indices[k] % (uint32_t)ow);
+            const int oy = (int)(ws->indices[k] / (uint32_t)ow);
+            const size_t o = (size_t)oy * ow + ox;
+            guided_pixel(p, u, v, cw, ch, g, &ax, &ay, sp_out, rw, rh, ox, oy, &out_u[o], &out_v[o]);
+        }
+    } else {
+        for (int oy = 0; oy < oh; ++oy)
+            for (int ox = 0; ox < ow; ++ox) {
+                const size_t o = (size_t)oy * ow + ox;
+                guided_pixel(p, u, v, cw, ch, g, &ax, &ay, sp_out, rw, rh, ox, oy, &out_u[o], &out_v[o]);
+            }
+    }
+    or_free_axis(&ax);
+    or_free_axis(&ay);
+}
+
+/* ------------------------------------------------------------------------ */
+/* back-projection: algos.cpp:663-677                                        */
+/* ------------------------------------------------------------------------ */
+
+void or_back_project(float *out, int ow, int oh, const float *src, int cw, int ch, float bp) {
+    for (int cy = 0; cy < ch; ++cy)
+        for (int cx = 0; cx < cw; ++cx) {
+            const int x0 = 2 * cx, y0 = 2 * cy;
+            const int xa = clampi(x0, 0, ow - 1), xb = clampi(x0 + 1, 0, ow - 1);
+            const int ya = clampi(y0, 0, oh - 1), yb = clampi(y0 + 1, 0, oh - 1);
+            const float p00 = out[(size_t)ya * ow + xa], p10 = out[(size_t)ya * ow + xb];
+            const float p01 = out[(size_t)yb * ow + xa], p11 = out[(size_t)yb * ow + xb];
+            const float r = src[(size_t)cy * cw + cx] - 0.25f * (p00 + p10 + p01 + p11);
+            const float d = bp * r;
+            out[(size_t)ya * ow + xa] += d;
+            out[(size_t)ya * ow + xb] += d;
+            out[(size_t)yb * ow + xa] += d;
+            out[(size_t)yb * ow + xb] += d;
+        }
+}
+
+/* ------------------------------------------------------------------------ */
+/* luma resampling: kernels.cpp:171-271 (scalar), 276-334 (radial, scalar)   */
+/* ------------------------------------------------------------------------ */
+
+static void resample_h(const float *src, int sw, int sh, float *dst, const or_weights *t) {
+    for (int y = 0; y < sh; ++y)
+        for (int x = 0; x < t->n; ++x) {
+            const float *wp = t->w + (size_t)x * t->sup;
+            const int s = t->start[x];
+            double acc = 0.0;
+            for (int j = 0; j < t->sup; ++j)
+                acc += (double)wp[j] * at(src, sw, sh, s + j, y);
+            dst[(size_t)y * t->n + x] = (float)acc;
+        }
+}
+
+static void resample_v(const float *src, int sw, int sh, float *dst, const or_weights *t) {
+    for (int y = 0; y < t->n; ++y) {
+        const float *wp = t->w + (size_t)y * t->sup;
+        const int s = t->start[y];
+        for (int x = 0; x < sw; ++x) {
+            double acc = 0.0;
+            for (int j = 0; j < t->sup; ++j)
+                acc += (double)wp[j] * at(src, sw, sh, x, s + j);
+            dst[(size_t)y * sw + x] = (float)acc;
+        }
+    }
+}
+
+static void resample_radial(const or_params *p, const float *src, int sw, int sh,
+                            float *dst, int dw, int dh) {
+    const double sx = (double)sw / dw, sy = (double)sh / dh;
+    const double widen = fmax(1.0, fmax(sx, sy));
+    const float inv_w = (float)(1.0 / widen);
+    const int radius = (int)ceil(p->support * widen);
+    const int sup = 2 * radius;
+    for (int oy = 0; oy < dh; ++oy) {
+        const double cy = (oy + 0.5) * sy - 0.5;
+        const int y0 = (int)ceil(cy - radius);
+        for (int ox = 0; ox < dw; ++ox) {
+            const double cx = (ox + 0.5) * sx - 0.5;
+            const int x0 = (int)ceil(cx - radius);
+            double acc = 0.0, wsum = 0.0;
+            for (int j = 0; j < sup; ++j) {
+                const int ty = clampi(y0 + j, 0, sh - 1);
+                const float dy = (float)((y0 + j - cy) * inv_w);
+                for (int i = 0; i < sup; ++i) {
+                    const int tx = clampi(x0 + i, 0, sw - 1);
+                    const float dx = (float)((x0 + i - cx) * inv_w);
+                    const float w = or_lut_lookup(p->lut, p->lut_n,
+                                                  sqrtf(dx * dx + dy * dy) * p->lut_scale);
+                    acc += (double)w * src[(size_t)ty * sw + tx];
+                    wsum += w;
+                }
+            }
+            dst[(size_t)oy * dw + ox] = (float)(wsum > 1e-6 ? acc / wsum : 0.0);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* whole-frame orchestration: plugin.cpp:177-721                             */
+/* ------------------------------------------------------------------------ */
+
+static __thread or_debug g_dbg;
+
+const or_debug *or_last_debug(void) { return &g_dbg; }
+
+void or_clear_debug(void) {
+    free(g_dbg.y); free(g_dbg.u); free(g_dbg.v);
+    free(g_dbg.jxx); free(g_dbg.jxy); free(g_dbg.jyy); free(g_dbg.lc); free(g_dbg.ms);
+    free(g_dbg.seed); free(g_dbg.mask);
+    if (g_dbg.has_ws)
+        or_free_workset(&g_dbg.ws);
+    free(g_dbg.plain_u); free(g_dbg.plain_v);
+    free(g_dbg.out_u); free(g_dbg.out_v);
+    memset(&g_dbg, 0, sizeof(g_dbg));
+}
+
+static void to_float(const or_frame_format *f, const void *src, ptrdiff_t stride, float *dst,
+                     int w, int h, double scale, double offset) {
+    if (f->sample_type == 1) {
+        for (int y = 0; y < h; ++y)
+            memcpy(dst + (size_t)y * w, (const uint8_t *)src + stride * y, sizeof(float) * (size_t)w);
+    } else if (f->bytes_per_sample == 1) {
+        or_u8_to_float((const uint8_t *)src, stride, dst, w, h, scale, offset);
+    } else {
+        or_u16_to_float((const uint16_t *)src, stride, dst, w, h, scale, offset);
+    }
+}
+
+static void from_float(const or_frame_format *f, const float *src, void *dst, ptrdiff_t stride,
+                       int w, int h, double scale, double offset) {
+    if (f->sample_type == 1) {
+        for (int y = 0; y < h; ++y)
+            memcpy((uint8_t *)dst + stride * y, src + (size_t)y * w, sizeof(float) * (size_t)w);
+    } else if (f->bytes_per_sample == 1) {
+        or_float_to_u8(src, (uint8_t *)dst, stride, w, h, scale, offset, 0, 255);
+    } else {
+        or_float_to_u16(src, (uint16_t *)dst, stride, w, h, scale, offset, 0, (1 << f->bits) - 1);
+    }
+}
+
+int or_recon_frame(const or_params *p, const or_frame_format *f,
+                   const void *const in[3], const ptrdiff_t in_stride[3],
+                   void *const out[3], const ptrdiff_t out_stride[3]) {
+    or_clear_debug();
+    if (p->algo != 2 && p->strength > 0.0)
+        return -2; /* algo 4/6 restated separately */
+    if (p->bm)
+        return -2;
+    const int sw = f->width, sh = f->height;
+    const int cw = sw >> f->ss_w, ch = sh >> f->ss_h;
+    const int ow = p->out_w, oh = p->out_h;
+    const int is_float = f->sample_type == 1;
+    const double yscale = is_float ? 1.0 : 1.0 / (double)((1 << f->bits) - 1);
+    const double coff = is_float ? 0.0 : -0.5;
+    or_debug *D = &g_dbg;
+    D->sw = sw; D->sh = sh; D->cw = cw; D->ch = ch; D->ow = ow; D->oh = oh;
+    const size_t ns = (size_t)sw * sh, nc = (size_t)cw * ch, no = (size_t)ow * oh;
+    D->y = (float *)malloc(sizeof(float) * ns);
+    D->u = (float *)malloc(sizeof(float) * nc);
+    D->v = (float *)malloc(sizeof(float) * nc);
+    to_float(f, in[0], in_stride[0], D->y, sw, sh, yscale, 0.0);
+    to_float(f, in[1], in_stride[1], D->u, cw, ch, yscale, coff);
+    to_float(f, in[2], in_stride[2], D->v, cw, ch, yscale, coff);
+    const double rw = (double)sw / cw, rh = (double)sh / ch;
+    const int same = ow == sw && oh == sh;
+    const double oscale = is_float ? 1.0 : (double)((1 << f->bits) - 1);
+
+    /* luma (plugin.cpp:273-316) */
+    if (same) {
+        for (int y = 0; y < sh; ++y)
+            memcpy((uint8_t *)out[0] + out_stride[0] * y, (const uint8_t *)in[0] + in_stride[0] * y,
+                   (size_t)sw * (is_float ? 4 : f->bytes_per_sample));
+    } else {
+        float *oy = (float *)malloc(sizeof(float) * no);
+        if (p->radial) {
+            resample_radial(p, D->y, sw, sh, oy, ow, oh);
+        } else {
+            or_weights th, tv;
+            or_build_weights(sw, ow, p->kernel, p->kp1, p->kp2, p->support, 0.0, &th);
+            or_build_weights(sh, oh, p->kernel, p->kp1, p->kp2, p->support, 0.0, &tv);
+            float *tmp = (float *)malloc(sizeof(float) * (size_t)ow * sh);
+            resample_h(D->y, sw, sh, tmp, &th);
+            resample_v(tmp, ow, sh, oy, &tv);
+            free(tmp);
+            or_free_weights(&th);
+            or_free_weights(&tv);
+        }
+        from_float(f, oy, out[0], out_stride[0], ow, oh, oscale, 0.0);
+        free(oy);
+    }
+
+    /* guide maps, trust mask, workset, mutual gate (plugin.cpp:318-369) */
+    or_axis ax, ay;
+    or_build_axis(sw, ow, rw, p->shift_x, p, &ax);
+    or_build_axis(sh, oh, rh, p->shift_y, p, &ay);
+    if (p->strength > 0.0) {
+        const double thr = p->algo == 4 ? 0.25 * p->sigma : p->sigma;
+        D->jxx = (float *)malloc(sizeof(float) * ns);
+        D->jxy = (float *)malloc(sizeof(float) * ns);
+        D->jyy = (float *)malloc(sizeof(float) * ns);
+        D->lc = (float *)malloc(sizeof(float) * nc);
+        if (p->sparse)
+            D->seed = (uint8_t *)malloc(ns);
+        or_sobel_tensor(D->y, sw, sh, D->jxx, D->jxy, D->jyy, D->seed, (float)(thr * thr));
+        or_lc_map(D->y, sw, sh, cw, ch, rw, rh, p->shift_x, p->shift_y, D->lc);
+        if (p->sparse) {
+            const double rmax = rw > rh ? rw : rh;
+            const int dil = (int)ceil(p->support * rmax) + 8;
+            D->mask = (uint8_t *)malloc(ns);
+            or_dilate(D->seed, sw, sh, dil, D->mask);
+            or_build_workset(D->mask, sw, sh, ow, oh, &ax, &ay, cw, ch, &D->ws);
+            D->has_ws = 1;
+        }
+        if (p->ms > 0.0) {
+            D->ms = (float *)malloc(sizeof(float) * nc);
+            or_mutual_gate(D->lc, D->u, D->v, cw, ch, D->has_ws ? D->ws.chroma_mask : NULL,
+                           D->has_ws ? or_workset_chroma_dense(&D->ws) : 1, D->ms);
+        }
+    }
+
+    D->out_u = (float *)malloc(sizeof(float) * no);
+    D->out_v = (float *)malloc(sizeof(float) * no);
+    if (p->strength == 0.0) {
+        or_plain_chroma(p, D->u, D->v, cw, ch, sw, sh, D->out_u, D->out_v);
+    } else {
+        if (D->has_ws) {
+            or_plain_chroma(p, D->u, D->v, cw, ch, sw, sh, D->out_u, D->out_v);
+            D->plain_u = (float *)malloc(sizeof(float) * no);
+            D->plain_v = (float *)malloc(sizeof(float) * no);
+            memcpy(D->plain_u, D->out_u, sizeof(float) * no);
+            memcpy(D->plain_v, D->out_v, sizeof(float) * no);
+        }
+        or_guide g = { D->y, sw, sh, D->jxx, D->jxy, D->jyy, D->lc, D->ms };
+        or_guided(p, D->u, D->v, cw, ch, &g, D->has_ws ? &D->ws : NULL, D->out_u, D->out_v);
+    }
+    or_free_axis(&ax);
+    or_free_axis(&ay);
+
+    /* back-projection, same-size 4:2:0 only (plugin.cpp:374-375, 683-689) */
+    if (p->bp > 0.0 && same && cw * 2 == sw && ch * 2 == sh) {
+        or_back_project(D->out_u, ow, oh, D->u, cw, ch, (float)p->bp);
+        or_back_project(D->out_v, ow, oh, D->v, cw, ch, (float)p->bp);
+    }
+
+    /* output conversion (plugin.cpp:691-711) */
+    from_float(f, D->out_u, out[1], out_stride[1], ow, oh, oscale, is_float ? 0.0 : 0.5 * oscale);
+    from_float(f, D->out_v, out[2], out_stride[2], ow, oh, oscale, is_float ? 0.0 : 0.5 * oscale);
+    return 0;
+}

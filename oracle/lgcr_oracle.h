@@ -1,0 +1,163 @@
+/*
+ * LGCR oracle — TEST INFRASTRUCTURE ONLY.
+ *
+ * A scalar CPU restatement, in plain C, of the reference's chroma
+ * reconstruction path as its scalar build computes it (ScalarBackend, every
+ * AVX2 block compiled out, no FMA contraction).  Only tests/, the smoke()
+ * entry and bench.py's CPU-baseline leg may load it, and only as the checker
+ * or the timed CPU baseline.  Nothing in the product library links it.
+ *
+ * Planes are contiguous row-major float arrays (stride == width) unless a
+ * stride argument says otherwise.
+ */
+#ifndef LGCR_ORACLE_H
+#define LGCR_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OR_BILINEAR = 0, OR_BICUBIC, OR_LANCZOS, OR_SPLINE16, OR_SPLINE36, OR_JINC };
+
+typedef struct or_params {
+    int kernel;
+    double kp1, kp2, support;
+    int radial;
+    const float *lut;      /* jinc profile LUT (radial only) */
+    int lut_n;
+    float lut_scale;
+    double strength, sigma, sratio, sdb, reg, stretch, gsigma, rescue;
+    int algo, ridge, cedge;
+    double ms, qgate;
+    int sparse;
+    double bp;
+    int bm;
+    double ar_margin;
+    double shift_x, shift_y;
+    int out_w, out_h;
+    /* TRecon */
+    int trad, tsearch;
+    double tsad;
+} or_params;
+
+typedef struct or_weights {
+    int n, sup;
+    int *start;
+    float *w;
+} or_weights;
+
+typedef struct or_axis {
+    int n, sup;
+    int *start;
+    float *w, *am;
+    float *wsum, *wabs;
+    int *active;
+    float *pos, *lpos;
+    int *cb_i0; float *cb_f;   /* pos on the source chroma grid */
+    int *lb_i0; float *lb_f;   /* lpos on the source luma grid */
+} or_axis;
+
+typedef struct or_workset {
+    int mask_w, mask_h, out_w, out_h, chroma_w, chroma_h;
+    uint8_t *mask;               /* mask_w * mask_h (owned copy) */
+    uint32_t *indices;           /* active output pixel indices, row-major */
+    size_t n_indices;
+    size_t *index_row_offsets;   /* out_h + 1 */
+    int *spans;                  /* (begin, end) pairs */
+    size_t n_spans;
+    size_t *span_row_offsets;    /* out_h + 1 */
+    uint8_t *chroma_mask;        /* chroma_w * chroma_h */
+    size_t active_chroma;
+} or_workset;
+
+/* closed forms (lgcr.h:53-117) */
+double or_kernel_eval(int kernel, double x, double p1, double p2);
+double or_jinc_windowed(double x, double taps);
+double or_kernel_support(int kernel, double p1);
+void or_jinc_lut(double support, float *lut, int n);
+float or_lut_lookup(const float *lut, int n, float idx);
+
+/* tables (kernels.cpp:135-167, 337-404; maps.cpp:190-249) */
+int or_build_weights(int src_n, int dst_n, int kernel, double kp1, double kp2,
+                     double support, double shift, or_weights *out);
+void or_free_weights(or_weights *w);
+int or_build_axis(int src_luma_n, int dst_luma_n, double r, double sit_shift,
+                  const or_params *p, or_axis *out);
+void or_free_axis(or_axis *a);
+void or_bilin_axis(const float *pos, int n, int src_n, int *i0, float *f);
+
+/* conversions (lgcr.h:986-1013) */
+void or_u8_to_float(const uint8_t *src, ptrdiff_t stride, float *dst, int w, int h,
+                    double scale, double offset);
+void or_u16_to_float(const uint16_t *src, ptrdiff_t stride, float *dst, int w, int h,
+                     double scale, double offset);
+void or_float_to_u8(const float *src, uint8_t *dst, ptrdiff_t stride, int w, int h,
+                    double scale, double offset, int lo, int hi);
+void or_float_to_u16(const float *src, uint16_t *dst, ptrdiff_t stride, int w, int h,
+                     double scale, double offset, int lo, int hi);
+
+/* guide maps (maps.cpp:9-168, 252-340, 425-499) */
+void or_sobel_tensor(const float *y, int w, int h, float *jxx, float *jxy, float *jyy,
+                     uint8_t *seed, float threshold);
+void or_lc_map(const float *y, int w, int h, int cw, int ch, double rw, double rh,
+               double shift_x, double shift_y, float *lc);
+void or_dilate(const uint8_t *src, int w, int h, int radius, uint8_t *dst);
+int or_build_workset(const uint8_t *mask, int mask_w, int mask_h, int out_w, int out_h,
+                     const or_axis *ax, const or_axis *ay, int cw, int ch,
+                     or_workset *out);
+void or_free_workset(or_workset *ws);
+int or_workset_output_dense(const or_workset *ws);
+int or_workset_chroma_dense(const or_workset *ws);
+
+/* mutual-structure gate (maps.cpp:831-948); chroma_mask may be NULL (dense) */
+void or_mutual_gate(const float *lc, const float *u, const float *v, int w, int h,
+                    const uint8_t *chroma_mask, int chroma_dense, float *gate);
+
+/* plain chroma (recon.cpp:1151-1461, 828-1029, 1031-1149) */
+void or_plain_chroma(const or_params *p, const float *u, const float *v, int cw, int ch,
+                     int sw, int sh, float *out_u, float *out_v);
+
+/* guided reconstruction, algo 2 (recon.cpp:14-819) without temporal taps.
+ * workset may be NULL (dense visit of every output pixel). */
+typedef struct or_guide {
+    const float *y; int sw, sh;          /* source luma */
+    const float *jxx, *jxy, *jyy;        /* sw x sh */
+    const float *lc;                     /* cw x ch */
+    const float *ms;                     /* cw x ch or NULL */
+} or_guide;
+void or_guided(const or_params *p, const float *u, const float *v, int cw, int ch,
+               const or_guide *g, const or_workset *ws, float *out_u, float *out_v);
+
+/* back-projection (algos.cpp:663-677) */
+void or_back_project(float *out, int ow, int oh, const float *src, int cw, int ch, float bp);
+
+/* whole-frame Recon (plugin.cpp:177-721).  sample_type 0 = integer, 1 = float.
+ * in[] / out[] are host planes with byte strides.  Intermediates of the last
+ * call are kept for inspection (or_last_*). */
+typedef struct or_frame_format {
+    int sample_type, bits, bytes_per_sample, ss_w, ss_h, width, height;
+} or_frame_format;
+int or_recon_frame(const or_params *p, const or_frame_format *fmt,
+                   const void *const in[3], const ptrdiff_t in_stride[3],
+                   void *const out[3], const ptrdiff_t out_stride[3]);
+
+/* intermediates of the last or_recon_frame on this thread */
+typedef struct or_debug {
+    int sw, sh, cw, ch, ow, oh;
+    float *y, *u, *v;            /* converted source planes */
+    float *jxx, *jxy, *jyy, *lc, *ms;
+    uint8_t *seed, *mask;
+    or_workset ws; int has_ws;
+    float *plain_u, *plain_v;    /* plain base (algo 2 sparse) */
+    float *out_u, *out_v;        /* float result before output conversion */
+} or_debug;
+const or_debug *or_last_debug(void);
+void or_clear_debug(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

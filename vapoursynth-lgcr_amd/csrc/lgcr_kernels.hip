@@ -1,0 +1,1029 @@
+// gfx950 kernels of the LGCR Recon path.
+//
+// Everything is FP32 with FP64 accumulators wherever the reference's scalar
+// build accumulates in double; the library is compiled with
+// -ffp-contract=off so every product and sum rounds as written.  Comments
+// cite the reference stage each kernel computes.
+#include "lgcr_internal.h"
+
+#include <algorithm>
+
+namespace lgcr_hip {
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+// std::min / std::max / std::clamp semantics (NaN propagation included).
+__device__ __forceinline__ float fmn(float a, float b) { return b < a ? b : a; }
+__device__ __forceinline__ float fmx(float a, float b) { return a < b ? b : a; }
+__device__ __forceinline__ float fclamp(float v, float lo, float hi) { return v < lo ? lo : (hi < v ? hi : v); }
+__device__ __forceinline__ double dclamp(double v, double lo, double hi) { return v < lo ? lo : (hi < v ? hi : v); }
+
+__device__ __forceinline__ float at(const float *p, int w, int h, int x, int y) {
+    return p[(size_t)clampi(y, 0, h - 1) * w + clampi(x, 0, w - 1)];
+}
+
+// Bilinear sample at double coordinates with clamped taps (lgcr.h:467-475).
+__device__ float bilinear(const float *p, int w, int h, double x, double y) {
+    const int x0 = (int)floor(x), y0 = (int)floor(y);
+    const float fx = (float)(x - x0), fy = (float)(y - y0);
+    const float a = at(p, w, h, x0, y0), b = at(p, w, h, x0 + 1, y0);
+    const float c = at(p, w, h, x0, y0 + 1), d = at(p, w, h, x0 + 1, y0 + 1);
+    return a + (b - a) * fx + (c - a) * fy + (a - b - c + d) * fx * fy;
+}
+
+// Bilinear sample with pre-clamped index/fraction (lgcr.h:494-508).
+__device__ float bilinear_fast(const float *p, int w, int h, int x0, float fx, int y0, float fy) {
+    if (w == 1) {
+        const float a = p[(size_t)y0 * w];
+        if (h == 1)
+            return a;
+        const float c = p[(size_t)(y0 + 1) * w];
+        return a + (c - a) * fy;
+    }
+    const float *r0 = p + (size_t)y0 * w + x0;
+    if (h == 1)
+        return r0[0] + (r0[1] - r0[0]) * fx;
+    const float *r1 = r0 + w;
+    const float a = r0[0], b = r0[1], c = r1[0], d = r1[1];
+    return a + (b - a) * fx + (c - a) * fy + (a - b - c + d) * fx * fy;
+}
+
+struct Dir { float coherence, nx, ny; };
+
+// Principal eigenvector of a 2x2 structure tensor (lgcr.h:516-538).
+__device__ Dir principal_dir(float jxx, float jxy, float jyy) {
+    Dir d{0.0f, 1.0f, 0.0f};
+    const float diff = jxx - jyy;
+    const float sum = jxx + jyy;
+    const float cross = 2.0f * jxy;
+    const float disc = sqrtf(diff * diff + cross * cross);
+    d.coherence = disc / (sum + 1e-12f);
+    if (disc <= 1e-20f)
+        return d;
+    float vx, vy;
+    if (diff >= 0.0f) { vx = disc + diff; vy = cross; }
+    else { vx = cross; vy = disc - diff; }
+    const float inv = 1.0f / sqrtf(vx * vx + vy * vy);
+    d.nx = vx * inv;
+    d.ny = vy * inv;
+    return d;
+}
+
+// 3x3 Sobel / 8 with clamped taps (maps.cpp:9-20).
+__device__ __forceinline__ void sobel(const float *p, int w, int h, int x, int y, float &gx, float &gy) {
+    const float tl = at(p, w, h, x - 1, y - 1), t = at(p, w, h, x, y - 1), tr = at(p, w, h, x + 1, y - 1);
+    const float l = at(p, w, h, x - 1, y), r = at(p, w, h, x + 1, y);
+    const float bl = at(p, w, h, x - 1, y + 1), b = at(p, w, h, x, y + 1), br = at(p, w, h, x + 1, y + 1);
+    gx = (tr + 2.0f * r + br - tl - 2.0f * l - bl) * 0.125f;
+    gy = (bl + 2.0f * b + br - tl - 2.0f * t - tr) * 0.125f;
+}
+
+// ---------------------------------------------------------------------------
+// format conversion (planeToFloat / floatToPlane, lgcr.h:986-1013)
+// ---------------------------------------------------------------------------
+
+struct InBatch { RawPlane p[3]; };
+struct OutBatch { RawOut p[3]; };
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_to_float(InBatch b) {
+    const RawPlane &P = b.p[blockIdx.z];
+    const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
+    if (x >= P.w || y >= P.h)
+        return;
+    const T *row = reinterpret_cast<const T *>(static_cast<const char *>(P.data) + P.stride * y);
+    P.dst[(size_t)y * P.w + x] = (float)((double)row[x] * P.scale + P.offset);
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy_f32(InBatch b) {
+    const RawPlane &P = b.p[blockIdx.z];
+    const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
+    if (x >= P.w || y >= P.h)
+        return;
+    const float *row = reinterpret_cast<const float *>(static_cast<const char *>(P.data) + P.stride * y);
+    P.dst[(size_t)y * P.w + x] = row[x];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_from_float(OutBatch b) {
+    const RawOut &P = b.p[blockIdx.z];
+    const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
+    if (x >= P.w || y >= P.h)
+        return;
+    T *row = reinterpret_cast<T *>(static_cast<char *>(P.data) + P.stride * y);
+    const float s = P.src[(size_t)y * P.w + x];
+    const int v = (int)(s * P.scale + P.offset + 0.5f);
+    row[x] = (T)clampi(v, P.lo, P.hi);
+}
+
+__global__ __launch_bounds__(kBlock) void k_from_float_f32(OutBatch b) {
+    const RawOut &P = b.p[blockIdx.z];
+    const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
+    if (x >= P.w || y >= P.h)
+        return;
+    float *row = reinterpret_cast<float *>(static_cast<char *>(P.data) + P.stride * y);
+    row[x] = (float)((double)P.src[(size_t)y * P.w + x] * 1.0 + 0.0);
+}
+
+// ---------------------------------------------------------------------------
+// structure tensor + fused trust seed (sobelTensor3x3, maps.cpp:64-168)
+//
+// One 64x4 output tile per block.  LDS holds the luma tile with a 2-pixel
+// halo, the Sobel gradients (1-pixel halo) evaluated at clamped centres, and
+// the vertical 3-row sums, so every border rule of the reference falls out
+// of the clamped centres.
+// ---------------------------------------------------------------------------
+
+constexpr int TW = 64, TH = 4;
+
+__global__ __launch_bounds__(TW * TH) void k_sobel_tensor(const float *__restrict__ y, int w, int h,
+                                                         float *jxx, float *jxy, float *jyy,
+                                                         uint8_t *seed, float thr) {
+    __shared__ float sy[TH + 4][TW + 4];
+    __shared__ float gx[TH + 2][TW + 2], gy[TH + 2][TW + 2];
+    __shared__ float vxx[TH][TW + 2], vxy[TH][TW + 2], vyy[TH][TW + 2];
+    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+    const int tid = threadIdx.y * TW + threadIdx.x;
+    for (int i = tid; i < (TH + 4) * (TW + 4); i += TW * TH) {
+        const int ly = i / (TW + 4), lx = i % (TW + 4);
+        sy[ly][lx] = y[(size_t)clampi(y0 - 2 + ly, 0, h - 1) * w + clampi(x0 - 2 + lx, 0, w - 1)];
+    }
+    __syncthreads();
+    for (int i = tid; i < (TH + 2) * (TW + 2); i += TW * TH) {
+        const int ly = i / (TW + 2), lx = i % (TW + 2);
+        const int cx = clampi(x0 - 1 + lx, 0, w - 1), cy = clampi(y0 - 1 + ly, 0, h - 1);
+        // tap coordinates clamped to the image, then made tile-relative
+        const int xl = clampi(cx - 1, 0, w - 1) - (x0 - 2), xc = cx - (x0 - 2), xr = clampi(cx + 1, 0, w - 1) - (x0 - 2);
+        const int yt = clampi(cy - 1, 0, h - 1) - (y0 - 2), yc = cy - (y0 - 2), yb = clampi(cy + 1, 0, h - 1) - (y0 - 2);
+        const float tl = sy[yt][xl], t = sy[yt][xc], tr = sy[yt][xr];
+        const float l = sy[yc][xl], r = sy[yc][xr];
+        const float bl = sy[yb][xl], b = sy[yb][xc], br = sy[yb][xr];
+        gx[ly][lx] = (tr + 2.0f * r + br - tl - 2.0f * l - bl) * 0.125f;
+        gy[ly][lx] = (bl + 2.0f * b + br - tl - 2.0f * t - tr) * 0.125f;
+    }
+    __syncthreads();
+    for (int i = tid; i < TH * (TW + 2); i += TW * TH) {
+        const int ly = i / (TW + 2), lx = i % (TW + 2);
+        const float a0 = gx[ly][lx], a1 = gx[ly + 1][lx], a2 = gx[ly + 2][lx];
+        const float b0 = gy[ly][lx], b1 = gy[ly + 1][lx], b2 = gy[ly + 2][lx];
+        vxx[ly][lx] = a0 * a0 + a1 * a1 + a2 * a2;
+        vxy[ly][lx] = a0 * b0 + a1 * b1 + a2 * b2;
+        vyy[ly][lx] = b0 * b0 + b1 * b1 + b2 * b2;
+    }
+    __syncthreads();
+    const int x = x0 + threadIdx.x, r = y0 + threadIdx.y;
+    if (x >= w || r >= h)
+        return;
+    const int ly = threadIdx.y;
+    const int c = x - (x0 - 1);  // column of x in the V arrays
+    float oxx, oxy, oyy;
+    if (x == 0) {
+        const int c1 = (w > 1 ? 1 : 0) - (x0 - 1);
+        oxx = (2.0f * vxx[ly][c] + vxx[ly][c1]) / 9.0f;
+        oxy = (2.0f * vxy[ly][c] + vxy[ly][c1]) / 9.0f;
+        oyy = (2.0f * vyy[ly][c] + vyy[ly][c1]) / 9.0f;
+    } else if (x == w - 1) {
+        oxx = (vxx[ly][c - 1] + 2.0f * vxx[ly][c]) / 9.0f;
+        oxy = (vxy[ly][c - 1] + 2.0f * vxy[ly][c]) / 9.0f;
+        oyy = (vyy[ly][c - 1] + 2.0f * vyy[ly][c]) / 9.0f;
+    } else {
+        const float ninth = 1.0f / 9.0f;
+        oxx = ninth * ((vxx[ly][c - 1] + vxx[ly][c]) + vxx[ly][c + 1]);
+        oxy = ninth * ((vxy[ly][c - 1] + vxy[ly][c]) + vxy[ly][c + 1]);
+        oyy = ninth * ((vyy[ly][c - 1] + vyy[ly][c]) + vyy[ly][c + 1]);
+    }
+    const size_t o = (size_t)r * w + x;
+    jxx[o] = oxx;
+    jxy[o] = oxy;
+    jyy[o] = oyy;
+    if (seed)
+        seed[o] = oxx + oyy > thr;
+}
+
+// ---------------------------------------------------------------------------
+// footprint luma map (buildLcMap, maps.cpp:425-499)
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_lc_map(const float *__restrict__ y, int w, float *lc, int cw,
+                                                   int ch, const int *xb, const int *xe, const int *yb,
+                                                   const int *ye) {
+    const int cx = blockIdx.x * kBlock + threadIdx.x, cy = blockIdx.y;
+    if (cx >= cw || cy >= ch)
+        return;
+    const int x0 = xb[cx], x1 = xe[cx], y0 = yb[cy], y1 = ye[cy];
+    double sum = 0.0;
+    for (int j = y0; j < y1; ++j)
+        for (int i = x0; i < x1; ++i)
+            sum += y[(size_t)j * w + i];
+    lc[(size_t)cy * cw + cx] = (float)(sum / ((double)(x1 - x0) * (y1 - y0)));
+}
+
+// ---------------------------------------------------------------------------
+// trust-mask dilation (buildTrustMask, maps.cpp:295-339): a pixel is set when
+// any seed lies within Chebyshev distance r.  Horizontal pass: one row per
+// block staged in LDS, running counts over 8-pixel runs; vertical pass:
+// running counts down 16-row column segments.
+// ---------------------------------------------------------------------------
+
+constexpr int kDilRun = 8;
+
+__global__ __launch_bounds__(kBlock) void k_dilate_h(const uint8_t *src, uint8_t *dst, int w, int r) {
+    extern __shared__ uint8_t row[];  // w + 2r, zero padded
+    const int y = blockIdx.x;
+    const uint8_t *s = src + (size_t)y * w;
+    for (int i = threadIdx.x; i < w + 2 * r; i += kBlock) {
+        const int x = i - r;
+        row[i] = (x >= 0 && x < w) ? (s[x] != 0) : 0;
+    }
+    __syncthreads();
+    for (int x0 = threadIdx.x * kDilRun; x0 < w; x0 += kBlock * kDilRun) {
+        int cnt = 0;
+        for (int i = 0; i <= 2 * r; ++i)  // window [x0 - r, x0 + r]
+            cnt += row[x0 + i];
+        const int x1 = min(w, x0 + kDilRun);
+        for (int x = x0; x < x1; ++x) {
+            dst[(size_t)y * w + x] = cnt != 0;
+            if (x + 1 < x1)  // slide to [x + 1 - r, x + 1 + r]; LDS is zero padded
+                cnt += row[x + 2 * r + 1] - row[x];
+        }
+    }
+}
+
+constexpr int kDilRows = 16;
+
+__global__ __launch_bounds__(kBlock) void k_dilate_v(const uint8_t *src, uint8_t *dst, int w, int h, int r) {
+    const int x = blockIdx.x * kBlock + threadIdx.x;
+    const int y0 = blockIdx.y * kDilRows;
+    if (x >= w)
+        return;
+    int cnt = 0;
+    for (int j = y0 - r; j <= y0 + r; ++j)
+        if (j >= 0 && j < h)
+            cnt += src[(size_t)j * w + x] != 0;
+    const int y1 = min(h, y0 + kDilRows);
+    for (int y = y0; y < y1; ++y) {
+        dst[(size_t)y * w + x] = cnt != 0;
+        const int rem = y - r, add = y + r + 1;
+        if (rem >= 0)
+            cnt -= src[(size_t)rem * w + x] != 0;
+        if (add < h)
+            cnt += src[(size_t)add * w + x] != 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// sparse workset (buildSparseWorkset, maps.cpp:342-422): row-major list of
+// active output pixels via per-row counts, an exclusive scan over rows and an
+// ordered per-row compaction (wave ballots), then the chroma mask of the
+// bilinear footprints and its population.
+// ---------------------------------------------------------------------------
+
+struct MaskMap { const uint8_t *mask; int mw, mh, ow, oh; };
+
+__device__ __forceinline__ bool out_active(const MaskMap &m, int x, int y) {
+    const int mx = min(m.mw - 1, (int)((int64_t)x * m.mw / m.ow));
+    const int my = min(m.mh - 1, (int)((int64_t)y * m.mh / m.oh));
+    return m.mask[(size_t)my * m.mw + mx] != 0;
+}
+
+__device__ __forceinline__ int block_sum(int v, int *scratch) {
+    // wave64 reduction then across the 4 waves of the block
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        scratch[wave] = v;
+    __syncthreads();
+    int t = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < kBlock / 64; ++i)
+            t += scratch[i];
+    __syncthreads();
+    return t;  // valid in thread 0
+}
+
+__global__ __launch_bounds__(kBlock) void k_row_count(MaskMap m, uint32_t *row_count) {
+    __shared__ int scratch[kBlock / 64];
+    const int y = blockIdx.x;
+    int c = 0;
+    for (int x = threadIdx.x; x < m.ow; x += kBlock)
+        c += out_active(m, x, y);
+    const int t = block_sum(c, scratch);
+    if (threadIdx.x == 0)
+        row_count[y] = (uint32_t)t;
+}
+
+// Exclusive scan of n row counts in place; off[n] receives the total.
+__global__ __launch_bounds__(1024) void k_scan_rows(uint32_t *off, int n) {
+    __shared__ uint32_t part[1024];
+    const int per = (n + 1023) / 1024;
+    const int b = threadIdx.x * per, e = min(n, b + per);
+    uint32_t s = 0;
+    for (int i = b; i < e; ++i)
+        s += off[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        const uint32_t v = threadIdx.x >= (unsigned)d ? part[threadIdx.x - d] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - s;  // exclusive prefix of this chunk
+    for (int i = b; i < e; ++i) {
+        const uint32_t c = off[i];
+        off[i] = run;
+        run += c;
+    }
+    if (threadIdx.x == 1023)
+        off[n] = part[1023];
+}
+
+__global__ __launch_bounds__(kBlock) void k_compact(MaskMap m, const uint32_t *off, uint32_t *indices) {
+    __shared__ int wave_cnt[kBlock / 64];
+    const int y = blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t base = off[y];
+    for (int x0 = 0; x0 < m.ow; x0 += kBlock) {
+        const int x = x0 + threadIdx.x;
+        const bool a = x < m.ow && out_active(m, x, y);
+        const uint64_t bal = __ballot(a);
+        const int before = __popcll(bal & ((1ull << lane) - 1ull));
+        if (lane == 0)
+            wave_cnt[wave] = __popcll(bal);
+        __syncthreads();
+        int prior = 0, total = 0;
+        for (int i = 0; i < kBlock / 64; ++i) {
+            prior += i < wave ? wave_cnt[i] : 0;
+            total += wave_cnt[i];
+        }
+        if (a)
+            indices[base + prior + before] = (uint32_t)((size_t)y * m.ow + x);
+        base += total;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_u8(uint8_t *p, size_t n, uint8_t v) {
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+        p[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_chroma_mark(MaskMap m, const int *cx_i0, const int *cy_i0,
+                                                         uint8_t *cmask, int cw, int ch) {
+    const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
+    if (x >= m.ow || !out_active(m, x, y))
+        return;
+    const int cy0 = cy_i0[y], cy1 = min(cy0 + 1, ch - 1);
+    const int cx0 = cx_i0[x], cx1 = min(cx0 + 1, cw - 1);
+    cmask[(size_t)cy0 * cw + cx0] = 1;
+    cmask[(size_t)cy0 * cw + cx1] = 1;
+    cmask[(size_t)cy1 * cw + cx0] = 1;
+    cmask[(size_t)cy1 * cw + cx1] = 1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_count_u8(const uint8_t *p, size_t n, uint32_t *counter) {
+    __shared__ int scratch[kBlock / 64];
+    int c = 0;
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+        c += p[i] != 0;
+    const int t = block_sum(c, scratch);
+    if (threadIdx.x == 0 && t)
+        atomicAdd(counter, (uint32_t)t);
+}
+
+__global__ void k_zero_u32(uint32_t *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// mutual-structure gate (buildMutualGate, maps.cpp:831-948)
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_grad3(const float *lc, const float *u, const float *v,
+                                                  int w, int h, float *g) {
+    const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
+    if (x >= w)
+        return;
+    const size_t n = (size_t)w * h, i = (size_t)y * w + x;
+    float a, b;
+    sobel(lc, w, h, x, y, a, b); g[i] = a; g[n + i] = b;
+    sobel(u, w, h, x, y, a, b); g[2 * n + i] = a; g[3 * n + i] = b;
+    sobel(v, w, h, x, y, a, b); g[4 * n + i] = a; g[5 * n + i] = b;
+}
+
+// SobelRowRing::sampleProjected (maps.cpp:782-791) at bilinearPoint
+// (maps.cpp:175-186).
+__device__ __forceinline__ float projected(const float *gx, const float *gy, int w, int h,
+                                           int x0, int x1, int y0, int y1, float fx, float fy,
+                                           float nx, float ny) {
+    const float a = gx[(size_t)y0 * w + x0] * nx + gy[(size_t)y0 * w + x0] * ny;
+    const float b = gx[(size_t)y0 * w + x1] * nx + gy[(size_t)y0 * w + x1] * ny;
+    const float c = gx[(size_t)y1 * w + x0] * nx + gy[(size_t)y1 * w + x0] * ny;
+    const float d = gx[(size_t)y1 * w + x1] * nx + gy[(size_t)y1 * w + x1] * ny;
+    return a + (b - a) * fx + (c - a) * fy + (a - b - c + d) * fx * fy;
+}
+
+__global__ __launch_bounds__(kBlock) void k_gate(const float *__restrict__ g, int w, int h,
+                                                 const uint8_t *cmask, const uint32_t *counters,
+                                                 float *gate) {
+    const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
+    if (x >= w)
+        return;
+    const size_t n = (size_t)w * h;
+    const bool dense = !cmask || (uint64_t)counters[0] * 2 >= (uint64_t)n;
+    float *out = gate + (size_t)y * w + x;
+    if (!dense && cmask[(size_t)y * w + x] == 0) {
+        *out = 0.0f;
+        return;
+    }
+    const float *lx = g, *ly = g + n, *ux = g + 2 * n, *uy = g + 3 * n, *vx = g + 4 * n, *vy = g + 5 * n;
+    float sxx = 0, sxy = 0, syy = 0;
+    for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+            const float a = at(lx, w, h, x + dx, y + dy);
+            const float b = at(ly, w, h, x + dx, y + dy);
+            sxx += a * a;
+            sxy += a * b;
+            syy += b * b;
+        }
+    if (sxx + syy < 1e-10f) {
+        *out = 0.0f;
+        return;
+    }
+    const Dir d = principal_dir(sxx, sxy, syy);
+    const float nx = d.nx, ny = d.ny;
+    float sumY = 0, sqY = 0, momY = 0, sumC = 0, sqC = 0, momC = 0;
+    for (int k = -3; k <= 3; ++k) {
+        const double px = (float)x + (float)k * nx;
+        const double py = (float)y + (float)k * ny;
+        const int rx = (int)floor(px), ry = (int)floor(py);
+        const int x0 = clampi(rx, 0, w - 1), x1 = clampi(rx + 1, 0, w - 1);
+        const int y0 = clampi(ry, 0, h - 1), y1 = clampi(ry + 1, 0, h - 1);
+        const float fx = (float)(px - rx), fy = (float)(py - ry);
+        const float gY = fabsf(projected(lx, ly, w, h, x0, x1, y0, y1, fx, fy, nx, ny));
+        const float au = projected(ux, uy, w, h, x0, x1, y0, y1, fx, fy, nx, ny);
+        const float av = projected(vx, vy, w, h, x0, x1, y0, y1, fx, fy, nx, ny);
+        const float gC = sqrtf(au * au + av * av);
+        sumY += gY; sqY += gY * gY; momY += (float)k * gY;
+        sumC += gC; sqC += gC * gC; momC += (float)k * gC;
+    }
+    if (sumY < 1e-9f || sumC < 1e-9f) {
+        *out = 0.0f;
+        return;
+    }
+    const float partY = sumY * sumY / (sqY + 1e-12f);
+    const float partC = sumC * sumC / (sqC + 1e-12f);
+    const float ratio = partC / (partY + 1e-6f);
+    const float dcent = fabsf(momY / sumY - momC / sumC);
+    const float tw = fclamp((1.6f - ratio) / 0.4f, 0.0f, 1.0f);
+    const float tp = fclamp((1.0f - dcent) / 0.5f, 0.0f, 1.0f);
+    *out = (tw * tw * (3.0f - 2.0f * tw)) * (tp * tp * (3.0f - 2.0f * tp));
+}
+
+// ---------------------------------------------------------------------------
+// plain chroma base (plainChromaSeparable, recon.cpp:1151-1461): horizontal
+// pass per source chroma row with 1D hulls, vertical pass per output pixel
+// with the separable normalisation, cancellation fallback and 2D hull clamp.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_plain_h(const float *__restrict__ u, const float *__restrict__ v,
+                                                    int cw, int ch, AxisDev ax, float *hb) {
+    const int ox = blockIdx.x * kBlock + threadIdx.x, sy = blockIdx.y;
+    if (ox >= ax.n)
+        return;
+    const int start = ax.start[ox];
+    const float *wt = ax.w + (size_t)ox * ax.sup;
+    const float *am = ax.am + (size_t)ox * ax.sup;
+    const float *su = u + (size_t)sy * cw, *sv = v + (size_t)sy * cw;
+    double vu = 0.0, vv = 0.0;
+    float mnu = 1e30f, mxu = -1e30f, mnv = 1e30f, mxv = -1e30f;
+    for (int i = 0; i < ax.sup; ++i) {
+        if (am[i] == 0.0f)
+            continue;
+        const int x = clampi(start + i, 0, cw - 1);
+        const float a = su[x], b = sv[x];
+        vu += (double)wt[i] * a;
+        vv += (double)wt[i] * b;
+        mnu = fmn(mnu, a); mxu = fmx(mxu, a);
+        mnv = fmn(mnv, b); mxv = fmx(mxv, b);
+    }
+    const size_t n = (size_t)ax.n * ch, k = (size_t)sy * ax.n + ox;
+    hb[k] = (float)vu;
+    hb[n + k] = (float)vv;
+    hb[2 * n + k] = mnu;
+    hb[3 * n + k] = mxu;
+    hb[4 * n + k] = mnv;
+    hb[5 * n + k] = mxv;
+}
+
+__global__ __launch_bounds__(kBlock) void k_plain_v(const float *__restrict__ u, const float *__restrict__ v,
+                                                    int cw, int ch, AxisDev ax, AxisDev ay,
+                                                    const float *__restrict__ hb, float *out_u,
+                                                    float *out_v, int clamp_hull, float margin) {
+    const int ox = blockIdx.x * kBlock + threadIdx.x, oy = blockIdx.y;
+    if (ox >= ax.n)
+        return;
+    const int ow = ax.n;
+    const size_t n = (size_t)ow * ch;
+    const int start = ay.start[oy];
+    const float *wt = ay.w + (size_t)oy * ay.sup;
+    const float *am = ay.am + (size_t)oy * ay.sup;
+    double vu = 0.0, vv = 0.0;
+    float mnu = 1e30f, mxu = -1e30f, mnv = 1e30f, mxv = -1e30f;
+    for (int j = 0; j < ay.sup; ++j) {
+        if (am[j] == 0.0f)
+            continue;
+        const size_t k = (size_t)clampi(start + j, 0, ch - 1) * ow + ox;
+        vu += (double)wt[j] * hb[k];
+        vv += (double)wt[j] * hb[n + k];
+        mnu = fmn(mnu, hb[2 * n + k]); mxu = fmx(mxu, hb[3 * n + k]);
+        mnv = fmn(mnv, hb[4 * n + k]); mxv = fmx(mxv, hb[5 * n + k]);
+    }
+    const double wsum = (double)ax.wsum[ox] * ay.wsum[oy];
+    const double asum = (double)ax.wabs[ox] * ay.wabs[oy];
+    if (wsum > 1e-6 && wsum >= 0.05 * asum) {
+        vu /= wsum;
+        vv /= wsum;
+    } else {
+        vu = bilinear(u, cw, ch, ax.pos[ox], ay.pos[oy]);
+        vv = bilinear(v, cw, ch, ax.pos[ox], ay.pos[oy]);
+    }
+    if (clamp_hull) {
+        vu = dclamp(vu, (double)(mnu - margin), (double)(mxu + margin));
+        vv = dclamp(vv, (double)(mnv - margin), (double)(mxv + margin));
+    }
+    const size_t o = (size_t)oy * ow + ox;
+    out_u[o] = (float)vu;
+    out_v[o] = (float)vv;
+}
+
+// plainChromaBilinear (recon.cpp:1031-1149)
+__global__ __launch_bounds__(kBlock) void k_plain_bilinear(const float *u, const float *v, int cw, int ch,
+                                                           AxisDev ax, AxisDev ay, float *out_u,
+                                                           float *out_v) {
+    const int ox = blockIdx.x * kBlock + threadIdx.x, oy = blockIdx.y;
+    if (ox >= ax.n)
+        return;
+    const size_t o = (size_t)oy * ax.n + ox;
+    out_u[o] = bilinear_fast(u, cw, ch, ax.cb_i0[ox], ax.cb_f[ox], ay.cb_i0[oy], ay.cb_f[oy]);
+    out_v[o] = bilinear_fast(v, cw, ch, ax.cb_i0[ox], ax.cb_f[ox], ay.cb_i0[oy], ay.cb_f[oy]);
+}
+
+// ---------------------------------------------------------------------------
+// guided reconstruction, algo 2 (reconstructChromaImpl, recon.cpp:14-819):
+// one thread per active output pixel of the compacted workset.
+// ---------------------------------------------------------------------------
+
+struct GuidedArgs {
+    const float *u, *v;
+    int cw, ch;
+    const float *y;
+    int sw, sh;
+    const float *jxx, *jxy, *jyy, *lc, *ms;
+    AxisDev ax, ay;
+    GuideParams gp;
+    const uint32_t *row_off;
+    const uint32_t *indices;
+    float *out_u, *out_v;
+};
+
+__device__ void guided_pixel(const GuidedArgs &A, int ox, int oy) {
+    const AxisDev &ax = A.ax, &ay = A.ay;
+    const GuideParams &gp = A.gp;
+    const int cw = A.cw, ch = A.ch, sw = A.sw, sh = A.sh;
+    const bool guided = gp.strength > 0.0f;
+
+    // --- guide metadata (recon.cpp:155-218)
+    float L0 = 0.0f, nxv = 0.0f, nyv = 0.0f, coherence = 0.0f, min_diff = 0.0f, ridge = 0.0f;
+    float phase = 0.0f, mfade = 1.0f;
+    bool has_dir = false;
+    const float scx = ax.pos[ox], scy = ay.pos[oy];
+    if (guided) {
+        const int bxi = ax.lb_i0[ox], byi = ay.lb_i0[oy];
+        const float bxf = ax.lb_f[ox], byf = ay.lb_f[oy];
+        L0 = bilinear_fast(A.y, sw, sh, bxi, bxf, byi, byf);
+        const float jxx = bilinear_fast(A.jxx, sw, sh, bxi, bxf, byi, byf);
+        const float jxy = bilinear_fast(A.jxy, sw, sh, bxi, bxf, byi, byf);
+        const float jyy = bilinear_fast(A.jyy, sw, sh, bxi, bxf, byi, byf);
+        const Dir d = principal_dir(jxx, jxy, jyy);
+        coherence = d.coherence;
+        nxv = d.nx;
+        nyv = d.ny;
+        has_dir = jxx + jyy > 1e-6f;
+        const float lx = ax.lpos[ox], ly = ay.lpos[oy];
+        if (has_dir) {
+            const float sp = gp.sp_out;
+            const float ya = bilinear(A.y, sw, sh, lx - sp * nxv, ly - sp * nyv);
+            const float yb = bilinear(A.y, sw, sh, lx + sp * nxv, ly + sp * nyv);
+            min_diff = fmn(fabsf(ya - L0), fabsf(yb - L0));
+            if (gp.ridge) {
+                const float dl = 1.5f * sp;
+                const float dl2 = 2.0f * dl;
+                const float ym2 = bilinear(A.y, sw, sh, lx - dl2 * nxv, ly - dl2 * nyv);
+                const float ym1 = bilinear(A.y, sw, sh, lx - dl * nxv, ly - dl * nyv);
+                const float yp1 = bilinear(A.y, sw, sh, lx + dl * nxv, ly + dl * nyv);
+                const float yp2 = bilinear(A.y, sw, sh, lx + dl2 * nxv, ly + dl2 * nyv);
+                const float df[4] = {ym1 - ym2, L0 - ym1, yp1 - L0, yp2 - yp1};
+                for (int q = 0; q < 3; ++q) {
+                    const float a = df[q], b = df[q + 1];
+                    if (a > 0.0f && b < 0.0f)
+                        ridge = fmx(ridge, fmn(a, -b));
+                    if (a < 0.0f && b > 0.0f)
+                        ridge = fmx(ridge, fmn(-a, b));
+                }
+            }
+        }
+        const float fx = scx - floorf(scx), fy = scy - floorf(scy);
+        const float phx = fmn(fx, 1.0f - fx), phy = fmn(fy, 1.0f - fy);
+        phase = (1.0f - 2.0f * phx) * (1.0f - 2.0f * phy);
+        if (gp.ms_on) {
+            const float gate = bilinear_fast(A.ms, cw, ch, ax.cb_i0[ox], ax.cb_f[ox], ay.cb_i0[oy], ay.cb_f[oy]);
+            mfade = 1.0f - gp.ms * (1.0f - gate);
+        }
+    }
+
+    // --- pass 1: window luma range + chroma hull (recon.cpp:304-363)
+    const int tx0 = ax.start[ox], ty0 = ay.start[oy];
+    const float *amx = ax.am + (size_t)ox * ax.sup, *amy = ay.am + (size_t)oy * ay.sup;
+    const float *wxp = ax.w + (size_t)ox * ax.sup, *wyp = ay.w + (size_t)oy * ay.sup;
+    float max_dl = 0.0f;
+    float mnu = 1e30f, mxu = -1e30f, mnv = 1e30f, mxv = -1e30f;
+    for (int j = 0; j < ay.sup; ++j) {
+        if (amy[j] == 0.0f)
+            continue;
+        const size_t row = (size_t)clampi(ty0 + j, 0, ch - 1) * cw;
+        for (int i = 0; i < ax.sup; ++i) {
+            if (amx[i] == 0.0f)
+                continue;
+            const size_t k = row + clampi(tx0 + i, 0, cw - 1);
+            if (guided)
+                max_dl = fmx(max_dl, fabsf(A.lc[k] - L0));
+            const float a = A.u[k], b = A.v[k];
+            mnu = fmn(mnu, a); mxu = fmx(mxu, a);
+            mnv = fmn(mnv, b); mxv = fmx(mxv, b);
+        }
+    }
+
+    // --- adaptive sigma and fades (recon.cpp:369-417)
+    const float t2 = fclamp(min_diff / (0.1f * max_dl + 1e-6f), 0.0f, 1.0f);
+    const float ss_ramp = t2 * t2 * (3.0f - 2.0f * t2);
+    const float sig_mult = gp.sratio + (gp.sdb - gp.sratio) * ss_ramp;
+    const float sig_loc = fmx(gp.sigma, sig_mult * max_dl);
+    const float inv_s2 = 1.0f / (sig_loc * sig_loc);
+    const float ridge_fade = 1.0f - fclamp(2.0f * ridge / (max_dl + 1e-6f), 0.0f, 1.0f);
+    float cedge_fade = 1.0f;
+    if (gp.cedge && guided && has_dir) {
+        const float rc = fmx(mxu - mnu, mxv - mnv);
+        if (rc > 1e-6f) {
+            float gmax = 0.0f;
+            const float anx = fabsf(nxv), any = fabsf(nyv);
+            for (int j = 0; j < ay.sup; ++j) {
+                const int ty = clampi(ty0 + j, 0, ch - 1), ty1 = min(ty + 1, ch - 1);
+                for (int i = 0; i < ax.sup; ++i) {
+                    const int tx = clampi(tx0 + i, 0, cw - 1), tx1 = min(tx + 1, cw - 1);
+                    const float c0u = A.u[(size_t)ty * cw + tx], c0v = A.v[(size_t)ty * cw + tx];
+                    const float gu = fmx(fabsf(A.u[(size_t)ty * cw + tx1] - c0u) * anx,
+                                         fabsf(A.u[(size_t)ty1 * cw + tx] - c0u) * any);
+                    const float gv = fmx(fabsf(A.v[(size_t)ty * cw + tx1] - c0v) * anx,
+                                         fabsf(A.v[(size_t)ty1 * cw + tx] - c0v) * any);
+                    gmax = fmx(gmax, fmx(gu, gv));
+                }
+            }
+            const float wc = rc / (gmax + 1e-6f);
+            cedge_fade = fclamp((2.2f - wc) / 0.7f, 0.0f, 1.0f);
+        }
+    }
+    const float guide_fade = ridge_fade * cedge_fade * mfade * gp.strength;
+
+    // --- phase-0 rescue with kappa^2 (recon.cpp:425-475)
+    float rescue = 0.0f;
+    if (guided) {
+        const int ncx = clampi((int)lroundf(scx), 0, cw - 1);
+        const int ncy = clampi((int)lroundf(scy), 0, ch - 1);
+        const float dl0 = A.lc[(size_t)ncy * cw + ncx] - L0;
+        const float sim0 = 1.0f / (1.0f + dl0 * dl0 * inv_s2);
+        rescue = gp.rescue * guide_fade * phase * phase * (1.0f - sim0);
+        if (rescue > 0.0f) {
+            float mxX = 0, ssX = 0, mxY = 0, ssY = 0;
+            for (int i = 0; i < ax.sup; ++i)
+                if (amx[i] > 0.0f) {
+                    const float w = wxp[i];
+                    mxX = fmx(mxX, fabsf(w));
+                    ssX += w * w;
+                }
+            for (int j = 0; j < ay.sup; ++j)
+                if (amy[j] > 0.0f) {
+                    const float w = wyp[j];
+                    mxY = fmx(mxY, fabsf(w));
+                    ssY += w * w;
+                }
+            const float kappa = (mxX * mxX * mxY * mxY) / (ssX * ssY + 1e-12f);
+            rescue *= kappa * kappa;
+        }
+    }
+    const float aniso = 1.0f + gp.stretch * coherence;
+    const float inv_g2 = gp.inv_gsigma2;
+    const float inv_g2p = inv_g2 / (aniso * aniso);
+
+    // --- tap loop (recon.cpp:581-627)
+    double acc_u = 0.0, acc_v = 0.0, wsum = 0.0, sum_abs = 0.0;
+    for (int j = 0; j < ay.sup; ++j) {
+        if (amy[j] == 0.0f)
+            continue;
+        const size_t row = (size_t)clampi(ty0 + j, 0, ch - 1) * cw;
+        const float wy = wyp[j];
+        const float dy = ((float)(ty0 + j) - scy) * gp.frh;
+        for (int i = 0; i < ax.sup; ++i) {
+            if (amx[i] == 0.0f)
+                continue;
+            const size_t k = row + clampi(tx0 + i, 0, cw - 1);
+            const float wx = wxp[i] * wy;
+            float w;
+            if (guided) {
+                const float dx = ((float)(tx0 + i) - scx) * gp.frw;
+                const float dL = A.lc[k] - L0;
+                const float dp = dx * nxv + dy * nyv;
+                const float dt = -dx * nyv + dy * nxv;
+                const float sim_raw = 1.0f / (1.0f + dL * dL * inv_s2);
+                const float sim = 1.0f - guide_fade * (1.0f - sim_raw);
+                const float bump = 1.0f / (1.0f + dp * dp * inv_g2 + dt * dt * inv_g2p);
+                w = sim * (wx + rescue * bump);
+            } else {
+                w = wx;
+            }
+            acc_u += (double)w * A.u[k];
+            acc_v += (double)w * A.v[k];
+            wsum += w;
+            sum_abs += fabsf(w);
+        }
+    }
+    double vu, vv;
+    if (wsum > 1e-6 && wsum >= 0.05 * sum_abs) {
+        vu = acc_u / wsum;
+        vv = acc_v / wsum;
+    } else {
+        vu = bilinear(A.u, cw, ch, scx, scy);
+        vv = bilinear(A.v, cw, ch, scx, scy);
+    }
+    if (gp.clamp_hull) {
+        vu = dclamp(vu, (double)(mnu - gp.margin), (double)(mxu + gp.margin));
+        vv = dclamp(vv, (double)(mnv - gp.margin), (double)(mxv + gp.margin));
+    }
+    const size_t o = (size_t)oy * ax.n + ox;
+    A.out_u[o] = (float)vu;
+    A.out_v[o] = (float)vv;
+}
+
+__global__ __launch_bounds__(kBlock) void k_guided(GuidedArgs A) {
+    const int ow = A.ax.n, oh = A.ay.n;
+    const uint32_t total = A.indices ? A.row_off[oh] : (uint32_t)ow * (uint32_t)oh;
+    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < total; k += gridDim.x * kBlock) {
+        const uint32_t idx = A.indices ? A.indices[k] : k;
+        guided_pixel(A, (int)(idx % (uint32_t)ow), (int)(idx / (uint32_t)ow));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// back-projection (backProject, algos.cpp:663-677); the reference applies it
+// only when the output is exactly 2x the chroma grid, so the 2x2 blocks are
+// disjoint and every chroma sample can be processed in parallel.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_back_project(float *ou, float *ov, int ow, int oh, const float *u,
+                                                         const float *v, int cw, int ch, float bp) {
+    const int cx = blockIdx.x * kBlock + threadIdx.x, cy = blockIdx.y;
+    if (cx >= cw)
+        return;
+    const int xa = min(2 * cx, ow - 1), xb = min(2 * cx + 1, ow - 1);
+    const int ya = min(2 * cy, oh - 1), yb = min(2 * cy + 1, oh - 1);
+    for (int c = 0; c < 2; ++c) {
+        float *o = c ? ov : ou;
+        const float *s = c ? v : u;
+        const float p00 = o[(size_t)ya * ow + xa], p10 = o[(size_t)ya * ow + xb];
+        const float p01 = o[(size_t)yb * ow + xa], p11 = o[(size_t)yb * ow + xb];
+        const float r = s[(size_t)cy * cw + cx] - 0.25f * (p00 + p10 + p01 + p11);
+        const float d = bp * r;
+        o[(size_t)ya * ow + xa] = p00 + d;
+        o[(size_t)ya * ow + xb] = p10 + d;
+        o[(size_t)yb * ow + xa] = p01 + d;
+        o[(size_t)yb * ow + xb] = p11 + d;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// luma resampling for resize (resampleH/V, kernels.cpp:171-271; radial jinc,
+// kernels.cpp:276-334)
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_resample_h(const float *src, int sw, int sh, float *dst, WeightsDev t) {
+    const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
+    if (x >= t.n)
+        return;
+    const float *wp = t.w + (size_t)x * t.sup;
+    const int s = t.start[x];
+    double acc = 0.0;
+    for (int j = 0; j < t.sup; ++j)
+        acc += (double)wp[j] * src[(size_t)y * sw + clampi(s + j, 0, sw - 1)];
+    dst[(size_t)y * t.n + x] = (float)acc;
+}
+
+__global__ __launch_bounds__(kBlock) void k_resample_v(const float *src, int sw, int sh, float *dst, WeightsDev t) {
+    const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
+    if (x >= sw)
+        return;
+    const float *wp = t.w + (size_t)y * t.sup;
+    const int s = t.start[y];
+    double acc = 0.0;
+    for (int j = 0; j < t.sup; ++j)
+        acc += (double)wp[j] * src[(size_t)clampi(s + j, 0, sh - 1) * sw + x];
+    dst[(size_t)y * sw + x] = (float)acc;
+}
+
+__device__ __forceinline__ float lut_lookup(const float *lut, int n, float idx) {
+    int i0 = (int)idx;
+    i0 = clampi(i0, 0, n - 2);
+    const float fr = fmn(idx - (float)i0, 1.0f);
+    return lut[i0] + (lut[i0 + 1] - lut[i0]) * fr;
+}
+
+__global__ __launch_bounds__(kBlock) void k_resample_radial(const float *src, int sw, int sh, float *dst, int dw,
+                                                            int dh, double sx, double sy, float inv_w,
+                                                            int radius, const float *lut, int lut_n,
+                                                            float lut_scale) {
+    const int ox = blockIdx.x * kBlock + threadIdx.x, oy = blockIdx.y;
+    if (ox >= dw)
+        return;
+    const int sup = 2 * radius;
+    const double cy = (oy + 0.5) * sy - 0.5, cx = (ox + 0.5) * sx - 0.5;
+    const int y0 = (int)ceil(cy - radius), x0 = (int)ceil(cx - radius);
+    double acc = 0.0, wsum = 0.0;
+    for (int j = 0; j < sup; ++j) {
+        const int ty = clampi(y0 + j, 0, sh - 1);
+        const float dy = (float)((y0 + j - cy) * inv_w);
+        for (int i = 0; i < sup; ++i) {
+            const int tx = clampi(x0 + i, 0, sw - 1);
+            const float dx = (float)((x0 + i - cx) * inv_w);
+            const float w = lut_lookup(lut, lut_n, sqrtf(dx * dx + dy * dy) * lut_scale);
+            acc += (double)w * src[(size_t)ty * sw + tx];
+            wsum += w;
+        }
+    }
+    dst[(size_t)oy * dw + ox] = (float)(wsum > 1e-6 ? acc / wsum : 0.0);
+}
+
+inline dim3 rows_grid(int w, int h, int z = 1) { return dim3((unsigned)((w + kBlock - 1) / kBlock), (unsigned)h, (unsigned)z); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+
+hipError_t launch_to_float(const RawPlane *planes, int count, int type, hipStream_t s) {
+    InBatch b{};
+    int mw = 0, mh = 0;
+    for (int i = 0; i < count; ++i) {
+        b.p[i] = planes[i];
+        mw = max(mw, planes[i].w);
+        mh = max(mh, planes[i].h);
+    }
+    const dim3 g = rows_grid(mw, mh, count);
+    if (type == RAW_U8)
+        hipLaunchKernelGGL(k_to_float<uint8_t>, g, dim3(kBlock), 0, s, b);
+    else if (type == RAW_U16)
+        hipLaunchKernelGGL(k_to_float<uint16_t>, g, dim3(kBlock), 0, s, b);
+    else
+        hipLaunchKernelGGL(k_copy_f32, g, dim3(kBlock), 0, s, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_from_float(const RawOut *planes, int count, int type, hipStream_t s) {
+    OutBatch b{};
+    int mw = 0, mh = 0;
+    for (int i = 0; i < count; ++i) {
+        b.p[i] = planes[i];
+        mw = max(mw, planes[i].w);
+        mh = max(mh, planes[i].h);
+    }
+    const dim3 g = rows_grid(mw, mh, count);
+    if (type == RAW_U8)
+        hipLaunchKernelGGL(k_from_float<uint8_t>, g, dim3(kBlock), 0, s, b);
+    else if (type == RAW_U16)
+        hipLaunchKernelGGL(k_from_float<uint16_t>, g, dim3(kBlock), 0, s, b);
+    else
+        hipLaunchKernelGGL(k_from_float_f32, g, dim3(kBlock), 0, s, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_sobel_tensor(const float *y, int w, int h, float *jxx, float *jxy, float *jyy,
+                               uint8_t *seed, float threshold, hipStream_t s) {
+    const dim3 g((unsigned)((w + TW - 1) / TW), (unsigned)((h + TH - 1) / TH));
+    hipLaunchKernelGGL(k_sobel_tensor, g, dim3(TW, TH), 0, s, y, w, h, jxx, jxy, jyy, seed, threshold);
+    return hipGetLastError();
+}
+
+hipError_t launch_lc_map(const float *y, int w, int h, float *lc, int cw, int ch, const int *xb,
+                         const int *xe, const int *yb, const int *ye, hipStream_t s) {
+    (void)h;
+    hipLaunchKernelGGL(k_lc_map, rows_grid(cw, ch), dim3(kBlock), 0, s, y, w, lc, cw, ch, xb, xe, yb, ye);
+    return hipGetLastError();
+}
+
+hipError_t launch_dilate(const uint8_t *src, uint8_t *tmp, uint8_t *dst, int w, int h, int radius,
+                         hipStream_t s) {
+    const size_t lds = (size_t)w + 2 * (size_t)radius;
+    if (lds > 60 * 1024)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_dilate_h, dim3((unsigned)h), dim3(kBlock), lds, s, src, tmp, w, radius);
+    hipLaunchKernelGGL(k_dilate_v, dim3((unsigned)((w + kBlock - 1) / kBlock), (unsigned)((h + kDilRows - 1) / kDilRows)),
+                       dim3(kBlock), 0, s, tmp, dst, w, h, radius);
+    return hipGetLastError();
+}
+
+hipError_t launch_workset(const uint8_t *mask, int w, int h, uint32_t *row_offsets, uint32_t *indices,
+                          uint8_t *chroma_mask, int cw, int ch, const AxisDev &ax, const AxisDev &ay,
+                          uint32_t *counters, hipStream_t s) {
+    const MaskMap m{mask, w, h, ax.n, ay.n};
+    hipLaunchKernelGGL(k_row_count, dim3((unsigned)ay.n), dim3(kBlock), 0, s, m, row_offsets);
+    hipLaunchKernelGGL(k_scan_rows, dim3(1), dim3(1024), 0, s, row_offsets, ay.n);
+    hipLaunchKernelGGL(k_compact, dim3((unsigned)ay.n), dim3(kBlock), 0, s, m, row_offsets, indices);
+    const size_t nc = (size_t)cw * ch;
+    hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)std::min<size_t>(1024, (nc + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                       chroma_mask, nc, (uint8_t)0);
+    hipLaunchKernelGGL(k_chroma_mark, rows_grid(ax.n, ay.n), dim3(kBlock), 0, s, m, ax.cb_i0, ay.cb_i0,
+                       chroma_mask, cw, ch);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, counters, 4);
+    hipLaunchKernelGGL(k_count_u8, dim3((unsigned)std::min<size_t>(512, (nc + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                       chroma_mask, nc, counters);
+    return hipGetLastError();
+}
+
+hipError_t launch_mutual_gate(const float *lc, const float *u, const float *v, int w, int h, float *grad,
+                              const uint8_t *chroma_mask, const uint32_t *counters, float *gate,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(k_grad3, rows_grid(w, h), dim3(kBlock), 0, s, lc, u, v, w, h, grad);
+    hipLaunchKernelGGL(k_gate, rows_grid(w, h), dim3(kBlock), 0, s, grad, w, h, chroma_mask, counters, gate);
+    return hipGetLastError();
+}
+
+hipError_t launch_plain_separable(const float *u, const float *v, int cw, int ch, const AxisDev &ax,
+                                  const AxisDev &ay, float *hbuf, float *out_u, float *out_v, int clamp_hull,
+                                  float margin, hipStream_t s) {
+    hipLaunchKernelGGL(k_plain_h, rows_grid(ax.n, ch), dim3(kBlock), 0, s, u, v, cw, ch, ax, hbuf);
+    hipLaunchKernelGGL(k_plain_v, rows_grid(ax.n, ay.n), dim3(kBlock), 0, s, u, v, cw, ch, ax, ay, hbuf, out_u,
+                       out_v, clamp_hull, margin);
+    return hipGetLastError();
+}
+
+hipError_t launch_plain_bilinear(const float *u, const float *v, int cw, int ch, const AxisDev &ax,
+                                 const AxisDev &ay, float *out_u, float *out_v, hipStream_t s) {
+    hipLaunchKernelGGL(k_plain_bilinear, rows_grid(ax.n, ay.n), dim3(kBlock), 0, s, u, v, cw, ch, ax, ay, out_u,
+                       out_v);
+    return hipGetLastError();
+}
+
+hipError_t launch_guided(const float *u, const float *v, int cw, int ch, const float *y, int sw, int sh,
+                         const float *jxx, const float *jxy, const float *jyy, const float *lc, const float *ms,
+                         const AxisDev &ax, const AxisDev &ay, const GuideParams &gp, const uint32_t *row_offsets,
+                         const uint32_t *indices, float *out_u, float *out_v, hipStream_t s) {
+    GuidedArgs A{u, v, cw, ch, y, sw, sh, jxx, jxy, jyy, lc, ms, ax, ay, gp, row_offsets, indices, out_u, out_v};
+    const size_t n = (size_t)ax.n * ay.n;
+    const unsigned blocks = (unsigned)std::min<size_t>(8192, (n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_guided, dim3(blocks), dim3(kBlock), 0, s, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_back_project(float *out_u, float *out_v, int ow, int oh, const float *u, const float *v,
+                               int cw, int ch, float bp, hipStream_t s) {
+    hipLaunchKernelGGL(k_back_project, rows_grid(cw, ch), dim3(kBlock), 0, s, out_u, out_v, ow, oh, u, v, cw,
+                       ch, bp);
+    return hipGetLastError();
+}
+
+hipError_t launch_resample_h(const float *src, int sw, int sh, float *dst, const WeightsDev &t, hipStream_t s) {
+    hipLaunchKernelGGL(k_resample_h, rows_grid(t.n, sh), dim3(kBlock), 0, s, src, sw, sh, dst, t);
+    return hipGetLastError();
+}
+
+hipError_t launch_resample_v(const float *src, int sw, int sh, float *dst, const WeightsDev &t, hipStream_t s) {
+    hipLaunchKernelGGL(k_resample_v, rows_grid(sw, t.n), dim3(kBlock), 0, s, src, sw, sh, dst, t);
+    return hipGetLastError();
+}
+
+hipError_t launch_resample_radial(const float *src, int sw, int sh, float *dst, int dw, int dh, double support,
+                                  const float *lut, int lut_n, float lut_scale, hipStream_t s) {
+    const double sx = (double)sw / dw, sy = (double)sh / dh;
+    const double widen = fmax(1.0, fmax(sx, sy));
+    const float inv_w = (float)(1.0 / widen);
+    const int radius = (int)ceil(support * widen);
+    hipLaunchKernelGGL(k_resample_radial, rows_grid(dw, dh), dim3(kBlock), 0, s, src, sw, sh, dst, dw, dh, sx, sy,
+                       inv_w, radius, lut, lut_n, lut_scale);
+    return hipGetLastError();
+}
+
+}  // namespace lgcr_hip

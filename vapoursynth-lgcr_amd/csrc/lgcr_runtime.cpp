@@ -1,0 +1,1121 @@
+// Host runtime of the LGCR HIP backend: device contexts, clips with resident
+// geometry tables, per-call frame workspaces (one HIP stream each), and the
+// per-frame orchestration of lgcrGetFrame (src/plugin.cpp:177-721).
+#include "lgcr_hip.h"
+#include "lgcr_internal.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <condition_variable>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+using namespace lgcr_hip;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+lgcr_status fail(lgcr_status st, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return st;
+}
+
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        const hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                               \
+            return fail(LGCR_BACKEND_FAILURE, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// host geometry: kernel closed forms (src/lgcr.h:53-117), polyphase weights
+// (src/kernels.cpp:135-167), chroma axes (src/kernels.cpp:337-404) and
+// bilinear axes (src/maps.cpp:190-209)
+// ---------------------------------------------------------------------------
+
+double sinc(double x) {
+    if (std::fabs(x) < 1e-8)
+        return 1.0;
+    x *= M_PI;
+    return std::sin(x) / x;
+}
+
+double kernel_eval(int k, double x, double p1, double p2) {
+    x = std::fabs(x);
+    switch (k) {
+    case LGCR_KERNEL_BILINEAR:
+        return x < 1.0 ? 1.0 - x : 0.0;
+    case LGCR_KERNEL_BICUBIC: {
+        const double b = p1, c = p2;
+        if (x < 1.0)
+            return ((12 - 9 * b - 6 * c) * x * x * x + (-18 + 12 * b + 6 * c) * x * x + (6 - 2 * b)) / 6.0;
+        if (x < 2.0)
+            return ((-b - 6 * c) * x * x * x + (6 * b + 30 * c) * x * x + (-12 * b - 48 * c) * x + (8 * b + 24 * c)) / 6.0;
+        return 0.0;
+    }
+    case LGCR_KERNEL_LANCZOS:
+        return x < p1 ? sinc(x) * sinc(x / p1) : 0.0;
+    case LGCR_KERNEL_SPLINE16:
+        if (x < 1.0)
+            return ((x - 9.0 / 5.0) * x - 1.0 / 5.0) * x + 1.0;
+        if (x < 2.0)
+            return ((-1.0 / 3.0 * (x - 1.0) + 4.0 / 5.0) * (x - 1.0) - 7.0 / 15.0) * (x - 1.0);
+        return 0.0;
+    case LGCR_KERNEL_SPLINE36:
+        if (x < 1.0)
+            return ((13.0 / 11.0 * x - 453.0 / 209.0) * x - 3.0 / 209.0) * x + 1.0;
+        if (x < 2.0)
+            return ((-6.0 / 11.0 * (x - 1.0) + 270.0 / 209.0) * (x - 1.0) - 156.0 / 209.0) * (x - 1.0);
+        if (x < 3.0)
+            return ((1.0 / 11.0 * (x - 2.0) - 45.0 / 209.0) * (x - 2.0) + 26.0 / 209.0) * (x - 2.0);
+        return 0.0;
+    default:
+        return 0.0;
+    }
+}
+
+double jinc_windowed(double x, double taps) {
+    x = std::fabs(x);
+    if (x >= taps)
+        return 0.0;
+    const double j = (x < 1e-8) ? 1.0 : 2.0 * std::cyl_bessel_j(1.0, M_PI * x) / (M_PI * x);
+    return j * sinc(x / taps);
+}
+
+struct HostWeights {
+    int n = 0, sup = 0;
+    std::vector<int> start;
+    std::vector<float> w;
+};
+
+HostWeights build_weights(int src_n, int dst_n, int k, double kp1, double kp2, double support, double shift) {
+    HostWeights t;
+    t.n = dst_n;
+    const double scale = double(src_n) / dst_n;
+    const double sup = scale > 1.0 ? support * scale : support;
+    const double inv_scale = scale > 1.0 ? 1.0 / scale : 1.0;
+    t.sup = std::max(1, 2 * int(std::ceil(sup)));
+    t.start.resize(dst_n);
+    t.w.assign(size_t(dst_n) * t.sup, 0.0f);
+    for (int i = 0; i < dst_n; ++i) {
+        const double center = (i + 0.5) * scale - 0.5 + shift;
+        int first = int(std::ceil(center - sup));
+        first = std::clamp(first, -t.sup + 1, src_n - 1);
+        if (first + t.sup > src_n)
+            first = std::max(-t.sup + 1, src_n - t.sup);
+        t.start[i] = first;
+        double sum = 0.0;
+        for (int j = 0; j < t.sup; ++j) {
+            const double w = kernel_eval(k, ((first + j) - center) * inv_scale, kp1, kp2);
+            t.w[size_t(i) * t.sup + j] = float(w);
+            sum += w;
+        }
+        if (std::fabs(sum) > 1e-9)
+            for (int j = 0; j < t.sup; ++j)
+                t.w[size_t(i) * t.sup + j] /= float(sum);
+    }
+    return t;
+}
+
+struct HostAxis {
+    int n = 0, sup = 0;
+    std::vector<int> start, cb_i0, lb_i0;
+    std::vector<float> w, am, wsum, wabs, pos, lpos, cb_f, lb_f;
+};
+
+void bilin_axis(const std::vector<float> &pos, int src_n, std::vector<int> &i0, std::vector<float> &f) {
+    const size_t n = pos.size();
+    i0.assign(n, 0);
+    f.assign(n, 0.0f);
+    if (src_n == 1)
+        return;
+    for (size_t i = 0; i < n; ++i) {
+        const float x = pos[i];
+        int j = int(std::floor(x));
+        float fr = x - float(j);
+        if (j < 0) { j = 0; fr = 0.0f; }
+        if (j > src_n - 2) { j = src_n - 2; fr = 1.0f; }
+        i0[i] = j;
+        f[i] = fr;
+    }
+}
+
+HostAxis build_axis(int src_luma_n, int dst_luma_n, double r, double sit_shift, const lgcr_params &p) {
+    HostAxis a;
+    a.n = dst_luma_n;
+    const double scale_c = (double(src_luma_n) / r) / dst_luma_n;
+    const double widen = p.radial ? std::max(1.0, scale_c) : 1.0;
+    a.sup = std::max(1, 2 * int(std::ceil(p.support * widen)));
+    const size_t n = size_t(a.n), ns = n * a.sup;
+    a.start.resize(n); a.pos.resize(n); a.lpos.resize(n);
+    a.w.assign(ns, 0.0f); a.am.assign(ns, 0.0f);
+    a.wsum.assign(n, 0.0f); a.wabs.assign(n, 0.0f);
+    const int src_chroma_n = (src_luma_n + int(r) - 1) / int(r);
+    const double scale = double(src_luma_n) / dst_luma_n;
+    for (int i = 0; i < a.n; ++i) {
+        const double lx = (i + 0.5) * scale - 0.5;
+        const double sc = (lx + 0.5 - sit_shift) / r - 0.5;
+        a.pos[i] = float(sc);
+        a.lpos[i] = float(lx);
+        int first = int(std::ceil(sc - p.support * widen));
+        first = std::clamp(first, 0, std::max(0, src_chroma_n - a.sup));
+        a.start[i] = first;
+        float *wi = &a.w[size_t(i) * a.sup];
+        float *mi = &a.am[size_t(i) * a.sup];
+        for (int j = 0; j < a.sup; ++j)
+            if (std::fabs((first + j) - sc) < p.support * widen)
+                mi[j] = 1.0f;
+        if (p.radial)
+            continue;
+        double sum = 0.0;
+        for (int j = 0; j < a.sup; ++j) {
+            const double w = kernel_eval(p.kernel, (first + j) - sc, p.kp1, p.kp2);
+            wi[j] = float(w);
+            sum += w;
+        }
+        if (std::fabs(sum) > 1e-9)
+            for (int j = 0; j < a.sup; ++j)
+                wi[j] /= float(sum);
+        for (int j = 0; j < a.sup; ++j)
+            if (mi[j] != 0.0f) {
+                a.wsum[i] += wi[j];
+                a.wabs[i] += std::fabs(wi[j]);
+            }
+    }
+    bilin_axis(a.pos, src_chroma_n, a.cb_i0, a.cb_f);
+    bilin_axis(a.lpos, src_luma_n, a.lb_i0, a.lb_f);
+    return a;
+}
+
+// Luma footprint of each chroma sample (src/maps.cpp:429-447).
+void footprint(int cn, double r, double shift, int luma_n, std::vector<int> &b, std::vector<int> &e) {
+    b.resize(cn);
+    e.resize(cn);
+    for (int c = 0; c < cn; ++c) {
+        const double l = (c + 0.5) * r - 0.5 + shift;
+        int first = int(std::ceil(l - 0.5 * r));
+        int last = int(std::ceil(l + 0.5 * r));
+        first = std::clamp(first, 0, luma_n - 1);
+        last = std::clamp(last, first + 1, luma_n);
+        b[c] = first;
+        e[c] = last;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// device blobs: pack host vectors into one allocation
+// ---------------------------------------------------------------------------
+
+struct Blob {
+    std::vector<uint8_t> host;
+    size_t add(const void *p, size_t bytes) {
+        const size_t off = (host.size() + 255) & ~size_t(255);
+        host.resize(off + bytes);
+        if (bytes)
+            std::memcpy(host.data() + off, p, bytes);
+        return off;
+    }
+    template <class T> size_t add(const std::vector<T> &v) { return add(v.data(), v.size() * sizeof(T)); }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// device / clip / workspace
+// ---------------------------------------------------------------------------
+
+struct lgcr_device {
+    int ordinal = 0;
+    uint64_t budget = 0;
+    std::atomic<uint64_t> reserved{0};
+
+    bool reserve(uint64_t bytes) {
+        uint64_t cur = reserved.load();
+        do {
+            if (cur + bytes > budget)
+                return false;
+        } while (!reserved.compare_exchange_weak(cur, cur + bytes));
+        return true;
+    }
+    void unreserve(uint64_t bytes) { reserved.fetch_sub(bytes); }
+};
+
+namespace {
+
+// Geometry for one chroma siting (the reference caches it per instance,
+// src/kernels.cpp:411-428).
+struct Geometry {
+    void *dev = nullptr;
+    AxisDev ax, ay;
+    const int *xb = nullptr, *xe = nullptr, *yb = nullptr, *ye = nullptr;
+};
+
+struct Workspace {
+    hipStream_t stream = nullptr;
+    void *base = nullptr;
+    size_t bytes = 0;
+    // raw staging for host frames
+    void *raw_in[3] = {nullptr, nullptr, nullptr};
+    void *raw_out[3] = {nullptr, nullptr, nullptr};
+    float *y = nullptr, *u = nullptr, *v = nullptr;
+    float *jxx = nullptr, *jxy = nullptr, *jyy = nullptr;
+    float *lc = nullptr, *ms = nullptr, *grad = nullptr, *hbuf = nullptr;
+    float *out_u = nullptr, *out_v = nullptr, *plain_u = nullptr, *plain_v = nullptr;
+    float *luma_tmp = nullptr, *luma_out = nullptr;
+    uint8_t *seed = nullptr, *mask = nullptr, *mtmp = nullptr, *cmask = nullptr;
+    uint32_t *row_off = nullptr, *indices = nullptr, *counters = nullptr;
+    // stage timeline of frames enqueued with timing on: (stage, event);
+    // stage -1 opens a frame.  Events come from a per-workspace pool.
+    std::vector<std::pair<int, hipEvent_t>> marks;
+    std::vector<hipEvent_t> pool;
+    bool has_ws = false;  // last frame built a sparse workset
+};
+
+}  // namespace
+
+struct lgcr_clip {
+    lgcr_device *dev = nullptr;
+    lgcr_params p{};
+    lgcr_format f{};
+    int sw = 0, sh = 0, cw = 0, ch = 0, ow = 0, oh = 0;
+    double rw = 1.0, rh = 1.0;
+    int raw_type = RAW_U8, bps = 1;
+    int dil = 0;
+    bool same = true;
+    bool capture = false;
+    size_t ws_bytes = 0;
+    int max_inflight = 4;
+    void *lut_dev = nullptr;
+    int lut_n = 0;
+    void *weights_dev = nullptr;
+    WeightsDev wh, wv;
+
+    std::mutex geo_mutex;
+    std::map<std::pair<uint64_t, uint64_t>, std::unique_ptr<Geometry>> geos;
+
+    std::atomic<bool> timing{false};
+    std::mutex timing_mutex;
+    double stage_us[LGCR_STAGE_COUNT] = {};
+    uint64_t stage_n[LGCR_STAGE_COUNT] = {};
+
+    std::mutex ws_mutex;
+    std::condition_variable ws_cv;
+    std::vector<std::unique_ptr<Workspace>> spaces;
+    std::vector<int> idle;
+};
+
+namespace {
+
+uint64_t bits_of(double v) {
+    uint64_t b;
+    std::memcpy(&b, &v, sizeof(b));
+    return b;
+}
+
+lgcr_status get_geometry(lgcr_clip *c, double sx, double sy, Geometry **out) {
+    std::lock_guard<std::mutex> lock(c->geo_mutex);
+    const auto key = std::make_pair(bits_of(sx), bits_of(sy));
+    auto it = c->geos.find(key);
+    if (it != c->geos.end()) {
+        *out = it->second.get();
+        return LGCR_OK;
+    }
+    const HostAxis ax = build_axis(c->sw, c->ow, c->rw, sx, c->p);
+    const HostAxis ay = build_axis(c->sh, c->oh, c->rh, sy, c->p);
+    std::vector<int> xb, xe, yb, ye;
+    footprint(c->cw, c->rw, sx, c->sw, xb, xe);
+    footprint(c->ch, c->rh, sy, c->sh, yb, ye);
+    Blob blob;
+    size_t o[2][11];
+    const HostAxis *axes[2] = {&ax, &ay};
+    for (int a = 0; a < 2; ++a) {
+        const HostAxis &h = *axes[a];
+        o[a][0] = blob.add(h.start); o[a][1] = blob.add(h.w); o[a][2] = blob.add(h.am);
+        o[a][3] = blob.add(h.wsum); o[a][4] = blob.add(h.wabs); o[a][5] = blob.add(h.pos);
+        o[a][6] = blob.add(h.lpos); o[a][7] = blob.add(h.cb_i0); o[a][8] = blob.add(h.cb_f);
+        o[a][9] = blob.add(h.lb_i0); o[a][10] = blob.add(h.lb_f);
+    }
+    const size_t oxb = blob.add(xb), oxe = blob.add(xe), oyb = blob.add(yb), oye = blob.add(ye);
+    auto g = std::make_unique<Geometry>();
+    HIP_TRY(hipSetDevice(c->dev->ordinal));
+    HIP_TRY(hipMalloc(&g->dev, blob.host.size()));
+    HIP_TRY(hipMemcpy(g->dev, blob.host.data(), blob.host.size(), hipMemcpyHostToDevice));
+    const uint8_t *base = static_cast<const uint8_t *>(g->dev);
+    AxisDev *dst[2] = {&g->ax, &g->ay};
+    for (int a = 0; a < 2; ++a) {
+        AxisDev &d = *dst[a];
+        d.n = axes[a]->n;
+        d.sup = axes[a]->sup;
+        d.start = reinterpret_cast<const int *>(base + o[a][0]);
+        d.w = reinterpret_cast<const float *>(base + o[a][1]);
+        d.am = reinterpret_cast<const float *>(base + o[a][2]);
+        d.wsum = reinterpret_cast<const float *>(base + o[a][3]);
+        d.wabs = reinterpret_cast<const float *>(base + o[a][4]);
+        d.pos = reinterpret_cast<const float *>(base + o[a][5]);
+        d.lpos = reinterpret_cast<const float *>(base + o[a][6]);
+        d.cb_i0 = reinterpret_cast<const int *>(base + o[a][7]);
+        d.cb_f = reinterpret_cast<const float *>(base + o[a][8]);
+        d.lb_i0 = reinterpret_cast<const int *>(base + o[a][9]);
+        d.lb_f = reinterpret_cast<const float *>(base + o[a][10]);
+    }
+    g->xb = reinterpret_cast<const int *>(base + oxb);
+    g->xe = reinterpret_cast<const int *>(base + oxe);
+    g->yb = reinterpret_cast<const int *>(base + oyb);
+    g->ye = reinterpret_cast<const int *>(base + oye);
+    *out = g.get();
+    c->geos.emplace(key, std::move(g));
+    return LGCR_OK;
+}
+
+// Carve one workspace out of a single allocation.
+struct Carver {
+    size_t off = 0;
+    uint8_t *base = nullptr;
+    template <class T> T *take(size_t count) {
+        off = (off + 255) & ~size_t(255);
+        T *p = base ? reinterpret_cast<T *>(base + off) : nullptr;
+        off += count * sizeof(T);
+        return p;
+    }
+};
+
+void layout(const lgcr_clip *c, Workspace *w, Carver &cv) {
+    const size_t ns = size_t(c->sw) * c->sh, nc = size_t(c->cw) * c->ch, no = size_t(c->ow) * c->oh;
+    w->raw_in[0] = cv.take<uint8_t>(ns * c->bps);
+    w->raw_in[1] = cv.take<uint8_t>(nc * c->bps);
+    w->raw_in[2] = cv.take<uint8_t>(nc * c->bps);
+    for (int i = 0; i < 3; ++i)
+        w->raw_out[i] = cv.take<uint8_t>(no * c->bps);
+    w->y = cv.take<float>(ns);
+    w->u = cv.take<float>(nc);
+    w->v = cv.take<float>(nc);
+    w->jxx = cv.take<float>(ns);
+    w->jxy = cv.take<float>(ns);
+    w->jyy = cv.take<float>(ns);
+    w->lc = cv.take<float>(nc);
+    w->ms = cv.take<float>(nc);
+    w->grad = cv.take<float>(6 * nc);
+    w->hbuf = cv.take<float>(6 * size_t(c->ow) * c->ch);
+    w->out_u = cv.take<float>(no);
+    w->out_v = cv.take<float>(no);
+    w->plain_u = cv.take<float>(no);
+    w->plain_v = cv.take<float>(no);
+    if (!c->same) {
+        w->luma_tmp = cv.take<float>(size_t(c->ow) * c->sh);
+        w->luma_out = cv.take<float>(no);
+    }
+    w->seed = cv.take<uint8_t>(ns);
+    w->mask = cv.take<uint8_t>(ns);
+    w->mtmp = cv.take<uint8_t>(ns);
+    w->cmask = cv.take<uint8_t>(nc);
+    w->row_off = cv.take<uint32_t>(size_t(c->oh) + 1);
+    w->indices = cv.take<uint32_t>(no);
+    w->counters = cv.take<uint32_t>(4);
+}
+
+lgcr_status create_workspace(lgcr_clip *c, std::unique_ptr<Workspace> &out) {
+    auto w = std::make_unique<Workspace>();
+    Carver probe;
+    layout(c, w.get(), probe);
+    const size_t bytes = probe.off + 256;
+    if (!c->dev->reserve(bytes))
+        return fail(LGCR_REJECTED, "clip workspace of %zu bytes exceeds the device HBM budget (%llu of %llu reserved)",
+                    bytes, (unsigned long long)c->dev->reserved.load(), (unsigned long long)c->dev->budget);
+    hipError_t e = hipSetDevice(c->dev->ordinal);
+    if (e == hipSuccess)
+        e = hipMalloc(&w->base, bytes);
+    if (e != hipSuccess) {
+        c->dev->unreserve(bytes);
+        return fail(LGCR_OUT_OF_MEMORY, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    w->bytes = bytes;
+    Carver cv;
+    cv.base = static_cast<uint8_t *>(w->base);
+    layout(c, w.get(), cv);
+    HIP_TRY(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
+    out = std::move(w);
+    return LGCR_OK;
+}
+
+void destroy_workspace(lgcr_clip *c, Workspace *w) {
+    if (w->stream) {
+        hipStreamSynchronize(w->stream);
+        hipStreamDestroy(w->stream);
+    }
+    for (auto &m : w->marks)
+        hipEventDestroy(m.second);
+    for (auto ev : w->pool)
+        hipEventDestroy(ev);
+    if (w->base) {
+        hipFree(w->base);
+        c->dev->unreserve(w->bytes);
+    }
+}
+
+lgcr_status acquire(lgcr_clip *c, int *slot) {
+    std::unique_lock<std::mutex> lock(c->ws_mutex);
+    for (;;) {
+        if (!c->idle.empty()) {
+            *slot = c->idle.back();
+            c->idle.pop_back();
+            return LGCR_OK;
+        }
+        if ((int)c->spaces.size() < c->max_inflight) {
+            std::unique_ptr<Workspace> w;
+            const lgcr_status st = create_workspace(c, w);
+            if (st == LGCR_OK) {
+                c->spaces.push_back(std::move(w));
+                *slot = (int)c->spaces.size() - 1;
+                return LGCR_OK;
+            }
+            if (c->spaces.empty())
+                return st;
+            // budget exhausted: wait for an existing workspace
+            c->max_inflight = (int)c->spaces.size();
+        }
+        c->ws_cv.wait(lock);
+    }
+}
+
+void release(lgcr_clip *c, int slot) {
+    {
+        std::lock_guard<std::mutex> lock(c->ws_mutex);
+        c->idle.push_back(slot);
+    }
+    c->ws_cv.notify_one();
+}
+
+// Turn the recorded stage timeline of a synchronised workspace into
+// per-stage microseconds; events go back to the pool.
+hipError_t collect_timeline(Workspace *w, double *us, uint64_t *n) {
+    hipError_t err = hipSuccess;
+    for (size_t i = 1; i < w->marks.size(); ++i) {
+        const int stage = w->marks[i].first;
+        if (stage < 0)
+            continue;
+        float ms = 0.0f;
+        const hipError_t e = hipEventElapsedTime(&ms, w->marks[i - 1].second, w->marks[i].second);
+        if (e != hipSuccess) {
+            err = e;
+            continue;
+        }
+        us[stage] += double(ms) * 1000.0;
+        n[stage] += 1;
+    }
+    for (auto &m : w->marks)
+        w->pool.push_back(m.second);
+    w->marks.clear();
+    return err;
+}
+
+void add_timing(lgcr_clip *c, const double *us, const uint64_t *n) {
+    std::lock_guard<std::mutex> lock(c->timing_mutex);
+    for (int i = 0; i < LGCR_STAGE_COUNT; ++i) {
+        c->stage_us[i] += us[i];
+        c->stage_n[i] += n[i];
+    }
+}
+
+// Enqueue one frame on workspace `w`.  `in` / `out` planes are device
+// pointers (already staged for host frames).
+lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const void *const in[3],
+                          const ptrdiff_t in_stride[3], void *const out[3], const ptrdiff_t out_stride[3],
+                          bool timed) {
+    const lgcr_params &p = c->p;
+    hipStream_t s = w->stream;
+    auto mark = [&](int stage) -> hipError_t {
+        if (!timed)
+            return hipSuccess;
+        hipEvent_t ev = nullptr;
+        if (!w->pool.empty()) {
+            ev = w->pool.back();
+            w->pool.pop_back();
+        } else {
+            const hipError_t e = hipEventCreate(&ev);
+            if (e != hipSuccess)
+                return e;
+        }
+        w->marks.emplace_back(stage, ev);
+        return hipEventRecord(ev, s);
+    };
+    const bool is_float = c->f.sample_type == 1;
+    const double yscale = is_float ? 1.0 : 1.0 / double((1 << c->f.bits_per_sample) - 1);
+    const double coff = is_float ? 0.0 : -0.5;
+
+    HIP_TRY(mark(-1));
+    // ConvertInput (plugin.cpp:210-253)
+    RawPlane rp[3] = {
+        {in[0], in_stride[0], c->sw, c->sh, w->y, yscale, 0.0},
+        {in[1], in_stride[1], c->cw, c->ch, w->u, yscale, coff},
+        {in[2], in_stride[2], c->cw, c->ch, w->v, yscale, coff},
+    };
+    HIP_TRY(launch_to_float(rp, 3, c->raw_type, s));
+    HIP_TRY(mark(LGCR_STAGE_CONVERT_INPUT));
+
+    const double oscale = is_float ? 1.0 : double((1 << c->f.bits_per_sample) - 1);
+    const int hi = is_float ? 0 : (1 << c->f.bits_per_sample) - 1;
+    // ResampleLuma (plugin.cpp:273-316)
+    if (!c->same) {
+        if (p.radial) {
+            HIP_TRY(launch_resample_radial(w->y, c->sw, c->sh, w->luma_out, c->ow, c->oh, p.support,
+                                           static_cast<const float *>(c->lut_dev), c->lut_n, 1024.0f, s));
+        } else {
+            HIP_TRY(launch_resample_h(w->y, c->sw, c->sh, w->luma_tmp, c->wh, s));
+            HIP_TRY(launch_resample_v(w->luma_tmp, c->ow, c->sh, w->luma_out, c->wv, s));
+        }
+        RawOut ro{w->luma_out, out[0], out_stride[0], c->ow, c->oh, float(oscale), 0.0f, 0, hi};
+        HIP_TRY(launch_from_float(&ro, 1, c->raw_type, s));
+        HIP_TRY(mark(LGCR_STAGE_RESAMPLE_LUMA));
+    }
+
+    w->has_ws = false;
+    const bool guided = p.strength > 0.0;
+    if (guided) {
+        // BuildGuideMaps (plugin.cpp:324-334, maps.cpp:252-276)
+        const double thr = p.algo == 4 ? 0.25 * p.sigma : p.sigma;
+        HIP_TRY(launch_sobel_tensor(w->y, c->sw, c->sh, w->jxx, w->jxy, w->jyy, p.sparse ? w->seed : nullptr,
+                                    float(thr * thr), s));
+        HIP_TRY(launch_lc_map(w->y, c->sw, c->sh, w->lc, c->cw, c->ch, g.xb, g.xe, g.yb, g.ye, s));
+        HIP_TRY(mark(LGCR_STAGE_BUILD_GUIDE_MAPS));
+        if (p.sparse) {
+            // BuildTrustMask + sparse workset (plugin.cpp:335-355)
+            HIP_TRY(launch_dilate(w->seed, w->mtmp, w->mask, c->sw, c->sh, c->dil, s));
+            HIP_TRY(launch_workset(w->mask, c->sw, c->sh, w->row_off, w->indices, w->cmask, c->cw, c->ch, g.ax,
+                                   g.ay, w->counters, s));
+            w->has_ws = true;
+            HIP_TRY(mark(LGCR_STAGE_BUILD_TRUST_MASK));
+        }
+        if (p.ms > 0.0) {
+            // BuildMutualGate (plugin.cpp:356-365)
+            HIP_TRY(launch_mutual_gate(w->lc, w->u, w->v, c->cw, c->ch, w->grad, w->has_ws ? w->cmask : nullptr,
+                                       w->counters, w->ms, s));
+            HIP_TRY(mark(LGCR_STAGE_BUILD_MUTUAL_GATE));
+        }
+    }
+
+    const int clamp_hull = p.ar_margin >= 0.0;
+    const float margin = float(std::max(0.0, p.ar_margin));
+    auto plain = [&]() -> hipError_t {
+        if (p.kernel == LGCR_KERNEL_BILINEAR)
+            return launch_plain_bilinear(w->u, w->v, c->cw, c->ch, g.ax, g.ay, w->out_u, w->out_v, s);
+        return launch_plain_separable(w->u, w->v, c->cw, c->ch, g.ax, g.ay, w->hbuf, w->out_u, w->out_v,
+                                      clamp_hull, margin, s);
+    };
+    if (!guided || w->has_ws) {
+        // BuildBaseChroma (plugin.cpp:627-648)
+        HIP_TRY(plain());
+        if (c->capture && guided) {
+            const size_t no = size_t(c->ow) * c->oh * sizeof(float);
+            HIP_TRY(hipMemcpyAsync(w->plain_u, w->out_u, no, hipMemcpyDeviceToDevice, s));
+            HIP_TRY(hipMemcpyAsync(w->plain_v, w->out_v, no, hipMemcpyDeviceToDevice, s));
+        }
+        HIP_TRY(mark(LGCR_STAGE_BUILD_BASE_CHROMA));
+    }
+    if (guided) {
+        // ApplyGuidedCorrection (plugin.cpp:649-674)
+        GuideParams gp;
+        gp.sigma = float(p.sigma);
+        gp.sratio = float(p.sratio);
+        gp.sdb = float(p.sdb);
+        gp.strength = float(p.strength);
+        gp.rescue = float(p.rescue);
+        gp.stretch = float(p.stretch);
+        gp.inv_gsigma2 = 1.0f / float(p.gsigma * p.gsigma);
+        gp.ms = float(p.ms);
+        gp.margin = margin;
+        gp.sp_out = float(std::max(c->rw, c->rh));
+        gp.frw = float(c->rw);
+        gp.frh = float(c->rh);
+        gp.ridge = p.ridge;
+        gp.cedge = p.cedge;
+        gp.clamp_hull = clamp_hull;
+        gp.ms_on = p.ms > 0.0;
+        gp.radial = p.radial;
+        HIP_TRY(launch_guided(w->u, w->v, c->cw, c->ch, w->y, c->sw, c->sh, w->jxx, w->jxy, w->jyy, w->lc,
+                              gp.ms_on ? w->ms : nullptr, g.ax, g.ay, gp, w->row_off,
+                              w->has_ws ? w->indices : nullptr, w->out_u, w->out_v, s));
+        HIP_TRY(mark(LGCR_STAGE_APPLY_GUIDED_CORRECTION));
+    }
+    // BackProject, same-size 4:2:0 only (plugin.cpp:374-375, 683-689)
+    if (p.bp > 0.0 && c->same && c->cw * 2 == c->sw && c->ch * 2 == c->sh) {
+        HIP_TRY(launch_back_project(w->out_u, w->out_v, c->ow, c->oh, w->u, w->v, c->cw, c->ch, float(p.bp), s));
+        HIP_TRY(mark(LGCR_STAGE_BACK_PROJECT));
+    }
+    // ConvertOutput (plugin.cpp:691-711)
+    const float offs = is_float ? 0.0f : float(0.5 * oscale);
+    RawOut ro[2] = {
+        {w->out_u, out[1], out_stride[1], c->ow, c->oh, float(oscale), offs, 0, hi},
+        {w->out_v, out[2], out_stride[2], c->ow, c->oh, float(oscale), offs, 0, hi},
+    };
+    HIP_TRY(launch_from_float(ro, 2, c->raw_type, s));
+    HIP_TRY(mark(LGCR_STAGE_CONVERT_OUTPUT));
+    return LGCR_OK;
+}
+
+lgcr_status validate(const lgcr_params &p, const lgcr_format &f) {
+    if (f.sample_type == 1 ? f.bits_per_sample != 32 : (f.bits_per_sample < 8 || f.bits_per_sample > 16))
+        return fail(LGCR_INVALID_ARGUMENT, "LGCR: input must be planar YUV (8-16 bit int or 32 bit float)");
+    if (f.sub_sampling_w < 0 || f.sub_sampling_w > 1 || f.sub_sampling_h < 0 || f.sub_sampling_h > 1)
+        return fail(LGCR_INVALID_ARGUMENT, "LGCR: only 4:4:4 / 4:2:2 / 4:2:0 input is supported");
+    if (f.width <= 0 || f.height <= 0)
+        return fail(LGCR_INVALID_ARGUMENT, "LGCR: constant input dimensions required");
+    if (p.out_w < 0 || p.out_h < 0)
+        return fail(LGCR_INVALID_ARGUMENT, "LGCR: invalid width/height");
+    if (p.algo != 2 && p.algo != 4 && p.algo != 6)
+        return fail(LGCR_INVALID_ARGUMENT, "LGCR: algo must be 2 (guided), 4 (selector) or 6 (detail transfer)");
+    if (p.strength < 0.0 || p.strength > 1.0 || p.sigma <= 0.0 || p.sratio <= 0.0 || p.sdb <= 0.0 ||
+        p.stretch < 0.0 || p.gsigma <= 0.0 || p.reg <= 0.0 || p.bp < 0.0 || p.bp > 1.0 || p.ms < 0.0 ||
+        p.ms > 1.0 || p.qgate < 0.0 || p.qgate > 1.0 || p.rescue < 0.0 || p.rescue > 1.0)
+        return fail(LGCR_INVALID_ARGUMENT,
+                    "LGCR: invalid parameter range (need 0<=strength/rescue<=1, 0<=bp/ms/qgate<=1, "
+                    "sigma/sratio/sdb/gsigma/reg>0, stretch>=0)");
+    if ((p.kernel == LGCR_KERNEL_LANCZOS || p.kernel == LGCR_KERNEL_JINC) && (p.kp1 < 1.0 || p.kp1 > 64.0))
+        return fail(LGCR_INVALID_ARGUMENT, "LGCR: taps must be in the range 1..64");
+    if (p.kernel < 0 || p.kernel > LGCR_KERNEL_JINC)
+        return fail(LGCR_INVALID_ARGUMENT, "LGCR: unknown kernel");
+    if (p.strength > 0.0 && p.algo != 2)
+        return fail(LGCR_UNSUPPORTED, "LGCR HIP backend: algo %d is not available in this build", p.algo);
+    if (p.bm)
+        return fail(LGCR_UNSUPPORTED, "LGCR HIP backend: bm=True is not available in this build");
+    if (p.radial)
+        return fail(LGCR_UNSUPPORTED, "LGCR HIP backend: the jinc kernel is not available in this build");
+    return LGCR_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+int lgcr_abi_version(void) { return LGCR_ABI_VERSION; }
+
+const char *lgcr_last_error(void) { return g_last_error.c_str(); }
+
+void lgcr_params_default(lgcr_params *p) {
+    std::memset(p, 0, sizeof(*p));
+    p->kernel = LGCR_KERNEL_LANCZOS;
+    p->kp1 = 3.0;
+    p->kp2 = 0.6;
+    p->support = 3.0;
+    p->strength = 0.8;
+    p->sigma = 0.01;
+    p->sratio = 0.15;
+    p->sdb = 3.0;
+    p->reg = 0.005;
+    p->stretch = 1.0;
+    p->gsigma = 2.5;
+    p->rescue = 1.0;
+    p->algo = 2;
+    p->ridge = 1;
+    p->cedge = 0;
+    p->ms = 1.0;
+    p->qgate = 1.0;
+    p->sparse = 1;
+    p->bp = 0.0;
+    p->bm = 0;
+    p->trad = 1;
+    p->tsearch = 6;
+    p->tsad = 0.02;
+    p->ar_margin = 0.0;
+    p->shift_x = -0.5;
+    p->shift_y = 0.0;
+}
+
+lgcr_status lgcr_device_open(int ordinal, uint64_t budget, lgcr_device **out) {
+    if (!out)
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_device_open: out is NULL");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+        return fail(LGCR_INVALID_CONTEXT, "LGCR: no HIP device is available");
+    if (ordinal < 0 || ordinal >= count)
+        return fail(LGCR_INVALID_ARGUMENT, "LGCR: device ordinal %d out of range (%d devices)", ordinal, count);
+    HIP_TRY(hipSetDevice(ordinal));
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    auto d = new lgcr_device();
+    d->ordinal = ordinal;
+    d->budget = budget ? budget : uint64_t(double(free_b) * 0.9);
+    *out = d;
+    return LGCR_OK;
+}
+
+void lgcr_device_close(lgcr_device *dev) { delete dev; }
+
+uint64_t lgcr_device_reserved(const lgcr_device *dev) { return dev ? dev->reserved.load() : 0; }
+
+lgcr_status lgcr_clip_create(lgcr_device *dev, const lgcr_params *params, const lgcr_format *fmt,
+                             int max_inflight, lgcr_clip **out) {
+    if (!dev || !params || !fmt || !out)
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_clip_create: NULL argument");
+    *out = nullptr;
+    lgcr_params p = *params;
+    if (p.out_w == 0)
+        p.out_w = fmt->width;
+    if (p.out_h == 0)
+        p.out_h = fmt->height;
+    lgcr_status st = validate(p, *fmt);
+    if (st != LGCR_OK)
+        return st;
+    // zero siting on non-subsampled axes (plugin.cpp:1130-1133)
+    if (fmt->sub_sampling_w == 0)
+        p.shift_x = 0.0;
+    if (fmt->sub_sampling_h == 0)
+        p.shift_y = 0.0;
+    auto c = std::make_unique<lgcr_clip>();
+    c->dev = dev;
+    c->p = p;
+    c->f = *fmt;
+    c->sw = fmt->width;
+    c->sh = fmt->height;
+    c->cw = fmt->width >> fmt->sub_sampling_w;
+    c->ch = fmt->height >> fmt->sub_sampling_h;
+    c->ow = p.out_w;
+    c->oh = p.out_h;
+    c->rw = double(c->sw) / c->cw;
+    c->rh = double(c->sh) / c->ch;
+    c->same = c->ow == c->sw && c->oh == c->sh;
+    c->raw_type = fmt->sample_type == 1 ? RAW_F32 : (fmt->bytes_per_sample == 1 ? RAW_U8 : RAW_U16);
+    c->bps = fmt->sample_type == 1 ? 4 : fmt->bytes_per_sample;
+    c->dil = int(std::ceil(p.support * std::max(c->rw, c->rh))) + 8;
+    c->max_inflight = max_inflight > 0 ? max_inflight : 4;
+    if (size_t(c->sw) + 2 * size_t(c->dil) > 60 * 1024)
+        return fail(LGCR_UNSUPPORTED, "LGCR HIP backend: frame width %d too large", c->sw);
+    if (const char *cap = std::getenv("LGCR_CAPTURE"))
+        c->capture = cap[0] && cap[0] != '0';
+    HIP_TRY(hipSetDevice(dev->ordinal));
+    if (!c->same && !p.radial) {
+        const HostWeights th = build_weights(c->sw, c->ow, p.kernel, p.kp1, p.kp2, p.support, 0.0);
+        const HostWeights tv = build_weights(c->sh, c->oh, p.kernel, p.kp1, p.kp2, p.support, 0.0);
+        Blob b;
+        const size_t a0 = b.add(th.start), a1 = b.add(th.w), a2 = b.add(tv.start), a3 = b.add(tv.w);
+        HIP_TRY(hipMalloc(&c->weights_dev, b.host.size()));
+        HIP_TRY(hipMemcpy(c->weights_dev, b.host.data(), b.host.size(), hipMemcpyHostToDevice));
+        const uint8_t *base = static_cast<const uint8_t *>(c->weights_dev);
+        c->wh = WeightsDev{th.n, th.sup, reinterpret_cast<const int *>(base + a0),
+                           reinterpret_cast<const float *>(base + a1)};
+        c->wv = WeightsDev{tv.n, tv.sup, reinterpret_cast<const int *>(base + a2),
+                           reinterpret_cast<const float *>(base + a3)};
+    }
+    if (p.radial) {
+        // jinc profile LUT, 1024 entries per unit distance (plugin.cpp:1071-1076)
+        const int n = int(p.support * 1024) + 2;
+        std::vector<float> lut(n);
+        for (int i = 0; i < n; ++i)
+            lut[i] = float(jinc_windowed(double(i) / 1024.0, p.support));
+        HIP_TRY(hipMalloc(&c->lut_dev, sizeof(float) * n));
+        HIP_TRY(hipMemcpy(c->lut_dev, lut.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+        c->lut_n = n;
+    }
+    Geometry *g = nullptr;
+    st = get_geometry(c.get(), p.shift_x, p.shift_y, &g);
+    if (st != LGCR_OK)
+        return st;
+    // admission: one workspace must fit the device budget
+    std::unique_ptr<Workspace> w;
+    st = create_workspace(c.get(), w);
+    if (st != LGCR_OK) {
+        lgcr_clip_destroy(c.release());
+        return st;
+    }
+    c->ws_bytes = w->bytes;
+    c->spaces.push_back(std::move(w));
+    c->idle.push_back(0);
+    *out = c.release();
+    return LGCR_OK;
+}
+
+void lgcr_clip_destroy(lgcr_clip *c) {
+    if (!c)
+        return;
+    hipSetDevice(c->dev->ordinal);
+    for (auto &w : c->spaces)
+        destroy_workspace(c, w.get());
+    for (auto &kv : c->geos)
+        hipFree(kv.second->dev);
+    if (c->lut_dev)
+        hipFree(c->lut_dev);
+    if (c->weights_dev)
+        hipFree(c->weights_dev);
+    delete c;
+}
+
+uint64_t lgcr_clip_workspace_bytes(const lgcr_clip *c) { return c ? c->ws_bytes : 0; }
+
+lgcr_status lgcr_recon_frame(lgcr_clip *c, const lgcr_frame *in, double shift_x, double shift_y,
+                             lgcr_frame *out, lgcr_stats *stats) {
+    if (!c || !in || !out)
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_recon_frame: NULL argument");
+    if (in->memory != out->memory)
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_recon_frame: input and output must be in the same memory");
+    for (int i = 0; i < 3; ++i)
+        if (!in->data[i] || ((i > 0 || !c->same) && !out->data[i]))
+            return fail(LGCR_INVALID_ARGUMENT, "lgcr_recon_frame: plane %d is NULL", i);
+    double sx = std::isnan(shift_x) ? c->p.shift_x : shift_x;
+    double sy = std::isnan(shift_y) ? c->p.shift_y : shift_y;
+    if (c->f.sub_sampling_w == 0)
+        sx = 0.0;
+    if (c->f.sub_sampling_h == 0)
+        sy = 0.0;
+    Geometry *g = nullptr;
+    lgcr_status st = get_geometry(c, sx, sy, &g);
+    if (st != LGCR_OK)
+        return st;
+    int slot = -1;
+    st = acquire(c, &slot);
+    if (st != LGCR_OK)
+        return st;
+    Workspace *w = c->spaces[slot].get();
+    struct Releaser {
+        lgcr_clip *c;
+        int slot;
+        ~Releaser() { release(c, slot); }
+    } rel{c, slot};
+    HIP_TRY(hipSetDevice(c->dev->ordinal));
+    const int pw[3] = {c->sw, c->cw, c->cw}, ph[3] = {c->sh, c->ch, c->ch};
+    const void *din[3];
+    ptrdiff_t din_stride[3];
+    void *dout[3];
+    ptrdiff_t dout_stride[3];
+    const bool host = in->memory == LGCR_MEMORY_HOST;
+    for (int i = 0; i < 3; ++i) {
+        if (host) {
+            const size_t rb = size_t(pw[i]) * c->bps;
+            HIP_TRY(hipMemcpy2DAsync(w->raw_in[i], rb, in->data[i], in->stride[i], rb, ph[i], hipMemcpyHostToDevice,
+                                     w->stream));
+            din[i] = w->raw_in[i];
+            din_stride[i] = ptrdiff_t(rb);
+            dout[i] = w->raw_out[i];
+            dout_stride[i] = ptrdiff_t(size_t(c->ow) * c->bps);
+        } else {
+            din[i] = in->data[i];
+            din_stride[i] = in->stride[i];
+            dout[i] = out->data[i];
+            dout_stride[i] = out->stride[i];
+        }
+    }
+    if (!host && c->same && !dout[0]) {
+        dout[0] = nullptr;
+    }
+    const bool timed = stats != nullptr || c->timing.load();
+    w->marks.clear();
+    st = enqueue_frame(c, w, *g, din, din_stride, dout, dout_stride, timed);
+    if (st != LGCR_OK)
+        return st;
+    if (host) {
+        for (int i = c->same ? 1 : 0; i < 3; ++i) {
+            const size_t rb = size_t(c->ow) * c->bps;
+            HIP_TRY(hipMemcpy2DAsync(out->data[i], out->stride[i], w->raw_out[i], rb, rb, c->oh,
+                                     hipMemcpyDeviceToHost, w->stream));
+        }
+        if (c->same && out->data[0] && out->data[0] != in->data[0]) {
+            const size_t rb = size_t(c->sw) * c->bps;
+            for (int yy = 0; yy < c->sh; ++yy)
+                std::memcpy(static_cast<uint8_t *>(out->data[0]) + out->stride[0] * yy,
+                            static_cast<const uint8_t *>(in->data[0]) + in->stride[0] * yy, rb);
+        }
+    } else if (c->same && out->data[0] && out->data[0] != in->data[0]) {
+        const size_t rb = size_t(c->sw) * c->bps;
+        HIP_TRY(hipMemcpy2DAsync(out->data[0], out->stride[0], in->data[0], in->stride[0], rb, c->sh,
+                                 hipMemcpyDeviceToDevice, w->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(w->stream));
+    double us[LGCR_STAGE_COUNT] = {};
+    uint64_t nst[LGCR_STAGE_COUNT] = {};
+    if (timed) {
+        HIP_TRY(collect_timeline(w, us, nst));
+        if (c->timing.load())
+            add_timing(c, us, nst);
+    }
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        for (int i = 0; i < LGCR_STAGE_COUNT; ++i) {
+            stats->stage_us[i] = us[i];
+            stats->total_us += us[i];
+        }
+        const uint64_t ns = uint64_t(c->sw) * c->sh, nc = uint64_t(c->cw) * c->ch, no = uint64_t(c->ow) * c->oh;
+        stats->stage_pixels[LGCR_STAGE_CONVERT_INPUT] = ns + 2 * nc;
+        stats->stage_pixels[LGCR_STAGE_CONVERT_OUTPUT] = 2 * no;
+        stats->output_pixels = uint64_t(c->ow) * c->oh;
+        if (w->has_ws) {
+            uint32_t cnt[1] = {0}, act = 0;
+            HIP_TRY(hipMemcpy(&act, w->row_off + c->oh, sizeof(uint32_t), hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(cnt, w->counters, sizeof(uint32_t), hipMemcpyDeviceToHost));
+            stats->active_output_pixels = act;
+            stats->active_chroma_pixels = cnt[0];
+            stats->sparse_total_pixels = uint64_t(c->sw) * c->sh;
+        }
+    }
+    return LGCR_OK;
+}
+
+lgcr_status lgcr_recon_batch(lgcr_clip *c, int n, const lgcr_frame *in, lgcr_frame *out) {
+    if (!c || n < 0 || (n && (!in || !out)))
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_recon_batch: bad arguments");
+    for (int k = 0; k < n; ++k)
+        if (in[k].memory != LGCR_MEMORY_DEVICE || out[k].memory != LGCR_MEMORY_DEVICE)
+            return fail(LGCR_INVALID_ARGUMENT, "lgcr_recon_batch: frames must be device resident");
+    Geometry *g = nullptr;
+    lgcr_status st = get_geometry(c, c->p.shift_x, c->p.shift_y, &g);
+    if (st != LGCR_OK)
+        return st;
+    HIP_TRY(hipSetDevice(c->dev->ordinal));
+    // take every workspace the clip may use and round-robin the frames
+    std::vector<int> slots;
+    {
+        int s0 = -1;
+        st = acquire(c, &s0);
+        if (st != LGCR_OK)
+            return st;
+        slots.push_back(s0);
+        for (;;) {
+            std::unique_lock<std::mutex> lock(c->ws_mutex);
+            if (!c->idle.empty()) {
+                slots.push_back(c->idle.back());
+                c->idle.pop_back();
+                continue;
+            }
+            if ((int)c->spaces.size() < c->max_inflight && (int)slots.size() < n) {
+                std::unique_ptr<Workspace> w;
+                if (create_workspace(c, w) == LGCR_OK) {
+                    c->spaces.push_back(std::move(w));
+                    slots.push_back((int)c->spaces.size() - 1);
+                    continue;
+                }
+                c->max_inflight = (int)c->spaces.size();
+            }
+            break;
+        }
+    }
+    lgcr_status result = LGCR_OK;
+    const bool timed = c->timing.load();
+    for (int s : slots)
+        c->spaces[s]->marks.clear();
+    for (int k = 0; k < n && result == LGCR_OK; ++k) {
+        Workspace *w = c->spaces[slots[k % slots.size()]].get();
+        void *dout[3] = {out[k].data[0], out[k].data[1], out[k].data[2]};
+        result = enqueue_frame(c, w, *g, in[k].data, in[k].stride, dout, out[k].stride, timed);
+        if (result == LGCR_OK && c->same && dout[0] && dout[0] != in[k].data[0]) {
+            const hipError_t e = hipMemcpy2DAsync(dout[0], out[k].stride[0], in[k].data[0], in[k].stride[0],
+                                                  size_t(c->sw) * c->bps, c->sh, hipMemcpyDeviceToDevice, w->stream);
+            if (e != hipSuccess)
+                result = fail(LGCR_BACKEND_FAILURE, "luma copy failed: %s", hipGetErrorString(e));
+        }
+    }
+    double us[LGCR_STAGE_COUNT] = {};
+    uint64_t nst[LGCR_STAGE_COUNT] = {};
+    for (int s : slots) {
+        Workspace *w = c->spaces[s].get();
+        const hipError_t e = hipStreamSynchronize(w->stream);
+        if (e != hipSuccess && result == LGCR_OK)
+            result = fail(LGCR_BACKEND_FAILURE, "hipStreamSynchronize failed: %s", hipGetErrorString(e));
+        if (timed && e == hipSuccess)
+            collect_timeline(w, us, nst);
+        w->marks.clear();
+        release(c, s);
+    }
+    if (timed)
+        add_timing(c, us, nst);
+    return result;
+}
+
+lgcr_status lgcr_clip_set_timing(lgcr_clip *c, int enable) {
+    if (!c)
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_clip_set_timing: NULL clip");
+    std::lock_guard<std::mutex> lock(c->timing_mutex);
+    c->timing.store(enable != 0);
+    std::memset(c->stage_us, 0, sizeof(c->stage_us));
+    std::memset(c->stage_n, 0, sizeof(c->stage_n));
+    return LGCR_OK;
+}
+
+lgcr_status lgcr_clip_stage_times(lgcr_clip *c, double *us, uint64_t *launches) {
+    if (!c)
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_clip_stage_times: NULL clip");
+    std::lock_guard<std::mutex> lock(c->timing_mutex);
+    for (int i = 0; i < LGCR_STAGE_COUNT; ++i) {
+        if (us)
+            us[i] = c->stage_us[i];
+        if (launches)
+            launches[i] = c->stage_n[i];
+    }
+    return LGCR_OK;
+}
+
+lgcr_status lgcr_clip_fetch(lgcr_clip *c, int what, void *dst, size_t bytes, size_t *needed) {
+    if (!c)
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_clip_fetch: NULL clip");
+    int slot = -1;
+    {
+        // workspace 0 only; wait until it is idle
+        std::unique_lock<std::mutex> lock(c->ws_mutex);
+        for (;;) {
+            auto it = std::find(c->idle.begin(), c->idle.end(), 0);
+            if (it != c->idle.end()) {
+                c->idle.erase(it);
+                slot = 0;
+                break;
+            }
+            c->ws_cv.wait(lock);
+        }
+    }
+    struct Releaser {
+        lgcr_clip *c;
+        ~Releaser() { release(c, 0); }
+    } rel{c};
+    Workspace *w = c->spaces[slot].get();
+    const size_t ns = size_t(c->sw) * c->sh, nc = size_t(c->cw) * c->ch, no = size_t(c->ow) * c->oh;
+    const void *src = nullptr;
+    size_t size = 0;
+    switch (what) {
+    case LGCR_FETCH_Y: src = w->y; size = ns * 4; break;
+    case LGCR_FETCH_U: src = w->u; size = nc * 4; break;
+    case LGCR_FETCH_V: src = w->v; size = nc * 4; break;
+    case LGCR_FETCH_JXX: src = w->jxx; size = ns * 4; break;
+    case LGCR_FETCH_JXY: src = w->jxy; size = ns * 4; break;
+    case LGCR_FETCH_JYY: src = w->jyy; size = ns * 4; break;
+    case LGCR_FETCH_LC: src = w->lc; size = nc * 4; break;
+    case LGCR_FETCH_MS: src = w->ms; size = nc * 4; break;
+    case LGCR_FETCH_SEED: src = w->seed; size = ns; break;
+    case LGCR_FETCH_MASK: src = w->mask; size = ns; break;
+    case LGCR_FETCH_CHROMA_MASK: src = w->cmask; size = nc; break;
+    case LGCR_FETCH_INDICES: {
+        uint32_t count = 0;
+        HIP_TRY(hipMemcpy(&count, w->row_off + c->oh, sizeof(count), hipMemcpyDeviceToHost));
+        src = w->indices;
+        size = w->has_ws ? size_t(count) * 4 : 0;
+        break;
+    }
+    case LGCR_FETCH_PLAIN_U: src = w->plain_u; size = no * 4; break;
+    case LGCR_FETCH_PLAIN_V: src = w->plain_v; size = no * 4; break;
+    case LGCR_FETCH_OUT_U: src = w->out_u; size = no * 4; break;
+    case LGCR_FETCH_OUT_V: src = w->out_v; size = no * 4; break;
+    default:
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_clip_fetch: unknown id %d", what);
+    }
+    if (needed)
+        *needed = size;
+    if (dst && bytes && size)
+        HIP_TRY(hipMemcpy(dst, src, std::min(bytes, size), hipMemcpyDeviceToHost));
+    return LGCR_OK;
+}
+
+}  // extern "C"
