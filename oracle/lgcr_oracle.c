@@ -1095,10 +1095,17 @@ static void from_float(const or_frame_format *f, const float *src, void *dst, pt
     }
 }
 
-int or_recon_frame(const or_params *p, const or_frame_format *f,
+int or_recon_frame(const or_params *pin, const or_frame_format *f,
                    const void *const in[3], const ptrdiff_t in_stride[3],
                    void *const out[3], const ptrdiff_t out_stride[3]) {
     or_clear_debug();
+    /* siting only applies on subsampled axes (plugin.cpp:159-162, 1130-1133) */
+    or_params pl = *pin;
+    if (f->ss_w == 0)
+        pl.shift_x = 0.0;
+    if (f->ss_h == 0)
+        pl.shift_y = 0.0;
+    const or_params *p = &pl;
     if (p->algo != 2 && p->strength > 0.0)
         return -2; /* algo 4/6 restated separately */
     if (p->bm)
