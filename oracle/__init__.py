@@ -127,6 +127,8 @@ def lib():
         L.or_plain_chroma.argtypes = [C.POINTER(Params), fp, fp, C.c_int, C.c_int, C.c_int, C.c_int, fp, fp]
         L.or_guided.argtypes = [C.POINTER(Params), fp, fp, C.c_int, C.c_int, C.POINTER(Guide),
                                 C.POINTER(Workset), fp, fp]
+        L.or_lgf_maps.argtypes = [fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                  C.c_double, fp, fp, C.c_double, C.c_int, u8p, fp]
         L.or_back_project.argtypes = [fp, C.c_int, C.c_int, fp, C.c_int, C.c_int, C.c_float]
         L.or_recon_frame.argtypes = [C.POINTER(Params), C.POINTER(FrameFormat), C.c_void_p * 3,
                                      C.c_ssize_t * 3, C.c_void_p * 3, C.c_ssize_t * 3]

@@ -713,13 +713,168 @@ static void plain_bilinear(const float *u, const float *v, int cw, int ch,
         }
 }
 
+
+/* ------------------------------------------------------------------------ */
+/* radial (jinc) phase tables: kernels.cpp:449-526                           */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    int valid, sup_x, sup_y, nx, ny;
+    uint16_t *xphase, *yphase;
+    float *w;                  /* nx*ny phases, sup_y*sup_x each */
+} radial_table;
+
+/* unique (offset bits, activity mask) phases in first-seen order */
+static int collect_phases(const or_axis *a, uint16_t *index, float *offsets, uint64_t *activity) {
+    if (a->sup > 64)
+        return -1;
+    int count = 0;
+    uint32_t bits[65];
+    for (int x = 0; x < a->n; ++x) {
+        const float off = (float)a->start[x] - a->pos[x];
+        uint32_t b;
+        memcpy(&b, &off, 4);
+        uint64_t m = 0;
+        for (int t = 0; t < a->sup; ++t)
+            if (a->am[(size_t)x * a->sup + t] != 0.0f)
+                m |= (uint64_t)1 << t;
+        int k = 0;
+        while (k < count && !(bits[k] == b && activity[k] == m))
+            ++k;
+        if (k == count) {
+            if (count == 64)
+                return -1;
+            bits[count] = b;
+            offsets[count] = off;
+            activity[count] = m;
+            ++count;
+        }
+        index[x] = (uint16_t)k;
+    }
+    return count;
+}
+
+static void build_radial_table(const or_params *p, const or_axis *ax, const or_axis *ay,
+                               int src_w, int src_h, radial_table *t) {
+    memset(t, 0, sizeof(*t));
+    t->sup_x = ax->sup;
+    t->sup_y = ay->sup;
+    float xo[64], yo[64];
+    uint64_t xa[64], ya[64];
+    t->xphase = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)ax->n);
+    t->yphase = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)ay->n);
+    const int nx = collect_phases(ax, t->xphase, xo, xa);
+    const int ny = nx < 0 ? -1 : collect_phases(ay, t->yphase, yo, ya);
+    if (nx < 0 || ny < 0)
+        return;
+    const size_t pairs = (size_t)nx * ny, per = (size_t)ax->sup * ay->sup;
+    if (pairs > 1024 || pairs * per > 262144)
+        return;
+    t->nx = nx;
+    t->ny = ny;
+    t->w = (float *)calloc(pairs * per, sizeof(float));
+    const float inv_x = (float)fmin(1.0, (double)ax->n / src_w);
+    const float inv_y = (float)fmin(1.0, (double)ay->n / src_h);
+    for (int py = 0; py < ny; ++py)
+        for (int px = 0; px < nx; ++px) {
+            float *w = t->w + ((size_t)py * nx + px) * per;
+            for (int j = 0; j < ay->sup; ++j)
+                for (int i = 0; i < ax->sup; ++i) {
+                    if (!(xa[px] & ((uint64_t)1 << i)) || !(ya[py] & ((uint64_t)1 << j)))
+                        continue;
+                    const float dx = (xo[px] + (float)i) * inv_x;
+                    const float dy = (yo[py] + (float)j) * inv_y;
+                    w[(size_t)j * ax->sup + i] = or_lut_lookup(p->lut, p->lut_n, sqrtf(dx * dx + dy * dy) * p->lut_scale);
+                }
+        }
+    t->valid = 1;
+}
+
+static const float *radial_at(const radial_table *t, int x, int y) {
+    if (!t || !t->valid)
+        return NULL;
+    return t->w + ((size_t)t->yphase[y] * t->nx + t->xphase[x]) * (size_t)t->sup_x * t->sup_y;
+}
+
+static void free_radial_table(radial_table *t) {
+    free(t->xphase); free(t->yphase); free(t->w);
+    memset(t, 0, sizeof(*t));
+}
+
+/* plainChromaImpl, scalar path (recon.cpp:828-1029, 970-1005) */
+static void plain_radial(const or_params *p, const float *u, const float *v, int cw, int ch,
+                         const or_axis *ax, const or_axis *ay, const radial_table *rt,
+                         int ow, int oh, float *out_u, float *out_v) {
+    const float inv_wx = (float)fmin(1.0, (double)ow / cw);
+    const float inv_wy = (float)fmin(1.0, (double)oh / ch);
+    const int clamp_hull = p->ar_margin >= 0.0;
+    const float margin = (float)(p->ar_margin > 0.0 ? p->ar_margin : 0.0);
+    for (int oy = 0; oy < oh; ++oy) {
+        const int ty0 = ay->start[oy];
+        const float *amy = ay->am + (size_t)oy * ay->sup;
+        const float scy = ay->pos[oy];
+        for (int ox = 0; ox < ow; ++ox) {
+            const int tx0 = ax->start[ox];
+            const float *amx = ax->am + (size_t)ox * ax->sup;
+            const float scx = ax->pos[ox];
+            const float *cw8 = radial_at(rt, ox, oy);
+            double au = 0.0, av = 0.0, ws = 0.0, sa = 0.0;
+            float mnu = 1e30f, mxu = -1e30f, mnv = 1e30f, mxv = -1e30f;
+            for (int j = 0; j < ay->sup; ++j) {
+                if (amy[j] == 0.0f)
+                    continue;
+                const int ty = clampi(ty0 + j, 0, ch - 1);
+                for (int i = 0; i < ax->sup; ++i) {
+                    if (amx[i] == 0.0f)
+                        continue;
+                    const int tx = clampi(tx0 + i, 0, cw - 1);
+                    float w;
+                    if (cw8) {
+                        w = cw8[(size_t)j * ax->sup + i];
+                    } else {
+                        const float dx = ((float)(tx0 + i) - scx) * inv_wx;
+                        const float dy = ((float)(ty0 + j) - scy) * inv_wy;
+                        w = or_lut_lookup(p->lut, p->lut_n, sqrtf(dx * dx + dy * dy) * p->lut_scale);
+                    }
+                    const float a = u[(size_t)ty * cw + tx], b = v[(size_t)ty * cw + tx];
+                    au += (double)w * a;
+                    ws += w;
+                    sa += fabsf(w);
+                    mnu = minf(mnu, a); mxu = maxf(mxu, a);
+                    av += (double)w * b;
+                    mnv = minf(mnv, b); mxv = maxf(mxv, b);
+                }
+            }
+            double vu, vv;
+            if (ws > 1e-6 && ws >= 0.05 * sa) {
+                vu = au / ws;
+                vv = av / ws;
+            } else {
+                vu = bilinear(u, cw, ch, scx, scy);
+                vv = bilinear(v, cw, ch, scx, scy);
+            }
+            if (clamp_hull) {
+                vu = clampd(vu, (double)(mnu - margin), (double)(mxu + margin));
+                vv = clampd(vv, (double)(mnv - margin), (double)(mxv + margin));
+            }
+            out_u[(size_t)oy * ow + ox] = (float)vu;
+            out_v[(size_t)oy * ow + ox] = (float)vv;
+        }
+    }
+}
+
 void or_plain_chroma(const or_params *p, const float *u, const float *v, int cw, int ch,
                      int sw, int sh, float *out_u, float *out_v) {
     const double rw = (double)sw / cw, rh = (double)sh / ch;
     or_axis ax, ay;
     or_build_axis(sw, p->out_w, rw, p->shift_x, p, &ax);
     or_build_axis(sh, p->out_h, rh, p->shift_y, p, &ay);
-    if (p->kernel == OR_BILINEAR && !p->radial)
+    if (p->radial) {
+        radial_table rt;
+        build_radial_table(p, &ax, &ay, cw, ch, &rt);
+        plain_radial(p, u, v, cw, ch, &ax, &ay, &rt, p->out_w, p->out_h, out_u, out_v);
+        free_radial_table(&rt);
+    } else if (p->kernel == OR_BILINEAR)
         plain_bilinear(u, v, cw, ch, &ax, &ay, p->out_w, p->out_h, out_u, out_v);
     else
         plain_separable(p, u, v, cw, ch, &ax, &ay, p->out_w, p->out_h, out_u, out_v);
@@ -788,10 +943,19 @@ static void guide_meta(const or_params *p, const or_guide *g, const or_axis *ax,
     }
 }
 
+typedef struct {
+    double vu, vv;
+    float ss_ramp, guide_fade, nx, ny, L0;
+    int has_dir;
+} gres;
+
 static void guided_pixel(const or_params *p, const float *U, const float *V, int cw, int ch,
                          const or_guide *g, const or_axis *ax, const or_axis *ay,
-                         float sp_out, double rw, double rh, int ox, int oy,
-                         float *out_u, float *out_v) {
+                         const radial_table *rt, float sp_out, double rw, double rh, int ox, int oy,
+                         gres *res) {
+    const float *radial_base = radial_at(rt, ox, oy);
+    const float inv_wxc = (float)fmin(1.0, (double)p->out_w / cw);
+    const float inv_wyc = (float)fmin(1.0, (double)p->out_h / ch);
     const int guided = p->strength > 0.0;
     guide_px meta;
     memset(&meta, 0, sizeof(meta));
@@ -885,7 +1049,23 @@ static void guided_pixel(const or_params *p, const float *U, const float *V, int
                     }
                 kappa = (mxX * mxX * mxY * mxY) / (ssX * ssY + 1e-12f);
             } else {
-                kappa = 1.0f; /* radial kernels are handled by the resize path */
+                float mx = 0, ss = 0;
+                for (int j = 0; j < ay->sup; ++j) {
+                    if (amy[j] == 0.0f)
+                        continue;
+                    const float dyc = ((float)(ty0 + j) - scy) * inv_wyc;
+                    for (int i = 0; i < ax->sup; ++i) {
+                        if (amx[i] == 0.0f)
+                            continue;
+                        const float dxc = ((float)(tx0 + i) - scx) * inv_wxc;
+                        const float w = radial_base ? radial_base[(size_t)j * ax->sup + i]
+                                                    : or_lut_lookup(p->lut, p->lut_n,
+                                                                    sqrtf(dxc * dxc + dyc * dyc) * p->lut_scale);
+                        mx = maxf(mx, fabsf(w));
+                        ss += w * w;
+                    }
+                }
+                kappa = (mx * mx) / (ss + 1e-12f);
             }
             rescue *= kappa * kappa;
         }
@@ -907,7 +1087,16 @@ static void guided_pixel(const or_params *p, const float *U, const float *V, int
             if (amx[i] == 0.0f)
                 continue;
             const int tx = clampi(tx0 + i, 0, cw - 1);
-            const float wx = wxp[i] * wy;
+            float wx;
+            if (radial_base) {
+                wx = radial_base[(size_t)j * ax->sup + i];
+            } else if (p->radial) {
+                const float dxc = ((float)(tx0 + i) - scx) * inv_wxc;
+                const float dyc = ((float)(ty0 + j) - scy) * inv_wyc;
+                wx = or_lut_lookup(p->lut, p->lut_n, sqrtf(dxc * dxc + dyc * dyc) * p->lut_scale);
+            } else {
+                wx = wxp[i] * wy;
+            }
             float w;
             if (guided) {
                 const float dx = ((float)(tx0 + i) - scx) * frw;
@@ -940,12 +1129,30 @@ static void guided_pixel(const or_params *p, const float *U, const float *V, int
         vu = clampd(vu, (double)(mnu - margin), (double)(mxu + margin));
         vv = clampd(vv, (double)(mnv - margin), (double)(mxv + margin));
     }
-    *out_u = (float)vu;
-    *out_v = (float)vv;
+    res->vu = vu;
+    res->vv = vv;
+    res->ss_ramp = ss_ramp;
+    res->guide_fade = guide_fade;
+    res->nx = nxv;
+    res->ny = nyv;
+    res->L0 = L0;
+    res->has_dir = meta.has_dir;
 }
 
-void or_guided(const or_params *p, const float *u, const float *v, int cw, int ch,
-               const or_guide *g, const or_workset *ws, float *out_u, float *out_v) {
+/* algo-4 selector modes of reconstructChromaImpl (recon.cpp:762-811) */
+enum { SEL_NONE = 0, SEL_MAPS, SEL_COMPRESSED, SEL_PLANES };
+typedef struct {
+    int mode;
+    float strength;                       /* selectorStrength */
+    const float *plain_u, *plain_v;       /* base (output grid) */
+    const float *au, *bu, *cu, *av, *bv, *cv;  /* LGF maps (chroma grid), SEL_MAPS */
+    float *du, *dv, *w3c;                 /* SEL_COMPRESSED, indexed by workset order */
+    float *w2p, *w3p;                     /* SEL_PLANES, output grid */
+} selector_cfg;
+
+static void guided_run(const or_params *p, const float *u, const float *v, int cw, int ch,
+                       const or_guide *g, const or_workset *ws, const selector_cfg *sel,
+                       float *out_u, float *out_v) {
     const int sw = g->sw, sh = g->sh;
     const double rw = (double)sw / cw, rh = (double)sh / ch;
     or_axis ax, ay;
@@ -953,22 +1160,214 @@ void or_guided(const or_params *p, const float *u, const float *v, int cw, int c
     or_build_axis(sh, p->out_h, rh, p->shift_y, p, &ay);
     const float sp_out = (float)(rw > rh ? rw : rh);
     const int ow = p->out_w, oh = p->out_h;
-    if (ws && ws->out_w == ow && ws->out_h == oh) {
-        for (size_t k = 0; k < ws->n_indices; ++k) {
-            const int ox = (int)(ws->indices[k] % (uint32_t)ow);
-            const int oy = (int)(ws->indices[k] / (uint32_t)ow);
-            const size_t o = (size_t)oy * ow + ox;
-            guided_pixel(p, u, v, cw, ch, g, &ax, &ay, sp_out, rw, rh, ox, oy, &out_u[o], &out_v[o]);
-        }
-    } else {
-        for (int oy = 0; oy < oh; ++oy)
-            for (int ox = 0; ox < ow; ++ox) {
-                const size_t o = (size_t)oy * ow + ox;
-                guided_pixel(p, u, v, cw, ch, g, &ax, &ay, sp_out, rw, rh, ox, oy, &out_u[o], &out_v[o]);
+    radial_table rt;
+    memset(&rt, 0, sizeof(rt));
+    if (p->radial)
+        build_radial_table(p, &ax, &ay, cw, ch, &rt);
+    const int sparse = ws && ws->out_w == ow && ws->out_h == oh;
+    const size_t total = sparse ? ws->n_indices : (size_t)ow * oh;
+    for (size_t k = 0; k < total; ++k) {
+        const uint32_t idx = sparse ? ws->indices[k] : (uint32_t)k;
+        const int ox = (int)(idx % (uint32_t)ow), oy = (int)(idx / (uint32_t)ow);
+        const size_t o = (size_t)oy * ow + ox;
+        gres r;
+        guided_pixel(p, u, v, cw, ch, g, &ax, &ay, &rt, sp_out, rw, rh, ox, oy, &r);
+        double vu = r.vu, vv = r.vv;
+        if (sel && sel->mode != SEL_NONE) {
+            const float diag = r.has_dir ? minf(1.0f, 2.0f * fabsf(r.nx * r.ny)) : 0.0f;
+            const float hedgy = (1.0f - r.ss_ramp) * r.guide_fade;
+            const float w2 = hedgy * (1.0f - diag) * sel->strength;
+            const float w3 = hedgy * diag * sel->strength;
+            if (sel->mode == SEL_COMPRESSED) {
+                const float bu = sel->plain_u[o], bv = sel->plain_v[o];
+                if (w2 < 1e-4f && w3 < 1e-4f) {
+                    sel->du[k] = 0.0f; sel->dv[k] = 0.0f; sel->w3c[k] = 0.0f;
+                } else {
+                    sel->du[k] = hedgy * (1.0f - diag) * ((float)vu - bu);
+                    sel->dv[k] = hedgy * (1.0f - diag) * ((float)vv - bv);
+                    sel->w3c[k] = hedgy * diag;
+                }
+                continue;  /* the compressed pass leaves the output alone */
+            } else if (sel->mode == SEL_PLANES) {
+                sel->w2p[o] = hedgy * (1.0f - diag);
+                sel->w3p[o] = hedgy * diag;
+            } else if (w2 >= 1e-4f || w3 >= 1e-4f) {
+                const int cx = ax.cb_i0[ox], cy = ay.cb_i0[oy];
+                const float cxf = ax.cb_f[ox], cyf = ay.cb_f[oy];
+                const float bu = out_u[o], bv = out_v[o];
+                const float au = bilinear_fast(sel->au, cw, ch, cx, cxf, cy, cyf);
+                const float bbu = bilinear_fast(sel->bu, cw, ch, cx, cxf, cy, cyf);
+                const float av = bilinear_fast(sel->av, cw, ch, cx, cxf, cy, cyf);
+                const float bbv = bilinear_fast(sel->bv, cw, ch, cx, cxf, cy, cyf);
+                const float cu = bilinear_fast(sel->cu, cw, ch, cx, cxf, cy, cyf);
+                const float cv = bilinear_fast(sel->cv, cw, ch, cx, cxf, cy, cyf);
+                vu = bu + w2 * ((float)vu - bu) + w3 * cu * (au * r.L0 + bbu - bu);
+                vv = bv + w2 * ((float)vv - bv) + w3 * cv * (av * r.L0 + bbv - bv);
+            } else {
+                vu = out_u[o];
+                vv = out_v[o];
             }
+        }
+        out_u[o] = (float)vu;
+        out_v[o] = (float)vv;
     }
+    free_radial_table(&rt);
     or_free_axis(&ax);
     or_free_axis(&ay);
+}
+
+void or_guided(const or_params *p, const float *u, const float *v, int cw, int ch,
+               const or_guide *g, const or_workset *ws, float *out_u, float *out_v) {
+    guided_run(p, u, v, cw, ch, g, ws, NULL, out_u, out_v);
+}
+
+
+/* ------------------------------------------------------------------------ */
+/* algo 4: LGF regression maps (algos.cpp:12-68, 241-315, 366-448)           */
+/* ------------------------------------------------------------------------ */
+
+/* pointSampledLuma (algos.cpp:300-315) */
+static void point_luma(const float *y, int w, int h, int cw, int ch, double rw, double rh,
+                       double sx, double sy, float *ls) {
+    float *xp = (float *)malloc(sizeof(float) * (size_t)cw), *yp = (float *)malloc(sizeof(float) * (size_t)ch);
+    int *xi = (int *)malloc(sizeof(int) * (size_t)cw), *yi = (int *)malloc(sizeof(int) * (size_t)ch);
+    float *xf = (float *)malloc(sizeof(float) * (size_t)cw), *yf = (float *)malloc(sizeof(float) * (size_t)ch);
+    for (int x = 0; x < cw; ++x) xp[x] = (float)((x + 0.5) * rw - 0.5 + sx);
+    for (int j = 0; j < ch; ++j) yp[j] = (float)((j + 0.5) * rh - 0.5 + sy);
+    or_bilin_axis(xp, cw, w, xi, xf);
+    or_bilin_axis(yp, ch, h, yi, yf);
+    for (int j = 0; j < ch; ++j)
+        for (int x = 0; x < cw; ++x)
+            ls[(size_t)j * cw + x] = bilinear_fast(y, w, h, xi[x], xf[x], yi[j], yf[j]);
+    free(xp); free(yp); free(xi); free(yi); free(xf); free(yf);
+}
+
+/* slidingMinMax / minMax5 (algos.cpp:70-139, 241-298): min and max over
+ * the in-bounds (2r+1)^2 window */
+static void window_minmax(const float *src, int w, int h, int r, float *mn, float *mx) {
+    float *hmn = (float *)malloc(sizeof(float) * (size_t)w * h), *hmx = (float *)malloc(sizeof(float) * (size_t)w * h);
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            float lo = src[(size_t)y * w + maxi(0, x - r)], hi = lo;
+            for (int i = maxi(0, x - r); i <= mini(w - 1, x + r); ++i) {
+                lo = minf(lo, src[(size_t)y * w + i]);
+                hi = maxf(hi, src[(size_t)y * w + i]);
+            }
+            hmn[(size_t)y * w + x] = lo;
+            hmx[(size_t)y * w + x] = hi;
+        }
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            float lo = hmn[(size_t)maxi(0, y - r) * w + x], hi = hmx[(size_t)maxi(0, y - r) * w + x];
+            for (int j = maxi(0, y - r); j <= mini(h - 1, y + r); ++j) {
+                lo = minf(lo, hmn[(size_t)j * w + x]);
+                hi = maxf(hi, hmx[(size_t)j * w + x]);
+            }
+            mn[(size_t)y * w + x] = lo;
+            mx[(size_t)y * w + x] = hi;
+        }
+    free(hmn); free(hmx);
+}
+
+typedef struct { float *au, *bu, *cu, *av, *bv, *cv; } lgf_maps;
+
+/* buildLGFPair (algos.cpp:366-448) with the rolling double window sums of
+ * forEachWindowSum<6> (algos.cpp:12-68); roi == NULL or a dense ROI
+ * finalises every sample, else only ROI samples (the rest stay 0). */
+static void lgf_pair(const float *y, int w, int h, int cw, int ch, double rw, double rh,
+                     double sx, double sy, const float *U, const float *V, int r, double eps,
+                     int cedge, const uint8_t *roi, lgf_maps *m) {
+    const size_t nc = (size_t)cw * ch;
+    float *ls = (float *)malloc(sizeof(float) * nc);
+    point_luma(y, w, h, cw, ch, rw, rh, sx, sy, ls);
+    float *mnu = NULL, *mxu = NULL, *mnv = NULL, *mxv = NULL;
+    if (cedge) {
+        mnu = (float *)malloc(sizeof(float) * nc); mxu = (float *)malloc(sizeof(float) * nc);
+        mnv = (float *)malloc(sizeof(float) * nc); mxv = (float *)malloc(sizeof(float) * nc);
+        window_minmax(U, cw, ch, r, mnu, mxu);
+        window_minmax(V, cw, ch, r, mnv, mxv);
+    }
+    int dense = !roi;
+    if (roi) {
+        size_t a = 0;
+        for (size_t i = 0; i < nc; ++i) a += roi[i] != 0;
+        dense = a * 2 >= nc;
+    }
+    if (roi)
+        for (size_t i = 0; i < nc; ++i)
+            m->au[i] = m->bu[i] = m->cu[i] = m->av[i] = m->bv[i] = m->cv[i] = 0.0f;
+    enum { N = 6 };
+    double *cols = (double *)calloc((size_t)cw * N, sizeof(double));
+    const int rows = maxi(1, 2 * r + 2);
+    double *cache = (double *)calloc((size_t)rows * cw * N, sizeof(double));
+    int top_cur = 0, bot_cur = -1;
+    for (int yy = 0; yy < ch; ++yy) {
+        const int top = maxi(0, yy - r), bottom = mini(ch - 1, yy + r);
+        while (bot_cur < bottom) {
+            ++bot_cur;
+            double *c = cache + (size_t)(bot_cur % rows) * cw * N;
+            for (int x = 0; x < cw; ++x) {
+                const double l = ls[(size_t)bot_cur * cw + x];
+                const double u = U[(size_t)bot_cur * cw + x], v = V[(size_t)bot_cur * cw + x];
+                double *val = c + (size_t)x * N;
+                val[0] = l; val[1] = l * l; val[2] = u; val[3] = l * u; val[4] = v; val[5] = l * v;
+                for (int k = 0; k < N; ++k) cols[(size_t)x * N + k] += val[k];
+            }
+        }
+        while (top_cur < top) {
+            const double *c = cache + (size_t)(top_cur % rows) * cw * N;
+            for (int x = 0; x < cw; ++x)
+                for (int k = 0; k < N; ++k) cols[(size_t)x * N + k] -= c[(size_t)x * N + k];
+            ++top_cur;
+        }
+        double tot[N] = {0, 0, 0, 0, 0, 0};
+        for (int x = 0; x <= mini(cw - 1, r); ++x)
+            for (int k = 0; k < N; ++k) tot[k] += cols[(size_t)x * N + k];
+        for (int x = 0; x < cw; ++x) {
+            const int left = maxi(0, x - r), right = mini(cw - 1, x + r);
+            const int n = (right - left + 1) * (bottom - top + 1);
+            const size_t i = (size_t)yy * cw + x;
+            if (dense || roi[i]) {
+                const double mean_y = tot[0] / n;
+                const double var = tot[1] / n - mean_y * mean_y;
+                const double mean_u = tot[2] / n, mean_v = tot[4] / n;
+                const double su = (tot[3] / n - mean_y * mean_u) / (var + eps);
+                const double sv = (tot[5] / n - mean_y * mean_v) / (var + eps);
+                m->au[i] = (float)su;
+                m->bu[i] = (float)(mean_u - su * mean_y);
+                m->av[i] = (float)sv;
+                m->bv[i] = (float)(mean_v - sv * mean_y);
+                const float base = (float)(var / (var + eps));
+                float fu = 1.0f, fv = 1.0f;
+                if (cedge) {
+                    const float lgx = (at(ls, cw, ch, x + 1, yy) - at(ls, cw, ch, x - 1, yy)) * 0.5f;
+                    const float lgy = (at(ls, cw, ch, x, yy + 1) - at(ls, cw, ch, x, yy - 1)) * 0.5f;
+                    const float gl = hypotf(lgx, lgy);
+                    for (int c = 0; c < 2; ++c) {
+                        const float *P = c ? V : U;
+                        const float range = (c ? mxv : mxu)[i] - (c ? mnv : mnu)[i];
+                        float f = 1.0f;
+                        if (!(gl <= 1e-9f || range <= 1e-6f)) {
+                            const float cgx = (at(P, cw, ch, x + 1, yy) - at(P, cw, ch, x - 1, yy)) * 0.5f;
+                            const float cgy = (at(P, cw, ch, x, yy + 1) - at(P, cw, ch, x, yy - 1)) * 0.5f;
+                            const float gn = fabsf(cgx * lgx / gl + cgy * lgy / gl);
+                            const float width = range / (gn + 1e-6f);
+                            f = clampf((2.2f - width) / 0.7f, 0.0f, 1.0f);
+                        }
+                        if (c) fv = f; else fu = f;
+                    }
+                }
+                m->cu[i] = base * fu;
+                m->cv[i] = base * fv;
+            }
+            if (x - r >= 0)
+                for (int k = 0; k < N; ++k) tot[k] -= cols[(size_t)(x - r) * N + k];
+            if (x + r + 1 < cw)
+                for (int k = 0; k < N; ++k) tot[k] += cols[(size_t)(x + r + 1) * N + k];
+        }
+    }
+    free(cols); free(cache); free(ls);
+    free(mnu); free(mxu); free(mnv); free(mxv);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -990,6 +1389,132 @@ void or_back_project(float *out, int ow, int oh, const float *src, int cw, int c
             out[(size_t)yb * ow + xa] += d;
             out[(size_t)yb * ow + xb] += d;
         }
+}
+
+
+void or_lgf_maps(const float *y, int w, int h, int cw, int ch, double rw, double rh, double sx,
+                 double sy, const float *u, const float *v, double eps, int cedge,
+                 const uint8_t *roi, float *out6) {
+    const size_t nc = (size_t)cw * ch;
+    lgf_maps m = { out6, out6 + nc, out6 + 2 * nc, out6 + 3 * nc, out6 + 4 * nc, out6 + 5 * nc };
+    lgf_pair(y, w, h, cw, ch, rw, rh, sx, sy, u, v, 2, eps, cedge, roi, &m);
+}
+
+/* algo 4 orchestration (plugin.cpp:396-572) on a plain-filled output */
+static void recon_algo4(const or_params *p, or_debug *D, const or_axis *ax, const or_axis *ay,
+                        int is_float, float *out_u, float *out_v) {
+    const int cw = D->cw, ch = D->ch, ow = D->ow, oh = D->oh;
+    const size_t nc = (size_t)cw * ch, no = (size_t)ow * oh;
+    const double rw = (double)D->sw / cw, rh = (double)D->sh / ch;
+    or_params p2 = *p;
+    p2.algo = 2;
+    p2.strength = 1.0;
+    lgf_maps m;
+    float *lgf = (float *)malloc(sizeof(float) * nc * 6);
+    m.au = lgf; m.bu = lgf + nc; m.cu = lgf + 2 * nc; m.av = lgf + 3 * nc; m.bv = lgf + 4 * nc; m.cv = lgf + 5 * nc;
+    or_guide g = { D->y, D->sw, D->sh, D->jxx, D->jxy, D->jyy, D->lc, D->ms };
+    const or_workset *ws = D->has_ws ? &D->ws : NULL;
+    const float lam = (float)p->strength;
+    if (!is_float) {
+        lgf_pair(D->y, D->sw, D->sh, cw, ch, rw, rh, p->shift_x, p->shift_y, D->u, D->v, 2,
+                 p->reg * p->reg, p->cedge, NULL, &m);
+        if (D->ms)
+            for (size_t i = 0; i < nc; ++i) {
+                const float f = 1.0f - (float)p->ms * (1.0f - D->ms[i]);
+                m.cu[i] *= f;
+                m.cv[i] *= f;
+            }
+        selector_cfg sel;
+        memset(&sel, 0, sizeof(sel));
+        sel.mode = SEL_MAPS;
+        sel.strength = lam;
+        sel.au = m.au; sel.bu = m.bu; sel.cu = m.cu; sel.av = m.av; sel.bv = m.bv; sel.cv = m.cv;
+        guided_run(&p2, D->u, D->v, cw, ch, &g, ws, &sel, out_u, out_v);
+        free(lgf);
+        return;
+    }
+    const int compressed = ws && !or_workset_output_dense(ws);
+    selector_cfg sel;
+    memset(&sel, 0, sizeof(sel));
+    sel.strength = lam;
+    float *gu = NULL, *gv = NULL;
+    if (compressed) {
+        sel.mode = SEL_COMPRESSED;
+        sel.plain_u = out_u; sel.plain_v = out_v;
+        sel.du = (float *)malloc(sizeof(float) * (ws->n_indices + 1));
+        sel.dv = (float *)malloc(sizeof(float) * (ws->n_indices + 1));
+        sel.w3c = (float *)malloc(sizeof(float) * (ws->n_indices + 1));
+        guided_run(&p2, D->u, D->v, cw, ch, &g, ws, &sel, out_u, out_v);
+    } else {
+        sel.mode = SEL_PLANES;
+        gu = (float *)calloc(no, sizeof(float)); gv = (float *)calloc(no, sizeof(float));
+        sel.w2p = (float *)calloc(no, sizeof(float)); sel.w3p = (float *)calloc(no, sizeof(float));
+        guided_run(&p2, D->u, D->v, cw, ch, &g, ws, &sel, gu, gv);
+    }
+    /* ROI of the diagonal route (plugin.cpp:496-531) */
+    uint8_t *roi = (uint8_t *)calloc(nc, 1);
+    const size_t total = ws ? ws->n_indices : no;
+    for (size_t k = 0; k < total; ++k) {
+        const uint32_t idx = ws ? ws->indices[k] : (uint32_t)k;
+        const int ox = (int)(idx % (uint32_t)ow), oy = (int)(idx / (uint32_t)ow);
+        const float w3 = compressed ? sel.w3c[k] : sel.w3p[idx];
+        if (w3 < 1e-4f)
+            continue;
+        const int cx0 = ax->cb_i0[ox], cx1 = mini(cx0 + 1, cw - 1);
+        const int cy0 = ay->cb_i0[oy], cy1 = mini(cy0 + 1, ch - 1);
+        roi[(size_t)cy0 * cw + cx0] = roi[(size_t)cy0 * cw + cx1] = 1;
+        roi[(size_t)cy1 * cw + cx0] = roi[(size_t)cy1 * cw + cx1] = 1;
+    }
+    lgf_pair(D->y, D->sw, D->sh, cw, ch, rw, rh, p->shift_x, p->shift_y, D->u, D->v, 2,
+             p->reg * p->reg, p->cedge, roi, &m);
+    if (D->ms)
+        for (size_t i = 0; i < nc; ++i) {
+            if (!roi[i])
+                continue;
+            const float f = 1.0f - (float)p->ms * (1.0f - D->ms[i]);
+            m.cu[i] *= f;
+            m.cv[i] *= f;
+        }
+    /* blend (blendSelectorCompressed / blendSelector, algos.cpp:453-593) */
+    for (size_t k = 0; k < total; ++k) {
+        const uint32_t idx = ws ? ws->indices[k] : (uint32_t)k;
+        const int ox = (int)(idx % (uint32_t)ow), oy = (int)(idx / (uint32_t)ow);
+        const int cx = ax->cb_i0[ox], cy = ay->cb_i0[oy];
+        const float cxf = ax->cb_f[ox], cyf = ay->cb_f[oy];
+        if (compressed) {
+            const float pu = out_u[idx], pv = out_v[idx];
+            const float w3 = lam * sel.w3c[k];
+            float vu = pu + lam * sel.du[k], vv = pv + lam * sel.dv[k];
+            if (w3 != 0.0f) {
+                const float luma = bilinear_fast(D->y, D->sw, D->sh, ax->lb_i0[ox], ax->lb_f[ox], ay->lb_i0[oy], ay->lb_f[oy]);
+                const float au = bilinear_fast(m.au, cw, ch, cx, cxf, cy, cyf);
+                const float bu = bilinear_fast(m.bu, cw, ch, cx, cxf, cy, cyf);
+                const float av = bilinear_fast(m.av, cw, ch, cx, cxf, cy, cyf);
+                const float bv = bilinear_fast(m.bv, cw, ch, cx, cxf, cy, cyf);
+                const float cu = bilinear_fast(m.cu, cw, ch, cx, cxf, cy, cyf);
+                const float cv = bilinear_fast(m.cv, cw, ch, cx, cxf, cy, cyf);
+                vu += w3 * cu * (au * luma + bu - pu);
+                vv += w3 * cv * (av * luma + bv - pv);
+            }
+            out_u[idx] = vu;
+            out_v[idx] = vv;
+        } else {
+            const float w2v = sel.w2p[idx] * lam, w3v = sel.w3p[idx] * lam;
+            if (w2v < 1e-4f && w3v < 1e-4f)
+                continue;
+            const float L0 = bilinear_fast(D->y, D->sw, D->sh, ax->lb_i0[ox], ax->lb_f[ox], ay->lb_i0[oy], ay->lb_f[oy]);
+            const float au = bilinear_fast(m.au, cw, ch, cx, cxf, cy, cyf);
+            const float bu = bilinear_fast(m.bu, cw, ch, cx, cxf, cy, cyf);
+            const float av = bilinear_fast(m.av, cw, ch, cx, cxf, cy, cyf);
+            const float bv = bilinear_fast(m.bv, cw, ch, cx, cxf, cy, cyf);
+            const float cu = bilinear_fast(m.cu, cw, ch, cx, cxf, cy, cyf);
+            const float cv = bilinear_fast(m.cv, cw, ch, cx, cxf, cy, cyf);
+            out_u[idx] += w2v * (gu[idx] - out_u[idx]) + w3v * cu * (au * L0 + bu - out_u[idx]);
+            out_v[idx] += w2v * (gv[idx] - out_v[idx]) + w3v * cv * (av * L0 + bv - out_v[idx]);
+        }
+    }
+    free(roi); free(lgf); free(gu); free(gv);
+    free(sel.du); free(sel.dv); free(sel.w3c); free(sel.w2p); free(sel.w3p);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1106,8 +1631,8 @@ int or_recon_frame(const or_params *pin, const or_frame_format *f,
     if (f->ss_h == 0)
         pl.shift_y = 0.0;
     const or_params *p = &pl;
-    if (p->algo != 2 && p->strength > 0.0)
-        return -2; /* algo 4/6 restated separately */
+    if (p->algo == 6 && p->strength > 0.0)
+        return -2; /* algo 6 restated separately */
     if (p->bm)
         return -2;
     const int sw = f->width, sh = f->height;
@@ -1184,7 +1709,11 @@ int or_recon_frame(const or_params *pin, const or_frame_format *f,
 
     D->out_u = (float *)malloc(sizeof(float) * no);
     D->out_v = (float *)malloc(sizeof(float) * no);
-    if (p->strength == 0.0) {
+    if (p->algo == 4) {
+        or_plain_chroma(p, D->u, D->v, cw, ch, sw, sh, D->out_u, D->out_v);
+        if (p->strength > 0.0)
+            recon_algo4(p, D, &ax, &ay, is_float, D->out_u, D->out_v);
+    } else if (p->strength == 0.0) {
         or_plain_chroma(p, D->u, D->v, cw, ch, sw, sh, D->out_u, D->out_v);
     } else {
         if (D->has_ws) {

@@ -131,6 +131,12 @@ typedef struct or_guide {
 void or_guided(const or_params *p, const float *u, const float *v, int cw, int ch,
                const or_guide *g, const or_workset *ws, float *out_u, float *out_v);
 
+/* LGF regression maps, radius 2 (buildLGFPair, algos.cpp:366-448); out6 =
+ * aU, bU, confU, aV, bV, confV planes of cw*ch.  roi may be NULL. */
+void or_lgf_maps(const float *y, int w, int h, int cw, int ch, double rw, double rh, double sx,
+                 double sy, const float *u, const float *v, double eps, int cedge,
+                 const uint8_t *roi, float *out6);
+
 /* back-projection (algos.cpp:663-677) */
 void or_back_project(float *out, int ow, int oh, const float *src, int cw, int ch, float bp);
 

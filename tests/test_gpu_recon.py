@@ -86,13 +86,32 @@ CASES = [
     ("p8-420-downscale", 96, 64, 1, 1, 8, {"width": 64, "height": 40}),
     ("ps-420-upscale-l4", 96, 64, 1, 1, 32, {"width": 192, "height": 128, "kernel": "lanczos", "taps": 4}),
     ("p10-422-upscale-bicubic", 96, 64, 1, 0, 10, {"width": 150, "height": 100, "kernel": "bicubic"}),
+    # radial (jinc) base and guided weights
+    ("p8-420-jinc", 96, 64, 1, 1, 8, {"kernel": "jinc"}),
+    ("ps-420-jinc-strength0", 96, 64, 1, 1, 32, {"kernel": "jinc", "strength": 0.0}),
+    ("p8-420-jinc4-dense", 96, 64, 1, 1, 8, {"kernel": "jinc", "taps": 4, "sparse": 0}),
+    ("p8-420-jinc-upscale", 96, 64, 1, 1, 8, {"kernel": "jinc", "width": 192, "height": 128}),
+    ("ps-444-jinc-downscale", 96, 64, 0, 0, 32, {"kernel": "jinc", "width": 60, "height": 40}),
+    # algo 4 selector: integer (single expression) and float (metadata + blend)
+    ("p8-420-algo4", 96, 64, 1, 1, 8, {"algo": 4}),
+    ("p10-420-algo4-bp", 96, 64, 1, 1, 10, {"algo": 4, "bp": 0.5}),
+    ("p8-420-algo4-cedge", 96, 64, 1, 1, 8, {"algo": 4, "cedge": 1}),
+    ("ps-420-algo4", 96, 64, 1, 1, 32, {"algo": 4}),
+    ("ps-420-algo4-dense", 96, 64, 1, 1, 32, {"algo": 4, "sparse": 0}),
+    ("ps-420-algo4-cedge", 96, 64, 1, 1, 32, {"algo": 4, "cedge": 1}),
+    ("ps-420-algo4-compressed", 192, 96, 1, 1, 32, {"algo": 4, "poly": 2}),
+    ("p8-420-algo4-compressed", 192, 96, 1, 1, 8, {"algo": 4, "poly": 2}),
+    ("ps-422-algo4-jinc", 96, 64, 1, 0, 32, {"algo": 4, "kernel": "jinc"}),
+    ("p8-420-algo4-upscale", 96, 64, 1, 1, 8, {"algo": 4, "width": 144, "height": 96}),
 ]
 
 
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
 def test_recon_matrix(device, case):
     tag, w, h, ssw, ssh, bits, params = case
-    y, u, v = frame(w, h, ss_w=ssw, ss_h=ssh, bits=bits, seed=7, polygons=12)
+    params = dict(params)
+    poly = params.pop("poly", 12)
+    y, u, v = frame(w, h, ss_w=ssw, ss_h=ssh, bits=bits, seed=7, polygons=poly)
     o_out, _, g_out, clip = run_both(y, u, v, ss_w=ssw, ss_h=ssh, bits=None if bits in (8, 16, 32) else bits,
                                      device=device, **params)
     clip.close()
@@ -111,6 +130,22 @@ def test_config2_frame_1080p_p10(device):
         print(f"1080p out_u float mismatches={n_ou} max_abs={mx:g}")
         assert n_ou == 0
         assert _report("1080p-p10", o_out, g_out, False) == 0
+    finally:
+        clip.close()
+
+
+@pytest.mark.parametrize("algo", [2, 4])
+def test_config2_algo_1080p_p10_float_stage(device, algo):
+    """1080p YUV420P10 with bp 0.5 for algo 2 and 4: float planes before
+    packing within 1e-5, packed planes within 1 code."""
+    y, u, v = frame(1920, 1080, bits=10, seed=1, frame=3)
+    o_out, dbg, g_out, clip = run_both(y, u, v, bits=10, device=device, bp=0.5, algo=algo)
+    try:
+        for name in ("out_u", "out_v"):
+            n, mx = compare_planes(dbg[name], clip.fetch(name), True)
+            print(f"1080p algo{algo} {name}: mismatches={n} max_abs={mx:g}")
+            assert n == 0
+        assert _report(f"1080p-algo{algo}", o_out, g_out, False) == 0
     finally:
         clip.close()
 

@@ -402,3 +402,58 @@ def test_integer_output_rounding():
     f = np.float32
     exp = [max(0, min(1023, int(f(s) * f(1023.0) + f(511.5) + f(0.5)))) for s in src[0]]
     assert list(out[0]) == exp
+
+
+# --------------------------------------------------------------------------
+# algo 4 (LGF regression + selector)
+# --------------------------------------------------------------------------
+
+def test_lgf_recovers_affine_chroma():
+    # chroma = a * Y + b exactly -> slope a, intercept b wherever the window
+    # has luma variance well above eps (closed form of the regression)
+    rng = np.random.default_rng(12)
+    y = rng.random((32, 48), dtype=np.float32)
+    ls = np.zeros((16, 24), np.float32)
+    # point-sampled luma at 4:4:4 == Y itself: use rw = rh = 1 and no shift
+    u = (np.float32(0.3) * y - np.float32(0.1)).astype(np.float32)
+    v = (np.float32(-0.2) * y + np.float32(0.05)).astype(np.float32)
+    out = np.zeros((6, 32, 48), np.float32)
+    L.or_lgf_maps(oracle.fptr(y), 48, 32, 48, 32, 1.0, 1.0, 0.0, 0.0, oracle.fptr(u), oracle.fptr(v), 1e-12, 0, None,
+                  oracle.fptr(out))
+    assert np.max(np.abs(out[0] - 0.3)) < 1e-4 and np.max(np.abs(out[1] + 0.1)) < 1e-4
+    assert np.max(np.abs(out[3] + 0.2)) < 1e-4 and np.max(np.abs(out[4] - 0.05)) < 1e-4
+    assert np.all(out[2] > 0.999) and np.all(out[5] > 0.999)
+
+
+def test_lgf_roi_only_changes_unselected_samples():
+    rng = np.random.default_rng(13)
+    y = rng.random((40, 64), dtype=np.float32)
+    u = rng.random((20, 32), dtype=np.float32) - 0.5
+    v = rng.random((20, 32), dtype=np.float32) - 0.5
+    full = np.zeros((6, 20, 32), np.float32)
+    part = np.zeros((6, 20, 32), np.float32)
+    roi = np.zeros((20, 32), np.uint8)
+    roi[5:9, 10:20] = 1
+    args = (oracle.fptr(y), 64, 40, 32, 20, 2.0, 2.0, -0.5, 0.0, oracle.fptr(u), oracle.fptr(v), 2.5e-5, 1)
+    L.or_lgf_maps(*args, None, oracle.fptr(full))
+    L.or_lgf_maps(*args, oracle.u8ptr(roi), oracle.fptr(part))
+    on = roi.astype(bool)
+    assert np.array_equal(full[:, on], part[:, on])
+    assert np.all(part[:, ~on] == 0)
+
+
+@pytest.mark.parametrize("bits", [8, 32])
+def test_algo4_sparse_equals_dense(bits):
+    y, u, v = frame(96, 64, bits=bits, seed=14, polygons=10)
+    dense, _ = _recon(y, u, v, algo=4, sparse=0)
+    sparse, _ = _recon(y, u, v, algo=4, sparse=1)
+    tol = 1e-6 if bits == 32 else 0
+    assert max(float(np.max(np.abs(dense[p].astype(np.float64) - sparse[p].astype(np.float64)))) for p in (1, 2)) <= tol
+
+
+def test_algo4_constant_chroma_preserved():
+    y, _, _ = frame(64, 48, bits=32, seed=15)
+    u = np.full((24, 32), np.float32(0.1))
+    v = np.full((24, 32), np.float32(-0.1))
+    out, _ = _recon(y, u, v, algo=4)
+    assert np.max(np.abs(out[1] - np.float32(0.1))) <= 1e-6 and np.max(np.abs(out[2] + np.float32(0.1))) <= 1e-6

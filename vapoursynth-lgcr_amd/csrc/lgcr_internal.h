@@ -33,12 +33,34 @@ struct WeightsDev {
     const float *w = nullptr;
 };
 
+// Radial (jinc) base-weight phase table (RadialWeightTable,
+// src/lgcr.h:793-811).  valid == 0 means the kernels evaluate the LUT per tap.
+struct RadialDev {
+    int valid = 0, sup_x = 0, sup_y = 0, nx = 0, ny = 0;
+    const uint16_t *xphase = nullptr, *yphase = nullptr;
+    const float *w = nullptr;
+    const float *lut = nullptr;
+    int lut_n = 0;
+    float lut_scale = 1024.0f;
+    float inv_wx = 1.0f, inv_wy = 1.0f;  // per-axis downscale AA factors
+};
+
 // Scalar parameters the kernels read, already narrowed exactly as the
 // reference narrows them (float(d.sigma) etc.).
 struct GuideParams {
     float sigma, sratio, sdb, strength, rescue, stretch, inv_gsigma2, ms, margin, sp_out;
     float frw, frh;
     int ridge, cedge, clamp_hull, ms_on, radial;
+};
+
+// algo-4 selector state of the guided pass (recon.cpp:762-811).
+enum SelectorMode : int { SEL_NONE = 0, SEL_MAPS = 1, SEL_FLOAT = 2 };
+struct SelectorDev {
+    int mode = SEL_NONE;
+    float strength = 0.0f;          // selectorStrength
+    const float *lgf = nullptr;     // aU, bU, confU, aV, bV, confV planes (chroma grid)
+    float *du = nullptr, *dv = nullptr, *w3c = nullptr;   // compressed, workset order
+    float *gu = nullptr, *gv = nullptr, *w2p = nullptr, *w3p = nullptr;  // planes, output grid
 };
 
 // Sample formats of raw planes.
@@ -85,12 +107,30 @@ hipError_t launch_plain_separable(const float *u, const float *v, int cw, int ch
 hipError_t launch_plain_bilinear(const float *u, const float *v, int cw, int ch,
                                  const AxisDev &ax, const AxisDev &ay, float *out_u,
                                  float *out_v, hipStream_t s);
+hipError_t launch_plain_radial(const float *u, const float *v, int cw, int ch,
+                               const AxisDev &ax, const AxisDev &ay, const RadialDev &rd,
+                               float *out_u, float *out_v, int clamp_hull, float margin,
+                               hipStream_t s);
 hipError_t launch_guided(const float *u, const float *v, int cw, int ch, const float *y,
                          int sw, int sh, const float *jxx, const float *jxy, const float *jyy,
                          const float *lc, const float *ms, const AxisDev &ax, const AxisDev &ay,
-                         const GuideParams &gp, const uint32_t *row_offsets,
+                         const RadialDev &rd, const GuideParams &gp, const uint32_t *row_offsets,
                          const uint32_t *indices, float *out_u, float *out_v,
-                         hipStream_t s);
+                         const SelectorDev &sel, hipStream_t s);
+// algo 4 (algos.cpp:241-593)
+hipError_t launch_point_luma(const float *y, int w, int h, float *ls, int cw, int ch,
+                             const int *xi, const float *xf, const int *yi, const float *yf,
+                             hipStream_t s);
+hipError_t launch_minmax5(const float *u, const float *v, int cw, int ch, float *mm, hipStream_t s);
+hipError_t launch_lgf(const float *ls, const float *u, const float *v, int cw, int ch, double eps,
+                      int cedge, const float *mm, const uint8_t *roi, const uint32_t *roi_count,
+                      const float *gate, float ms, float *lgf, hipStream_t s);
+hipError_t launch_roi_mark(const uint32_t *row_offsets, const uint32_t *indices, int ow, int oh,
+                           const SelectorDev &sel, const AxisDev &ax, const AxisDev &ay,
+                           uint8_t *roi, int cw, int ch, uint32_t *roi_count, hipStream_t s);
+hipError_t launch_blend4(const uint32_t *row_offsets, const uint32_t *indices, const float *y,
+                         int sw, int sh, const AxisDev &ax, const AxisDev &ay, int cw, int ch,
+                         const SelectorDev &sel, float *out_u, float *out_v, hipStream_t s);
 hipError_t launch_back_project(float *out_u, float *out_v, int ow, int oh, const float *u,
                                const float *v, int cw, int ch, float bp, hipStream_t s);
 hipError_t launch_resample_h(const float *src, int sw, int sh, float *dst,

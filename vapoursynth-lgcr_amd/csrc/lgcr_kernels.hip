@@ -574,6 +574,79 @@ __global__ __launch_bounds__(kBlock) void k_plain_bilinear(const float *u, const
     out_v[o] = bilinear_fast(v, cw, ch, ax.cb_i0[ox], ax.cb_f[ox], ay.cb_i0[oy], ay.cb_f[oy]);
 }
 
+__device__ __forceinline__ float lut_lookup(const float *lut, int n, float idx) {
+    int i0 = (int)idx;
+    i0 = clampi(i0, 0, n - 2);
+    const float fr = fmn(idx - (float)i0, 1.0f);
+    return lut[i0] + (lut[i0 + 1] - lut[i0]) * fr;
+}
+
+__device__ __forceinline__ const float *radial_at(const RadialDev &r, int x, int y) {
+    if (!r.valid)
+        return nullptr;
+    return r.w + ((size_t)r.yphase[y] * r.nx + r.xphase[x]) * (size_t)r.sup_x * r.sup_y;
+}
+
+// Radial (jinc) base weight of tap (i, j) of output (ox, oy): the phase
+// table entry when the table exists, else the LUT at the AA-scaled distance
+// (recon.cpp:598-606, 984-991).
+__device__ __forceinline__ float radial_weight(const RadialDev &r, const float *base, int sup_x, int i, int j,
+                                               int tx0, int ty0, float scx, float scy) {
+    if (base)
+        return base[(size_t)j * sup_x + i];
+    const float dx = ((float)(tx0 + i) - scx) * r.inv_wx;
+    const float dy = ((float)(ty0 + j) - scy) * r.inv_wy;
+    return lut_lookup(r.lut, r.lut_n, sqrtf(dx * dx + dy * dy) * r.lut_scale);
+}
+
+// plainChromaImpl for the radial kernel (recon.cpp:828-1029)
+__global__ __launch_bounds__(kBlock) void k_plain_radial(const float *__restrict__ u, const float *__restrict__ v,
+                                                         int cw, int ch, AxisDev ax, AxisDev ay, RadialDev rd,
+                                                         float *out_u, float *out_v, int clamp_hull, float margin) {
+    const int ox = blockIdx.x * kBlock + threadIdx.x, oy = blockIdx.y;
+    if (ox >= ax.n)
+        return;
+    const int tx0 = ax.start[ox], ty0 = ay.start[oy];
+    const float *amx = ax.am + (size_t)ox * ax.sup, *amy = ay.am + (size_t)oy * ay.sup;
+    const float scx = ax.pos[ox], scy = ay.pos[oy];
+    const float *base = radial_at(rd, ox, oy);
+    double au = 0.0, av = 0.0, ws = 0.0, sa = 0.0;
+    float mnu = 1e30f, mxu = -1e30f, mnv = 1e30f, mxv = -1e30f;
+    for (int j = 0; j < ay.sup; ++j) {
+        if (amy[j] == 0.0f)
+            continue;
+        const size_t row = (size_t)clampi(ty0 + j, 0, ch - 1) * cw;
+        for (int i = 0; i < ax.sup; ++i) {
+            if (amx[i] == 0.0f)
+                continue;
+            const float w = radial_weight(rd, base, ax.sup, i, j, tx0, ty0, scx, scy);
+            const size_t k = row + clampi(tx0 + i, 0, cw - 1);
+            const float a = u[k], b = v[k];
+            au += (double)w * a;
+            ws += w;
+            sa += fabsf(w);
+            mnu = fmn(mnu, a); mxu = fmx(mxu, a);
+            av += (double)w * b;
+            mnv = fmn(mnv, b); mxv = fmx(mxv, b);
+        }
+    }
+    double vu, vv;
+    if (ws > 1e-6 && ws >= 0.05 * sa) {
+        vu = au / ws;
+        vv = av / ws;
+    } else {
+        vu = bilinear(u, cw, ch, scx, scy);
+        vv = bilinear(v, cw, ch, scx, scy);
+    }
+    if (clamp_hull) {
+        vu = dclamp(vu, (double)(mnu - margin), (double)(mxu + margin));
+        vv = dclamp(vv, (double)(mnv - margin), (double)(mxv + margin));
+    }
+    const size_t o = (size_t)oy * ax.n + ox;
+    out_u[o] = (float)vu;
+    out_v[o] = (float)vv;
+}
+
 // ---------------------------------------------------------------------------
 // guided reconstruction, algo 2 (reconstructChromaImpl, recon.cpp:14-819):
 // one thread per active output pixel of the compacted workset.
@@ -586,13 +659,21 @@ struct GuidedArgs {
     int sw, sh;
     const float *jxx, *jxy, *jyy, *lc, *ms;
     AxisDev ax, ay;
+    RadialDev rd;
     GuideParams gp;
     const uint32_t *row_off;
     const uint32_t *indices;
     float *out_u, *out_v;
+    SelectorDev sel;
 };
 
-__device__ void guided_pixel(const GuidedArgs &A, int ox, int oy) {
+// True when the float selector keeps compressed per-active arrays: a sparse
+// workset that is not in dense fallback (plugin.cpp:455-456).
+__device__ __forceinline__ bool sel_compressed(const uint32_t *row_off, const uint32_t *indices, int ow, int oh) {
+    return indices && (uint64_t)row_off[oh] * 2 < (uint64_t)ow * (uint64_t)oh;
+}
+
+__device__ void guided_pixel(const GuidedArgs &A, int ox, int oy, uint32_t k) {
     const AxisDev &ax = A.ax, &ay = A.ay;
     const GuideParams &gp = A.gp;
     const int cw = A.cw, ch = A.ch, sw = A.sw, sh = A.sh;
@@ -710,19 +791,39 @@ __device__ void guided_pixel(const GuidedArgs &A, int ox, int oy) {
         rescue = gp.rescue * guide_fade * phase * phase * (1.0f - sim0);
         if (rescue > 0.0f) {
             float mxX = 0, ssX = 0, mxY = 0, ssY = 0;
+            if (!gp.radial)
             for (int i = 0; i < ax.sup; ++i)
                 if (amx[i] > 0.0f) {
                     const float w = wxp[i];
                     mxX = fmx(mxX, fabsf(w));
                     ssX += w * w;
                 }
+            if (!gp.radial)
             for (int j = 0; j < ay.sup; ++j)
                 if (amy[j] > 0.0f) {
                     const float w = wyp[j];
                     mxY = fmx(mxY, fabsf(w));
                     ssY += w * w;
                 }
-            const float kappa = (mxX * mxX * mxY * mxY) / (ssX * ssY + 1e-12f);
+            float kappa;
+            if (!gp.radial) {
+                kappa = (mxX * mxX * mxY * mxY) / (ssX * ssY + 1e-12f);
+            } else {
+                const float *base = radial_at(A.rd, ox, oy);
+                float mx = 0, ss = 0;
+                for (int j = 0; j < ay.sup; ++j) {
+                    if (amy[j] == 0.0f)
+                        continue;
+                    for (int i = 0; i < ax.sup; ++i) {
+                        if (amx[i] == 0.0f)
+                            continue;
+                        const float w = radial_weight(A.rd, base, ax.sup, i, j, tx0, ty0, scx, scy);
+                        mx = fmx(mx, fabsf(w));
+                        ss += w * w;
+                    }
+                }
+                kappa = (mx * mx) / (ss + 1e-12f);
+            }
             rescue *= kappa * kappa;
         }
     }
@@ -731,6 +832,7 @@ __device__ void guided_pixel(const GuidedArgs &A, int ox, int oy) {
     const float inv_g2p = inv_g2 / (aniso * aniso);
 
     // --- tap loop (recon.cpp:581-627)
+    const float *rbase = gp.radial ? radial_at(A.rd, ox, oy) : nullptr;
     double acc_u = 0.0, acc_v = 0.0, wsum = 0.0, sum_abs = 0.0;
     for (int j = 0; j < ay.sup; ++j) {
         if (amy[j] == 0.0f)
@@ -742,7 +844,7 @@ __device__ void guided_pixel(const GuidedArgs &A, int ox, int oy) {
             if (amx[i] == 0.0f)
                 continue;
             const size_t k = row + clampi(tx0 + i, 0, cw - 1);
-            const float wx = wxp[i] * wy;
+            const float wx = gp.radial ? radial_weight(A.rd, rbase, ax.sup, i, j, tx0, ty0, scx, scy) : wxp[i] * wy;
             float w;
             if (guided) {
                 const float dx = ((float)(tx0 + i) - scx) * gp.frw;
@@ -775,6 +877,51 @@ __device__ void guided_pixel(const GuidedArgs &A, int ox, int oy) {
         vv = dclamp(vv, (double)(mnv - gp.margin), (double)(mxv + gp.margin));
     }
     const size_t o = (size_t)oy * ax.n + ox;
+    const SelectorDev &S = A.sel;
+    if (S.mode != SEL_NONE) {
+        // algo-4 selector weights (recon.cpp:762-811)
+        const float diag = has_dir ? fmn(1.0f, 2.0f * fabsf(nxv * nyv)) : 0.0f;
+        const float hedgy = (1.0f - ss_ramp) * guide_fade;
+        const float w2 = hedgy * (1.0f - diag) * S.strength;
+        const float w3 = hedgy * diag * S.strength;
+        if (S.mode == SEL_FLOAT) {
+            if (sel_compressed(A.row_off, A.indices, ax.n, ay.n)) {
+                const float bu = A.out_u[o], bv = A.out_v[o];
+                if (w2 < 1e-4f && w3 < 1e-4f) {
+                    S.du[k] = 0.0f;
+                    S.dv[k] = 0.0f;
+                    S.w3c[k] = 0.0f;
+                } else {
+                    S.du[k] = hedgy * (1.0f - diag) * ((float)vu - bu);
+                    S.dv[k] = hedgy * (1.0f - diag) * ((float)vv - bv);
+                    S.w3c[k] = hedgy * diag;
+                }
+            } else {
+                S.w2p[o] = hedgy * (1.0f - diag);
+                S.w3p[o] = hedgy * diag;
+                S.gu[o] = (float)vu;
+                S.gv[o] = (float)vv;
+            }
+            return;
+        }
+        if (w2 >= 1e-4f || w3 >= 1e-4f) {
+            const size_t nc = (size_t)cw * ch;
+            const int cx = ax.cb_i0[ox], cy = ay.cb_i0[oy];
+            const float cxf = ax.cb_f[ox], cyf = ay.cb_f[oy];
+            const float bu = A.out_u[o], bv = A.out_v[o];
+            const float au = bilinear_fast(S.lgf, cw, ch, cx, cxf, cy, cyf);
+            const float bbu = bilinear_fast(S.lgf + nc, cw, ch, cx, cxf, cy, cyf);
+            const float cu = bilinear_fast(S.lgf + 2 * nc, cw, ch, cx, cxf, cy, cyf);
+            const float av = bilinear_fast(S.lgf + 3 * nc, cw, ch, cx, cxf, cy, cyf);
+            const float bbv = bilinear_fast(S.lgf + 4 * nc, cw, ch, cx, cxf, cy, cyf);
+            const float cv = bilinear_fast(S.lgf + 5 * nc, cw, ch, cx, cxf, cy, cyf);
+            vu = bu + w2 * ((float)vu - bu) + w3 * cu * (au * L0 + bbu - bu);
+            vv = bv + w2 * ((float)vv - bv) + w3 * cv * (av * L0 + bbv - bv);
+        } else {
+            vu = A.out_u[o];
+            vv = A.out_v[o];
+        }
+    }
     A.out_u[o] = (float)vu;
     A.out_v[o] = (float)vv;
 }
@@ -784,7 +931,179 @@ __global__ __launch_bounds__(kBlock) void k_guided(GuidedArgs A) {
     const uint32_t total = A.indices ? A.row_off[oh] : (uint32_t)ow * (uint32_t)oh;
     for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < total; k += gridDim.x * kBlock) {
         const uint32_t idx = A.indices ? A.indices[k] : k;
-        guided_pixel(A, (int)(idx % (uint32_t)ow), (int)(idx / (uint32_t)ow));
+        guided_pixel(A, (int)(idx % (uint32_t)ow), (int)(idx / (uint32_t)ow), k);
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// algo 4: LGF regression maps and selector blend (algos.cpp:241-593)
+// ---------------------------------------------------------------------------
+
+// pointSampledLuma (algos.cpp:300-315)
+__global__ __launch_bounds__(kBlock) void k_point_luma(const float *y, int w, int h, float *ls, int cw, int ch,
+                                                       const int *xi, const float *xf, const int *yi,
+                                                       const float *yf) {
+    const int x = blockIdx.x * kBlock + threadIdx.x, j = blockIdx.y;
+    if (x >= cw)
+        return;
+    ls[(size_t)j * cw + x] = bilinear_fast(y, w, h, xi[x], xf[x], yi[j], yf[j]);
+}
+
+// slidingMinMax, radius 2 (algos.cpp:70-139): in-bounds 5x5 window
+__global__ __launch_bounds__(kBlock) void k_minmax5(const float *u, const float *v, int cw, int ch, float *mm) {
+    const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
+    if (x >= cw)
+        return;
+    const size_t n = (size_t)cw * ch, i = (size_t)y * cw + x;
+    for (int c = 0; c < 2; ++c) {
+        const float *p = c ? v : u;
+        float lo = p[(size_t)max(0, y - 2) * cw + max(0, x - 2)], hi = lo;
+        for (int j = max(0, y - 2); j <= min(ch - 1, y + 2); ++j)
+            for (int k = max(0, x - 2); k <= min(cw - 1, x + 2); ++k) {
+                const float q = p[(size_t)j * cw + k];
+                lo = fmn(lo, q);
+                hi = fmx(hi, q);
+            }
+        mm[(2 * c) * n + i] = lo;
+        mm[(2 * c + 1) * n + i] = hi;
+    }
+}
+
+// buildLGFPair finalize (algos.cpp:381-414) over double 5x5 window sums,
+// plus the mutual-structure confidence fade (plugin.cpp:418-425, 543-551).
+__global__ __launch_bounds__(kBlock) void k_lgf(const float *__restrict__ ls, const float *__restrict__ u,
+                                                const float *__restrict__ v, int cw, int ch, double eps, int cedge,
+                                                const float *mm, const uint8_t *roi, const uint32_t *roi_count,
+                                                const float *gate, float ms, float *lgf) {
+    const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
+    if (x >= cw)
+        return;
+    const size_t n = (size_t)cw * ch, i = (size_t)y * cw + x;
+    const bool dense = !roi || (uint64_t)roi_count[0] * 2 >= (uint64_t)n;
+    if (!dense && !roi[i]) {
+        for (int c = 0; c < 6; ++c)
+            lgf[c * n + i] = 0.0f;
+        return;
+    }
+    const int x0 = max(0, x - 2), x1 = min(cw - 1, x + 2), y0 = max(0, y - 2), y1 = min(ch - 1, y + 2);
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
+    for (int j = y0; j <= y1; ++j)
+        for (int k = x0; k <= x1; ++k) {
+            const size_t q = (size_t)j * cw + k;
+            const double l = ls[q], a = u[q], b = v[q];
+            s0 += l; s1 += l * l; s2 += a; s3 += l * a; s4 += b; s5 += l * b;
+        }
+    const int cnt = (x1 - x0 + 1) * (y1 - y0 + 1);
+    const double mean_y = s0 / cnt;
+    const double var = s1 / cnt - mean_y * mean_y;
+    const double mean_u = s2 / cnt, mean_v = s4 / cnt;
+    const double su = (s3 / cnt - mean_y * mean_u) / (var + eps);
+    const double sv = (s5 / cnt - mean_y * mean_v) / (var + eps);
+    const float base = (float)(var / (var + eps));
+    float fu = 1.0f, fv = 1.0f;
+    if (cedge) {
+        const float lgx = (at(ls, cw, ch, x + 1, y) - at(ls, cw, ch, x - 1, y)) * 0.5f;
+        const float lgy = (at(ls, cw, ch, x, y + 1) - at(ls, cw, ch, x, y - 1)) * 0.5f;
+        const float gl = hypotf(lgx, lgy);
+        for (int c = 0; c < 2; ++c) {
+            const float *P = c ? v : u;
+            const float range = mm[(2 * c + 1) * n + i] - mm[(2 * c) * n + i];
+            float f = 1.0f;
+            if (!(gl <= 1e-9f || range <= 1e-6f)) {
+                const float cgx = (at(P, cw, ch, x + 1, y) - at(P, cw, ch, x - 1, y)) * 0.5f;
+                const float cgy = (at(P, cw, ch, x, y + 1) - at(P, cw, ch, x, y - 1)) * 0.5f;
+                const float gn = fabsf(cgx * lgx / gl + cgy * lgy / gl);
+                f = fclamp((2.2f - range / (gn + 1e-6f)) / 0.7f, 0.0f, 1.0f);
+            }
+            if (c) fv = f; else fu = f;
+        }
+    }
+    float cu = base * fu, cv = base * fv;
+    if (gate && (!roi || roi[i])) {
+        const float f = 1.0f - ms * (1.0f - gate[i]);
+        cu *= f;
+        cv *= f;
+    }
+    lgf[i] = (float)su;
+    lgf[n + i] = (float)(mean_u - su * mean_y);
+    lgf[2 * n + i] = cu;
+    lgf[3 * n + i] = (float)sv;
+    lgf[4 * n + i] = (float)(mean_v - sv * mean_y);
+    lgf[5 * n + i] = cv;
+}
+
+struct WorkList { const uint32_t *row_off; const uint32_t *indices; int ow, oh; };
+
+__device__ __forceinline__ uint32_t work_total(const WorkList &L) {
+    return L.indices ? L.row_off[L.oh] : (uint32_t)L.ow * (uint32_t)L.oh;
+}
+
+// ROI of the diagonal route (plugin.cpp:496-531)
+__global__ __launch_bounds__(kBlock) void k_roi_mark(WorkList L, SelectorDev S, AxisDev ax, AxisDev ay,
+                                                     uint8_t *roi, int cw, int ch) {
+    const uint32_t total = work_total(L);
+    const bool comp = sel_compressed(L.row_off, L.indices, L.ow, L.oh);
+    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < total; k += gridDim.x * kBlock) {
+        const uint32_t idx = L.indices ? L.indices[k] : k;
+        const float w3 = comp ? S.w3c[k] : S.w3p[idx];
+        if (w3 < 1e-4f)
+            continue;
+        const int ox = (int)(idx % (uint32_t)L.ow), oy = (int)(idx / (uint32_t)L.ow);
+        const int cx0 = ax.cb_i0[ox], cx1 = min(cx0 + 1, cw - 1);
+        const int cy0 = ay.cb_i0[oy], cy1 = min(cy0 + 1, ch - 1);
+        roi[(size_t)cy0 * cw + cx0] = 1;
+        roi[(size_t)cy0 * cw + cx1] = 1;
+        roi[(size_t)cy1 * cw + cx0] = 1;
+        roi[(size_t)cy1 * cw + cx1] = 1;
+    }
+}
+
+// blendSelectorCompressed / blendSelector (algos.cpp:453-593)
+__global__ __launch_bounds__(kBlock) void k_blend4(WorkList L, const float *y, int sw, int sh, AxisDev ax,
+                                                   AxisDev ay, int cw, int ch, SelectorDev S, float *out_u,
+                                                   float *out_v) {
+    const uint32_t total = work_total(L);
+    const bool comp = sel_compressed(L.row_off, L.indices, L.ow, L.oh);
+    const size_t n = (size_t)cw * ch;
+    const float lam = S.strength;
+    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < total; k += gridDim.x * kBlock) {
+        const uint32_t idx = L.indices ? L.indices[k] : k;
+        const int ox = (int)(idx % (uint32_t)L.ow), oy = (int)(idx / (uint32_t)L.ow);
+        const int cx = ax.cb_i0[ox], cy = ay.cb_i0[oy];
+        const float cxf = ax.cb_f[ox], cyf = ay.cb_f[oy];
+        if (comp) {
+            const float pu = out_u[idx], pv = out_v[idx];
+            const float w3 = lam * S.w3c[k];
+            float vu = pu + lam * S.du[k], vv = pv + lam * S.dv[k];
+            if (w3 != 0.0f) {
+                const float luma = bilinear_fast(y, sw, sh, ax.lb_i0[ox], ax.lb_f[ox], ay.lb_i0[oy], ay.lb_f[oy]);
+                const float au = bilinear_fast(S.lgf, cw, ch, cx, cxf, cy, cyf);
+                const float bu = bilinear_fast(S.lgf + n, cw, ch, cx, cxf, cy, cyf);
+                const float cu = bilinear_fast(S.lgf + 2 * n, cw, ch, cx, cxf, cy, cyf);
+                const float av = bilinear_fast(S.lgf + 3 * n, cw, ch, cx, cxf, cy, cyf);
+                const float bv = bilinear_fast(S.lgf + 4 * n, cw, ch, cx, cxf, cy, cyf);
+                const float cv = bilinear_fast(S.lgf + 5 * n, cw, ch, cx, cxf, cy, cyf);
+                vu += w3 * cu * (au * luma + bu - pu);
+                vv += w3 * cv * (av * luma + bv - pv);
+            }
+            out_u[idx] = vu;
+            out_v[idx] = vv;
+        } else {
+            const float w2v = S.w2p[idx] * lam, w3v = S.w3p[idx] * lam;
+            if (w2v < 1e-4f && w3v < 1e-4f)
+                continue;
+            const float L0 = bilinear_fast(y, sw, sh, ax.lb_i0[ox], ax.lb_f[ox], ay.lb_i0[oy], ay.lb_f[oy]);
+            const float au = bilinear_fast(S.lgf, cw, ch, cx, cxf, cy, cyf);
+            const float bu = bilinear_fast(S.lgf + n, cw, ch, cx, cxf, cy, cyf);
+            const float cu = bilinear_fast(S.lgf + 2 * n, cw, ch, cx, cxf, cy, cyf);
+            const float av = bilinear_fast(S.lgf + 3 * n, cw, ch, cx, cxf, cy, cyf);
+            const float bv = bilinear_fast(S.lgf + 4 * n, cw, ch, cx, cxf, cy, cyf);
+            const float cv = bilinear_fast(S.lgf + 5 * n, cw, ch, cx, cxf, cy, cyf);
+            const float ru = out_u[idx], rv = out_v[idx];
+            out_u[idx] = ru + (w2v * (S.gu[idx] - ru) + w3v * cu * (au * L0 + bu - ru));
+            out_v[idx] = rv + (w2v * (S.gv[idx] - rv) + w3v * cv * (av * L0 + bv - rv));
+        }
     }
 }
 
@@ -842,13 +1161,6 @@ __global__ __launch_bounds__(kBlock) void k_resample_v(const float *src, int sw,
     for (int j = 0; j < t.sup; ++j)
         acc += (double)wp[j] * src[(size_t)clampi(s + j, 0, sh - 1) * sw + x];
     dst[(size_t)y * sw + x] = (float)acc;
-}
-
-__device__ __forceinline__ float lut_lookup(const float *lut, int n, float idx) {
-    int i0 = (int)idx;
-    i0 = clampi(i0, 0, n - 2);
-    const float fr = fmn(idx - (float)i0, 1.0f);
-    return lut[i0] + (lut[i0 + 1] - lut[i0]) * fr;
 }
 
 __global__ __launch_bounds__(kBlock) void k_resample_radial(const float *src, int sw, int sh, float *dst, int dw,
@@ -989,12 +1301,66 @@ hipError_t launch_plain_bilinear(const float *u, const float *v, int cw, int ch,
 
 hipError_t launch_guided(const float *u, const float *v, int cw, int ch, const float *y, int sw, int sh,
                          const float *jxx, const float *jxy, const float *jyy, const float *lc, const float *ms,
-                         const AxisDev &ax, const AxisDev &ay, const GuideParams &gp, const uint32_t *row_offsets,
-                         const uint32_t *indices, float *out_u, float *out_v, hipStream_t s) {
-    GuidedArgs A{u, v, cw, ch, y, sw, sh, jxx, jxy, jyy, lc, ms, ax, ay, gp, row_offsets, indices, out_u, out_v};
+                         const AxisDev &ax, const AxisDev &ay, const RadialDev &rd, const GuideParams &gp,
+                         const uint32_t *row_offsets, const uint32_t *indices, float *out_u, float *out_v,
+                         const SelectorDev &sel, hipStream_t s) {
+    GuidedArgs A{u, v, cw, ch, y, sw, sh, jxx, jxy, jyy, lc, ms, ax, ay, rd, gp, row_offsets, indices, out_u, out_v,
+                 sel};
     const size_t n = (size_t)ax.n * ay.n;
     const unsigned blocks = (unsigned)std::min<size_t>(8192, (n + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_guided, dim3(blocks), dim3(kBlock), 0, s, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_plain_radial(const float *u, const float *v, int cw, int ch, const AxisDev &ax,
+                               const AxisDev &ay, const RadialDev &rd, float *out_u, float *out_v, int clamp_hull,
+                               float margin, hipStream_t s) {
+    hipLaunchKernelGGL(k_plain_radial, rows_grid(ax.n, ay.n), dim3(kBlock), 0, s, u, v, cw, ch, ax, ay, rd, out_u,
+                       out_v, clamp_hull, margin);
+    return hipGetLastError();
+}
+
+hipError_t launch_point_luma(const float *y, int w, int h, float *ls, int cw, int ch, const int *xi, const float *xf,
+                             const int *yi, const float *yf, hipStream_t s) {
+    hipLaunchKernelGGL(k_point_luma, rows_grid(cw, ch), dim3(kBlock), 0, s, y, w, h, ls, cw, ch, xi, xf, yi, yf);
+    return hipGetLastError();
+}
+
+hipError_t launch_minmax5(const float *u, const float *v, int cw, int ch, float *mm, hipStream_t s) {
+    hipLaunchKernelGGL(k_minmax5, rows_grid(cw, ch), dim3(kBlock), 0, s, u, v, cw, ch, mm);
+    return hipGetLastError();
+}
+
+hipError_t launch_lgf(const float *ls, const float *u, const float *v, int cw, int ch, double eps, int cedge,
+                      const float *mm, const uint8_t *roi, const uint32_t *roi_count, const float *gate, float ms,
+                      float *lgf, hipStream_t s) {
+    hipLaunchKernelGGL(k_lgf, rows_grid(cw, ch), dim3(kBlock), 0, s, ls, u, v, cw, ch, eps, cedge, mm, roi,
+                       roi_count, gate, ms, lgf);
+    return hipGetLastError();
+}
+
+hipError_t launch_roi_mark(const uint32_t *row_offsets, const uint32_t *indices, int ow, int oh,
+                           const SelectorDev &sel, const AxisDev &ax, const AxisDev &ay, uint8_t *roi, int cw, int ch,
+                           uint32_t *roi_count, hipStream_t s) {
+    const size_t nc = (size_t)cw * ch, n = (size_t)ow * oh;
+    hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)std::min<size_t>(1024, (nc + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       s, roi, nc, (uint8_t)0);
+    const WorkList L{row_offsets, indices, ow, oh};
+    hipLaunchKernelGGL(k_roi_mark, dim3((unsigned)std::min<size_t>(8192, (n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       s, L, sel, ax, ay, roi, cw, ch);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, roi_count, 1);
+    hipLaunchKernelGGL(k_count_u8, dim3((unsigned)std::min<size_t>(512, (nc + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       s, roi, nc, roi_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_blend4(const uint32_t *row_offsets, const uint32_t *indices, const float *y, int sw, int sh,
+                         const AxisDev &ax, const AxisDev &ay, int cw, int ch, const SelectorDev &sel, float *out_u,
+                         float *out_v, hipStream_t s) {
+    const size_t n = (size_t)ax.n * ay.n;
+    const WorkList L{row_offsets, indices, ax.n, ay.n};
+    hipLaunchKernelGGL(k_blend4, dim3((unsigned)std::min<size_t>(8192, (n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       s, L, y, sw, sh, ax, ay, cw, ch, sel, out_u, out_v);
     return hipGetLastError();
 }
 

@@ -217,6 +217,84 @@ void footprint(int cn, double r, double shift, int luma_n, std::vector<int> &b, 
     }
 }
 
+// Radial (jinc) phase tables (src/kernels.cpp:449-526): unique (offset,
+// activity) phases per axis in first-seen order, one sup_y x sup_x weight
+// block per phase pair.  Empty when the phase counts exceed the limits.
+struct HostRadial {
+    bool valid = false;
+    int nx = 0, ny = 0;
+    std::vector<uint16_t> xphase, yphase;
+    std::vector<float> w;
+};
+
+int collect_phases(const HostAxis &a, std::vector<uint16_t> &index, std::vector<float> &offsets,
+                   std::vector<uint64_t> &activity) {
+    if (a.sup > 64)
+        return -1;
+    std::vector<uint32_t> bits;
+    index.assign(a.n, 0);
+    for (int x = 0; x < a.n; ++x) {
+        const float off = float(a.start[x]) - a.pos[x];
+        uint32_t b;
+        std::memcpy(&b, &off, 4);
+        uint64_t m = 0;
+        for (int t = 0; t < a.sup; ++t)
+            if (a.am[size_t(x) * a.sup + t] != 0.0f)
+                m |= uint64_t(1) << t;
+        size_t k = 0;
+        while (k < bits.size() && !(bits[k] == b && activity[k] == m))
+            ++k;
+        if (k == bits.size()) {
+            if (bits.size() == 64)
+                return -1;
+            bits.push_back(b);
+            offsets.push_back(off);
+            activity.push_back(m);
+        }
+        index[x] = uint16_t(k);
+    }
+    return int(bits.size());
+}
+
+float lut_lookup(const std::vector<float> &lut, float idx) {
+    const int n = int(lut.size());
+    const int i0 = std::clamp(int(idx), 0, n - 2);
+    const float fr = std::min(idx - float(i0), 1.0f);
+    return lut[i0] + (lut[i0 + 1] - lut[i0]) * fr;
+}
+
+HostRadial build_radial(const HostAxis &ax, const HostAxis &ay, int src_w, int src_h, const std::vector<float> &lut) {
+    HostRadial r;
+    std::vector<float> xo, yo;
+    std::vector<uint64_t> xa, ya;
+    const int nx = collect_phases(ax, r.xphase, xo, xa);
+    const int ny = nx < 0 ? -1 : collect_phases(ay, r.yphase, yo, ya);
+    if (nx < 0 || ny < 0)
+        return r;
+    const size_t pairs = size_t(nx) * ny, per = size_t(ax.sup) * ay.sup;
+    if (pairs > 1024 || pairs * per > 262144)
+        return r;
+    r.nx = nx;
+    r.ny = ny;
+    r.w.assign(pairs * per, 0.0f);
+    const float inv_x = float(std::min(1.0, double(ax.n) / src_w));
+    const float inv_y = float(std::min(1.0, double(ay.n) / src_h));
+    for (int py = 0; py < ny; ++py)
+        for (int px = 0; px < nx; ++px) {
+            float *w = &r.w[(size_t(py) * nx + px) * per];
+            for (int j = 0; j < ay.sup; ++j)
+                for (int i = 0; i < ax.sup; ++i) {
+                    if (!(xa[px] & (uint64_t(1) << i)) || !(ya[py] & (uint64_t(1) << j)))
+                        continue;
+                    const float dx = (xo[px] + float(i)) * inv_x;
+                    const float dy = (yo[py] + float(j)) * inv_y;
+                    w[size_t(j) * ax.sup + i] = lut_lookup(lut, std::sqrt(dx * dx + dy * dy) * 1024.0f);
+                }
+        }
+    r.valid = true;
+    return r;
+}
+
 // ---------------------------------------------------------------------------
 // device blobs: pack host vectors into one allocation
 // ---------------------------------------------------------------------------
@@ -262,7 +340,11 @@ namespace {
 struct Geometry {
     void *dev = nullptr;
     AxisDev ax, ay;
+    RadialDev rd;
     const int *xb = nullptr, *xe = nullptr, *yb = nullptr, *ye = nullptr;
+    // point-sampled luma at chroma sites (algos.cpp:300-315)
+    const int *pl_xi = nullptr, *pl_yi = nullptr;
+    const float *pl_xf = nullptr, *pl_yf = nullptr;
 };
 
 struct Workspace {
@@ -279,6 +361,9 @@ struct Workspace {
     float *luma_tmp = nullptr, *luma_out = nullptr;
     uint8_t *seed = nullptr, *mask = nullptr, *mtmp = nullptr, *cmask = nullptr;
     uint32_t *row_off = nullptr, *indices = nullptr, *counters = nullptr;
+    // algo 4
+    float *ls = nullptr, *mm = nullptr, *lgf = nullptr, *sel = nullptr, *selp = nullptr;
+    uint8_t *roi = nullptr;
     // stage timeline of frames enqueued with timing on: (stage, event);
     // stage -1 opens a frame.  Events come from a per-workspace pool.
     std::vector<std::pair<int, hipEvent_t>> marks;
@@ -302,6 +387,7 @@ struct lgcr_clip {
     int max_inflight = 4;
     void *lut_dev = nullptr;
     int lut_n = 0;
+    std::vector<float> lut;
     void *weights_dev = nullptr;
     WeightsDev wh, wv;
 
@@ -351,6 +437,20 @@ lgcr_status get_geometry(lgcr_clip *c, double sx, double sy, Geometry **out) {
         o[a][9] = blob.add(h.lb_i0); o[a][10] = blob.add(h.lb_f);
     }
     const size_t oxb = blob.add(xb), oxe = blob.add(xe), oyb = blob.add(yb), oye = blob.add(ye);
+    std::vector<float> plx(c->cw), ply(c->ch);
+    for (int x = 0; x < c->cw; ++x)
+        plx[x] = float((x + 0.5) * c->rw - 0.5 + sx);
+    for (int y = 0; y < c->ch; ++y)
+        ply[y] = float((y + 0.5) * c->rh - 0.5 + sy);
+    std::vector<int> plxi, plyi;
+    std::vector<float> plxf, plyf;
+    bilin_axis(plx, c->sw, plxi, plxf);
+    bilin_axis(ply, c->sh, plyi, plyf);
+    const size_t opxi = blob.add(plxi), opxf = blob.add(plxf), opyi = blob.add(plyi), opyf = blob.add(plyf);
+    HostRadial hr;
+    if (c->p.radial)
+        hr = build_radial(ax, ay, c->cw, c->ch, c->lut);
+    const size_t orx = blob.add(hr.xphase), ory = blob.add(hr.yphase), orw = blob.add(hr.w);
     auto g = std::make_unique<Geometry>();
     HIP_TRY(hipSetDevice(c->dev->ordinal));
     HIP_TRY(hipMalloc(&g->dev, blob.host.size()));
@@ -377,6 +477,23 @@ lgcr_status get_geometry(lgcr_clip *c, double sx, double sy, Geometry **out) {
     g->xe = reinterpret_cast<const int *>(base + oxe);
     g->yb = reinterpret_cast<const int *>(base + oyb);
     g->ye = reinterpret_cast<const int *>(base + oye);
+    g->pl_xi = reinterpret_cast<const int *>(base + opxi);
+    g->pl_xf = reinterpret_cast<const float *>(base + opxf);
+    g->pl_yi = reinterpret_cast<const int *>(base + opyi);
+    g->pl_yf = reinterpret_cast<const float *>(base + opyf);
+    g->rd.valid = hr.valid ? 1 : 0;
+    g->rd.sup_x = ax.sup;
+    g->rd.sup_y = ay.sup;
+    g->rd.nx = hr.nx;
+    g->rd.ny = hr.ny;
+    g->rd.xphase = reinterpret_cast<const uint16_t *>(base + orx);
+    g->rd.yphase = reinterpret_cast<const uint16_t *>(base + ory);
+    g->rd.w = reinterpret_cast<const float *>(base + orw);
+    g->rd.lut = static_cast<const float *>(c->lut_dev);
+    g->rd.lut_n = c->lut_n;
+    g->rd.lut_scale = 1024.0f;
+    g->rd.inv_wx = float(std::min(1.0, double(c->ow) / c->cw));
+    g->rd.inv_wy = float(std::min(1.0, double(c->oh) / c->ch));
     *out = g.get();
     c->geos.emplace(key, std::move(g));
     return LGCR_OK;
@@ -426,6 +543,14 @@ void layout(const lgcr_clip *c, Workspace *w, Carver &cv) {
     w->row_off = cv.take<uint32_t>(size_t(c->oh) + 1);
     w->indices = cv.take<uint32_t>(no);
     w->counters = cv.take<uint32_t>(4);
+    if (c->p.algo == 4 && c->p.strength > 0.0) {
+        w->ls = cv.take<float>(nc);
+        w->mm = cv.take<float>(4 * nc);
+        w->lgf = cv.take<float>(6 * nc);
+        w->roi = cv.take<uint8_t>(nc);
+        w->sel = cv.take<float>(3 * no);
+        w->selp = cv.take<float>(4 * no);
+    }
 }
 
 lgcr_status create_workspace(lgcr_clip *c, std::unique_ptr<Workspace> &out) {
@@ -611,13 +736,17 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
     const int clamp_hull = p.ar_margin >= 0.0;
     const float margin = float(std::max(0.0, p.ar_margin));
     auto plain = [&]() -> hipError_t {
+        if (p.radial)
+            return launch_plain_radial(w->u, w->v, c->cw, c->ch, g.ax, g.ay, g.rd, w->out_u, w->out_v, clamp_hull,
+                                       margin, s);
         if (p.kernel == LGCR_KERNEL_BILINEAR)
             return launch_plain_bilinear(w->u, w->v, c->cw, c->ch, g.ax, g.ay, w->out_u, w->out_v, s);
         return launch_plain_separable(w->u, w->v, c->cw, c->ch, g.ax, g.ay, w->hbuf, w->out_u, w->out_v,
                                       clamp_hull, margin, s);
     };
-    if (!guided || w->has_ws) {
-        // BuildBaseChroma (plugin.cpp:627-648)
+    const bool algo4 = p.algo == 4 && guided;
+    if (!guided || w->has_ws || algo4) {
+        // BuildBaseChroma (plugin.cpp:396-402, 627-648)
         HIP_TRY(plain());
         if (c->capture && guided) {
             const size_t no = size_t(c->ow) * c->oh * sizeof(float);
@@ -627,12 +756,13 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
         HIP_TRY(mark(LGCR_STAGE_BUILD_BASE_CHROMA));
     }
     if (guided) {
-        // ApplyGuidedCorrection (plugin.cpp:649-674)
+        // ApplyGuidedCorrection (plugin.cpp:649-674); algo 4 runs the guided
+        // pass at strength 1 with the selector (plugin.cpp:427-440, 452-490)
         GuideParams gp;
         gp.sigma = float(p.sigma);
         gp.sratio = float(p.sratio);
         gp.sdb = float(p.sdb);
-        gp.strength = float(p.strength);
+        gp.strength = algo4 ? 1.0f : float(p.strength);
         gp.rescue = float(p.rescue);
         gp.stretch = float(p.stretch);
         gp.inv_gsigma2 = 1.0f / float(p.gsigma * p.gsigma);
@@ -646,10 +776,48 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
         gp.clamp_hull = clamp_hull;
         gp.ms_on = p.ms > 0.0;
         gp.radial = p.radial;
+        const size_t nc = size_t(c->cw) * c->ch, no = size_t(c->ow) * c->oh;
+        const uint32_t *idx = w->has_ws ? w->indices : nullptr;
+        SelectorDev sel;
+        const double eps = p.reg * p.reg;
+        const float *gate = gp.ms_on ? w->ms : nullptr;
+        if (algo4) {
+            sel.strength = float(p.strength);
+            sel.lgf = w->lgf;
+            HIP_TRY(launch_point_luma(w->y, c->sw, c->sh, w->ls, c->cw, c->ch, g.pl_xi, g.pl_xf, g.pl_yi, g.pl_yf, s));
+            if (p.cedge)
+                HIP_TRY(launch_minmax5(w->u, w->v, c->cw, c->ch, w->mm, s));
+            if (!is_float) {
+                // integer output: dense LGF + single-expression selector
+                HIP_TRY(launch_lgf(w->ls, w->u, w->v, c->cw, c->ch, eps, p.cedge, w->mm, nullptr, nullptr, gate,
+                                   float(p.ms), w->lgf, s));
+                HIP_TRY(mark(LGCR_STAGE_BUILD_LGF));
+                sel.mode = SEL_MAPS;
+            } else {
+                sel.mode = SEL_FLOAT;
+                sel.du = w->sel;
+                sel.dv = w->sel + no;
+                sel.w3c = w->sel + 2 * no;
+                sel.gu = w->selp;
+                sel.gv = w->selp + no;
+                sel.w2p = w->selp + 2 * no;
+                sel.w3p = w->selp + 3 * no;
+            }
+        }
         HIP_TRY(launch_guided(w->u, w->v, c->cw, c->ch, w->y, c->sw, c->sh, w->jxx, w->jxy, w->jyy, w->lc,
-                              gp.ms_on ? w->ms : nullptr, g.ax, g.ay, gp, w->row_off,
-                              w->has_ws ? w->indices : nullptr, w->out_u, w->out_v, s));
+                              gate, g.ax, g.ay, g.rd, gp, w->row_off, idx, w->out_u, w->out_v, sel, s));
         HIP_TRY(mark(LGCR_STAGE_APPLY_GUIDED_CORRECTION));
+        if (algo4 && is_float) {
+            HIP_TRY(launch_roi_mark(w->row_off, idx, c->ow, c->oh, sel, g.ax, g.ay, w->roi, c->cw, c->ch,
+                                    w->counters + 1, s));
+            HIP_TRY(launch_lgf(w->ls, w->u, w->v, c->cw, c->ch, eps, p.cedge, w->mm, w->roi, w->counters + 1, gate,
+                               float(p.ms), w->lgf, s));
+            HIP_TRY(mark(LGCR_STAGE_BUILD_LGF));
+            HIP_TRY(launch_blend4(w->row_off, idx, w->y, c->sw, c->sh, g.ax, g.ay, c->cw, c->ch, sel, w->out_u,
+                                  w->out_v, s));
+            HIP_TRY(mark(LGCR_STAGE_APPLY_SELECTOR));
+        }
+        (void)nc;
     }
     // BackProject, same-size 4:2:0 only (plugin.cpp:374-375, 683-689)
     if (p.bp > 0.0 && c->same && c->cw * 2 == c->sw && c->ch * 2 == c->sh) {
@@ -688,12 +856,10 @@ lgcr_status validate(const lgcr_params &p, const lgcr_format &f) {
         return fail(LGCR_INVALID_ARGUMENT, "LGCR: taps must be in the range 1..64");
     if (p.kernel < 0 || p.kernel > LGCR_KERNEL_JINC)
         return fail(LGCR_INVALID_ARGUMENT, "LGCR: unknown kernel");
-    if (p.strength > 0.0 && p.algo != 2)
+    if (p.strength > 0.0 && p.algo == 6)
         return fail(LGCR_UNSUPPORTED, "LGCR HIP backend: algo %d is not available in this build", p.algo);
     if (p.bm)
         return fail(LGCR_UNSUPPORTED, "LGCR HIP backend: bm=True is not available in this build");
-    if (p.radial)
-        return fail(LGCR_UNSUPPORTED, "LGCR HIP backend: the jinc kernel is not available in this build");
     return LGCR_OK;
 }
 
@@ -824,6 +990,7 @@ lgcr_status lgcr_clip_create(lgcr_device *dev, const lgcr_params *params, const 
         HIP_TRY(hipMalloc(&c->lut_dev, sizeof(float) * n));
         HIP_TRY(hipMemcpy(c->lut_dev, lut.data(), sizeof(float) * n, hipMemcpyHostToDevice));
         c->lut_n = n;
+        c->lut = lut;
     }
     Geometry *g = nullptr;
     st = get_geometry(c.get(), p.shift_x, p.shift_y, &g);
