@@ -1578,6 +1578,325 @@ static void resample_radial(const or_params *p, const float *src, int sw, int sh
 }
 
 /* ------------------------------------------------------------------------ */
+/* algo 6: affine credibility maps, detail map, detail transfer             */
+/* (maps.cpp:501-680, algos.cpp:809-1698)                                    */
+/* ------------------------------------------------------------------------ */
+
+static float median3f(float a, float b, float c) {
+    const float mn = minf(minf(a, b), c) , mx = maxf(maxf(a, b), c);
+    return a + b + c - mn - mx;
+}
+
+typedef struct {
+    float *au, *av, *g, *mnu, *mxu, *mnv, *mxv, *rng, *deg;  /* chroma grid */
+} affine_maps;
+
+/* candidate degradations D_k(Y) on the chroma grid: box (= lc), bilinear and
+ * bicubic(0, 0.6) polyphase downsamples (maps.cpp:512-680) */
+static void yc_filtered(const float *y, int w, int h, int cw, int ch, double sx, double sy,
+                        float *yc1, float *yc2) {
+    for (int c = 0; c < 2; ++c) {
+        const int k = c ? OR_BICUBIC : OR_BILINEAR;
+        const double p1 = 0.0, p2 = c ? 0.6 : 0.0, sup = c ? 2.0 : 1.0;
+        or_weights th, tv;
+        or_build_weights(w, cw, k, p1, p2, sup, sx, &th);
+        or_build_weights(h, ch, k, p1, p2, sup, sy, &tv);
+        float *tmp = (float *)malloc(sizeof(float) * (size_t)cw * h);
+        resample_h(y, w, h, tmp, &th);
+        resample_v(tmp, cw, h, c ? yc2 : yc1, &tv);
+        free(tmp);
+        or_free_weights(&th);
+        or_free_weights(&tv);
+    }
+}
+
+/* the per-sample finalise shared by both affine paths; `inv` selects the
+ * sparse path's reciprocal multiply (algos.cpp:1082-1120) over the dense
+ * path's divisions (algos.cpp:1483-1512) */
+static void affine_finalize(const double *sum, int n, int sparse_form, double eps, double eps_sig,
+                            float *au, float *av, float *g) {
+    double meanU, meanV, varU, varV, qk[3], aUk[3], aVk[3];
+    const double inv = 1.0 / n;
+    if (sparse_form) {
+        meanU = sum[0] * inv; meanV = sum[2] * inv;
+        varU = fmax(0.0, sum[1] * inv - meanU * meanU);
+        varV = fmax(0.0, sum[3] * inv - meanV * meanV);
+    } else {
+        meanU = sum[0] / n; meanV = sum[2] / n;
+        varU = fmax(0.0, sum[1] / n - meanU * meanU);
+        varV = fmax(0.0, sum[3] / n - meanV * meanV);
+    }
+    for (int k = 0; k < 3; ++k) {
+        const int b = 4 + 4 * k;
+        double meanY, varY, covU, covV;
+        if (sparse_form) {
+            meanY = sum[b] * inv;
+            varY = fmax(0.0, sum[b + 1] * inv - meanY * meanY);
+            covU = sum[b + 2] * inv - meanY * meanU;
+            covV = sum[b + 3] * inv - meanY * meanV;
+        } else {
+            meanY = sum[b] / n;
+            varY = fmax(0.0, sum[b + 1] / n - meanY * meanY);
+            covU = sum[b + 2] / n - meanY * meanU;
+            covV = sum[b + 3] / n - meanY * meanV;
+        }
+        aUk[k] = clampd(covU / (varY + eps), -16.0, 16.0);
+        aVk[k] = clampd(covV / (varY + eps), -16.0, 16.0);
+        qk[k] = clampd((covU * covU + covV * covV) / ((varY + eps) * (varU + varV + eps)), 0.0, 1.0);
+    }
+    *au = median3f((float)aUk[0], (float)aUk[1], (float)aUk[2]);
+    *av = median3f((float)aVk[0], (float)aVk[1], (float)aVk[2]);
+    const double qmax = fmax(fmax(qk[0], qk[1]), qk[2]);
+    const double qmin = fmin(fmin(qk[0], qk[1]), qk[2]);
+    const double stab = qmax > 1e-9 ? qmin / qmax : 0.0;
+    const double sig = (varU + varV) / (varU + varV + eps_sig);
+    if (sparse_form)
+        *g = (float)(((qk[0] + qk[1] + qk[2]) / 3.0) * stab * sig);
+    else
+        *g = (float)(((qk[0] + qk[1] + qk[2]) / 3.0) * stab * sig);
+}
+
+/* buildAffineMaps (algos.cpp:1382-1526): sparse ROI path = direct window sums
+ * over chroma-mask samples (affineSparseFixed5, algos.cpp:1026-1137); dense
+ * path = rolling sums (forEachWindowSum<16>) + sliding min/max. */
+static void affine_maps_build(const or_params *p, const float *U, const float *V, int cw, int ch,
+                              const float *yc[3], const uint8_t *cmask, int sparse_roi, affine_maps *m) {
+    const size_t nc = (size_t)cw * ch;
+    const double eps = p->reg * p->reg;
+    const double eps_sig = (0.25 * p->sigma) * (0.25 * p->sigma);
+    for (size_t i = 0; i < nc; ++i)
+        m->deg[i] = median3f(yc[0][i], yc[1][i], yc[2][i]);
+    if (sparse_roi) {
+        for (size_t i = 0; i < nc; ++i)
+            m->au[i] = m->av[i] = m->g[i] = m->mnu[i] = m->mxu[i] = m->mnv[i] = m->mxv[i] = m->rng[i] = 0.0f;
+        for (int y = 0; y < ch; ++y)
+            for (int x = 0; x < cw; ++x) {
+                const size_t i = (size_t)y * cw + x;
+                if (!cmask[i])
+                    continue;
+                double sum[16] = {0};
+                const int top = maxi(0, y - 2), bot = mini(ch - 1, y + 2);
+                const int lef = maxi(0, x - 2), rig = mini(cw - 1, x + 2);
+                float hmnu = U[(size_t)top * cw + lef], hmxu = hmnu;
+                float hmnv = V[(size_t)top * cw + lef], hmxv = hmnv;
+                for (int py = top; py <= bot; ++py)
+                    for (int px = lef; px <= rig; ++px) {
+                        const size_t q = (size_t)py * cw + px;
+                        const double u = U[q], v = V[q];
+                        sum[0] += u; sum[1] += u * u; sum[2] += v; sum[3] += v * v;
+                        hmnu = minf(hmnu, (float)u); hmxu = maxf(hmxu, (float)u);
+                        hmnv = minf(hmnv, (float)v); hmxv = maxf(hmxv, (float)v);
+                        for (int k = 0; k < 3; ++k) {
+                            const double l = yc[k][q];
+                            const int b = 4 + 4 * k;
+                            sum[b] += l; sum[b + 1] += l * l; sum[b + 2] += l * u; sum[b + 3] += l * v;
+                        }
+                    }
+                const int n = (rig - lef + 1) * (bot - top + 1);
+                m->mnu[i] = hmnu; m->mxu[i] = hmxu; m->mnv[i] = hmnv; m->mxv[i] = hmxv;
+                m->rng[i] = maxf(hmxu - hmnu, hmxv - hmnv);
+                affine_finalize(sum, n, 1, eps, eps_sig, &m->au[i], &m->av[i], &m->g[i]);
+            }
+        return;
+    }
+    window_minmax(U, cw, ch, 2, m->mnu, m->mxu);
+    window_minmax(V, cw, ch, 2, m->mnv, m->mxv);
+    enum { N = 16 };
+    const int r = 2;
+    double *cols = (double *)calloc((size_t)cw * N, sizeof(double));
+    const int rows = 2 * r + 2;
+    double *cache = (double *)calloc((size_t)rows * cw * N, sizeof(double));
+    int top_cur = 0, bot_cur = -1;
+    for (int yy = 0; yy < ch; ++yy) {
+        const int top = maxi(0, yy - r), bottom = mini(ch - 1, yy + r);
+        while (bot_cur < bottom) {
+            ++bot_cur;
+            double *c = cache + (size_t)(bot_cur % rows) * cw * N;
+            for (int x = 0; x < cw; ++x) {
+                const size_t q = (size_t)bot_cur * cw + x;
+                const double u = U[q], v = V[q];
+                double *val = c + (size_t)x * N;
+                val[0] = u; val[1] = u * u; val[2] = v; val[3] = v * v;
+                for (int k = 0; k < 3; ++k) {
+                    const double l = yc[k][q];
+                    const int b = 4 + 4 * k;
+                    val[b] = l; val[b + 1] = l * l; val[b + 2] = l * u; val[b + 3] = l * v;
+                }
+                for (int k = 0; k < N; ++k) cols[(size_t)x * N + k] += val[k];
+            }
+        }
+        while (top_cur < top) {
+            const double *c = cache + (size_t)(top_cur % rows) * cw * N;
+            for (int x = 0; x < cw; ++x)
+                for (int k = 0; k < N; ++k) cols[(size_t)x * N + k] -= c[(size_t)x * N + k];
+            ++top_cur;
+        }
+        double tot[N];
+        for (int k = 0; k < N; ++k) tot[k] = 0.0;
+        for (int x = 0; x <= mini(cw - 1, r); ++x)
+            for (int k = 0; k < N; ++k) tot[k] += cols[(size_t)x * N + k];
+        for (int x = 0; x < cw; ++x) {
+            const int left = maxi(0, x - r), right = mini(cw - 1, x + r);
+            const int n = (right - left + 1) * (bottom - top + 1);
+            const size_t i = (size_t)yy * cw + x;
+            m->rng[i] = maxf(m->mxu[i] - m->mnu[i], m->mxv[i] - m->mnv[i]);
+            affine_finalize(tot, n, 0, eps, eps_sig, &m->au[i], &m->av[i], &m->g[i]);
+            if (x - r >= 0)
+                for (int k = 0; k < N; ++k) tot[k] -= cols[(size_t)(x - r) * N + k];
+            if (x + r + 1 < cw)
+                for (int k = 0; k < N; ++k) tot[k] += cols[(size_t)(x + r + 1) * N + k];
+        }
+    }
+    free(cols);
+    free(cache);
+}
+
+static int nyquist_levels(double ratio) {
+    int levels = 0;
+    while (ratio > 1.5 && levels < 4) {
+        ratio *= 0.5;
+        ++levels;
+    }
+    return levels;
+}
+
+/* dyadic [1/4, 1/2, 1/4] passes with step 2^pass along x then y, edge
+ * clamped (DetailRowPipeline / suppressSourceNyquist, algos.cpp:1148-1364) */
+static void nyquist_passes(float *cur, int w, int h, double rw, double rh) {
+    const int xl = nyquist_levels(rw), yl = nyquist_levels(rh);
+    float *nx = (float *)malloc(sizeof(float) * (size_t)w * h);
+    for (int pass = 0; pass < xl; ++pass) {
+        const int step = 1 << pass;
+        for (int y = 0; y < h; ++y)
+            for (int x = 0; x < w; ++x) {
+                const float *r = cur + (size_t)y * w;
+                nx[(size_t)y * w + x] = 0.25f * r[maxi(0, x - step)] + 0.5f * r[x] + 0.25f * r[mini(w - 1, x + step)];
+            }
+        memcpy(cur, nx, sizeof(float) * (size_t)w * h);
+    }
+    for (int pass = 0; pass < yl; ++pass) {
+        const int step = 1 << pass;
+        for (int y = 0; y < h; ++y) {
+            const float *t = cur + (size_t)maxi(0, y - step) * w, *m = cur + (size_t)y * w;
+            const float *b = cur + (size_t)mini(h - 1, y + step) * w;
+            for (int x = 0; x < w; ++x)
+                nx[(size_t)y * w + x] = 0.25f * t[x] + 0.5f * m[x] + 0.25f * b[x];
+        }
+        memcpy(cur, nx, sizeof(float) * (size_t)w * h);
+    }
+    free(nx);
+}
+
+/* buildDetailMap (algos.cpp:1528-1585): detail = Y - P(deg) on the source
+ * grid, Nyquist-suppressed, then resampled to the output grid */
+static void detail_map(const or_params *p, const float *y, int sw, int sh, const float *deg, int cw,
+                       int ch, float *detail_out) {
+    const double rw = (double)sw / cw, rh = (double)sh / ch;
+    or_params ps = *p;
+    ps.out_w = sw;
+    ps.out_h = sh;
+    const size_t ns = (size_t)sw * sh;
+    float *rec = (float *)malloc(sizeof(float) * ns), *tmp = (float *)malloc(sizeof(float) * ns);
+    or_plain_chroma(&ps, deg, deg, cw, ch, sw, sh, rec, tmp);
+    for (size_t i = 0; i < ns; ++i)
+        rec[i] = y[i] - rec[i];
+    nyquist_passes(rec, sw, sh, rw, rh);
+    if (p->out_w == sw && p->out_h == sh) {
+        memcpy(detail_out, rec, sizeof(float) * ns);
+    } else if (p->radial) {
+        resample_radial(p, rec, sw, sh, detail_out, p->out_w, p->out_h);
+    } else {
+        or_weights th, tv;
+        or_build_weights(sw, p->out_w, p->kernel, p->kp1, p->kp2, p->support, 0.0, &th);
+        or_build_weights(sh, p->out_h, p->kernel, p->kp1, p->kp2, p->support, 0.0, &tv);
+        float *t2 = (float *)malloc(sizeof(float) * (size_t)p->out_w * sh);
+        resample_h(rec, sw, sh, t2, &th);
+        resample_v(t2, p->out_w, sh, detail_out, &tv);
+        free(t2);
+        or_free_weights(&th);
+        or_free_weights(&tv);
+    }
+    free(rec);
+    free(tmp);
+}
+
+/* detailTransferImpl (algos.cpp:1587-1689) */
+static void detail_transfer(const or_params *p, const or_debug *D, const affine_maps *m,
+                            const float *detail, const or_axis *ax, const or_axis *ay,
+                            float *out_u, float *out_v) {
+    const int ow = D->ow, oh = D->oh, cw = D->cw, ch = D->ch, sw = D->sw, sh = D->sh;
+    const int use_ms = p->ms > 0.0 && D->ms;
+    const float mss = (float)p->ms, qs = (float)p->qgate, st = (float)p->strength;
+    const float ar = (float)(p->ar_margin > 0.0 ? p->ar_margin : 0.0);
+    const int clamp_hull = p->ar_margin >= 0.0;
+    const or_workset *ws = D->has_ws ? &D->ws : NULL;
+    const size_t total = ws ? ws->n_indices : (size_t)ow * oh;
+    for (size_t k = 0; k < total; ++k) {
+        const uint32_t idx = ws ? ws->indices[k] : (uint32_t)k;
+        const int ox = (int)(idx % (uint32_t)ow), oy = (int)(idx / (uint32_t)ow);
+        const int lxi = ax->lb_i0[ox], lyi = ay->lb_i0[oy];
+        const float lxf = ax->lb_f[ox], lyf = ay->lb_f[oy];
+        const float jxx = bilinear_fast(D->jxx, sw, sh, lxi, lxf, lyi, lyf);
+        const float jxy = bilinear_fast(D->jxy, sw, sh, lxi, lxf, lyi, lyf);
+        const float jyy = bilinear_fast(D->jyy, sw, sh, lxi, lxf, lyi, lyf);
+        const tdir d = principal_dir(jxx, jxy, jyy);
+        const int cxi = ax->cb_i0[ox], cyi = ay->cb_i0[oy];
+        const float cxf = ax->cb_f[ox], cyf = ay->cb_f[oy];
+        const float q = bilinear_fast(m->g, cw, ch, cxi, cxf, cyi, cyf);
+        float g = (1.0f - qs * (1.0f - q)) * st;
+        if (use_ms) {
+            const float msv = bilinear_fast(D->ms, cw, ch, cxi, cxf, cyi, cyf);
+            g *= 1.0f - mss * (1.0f - msv);
+        }
+        const float au = bilinear_fast(m->au, cw, ch, cxi, cxf, cyi, cyf);
+        const float av = bilinear_fast(m->av, cw, ch, cxi, cxf, cyi, cyf);
+        const float coh = d.coherence;
+        const float tx = -d.ny, ty = d.nx;
+        const float dc = detail[idx];
+        const float s = 1.0f + (float)p->stretch * coh;
+        const float dtp = bilinear(detail, ow, oh, (float)ox + tx * s, (float)oy + ty * s);
+        const float dtm = bilinear(detail, ow, oh, (float)ox - tx * s, (float)oy - ty * s);
+        const float dt = 0.25f * dtp + 0.5f * dc + 0.25f * dtm;
+        const float det = dc + (dt - dc) * coh;
+        const float cap = 0.5f * bilinear_fast(m->rng, cw, ch, cxi, cxf, cyi, cyf);
+        const float cu = clampf(g * au * det, -cap, cap);
+        const float cv = clampf(g * av * det, -cap, cap);
+        float ou = out_u[idx] + cu, ov = out_v[idx] + cv;
+        if (clamp_hull) {
+            const float bu = out_u[idx], bv = out_v[idx];
+            const float lou = minf(bu, bilinear_fast(m->mnu, cw, ch, cxi, cxf, cyi, cyf) - ar);
+            const float hiu = maxf(bu, bilinear_fast(m->mxu, cw, ch, cxi, cxf, cyi, cyf) + ar);
+            const float lov = minf(bv, bilinear_fast(m->mnv, cw, ch, cxi, cxf, cyi, cyf) - ar);
+            const float hiv = maxf(bv, bilinear_fast(m->mxv, cw, ch, cxi, cxf, cyi, cyf) + ar);
+            ou = clampf(ou, lou, hiu);
+            ov = clampf(ov, lov, hiv);
+        }
+        out_u[idx] = ou;
+        out_v[idx] = ov;
+    }
+}
+
+/* algo 6 orchestration (plugin.cpp:573-626) on a plain-filled output */
+static void recon_algo6(const or_params *p, or_debug *D, const or_axis *ax, const or_axis *ay,
+                        float *out_u, float *out_v) {
+    const int cw = D->cw, ch = D->ch;
+    const size_t nc = (size_t)cw * ch;
+    float *buf = (float *)malloc(sizeof(float) * nc * 11);
+    affine_maps m = { buf, buf + nc, buf + 2 * nc, buf + 3 * nc, buf + 4 * nc, buf + 5 * nc,
+                      buf + 6 * nc, buf + 7 * nc, buf + 8 * nc };
+    float *yc1 = buf + 9 * nc, *yc2 = buf + 10 * nc;
+    yc_filtered(D->y, D->sw, D->sh, cw, ch, p->shift_x, p->shift_y, yc1, yc2);
+    const float *yc[3] = { D->lc, yc1, yc2 };
+    const int sparse_roi = D->has_ws && !or_workset_chroma_dense(&D->ws);
+    affine_maps_build(p, D->u, D->v, cw, ch, yc, D->has_ws ? D->ws.chroma_mask : NULL, sparse_roi, &m);
+    float *detail = (float *)malloc(sizeof(float) * (size_t)D->ow * D->oh);
+    detail_map(p, D->y, D->sw, D->sh, m.deg, cw, ch, detail);
+    detail_transfer(p, D, &m, detail, ax, ay, out_u, out_v);
+    free(detail);
+    free(buf);
+}
+
+/* ------------------------------------------------------------------------ */
 /* whole-frame orchestration: plugin.cpp:177-721                             */
 /* ------------------------------------------------------------------------ */
 
@@ -1631,8 +1950,6 @@ int or_recon_frame(const or_params *pin, const or_frame_format *f,
     if (f->ss_h == 0)
         pl.shift_y = 0.0;
     const or_params *p = &pl;
-    if (p->algo == 6 && p->strength > 0.0)
-        return -2; /* algo 6 restated separately */
     if (p->bm)
         return -2;
     const int sw = f->width, sh = f->height;
@@ -1713,6 +2030,10 @@ int or_recon_frame(const or_params *pin, const or_frame_format *f,
         or_plain_chroma(p, D->u, D->v, cw, ch, sw, sh, D->out_u, D->out_v);
         if (p->strength > 0.0)
             recon_algo4(p, D, &ax, &ay, is_float, D->out_u, D->out_v);
+    } else if (p->algo == 6) {
+        or_plain_chroma(p, D->u, D->v, cw, ch, sw, sh, D->out_u, D->out_v);
+        if (p->strength > 0.0)
+            recon_algo6(p, D, &ax, &ay, D->out_u, D->out_v);
     } else if (p->strength == 0.0) {
         or_plain_chroma(p, D->u, D->v, cw, ch, sw, sh, D->out_u, D->out_v);
     } else {

@@ -103,6 +103,21 @@ CASES = [
     ("p8-420-algo4-compressed", 192, 96, 1, 1, 8, {"algo": 4, "poly": 2}),
     ("ps-422-algo4-jinc", 96, 64, 1, 0, 32, {"algo": 4, "kernel": "jinc"}),
     ("p8-420-algo4-upscale", 96, 64, 1, 1, 8, {"algo": 4, "width": 144, "height": 96}),
+    # algo 6 detail transfer: dense and sparse affine ROI, resize via source grid
+    ("p8-420-algo6", 96, 64, 1, 1, 8, {"algo": 6}),
+    ("p10-420-algo6-bp", 96, 64, 1, 1, 10, {"algo": 6, "bp": 0.5}),
+    ("ps-420-algo6", 96, 64, 1, 1, 32, {"algo": 6}),
+    ("ps-420-algo6-dense", 96, 64, 1, 1, 32, {"algo": 6, "sparse": 0}),
+    ("ps-420-algo6-sparse-roi", 192, 96, 1, 1, 32, {"algo": 6, "poly": 2}),
+    ("p8-420-algo6-bilinear", 96, 64, 1, 1, 8, {"algo": 6, "kernel": "bilinear"}),
+    ("p8-420-algo6-jinc", 96, 64, 1, 1, 8, {"algo": 6, "kernel": "jinc"}),
+    ("p8-422-algo6", 96, 64, 1, 0, 8, {"algo": 6}),
+    ("ps-444-algo6", 96, 64, 0, 0, 32, {"algo": 6}),
+    ("ps-420-algo6-qgate", 96, 64, 1, 1, 32, {"algo": 6, "qgate": 0.3, "ms": 0.5}),
+    ("ps-420-resize-jinc3-algo6", 120, 68, 1, 1, 32, {"algo": 6, "kernel": "jinc", "width": 240, "height": 136}),
+    ("ps-420-resize-lanczos4-algo6", 120, 68, 1, 1, 32,
+     {"algo": 6, "kernel": "lanczos", "taps": 4, "width": 240, "height": 136}),
+    ("p8-420-downscale-algo6", 96, 64, 1, 1, 8, {"algo": 6, "width": 72, "height": 48}),
 ]
 
 
@@ -134,7 +149,18 @@ def test_config2_frame_1080p_p10(device):
         clip.close()
 
 
-@pytest.mark.parametrize("algo", [2, 4])
+@pytest.mark.parametrize("kernel,taps", [("jinc", 3), ("lanczos", 4)])
+def test_config3_resize_algo6_float(device, kernel, taps):
+    """YUV420PS -> 2x YUV444PS with algo 6 (the 1080p -> 2160p workload at
+    480x270 so the oracle finishes quickly)."""
+    y, u, v = frame(480, 270, bits=32, seed=2)
+    o_out, _, g_out, clip = run_both(y, u, v, device=device, algo=6, kernel=kernel, taps=taps, width=960,
+                                     height=540)
+    clip.close()
+    assert _report(f"resize-{kernel}{taps}-algo6", o_out, g_out, True) == 0
+
+
+@pytest.mark.parametrize("algo", [2, 4, 6])
 def test_config2_algo_1080p_p10_float_stage(device, algo):
     """1080p YUV420P10 with bp 0.5 for algo 2 and 4: float planes before
     packing within 1e-5, packed planes within 1 code."""

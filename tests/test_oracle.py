@@ -457,3 +457,38 @@ def test_algo4_constant_chroma_preserved():
     v = np.full((24, 32), np.float32(-0.1))
     out, _ = _recon(y, u, v, algo=4)
     assert np.max(np.abs(out[1] - np.float32(0.1))) <= 1e-6 and np.max(np.abs(out[2] + np.float32(0.1))) <= 1e-6
+
+
+# --------------------------------------------------------------------------
+# algo 6 (affine credibility + detail transfer)
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("kw", [{}, {"kernel": "jinc"}, {"kernel": "bilinear"}])
+@pytest.mark.parametrize("poly", [2, 10])
+def test_algo6_sparse_equals_dense(kw, poly):
+    # test_backend_matrix.py:104-111 and 132-138 (algo 6 sparse ROI)
+    y, u, v = frame(192, 96, bits=32, seed=16, polygons=poly)
+    dense, _ = _recon(y, u, v, algo=6, sparse=0, **kw)
+    sparse, dbg = _recon(y, u, v, algo=6, sparse=1, **kw)
+    assert max(float(np.max(np.abs(dense[p] - sparse[p]))) for p in (1, 2)) <= 1e-6
+
+
+def test_algo6_flat_luma_keeps_plain():
+    # no luma structure: the detail field is zero, algo 6 returns the base
+    y = np.full((48, 64), np.float32(0.4))
+    rng = np.random.default_rng(17)
+    u = (rng.random((24, 32), dtype=np.float32) - 0.5).astype(np.float32)
+    v = (rng.random((24, 32), dtype=np.float32) - 0.5).astype(np.float32)
+    base, _ = _recon(y, u, v, strength=0.0)
+    out, _ = _recon(y, u, v, algo=6, sparse=0)
+    assert np.max(np.abs(out[1] - base[1])) <= 1e-7 and np.max(np.abs(out[2] - base[2])) <= 1e-7
+
+
+def test_algo6_strength_continuity():
+    y, u, v = frame(64, 48, bits=32, seed=18, polygons=8)
+    zero, _ = _recon(y, u, v, algo=6, strength=0.0, sparse=0)
+    tiny, _ = _recon(y, u, v, algo=6, strength=1e-6, sparse=0)
+    full, _ = _recon(y, u, v, algo=6, strength=1.0, sparse=0)
+    step = max(float(np.max(np.abs(zero[p] - tiny[p]))) for p in (1, 2))
+    total = max(float(np.max(np.abs(zero[p] - full[p]))) for p in (1, 2))
+    assert total > 1e-4 and step <= total * 3e-6 + 2e-7

@@ -131,6 +131,24 @@ hipError_t launch_roi_mark(const uint32_t *row_offsets, const uint32_t *indices,
 hipError_t launch_blend4(const uint32_t *row_offsets, const uint32_t *indices, const float *y,
                          int sw, int sh, const AxisDev &ax, const AxisDev &ay, int cw, int ch,
                          const SelectorDev &sel, float *out_u, float *out_v, hipStream_t s);
+// algo 6 (algos.cpp:809-1698)
+struct AffineDev {
+    float *au, *av, *g, *mnu, *mxu, *mnv, *mxv, *rng, *deg;  // chroma grid
+};
+hipError_t launch_affine(const float *u, const float *v, int cw, int ch, const float *yc0,
+                         const float *yc1, const float *yc2, const uint8_t *chroma_mask,
+                         const uint32_t *chroma_count, double eps, double eps_sig,
+                         const AffineDev &m, hipStream_t s);
+hipError_t launch_detail_sub(const float *y, float *rec, size_t n, hipStream_t s);
+hipError_t launch_nyquist(float *a, float *b, int w, int h, int xlevels, int ylevels,
+                          float **result, hipStream_t s);
+hipError_t launch_detail_transfer(const uint32_t *row_offsets, const uint32_t *indices,
+                                  const float *jxx, const float *jxy, const float *jyy, int sw,
+                                  int sh, const AxisDev &ax, const AxisDev &ay, int cw, int ch,
+                                  const AffineDev &m, const float *ms, const float *detail,
+                                  float ms_strength, float qgate, float strength, float stretch,
+                                  float ar, int clamp_hull, float *out_u, float *out_v,
+                                  hipStream_t s);
 hipError_t launch_back_project(float *out_u, float *out_v, int ow, int oh, const float *u,
                                const float *v, int cw, int ch, float bp, hipStream_t s);
 hipError_t launch_resample_h(const float *src, int sw, int sh, float *dst,

@@ -1107,6 +1107,160 @@ __global__ __launch_bounds__(kBlock) void k_blend4(WorkList L, const float *y, i
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// algo 6: affine credibility maps (buildAffineMaps, algos.cpp:1382-1526),
+// Nyquist detail (algos.cpp:1139-1380), detail transfer (algos.cpp:1587-1689)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float median3(float a, float b, float c) {
+    const float mn = fmn(fmn(a, b), c), mx = fmx(fmx(a, b), c);
+    return a + b + c - mn - mx;
+}
+
+__device__ __forceinline__ double dmax0(double x) { return 0.0 < x ? x : 0.0; }
+
+__global__ __launch_bounds__(kBlock) void k_affine(const float *__restrict__ u, const float *__restrict__ v, int cw,
+                                                   int ch, const float *__restrict__ yc0,
+                                                   const float *__restrict__ yc1, const float *__restrict__ yc2,
+                                                   const uint8_t *cmask, const uint32_t *ccount, double eps,
+                                                   double eps_sig, AffineDev m) {
+    const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
+    if (x >= cw)
+        return;
+    const size_t n = (size_t)cw * ch, i = (size_t)y * cw + x;
+    m.deg[i] = median3(yc0[i], yc1[i], yc2[i]);
+    // sparse ROI: chroma mask present and not in dense fallback (algos.cpp:1414-1415)
+    const bool sparse = cmask && (uint64_t)ccount[0] * 2 < (uint64_t)n;
+    if (sparse && !cmask[i]) {
+        m.au[i] = m.av[i] = m.g[i] = m.mnu[i] = m.mxu[i] = m.mnv[i] = m.mxv[i] = m.rng[i] = 0.0f;
+        return;
+    }
+    const int top = max(0, y - 2), bot = min(ch - 1, y + 2), lef = max(0, x - 2), rig = min(cw - 1, x + 2);
+    const float *ycs[3] = {yc0, yc1, yc2};
+    double sum[16];
+    for (int k = 0; k < 16; ++k)
+        sum[k] = 0.0;
+    float hmnu = u[(size_t)top * cw + lef], hmxu = hmnu, hmnv = v[(size_t)top * cw + lef], hmxv = hmnv;
+    for (int py = top; py <= bot; ++py)
+        for (int px = lef; px <= rig; ++px) {
+            const size_t q = (size_t)py * cw + px;
+            const double a = u[q], b = v[q];
+            sum[0] += a; sum[1] += a * a; sum[2] += b; sum[3] += b * b;
+            hmnu = fmn(hmnu, (float)a); hmxu = fmx(hmxu, (float)a);
+            hmnv = fmn(hmnv, (float)b); hmxv = fmx(hmxv, (float)b);
+            for (int k = 0; k < 3; ++k) {
+                const double l = ycs[k][q];
+                sum[4 + 4 * k] += l;
+                sum[5 + 4 * k] += l * l;
+                sum[6 + 4 * k] += l * a;
+                sum[7 + 4 * k] += l * b;
+            }
+        }
+    const int cnt = (rig - lef + 1) * (bot - top + 1);
+    const double inv = 1.0 / cnt;
+    // sparse path multiplies by 1/n (algos.cpp:1082-1102); dense path divides
+    // (algos.cpp:1483-1497)
+    auto mean = [&](double s) { return sparse ? s * inv : s / cnt; };
+    const double meanU = mean(sum[0]), meanV = mean(sum[2]);
+    const double varU = dmax0(mean(sum[1]) - meanU * meanU);
+    const double varV = dmax0(mean(sum[3]) - meanV * meanV);
+    double qk[3], aUk[3], aVk[3];
+    for (int k = 0; k < 3; ++k) {
+        const int b = 4 + 4 * k;
+        const double meanY = mean(sum[b]);
+        const double varY = dmax0(mean(sum[b + 1]) - meanY * meanY);
+        const double covU = mean(sum[b + 2]) - meanY * meanU;
+        const double covV = mean(sum[b + 3]) - meanY * meanV;
+        aUk[k] = dclamp(covU / (varY + eps), -16.0, 16.0);
+        aVk[k] = dclamp(covV / (varY + eps), -16.0, 16.0);
+        qk[k] = dclamp((covU * covU + covV * covV) / ((varY + eps) * (varU + varV + eps)), 0.0, 1.0);
+    }
+    m.mnu[i] = hmnu; m.mxu[i] = hmxu; m.mnv[i] = hmnv; m.mxv[i] = hmxv;
+    m.rng[i] = fmx(hmxu - hmnu, hmxv - hmnv);
+    m.au[i] = median3((float)aUk[0], (float)aUk[1], (float)aUk[2]);
+    m.av[i] = median3((float)aVk[0], (float)aVk[1], (float)aVk[2]);
+    const double qmax = fmax(fmax(qk[0], qk[1]), qk[2]);
+    const double qmin = fmin(fmin(qk[0], qk[1]), qk[2]);
+    const double stab = qmax > 1e-9 ? qmin / qmax : 0.0;
+    const double sig = (varU + varV) / (varU + varV + eps_sig);
+    m.g[i] = (float)(((qk[0] + qk[1] + qk[2]) / 3.0) * stab * sig);
+}
+
+__global__ __launch_bounds__(kBlock) void k_detail_sub(const float *y, float *rec, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+        rec[i] = y[i] - rec[i];
+}
+
+// one dyadic [1/4, 1/2, 1/4] pass, edge clamped (algos.cpp:1195-1217, 1252-1278)
+__global__ __launch_bounds__(kBlock) void k_nyq(const float *src, float *dst, int w, int h, int step, int vertical) {
+    const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
+    if (x >= w)
+        return;
+    float a, b, c;
+    if (vertical) {
+        a = src[(size_t)max(0, y - step) * w + x];
+        b = src[(size_t)y * w + x];
+        c = src[(size_t)min(h - 1, y + step) * w + x];
+    } else {
+        const float *r = src + (size_t)y * w;
+        a = r[max(0, x - step)];
+        b = r[x];
+        c = r[min(w - 1, x + step)];
+    }
+    dst[(size_t)y * w + x] = 0.25f * a + 0.5f * b + 0.25f * c;
+}
+
+__global__ __launch_bounds__(kBlock) void k_detail_transfer(WorkList L, const float *jxx, const float *jxy,
+                                                            const float *jyy, int sw, int sh, AxisDev ax, AxisDev ay,
+                                                            int cw, int ch, AffineDev m, const float *ms,
+                                                            const float *detail, float mss, float qs, float st,
+                                                            float stretch, float ar, int clamp_hull, float *out_u,
+                                                            float *out_v) {
+    const uint32_t total = work_total(L);
+    const int ow = L.ow, oh = L.oh;
+    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < total; k += gridDim.x * kBlock) {
+        const uint32_t idx = L.indices ? L.indices[k] : k;
+        const int ox = (int)(idx % (uint32_t)ow), oy = (int)(idx / (uint32_t)ow);
+        const int lxi = ax.lb_i0[ox], lyi = ay.lb_i0[oy];
+        const float lxf = ax.lb_f[ox], lyf = ay.lb_f[oy];
+        const Dir d = principal_dir(bilinear_fast(jxx, sw, sh, lxi, lxf, lyi, lyf),
+                                    bilinear_fast(jxy, sw, sh, lxi, lxf, lyi, lyf),
+                                    bilinear_fast(jyy, sw, sh, lxi, lxf, lyi, lyf));
+        const int cxi = ax.cb_i0[ox], cyi = ay.cb_i0[oy];
+        const float cxf = ax.cb_f[ox], cyf = ay.cb_f[oy];
+        const float q = bilinear_fast(m.g, cw, ch, cxi, cxf, cyi, cyf);
+        float g = (1.0f - qs * (1.0f - q)) * st;
+        if (ms)
+            g *= 1.0f - mss * (1.0f - bilinear_fast(ms, cw, ch, cxi, cxf, cyi, cyf));
+        const float au = bilinear_fast(m.au, cw, ch, cxi, cxf, cyi, cyf);
+        const float av = bilinear_fast(m.av, cw, ch, cxi, cxf, cyi, cyf);
+        const float coh = d.coherence;
+        const float tx = -d.ny, ty = d.nx;
+        const float dc = detail[idx];
+        const float s = 1.0f + stretch * coh;
+        const float dtp = bilinear(detail, ow, oh, (float)ox + tx * s, (float)oy + ty * s);
+        const float dtm = bilinear(detail, ow, oh, (float)ox - tx * s, (float)oy - ty * s);
+        const float dt = 0.25f * dtp + 0.5f * dc + 0.25f * dtm;
+        const float det = dc + (dt - dc) * coh;
+        const float cap = 0.5f * bilinear_fast(m.rng, cw, ch, cxi, cxf, cyi, cyf);
+        const float cu = fclamp(g * au * det, -cap, cap);
+        const float cv = fclamp(g * av * det, -cap, cap);
+        const float bu = out_u[idx], bv = out_v[idx];
+        float ou = bu + cu, ov = bv + cv;
+        if (clamp_hull) {
+            const float lou = fmn(bu, bilinear_fast(m.mnu, cw, ch, cxi, cxf, cyi, cyf) - ar);
+            const float hiu = fmx(bu, bilinear_fast(m.mxu, cw, ch, cxi, cxf, cyi, cyf) + ar);
+            const float lov = fmn(bv, bilinear_fast(m.mnv, cw, ch, cxi, cxf, cyi, cyf) - ar);
+            const float hiv = fmx(bv, bilinear_fast(m.mxv, cw, ch, cxi, cxf, cyi, cyf) + ar);
+            ou = fclamp(ou, lou, hiu);
+            ov = fclamp(ov, lov, hiv);
+        }
+        out_u[idx] = ou;
+        out_v[idx] = ov;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // back-projection (backProject, algos.cpp:663-677); the reference applies it
 // only when the output is exactly 2x the chroma grid, so the 2x2 blocks are
@@ -1361,6 +1515,47 @@ hipError_t launch_blend4(const uint32_t *row_offsets, const uint32_t *indices, c
     const WorkList L{row_offsets, indices, ax.n, ay.n};
     hipLaunchKernelGGL(k_blend4, dim3((unsigned)std::min<size_t>(8192, (n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                        s, L, y, sw, sh, ax, ay, cw, ch, sel, out_u, out_v);
+    return hipGetLastError();
+}
+
+hipError_t launch_affine(const float *u, const float *v, int cw, int ch, const float *yc0, const float *yc1,
+                         const float *yc2, const uint8_t *chroma_mask, const uint32_t *chroma_count, double eps,
+                         double eps_sig, const AffineDev &m, hipStream_t s) {
+    hipLaunchKernelGGL(k_affine, rows_grid(cw, ch), dim3(kBlock), 0, s, u, v, cw, ch, yc0, yc1, yc2, chroma_mask,
+                       chroma_count, eps, eps_sig, m);
+    return hipGetLastError();
+}
+
+hipError_t launch_detail_sub(const float *y, float *rec, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_detail_sub, dim3((unsigned)std::min<size_t>(4096, (n + kBlock - 1) / kBlock)), dim3(kBlock),
+                       0, s, y, rec, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_nyquist(float *a, float *b, int w, int h, int xlevels, int ylevels, float **result, hipStream_t s) {
+    float *cur = a, *nxt = b;
+    for (int pass = 0; pass < xlevels; ++pass) {
+        hipLaunchKernelGGL(k_nyq, rows_grid(w, h), dim3(kBlock), 0, s, cur, nxt, w, h, 1 << pass, 0);
+        std::swap(cur, nxt);
+    }
+    for (int pass = 0; pass < ylevels; ++pass) {
+        hipLaunchKernelGGL(k_nyq, rows_grid(w, h), dim3(kBlock), 0, s, cur, nxt, w, h, 1 << pass, 1);
+        std::swap(cur, nxt);
+    }
+    *result = cur;
+    return hipGetLastError();
+}
+
+hipError_t launch_detail_transfer(const uint32_t *row_offsets, const uint32_t *indices, const float *jxx,
+                                  const float *jxy, const float *jyy, int sw, int sh, const AxisDev &ax,
+                                  const AxisDev &ay, int cw, int ch, const AffineDev &m, const float *ms,
+                                  const float *detail, float ms_strength, float qgate, float strength, float stretch,
+                                  float ar, int clamp_hull, float *out_u, float *out_v, hipStream_t s) {
+    const size_t n = (size_t)ax.n * ay.n;
+    const WorkList L{row_offsets, indices, ax.n, ay.n};
+    hipLaunchKernelGGL(k_detail_transfer, dim3((unsigned)std::min<size_t>(8192, (n + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, s, L, jxx, jxy, jyy, sw, sh, ax, ay, cw, ch, m, ms, detail, ms_strength,
+                       qgate, strength, stretch, ar, clamp_hull, out_u, out_v);
     return hipGetLastError();
 }
 

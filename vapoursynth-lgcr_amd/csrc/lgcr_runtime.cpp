@@ -345,6 +345,12 @@ struct Geometry {
     // point-sampled luma at chroma sites (algos.cpp:300-315)
     const int *pl_xi = nullptr, *pl_yi = nullptr;
     const float *pl_xf = nullptr, *pl_yf = nullptr;
+    // algo 6: bilinear / bicubic(0, 0.6) candidate downsamples of luma to the
+    // chroma grid (maps.cpp:529-680) and the source-grid axes of the detail
+    // reconstruction (algos.cpp:1554-1563)
+    WeightsDev yc_bx, yc_by, yc_cx, yc_cy;
+    AxisDev sax, say;
+    RadialDev srd;
 };
 
 struct Workspace {
@@ -364,6 +370,9 @@ struct Workspace {
     // algo 4
     float *ls = nullptr, *mm = nullptr, *lgf = nullptr, *sel = nullptr, *selp = nullptr;
     uint8_t *roi = nullptr;
+    // algo 6
+    float *yct = nullptr, *yc1 = nullptr, *yc2 = nullptr, *aff = nullptr, *rec = nullptr, *rec2 = nullptr,
+          *nyq = nullptr, *detail_out = nullptr;
     // stage timeline of frames enqueued with timing on: (stage, event);
     // stage -1 opens a frame.  Events come from a per-workspace pool.
     std::vector<std::pair<int, hipEvent_t>> marks;
@@ -451,6 +460,38 @@ lgcr_status get_geometry(lgcr_clip *c, double sx, double sy, Geometry **out) {
     if (c->p.radial)
         hr = build_radial(ax, ay, c->cw, c->ch, c->lut);
     const size_t orx = blob.add(hr.xphase), ory = blob.add(hr.yphase), orw = blob.add(hr.w);
+    const bool a6 = c->p.algo == 6 && c->p.strength > 0.0;
+    HostWeights ycw[4];
+    size_t oyc[4][2] = {};
+    if (a6) {
+        ycw[0] = build_weights(c->sw, c->cw, LGCR_KERNEL_BILINEAR, 0.0, 0.0, 1.0, sx);
+        ycw[1] = build_weights(c->sh, c->ch, LGCR_KERNEL_BILINEAR, 0.0, 0.0, 1.0, sy);
+        ycw[2] = build_weights(c->sw, c->cw, LGCR_KERNEL_BICUBIC, 0.0, 0.6, 2.0, sx);
+        ycw[3] = build_weights(c->sh, c->ch, LGCR_KERNEL_BICUBIC, 0.0, 0.6, 2.0, sy);
+        for (int k = 0; k < 4; ++k) {
+            oyc[k][0] = blob.add(ycw[k].start);
+            oyc[k][1] = blob.add(ycw[k].w);
+        }
+    }
+    const bool src_axes = a6 && !c->same;
+    HostAxis sax, say;
+    HostRadial shr;
+    size_t os[2][11] = {};
+    if (src_axes) {
+        sax = build_axis(c->sw, c->sw, c->rw, sx, c->p);
+        say = build_axis(c->sh, c->sh, c->rh, sy, c->p);
+        if (c->p.radial)
+            shr = build_radial(sax, say, c->cw, c->ch, c->lut);
+        const HostAxis *sa[2] = {&sax, &say};
+        for (int a = 0; a < 2; ++a) {
+            const HostAxis &h = *sa[a];
+            os[a][0] = blob.add(h.start); os[a][1] = blob.add(h.w); os[a][2] = blob.add(h.am);
+            os[a][3] = blob.add(h.wsum); os[a][4] = blob.add(h.wabs); os[a][5] = blob.add(h.pos);
+            os[a][6] = blob.add(h.lpos); os[a][7] = blob.add(h.cb_i0); os[a][8] = blob.add(h.cb_f);
+            os[a][9] = blob.add(h.lb_i0); os[a][10] = blob.add(h.lb_f);
+        }
+    }
+    const size_t osrx = blob.add(shr.xphase), osry = blob.add(shr.yphase), osrw = blob.add(shr.w);
     auto g = std::make_unique<Geometry>();
     HIP_TRY(hipSetDevice(c->dev->ordinal));
     HIP_TRY(hipMalloc(&g->dev, blob.host.size()));
@@ -494,6 +535,47 @@ lgcr_status get_geometry(lgcr_clip *c, double sx, double sy, Geometry **out) {
     g->rd.lut_scale = 1024.0f;
     g->rd.inv_wx = float(std::min(1.0, double(c->ow) / c->cw));
     g->rd.inv_wy = float(std::min(1.0, double(c->oh) / c->ch));
+    if (a6) {
+        WeightsDev *wd[4] = {&g->yc_bx, &g->yc_by, &g->yc_cx, &g->yc_cy};
+        for (int k = 0; k < 4; ++k)
+            *wd[k] = WeightsDev{ycw[k].n, ycw[k].sup, reinterpret_cast<const int *>(base + oyc[k][0]),
+                                reinterpret_cast<const float *>(base + oyc[k][1])};
+    }
+    if (src_axes) {
+        AxisDev *sd[2] = {&g->sax, &g->say};
+        const HostAxis *sa[2] = {&sax, &say};
+        for (int a = 0; a < 2; ++a) {
+            AxisDev &d = *sd[a];
+            d.n = sa[a]->n;
+            d.sup = sa[a]->sup;
+            d.start = reinterpret_cast<const int *>(base + os[a][0]);
+            d.w = reinterpret_cast<const float *>(base + os[a][1]);
+            d.am = reinterpret_cast<const float *>(base + os[a][2]);
+            d.wsum = reinterpret_cast<const float *>(base + os[a][3]);
+            d.wabs = reinterpret_cast<const float *>(base + os[a][4]);
+            d.pos = reinterpret_cast<const float *>(base + os[a][5]);
+            d.lpos = reinterpret_cast<const float *>(base + os[a][6]);
+            d.cb_i0 = reinterpret_cast<const int *>(base + os[a][7]);
+            d.cb_f = reinterpret_cast<const float *>(base + os[a][8]);
+            d.lb_i0 = reinterpret_cast<const int *>(base + os[a][9]);
+            d.lb_f = reinterpret_cast<const float *>(base + os[a][10]);
+        }
+        g->srd = g->rd;
+        g->srd.valid = shr.valid ? 1 : 0;
+        g->srd.sup_x = sax.sup;
+        g->srd.sup_y = say.sup;
+        g->srd.nx = shr.nx;
+        g->srd.ny = shr.ny;
+        g->srd.xphase = reinterpret_cast<const uint16_t *>(base + osrx);
+        g->srd.yphase = reinterpret_cast<const uint16_t *>(base + osry);
+        g->srd.w = reinterpret_cast<const float *>(base + osrw);
+        g->srd.inv_wx = float(std::min(1.0, double(c->sw) / c->cw));
+        g->srd.inv_wy = float(std::min(1.0, double(c->sh) / c->ch));
+    } else {
+        g->sax = g->ax;
+        g->say = g->ay;
+        g->srd = g->rd;
+    }
     *out = g.get();
     c->geos.emplace(key, std::move(g));
     return LGCR_OK;
@@ -527,7 +609,7 @@ void layout(const lgcr_clip *c, Workspace *w, Carver &cv) {
     w->lc = cv.take<float>(nc);
     w->ms = cv.take<float>(nc);
     w->grad = cv.take<float>(6 * nc);
-    w->hbuf = cv.take<float>(6 * size_t(c->ow) * c->ch);
+    w->hbuf = cv.take<float>(6 * size_t(std::max(c->ow, c->sw)) * c->ch);
     w->out_u = cv.take<float>(no);
     w->out_v = cv.take<float>(no);
     w->plain_u = cv.take<float>(no);
@@ -550,6 +632,17 @@ void layout(const lgcr_clip *c, Workspace *w, Carver &cv) {
         w->roi = cv.take<uint8_t>(nc);
         w->sel = cv.take<float>(3 * no);
         w->selp = cv.take<float>(4 * no);
+    }
+    if (c->p.algo == 6 && c->p.strength > 0.0) {
+        w->yct = cv.take<float>(size_t(c->cw) * c->sh);
+        w->yc1 = cv.take<float>(nc);
+        w->yc2 = cv.take<float>(nc);
+        w->aff = cv.take<float>(9 * nc);
+        w->rec = cv.take<float>(ns);
+        w->rec2 = cv.take<float>(ns);
+        w->nyq = cv.take<float>(ns);
+        if (!c->same)
+            w->detail_out = cv.take<float>(no);
     }
 }
 
@@ -745,7 +838,7 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
                                       clamp_hull, margin, s);
     };
     const bool algo4 = p.algo == 4 && guided;
-    if (!guided || w->has_ws || algo4) {
+    if (!guided || w->has_ws || algo4 || p.algo == 6) {
         // BuildBaseChroma (plugin.cpp:396-402, 627-648)
         HIP_TRY(plain());
         if (c->capture && guided) {
@@ -755,7 +848,7 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
         }
         HIP_TRY(mark(LGCR_STAGE_BUILD_BASE_CHROMA));
     }
-    if (guided) {
+    if (guided && p.algo != 6) {
         // ApplyGuidedCorrection (plugin.cpp:649-674); algo 4 runs the guided
         // pass at strength 1 with the selector (plugin.cpp:427-440, 452-490)
         GuideParams gp;
@@ -819,6 +912,58 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
         }
         (void)nc;
     }
+    if (p.algo == 6 && guided) {
+        // BuildAffineMaps (plugin.cpp:581-591, algos.cpp:1382-1526)
+        HIP_TRY(launch_resample_h(w->y, c->sw, c->sh, w->yct, g.yc_bx, s));
+        HIP_TRY(launch_resample_v(w->yct, c->cw, c->sh, w->yc1, g.yc_by, s));
+        HIP_TRY(launch_resample_h(w->y, c->sw, c->sh, w->yct, g.yc_cx, s));
+        HIP_TRY(launch_resample_v(w->yct, c->cw, c->sh, w->yc2, g.yc_cy, s));
+        const size_t nc = size_t(c->cw) * c->ch, ns = size_t(c->sw) * c->sh;
+        AffineDev af{w->aff, w->aff + nc, w->aff + 2 * nc, w->aff + 3 * nc, w->aff + 4 * nc,
+                     w->aff + 5 * nc, w->aff + 6 * nc, w->aff + 7 * nc, w->aff + 8 * nc};
+        const double eps_sig = (0.25 * p.sigma) * (0.25 * p.sigma);
+        HIP_TRY(launch_affine(w->u, w->v, c->cw, c->ch, w->lc, w->yc1, w->yc2, w->has_ws ? w->cmask : nullptr,
+                              w->counters, p.reg * p.reg, eps_sig, af, s));
+        HIP_TRY(mark(LGCR_STAGE_BUILD_AFFINE_MAPS));
+        // BuildDetail (algos.cpp:1528-1585): detail = Y - P(D(Y)) on the source
+        // grid, Nyquist passes, then resampled to the output grid
+        if (p.radial)
+            HIP_TRY(launch_plain_radial(af.deg, af.deg, c->cw, c->ch, g.sax, g.say, g.srd, w->rec, w->rec2,
+                                        clamp_hull, margin, s));
+        else if (p.kernel == LGCR_KERNEL_BILINEAR)
+            HIP_TRY(launch_plain_bilinear(af.deg, af.deg, c->cw, c->ch, g.sax, g.say, w->rec, w->rec2, s));
+        else
+            HIP_TRY(launch_plain_separable(af.deg, af.deg, c->cw, c->ch, g.sax, g.say, w->hbuf, w->rec, w->rec2,
+                                           clamp_hull, margin, s));
+        HIP_TRY(launch_detail_sub(w->y, w->rec, ns, s));
+        auto levels = [](double ratio) {
+            int l = 0;
+            while (ratio > 1.5 && l < 4) {
+                ratio *= 0.5;
+                ++l;
+            }
+            return l;
+        };
+        float *det = nullptr;
+        HIP_TRY(launch_nyquist(w->rec, w->nyq, c->sw, c->sh, levels(c->rw), levels(c->rh), &det, s));
+        if (!c->same) {
+            if (p.radial) {
+                HIP_TRY(launch_resample_radial(det, c->sw, c->sh, w->detail_out, c->ow, c->oh, p.support,
+                                               static_cast<const float *>(c->lut_dev), c->lut_n, 1024.0f, s));
+            } else {
+                HIP_TRY(launch_resample_h(det, c->sw, c->sh, w->luma_tmp, c->wh, s));
+                HIP_TRY(launch_resample_v(w->luma_tmp, c->ow, c->sh, w->detail_out, c->wv, s));
+            }
+            det = w->detail_out;
+        }
+        HIP_TRY(mark(LGCR_STAGE_BUILD_DETAIL));
+        // ApplyDetailTransfer (plugin.cpp:607-625)
+        HIP_TRY(launch_detail_transfer(w->row_off, w->has_ws ? w->indices : nullptr, w->jxx, w->jxy, w->jyy, c->sw,
+                                       c->sh, g.ax, g.ay, c->cw, c->ch, af, p.ms > 0.0 ? w->ms : nullptr, det,
+                                       float(p.ms), float(p.qgate), float(p.strength), float(p.stretch), margin,
+                                       clamp_hull, w->out_u, w->out_v, s));
+        HIP_TRY(mark(LGCR_STAGE_APPLY_DETAIL_TRANSFER));
+    }
     // BackProject, same-size 4:2:0 only (plugin.cpp:374-375, 683-689)
     if (p.bp > 0.0 && c->same && c->cw * 2 == c->sw && c->ch * 2 == c->sh) {
         HIP_TRY(launch_back_project(w->out_u, w->out_v, c->ow, c->oh, w->u, w->v, c->cw, c->ch, float(p.bp), s));
@@ -856,8 +1001,6 @@ lgcr_status validate(const lgcr_params &p, const lgcr_format &f) {
         return fail(LGCR_INVALID_ARGUMENT, "LGCR: taps must be in the range 1..64");
     if (p.kernel < 0 || p.kernel > LGCR_KERNEL_JINC)
         return fail(LGCR_INVALID_ARGUMENT, "LGCR: unknown kernel");
-    if (p.strength > 0.0 && p.algo == 6)
-        return fail(LGCR_UNSUPPORTED, "LGCR HIP backend: algo %d is not available in this build", p.algo);
     if (p.bm)
         return fail(LGCR_UNSUPPORTED, "LGCR HIP backend: bm=True is not available in this build");
     return LGCR_OK;
