@@ -167,6 +167,21 @@ lgcr_status lgcr_recon_frame(lgcr_clip *clip, const lgcr_frame *in, double shift
 lgcr_status lgcr_recon_batch(lgcr_clip *clip, int n, const lgcr_frame *in,
                              lgcr_frame *out);
 
+/* TRecon (tReconCreate / tReconGetFrame, src/plugin.cpp:740-990).  Params
+ * from lgcr_params_default_trecon (lanczos 3, algo 2, sparse 0; strength,
+ * sigma, sratio, sdb, gsigma, stretch, ar, ridge, sparse, ms, trad, tsearch,
+ * tsad and bm are the TRecon arguments).  The output is 4:4:4 at the input
+ * size with the luma plane rewritten (plugin.cpp:850-860).  The caller
+ * supplies the neighbour frames chosen by temporalFrameNumbers
+ * (plugin.cpp:729-738): n-trad..n+trad clamped to the clip, without the
+ * centre and duplicates; at most 2*trad of them. */
+void lgcr_params_default_trecon(lgcr_params *p);
+lgcr_status lgcr_trecon_create(lgcr_device *dev, const lgcr_params *p, const lgcr_format *fmt,
+                               int max_inflight, lgcr_clip **out);
+lgcr_status lgcr_trecon_frame(lgcr_clip *clip, const lgcr_frame *in, const lgcr_frame *nbrs,
+                              int n_nbrs, double shift_x, double shift_y, lgcr_frame *out,
+                              lgcr_stats *stats);
+
 /* Per-stage device time accumulated while timing is on: the nanosecond
  * counters of PipelineMetrics (src/pipeline.h:315-336), here measured with
  * HIP events recorded around each stage on the workspace stream that runs

@@ -88,6 +88,7 @@ class Guide(C.Structure):
         ("y", C.POINTER(C.c_float)), ("sw", C.c_int), ("sh", C.c_int),
         ("jxx", C.POINTER(C.c_float)), ("jxy", C.POINTER(C.c_float)), ("jyy", C.POINTER(C.c_float)),
         ("lc", C.POINTER(C.c_float)), ("ms", C.POINTER(C.c_float)),
+        ("nbrs", C.c_void_p), ("n_nbrs", C.c_int),
     ]
 
 
@@ -133,6 +134,12 @@ def lib():
         L.or_recon_frame.argtypes = [C.POINTER(Params), C.POINTER(FrameFormat), C.c_void_p * 3,
                                      C.c_ssize_t * 3, C.c_void_p * 3, C.c_ssize_t * 3]
         L.or_last_debug.restype = C.POINTER(Debug)
+        L.or_trecon_frame.argtypes = [C.POINTER(Params), C.POINTER(FrameFormat), C.c_void_p * 3, C.c_ssize_t * 3,
+                                      C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_ssize_t),
+                                      C.c_void_p * 3, C.c_ssize_t * 3]
+        i16p = C.POINTER(C.c_int16)
+        L.or_block_match.argtypes = [fp, fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, i16p, i16p, fp,
+                                     C.POINTER(C.c_int), C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -240,3 +247,44 @@ def last_debug():
         out["chroma_mask"] = arr(ws.chroma_mask, nc, np.uint8)
         out["active_chroma"] = ws.active_chroma
     return out
+
+
+def temporal_frame_numbers(n, n_last, trad):
+    """Neighbour frames of TRecon: n-trad..n+trad clamped to the clip,
+    centre and duplicates removed (plugin.cpp:729-738)."""
+    out = []
+    for k in range(-trad, trad + 1):
+        fn = min(max(n + k, 0), n_last)
+        if fn != n and fn not in out:
+            out.append(fn)
+    return out
+
+
+def trecon_params(strength=0.8, sigma=0.01, sratio=0.15, sdb=3.0, gsigma=2.5, stretch=1.0, ar=0.0,
+                  ridge=1, sparse=0, ms=1.0, trad=1, tsearch=6, tsad=0.02, bm=0, shift_x=-0.5, shift_y=0.0):
+    """TRecon parameters (plugin.cpp:939-976): lanczos 3, algo 2."""
+    return make_params(0, 0, kernel="lanczos", taps=3, strength=strength, sigma=sigma, sratio=sratio, sdb=sdb,
+                       gsigma=gsigma, stretch=stretch, ar=ar, ridge=ridge, sparse=sparse, ms=ms, trad=trad,
+                       tsearch=tsearch, tsad=tsad, bm=bm, shift_x=shift_x, shift_y=shift_y)
+
+
+def trecon_frame(params, center, nbrs, bits=None, ss_w=1, ss_h=1):
+    """center: (y, u, v); nbrs: list of (y, u, v) neighbour frames."""
+    y, u, v = center
+    is_float = y.dtype == np.float32
+    if bits is None:
+        bits = 32 if is_float else (8 if y.dtype == np.uint8 else 16)
+    fmt = FrameFormat(1 if is_float else 0, bits, y.dtype.itemsize, ss_w, ss_h, y.shape[1], y.shape[0])
+    h, w = y.shape
+    outs = [np.zeros((h, w), y.dtype) for _ in range(3)]
+    ins = [np.ascontiguousarray(a) for a in center]
+    nb = [np.ascontiguousarray(a) for f in nbrs for a in f]
+    nptr = (C.c_void_p * max(1, len(nb)))(*[a.ctypes.data for a in nb])
+    nstr = (C.c_ssize_t * max(1, len(nb)))(*[a.strides[0] for a in nb])
+    rc = lib().or_trecon_frame(C.byref(params), C.byref(fmt), (C.c_void_p * 3)(*[a.ctypes.data for a in ins]),
+                               (C.c_ssize_t * 3)(*[a.strides[0] for a in ins]), len(nbrs), nptr, nstr,
+                               (C.c_void_p * 3)(*[a.ctypes.data for a in outs]),
+                               (C.c_ssize_t * 3)(*[a.strides[0] for a in outs]))
+    if rc != 0:
+        raise RuntimeError(f"oracle trecon_frame failed ({rc})")
+    return outs

@@ -1116,6 +1116,66 @@ static void guided_pixel(const or_params *p, const float *U, const float *V, int
             sum_abs += fabsf(w);
         }
     }
+    /* temporal taps of TRecon (recon.cpp:642-745) */
+    if (guided && g->n_nbrs > 0) {
+        const int lxi = clampi((int)ax->lpos[ox], 0, g->sw - 1);
+        const int lyi = clampi((int)ay->lpos[oy], 0, g->sh - 1);
+        for (int q = 0; q < g->n_nbrs; ++q) {
+            const or_nbr *nb = &g->nbrs[q];
+            const int bxx = mini(nb->bw - 1, lxi / nb->block);
+            const int byy = mini(nb->bh - 1, lyi / nb->block);
+            const size_t bi = (size_t)byy * nb->bw + bxx;
+            const float tconf = nb->conf[bi];
+            if (tconf < 0.05f)
+                continue;
+            const float mx = (float)nb->mvx[bi], my = (float)nb->mvy[bi];
+            const int tnx = clampi((int)lroundf(scx + mx / frw), 0, cw - 1);
+            const int tny = clampi((int)lroundf(scy + my / frh), 0, ch - 1);
+            const float ddx = ((float)tnx - scx) * frw - mx;
+            const float ddy = ((float)tny - scy) * frh - my;
+            const int stat = (fabsf(mx) + fabsf(my) <= 1.0f) && tconf > 0.5f;
+            float dL;
+            if (stat) {
+                dL = at(nb->lc, cw, ch, tnx, tny) - L0;
+            } else {
+                const double lx = (tnx + 0.5) * rw - 0.5 + p->shift_x - mx;
+                const double ly = (tny + 0.5) * rh - 0.5 + p->shift_y - my;
+                const int x0 = (int)ceil(lx - 0.5 * rw), x1 = (int)ceil(lx + 0.5 * rw);
+                const int y0 = (int)ceil(ly - 0.5 * rh), y1 = (int)ceil(ly + 0.5 * rh);
+                float md = 0.0f;
+                for (int jj = y0; jj < y1; ++jj)
+                    for (int ii = x0; ii < x1; ++ii)
+                        md = maxf(md, fabsf(at(g->y, g->sw, g->sh, ii, jj) - L0));
+                dL = md;
+            }
+            const float sig_t = stat ? sig_loc : 0.5f * sig_loc;
+            const float sim_raw = 1.0f / (1.0f + dL * dL / (sig_t * sig_t));
+            const float sim = stat ? 1.0f - guide_fade * (1.0f - sim_raw) : sim_raw;
+            float kw;
+            if (p->radial) {
+                const float dxc = ddx / frw * inv_wxc, dyc = ddy / frh * inv_wyc;
+                kw = or_lut_lookup(p->lut, p->lut_n, sqrtf(dxc * dxc + dyc * dyc) * p->lut_scale);
+            } else {
+                kw = (float)(or_kernel_eval(p->kernel, fabsf(ddx) / rw * inv_wxc, p->kp1, p->kp2) *
+                             or_kernel_eval(p->kernel, fabsf(ddy) / rh * inv_wyc, p->kp1, p->kp2));
+            }
+            float nw = 1.0f;
+            if (!stat) {
+                const float fx = mx / frw, fy = my / frh;
+                const float px = fx - floorf(fx), py = fy - floorf(fy);
+                nw = maxf(fabsf(sinf((float)M_PI * px)), fabsf(sinf((float)M_PI * py)));
+                nw *= 1.0f - ss_ramp;
+            }
+            const float w = tconf * sim * kw * nw;
+            const float nu = at(nb->u, cw, ch, tnx, tny), nv = at(nb->v, cw, ch, tnx, tny);
+            acc_u += (double)w * nu;
+            acc_v += (double)w * nv;
+            wsum += w;
+            sum_abs += fabsf(w);
+            mnu = minf(mnu, nu); mxu = maxf(mxu, nu);
+            mnv = minf(mnv, nv); mxv = maxf(mxv, nv);
+        }
+    }
     double vu, vv;
     if (wsum > 1e-6 && wsum >= 0.05 * sum_abs) {
         vu = acc_u / wsum;
@@ -1412,7 +1472,7 @@ static void recon_algo4(const or_params *p, or_debug *D, const or_axis *ax, cons
     lgf_maps m;
     float *lgf = (float *)malloc(sizeof(float) * nc * 6);
     m.au = lgf; m.bu = lgf + nc; m.cu = lgf + 2 * nc; m.av = lgf + 3 * nc; m.bv = lgf + 4 * nc; m.cv = lgf + 5 * nc;
-    or_guide g = { D->y, D->sw, D->sh, D->jxx, D->jxy, D->jyy, D->lc, D->ms };
+    or_guide g = { D->y, D->sw, D->sh, D->jxx, D->jxy, D->jyy, D->lc, D->ms, NULL, 0 };
     const or_workset *ws = D->has_ws ? &D->ws : NULL;
     const float lam = (float)p->strength;
     if (!is_float) {
@@ -2044,7 +2104,7 @@ int or_recon_frame(const or_params *pin, const or_frame_format *f,
             memcpy(D->plain_u, D->out_u, sizeof(float) * no);
             memcpy(D->plain_v, D->out_v, sizeof(float) * no);
         }
-        or_guide g = { D->y, sw, sh, D->jxx, D->jxy, D->jyy, D->lc, D->ms };
+        or_guide g = { D->y, sw, sh, D->jxx, D->jxy, D->jyy, D->lc, D->ms, NULL, 0 };
         or_guided(p, D->u, D->v, cw, ch, &g, D->has_ws ? &D->ws : NULL, D->out_u, D->out_v);
     }
     or_free_axis(&ax);
@@ -2059,5 +2119,208 @@ int or_recon_frame(const or_params *pin, const or_frame_format *f,
     /* output conversion (plugin.cpp:691-711) */
     from_float(f, D->out_u, out[1], out_stride[1], ow, oh, oscale, is_float ? 0.0 : 0.5 * oscale);
     from_float(f, D->out_v, out[2], out_stride[2], ow, oh, oscale, is_float ? 0.0 : 0.5 * oscale);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* TRecon: block ME, chroma consistency, temporal taps                      */
+/* (algos.cpp:683-807, plugin.cpp:740-919)                                   */
+/* ------------------------------------------------------------------------ */
+
+void or_block_match(const float *cur, const float *nbr, int w, int h, int block, int search,
+                    float tsad, int16_t *mvx, int16_t *mvy, float *conf, int *pbw, int *pbh) {
+    const int bw = (w + block - 1) / block, bh = (h + block - 1) / block;
+    *pbw = bw;
+    *pbh = bh;
+    const float inv_t2 = 1.0f / (tsad * tsad);
+    for (int by = 0; by < bh; ++by)
+        for (int bx = 0; bx < bw; ++bx) {
+            const int x0 = bx * block, y0 = by * block;
+            const int bw_ = mini(block, w - x0), bh_ = mini(block, h - y0);
+            float best = 1e30f, sad0 = 1e30f;
+            double sum = 0.0, sum2 = 0.0;
+            int bdx = 0, bdy = 0;
+            for (int j = 0; j < bh_; j += 2)
+                for (int i = 0; i < bw_; i += 2) {
+                    const float v = at(cur, w, h, x0 + i, y0 + j);
+                    sum += v;
+                    sum2 += (double)v * v;
+                }
+            const float npix = (float)((bw_ + 1) / 2 * ((bh_ + 1) / 2));
+            for (int dy = -search; dy <= search; ++dy)
+                for (int dx = -search; dx <= search; ++dx) {
+                    if (x0 + dx < 0 || y0 + dy < 0 || x0 + dx + bw_ > w || y0 + dy + bh_ > h)
+                        continue;
+                    float sad = 0;
+                    for (int j = 0; j < bh_; j += 2)
+                        for (int i = 0; i < bw_; i += 2)
+                            sad += fabsf(at(cur, w, h, x0 + i, y0 + j) - at(nbr, w, h, x0 + dx + i, y0 + dy + j));
+                    if (dx == 0 && dy == 0)
+                        sad0 = sad;
+                    const int mn = abs(dx) + abs(dy), bn = abs(bdx) + abs(bdy);
+                    if (sad < best || (sad == best && mn < bn)) {
+                        best = sad;
+                        bdx = dx;
+                        bdy = dy;
+                    }
+                }
+            if ((bdx != 0 || bdy != 0) && sad0 - best <= 0.10f * tsad * npix) {
+                bdx = 0;
+                bdy = 0;
+                best = sad0;
+            }
+            const size_t idx = (size_t)by * bw + bx;
+            mvx[idx] = (int16_t)bdx;
+            mvy[idx] = (int16_t)bdy;
+            const float sad_px = best / npix;
+            const float var = (float)fmax(0.0, sum2 / npix - (sum / npix) * (sum / npix));
+            const float match_q = 1.0f / (1.0f + sad_px * sad_px * inv_t2);
+            const float uniq = var / (var + 0.25f * tsad * tsad);
+            conf[idx] = match_q * uniq;
+        }
+}
+
+static int cmp_float(const void *a, const void *b) {
+    const float x = *(const float *)a, y = *(const float *)b;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+void or_chroma_consistency(const float *cu, const float *cv, const float *nu, const float *nv,
+                           int cw, int ch, int luma_w, int luma_h, int block, const int16_t *mvx,
+                           const int16_t *mvy, int bw, int bh, float *conf) {
+    const double rw = (double)luma_w / cw, rh = (double)luma_h / ch;
+    float *delta = (float *)malloc(sizeof(float) * (size_t)(block + 2) * (block + 2));
+    for (int by = 0; by < bh; ++by)
+        for (int bx = 0; bx < bw; ++bx) {
+            const size_t bi = (size_t)by * bw + bx;
+            if (conf[bi] <= 0.0f)
+                continue;
+            const int lx0 = bx * block, ly0 = by * block;
+            const int lx1 = mini(luma_w, lx0 + block), ly1 = mini(luma_h, ly0 + block);
+            const int cx0 = clampi((int)floor(lx0 / rw), 0, cw - 1);
+            const int cy0 = clampi((int)floor(ly0 / rh), 0, ch - 1);
+            const int cx1 = clampi((int)ceil(lx1 / rw), cx0 + 1, cw);
+            const int cy1 = clampi((int)ceil(ly1 / rh), cy0 + 1, ch);
+            const double mx = mvx[bi] / rw, my = mvy[bi] / rh;
+            size_t n = 0;
+            for (int y = cy0; y < cy1; ++y)
+                for (int x = cx0; x < cx1; ++x) {
+                    const int nx = clampi((int)lround(x + mx), 0, cw - 1);
+                    const int ny = clampi((int)lround(y + my), 0, ch - 1);
+                    delta[n++] = maxf(fabsf(at(cu, cw, ch, x, y) - at(nu, cw, ch, nx, ny)),
+                                      fabsf(at(cv, cw, ch, x, y) - at(nv, cw, ch, nx, ny)));
+                }
+            /* nth_element median; even counts average with the largest of
+             * the lower half (algos.cpp:794-800) */
+            qsort(delta, n, sizeof(float), cmp_float);
+            const size_t mid = n / 2;
+            float median = delta[mid];
+            if ((n & 1) == 0)
+                median = 0.5f * (delta[mid - 1] + median);
+            const float ratio = median / 0.04f;
+            const float r2 = ratio * ratio;
+            conf[bi] *= 1.0f / (1.0f + r2 * r2);
+        }
+    free(delta);
+}
+
+int or_trecon_frame(const or_params *pin, const or_frame_format *f,
+                    const void *const in[3], const ptrdiff_t in_stride[3],
+                    int n_nbrs, const void *const *nbr_planes, const ptrdiff_t *nbr_strides,
+                    void *const out[3], const ptrdiff_t out_stride[3]) {
+    or_clear_debug();
+    or_params pl = *pin;
+    if (f->ss_w == 0) pl.shift_x = 0.0;
+    if (f->ss_h == 0) pl.shift_y = 0.0;
+    pl.out_w = f->width;
+    pl.out_h = f->height;
+    const or_params *p = &pl;
+    const int sw = f->width, sh = f->height, cw = sw >> f->ss_w, ch = sh >> f->ss_h;
+    const int is_float = f->sample_type == 1;
+    const double ys = is_float ? 1.0 : 1.0 / (double)((1 << f->bits) - 1);
+    const double co = is_float ? 0.0 : -0.5;
+    const double rw = (double)sw / cw, rh = (double)sh / ch;
+    const size_t ns = (size_t)sw * sh, nc = (size_t)cw * ch;
+    or_debug *D = &g_dbg;
+    D->sw = D->ow = sw; D->sh = D->oh = sh; D->cw = cw; D->ch = ch;
+    D->y = (float *)malloc(sizeof(float) * ns);
+    D->u = (float *)malloc(sizeof(float) * nc);
+    D->v = (float *)malloc(sizeof(float) * nc);
+    to_float(f, in[0], in_stride[0], D->y, sw, sh, ys, 0.0);
+    to_float(f, in[1], in_stride[1], D->u, cw, ch, ys, co);
+    to_float(f, in[2], in_stride[2], D->v, cw, ch, ys, co);
+    const int strength = p->strength > 0.0;
+    const int nn = strength ? n_nbrs : 0;
+    or_nbr *nbrs = (or_nbr *)calloc((size_t)(nn > 0 ? nn : 1), sizeof(or_nbr));
+    float **keep = (float **)calloc((size_t)(nn > 0 ? nn : 1) * 5, sizeof(float *));
+    for (int k = 0; k < nn; ++k) {
+        float *ny = (float *)malloc(sizeof(float) * ns), *nu = (float *)malloc(sizeof(float) * nc);
+        float *nv = (float *)malloc(sizeof(float) * nc), *nl = (float *)malloc(sizeof(float) * nc);
+        to_float(f, nbr_planes[3 * k], nbr_strides[3 * k], ny, sw, sh, ys, 0.0);
+        to_float(f, nbr_planes[3 * k + 1], nbr_strides[3 * k + 1], nu, cw, ch, ys, co);
+        to_float(f, nbr_planes[3 * k + 2], nbr_strides[3 * k + 2], nv, cw, ch, ys, co);
+        const int bw = (sw + 15) / 16, bh = (sh + 15) / 16;
+        int16_t *mv = (int16_t *)malloc(sizeof(int16_t) * (size_t)bw * bh * 2);
+        float *cf = (float *)malloc(sizeof(float) * (size_t)bw * bh);
+        int obw, obh;
+        or_block_match(D->y, ny, sw, sh, 16, p->tsearch, (float)p->tsad, mv, mv + (size_t)bw * bh, cf, &obw, &obh);
+        or_chroma_consistency(D->u, D->v, nu, nv, cw, ch, sw, sh, 16, mv, mv + (size_t)bw * bh, bw, bh, cf);
+        or_lc_map(ny, sw, sh, cw, ch, rw, rh, p->shift_x, p->shift_y, nl);
+        nbrs[k].u = nu; nbrs[k].v = nv; nbrs[k].lc = nl;
+        nbrs[k].mvx = mv; nbrs[k].mvy = mv + (size_t)bw * bh; nbrs[k].conf = cf;
+        nbrs[k].bw = bw; nbrs[k].bh = bh; nbrs[k].block = 16;
+        keep[5 * k] = ny; keep[5 * k + 1] = nu; keep[5 * k + 2] = nv; keep[5 * k + 3] = nl;
+        keep[5 * k + 4] = (float *)(void *)mv;
+    }
+    if (strength) {
+        D->jxx = (float *)malloc(sizeof(float) * ns);
+        D->jxy = (float *)malloc(sizeof(float) * ns);
+        D->jyy = (float *)malloc(sizeof(float) * ns);
+        D->lc = (float *)malloc(sizeof(float) * nc);
+        or_sobel_tensor(D->y, sw, sh, D->jxx, D->jxy, D->jyy, NULL, 0.0f);
+        or_lc_map(D->y, sw, sh, cw, ch, rw, rh, p->shift_x, p->shift_y, D->lc);
+        if (p->ms > 0.0) {
+            D->ms = (float *)malloc(sizeof(float) * nc);
+            or_mutual_gate(D->lc, D->u, D->v, cw, ch, NULL, 1, D->ms);
+        }
+    }
+    /* Y is rewritten through the output conversion (plugin.cpp:850-860) */
+    const double os = is_float ? 1.0 : (double)((1 << f->bits) - 1);
+    from_float(f, D->y, out[0], out_stride[0], sw, sh, os, 0.0);
+    D->out_u = (float *)calloc(ns, sizeof(float));
+    D->out_v = (float *)calloc(ns, sizeof(float));
+    or_workset ws;
+    int has_mask = 0;
+    if (p->sparse && strength) {
+        uint8_t *seed = (uint8_t *)malloc(ns);
+        const float eth = (float)(p->sigma * p->sigma);
+        for (size_t i = 0; i < ns; ++i)
+            seed[i] = D->jxx[i] + D->jyy[i] > eth;
+        D->mask = (uint8_t *)malloc(ns);
+        const int dil = (int)ceil(p->support * (rw > rh ? rw : rh)) + 8;
+        or_dilate(seed, sw, sh, dil, D->mask);
+        free(seed);
+        or_axis ax, ay;
+        or_build_axis(sw, sw, rw, p->shift_x, p, &ax);
+        or_build_axis(sh, sh, rh, p->shift_y, p, &ay);
+        or_build_workset(D->mask, sw, sh, sw, sh, &ax, &ay, cw, ch, &ws);
+        or_free_axis(&ax);
+        or_free_axis(&ay);
+        has_mask = 1;
+        or_plain_chroma(p, D->u, D->v, cw, ch, sw, sh, D->out_u, D->out_v);
+    }
+    or_guide g = { D->y, sw, sh, D->jxx, D->jxy, D->jyy, D->lc, D->ms, nbrs, nn };
+    guided_run(p, D->u, D->v, cw, ch, &g, has_mask ? &ws : NULL, NULL, D->out_u, D->out_v);
+    if (has_mask)
+        or_free_workset(&ws);
+    from_float(f, D->out_u, out[1], out_stride[1], sw, sh, os, is_float ? 0.0 : 0.5 * os);
+    from_float(f, D->out_v, out[2], out_stride[2], sw, sh, os, is_float ? 0.0 : 0.5 * os);
+    for (int k = 0; k < nn; ++k) {
+        for (int j = 0; j < 4; ++j) free(keep[5 * k + j]);
+        free(keep[5 * k + 4]);
+        free((void *)nbrs[k].conf);
+    }
+    free(keep);
+    free(nbrs);
     return 0;
 }

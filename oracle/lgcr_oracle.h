@@ -122,11 +122,21 @@ void or_plain_chroma(const or_params *p, const float *u, const float *v, int cw,
 
 /* guided reconstruction, algo 2 (recon.cpp:14-819) without temporal taps.
  * workset may be NULL (dense visit of every output pixel). */
+/* one motion-compensated neighbour frame of TRecon (TemporalNbr,
+ * lgcr.h:729-736) */
+typedef struct or_nbr {
+    const float *u, *v, *lc;             /* neighbour chroma + footprint luma (cw x ch) */
+    const int16_t *mvx, *mvy;            /* block motion (luma px) */
+    const float *conf;                   /* block confidence */
+    int bw, bh, block;
+} or_nbr;
+
 typedef struct or_guide {
     const float *y; int sw, sh;          /* source luma */
     const float *jxx, *jxy, *jyy;        /* sw x sh */
     const float *lc;                     /* cw x ch */
     const float *ms;                     /* cw x ch or NULL */
+    const or_nbr *nbrs; int n_nbrs;      /* temporal taps (TRecon) */
 } or_guide;
 void or_guided(const or_params *p, const float *u, const float *v, int cw, int ch,
                const or_guide *g, const or_workset *ws, float *out_u, float *out_v);
@@ -149,6 +159,21 @@ typedef struct or_frame_format {
 int or_recon_frame(const or_params *p, const or_frame_format *fmt,
                    const void *const in[3], const ptrdiff_t in_stride[3],
                    void *const out[3], const ptrdiff_t out_stride[3]);
+
+/* TRecon frame (plugin.cpp:740-919): center planes plus the neighbour
+ * frames already chosen by temporalFrameNumbers (plugin.cpp:729-738);
+ * output is 4:4:4 at the input size with Y rewritten. */
+int or_trecon_frame(const or_params *p, const or_frame_format *fmt,
+                    const void *const in[3], const ptrdiff_t in_stride[3],
+                    int n_nbrs, const void *const *nbr_planes, const ptrdiff_t *nbr_strides,
+                    void *const out[3], const ptrdiff_t out_stride[3]);
+
+/* block matching and chroma consistency (algos.cpp:683-807) */
+void or_block_match(const float *cur, const float *nbr, int w, int h, int block, int search,
+                    float tsad, int16_t *mvx, int16_t *mvy, float *conf, int *bw, int *bh);
+void or_chroma_consistency(const float *cu, const float *cv, const float *nu, const float *nv,
+                           int cw, int ch, int luma_w, int luma_h, int block, const int16_t *mvx,
+                           const int16_t *mvy, int bw, int bh, float *conf);
 
 /* intermediates of the last or_recon_frame on this thread */
 typedef struct or_debug {

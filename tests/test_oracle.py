@@ -492,3 +492,80 @@ def test_algo6_strength_continuity():
     step = max(float(np.max(np.abs(zero[p] - tiny[p]))) for p in (1, 2))
     total = max(float(np.max(np.abs(zero[p] - full[p]))) for p in (1, 2))
     assert total > 1e-4 and step <= total * 3e-6 + 2e-7
+
+
+# --------------------------------------------------------------------------
+# TRecon (temporal taps)
+# --------------------------------------------------------------------------
+
+def test_temporal_frame_numbers():
+    # plugin.cpp:729-738: clamp to the clip, drop the centre and duplicates
+    assert oracle.temporal_frame_numbers(0, 1, 3) == [1]
+    assert oracle.temporal_frame_numbers(1, 1, 2) == [0]
+    assert oracle.temporal_frame_numbers(5, 9, 2) == [3, 4, 6, 7]
+    assert oracle.temporal_frame_numbers(0, 0, 1) == []
+
+
+def test_trecon_chroma_inconsistent_neighbour_rejected():
+    # test_regressions.py:76-93: a neighbour whose chroma disagrees is gated
+    # out, so TRecon == single-frame Recon(sparse=0) exactly
+    n = 64
+    yy, xx = np.mgrid[0:n, 0:n]
+    y = (0.35 + 0.08 * np.sin(xx * 0.31) * np.cos(yy * 0.27)).astype(np.float32)
+    f0 = (y, np.zeros((32, 32), np.float32), np.zeros((32, 32), np.float32))
+    f1 = (y, np.full((32, 32), np.float32(0.2)), np.full((32, 32), np.float32(-0.2)))
+    single = oracle.recon_frame(oracle.make_params(n, n, strength=0.8, sparse=0), *f0)
+    temporal = oracle.trecon_frame(oracle.trecon_params(strength=0.8, trad=3), f0,
+                                   [f1 if k == 1 else f0 for k in oracle.temporal_frame_numbers(0, 1, 3)])
+    for p in range(3):
+        assert np.array_equal(single[p], temporal[p])
+
+
+def test_trecon_single_frame_equals_recon():
+    y, u, v = frame(96, 64, bits=8, seed=20)
+    single = oracle.recon_frame(oracle.make_params(96, 64, sparse=0), y, u, v)
+    temporal = oracle.trecon_frame(oracle.trecon_params(), (y, u, v), [])
+    for p in range(3):
+        assert np.array_equal(single[p], temporal[p])
+
+
+def test_trecon_444_strength0_identity():
+    # test_regressions.py:119-136
+    rng = np.random.default_rng(23)
+    planes = [rng.uniform(-0.4 if p else 0.0, 0.4 if p else 1.0, (32, 48)).astype(np.float32) for p in range(3)]
+    out = oracle.trecon_frame(oracle.trecon_params(strength=0.0), planes, [planes], ss_w=0, ss_h=0)
+    for p in range(3):
+        assert np.array_equal(out[p], planes[p])
+
+
+def test_block_match_recovers_shift():
+    # a textured frame shifted by (dx, dy) is found by the full search
+    rng = np.random.default_rng(21)
+    base = rng.random((96, 128), dtype=np.float32)
+    for dx, dy in ((3, -2), (0, 5), (-6, 1)):
+        nbr = np.roll(base, (dy, dx), axis=(0, 1)).astype(np.float32)
+        bw, bh = C.c_int(), C.c_int()
+        nb = (128 // 16) * (96 // 16)
+        mvx = np.zeros(nb, np.int16)
+        mvy = np.zeros(nb, np.int16)
+        conf = np.zeros(nb, np.float32)
+        L.or_block_match(oracle.fptr(base), oracle.fptr(nbr), 128, 96, 16, 6, 0.02,
+                         mvx.ctypes.data_as(C.POINTER(C.c_int16)), mvy.ctypes.data_as(C.POINTER(C.c_int16)),
+                         oracle.fptr(conf), C.byref(bw), C.byref(bh))
+        inner = np.zeros((6, 8), bool)
+        inner[1:-1, 1:-1] = True
+        assert np.all(mvx.reshape(6, 8)[inner] == dx) and np.all(mvy.reshape(6, 8)[inner] == dy)
+        assert np.all(conf.reshape(6, 8)[inner] > 0.9)
+
+
+def test_trecon_static_noise_gain():
+    # battery.py:236, 313-315: static scene, independent noise per frame ->
+    # temporal averaging beats single-frame guided recon
+    gy, gu, gv = frame(128, 96, ss_w=0, ss_h=0, bits=32, seed=22, polygons=6, luma_noise=0.0)
+    frames = [frame(128, 96, bits=32, seed=22, frame=k, polygons=6, luma_noise=0.0, chroma_noise=0.01)
+              for k in range(3)]
+    single = oracle.recon_frame(oracle.make_params(128, 96, sparse=0), *frames[1])
+    temporal = oracle.trecon_frame(oracle.trecon_params(), frames[1], [frames[0], frames[2]])
+    e_s = float(np.mean(np.abs(single[1] - gu)) + np.mean(np.abs(single[2] - gv)))
+    e_t = float(np.mean(np.abs(temporal[1] - gu)) + np.mean(np.abs(temporal[2] - gv)))
+    assert e_t < e_s, (e_t, e_s)

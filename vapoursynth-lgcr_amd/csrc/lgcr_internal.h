@@ -53,6 +53,26 @@ struct GuideParams {
     int ridge, cedge, clamp_hull, ms_on, radial;
 };
 
+// One motion-compensated neighbour of TRecon (TemporalNbr, src/lgcr.h:729-736).
+struct NbrDev {
+    const float *u, *v, *lc;
+    const int16_t *mvx, *mvy;
+    const float *conf;
+    int bw, bh, block;
+};
+
+// Temporal-tap context of the guided pass (recon.cpp:642-745).
+struct TemporalDev {
+    const NbrDev *nbrs = nullptr;   // device array
+    int n = 0;
+    float shift_x = 0.0f, shift_y = 0.0f;  // unused placeholders for alignment
+    double sx = 0.0, sy = 0.0;      // frame siting (luma units)
+    double rw = 1.0, rh = 1.0;
+    int kernel = 2;
+    double kp1 = 3.0, kp2 = 0.6;
+    float inv_wx = 1.0f, inv_wy = 1.0f;
+};
+
 // algo-4 selector state of the guided pass (recon.cpp:762-811).
 enum SelectorMode : int { SEL_NONE = 0, SEL_MAPS = 1, SEL_FLOAT = 2 };
 struct SelectorDev {
@@ -116,7 +136,15 @@ hipError_t launch_guided(const float *u, const float *v, int cw, int ch, const f
                          const float *lc, const float *ms, const AxisDev &ax, const AxisDev &ay,
                          const RadialDev &rd, const GuideParams &gp, const uint32_t *row_offsets,
                          const uint32_t *indices, float *out_u, float *out_v,
-                         const SelectorDev &sel, hipStream_t s);
+                         const SelectorDev &sel, const TemporalDev &tmp, hipStream_t s);
+// TRecon motion search and chroma consistency (algos.cpp:683-807)
+hipError_t launch_block_match(const float *cur, const float *nbr, int w, int h, int block,
+                              int search, float tsad, int16_t *mvx, int16_t *mvy, float *conf,
+                              hipStream_t s);
+hipError_t launch_chroma_consistency(const float *cu, const float *cv, const float *nu,
+                                     const float *nv, int cw, int ch, int luma_w, int luma_h,
+                                     int block, const int16_t *mvx, const int16_t *mvy, int bw,
+                                     int bh, float *conf, hipStream_t s);
 // algo 4 (algos.cpp:241-593)
 hipError_t launch_point_luma(const float *y, int w, int h, float *ls, int cw, int ch,
                              const int *xi, const float *xf, const int *yi, const float *yf,

@@ -665,7 +665,50 @@ struct GuidedArgs {
     const uint32_t *indices;
     float *out_u, *out_v;
     SelectorDev sel;
+    TemporalDev tmp;
 };
+
+// double-precision closed-form kernels for the temporal tap weight
+// (lgcr.h:53-95; the host builds every other table)
+__device__ double dsinc(double x) {
+    if (fabs(x) < 1e-8)
+        return 1.0;
+    x *= 3.14159265358979323846;
+    return sin(x) / x;
+}
+
+__device__ double dkernel(int k, double x, double p1, double p2) {
+    x = fabs(x);
+    switch (k) {
+    case 0:
+        return x < 1.0 ? 1.0 - x : 0.0;
+    case 1: {
+        const double b = p1, c = p2;
+        if (x < 1.0)
+            return ((12 - 9 * b - 6 * c) * x * x * x + (-18 + 12 * b + 6 * c) * x * x + (6 - 2 * b)) / 6.0;
+        if (x < 2.0)
+            return ((-b - 6 * c) * x * x * x + (6 * b + 30 * c) * x * x + (-12 * b - 48 * c) * x + (8 * b + 24 * c)) / 6.0;
+        return 0.0;
+    }
+    case 2:
+        return x < p1 ? dsinc(x) * dsinc(x / p1) : 0.0;
+    case 3:
+        if (x < 1.0)
+            return ((x - 9.0 / 5.0) * x - 1.0 / 5.0) * x + 1.0;
+        if (x < 2.0)
+            return ((-1.0 / 3.0 * (x - 1.0) + 4.0 / 5.0) * (x - 1.0) - 7.0 / 15.0) * (x - 1.0);
+        return 0.0;
+    case 4:
+        if (x < 1.0)
+            return ((13.0 / 11.0 * x - 453.0 / 209.0) * x - 3.0 / 209.0) * x + 1.0;
+        if (x < 2.0)
+            return ((-6.0 / 11.0 * (x - 1.0) + 270.0 / 209.0) * (x - 1.0) - 156.0 / 209.0) * (x - 1.0);
+        if (x < 3.0)
+            return ((1.0 / 11.0 * (x - 2.0) - 45.0 / 209.0) * (x - 2.0) + 26.0 / 209.0) * (x - 2.0);
+        return 0.0;
+    }
+    return 0.0;
+}
 
 // True when the float selector keeps compressed per-active arrays: a sparse
 // workset that is not in dense fallback (plugin.cpp:455-456).
@@ -862,6 +905,67 @@ __device__ void guided_pixel(const GuidedArgs &A, int ox, int oy, uint32_t k) {
             acc_v += (double)w * A.v[k];
             wsum += w;
             sum_abs += fabsf(w);
+        }
+    }
+    // --- temporal taps of TRecon (recon.cpp:642-745)
+    if (guided && A.tmp.n > 0) {
+        const TemporalDev &T = A.tmp;
+        const int lxi = clampi((int)ax.lpos[ox], 0, sw - 1), lyi = clampi((int)ay.lpos[oy], 0, sh - 1);
+        const float frw = gp.frw, frh = gp.frh;
+        for (int q = 0; q < T.n; ++q) {
+            const NbrDev nb = T.nbrs[q];
+            const int bxx = min(nb.bw - 1, lxi / nb.block), byy = min(nb.bh - 1, lyi / nb.block);
+            const size_t bi = (size_t)byy * nb.bw + bxx;
+            const float tconf = nb.conf[bi];
+            if (tconf < 0.05f)
+                continue;
+            const float mx = (float)nb.mvx[bi], my = (float)nb.mvy[bi];
+            const int tnx = clampi((int)lroundf(scx + mx / frw), 0, cw - 1);
+            const int tny = clampi((int)lroundf(scy + my / frh), 0, ch - 1);
+            const float ddx = ((float)tnx - scx) * frw - mx;
+            const float ddy = ((float)tny - scy) * frh - my;
+            const bool stat = (fabsf(mx) + fabsf(my) <= 1.0f) && tconf > 0.5f;
+            float dL;
+            if (stat) {
+                dL = nb.lc[(size_t)tny * cw + tnx] - L0;
+            } else {
+                const double lx = (tnx + 0.5) * T.rw - 0.5 + T.sx - mx;
+                const double ly = (tny + 0.5) * T.rh - 0.5 + T.sy - my;
+                const int x0 = (int)ceil(lx - 0.5 * T.rw), x1 = (int)ceil(lx + 0.5 * T.rw);
+                const int y0 = (int)ceil(ly - 0.5 * T.rh), y1 = (int)ceil(ly + 0.5 * T.rh);
+                float md = 0.0f;
+                for (int jj = y0; jj < y1; ++jj)
+                    for (int ii = x0; ii < x1; ++ii)
+                        md = fmx(md, fabsf(at(A.y, sw, sh, ii, jj) - L0));
+                dL = md;
+            }
+            const float sig_t = stat ? sig_loc : 0.5f * sig_loc;
+            const float sim_raw = 1.0f / (1.0f + dL * dL / (sig_t * sig_t));
+            const float sim = stat ? 1.0f - guide_fade * (1.0f - sim_raw) : sim_raw;
+            float kw;
+            if (gp.radial) {
+                const float dxc = ddx / frw * T.inv_wx, dyc = ddy / frh * T.inv_wy;
+                kw = lut_lookup(A.rd.lut, A.rd.lut_n, sqrtf(dxc * dxc + dyc * dyc) * A.rd.lut_scale);
+            } else {
+                kw = (float)(dkernel(T.kernel, fabsf(ddx) / T.rw * T.inv_wx, T.kp1, T.kp2) *
+                             dkernel(T.kernel, fabsf(ddy) / T.rh * T.inv_wy, T.kp1, T.kp2));
+            }
+            float nw = 1.0f;
+            if (!stat) {
+                const float fx = mx / frw, fy = my / frh;
+                const float px = fx - floorf(fx), py = fy - floorf(fy);
+                nw = fmx(fabsf(sinf(3.14159265358979323846f * px)), fabsf(sinf(3.14159265358979323846f * py)));
+                nw *= 1.0f - ss_ramp;
+            }
+            const float w = tconf * sim * kw * nw;
+            const size_t tk = (size_t)tny * cw + tnx;
+            const float nu = nb.u[tk], nv = nb.v[tk];
+            acc_u += (double)w * nu;
+            acc_v += (double)w * nv;
+            wsum += w;
+            sum_abs += fabsf(w);
+            mnu = fmn(mnu, nu); mxu = fmx(mxu, nu);
+            mnv = fmn(mnv, nv); mxv = fmx(mxv, nv);
         }
     }
     double vu, vv;
@@ -1261,6 +1365,171 @@ __global__ __launch_bounds__(kBlock) void k_detail_transfer(WorkList L, const fl
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// TRecon: integer-pel block matching (blockMatch, algos.cpp:683-755) and the
+// chroma-consistency confidence (applyChromaConsistency, algos.cpp:757-807).
+// One workgroup per block; each thread scores candidates sequentially in the
+// reference's (j, i) order and the argmin reduces on (sad, |dx|+|dy|, scan
+// index), which is exactly the first-wins scan of the reference.
+// ---------------------------------------------------------------------------
+
+constexpr int kMeThreads = 256;
+
+__global__ __launch_bounds__(kMeThreads) void k_block_match(const float *__restrict__ cur, const float *__restrict__ nbr,
+                                                            int w, int h, int block, int search, float tsad,
+                                                            int16_t *mvx, int16_t *mvy, float *conf) {
+    __shared__ float sc[32 * 32];
+    __shared__ float ksad[kMeThreads];
+    __shared__ int kidx[kMeThreads];
+    __shared__ float s_sad0;
+    const int bw = (w + block - 1) / block;
+    const int bx = blockIdx.x % bw, by = blockIdx.x / bw;
+    const int x0 = bx * block, y0 = by * block;
+    const int bw_ = min(block, w - x0), bh_ = min(block, h - y0);
+    const int nx = (bw_ + 1) / 2, ny = (bh_ + 1) / 2;
+    for (int t = threadIdx.x; t < nx * ny; t += kMeThreads) {
+        const int i = 2 * (t % nx), j = 2 * (t / nx);
+        sc[t] = at(cur, w, h, x0 + i, y0 + j);
+    }
+    if (threadIdx.x == 0)
+        s_sad0 = 1e30f;
+    __syncthreads();
+    const int side = 2 * search + 1, ncand = side * side;
+    float best = 1e30f;
+    int bidx = 0x7fffffff;
+    int bnorm = 0x7fffffff;
+    for (int c = threadIdx.x; c < ncand; c += kMeThreads) {
+        const int dy = c / side - search, dx = c % side - search;
+        if (x0 + dx < 0 || y0 + dy < 0 || x0 + dx + bw_ > w || y0 + dy + bh_ > h)
+            continue;
+        float sad = 0;
+        for (int t = 0; t < nx * ny; ++t) {
+            const int i = 2 * (t % nx), j = 2 * (t / nx);
+            sad += fabsf(sc[t] - nbr[(size_t)(y0 + dy + j) * w + (x0 + dx + i)]);
+        }
+        if (dx == 0 && dy == 0)
+            s_sad0 = sad;
+        const int norm = abs(dx) + abs(dy);
+        if (sad < best || (sad == best && (norm < bnorm || (norm == bnorm && c < bidx)))) {
+            best = sad;
+            bidx = c;
+            bnorm = norm;
+        }
+    }
+    ksad[threadIdx.x] = best;
+    kidx[threadIdx.x] = bidx;
+    __syncthreads();
+    for (int off = kMeThreads / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            const float s2 = ksad[threadIdx.x + off];
+            const int i2 = kidx[threadIdx.x + off];
+            const float s1 = ksad[threadIdx.x];
+            const int i1 = kidx[threadIdx.x];
+            bool take = false;
+            if (i2 != 0x7fffffff) {
+                if (i1 == 0x7fffffff || s2 < s1) {
+                    take = true;
+                } else if (s2 == s1) {
+                    const int n1 = abs(i1 / side - search) + abs(i1 % side - search);
+                    const int n2 = abs(i2 / side - search) + abs(i2 % side - search);
+                    take = n2 < n1 || (n2 == n1 && i2 < i1);
+                }
+            }
+            if (take) {
+                ksad[threadIdx.x] = s2;
+                kidx[threadIdx.x] = i2;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0)
+        return;
+    float bsad = ksad[0];
+    int bdx = 0, bdy = 0;
+    if (kidx[0] != 0x7fffffff) {
+        bdy = kidx[0] / side - search;
+        bdx = kidx[0] % side - search;
+    }
+    // a tie against the (0,0) candidate keeps zero motion: norm 0 is smallest
+    double sum = 0.0, sum2 = 0.0;
+    for (int t = 0; t < nx * ny; ++t) {
+        const float v = sc[t];
+        sum += v;
+        sum2 += (double)v * v;
+    }
+    const float npix = (float)(nx * ny);
+    const float sad0 = s_sad0;
+    if ((bdx != 0 || bdy != 0) && sad0 - bsad <= 0.10f * tsad * npix) {
+        bdx = 0;
+        bdy = 0;
+        bsad = sad0;
+    }
+    const int idx = blockIdx.x;
+    mvx[idx] = (int16_t)bdx;
+    mvy[idx] = (int16_t)bdy;
+    const float sad_px = bsad / npix;
+    const double vv = sum2 / npix - (sum / npix) * (sum / npix);
+    const float var = (float)(0.0 < vv ? vv : 0.0);
+    const float inv_t2 = 1.0f / (tsad * tsad);
+    const float match_q = 1.0f / (1.0f + sad_px * sad_px * inv_t2);
+    const float uniq = var / (var + 0.25f * tsad * tsad);
+    conf[idx] = match_q * uniq;
+}
+
+constexpr int kCcMax = 1024;
+
+__global__ __launch_bounds__(kBlock) void k_chroma_consistency(const float *cu, const float *cv, const float *nu,
+                                                               const float *nv, int cw, int ch, int luma_w,
+                                                               int luma_h, int block, const int16_t *mvx,
+                                                               const int16_t *mvy, int bw, float *conf) {
+    __shared__ float delta[kCcMax];
+    __shared__ float lo_val, mid_val;
+    const int bi = blockIdx.x;
+    if (conf[bi] <= 0.0f)
+        return;
+    const int bx = bi % bw, by = bi / bw;
+    const double rw = (double)luma_w / cw, rh = (double)luma_h / ch;
+    const int lx0 = bx * block, ly0 = by * block;
+    const int lx1 = min(luma_w, lx0 + block), ly1 = min(luma_h, ly0 + block);
+    const int cx0 = clampi((int)floor(lx0 / rw), 0, cw - 1), cy0 = clampi((int)floor(ly0 / rh), 0, ch - 1);
+    const int cx1 = clampi((int)ceil(lx1 / rw), cx0 + 1, cw), cy1 = clampi((int)ceil(ly1 / rh), cy0 + 1, ch);
+    const double mx = mvx[bi] / rw, my = mvy[bi] / rh;
+    const int nxc = cx1 - cx0, n = nxc * (cy1 - cy0);
+    if (n > kCcMax)
+        return;  // the host rejects block sizes that could overflow
+    for (int t = threadIdx.x; t < n; t += kBlock) {
+        const int x = cx0 + t % nxc, y = cy0 + t / nxc;
+        const int qx = clampi((int)llround(x + mx), 0, cw - 1), qy = clampi((int)llround(y + my), 0, ch - 1);
+        const size_t a = (size_t)y * cw + x, b = (size_t)qy * cw + qx;
+        delta[t] = fmx(fabsf(cu[a] - nu[b]), fabsf(cv[a] - nv[b]));
+    }
+    __syncthreads();
+    // order statistics by rank: element t has rank #{smaller} + #{equal, lower index}
+    const int mid = n / 2;
+    for (int t = threadIdx.x; t < n; t += kBlock) {
+        const float d = delta[t];
+        int rank = 0;
+        for (int k = 0; k < n; ++k) {
+            const float e = delta[k];
+            rank += (e < d) || (e == d && k < t);
+        }
+        if (rank == mid)
+            mid_val = d;
+        if (rank == mid - 1)
+            lo_val = d;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0)
+        return;
+    float median = mid_val;
+    if ((n & 1) == 0)
+        median = 0.5f * (lo_val + median);
+    const float ratio = median / 0.04f;
+    const float r2 = ratio * ratio;
+    conf[bi] *= 1.0f / (1.0f + r2 * r2);
+}
+
 // ---------------------------------------------------------------------------
 // back-projection (backProject, algos.cpp:663-677); the reference applies it
 // only when the output is exactly 2x the chroma grid, so the 2x2 blocks are
@@ -1457,9 +1726,9 @@ hipError_t launch_guided(const float *u, const float *v, int cw, int ch, const f
                          const float *jxx, const float *jxy, const float *jyy, const float *lc, const float *ms,
                          const AxisDev &ax, const AxisDev &ay, const RadialDev &rd, const GuideParams &gp,
                          const uint32_t *row_offsets, const uint32_t *indices, float *out_u, float *out_v,
-                         const SelectorDev &sel, hipStream_t s) {
+                         const SelectorDev &sel, const TemporalDev &tmp, hipStream_t s) {
     GuidedArgs A{u, v, cw, ch, y, sw, sh, jxx, jxy, jyy, lc, ms, ax, ay, rd, gp, row_offsets, indices, out_u, out_v,
-                 sel};
+                 sel, tmp};
     const size_t n = (size_t)ax.n * ay.n;
     const unsigned blocks = (unsigned)std::min<size_t>(8192, (n + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_guided, dim3(blocks), dim3(kBlock), 0, s, A);
@@ -1556,6 +1825,24 @@ hipError_t launch_detail_transfer(const uint32_t *row_offsets, const uint32_t *i
     hipLaunchKernelGGL(k_detail_transfer, dim3((unsigned)std::min<size_t>(8192, (n + kBlock - 1) / kBlock)),
                        dim3(kBlock), 0, s, L, jxx, jxy, jyy, sw, sh, ax, ay, cw, ch, m, ms, detail, ms_strength,
                        qgate, strength, stretch, ar, clamp_hull, out_u, out_v);
+    return hipGetLastError();
+}
+
+hipError_t launch_block_match(const float *cur, const float *nbr, int w, int h, int block, int search, float tsad,
+                              int16_t *mvx, int16_t *mvy, float *conf, hipStream_t s) {
+    if (block > 64)
+        return hipErrorInvalidValue;  // LDS block cache holds 32x32 subsampled samples
+    const int bw = (w + block - 1) / block, bh = (h + block - 1) / block;
+    hipLaunchKernelGGL(k_block_match, dim3((unsigned)(bw * bh)), dim3(kMeThreads), 0, s, cur, nbr, w, h, block, search,
+                       tsad, mvx, mvy, conf);
+    return hipGetLastError();
+}
+
+hipError_t launch_chroma_consistency(const float *cu, const float *cv, const float *nu, const float *nv, int cw,
+                                     int ch, int luma_w, int luma_h, int block, const int16_t *mvx,
+                                     const int16_t *mvy, int bw, int bh, float *conf, hipStream_t s) {
+    hipLaunchKernelGGL(k_chroma_consistency, dim3((unsigned)(bw * bh)), dim3(kBlock), 0, s, cu, cv, nu, nv, cw, ch,
+                       luma_w, luma_h, block, mvx, mvy, bw, conf);
     return hipGetLastError();
 }
 

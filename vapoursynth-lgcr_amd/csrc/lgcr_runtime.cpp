@@ -339,6 +339,7 @@ namespace {
 // src/kernels.cpp:411-428).
 struct Geometry {
     void *dev = nullptr;
+    double sx = 0.0, sy = 0.0;  // siting this geometry was built for
     AxisDev ax, ay;
     RadialDev rd;
     const int *xb = nullptr, *xe = nullptr, *yb = nullptr, *ye = nullptr;
@@ -370,6 +371,15 @@ struct Workspace {
     // algo 4
     float *ls = nullptr, *mm = nullptr, *lgf = nullptr, *sel = nullptr, *selp = nullptr;
     uint8_t *roi = nullptr;
+    // TRecon neighbours: converted planes, footprint luma, motion, confidence
+    struct Nbr {
+        float *y = nullptr, *u = nullptr, *v = nullptr, *lc = nullptr, *conf = nullptr;
+        int16_t *mv = nullptr;
+    };
+    std::vector<Nbr> nb;
+    NbrDev *nbr_dev = nullptr;
+    std::vector<NbrDev> nbr_host;
+    void *nbr_raw = nullptr;  // staging of host neighbour frames
     // algo 6
     float *yct = nullptr, *yc1 = nullptr, *yc2 = nullptr, *aff = nullptr, *rec = nullptr, *rec2 = nullptr,
           *nyq = nullptr, *detail_out = nullptr;
@@ -392,6 +402,7 @@ struct lgcr_clip {
     int dil = 0;
     bool same = true;
     bool capture = false;
+    bool temporal = false;  // TRecon clip (neighbour buffers)
     size_t ws_bytes = 0;
     int max_inflight = 4;
     void *lut_dev = nullptr;
@@ -493,6 +504,8 @@ lgcr_status get_geometry(lgcr_clip *c, double sx, double sy, Geometry **out) {
     }
     const size_t osrx = blob.add(shr.xphase), osry = blob.add(shr.yphase), osrw = blob.add(shr.w);
     auto g = std::make_unique<Geometry>();
+    g->sx = sx;
+    g->sy = sy;
     HIP_TRY(hipSetDevice(c->dev->ordinal));
     HIP_TRY(hipMalloc(&g->dev, blob.host.size()));
     HIP_TRY(hipMemcpy(g->dev, blob.host.data(), blob.host.size(), hipMemcpyHostToDevice));
@@ -624,7 +637,7 @@ void layout(const lgcr_clip *c, Workspace *w, Carver &cv) {
     w->cmask = cv.take<uint8_t>(nc);
     w->row_off = cv.take<uint32_t>(size_t(c->oh) + 1);
     w->indices = cv.take<uint32_t>(no);
-    w->counters = cv.take<uint32_t>(4);
+    w->counters = cv.take<uint32_t>(8);
     if (c->p.algo == 4 && c->p.strength > 0.0) {
         w->ls = cv.take<float>(nc);
         w->mm = cv.take<float>(4 * nc);
@@ -632,6 +645,21 @@ void layout(const lgcr_clip *c, Workspace *w, Carver &cv) {
         w->roi = cv.take<uint8_t>(nc);
         w->sel = cv.take<float>(3 * no);
         w->selp = cv.take<float>(4 * no);
+    }
+    const int n_nbr = 2 * std::max(0, c->p.trad);
+    if (c->temporal && n_nbr > 0) {
+        const size_t nblk = size_t((c->sw + 15) / 16) * size_t((c->sh + 15) / 16);
+        w->nb.resize(n_nbr);
+        for (auto &q : w->nb) {
+            q.y = cv.take<float>(ns);
+            q.u = cv.take<float>(nc);
+            q.v = cv.take<float>(nc);
+            q.lc = cv.take<float>(nc);
+            q.conf = cv.take<float>(nblk);
+            q.mv = cv.take<int16_t>(2 * nblk);
+        }
+        w->nbr_dev = cv.take<NbrDev>(n_nbr);
+        w->nbr_host.resize(n_nbr);
     }
     if (c->p.algo == 6 && c->p.strength > 0.0) {
         w->yct = cv.take<float>(size_t(c->cw) * c->sh);
@@ -683,6 +711,8 @@ void destroy_workspace(lgcr_clip *c, Workspace *w) {
         hipFree(w->base);
         c->dev->unreserve(w->bytes);
     }
+    if (w->nbr_raw)
+        hipFree(w->nbr_raw);
 }
 
 lgcr_status acquire(lgcr_clip *c, int *slot) {
@@ -898,7 +928,8 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
             }
         }
         HIP_TRY(launch_guided(w->u, w->v, c->cw, c->ch, w->y, c->sw, c->sh, w->jxx, w->jxy, w->jyy, w->lc,
-                              gate, g.ax, g.ay, g.rd, gp, w->row_off, idx, w->out_u, w->out_v, sel, s));
+                              gate, g.ax, g.ay, g.rd, gp, w->row_off, idx, w->out_u, w->out_v, sel,
+                              TemporalDev{}, s));
         HIP_TRY(mark(LGCR_STAGE_APPLY_GUIDED_CORRECTION));
         if (algo4 && is_float) {
             HIP_TRY(launch_roi_mark(w->row_off, idx, c->ow, c->oh, sel, g.ax, g.ay, w->roi, c->cw, c->ch,
@@ -977,6 +1008,149 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
     };
     HIP_TRY(launch_from_float(ro, 2, c->raw_type, s));
     HIP_TRY(mark(LGCR_STAGE_CONVERT_OUTPUT));
+    return LGCR_OK;
+}
+
+// TRecon frame (tReconGetFrame, src/plugin.cpp:740-919) on workspace w:
+// neighbour ME + chroma consistency + footprint luma, guide maps with a dense
+// mutual gate, rewritten Y, optional sparse plain base, guided pass with
+// temporal taps.
+lgcr_status enqueue_trecon(lgcr_clip *c, Workspace *w, const Geometry &g, const void *const in[3],
+                           const ptrdiff_t in_stride[3], int n_nbrs, const void *const *nbr_data,
+                           const ptrdiff_t *nbr_stride, void *const out[3], const ptrdiff_t out_stride[3],
+                           bool timed) {
+    const lgcr_params &p = c->p;
+    hipStream_t s = w->stream;
+    auto mark = [&](int stage) -> hipError_t {
+        if (!timed)
+            return hipSuccess;
+        hipEvent_t ev = nullptr;
+        if (!w->pool.empty()) {
+            ev = w->pool.back();
+            w->pool.pop_back();
+        } else {
+            const hipError_t e = hipEventCreate(&ev);
+            if (e != hipSuccess)
+                return e;
+        }
+        w->marks.emplace_back(stage, ev);
+        return hipEventRecord(ev, s);
+    };
+    const bool is_float = c->f.sample_type == 1;
+    const double yscale = is_float ? 1.0 : 1.0 / double((1 << c->f.bits_per_sample) - 1);
+    const double coff = is_float ? 0.0 : -0.5;
+    HIP_TRY(mark(-1));
+    RawPlane rp[3] = {
+        {in[0], in_stride[0], c->sw, c->sh, w->y, yscale, 0.0},
+        {in[1], in_stride[1], c->cw, c->ch, w->u, yscale, coff},
+        {in[2], in_stride[2], c->cw, c->ch, w->v, yscale, coff},
+    };
+    HIP_TRY(launch_to_float(rp, 3, c->raw_type, s));
+    HIP_TRY(mark(LGCR_STAGE_CONVERT_INPUT));
+    const bool guided = p.strength > 0.0;
+    const int nn = guided ? n_nbrs : 0;
+    const int bw = (c->sw + 15) / 16, bh = (c->sh + 15) / 16;
+    for (int k = 0; k < nn; ++k) {
+        auto &q = w->nb[k];
+        RawPlane np[3] = {
+            {nbr_data[3 * k], nbr_stride[3 * k], c->sw, c->sh, q.y, yscale, 0.0},
+            {nbr_data[3 * k + 1], nbr_stride[3 * k + 1], c->cw, c->ch, q.u, yscale, coff},
+            {nbr_data[3 * k + 2], nbr_stride[3 * k + 2], c->cw, c->ch, q.v, yscale, coff},
+        };
+        HIP_TRY(launch_to_float(np, 3, c->raw_type, s));
+        HIP_TRY(launch_block_match(w->y, q.y, c->sw, c->sh, 16, p.tsearch, float(p.tsad), q.mv, q.mv + size_t(bw) * bh,
+                                   q.conf, s));
+        HIP_TRY(launch_chroma_consistency(w->u, w->v, q.u, q.v, c->cw, c->ch, c->sw, c->sh, 16, q.mv,
+                                          q.mv + size_t(bw) * bh, bw, bh, q.conf, s));
+        HIP_TRY(launch_lc_map(q.y, c->sw, c->sh, q.lc, c->cw, c->ch, g.xb, g.xe, g.yb, g.ye, s));
+        w->nbr_host[k] = NbrDev{q.u, q.v, q.lc, q.mv, q.mv + size_t(bw) * bh, q.conf, bw, bh, 16};
+    }
+    if (nn > 0)
+        HIP_TRY(hipMemcpyAsync(w->nbr_dev, w->nbr_host.data(), sizeof(NbrDev) * nn, hipMemcpyHostToDevice, s));
+    if (guided) {
+        HIP_TRY(launch_sobel_tensor(w->y, c->sw, c->sh, w->jxx, w->jxy, w->jyy, p.sparse ? w->seed : nullptr,
+                                    float(p.sigma * p.sigma), s));
+        HIP_TRY(launch_lc_map(w->y, c->sw, c->sh, w->lc, c->cw, c->ch, g.xb, g.xe, g.yb, g.ye, s));
+        if (p.ms > 0.0)
+            HIP_TRY(launch_mutual_gate(w->lc, w->u, w->v, c->cw, c->ch, w->grad, nullptr, w->counters, w->ms, s));
+        HIP_TRY(mark(LGCR_STAGE_BUILD_GUIDE_MAPS));
+    }
+    const double oscale = is_float ? 1.0 : double((1 << c->f.bits_per_sample) - 1);
+    const int hi = is_float ? 0 : (1 << c->f.bits_per_sample) - 1;
+    RawOut ry{w->y, out[0], out_stride[0], c->sw, c->sh, float(oscale), 0.0f, 0, hi};
+    HIP_TRY(launch_from_float(&ry, 1, c->raw_type, s));
+    const int clamp_hull = p.ar_margin >= 0.0;
+    const float margin = float(std::max(0.0, p.ar_margin));
+    w->has_ws = false;
+    if (p.sparse && guided) {
+        HIP_TRY(launch_dilate(w->seed, w->mtmp, w->mask, c->sw, c->sh, c->dil, s));
+        HIP_TRY(launch_workset(w->mask, c->sw, c->sh, w->row_off, w->indices, w->cmask, c->cw, c->ch, g.ax, g.ay,
+                               w->counters, s));
+        w->has_ws = true;
+        HIP_TRY(launch_plain_separable(w->u, w->v, c->cw, c->ch, g.ax, g.ay, w->hbuf, w->out_u, w->out_v, clamp_hull,
+                                       margin, s));
+        HIP_TRY(mark(LGCR_STAGE_BUILD_BASE_CHROMA));
+    }
+    GuideParams gp;
+    gp.sigma = float(p.sigma);
+    gp.sratio = float(p.sratio);
+    gp.sdb = float(p.sdb);
+    gp.strength = float(p.strength);
+    gp.rescue = float(p.rescue);
+    gp.stretch = float(p.stretch);
+    gp.inv_gsigma2 = 1.0f / float(p.gsigma * p.gsigma);
+    gp.ms = float(p.ms);
+    gp.margin = margin;
+    gp.sp_out = float(std::max(c->rw, c->rh));
+    gp.frw = float(c->rw);
+    gp.frh = float(c->rh);
+    gp.ridge = p.ridge;
+    gp.cedge = p.cedge;
+    gp.clamp_hull = clamp_hull;
+    gp.ms_on = guided && p.ms > 0.0;
+    gp.radial = p.radial;
+    TemporalDev td;
+    td.nbrs = w->nbr_dev;
+    td.n = nn;
+    td.sx = g.sx;
+    td.sy = g.sy;
+    td.rw = c->rw;
+    td.rh = c->rh;
+    td.kernel = p.kernel;
+    td.kp1 = p.kp1;
+    td.kp2 = p.kp2;
+    td.inv_wx = float(std::min(1.0, double(c->ow) / c->cw));
+    td.inv_wy = float(std::min(1.0, double(c->oh) / c->ch));
+    HIP_TRY(launch_guided(w->u, w->v, c->cw, c->ch, w->y, c->sw, c->sh, w->jxx, w->jxy, w->jyy, w->lc,
+                          gp.ms_on ? w->ms : nullptr, g.ax, g.ay, g.rd, gp, w->row_off,
+                          w->has_ws ? w->indices : nullptr, w->out_u, w->out_v, SelectorDev{}, td, s));
+    HIP_TRY(mark(LGCR_STAGE_APPLY_GUIDED_CORRECTION));
+    const float offs = is_float ? 0.0f : float(0.5 * oscale);
+    RawOut ro[2] = {
+        {w->out_u, out[1], out_stride[1], c->ow, c->oh, float(oscale), offs, 0, hi},
+        {w->out_v, out[2], out_stride[2], c->ow, c->oh, float(oscale), offs, 0, hi},
+    };
+    HIP_TRY(launch_from_float(ro, 2, c->raw_type, s));
+    HIP_TRY(mark(LGCR_STAGE_CONVERT_OUTPUT));
+    return LGCR_OK;
+}
+
+lgcr_status validate_trecon(const lgcr_params &p, const lgcr_format &f) {
+    // tReconCreate (plugin.cpp:928-959)
+    if (f.sample_type == 1 ? f.bits_per_sample != 32 : (f.bits_per_sample < 8 || f.bits_per_sample > 16) ||
+        f.sub_sampling_w < 0 || f.sub_sampling_w > 1 || f.sub_sampling_h < 0 || f.sub_sampling_h > 1)
+        return fail(LGCR_INVALID_ARGUMENT, "TRecon: planar YUV 444/422/420, 8-16 bit int or 32 bit float");
+    if (f.width <= 0 || f.height <= 0)
+        return fail(LGCR_INVALID_ARGUMENT, "TRecon: constant dimensions and frame count required");
+    if (p.strength < 0.0 || p.strength > 1.0 || p.sigma <= 0.0 || p.sratio <= 0.0 || p.sdb <= 0.0 ||
+        p.stretch < 0.0 || p.gsigma <= 0.0 || p.ms < 0.0 || p.ms > 1.0 || p.trad < 0 || p.trad > 8 ||
+        p.tsearch < 0 || p.tsearch > 64 || p.tsad <= 0.0)
+        return fail(LGCR_INVALID_ARGUMENT, "TRecon: invalid parameter range (need 0<=strength<=1, "
+                                           "sigma/sratio/sdb/gsigma/tsad>0, stretch>=0, 0<=trad<=8, 0<=tsearch<=64)");
+    if (p.kernel != LGCR_KERNEL_LANCZOS || p.kp1 != 3.0 || p.algo != 2 || p.radial)
+        return fail(LGCR_INVALID_ARGUMENT, "TRecon: the base kernel is lanczos 3, algo 2 (plugin.cpp:973-976)");
+    if (p.bm)
+        return fail(LGCR_UNSUPPORTED, "LGCR HIP backend: bm=True is not available in this build");
     return LGCR_OK;
 }
 
@@ -1071,17 +1245,17 @@ void lgcr_device_close(lgcr_device *dev) { delete dev; }
 
 uint64_t lgcr_device_reserved(const lgcr_device *dev) { return dev ? dev->reserved.load() : 0; }
 
-lgcr_status lgcr_clip_create(lgcr_device *dev, const lgcr_params *params, const lgcr_format *fmt,
-                             int max_inflight, lgcr_clip **out) {
+static lgcr_status create_clip(lgcr_device *dev, const lgcr_params *params, const lgcr_format *fmt,
+                               int max_inflight, bool temporal, lgcr_clip **out) {
     if (!dev || !params || !fmt || !out)
         return fail(LGCR_INVALID_ARGUMENT, "lgcr_clip_create: NULL argument");
     *out = nullptr;
     lgcr_params p = *params;
-    if (p.out_w == 0)
+    if (p.out_w == 0 || temporal)
         p.out_w = fmt->width;
-    if (p.out_h == 0)
+    if (p.out_h == 0 || temporal)
         p.out_h = fmt->height;
-    lgcr_status st = validate(p, *fmt);
+    lgcr_status st = temporal ? validate_trecon(p, *fmt) : validate(p, *fmt);
     if (st != LGCR_OK)
         return st;
     // zero siting on non-subsampled axes (plugin.cpp:1130-1133)
@@ -1092,6 +1266,7 @@ lgcr_status lgcr_clip_create(lgcr_device *dev, const lgcr_params *params, const 
     auto c = std::make_unique<lgcr_clip>();
     c->dev = dev;
     c->p = p;
+    c->temporal = temporal;
     c->f = *fmt;
     c->sw = fmt->width;
     c->sh = fmt->height;
@@ -1150,6 +1325,130 @@ lgcr_status lgcr_clip_create(lgcr_device *dev, const lgcr_params *params, const 
     c->spaces.push_back(std::move(w));
     c->idle.push_back(0);
     *out = c.release();
+    return LGCR_OK;
+}
+
+lgcr_status lgcr_clip_create(lgcr_device *dev, const lgcr_params *params, const lgcr_format *fmt,
+                             int max_inflight, lgcr_clip **out) {
+    return create_clip(dev, params, fmt, max_inflight, false, out);
+}
+
+lgcr_status lgcr_trecon_create(lgcr_device *dev, const lgcr_params *params, const lgcr_format *fmt,
+                               int max_inflight, lgcr_clip **out) {
+    return create_clip(dev, params, fmt, max_inflight, true, out);
+}
+
+void lgcr_params_default_trecon(lgcr_params *p) {
+    lgcr_params_default(p);
+    p->sparse = 0;  // static-region temporal averaging is part of the value (plugin.cpp:966-972)
+}
+
+lgcr_status lgcr_trecon_frame(lgcr_clip *c, const lgcr_frame *in, const lgcr_frame *nbrs, int n_nbrs,
+                              double shift_x, double shift_y, lgcr_frame *out, lgcr_stats *stats) {
+    if (!c || !in || !out || n_nbrs < 0 || (n_nbrs && !nbrs))
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_trecon_frame: NULL argument");
+    if (!c->temporal)
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_trecon_frame: clip was not created by lgcr_trecon_create");
+    if (n_nbrs > 2 * c->p.trad)
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_trecon_frame: %d neighbours exceed 2*trad", n_nbrs);
+    for (int k = 0; k < n_nbrs; ++k)
+        if (nbrs[k].memory != in->memory)
+            return fail(LGCR_INVALID_ARGUMENT, "lgcr_trecon_frame: neighbours must be in the input's memory");
+    if (in->memory != out->memory)
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_trecon_frame: input and output must be in the same memory");
+    double sx = std::isnan(shift_x) ? c->p.shift_x : shift_x;
+    double sy = std::isnan(shift_y) ? c->p.shift_y : shift_y;
+    if (c->f.sub_sampling_w == 0)
+        sx = 0.0;
+    if (c->f.sub_sampling_h == 0)
+        sy = 0.0;
+    Geometry *g = nullptr;
+    lgcr_status st = get_geometry(c, sx, sy, &g);
+    if (st != LGCR_OK)
+        return st;
+    int slot = -1;
+    st = acquire(c, &slot);
+    if (st != LGCR_OK)
+        return st;
+    Workspace *w = c->spaces[slot].get();
+    struct Releaser {
+        lgcr_clip *c;
+        int slot;
+        ~Releaser() { release(c, slot); }
+    } rel{c, slot};
+    HIP_TRY(hipSetDevice(c->dev->ordinal));
+    const bool host = in->memory == LGCR_MEMORY_HOST;
+    const int pw[3] = {c->sw, c->cw, c->cw}, ph[3] = {c->sh, c->ch, c->ch};
+    const void *din[3];
+    ptrdiff_t dis[3];
+    void *dout[3];
+    ptrdiff_t dos[3];
+    std::vector<const void *> nd(size_t(3) * n_nbrs);
+    std::vector<ptrdiff_t> ns(size_t(3) * n_nbrs);
+    for (int i = 0; i < 3; ++i) {
+        const size_t rb = size_t(pw[i]) * c->bps;
+        if (host) {
+            HIP_TRY(hipMemcpy2DAsync(w->raw_in[i], rb, in->data[i], in->stride[i], rb, ph[i], hipMemcpyHostToDevice,
+                                     w->stream));
+            din[i] = w->raw_in[i];
+            dis[i] = ptrdiff_t(rb);
+            dout[i] = w->raw_out[i];
+            dos[i] = ptrdiff_t(size_t(c->ow) * c->bps);
+        } else {
+            din[i] = in->data[i];
+            dis[i] = in->stride[i];
+            dout[i] = out->data[i];
+            dos[i] = out->stride[i];
+        }
+    }
+    if (host && n_nbrs > 0 && !w->nbr_raw) {
+        const size_t per = (size_t(c->sw) * c->sh + 2 * size_t(c->cw) * c->ch) * c->bps;
+        HIP_TRY(hipMalloc(&w->nbr_raw, per * 2 * std::max(1, c->p.trad)));
+    }
+    for (int k = 0; k < n_nbrs; ++k) {
+        size_t off = (size_t(c->sw) * c->sh + 2 * size_t(c->cw) * c->ch) * c->bps * k;
+        for (int i = 0; i < 3; ++i) {
+            const size_t rb = size_t(pw[i]) * c->bps;
+            if (host) {
+                uint8_t *dst = static_cast<uint8_t *>(w->nbr_raw) + off;
+                HIP_TRY(hipMemcpy2DAsync(dst, rb, nbrs[k].data[i], nbrs[k].stride[i], rb, ph[i], hipMemcpyHostToDevice,
+                                         w->stream));
+                nd[3 * k + i] = dst;
+                ns[3 * k + i] = ptrdiff_t(rb);
+                off += rb * ph[i];
+            } else {
+                nd[3 * k + i] = nbrs[k].data[i];
+                ns[3 * k + i] = nbrs[k].stride[i];
+            }
+        }
+    }
+    const bool timed = stats != nullptr || c->timing.load();
+    w->marks.clear();
+    st = enqueue_trecon(c, w, *g, din, dis, n_nbrs, nd.data(), ns.data(), dout, dos, timed);
+    if (st != LGCR_OK)
+        return st;
+    if (host)
+        for (int i = 0; i < 3; ++i) {
+            const size_t rb = size_t(c->ow) * c->bps;
+            HIP_TRY(hipMemcpy2DAsync(out->data[i], out->stride[i], w->raw_out[i], rb, rb, c->oh, hipMemcpyDeviceToHost,
+                                     w->stream));
+        }
+    HIP_TRY(hipStreamSynchronize(w->stream));
+    double us[LGCR_STAGE_COUNT] = {};
+    uint64_t nst[LGCR_STAGE_COUNT] = {};
+    if (timed) {
+        HIP_TRY(collect_timeline(w, us, nst));
+        if (c->timing.load())
+            add_timing(c, us, nst);
+    }
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        for (int i = 0; i < LGCR_STAGE_COUNT; ++i) {
+            stats->stage_us[i] = us[i];
+            stats->total_us += us[i];
+        }
+        stats->output_pixels = uint64_t(c->ow) * c->oh;
+    }
     return LGCR_OK;
 }
 

@@ -30,7 +30,8 @@ SYMBOLS = [
     "lgcr_abi_version", "lgcr_last_error", "lgcr_params_default", "lgcr_device_open",
     "lgcr_device_close", "lgcr_device_reserved", "lgcr_clip_create", "lgcr_clip_destroy",
     "lgcr_clip_workspace_bytes", "lgcr_recon_frame", "lgcr_recon_batch", "lgcr_clip_set_timing",
-    "lgcr_clip_stage_times", "lgcr_clip_fetch",
+    "lgcr_clip_stage_times", "lgcr_clip_fetch", "lgcr_params_default_trecon", "lgcr_trecon_create",
+    "lgcr_trecon_frame",
 ]
 STAGE_NAMES = [
     "ConvertInput", "ResampleLuma", "BuildGuideMaps", "BuildTrustMask", "BuildMutualGate",
@@ -102,6 +103,10 @@ def lib():
         L.lgcr_recon_frame.argtypes = [vp, C.POINTER(Frame), C.c_double, C.c_double, C.POINTER(Frame),
                                        C.POINTER(Stats)]
         L.lgcr_recon_batch.argtypes = [vp, C.c_int, C.POINTER(Frame), C.POINTER(Frame)]
+        L.lgcr_params_default_trecon.argtypes = [C.POINTER(Params)]
+        L.lgcr_trecon_create.argtypes = [vp, C.POINTER(Params), C.POINTER(Format), C.c_int, C.POINTER(vp)]
+        L.lgcr_trecon_frame.argtypes = [vp, C.POINTER(Frame), C.POINTER(Frame), C.c_int, C.c_double, C.c_double,
+                                        C.POINTER(Frame), C.POINTER(Stats)]
         L.lgcr_clip_set_timing.argtypes = [vp, C.c_int]
         L.lgcr_clip_stage_times.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
         L.lgcr_clip_fetch.argtypes = [vp, C.c_int, vp, C.c_size_t, C.POINTER(C.c_size_t)]
@@ -157,6 +162,24 @@ def make_format(dtype, width, height, ss_w=1, ss_h=1, bits=None) -> Format:
     return Format(0, bits, dt.itemsize, ss_w, ss_h, width, height)
 
 
+def trecon_params(**kw) -> Params:
+    """TRecon defaults (plugin.cpp:941-976) with overrides."""
+    p = Params()
+    lib().lgcr_params_default_trecon(C.byref(p))
+    if "ar" in kw:
+        kw["ar_margin"] = kw.pop("ar")
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def host_frame(planes):
+    arrs = [np.ascontiguousarray(a) for a in planes]
+    f = Frame((C.c_void_p * 3)(*[a.ctypes.data for a in arrs]), (C.c_ssize_t * 3)(*[a.strides[0] for a in arrs]),
+              MEMORY_HOST)
+    return f, arrs
+
+
 class Device:
     def __init__(self, ordinal=0, budget=0):
         h = C.c_void_p()
@@ -175,9 +198,10 @@ class Device:
 class Clip:
     """One filter instance: parameters + input format, resident on a device."""
 
-    def __init__(self, device: Device, params: Params, fmt: Format, max_inflight=4):
+    def __init__(self, device: Device, params: Params, fmt: Format, max_inflight=4, temporal=False):
         h = C.c_void_p()
-        check(lib().lgcr_clip_create(device.handle, C.byref(params), C.byref(fmt), max_inflight, C.byref(h)))
+        create = lib().lgcr_trecon_create if temporal else lib().lgcr_clip_create
+        check(create(device.handle, C.byref(params), C.byref(fmt), max_inflight, C.byref(h)))
         self.handle = h
         self.fmt = fmt
         self.ow = params.out_w or fmt.width
@@ -202,6 +226,21 @@ class Clip:
         if out[0] is None:
             out[0] = y
         return (out, st) if stats else out
+
+    def trecon_host(self, center, nbrs, shift_x=math.nan, shift_y=math.nan):
+        """TRecon on host planes: center (y, u, v) and a list of neighbours."""
+        fin, keep = host_frame(center)
+        n = len(nbrs)
+        fn = (Frame * max(1, n))()
+        keeps = [keep]
+        for k, f in enumerate(nbrs):
+            fn[k], kk = host_frame(f)
+            keeps.append(kk)
+        dt = center[0].dtype
+        out = [np.empty((self.oh, self.ow), dt) for _ in range(3)]
+        fout, _ = host_frame(out)
+        check(lib().lgcr_trecon_frame(self.handle, C.byref(fin), fn, n, shift_x, shift_y, C.byref(fout), None))
+        return out
 
     def recon_device(self, planes_in, planes_out, shift_x=math.nan, shift_y=math.nan):
         """Device pointers: planes_* are sequences of (ptr, stride_bytes)."""
