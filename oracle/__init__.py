@@ -131,6 +131,7 @@ def lib():
         L.or_lgf_maps.argtypes = [fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                   C.c_double, fp, fp, C.c_double, C.c_int, u8p, fp]
         L.or_back_project.argtypes = [fp, C.c_int, C.c_int, fp, C.c_int, C.c_int, C.c_float]
+        L.or_collaborative.argtypes = [fp, C.c_int, C.c_int, fp, fp, C.c_int, C.c_int, C.c_float]
         L.or_recon_frame.argtypes = [C.POINTER(Params), C.POINTER(FrameFormat), C.c_void_p * 3,
                                      C.c_ssize_t * 3, C.c_void_p * 3, C.c_ssize_t * 3]
         L.or_last_debug.restype = C.POINTER(Debug)
@@ -220,6 +221,17 @@ def recon_frame(params: Params, y, u, v, bits=None, ss_w=1, ss_h=1):
     if rc != 0:
         raise RuntimeError(f"oracle recon_frame failed ({rc})")
     return outs
+
+
+def collaborative(guide, u, v, sigma):
+    """Collaborative chroma filter (bm.cpp:250-442) on float planes; returns
+    filtered copies of (u, v).  guide may differ in size (bilinear resize)."""
+    g = np.ascontiguousarray(guide, np.float32)
+    uo = np.array(u, np.float32, copy=True)
+    vo = np.array(v, np.float32, copy=True)
+    h, w = uo.shape
+    lib().or_collaborative(fptr(g), g.shape[1], g.shape[0], fptr(uo), fptr(vo), w, h, float(sigma))
+    return uo, vo
 
 
 def last_debug():

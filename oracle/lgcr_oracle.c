@@ -1957,6 +1957,230 @@ static void recon_algo6(const or_params *p, or_debug *D, const or_axis *ax, cons
 }
 
 /* ------------------------------------------------------------------------ */
+/* collaborative (BM3D-style) chroma filter: bm.cpp:15-442                   */
+/* ------------------------------------------------------------------------ */
+
+#define BM_P 8
+#define BM_G 4
+static const float bm_inv_sqrt2 = 0.7071067811865475f;
+
+typedef struct { float cost; int x, y; } bm_match;
+
+static void bm_insert(bm_match *m, int *count, bm_match c) {
+    int pos = mini(*count, BM_G - 1);
+    if (*count == BM_G && c.cost >= m[BM_G - 1].cost)
+        return;
+    if (*count < BM_G)
+        ++*count;
+    while (pos > 0 && c.cost < m[pos - 1].cost) {
+        if (pos < BM_G)
+            m[pos] = m[pos - 1];
+        --pos;
+    }
+    m[pos] = c;
+}
+
+static void haar_fwd(float *v, int n) {
+    float t[BM_P];
+    for (int len = n; len > 1; len /= 2) {
+        const int half = len / 2;
+        for (int i = 0; i < half; ++i) {
+            const float a = v[2 * i], b = v[2 * i + 1];
+            t[i] = (a + b) * bm_inv_sqrt2;
+            t[half + i] = (a - b) * bm_inv_sqrt2;
+        }
+        memcpy(v, t, sizeof(float) * (size_t)len);
+    }
+}
+
+static void haar_inv(float *v, int n) {
+    float t[BM_P];
+    for (int len = 2; len <= n; len *= 2) {
+        const int half = len / 2;
+        for (int i = 0; i < half; ++i) {
+            const float a = v[i], b = v[half + i];
+            t[2 * i] = (a + b) * bm_inv_sqrt2;
+            t[2 * i + 1] = (a - b) * bm_inv_sqrt2;
+        }
+        memcpy(v, t, sizeof(float) * (size_t)len);
+    }
+}
+
+static void haar2d_fwd(float *p) {
+    float col[BM_P];
+    for (int y = 0; y < BM_P; ++y)
+        haar_fwd(p + y * BM_P, BM_P);
+    for (int x = 0; x < BM_P; ++x) {
+        for (int y = 0; y < BM_P; ++y) col[y] = p[y * BM_P + x];
+        haar_fwd(col, BM_P);
+        for (int y = 0; y < BM_P; ++y) p[y * BM_P + x] = col[y];
+    }
+}
+
+static void haar2d_inv(float *p) {
+    float col[BM_P];
+    for (int x = 0; x < BM_P; ++x) {
+        for (int y = 0; y < BM_P; ++y) col[y] = p[y * BM_P + x];
+        haar_inv(col, BM_P);
+        for (int y = 0; y < BM_P; ++y) p[y * BM_P + x] = col[y];
+    }
+    for (int y = 0; y < BM_P; ++y)
+        haar_inv(p + y * BM_P, BM_P);
+}
+
+static int block_starts(int extent, int *starts) {
+    int n = 0;
+    if (extent < BM_P)
+        return 0;
+    const int last = extent - BM_P;
+    for (int pos = 0; pos <= last; pos += 8)
+        starts[n++] = pos;
+    if (n == 0 || starts[n - 1] != last)
+        starts[n++] = last;
+    return n;
+}
+
+static float patch_cost(const float *g, const float *u, const float *v, int w, int ax, int ay, int bx, int by) {
+    float cost = 0.0f;
+    int samples = 0;
+    for (int py = 0; py < BM_P; py += 2)
+        for (int px = 0; px < BM_P; px += 2) {
+            const size_t a = (size_t)(ay + py) * w + ax + px, b = (size_t)(by + py) * w + bx + px;
+            const float dg = g[a] - g[b], du = u[a] - u[b], dv = v[a] - v[b];
+            cost += dg * dg + 0.25f * (du * du + dv * dv);
+            ++samples;
+        }
+    return cost / samples;
+}
+
+/* collaborativeChromaFilter (bm.cpp:250-442); guide is the source luma,
+ * u/v the reconstructed output chroma (modified in place). */
+void or_collaborative(const float *guide, int gw, int gh, float *u, float *v, int w, int h, float guide_sigma) {
+    if (w < BM_P || h < BM_P)
+        return;
+    const size_t n = (size_t)w * h;
+    float *rg = NULL;
+    const float *mg = guide;
+    if (gw != w || gh != h) {
+        rg = (float *)malloc(sizeof(float) * n);
+        const double sx = (double)gw / w, sy = (double)gh / h;
+        for (int y = 0; y < h; ++y) {
+            const double gy = (y + 0.5) * sy - 0.5;
+            for (int x = 0; x < w; ++x)
+                rg[(size_t)y * w + x] = bilinear(guide, gw, gh, (x + 0.5) * sx - 0.5, gy);
+        }
+        mg = rg;
+    }
+    float *su = (float *)malloc(sizeof(float) * n), *sv = (float *)malloc(sizeof(float) * n);
+    float *wt = (float *)calloc(n, sizeof(float));
+    memcpy(su, u, sizeof(float) * n);
+    memcpy(sv, v, sizeof(float) * n);
+    memset(u, 0, sizeof(float) * n);
+    memset(v, 0, sizeof(float) * n);
+    int *sxs = (int *)malloc(sizeof(int) * (size_t)(w / 8 + 2)), *sys = (int *)malloc(sizeof(int) * (size_t)(h / 8 + 2));
+    const int nx = block_starts(w, sxs), ny = block_starts(h, sys);
+    const float sigma = clampf(0.6f * guide_sigma, 0.002f, 0.012f);
+    const float thr = 2.7f * sigma;
+    const float limit = maxf(4e-5f, 4.0f * guide_sigma * guide_sigma);
+    static const float win[BM_P] = {0.25f, 0.50f, 0.75f, 1.0f, 1.0f, 0.75f, 0.50f, 0.25f};
+    for (int iy = 0; iy < ny; ++iy)
+        for (int ix = 0; ix < nx; ++ix) {
+            const int ax = sxs[ix], ay = sys[iy];
+            bm_match m[BM_G];
+            for (int k = 0; k < BM_G; ++k) { m[k].cost = INFINITY; m[k].x = m[k].y = 0; }
+            int mc = 0, cx[25], cy[25], cc = 0;
+            for (int dy = -8; dy <= 8; dy += 4)
+                for (int dx = -8; dx <= 8; dx += 4) {
+                    const int bx = clampi(ax + dx, 0, w - BM_P), by = clampi(ay + dy, 0, h - BM_P);
+                    int dup = 0;
+                    for (int i = 0; i < cc; ++i)
+                        if (cx[i] == bx && cy[i] == by) { dup = 1; break; }
+                    if (dup)
+                        continue;
+                    cx[cc] = bx; cy[cc] = by; ++cc;
+                    bm_match c = { bx == ax && by == ay ? 0.0f : patch_cost(mg, su, sv, w, ax, ay, bx, by), bx, by };
+                    bm_insert(m, &mc, c);
+                }
+            int elig = 0;
+            while (elig < mc && m[elig].cost <= limit)
+                ++elig;
+            const int gc = elig >= 4 ? 4 : (elig >= 2 ? 2 : 1);
+            if (gc < 2)
+                continue;
+            float fu[BM_G * 64], fv[BM_G * 64], mnu[64], mxu[64], mnv[64], mxv[64];
+            memset(fu, 0, sizeof(fu)); memset(fv, 0, sizeof(fv));
+            for (int o = 0; o < 64; ++o) { mnu[o] = mnv[o] = INFINITY; mxu[o] = mxv[o] = -INFINITY; }
+            for (int k = 0; k < gc; ++k)
+                for (int py = 0; py < BM_P; ++py)
+                    for (int px = 0; px < BM_P; ++px) {
+                        const int o = py * BM_P + px;
+                        const size_t q = (size_t)(m[k].y + py) * w + m[k].x + px;
+                        const float a = su[q], b = sv[q];
+                        fu[o * BM_G + k] = a; fv[o * BM_G + k] = b;
+                        mnu[o] = minf(mnu[o], a); mxu[o] = maxf(mxu[o], a);
+                        mnv[o] = minf(mnv[o], b); mxv[o] = maxf(mxv[o], b);
+                    }
+            float patch[64];
+            for (int c = 0; c < 2; ++c) {
+                float *pl = c ? fv : fu;
+                for (int k = 0; k < gc; ++k) {
+                    for (int o = 0; o < 64; ++o) patch[o] = pl[o * BM_G + k];
+                    haar2d_fwd(patch);
+                    for (int o = 0; o < 64; ++o) pl[o * BM_G + k] = patch[o];
+                }
+            }
+            int retained = 0;
+            for (int co = 0; co < 64; ++co) {
+                float *vu = fu + co * BM_G, *vv = fv + co * BM_G;
+                haar_fwd(vu, gc);
+                haar_fwd(vv, gc);
+                for (int k = 0; k < gc; ++k) {
+                    const int dc = co == 0 && k == 0;
+                    if (!dc && fabsf(vu[k]) < thr) vu[k] = 0.0f; else ++retained;
+                    if (!dc && fabsf(vv[k]) < thr) vv[k] = 0.0f; else ++retained;
+                }
+                haar_inv(vu, gc);
+                haar_inv(vv, gc);
+            }
+            for (int c = 0; c < 2; ++c) {
+                float *pl = c ? fv : fu;
+                for (int k = 0; k < gc; ++k) {
+                    for (int o = 0; o < 64; ++o) patch[o] = pl[o * BM_G + k];
+                    haar2d_inv(patch);
+                    for (int o = 0; o < 64; ++o) pl[o * BM_G + k] = patch[o];
+                }
+            }
+            for (int k = 0; k < gc; ++k)
+                for (int o = 0; o < 64; ++o) {
+                    fu[o * BM_G + k] = clampf(fu[o * BM_G + k], mnu[o], mxu[o]);
+                    fv[o * BM_G + k] = clampf(fv[o * BM_G + k], mnv[o], mxv[o]);
+                }
+            const float gwt = 1.0f / (1.0f + retained);
+            for (int k = 0; k < gc; ++k)
+                for (int py = 0; py < BM_P; ++py)
+                    for (int px = 0; px < BM_P; ++px) {
+                        const int o = py * BM_P + px;
+                        const float wgt = gwt * win[px] * win[py];
+                        const size_t q = (size_t)(m[k].y + py) * w + m[k].x + px;
+                        u[q] += wgt * fu[o * BM_G + k];
+                        v[q] += wgt * fv[o * BM_G + k];
+                        wt[q] += wgt;
+                    }
+        }
+    for (size_t i = 0; i < n; ++i) {
+        if (wt[i] > 0.0f) {
+            const float fu = u[i] / wt[i], fv = v[i] / wt[i];
+            u[i] = su[i] + 0.65f * (fu - su[i]);
+            v[i] = sv[i] + 0.65f * (fv - sv[i]);
+        } else {
+            u[i] = su[i];
+            v[i] = sv[i];
+        }
+    }
+    free(sxs); free(sys); free(su); free(sv); free(wt); free(rg);
+}
+
+/* ------------------------------------------------------------------------ */
 /* whole-frame orchestration: plugin.cpp:177-721                             */
 /* ------------------------------------------------------------------------ */
 
@@ -2010,8 +2234,6 @@ int or_recon_frame(const or_params *pin, const or_frame_format *f,
     if (f->ss_h == 0)
         pl.shift_y = 0.0;
     const or_params *p = &pl;
-    if (p->bm)
-        return -2;
     const int sw = f->width, sh = f->height;
     const int cw = sw >> f->ss_w, ch = sh >> f->ss_h;
     const int ow = p->out_w, oh = p->out_h;
@@ -2109,6 +2331,8 @@ int or_recon_frame(const or_params *pin, const or_frame_format *f,
     }
     or_free_axis(&ax);
     or_free_axis(&ay);
+    if (p->bm)
+        or_collaborative(D->y, sw, sh, D->out_u, D->out_v, ow, oh, (float)p->sigma);
 
     /* back-projection, same-size 4:2:0 only (plugin.cpp:374-375, 683-689) */
     if (p->bp > 0.0 && same && cw * 2 == sw && ch * 2 == sh) {
@@ -2313,6 +2537,8 @@ int or_trecon_frame(const or_params *pin, const or_frame_format *f,
     guided_run(p, D->u, D->v, cw, ch, &g, has_mask ? &ws : NULL, NULL, D->out_u, D->out_v);
     if (has_mask)
         or_free_workset(&ws);
+    if (p->bm)
+        or_collaborative(D->y, sw, sh, D->out_u, D->out_v, sw, sh, (float)p->sigma);
     from_float(f, D->out_u, out[1], out_stride[1], sw, sh, os, is_float ? 0.0 : 0.5 * os);
     from_float(f, D->out_v, out[2], out_stride[2], sw, sh, os, is_float ? 0.0 : 0.5 * os);
     for (int k = 0; k < nn; ++k) {

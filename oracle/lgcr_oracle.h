@@ -147,6 +147,11 @@ void or_lgf_maps(const float *y, int w, int h, int cw, int ch, double rw, double
                  double sy, const float *u, const float *v, double eps, int cedge,
                  const uint8_t *roi, float *out6);
 
+/* collaborative chroma filter (bm.cpp:250-442), in place on u/v (w x h)
+ * guided by the source luma (gw x gh) */
+void or_collaborative(const float *guide, int gw, int gh, float *u, float *v, int w, int h,
+                      float guide_sigma);
+
 /* back-projection (algos.cpp:663-677) */
 void or_back_project(float *out, int ow, int oh, const float *src, int cw, int ch, float bp);
 

@@ -118,6 +118,19 @@ CASES = [
     ("ps-420-resize-lanczos4-algo6", 120, 68, 1, 1, 32,
      {"algo": 6, "kernel": "lanczos", "taps": 4, "width": 240, "height": 136}),
     ("p8-420-downscale-algo6", 96, 64, 1, 1, 8, {"algo": 6, "width": 72, "height": 48}),
+    # bm=True collaborative chroma filter after each algo, before bp
+    ("p8-420-bm", 96, 64, 1, 1, 8, {"bm": 1, "cn": 3.0}),
+    ("p10-420-bm-bp", 96, 64, 1, 1, 10, {"bm": 1, "bp": 0.5, "cn": 6.0}),
+    ("ps-444-bm-strength0", 64, 64, 0, 0, 32, {"bm": 1, "strength": 0.0, "cn": 0.006}),
+    ("p8-422-bm-dense", 96, 64, 1, 0, 8, {"bm": 1, "sparse": 0, "cn": 3.0}),
+    ("p8-420-bm-odd", 98, 46, 1, 1, 8, {"bm": 1, "cn": 3.0}),
+    ("p8-444-bm-narrow", 6, 64, 0, 0, 8, {"bm": 1, "cn": 3.0}),
+    ("p8-420-bm-upscale", 96, 64, 1, 1, 8, {"bm": 1, "width": 160, "height": 120, "cn": 3.0}),
+    ("ps-420-bm-downscale-jinc", 96, 64, 1, 1, 32, {"bm": 1, "kernel": "jinc", "width": 72, "height": 40,
+                                                   "cn": 0.01}),
+    ("ps-420-bm-algo4", 96, 64, 1, 1, 32, {"bm": 1, "algo": 4, "cn": 0.01}),
+    ("p8-420-bm-algo6", 96, 64, 1, 1, 8, {"bm": 1, "algo": 6, "cn": 3.0}),
+    ("ps-420-bm-sigma", 96, 64, 1, 1, 32, {"bm": 1, "sigma": 0.002, "cn": 0.004}),
 ]
 
 
@@ -126,7 +139,9 @@ def test_recon_matrix(device, case):
     tag, w, h, ssw, ssh, bits, params = case
     params = dict(params)
     poly = params.pop("poly", 12)
-    y, u, v = frame(w, h, ss_w=ssw, ss_h=ssh, bits=bits, seed=7, polygons=poly)
+    cn = params.pop("cn", 0.0)
+    y, u, v = frame(w, h, ss_w=ssw, ss_h=ssh, bits=bits, seed=7, polygons=poly, chroma_noise=cn,
+                    luma_noise=cn if cn else None)
     o_out, _, g_out, clip = run_both(y, u, v, ss_w=ssw, ss_h=ssh, bits=None if bits in (8, 16, 32) else bits,
                                      device=device, **params)
     clip.close()
@@ -172,6 +187,22 @@ def test_config2_algo_1080p_p10_float_stage(device, algo):
             print(f"1080p algo{algo} {name}: mismatches={n} max_abs={mx:g}")
             assert n == 0
         assert _report(f"1080p-algo{algo}", o_out, g_out, False) == 0
+    finally:
+        clip.close()
+
+
+def test_config4_bm_1080p_422p8(device):
+    """1920x1080 YUV422P8 with 3-LSB noise on every plane, algo 2, bm=True,
+    bp 0.5: the float planes after the collaborative filter within 1e-5 and
+    the packed planes within 1 code."""
+    y, u, v = frame(1920, 1080, ss_w=1, ss_h=0, bits=8, seed=3, luma_noise=3.0, chroma_noise=3.0)
+    o_out, dbg, g_out, clip = run_both(y, u, v, ss_w=1, ss_h=0, device=device, bm=1, bp=0.5)
+    try:
+        for name in ("out_u", "out_v"):
+            n, mx = compare_planes(dbg[name], clip.fetch(name), True)
+            print(f"1080p422 bm {name}: mismatches={n} max_abs={mx:g}")
+            assert n == 0
+        assert _report("1080p422-bm", o_out, g_out, False) == 0
     finally:
         clip.close()
 

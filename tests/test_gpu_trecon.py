@@ -22,6 +22,8 @@ CASES = [
     ("p8-420-strength0", 160, 96, 1, 1, 8, 3, 1, {"strength": 0.0}),
     ("p8-420-noms-ar", 160, 96, 1, 1, 8, 3, 1, {"ms": 0.0, "ar": -1.0}),
     ("p8-420-odd", 98, 62, 1, 1, 8, 3, 1, {}),
+    ("p8-420-bm", 160, 96, 1, 1, 8, 3, 1, {"bm": 1}),
+    ("ps-444-bm-strength0", 64, 64, 0, 0, 32, 3, 1, {"bm": 1, "strength": 0.0}),
 ]
 
 

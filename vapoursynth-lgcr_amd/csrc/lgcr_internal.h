@@ -177,6 +177,12 @@ hipError_t launch_detail_transfer(const uint32_t *row_offsets, const uint32_t *i
                                   float ms_strength, float qgate, float strength, float stretch,
                                   float ar, int clamp_hull, float *out_u, float *out_v,
                                   hipStream_t s);
+// collaborative chroma filter (bm.cpp:250-442), in place on u/v (w x h);
+// guide_tmp (w x h floats) is used when the guide differs in size; scratch
+// holds bm_scratch_bytes(w, h) bytes of per-anchor groups.
+size_t bm_scratch_bytes(int w, int h);
+hipError_t launch_collaborative(const float *guide, int gw, int gh, float *u, float *v, int w, int h,
+                                float guide_sigma, float *guide_tmp, void *scratch, hipStream_t s);
 hipError_t launch_back_project(float *out_u, float *out_v, int ow, int oh, const float *u,
                                const float *v, int cw, int ch, float bp, hipStream_t s);
 hipError_t launch_resample_h(const float *src, int sw, int sh, float *dst,

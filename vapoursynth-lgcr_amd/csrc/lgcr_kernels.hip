@@ -1611,6 +1611,292 @@ __global__ __launch_bounds__(kBlock) void k_resample_radial(const float *src, in
     dst[(size_t)oy * dw + ox] = (float)(wsum > 1e-6 ? acc / wsum : 0.0);
 }
 
+// ---------------------------------------------------------------------------
+// collaborative chroma filter (collaborativeChromaFilter, bm.cpp:250-442).
+// Pass 1: one 64-lane block per anchor patch matches up to 4 patches, runs
+// the 2D + group Haar shrinkage in LDS and stores the clamped filtered
+// patches.  Pass 2: every pixel gathers the contributions of the groups that
+// cover it in the reference's anchor order (anchor row, anchor column, group
+// member), so the float sums round exactly as the scalar scatter does.
+// ---------------------------------------------------------------------------
+
+constexpr int kBmP = 8, kBmG = 4;
+constexpr float kBmInvSqrt2 = 0.7071067811865475f;
+
+template <int N> __device__ __forceinline__ void haar_fwd(float *v) {
+    float t[N];
+#pragma unroll
+    for (int len = N; len > 1; len /= 2) {
+        const int half = len / 2;
+#pragma unroll
+        for (int i = 0; i < half; ++i) {
+            const float a = v[2 * i], b = v[2 * i + 1];
+            t[i] = (a + b) * kBmInvSqrt2;
+            t[half + i] = (a - b) * kBmInvSqrt2;
+        }
+#pragma unroll
+        for (int i = 0; i < len; ++i)
+            v[i] = t[i];
+    }
+}
+
+template <int N> __device__ __forceinline__ void haar_inv(float *v) {
+    float t[N];
+#pragma unroll
+    for (int len = 2; len <= N; len *= 2) {
+        const int half = len / 2;
+#pragma unroll
+        for (int i = 0; i < half; ++i) {
+            const float a = v[i], b = v[half + i];
+            t[2 * i] = (a + b) * kBmInvSqrt2;
+            t[2 * i + 1] = (a - b) * kBmInvSqrt2;
+        }
+#pragma unroll
+        for (int i = 0; i < len; ++i)
+            v[i] = t[i];
+    }
+}
+
+// group Haar across the G members of coefficient l with hard threshold,
+// keeping the DC of the DC (bm.cpp:374-393); returns the retained count
+template <int G> __device__ __forceinline__ int bm_shrink(float (*P)[kBmG][64], int l, float thr) {
+    int kept = 0;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        float gv[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k) gv[k] = P[c][k][l];
+        haar_fwd<G>(gv);
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            if (!(l == 0 && k == 0) && fabsf(gv[k]) < thr)
+                gv[k] = 0.0f;
+            else
+                ++kept;
+        }
+        haar_inv<G>(gv);
+#pragma unroll
+        for (int k = 0; k < G; ++k) P[c][k][l] = gv[k];
+    }
+    return kept;
+}
+
+// window weight of patch offset i: 0.25, 0.5, 0.75, 1, 1, 0.75, 0.5, 0.25 (bm.cpp:285-287)
+__device__ __forceinline__ float bm_window(int i) { return 0.25f * (float)(i < 4 ? i + 1 : 8 - i); }
+
+__device__ __forceinline__ int bm_start(int i, int n, int extent) { return i == n - 1 ? extent - kBmP : i * 8; }
+
+struct BmArgs {
+    const float *g;           // match guide, w x h
+    float *u, *v;             // chroma, w x h (filtered in place by pass 2)
+    int w, h, nx, ny;
+    float limit, thr;
+    int *gc;                  // per anchor: group size (0 = no group)
+    int *pos;                 // per anchor: x0, y0, .. x3, y3
+    float *gw;                // per anchor: 1 / (1 + retained)
+    float *filt;              // per anchor: [plane][k][64]
+};
+
+__global__ __launch_bounds__(64) void k_bm_group(BmArgs A) {
+    __shared__ float cost[25];
+    __shared__ int cand[25];          // packed (by << 16) | bx, or -1 for duplicates
+    __shared__ int sel[1 + kBmG];     // group size, then packed (y << 16) | x per member
+    __shared__ float P[2][kBmG][64];
+    __shared__ int retained;
+    const int a = blockIdx.x, l = threadIdx.x;
+    const int iy = a / A.nx, ix = a - iy * A.nx;
+    const int ax = bm_start(ix, A.nx, A.w), ay = bm_start(iy, A.ny, A.h);
+    const int w = A.w;
+    if (l < 25) {
+        const int bx = clampi(ax + (-8 + 4 * (l % 5)), 0, A.w - kBmP);
+        const int by = clampi(ay + (-8 + 4 * (l / 5)), 0, A.h - kBmP);
+        bool dup = false;
+        for (int j = 0; j < l; ++j) {
+            const int jx = clampi(ax + (-8 + 4 * (j % 5)), 0, A.w - kBmP);
+            const int jy = clampi(ay + (-8 + 4 * (j / 5)), 0, A.h - kBmP);
+            dup |= jx == bx && jy == by;
+        }
+        float c = 0.0f;
+        if (!dup && !(bx == ax && by == ay)) {
+            int samples = 0;
+            for (int py = 0; py < kBmP; py += 2)
+                for (int px = 0; px < kBmP; px += 2) {
+                    const size_t qa = (size_t)(ay + py) * w + ax + px, qb = (size_t)(by + py) * w + bx + px;
+                    const float dg = A.g[qa] - A.g[qb], du = A.u[qa] - A.u[qb], dv = A.v[qa] - A.v[qb];
+                    c += dg * dg + 0.25f * (du * du + dv * dv);
+                    ++samples;
+                }
+            c = c / samples;
+        }
+        cost[l] = c;
+        cand[l] = dup ? -1 : (by << 16) | bx;
+    }
+    if (l == 0)
+        retained = 0;
+    __syncthreads();
+    if (l == 0) {
+        // insertMatch (bm.cpp:29-42) over the candidates in scan order
+        float *mc = P[0][0];
+        int *mp = sel + 1;
+        int count = 0;
+        for (int k = 0; k < kBmG; ++k)
+            mp[k] = (ay << 16) | ax;
+        for (int i = 0; i < 25; ++i) {
+            if (cand[i] < 0)
+                continue;
+            const float c = cost[i];
+            int position = count < kBmG - 1 ? count : kBmG - 1;
+            if (count == kBmG && !(c < mc[kBmG - 1]))
+                continue;
+            if (count < kBmG)
+                ++count;
+            while (position > 0 && c < mc[position - 1]) {
+                mc[position] = mc[position - 1];
+                mp[position] = mp[position - 1];
+                --position;
+            }
+            mc[position] = c;
+            mp[position] = cand[i];
+        }
+        int eligible = 0;
+        while (eligible < count && mc[eligible] <= A.limit)
+            ++eligible;
+        sel[0] = eligible >= 4 ? 4 : (eligible >= 2 ? 2 : 0);
+    }
+    __syncthreads();
+    const int gc = sel[0];
+    if (l == 0)
+        A.gc[a] = gc;
+    if (gc == 0)
+        return;
+    int mx[kBmG], my[kBmG];
+#pragma unroll
+    for (int k = 0; k < kBmG; ++k) {
+        mx[k] = sel[1 + k] & 0xffff;
+        my[k] = sel[1 + k] >> 16;
+    }
+    // gather (bm.cpp:353-371); lane l owns patch offset l
+    const int py = l >> 3, px = l & 7;
+    float mnu = INFINITY, mxu = -INFINITY, mnv = INFINITY, mxv = -INFINITY;
+#pragma unroll
+    for (int k = 0; k < kBmG; ++k) {
+        if (k >= gc)
+            break;
+        const size_t q = (size_t)(my[k] + py) * w + mx[k] + px;
+        const float cu = A.u[q], cv = A.v[q];
+        P[0][k][l] = cu;
+        P[1][k][l] = cv;
+        mnu = fmn(mnu, cu); mxu = fmx(mxu, cu);
+        mnv = fmn(mnv, cv); mxv = fmx(mxv, cv);
+    }
+    __syncthreads();
+    // 2D forward: rows then columns of every patch; lane = (plane, k, line)
+    const int tp = l >> 5, tk = (l >> 3) & 3, tl = l & 7;
+    float r[kBmP];
+    if (tk < gc) {
+#pragma unroll
+        for (int i = 0; i < kBmP; ++i) r[i] = P[tp][tk][tl * kBmP + i];
+        haar_fwd<kBmP>(r);
+#pragma unroll
+        for (int i = 0; i < kBmP; ++i) P[tp][tk][tl * kBmP + i] = r[i];
+    }
+    __syncthreads();
+    if (tk < gc) {
+#pragma unroll
+        for (int i = 0; i < kBmP; ++i) r[i] = P[tp][tk][i * kBmP + tl];
+        haar_fwd<kBmP>(r);
+#pragma unroll
+        for (int i = 0; i < kBmP; ++i) P[tp][tk][i * kBmP + tl] = r[i];
+    }
+    __syncthreads();
+    // group transform with hard threshold (bm.cpp:374-393); lane = coefficient
+    const int kept = gc == 4 ? bm_shrink<4>(P, l, A.thr) : bm_shrink<2>(P, l, A.thr);
+    atomicAdd(&retained, kept);
+    __syncthreads();
+    // 2D inverse: columns then rows
+    if (tk < gc) {
+#pragma unroll
+        for (int i = 0; i < kBmP; ++i) r[i] = P[tp][tk][i * kBmP + tl];
+        haar_inv<kBmP>(r);
+#pragma unroll
+        for (int i = 0; i < kBmP; ++i) P[tp][tk][i * kBmP + tl] = r[i];
+    }
+    __syncthreads();
+    if (tk < gc) {
+#pragma unroll
+        for (int i = 0; i < kBmP; ++i) r[i] = P[tp][tk][tl * kBmP + i];
+        haar_inv<kBmP>(r);
+#pragma unroll
+        for (int i = 0; i < kBmP; ++i) P[tp][tk][tl * kBmP + i] = r[i];
+    }
+    __syncthreads();
+    float *f = A.filt + (size_t)a * (2 * kBmG * 64);
+    for (int k = 0; k < gc; ++k) {
+        f[k * 64 + l] = fclamp(P[0][k][l], mnu, mxu);
+        f[(kBmG + k) * 64 + l] = fclamp(P[1][k][l], mnv, mxv);
+    }
+    if (l < kBmG) {
+        A.pos[a * 2 * kBmG + 2 * l] = mx[l];
+        A.pos[a * 2 * kBmG + 2 * l + 1] = my[l];
+    }
+    if (l == 0)
+        A.gw[a] = 1.0f / (1.0f + retained);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bm_aggregate(BmArgs A) {
+    const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
+    if (x >= A.w)
+        return;
+    // anchors whose matches can cover (x, y): start within [p - 15, p + 8]
+    const int ix0 = max(0, (x - 15) / 8 - 1), iy0 = max(0, (y - 15) / 8 - 1);
+    float su = 0.0f, sv = 0.0f, sw = 0.0f;
+    for (int iy = iy0; iy < A.ny; ++iy) {
+        const int ay = bm_start(iy, A.ny, A.h);
+        if (ay > y + 8)
+            break;
+        if (ay < y - 15)
+            continue;
+        for (int ix = ix0; ix < A.nx; ++ix) {
+            const int ax = bm_start(ix, A.nx, A.w);
+            if (ax > x + 8)
+                break;
+            if (ax < x - 15)
+                continue;
+            const int a = iy * A.nx + ix;
+            const int gc = A.gc[a];
+            if (gc == 0)
+                continue;
+            const int *ps = A.pos + a * 2 * kBmG;
+            const float gwt = A.gw[a];
+            const float *f = A.filt + (size_t)a * (2 * kBmG * 64);
+            for (int k = 0; k < gc; ++k) {
+                const int px = x - ps[2 * k], py = y - ps[2 * k + 1];
+                if ((unsigned)px >= (unsigned)kBmP || (unsigned)py >= (unsigned)kBmP)
+                    continue;
+                const float wgt = gwt * bm_window(px) * bm_window(py);
+                su += wgt * f[k * 64 + py * kBmP + px];
+                sv += wgt * f[(kBmG + k) * 64 + py * kBmP + px];
+                sw += wgt;
+            }
+        }
+    }
+    const size_t q = (size_t)y * A.w + x;
+    if (sw > 0.0f) {
+        const float s0 = A.u[q], s1 = A.v[q];
+        const float fu = su / sw, fv = sv / sw;
+        A.u[q] = s0 + 0.65f * (fu - s0);
+        A.v[q] = s1 + 0.65f * (fv - s1);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bm_guide(const float *g, int gw, int gh, float *dst, int w, int h,
+                                                     double sx, double sy) {
+    const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
+    if (x >= w)
+        return;
+    dst[(size_t)y * w + x] = bilinear(g, gw, gh, (x + 0.5) * sx - 0.5, (y + 0.5) * sy - 0.5);
+}
+
 inline dim3 rows_grid(int w, int h, int z = 1) { return dim3((unsigned)((w + kBlock - 1) / kBlock), (unsigned)h, (unsigned)z); }
 
 }  // namespace
@@ -1871,6 +2157,52 @@ hipError_t launch_resample_radial(const float *src, int sw, int sh, float *dst, 
     const int radius = (int)ceil(support * widen);
     hipLaunchKernelGGL(k_resample_radial, rows_grid(dw, dh), dim3(kBlock), 0, s, src, sw, sh, dst, dw, dh, sx, sy,
                        inv_w, radius, lut, lut_n, lut_scale);
+    return hipGetLastError();
+}
+
+size_t bm_scratch_bytes(int w, int h) {
+    if (w < kBmP || h < kBmP)
+        return 0;
+    const int nx = (w - kBmP) / 8 + 1 + ((w - kBmP) % 8 != 0), ny = (h - kBmP) / 8 + 1 + ((h - kBmP) % 8 != 0);
+    const size_t na = (size_t)nx * ny;
+    return na * (sizeof(int) * (1 + 2 * kBmG) + sizeof(float) * (1 + 2 * kBmG * 64)) + 4 * 256;
+}
+
+hipError_t launch_collaborative(const float *guide, int gw, int gh, float *u, float *v, int w, int h,
+                                float guide_sigma, float *guide_tmp, void *scratch, hipStream_t s) {
+    if (w < kBmP || h < kBmP)
+        return hipSuccess;
+    const int nx = (w - kBmP) / 8 + 1 + ((w - kBmP) % 8 != 0), ny = (h - kBmP) / 8 + 1 + ((h - kBmP) % 8 != 0);
+    const size_t na = (size_t)nx * ny;
+    auto carve = [](uint8_t *&p, size_t bytes) {
+        uint8_t *r = p;
+        p += (bytes + 255) & ~size_t(255);
+        return r;
+    };
+    uint8_t *p = static_cast<uint8_t *>(scratch);
+    BmArgs A;
+    A.gc = reinterpret_cast<int *>(carve(p, na * sizeof(int)));
+    A.pos = reinterpret_cast<int *>(carve(p, na * 2 * kBmG * sizeof(int)));
+    A.gw = reinterpret_cast<float *>(carve(p, na * sizeof(float)));
+    A.filt = reinterpret_cast<float *>(carve(p, na * 2 * kBmG * 64 * sizeof(float)));
+    A.u = u;
+    A.v = v;
+    A.w = w;
+    A.h = h;
+    A.nx = nx;
+    A.ny = ny;
+    // bm.cpp:282-284
+    const float sigma = fminf(fmaxf(0.6f * guide_sigma, 0.002f), 0.012f);
+    A.thr = 2.7f * sigma;
+    A.limit = std::max(4e-5f, 4.0f * guide_sigma * guide_sigma);
+    A.g = guide;
+    if (gw != w || gh != h) {
+        hipLaunchKernelGGL(k_bm_guide, rows_grid(w, h), dim3(kBlock), 0, s, guide, gw, gh, guide_tmp, w, h,
+                           (double)gw / w, (double)gh / h);
+        A.g = guide_tmp;
+    }
+    hipLaunchKernelGGL(k_bm_group, dim3((unsigned)na), dim3(64), 0, s, A);
+    hipLaunchKernelGGL(k_bm_aggregate, rows_grid(w, h), dim3(kBlock), 0, s, A);
     return hipGetLastError();
 }
 

@@ -383,6 +383,9 @@ struct Workspace {
     // algo 6
     float *yct = nullptr, *yc1 = nullptr, *yc2 = nullptr, *aff = nullptr, *rec = nullptr, *rec2 = nullptr,
           *nyq = nullptr, *detail_out = nullptr;
+    // collaborative filter: per-anchor groups and the resized match guide
+    void *bm_scratch = nullptr;
+    float *bm_guide = nullptr;
     // stage timeline of frames enqueued with timing on: (stage, event);
     // stage -1 opens a frame.  Events come from a per-workspace pool.
     std::vector<std::pair<int, hipEvent_t>> marks;
@@ -671,6 +674,11 @@ void layout(const lgcr_clip *c, Workspace *w, Carver &cv) {
         w->nyq = cv.take<float>(ns);
         if (!c->same)
             w->detail_out = cv.take<float>(no);
+    }
+    if (c->p.bm) {
+        w->bm_scratch = cv.take<uint8_t>(bm_scratch_bytes(c->ow, c->oh));
+        if (c->ow != c->sw || c->oh != c->sh)
+            w->bm_guide = cv.take<float>(no);
     }
 }
 
@@ -995,6 +1003,12 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
                                        clamp_hull, w->out_u, w->out_v, s));
         HIP_TRY(mark(LGCR_STAGE_APPLY_DETAIL_TRANSFER));
     }
+    // ApplyCollaborativeFilter (plugin.cpp:676-679)
+    if (p.bm) {
+        HIP_TRY(launch_collaborative(w->y, c->sw, c->sh, w->out_u, w->out_v, c->ow, c->oh, float(p.sigma),
+                                     w->bm_guide, w->bm_scratch, s));
+        HIP_TRY(mark(LGCR_STAGE_APPLY_COLLABORATIVE_FILTER));
+    }
     // BackProject, same-size 4:2:0 only (plugin.cpp:374-375, 683-689)
     if (p.bp > 0.0 && c->same && c->cw * 2 == c->sw && c->ch * 2 == c->sh) {
         HIP_TRY(launch_back_project(w->out_u, w->out_v, c->ow, c->oh, w->u, w->v, c->cw, c->ch, float(p.bp), s));
@@ -1125,6 +1139,12 @@ lgcr_status enqueue_trecon(lgcr_clip *c, Workspace *w, const Geometry &g, const 
                           gp.ms_on ? w->ms : nullptr, g.ax, g.ay, g.rd, gp, w->row_off,
                           w->has_ws ? w->indices : nullptr, w->out_u, w->out_v, SelectorDev{}, td, s));
     HIP_TRY(mark(LGCR_STAGE_APPLY_GUIDED_CORRECTION));
+    // collaborative filter after the temporal pass (plugin.cpp:898-899)
+    if (p.bm) {
+        HIP_TRY(launch_collaborative(w->y, c->sw, c->sh, w->out_u, w->out_v, c->ow, c->oh, float(p.sigma),
+                                     w->bm_guide, w->bm_scratch, s));
+        HIP_TRY(mark(LGCR_STAGE_APPLY_COLLABORATIVE_FILTER));
+    }
     const float offs = is_float ? 0.0f : float(0.5 * oscale);
     RawOut ro[2] = {
         {w->out_u, out[1], out_stride[1], c->ow, c->oh, float(oscale), offs, 0, hi},
@@ -1149,8 +1169,6 @@ lgcr_status validate_trecon(const lgcr_params &p, const lgcr_format &f) {
                                            "sigma/sratio/sdb/gsigma/tsad>0, stretch>=0, 0<=trad<=8, 0<=tsearch<=64)");
     if (p.kernel != LGCR_KERNEL_LANCZOS || p.kp1 != 3.0 || p.algo != 2 || p.radial)
         return fail(LGCR_INVALID_ARGUMENT, "TRecon: the base kernel is lanczos 3, algo 2 (plugin.cpp:973-976)");
-    if (p.bm)
-        return fail(LGCR_UNSUPPORTED, "LGCR HIP backend: bm=True is not available in this build");
     return LGCR_OK;
 }
 
@@ -1175,8 +1193,6 @@ lgcr_status validate(const lgcr_params &p, const lgcr_format &f) {
         return fail(LGCR_INVALID_ARGUMENT, "LGCR: taps must be in the range 1..64");
     if (p.kernel < 0 || p.kernel > LGCR_KERNEL_JINC)
         return fail(LGCR_INVALID_ARGUMENT, "LGCR: unknown kernel");
-    if (p.bm)
-        return fail(LGCR_UNSUPPORTED, "LGCR HIP backend: bm=True is not available in this build");
     return LGCR_OK;
 }
 
