@@ -76,8 +76,8 @@ typedef struct lgcr_format {
     int sample_type;          /* 0 integer, 1 float (32-bit) */
     int bits_per_sample;      /* 8..16, or 32 for float */
     int bytes_per_sample;     /* 1, 2 or 4 */
-    int sub_sampling_w;       /* 0 or 1 */
-    int sub_sampling_h;       /* 0 or 1 */
+    int sub_sampling_w;       /* 0 or 1 (Sharpen: 0..2) */
+    int sub_sampling_h;       /* 0 or 1 (Sharpen: 0..2) */
     int width, height;        /* luma size */
 } lgcr_format;
 
@@ -181,6 +181,21 @@ lgcr_status lgcr_trecon_create(lgcr_device *dev, const lgcr_params *p, const lgc
 lgcr_status lgcr_trecon_frame(lgcr_clip *clip, const lgcr_frame *in, const lgcr_frame *nbrs,
                               int n_nbrs, double shift_x, double shift_y, lgcr_frame *out,
                               lgcr_stats *stats);
+
+/* Sharpen (sharpenCreate / sharpenGetFrame, src/plugin.cpp:14-111): the
+ * fields of SharpenData (src/lgcr.h:968-976).  The clip keeps its input
+ * format (any planar YUV; chroma subsampling up to 2 per axis here).  Luma
+ * is sharpened self-guided; subsampled chroma is gated by the footprint luma
+ * of the sharpened luma with MPEG-2 siting (plugin.cpp:41-53).  Destroy
+ * with lgcr_clip_destroy. */
+typedef struct lgcr_sharpen_params {
+    double alpha, sigma, sratio, gspatial;
+    double ar_margin;         /* <0 disables the hull clamp */
+} lgcr_sharpen_params;
+void lgcr_sharpen_params_default(lgcr_sharpen_params *p);
+lgcr_status lgcr_sharpen_create(lgcr_device *dev, const lgcr_sharpen_params *p, const lgcr_format *fmt,
+                                int max_inflight, lgcr_clip **out);
+lgcr_status lgcr_sharpen_frame(lgcr_clip *clip, const lgcr_frame *in, lgcr_frame *out);
 
 /* Per-stage device time accumulated while timing is on: the nanosecond
  * counters of PipelineMetrics (src/pipeline.h:315-336), here measured with

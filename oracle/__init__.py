@@ -131,6 +131,10 @@ def lib():
         L.or_lgf_maps.argtypes = [fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                   C.c_double, fp, fp, C.c_double, C.c_int, u8p, fp]
         L.or_back_project.argtypes = [fp, C.c_int, C.c_int, fp, C.c_int, C.c_int, C.c_float]
+        L.or_sharpen_frame.argtypes = [C.POINTER(SharpenParams), C.POINTER(FrameFormat), C.c_void_p * 3,
+                                       C.c_ssize_t * 3, C.c_void_p * 3, C.c_ssize_t * 3]
+        L.or_sharpen_plane.argtypes = [fp, C.c_int, C.c_int, fp, fp, C.c_int, C.c_int, C.c_double, C.c_double,
+                                       C.c_double, C.c_double, C.POINTER(SharpenParams)]
         L.or_collaborative.argtypes = [fp, C.c_int, C.c_int, fp, fp, C.c_int, C.c_int, C.c_float]
         L.or_recon_frame.argtypes = [C.POINTER(Params), C.POINTER(FrameFormat), C.c_void_p * 3,
                                      C.c_ssize_t * 3, C.c_void_p * 3, C.c_ssize_t * 3]
@@ -221,6 +225,43 @@ def recon_frame(params: Params, y, u, v, bits=None, ss_w=1, ss_h=1):
     if rc != 0:
         raise RuntimeError(f"oracle recon_frame failed ({rc})")
     return outs
+
+
+class SharpenParams(C.Structure):
+    _fields_ = [("alpha", C.c_double), ("sigma", C.c_double), ("sratio", C.c_double), ("gspatial", C.c_double),
+                ("ar_margin", C.c_double)]
+
+
+def sharpen_params(alpha=0.3, sigma=0.01, sratio=0.15, gspatial=1.2, ar=0.0):
+    """Sharpen defaults (plugin.cpp:94-98)."""
+    return SharpenParams(alpha, sigma, sratio, gspatial, ar)
+
+
+def sharpen_frame(params, y, u, v, bits=None, ss_w=1, ss_h=1):
+    """Whole Sharpen frame (sharpenGetFrame, plugin.cpp:14-90) on host planes."""
+    is_float = y.dtype == np.float32
+    if bits is None:
+        bits = 32 if is_float else (8 if y.dtype == np.uint8 else 16)
+    fmt = FrameFormat(1 if is_float else 0, bits, y.dtype.itemsize, ss_w, ss_h, y.shape[1], y.shape[0])
+    ins = [np.ascontiguousarray(a) for a in (y, u, v)]
+    outs = [np.zeros_like(a) for a in ins]
+    rc = lib().or_sharpen_frame(C.byref(params), C.byref(fmt), (C.c_void_p * 3)(*[a.ctypes.data for a in ins]),
+                                (C.c_ssize_t * 3)(*[a.strides[0] for a in ins]),
+                                (C.c_void_p * 3)(*[a.ctypes.data for a in outs]),
+                                (C.c_ssize_t * 3)(*[a.strides[0] for a in outs]))
+    if rc != 0:
+        raise RuntimeError(f"oracle sharpen_frame failed ({rc})")
+    return outs
+
+
+def sharpen_plane(params, p, lc=None, gy=None, rw=1.0, rh=1.0, sx=0.0, sy=0.0):
+    """sharpenPlane on one float plane; returns the sharpened copy."""
+    out = np.array(p, np.float32, copy=True)
+    h, w = out.shape
+    g = np.ascontiguousarray(gy if gy is not None else out, np.float32)
+    lcp = fptr(np.ascontiguousarray(lc, np.float32)) if lc is not None else None
+    lib().or_sharpen_plane(fptr(out), w, h, lcp, fptr(g), g.shape[1], g.shape[0], rw, rh, sx, sy, C.byref(params))
+    return out
 
 
 def collaborative(guide, u, v, sigma):

@@ -2223,6 +2223,99 @@ static void from_float(const or_frame_format *f, const float *src, void *dst, pt
     }
 }
 
+/* ------------------------------------------------------------------------ */
+/* Sharpen: sharpenPlane (algos.cpp:595-659) and sharpenGetFrame            */
+/* (plugin.cpp:14-90)                                                       */
+/* ------------------------------------------------------------------------ */
+
+static void sort4(float *v) {
+    for (int i = 1; i < 4; ++i) {
+        const float t = v[i];
+        int j = i;
+        while (j > 0 && t < v[j - 1]) { v[j] = v[j - 1]; --j; }
+        v[j] = t;
+    }
+}
+
+void or_sharpen_plane(float *p, int w, int h, const float *lc, const float *gy, int gw, int gh,
+                      double rw, double rh, double sx, double sy, const or_sharpen_params *d) {
+    float *src = (float *)malloc(sizeof(float) * (size_t)w * h);
+    memcpy(src, p, sizeof(float) * (size_t)w * h);
+    const float inv_g2 = 1.0f / (float)(d->gspatial * d->gspatial);
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            const float c0 = at(src, w, h, x, y);
+            const float l0 = lc ? bilinear(gy, gw, gh, (x + 0.5) * rw - 0.5 + sx, (y + 0.5) * rh - 0.5 + sy) : c0;
+            float max_dl = 0.0f, lo = 1e30f, hi = -1e30f;
+            for (int j = -2; j <= 2; ++j)
+                for (int i = -2; i <= 2; ++i) {
+                    const float g = lc ? at(lc, w, h, x + i, y + j) : at(src, w, h, x + i, y + j);
+                    max_dl = maxf(max_dl, fabsf(g - l0));
+                    lo = minf(lo, at(src, w, h, x + i, y + j));
+                    hi = maxf(hi, at(src, w, h, x + i, y + j));
+                }
+            float nd[4] = {
+                fabsf(at(src, w, h, x + 1, y) - c0), fabsf(at(src, w, h, x - 1, y) - c0),
+                fabsf(at(src, w, h, x, y + 1) - c0), fabsf(at(src, w, h, x, y - 1) - c0),
+            };
+            sort4(nd);
+            const float med = (nd[1] + nd[2]) * 0.5f;
+            const float sig = maxf(maxf((float)d->sigma, (float)d->sratio * max_dl), 3.0f * med);
+            const float inv_s2 = 1.0f / (sig * sig);
+            float hp = 0.0f, ws = 0.0f;
+            for (int j = -2; j <= 2; ++j)
+                for (int i = -2; i <= 2; ++i) {
+                    if (i == 0 && j == 0)
+                        continue;
+                    const float dl = (lc ? at(lc, w, h, x + i, y + j) : at(src, w, h, x + i, y + j)) - l0;
+                    const float sim = 1.0f / (1.0f + dl * dl * inv_s2);
+                    const float sp = 1.0f / (1.0f + (float)(i * i + j * j) * inv_g2);
+                    const float n = sp * sim;
+                    hp += n * (c0 - at(src, w, h, x + i, y + j));
+                    ws += n;
+                }
+            float out = c0;
+            if (ws > 1e-6f)
+                out = c0 + (float)d->alpha * hp / ws;
+            if (d->ar_margin >= 0.0)
+                out = clampf(out, lo - (float)d->ar_margin, hi + (float)d->ar_margin);
+            p[(size_t)y * w + x] = out;
+        }
+    free(src);
+}
+
+int or_sharpen_frame(const or_sharpen_params *d, const or_frame_format *f, const void *const in[3],
+                     const ptrdiff_t in_stride[3], void *const out[3], const ptrdiff_t out_stride[3]) {
+    const int sw = f->width, sh = f->height;
+    const int cw = sw >> f->ss_w, ch = sh >> f->ss_h;
+    const int is_float = f->sample_type == 1;
+    const double ysc = is_float ? 1.0 : 1.0 / (double)((1 << f->bits) - 1);
+    float *pl[3];
+    const int pw[3] = {sw, cw, cw}, ph[3] = {sh, ch, ch};
+    for (int i = 0; i < 3; ++i) {
+        pl[i] = (float *)malloc(sizeof(float) * (size_t)pw[i] * ph[i]);
+        to_float(f, in[i], in_stride[i], pl[i], pw[i], ph[i], ysc, (i == 0 || is_float) ? 0.0 : -0.5);
+    }
+    or_sharpen_plane(pl[0], sw, sh, NULL, pl[0], sw, sh, 1.0, 1.0, 0.0, 0.0, d);
+    if (cw != sw || ch != sh) {
+        const double rw = (double)sw / cw, rh = (double)sh / ch;
+        float *lc = (float *)malloc(sizeof(float) * (size_t)cw * ch);
+        or_lc_map(pl[0], sw, sh, cw, ch, rw, rh, -0.5, 0.0, lc);
+        or_sharpen_plane(pl[1], cw, ch, lc, pl[0], sw, sh, rw, rh, -0.5, 0.0, d);
+        or_sharpen_plane(pl[2], cw, ch, lc, pl[0], sw, sh, rw, rh, -0.5, 0.0, d);
+        free(lc);
+    } else {
+        or_sharpen_plane(pl[1], cw, ch, NULL, pl[1], cw, ch, 1.0, 1.0, 0.0, 0.0, d);
+        or_sharpen_plane(pl[2], cw, ch, NULL, pl[2], cw, ch, 1.0, 1.0, 0.0, 0.0, d);
+    }
+    const double osc = is_float ? 1.0 : (double)((1 << f->bits) - 1);
+    for (int i = 0; i < 3; ++i) {
+        from_float(f, pl[i], out[i], out_stride[i], pw[i], ph[i], osc, (i == 0 || is_float) ? 0.0 : 0.5 * osc);
+        free(pl[i]);
+    }
+    return 0;
+}
+
 int or_recon_frame(const or_params *pin, const or_frame_format *f,
                    const void *const in[3], const ptrdiff_t in_stride[3],
                    void *const out[3], const ptrdiff_t out_stride[3]) {

@@ -173,6 +173,14 @@ int or_trecon_frame(const or_params *p, const or_frame_format *fmt,
                     int n_nbrs, const void *const *nbr_planes, const ptrdiff_t *nbr_strides,
                     void *const out[3], const ptrdiff_t out_stride[3]);
 
+/* Sharpen (SharpenData, lgcr.h; sharpenPlane algos.cpp:595-659;
+ * sharpenGetFrame plugin.cpp:14-90).  lc == NULL: self-guided plane. */
+typedef struct or_sharpen_params { double alpha, sigma, sratio, gspatial, ar_margin; } or_sharpen_params;
+void or_sharpen_plane(float *p, int w, int h, const float *lc, const float *gy, int gw, int gh,
+                      double rw, double rh, double sx, double sy, const or_sharpen_params *d);
+int or_sharpen_frame(const or_sharpen_params *d, const or_frame_format *f, const void *const in[3],
+                     const ptrdiff_t in_stride[3], void *const out[3], const ptrdiff_t out_stride[3]);
+
 /* block matching and chroma consistency (algos.cpp:683-807) */
 void or_block_match(const float *cur, const float *nbr, int w, int h, int block, int search,
                     float tsad, int16_t *mvx, int16_t *mvy, float *conf, int *bw, int *bh);

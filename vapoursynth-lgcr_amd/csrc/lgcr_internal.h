@@ -177,6 +177,15 @@ hipError_t launch_detail_transfer(const uint32_t *row_offsets, const uint32_t *i
                                   float ms_strength, float qgate, float strength, float stretch,
                                   float ar, int clamp_hull, float *out_u, float *out_v,
                                   hipStream_t s);
+// Sharpen (sharpenPlane, algos.cpp:595-659); scalars narrowed as the
+// reference narrows them (float(d.alpha) etc.)
+struct SharpenParams {
+    float inv_g2, sigma, sratio, alpha, margin;
+    int clamp_hull;
+};
+hipError_t launch_sharpen(const float *src, float *dst, int w, int h, const float *lc, const float *gy, int gw,
+                          int gh, double rw, double rh, double sx, double sy, const SharpenParams &p,
+                          hipStream_t s);
 // collaborative chroma filter (bm.cpp:250-442), in place on u/v (w x h);
 // guide_tmp (w x h floats) is used when the guide differs in size; scratch
 // holds bm_scratch_bytes(w, h) bytes of per-anchor groups.

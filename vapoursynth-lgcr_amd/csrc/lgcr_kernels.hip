@@ -1897,6 +1897,69 @@ __global__ __launch_bounds__(kBlock) void k_bm_guide(const float *g, int gw, int
     dst[(size_t)y * w + x] = bilinear(g, gw, gh, (x + 0.5) * sx - 0.5, (y + 0.5) * sy - 0.5);
 }
 
+// ---------------------------------------------------------------------------
+// Sharpen (sharpenPlane, algos.cpp:595-659): 5x5 luma-gated Laplacian with a
+// slope-median sigma floor and hull clamp; lc == nullptr self-guides.
+// ---------------------------------------------------------------------------
+
+struct SharpenArgs {
+    const float *src;
+    float *dst;
+    int w, h;
+    const float *lc;        // chroma-grid footprint luma, or nullptr
+    const float *gy;        // luma guide (gw x gh) for L0
+    int gw, gh;
+    double rw, rh, sx, sy;
+    float inv_g2, sigma, sratio, alpha, margin;
+    int clamp_hull;
+};
+
+__global__ __launch_bounds__(kBlock) void k_sharpen(SharpenArgs A) {
+    const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
+    if (x >= A.w)
+        return;
+    const int w = A.w, h = A.h;
+    const float *src = A.src;
+    const float *gd = A.lc ? A.lc : src;
+    const float c0 = at(src, w, h, x, y);
+    const float l0 = A.lc ? bilinear(A.gy, A.gw, A.gh, (x + 0.5) * A.rw - 0.5 + A.sx, (y + 0.5) * A.rh - 0.5 + A.sy)
+                          : c0;
+    float max_dl = 0.0f, lo = 1e30f, hi = -1e30f;
+    for (int j = -2; j <= 2; ++j)
+        for (int i = -2; i <= 2; ++i) {
+            const float s = at(src, w, h, x + i, y + j);
+            max_dl = fmx(max_dl, fabsf(at(gd, w, h, x + i, y + j) - l0));
+            lo = fmn(lo, s);
+            hi = fmx(hi, s);
+        }
+    // median of the four neighbour slopes: mean of the middle two
+    const float a = fabsf(at(src, w, h, x + 1, y) - c0), b = fabsf(at(src, w, h, x - 1, y) - c0);
+    const float c = fabsf(at(src, w, h, x, y + 1) - c0), d = fabsf(at(src, w, h, x, y - 1) - c0);
+    const float a1 = fmn(a, b), b1 = fmx(a, b), c1 = fmn(c, d), d1 = fmx(c, d);
+    const float m1 = fmx(a1, c1), m2 = fmn(b1, d1);
+    const float med = (fmn(m1, m2) + fmx(m1, m2)) * 0.5f;
+    const float sig = fmx(fmx(A.sigma, A.sratio * max_dl), 3.0f * med);
+    const float inv_s2 = 1.0f / (sig * sig);
+    float hp = 0.0f, ws = 0.0f;
+    for (int j = -2; j <= 2; ++j)
+        for (int i = -2; i <= 2; ++i) {
+            if (i == 0 && j == 0)
+                continue;
+            const float dl = at(gd, w, h, x + i, y + j) - l0;
+            const float sim = 1.0f / (1.0f + dl * dl * inv_s2);
+            const float sp = 1.0f / (1.0f + (float)(i * i + j * j) * A.inv_g2);
+            const float n = sp * sim;
+            hp += n * (c0 - at(src, w, h, x + i, y + j));
+            ws += n;
+        }
+    float out = c0;
+    if (ws > 1e-6f)
+        out = c0 + A.alpha * hp / ws;
+    if (A.clamp_hull)
+        out = fclamp(out, lo - A.margin, hi + A.margin);
+    A.dst[(size_t)y * w + x] = out;
+}
+
 inline dim3 rows_grid(int w, int h, int z = 1) { return dim3((unsigned)((w + kBlock - 1) / kBlock), (unsigned)h, (unsigned)z); }
 
 }  // namespace
@@ -2157,6 +2220,15 @@ hipError_t launch_resample_radial(const float *src, int sw, int sh, float *dst, 
     const int radius = (int)ceil(support * widen);
     hipLaunchKernelGGL(k_resample_radial, rows_grid(dw, dh), dim3(kBlock), 0, s, src, sw, sh, dst, dw, dh, sx, sy,
                        inv_w, radius, lut, lut_n, lut_scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_sharpen(const float *src, float *dst, int w, int h, const float *lc, const float *gy, int gw,
+                          int gh, double rw, double rh, double sx, double sy, const SharpenParams &p,
+                          hipStream_t s) {
+    SharpenArgs A{src, dst, w, h, lc, gy, gw, gh, rw, rh, sx, sy, p.inv_g2, p.sigma, p.sratio, p.alpha, p.margin,
+                  p.clamp_hull};
+    hipLaunchKernelGGL(k_sharpen, rows_grid(w, h), dim3(kBlock), 0, s, A);
     return hipGetLastError();
 }
 

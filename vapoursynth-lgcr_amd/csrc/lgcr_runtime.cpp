@@ -406,6 +406,10 @@ struct lgcr_clip {
     bool same = true;
     bool capture = false;
     bool temporal = false;  // TRecon clip (neighbour buffers)
+    bool sharpen = false;   // Sharpen clip
+    lgcr_sharpen_params sp{};
+    void *sharpen_tab = nullptr;  // luma footprints of the chroma samples
+    const int *sxb = nullptr, *sxe = nullptr, *syb = nullptr, *sye = nullptr;
     size_t ws_bytes = 0;
     int max_inflight = 4;
     void *lut_dev = nullptr;
@@ -611,6 +615,22 @@ struct Carver {
 
 void layout(const lgcr_clip *c, Workspace *w, Carver &cv) {
     const size_t ns = size_t(c->sw) * c->sh, nc = size_t(c->cw) * c->ch, no = size_t(c->ow) * c->oh;
+    if (c->sharpen) {
+        // converted planes, sharpened planes, footprint luma
+        const size_t nps[3] = {ns, nc, nc};
+        for (int i = 0; i < 3; ++i) {
+            w->raw_in[i] = cv.take<uint8_t>(nps[i] * c->bps);
+            w->raw_out[i] = cv.take<uint8_t>(nps[i] * c->bps);
+        }
+        w->y = cv.take<float>(ns);
+        w->u = cv.take<float>(nc);
+        w->v = cv.take<float>(nc);
+        w->luma_out = cv.take<float>(ns);
+        w->out_u = cv.take<float>(nc);
+        w->out_v = cv.take<float>(nc);
+        w->lc = cv.take<float>(nc);
+        return;
+    }
     w->raw_in[0] = cv.take<uint8_t>(ns * c->bps);
     w->raw_in[1] = cv.take<uint8_t>(nc * c->bps);
     w->raw_in[2] = cv.take<uint8_t>(nc * c->bps);
@@ -1480,6 +1500,8 @@ void lgcr_clip_destroy(lgcr_clip *c) {
         hipFree(c->lut_dev);
     if (c->weights_dev)
         hipFree(c->weights_dev);
+    if (c->sharpen_tab)
+        hipFree(c->sharpen_tab);
     delete c;
 }
 
@@ -1660,6 +1682,181 @@ lgcr_status lgcr_recon_batch(lgcr_clip *c, int n, const lgcr_frame *in, lgcr_fra
     if (timed)
         add_timing(c, us, nst);
     return result;
+}
+
+// ---------------------------------------------------------------------------
+// Sharpen (sharpenCreate / sharpenGetFrame, src/plugin.cpp:14-111)
+// ---------------------------------------------------------------------------
+
+namespace {
+
+lgcr_status enqueue_sharpen(lgcr_clip *c, Workspace *w, const void *const in[3], const ptrdiff_t in_stride[3],
+                            void *const out[3], const ptrdiff_t out_stride[3]) {
+    hipStream_t s = w->stream;
+    const bool is_float = c->f.sample_type == 1;
+    const double yscale = is_float ? 1.0 : 1.0 / double((1 << c->f.bits_per_sample) - 1);
+    const double coff = is_float ? 0.0 : -0.5;
+    RawPlane rp[3] = {
+        {in[0], in_stride[0], c->sw, c->sh, w->y, yscale, 0.0},
+        {in[1], in_stride[1], c->cw, c->ch, w->u, yscale, coff},
+        {in[2], in_stride[2], c->cw, c->ch, w->v, yscale, coff},
+    };
+    HIP_TRY(launch_to_float(rp, 3, c->raw_type, s));
+    SharpenParams sp;
+    sp.inv_g2 = 1.0f / float(c->sp.gspatial * c->sp.gspatial);
+    sp.sigma = float(c->sp.sigma);
+    sp.sratio = float(c->sp.sratio);
+    sp.alpha = float(c->sp.alpha);
+    sp.margin = float(c->sp.ar_margin);
+    sp.clamp_hull = c->sp.ar_margin >= 0.0;
+    // luma self-guide, then chroma guided by the sharpened luma (plugin.cpp:41-53)
+    HIP_TRY(launch_sharpen(w->y, w->luma_out, c->sw, c->sh, nullptr, nullptr, 0, 0, 1.0, 1.0, 0.0, 0.0, sp, s));
+    if (c->cw != c->sw || c->ch != c->sh) {
+        HIP_TRY(launch_lc_map(w->luma_out, c->sw, c->sh, w->lc, c->cw, c->ch, c->sxb, c->sxe, c->syb, c->sye, s));
+        HIP_TRY(launch_sharpen(w->u, w->out_u, c->cw, c->ch, w->lc, w->luma_out, c->sw, c->sh, c->rw, c->rh, -0.5,
+                               0.0, sp, s));
+        HIP_TRY(launch_sharpen(w->v, w->out_v, c->cw, c->ch, w->lc, w->luma_out, c->sw, c->sh, c->rw, c->rh, -0.5,
+                               0.0, sp, s));
+    } else {
+        HIP_TRY(launch_sharpen(w->u, w->out_u, c->cw, c->ch, nullptr, nullptr, 0, 0, 1.0, 1.0, 0.0, 0.0, sp, s));
+        HIP_TRY(launch_sharpen(w->v, w->out_v, c->cw, c->ch, nullptr, nullptr, 0, 0, 1.0, 1.0, 0.0, 0.0, sp, s));
+    }
+    const double oscale = is_float ? 1.0 : double((1 << c->f.bits_per_sample) - 1);
+    const int hi = is_float ? 0 : (1 << c->f.bits_per_sample) - 1;
+    const float coffo = is_float ? 0.0f : float(0.5 * oscale);
+    RawOut ro[3] = {
+        {w->luma_out, out[0], out_stride[0], c->sw, c->sh, float(oscale), 0.0f, 0, hi},
+        {w->out_u, out[1], out_stride[1], c->cw, c->ch, float(oscale), coffo, 0, hi},
+        {w->out_v, out[2], out_stride[2], c->cw, c->ch, float(oscale), coffo, 0, hi},
+    };
+    HIP_TRY(launch_from_float(ro, 3, c->raw_type, s));
+    return LGCR_OK;
+}
+
+}  // namespace
+
+void lgcr_sharpen_params_default(lgcr_sharpen_params *p) {
+    if (!p)
+        return;
+    // sharpenCreate defaults (plugin.cpp:94-98)
+    p->alpha = 0.3;
+    p->sigma = 0.01;
+    p->sratio = 0.15;
+    p->gspatial = 1.2;
+    p->ar_margin = 0.0;
+}
+
+lgcr_status lgcr_sharpen_create(lgcr_device *dev, const lgcr_sharpen_params *params, const lgcr_format *fmt,
+                                int max_inflight, lgcr_clip **out) {
+    if (!dev || !params || !fmt || !out)
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_sharpen_create: NULL argument");
+    *out = nullptr;
+    // sharpenCreate (plugin.cpp:85-105)
+    if (fmt->sample_type == 1 ? fmt->bits_per_sample != 32 : (fmt->bits_per_sample < 8 || fmt->bits_per_sample > 16))
+        return fail(LGCR_INVALID_ARGUMENT, "LGCR Sharpen: planar YUV only");
+    if (fmt->sub_sampling_w < 0 || fmt->sub_sampling_w > 2 || fmt->sub_sampling_h < 0 || fmt->sub_sampling_h > 2 ||
+        fmt->width <= 0 || fmt->height <= 0 || (fmt->width >> fmt->sub_sampling_w) <= 0 ||
+        (fmt->height >> fmt->sub_sampling_h) <= 0)
+        return fail(LGCR_INVALID_ARGUMENT, "LGCR Sharpen: planar YUV only");
+    if (params->sigma <= 0.0 || params->sratio <= 0.0 || params->gspatial <= 0.0)
+        return fail(LGCR_INVALID_ARGUMENT, "LGCR Sharpen: sigma/sratio/gspatial must be > 0");
+    auto c = std::make_unique<lgcr_clip>();
+    c->dev = dev;
+    c->sharpen = true;
+    c->sp = *params;
+    c->f = *fmt;
+    c->sw = c->ow = fmt->width;
+    c->sh = c->oh = fmt->height;
+    c->cw = fmt->width >> fmt->sub_sampling_w;
+    c->ch = fmt->height >> fmt->sub_sampling_h;
+    c->rw = double(c->sw) / c->cw;
+    c->rh = double(c->sh) / c->ch;
+    c->raw_type = fmt->sample_type == 1 ? RAW_F32 : (fmt->bytes_per_sample == 1 ? RAW_U8 : RAW_U16);
+    c->bps = fmt->sample_type == 1 ? 4 : fmt->bytes_per_sample;
+    c->max_inflight = max_inflight > 0 ? max_inflight : 4;
+    HIP_TRY(hipSetDevice(dev->ordinal));
+    if (c->cw != c->sw || c->ch != c->sh) {
+        // footprint luma map with MPEG-2 siting (plugin.cpp:46-48)
+        std::vector<int> xb, xe, yb, ye;
+        footprint(c->cw, c->rw, -0.5, c->sw, xb, xe);
+        footprint(c->ch, c->rh, 0.0, c->sh, yb, ye);
+        Blob b;
+        const size_t o0 = b.add(xb), o1 = b.add(xe), o2 = b.add(yb), o3 = b.add(ye);
+        HIP_TRY(hipMalloc(&c->sharpen_tab, b.host.size()));
+        HIP_TRY(hipMemcpy(c->sharpen_tab, b.host.data(), b.host.size(), hipMemcpyHostToDevice));
+        const uint8_t *base = static_cast<const uint8_t *>(c->sharpen_tab);
+        c->sxb = reinterpret_cast<const int *>(base + o0);
+        c->sxe = reinterpret_cast<const int *>(base + o1);
+        c->syb = reinterpret_cast<const int *>(base + o2);
+        c->sye = reinterpret_cast<const int *>(base + o3);
+    }
+    std::unique_ptr<Workspace> w;
+    const lgcr_status st = create_workspace(c.get(), w);
+    if (st != LGCR_OK) {
+        lgcr_clip_destroy(c.release());
+        return st;
+    }
+    c->ws_bytes = w->bytes;
+    c->spaces.push_back(std::move(w));
+    c->idle.push_back(0);
+    *out = c.release();
+    return LGCR_OK;
+}
+
+lgcr_status lgcr_sharpen_frame(lgcr_clip *c, const lgcr_frame *in, lgcr_frame *out) {
+    if (!c || !in || !out)
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_sharpen_frame: NULL argument");
+    if (!c->sharpen)
+        return fail(LGCR_INVALID_CONTEXT, "lgcr_sharpen_frame: clip was not created by lgcr_sharpen_create");
+    if (in->memory != out->memory)
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_sharpen_frame: input and output must be in the same memory");
+    for (int i = 0; i < 3; ++i)
+        if (!in->data[i] || !out->data[i])
+            return fail(LGCR_INVALID_ARGUMENT, "lgcr_sharpen_frame: plane %d is NULL", i);
+    int slot = -1;
+    lgcr_status st = acquire(c, &slot);
+    if (st != LGCR_OK)
+        return st;
+    Workspace *w = c->spaces[slot].get();
+    struct Releaser {
+        lgcr_clip *c;
+        int slot;
+        ~Releaser() { release(c, slot); }
+    } rel{c, slot};
+    HIP_TRY(hipSetDevice(c->dev->ordinal));
+    const int pw[3] = {c->sw, c->cw, c->cw}, ph[3] = {c->sh, c->ch, c->ch};
+    const void *din[3];
+    ptrdiff_t din_stride[3];
+    void *dout[3];
+    ptrdiff_t dout_stride[3];
+    const bool host = in->memory == LGCR_MEMORY_HOST;
+    for (int i = 0; i < 3; ++i) {
+        const size_t rb = size_t(pw[i]) * c->bps;
+        if (host) {
+            HIP_TRY(hipMemcpy2DAsync(w->raw_in[i], rb, in->data[i], in->stride[i], rb, ph[i], hipMemcpyHostToDevice,
+                                     w->stream));
+            din[i] = w->raw_in[i];
+            din_stride[i] = ptrdiff_t(rb);
+            dout[i] = w->raw_out[i];
+            dout_stride[i] = ptrdiff_t(rb);
+        } else {
+            din[i] = in->data[i];
+            din_stride[i] = in->stride[i];
+            dout[i] = out->data[i];
+            dout_stride[i] = out->stride[i];
+        }
+    }
+    st = enqueue_sharpen(c, w, din, din_stride, dout, dout_stride);
+    if (st != LGCR_OK)
+        return st;
+    if (host)
+        for (int i = 0; i < 3; ++i) {
+            const size_t rb = size_t(pw[i]) * c->bps;
+            HIP_TRY(hipMemcpy2DAsync(out->data[i], out->stride[i], w->raw_out[i], rb, rb, ph[i],
+                                     hipMemcpyDeviceToHost, w->stream));
+        }
+    HIP_TRY(hipStreamSynchronize(w->stream));
+    return LGCR_OK;
 }
 
 lgcr_status lgcr_clip_set_timing(lgcr_clip *c, int enable) {

@@ -31,7 +31,7 @@ SYMBOLS = [
     "lgcr_device_close", "lgcr_device_reserved", "lgcr_clip_create", "lgcr_clip_destroy",
     "lgcr_clip_workspace_bytes", "lgcr_recon_frame", "lgcr_recon_batch", "lgcr_clip_set_timing",
     "lgcr_clip_stage_times", "lgcr_clip_fetch", "lgcr_params_default_trecon", "lgcr_trecon_create",
-    "lgcr_trecon_frame",
+    "lgcr_trecon_frame", "lgcr_sharpen_params_default", "lgcr_sharpen_create", "lgcr_sharpen_frame",
 ]
 STAGE_NAMES = [
     "ConvertInput", "ResampleLuma", "BuildGuideMaps", "BuildTrustMask", "BuildMutualGate",
@@ -53,6 +53,11 @@ class Params(C.Structure):
         ("ar_margin", C.c_double), ("shift_x", C.c_double), ("shift_y", C.c_double),
         ("loc_set", C.c_int), ("out_w", C.c_int), ("out_h", C.c_int),
     ]
+
+
+class SharpenParams(C.Structure):
+    _fields_ = [("alpha", C.c_double), ("sigma", C.c_double), ("sratio", C.c_double), ("gspatial", C.c_double),
+                ("ar_margin", C.c_double)]
 
 
 class Format(C.Structure):
@@ -110,6 +115,9 @@ def lib():
         L.lgcr_clip_set_timing.argtypes = [vp, C.c_int]
         L.lgcr_clip_stage_times.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
         L.lgcr_clip_fetch.argtypes = [vp, C.c_int, vp, C.c_size_t, C.POINTER(C.c_size_t)]
+        L.lgcr_sharpen_params_default.argtypes = [C.POINTER(SharpenParams)]
+        L.lgcr_sharpen_create.argtypes = [vp, C.POINTER(SharpenParams), C.POINTER(Format), C.c_int, C.POINTER(vp)]
+        L.lgcr_sharpen_frame.argtypes = [vp, C.POINTER(Frame), C.POINTER(Frame)]
         _lib = L
     return _lib
 
@@ -166,6 +174,17 @@ def trecon_params(**kw) -> Params:
     """TRecon defaults (plugin.cpp:941-976) with overrides."""
     p = Params()
     lib().lgcr_params_default_trecon(C.byref(p))
+    if "ar" in kw:
+        kw["ar_margin"] = kw.pop("ar")
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def sharpen_params(**kw) -> SharpenParams:
+    """Sharpen defaults (plugin.cpp:94-98) with overrides; ar -> ar_margin."""
+    p = SharpenParams()
+    lib().lgcr_sharpen_params_default(C.byref(p))
     if "ar" in kw:
         kw["ar_margin"] = kw.pop("ar")
     for k, v in kw.items():
@@ -279,6 +298,35 @@ class Clip:
         if needed.value:
             check(lib().lgcr_clip_fetch(self.handle, FETCH[what], buf.ctypes.data, needed.value, None))
         return buf
+
+    def close(self):
+        if self.handle:
+            lib().lgcr_clip_destroy(self.handle)
+            self.handle = None
+
+
+class SharpenClip:
+    """One Sharpen instance on a device; frames keep their format."""
+
+    def __init__(self, device: Device, params: SharpenParams, fmt: Format, max_inflight=4):
+        h = C.c_void_p()
+        check(lib().lgcr_sharpen_create(device.handle, C.byref(params), C.byref(fmt), max_inflight, C.byref(h)))
+        self.handle = h
+        self.fmt = fmt
+
+    def sharpen_host(self, y, u, v):
+        fin, keep = host_frame((y, u, v))
+        out = [np.empty_like(a) for a in (y, u, v)]
+        fout, _ = host_frame(out)
+        check(lib().lgcr_sharpen_frame(self.handle, C.byref(fin), C.byref(fout)))
+        return out
+
+    def sharpen_device(self, planes_in, planes_out):
+        fin = Frame((C.c_void_p * 3)(*[p for p, _ in planes_in]), (C.c_ssize_t * 3)(*[s for _, s in planes_in]),
+                    MEMORY_DEVICE)
+        fout = Frame((C.c_void_p * 3)(*[p for p, _ in planes_out]),
+                     (C.c_ssize_t * 3)(*[s for _, s in planes_out]), MEMORY_DEVICE)
+        check(lib().lgcr_sharpen_frame(self.handle, C.byref(fin), C.byref(fout)))
 
     def close(self):
         if self.handle:
