@@ -9,6 +9,20 @@
  *
  * Planes are contiguous row-major float arrays (stride == width) unless a
  * stride argument says otherwise.
+ *
+ * Pinning: closed forms, tables, the Sobel tensor, footprint luma, dilation,
+ * workset, plain base, back-projection, LGF regression, the collaborative
+ * filter and Sharpen are checked against the reference's known-answer tests,
+ * brute force, closed forms or independent models in tests/.  Parity
+ * unpinned (only invariants such as sparse == dense, continuity, constants
+ * and identities constrain them; no golden vector exists and the reference
+ * cannot be built without VapourSynth headers): the heuristic internals of
+ * or_guided (metadata probes, knee, ridge and cedge fades, rescue, fallback;
+ * recon.cpp:155-760), or_mutual_gate (maps.cpp:853-908), the LGF cedge fade
+ * and selector weights (algos.cpp:366-593), the algo-6 affine q/stability
+ * maps, Nyquist detail and detail transfer (algos.cpp:1382-1689), the block
+ * match confidence and chroma consistency (algos.cpp:683-807) and the
+ * TRecon temporal taps (recon.cpp:642-745).
  */
 #ifndef LGCR_ORACLE_H
 #define LGCR_ORACLE_H
