@@ -192,6 +192,10 @@ hipError_t launch_sharpen(const float *src, float *dst, int w, int h, const floa
 size_t bm_scratch_bytes(int w, int h);
 hipError_t launch_collaborative(const float *guide, int gw, int gh, float *u, float *v, int w, int h,
                                 float guide_sigma, float *guide_tmp, void *scratch, hipStream_t s);
+// back-projection fused with packing of the two chroma planes (same-size
+// 4:2:0); keep_float also stores the corrected float planes.
+hipError_t launch_bp_pack(float *out_u, float *out_v, int ow, int oh, const float *u, const float *v, int cw,
+                          int ch, float bp, const RawOut *planes, int type, int keep_float, hipStream_t s);
 hipError_t launch_back_project(float *out_u, float *out_v, int ow, int oh, const float *u,
                                const float *v, int cw, int ch, float bp, hipStream_t s);
 hipError_t launch_resample_h(const float *src, int sw, int sh, float *dst,

@@ -87,44 +87,147 @@ __device__ __forceinline__ void sobel(const float *p, int w, int h, int x, int y
 struct InBatch { RawPlane p[3]; };
 struct OutBatch { RawOut p[3]; };
 
+// One block per row and plane; each thread converts columns x, x+256, ...
+// four at a time so several loads are in flight per lane.
+constexpr int kConvUnroll = 4;
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_to_float(InBatch b) {
     const RawPlane &P = b.p[blockIdx.z];
-    const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
-    if (x >= P.w || y >= P.h)
+    const int y = blockIdx.y;
+    if (y >= P.h)
         return;
     const T *row = reinterpret_cast<const T *>(static_cast<const char *>(P.data) + P.stride * y);
-    P.dst[(size_t)y * P.w + x] = (float)((double)row[x] * P.scale + P.offset);
+    float *dst = P.dst + (size_t)y * P.w;
+    const double scale = P.scale, offset = P.offset;
+    for (int x0 = threadIdx.x; x0 < P.w; x0 += kBlock * kConvUnroll) {
+        T v[kConvUnroll];
+#pragma unroll
+        for (int i = 0; i < kConvUnroll; ++i) {
+            const int x = x0 + i * kBlock;
+            v[i] = x < P.w ? row[x] : T(0);
+        }
+#pragma unroll
+        for (int i = 0; i < kConvUnroll; ++i) {
+            const int x = x0 + i * kBlock;
+            if (x < P.w)
+                dst[x] = (float)((double)v[i] * scale + offset);
+        }
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_copy_f32(InBatch b) {
     const RawPlane &P = b.p[blockIdx.z];
-    const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
-    if (x >= P.w || y >= P.h)
+    const int y = blockIdx.y;
+    if (y >= P.h)
         return;
     const float *row = reinterpret_cast<const float *>(static_cast<const char *>(P.data) + P.stride * y);
-    P.dst[(size_t)y * P.w + x] = row[x];
+    float *dst = P.dst + (size_t)y * P.w;
+    for (int x0 = threadIdx.x; x0 < P.w; x0 += kBlock * kConvUnroll) {
+        float v[kConvUnroll];
+#pragma unroll
+        for (int i = 0; i < kConvUnroll; ++i) {
+            const int x = x0 + i * kBlock;
+            v[i] = x < P.w ? row[x] : 0.0f;
+        }
+#pragma unroll
+        for (int i = 0; i < kConvUnroll; ++i) {
+            const int x = x0 + i * kBlock;
+            if (x < P.w)
+                dst[x] = v[i];
+        }
+    }
+}
+
+template <typename T> __device__ __forceinline__ T pack_sample(float s, const RawOut &P) {
+    const int v = (int)(s * P.scale + P.offset + 0.5f);
+    return (T)clampi(v, P.lo, P.hi);
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_from_float(OutBatch b) {
     const RawOut &P = b.p[blockIdx.z];
-    const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
-    if (x >= P.w || y >= P.h)
+    const int y = blockIdx.y;
+    if (y >= P.h)
         return;
     T *row = reinterpret_cast<T *>(static_cast<char *>(P.data) + P.stride * y);
-    const float s = P.src[(size_t)y * P.w + x];
-    const int v = (int)(s * P.scale + P.offset + 0.5f);
-    row[x] = (T)clampi(v, P.lo, P.hi);
+    const float *src = P.src + (size_t)y * P.w;
+    for (int x0 = threadIdx.x; x0 < P.w; x0 += kBlock * kConvUnroll) {
+        float v[kConvUnroll];
+#pragma unroll
+        for (int i = 0; i < kConvUnroll; ++i) {
+            const int x = x0 + i * kBlock;
+            v[i] = x < P.w ? src[x] : 0.0f;
+        }
+#pragma unroll
+        for (int i = 0; i < kConvUnroll; ++i) {
+            const int x = x0 + i * kBlock;
+            if (x < P.w)
+                row[x] = pack_sample<T>(v[i], P);
+        }
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_from_float_f32(OutBatch b) {
     const RawOut &P = b.p[blockIdx.z];
-    const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
-    if (x >= P.w || y >= P.h)
+    const int y = blockIdx.y;
+    if (y >= P.h)
         return;
     float *row = reinterpret_cast<float *>(static_cast<char *>(P.data) + P.stride * y);
-    row[x] = (float)((double)P.src[(size_t)y * P.w + x] * 1.0 + 0.0);
+    const float *src = P.src + (size_t)y * P.w;
+    for (int x0 = threadIdx.x; x0 < P.w; x0 += kBlock * kConvUnroll) {
+        float v[kConvUnroll];
+#pragma unroll
+        for (int i = 0; i < kConvUnroll; ++i) {
+            const int x = x0 + i * kBlock;
+            v[i] = x < P.w ? src[x] : 0.0f;
+        }
+#pragma unroll
+        for (int i = 0; i < kConvUnroll; ++i) {
+            const int x = x0 + i * kBlock;
+            if (x < P.w)
+                row[x] = v[i];
+        }
+    }
+}
+
+// Back-projection fused with output packing (backProject, algos.cpp:663-677,
+// then floatToPlane): same-size 4:2:0 only, where every output pixel lies in
+// exactly one 2x2 block of one chroma sample.  keep_float also writes the
+// corrected float planes back (intermediate capture).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bp_pack(float *ou, float *ov, int ow, int oh, const float *u,
+                                                    const float *v, int cw, int ch, float bp, OutBatch b,
+                                                    int keep_float) {
+    const int cx = blockIdx.x * kBlock + threadIdx.x, cy = blockIdx.y;
+    if (cx >= cw)
+        return;
+    const int xa = min(2 * cx, ow - 1), xb = min(2 * cx + 1, ow - 1);
+    const int ya = min(2 * cy, oh - 1), yb = min(2 * cy + 1, oh - 1);
+    for (int c = 0; c < 2; ++c) {
+        float *o = c ? ov : ou;
+        const float *s = c ? v : u;
+        const RawOut &P = b.p[c];
+        const float p00 = o[(size_t)ya * ow + xa], p10 = o[(size_t)ya * ow + xb];
+        const float p01 = o[(size_t)yb * ow + xa], p11 = o[(size_t)yb * ow + xb];
+        const float r = s[(size_t)cy * cw + cx] - 0.25f * (p00 + p10 + p01 + p11);
+        const float d = bp * r;
+        const float q00 = p00 + d, q10 = p10 + d, q01 = p01 + d, q11 = p11 + d;
+        if (keep_float) {
+            o[(size_t)ya * ow + xa] = q00;
+            o[(size_t)ya * ow + xb] = q10;
+            o[(size_t)yb * ow + xa] = q01;
+            o[(size_t)yb * ow + xb] = q11;
+        }
+        T *ra = reinterpret_cast<T *>(static_cast<char *>(P.data) + P.stride * ya);
+        T *rb = reinterpret_cast<T *>(static_cast<char *>(P.data) + P.stride * yb);
+        if constexpr (sizeof(T) == 4) {
+            ra[xa] = q00; ra[xb] = q10; rb[xa] = q01; rb[xb] = q11;
+        } else {
+            ra[xa] = pack_sample<T>(q00, P); ra[xb] = pack_sample<T>(q10, P);
+            rb[xa] = pack_sample<T>(q01, P); rb[xb] = pack_sample<T>(q11, P);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -716,6 +819,9 @@ __device__ __forceinline__ bool sel_compressed(const uint32_t *row_off, const ui
     return indices && (uint64_t)row_off[oh] * 2 < (uint64_t)ow * (uint64_t)oh;
 }
 
+// Feature switches are template parameters so each launch carries only the
+// code (and registers) of the features it uses.
+template <bool RADIAL, bool TEMPORAL, int SEL, bool CEDGE>
 __device__ void guided_pixel(const GuidedArgs &A, int ox, int oy, uint32_t k) {
     const AxisDev &ax = A.ax, &ay = A.ay;
     const GuideParams &gp = A.gp;
@@ -801,7 +907,7 @@ __device__ void guided_pixel(const GuidedArgs &A, int ox, int oy, uint32_t k) {
     const float inv_s2 = 1.0f / (sig_loc * sig_loc);
     const float ridge_fade = 1.0f - fclamp(2.0f * ridge / (max_dl + 1e-6f), 0.0f, 1.0f);
     float cedge_fade = 1.0f;
-    if (gp.cedge && guided && has_dir) {
+    if (CEDGE && guided && has_dir) {
         const float rc = fmx(mxu - mnu, mxv - mnv);
         if (rc > 1e-6f) {
             float gmax = 0.0f;
@@ -834,14 +940,14 @@ __device__ void guided_pixel(const GuidedArgs &A, int ox, int oy, uint32_t k) {
         rescue = gp.rescue * guide_fade * phase * phase * (1.0f - sim0);
         if (rescue > 0.0f) {
             float mxX = 0, ssX = 0, mxY = 0, ssY = 0;
-            if (!gp.radial)
+            if (!RADIAL)
             for (int i = 0; i < ax.sup; ++i)
                 if (amx[i] > 0.0f) {
                     const float w = wxp[i];
                     mxX = fmx(mxX, fabsf(w));
                     ssX += w * w;
                 }
-            if (!gp.radial)
+            if (!RADIAL)
             for (int j = 0; j < ay.sup; ++j)
                 if (amy[j] > 0.0f) {
                     const float w = wyp[j];
@@ -849,7 +955,7 @@ __device__ void guided_pixel(const GuidedArgs &A, int ox, int oy, uint32_t k) {
                     ssY += w * w;
                 }
             float kappa;
-            if (!gp.radial) {
+            if (!RADIAL) {
                 kappa = (mxX * mxX * mxY * mxY) / (ssX * ssY + 1e-12f);
             } else {
                 const float *base = radial_at(A.rd, ox, oy);
@@ -875,7 +981,7 @@ __device__ void guided_pixel(const GuidedArgs &A, int ox, int oy, uint32_t k) {
     const float inv_g2p = inv_g2 / (aniso * aniso);
 
     // --- tap loop (recon.cpp:581-627)
-    const float *rbase = gp.radial ? radial_at(A.rd, ox, oy) : nullptr;
+    const float *rbase = RADIAL ? radial_at(A.rd, ox, oy) : nullptr;
     double acc_u = 0.0, acc_v = 0.0, wsum = 0.0, sum_abs = 0.0;
     for (int j = 0; j < ay.sup; ++j) {
         if (amy[j] == 0.0f)
@@ -887,7 +993,7 @@ __device__ void guided_pixel(const GuidedArgs &A, int ox, int oy, uint32_t k) {
             if (amx[i] == 0.0f)
                 continue;
             const size_t k = row + clampi(tx0 + i, 0, cw - 1);
-            const float wx = gp.radial ? radial_weight(A.rd, rbase, ax.sup, i, j, tx0, ty0, scx, scy) : wxp[i] * wy;
+            const float wx = RADIAL ? radial_weight(A.rd, rbase, ax.sup, i, j, tx0, ty0, scx, scy) : wxp[i] * wy;
             float w;
             if (guided) {
                 const float dx = ((float)(tx0 + i) - scx) * gp.frw;
@@ -908,7 +1014,7 @@ __device__ void guided_pixel(const GuidedArgs &A, int ox, int oy, uint32_t k) {
         }
     }
     // --- temporal taps of TRecon (recon.cpp:642-745)
-    if (guided && A.tmp.n > 0) {
+    if (TEMPORAL && guided && A.tmp.n > 0) {
         const TemporalDev &T = A.tmp;
         const int lxi = clampi((int)ax.lpos[ox], 0, sw - 1), lyi = clampi((int)ay.lpos[oy], 0, sh - 1);
         const float frw = gp.frw, frh = gp.frh;
@@ -943,7 +1049,7 @@ __device__ void guided_pixel(const GuidedArgs &A, int ox, int oy, uint32_t k) {
             const float sim_raw = 1.0f / (1.0f + dL * dL / (sig_t * sig_t));
             const float sim = stat ? 1.0f - guide_fade * (1.0f - sim_raw) : sim_raw;
             float kw;
-            if (gp.radial) {
+            if (RADIAL) {
                 const float dxc = ddx / frw * T.inv_wx, dyc = ddy / frh * T.inv_wy;
                 kw = lut_lookup(A.rd.lut, A.rd.lut_n, sqrtf(dxc * dxc + dyc * dyc) * A.rd.lut_scale);
             } else {
@@ -982,13 +1088,13 @@ __device__ void guided_pixel(const GuidedArgs &A, int ox, int oy, uint32_t k) {
     }
     const size_t o = (size_t)oy * ax.n + ox;
     const SelectorDev &S = A.sel;
-    if (S.mode != SEL_NONE) {
+    if (SEL != SEL_NONE) {
         // algo-4 selector weights (recon.cpp:762-811)
         const float diag = has_dir ? fmn(1.0f, 2.0f * fabsf(nxv * nyv)) : 0.0f;
         const float hedgy = (1.0f - ss_ramp) * guide_fade;
         const float w2 = hedgy * (1.0f - diag) * S.strength;
         const float w3 = hedgy * diag * S.strength;
-        if (S.mode == SEL_FLOAT) {
+        if (SEL == SEL_FLOAT) {
             if (sel_compressed(A.row_off, A.indices, ax.n, ay.n)) {
                 const float bu = A.out_u[o], bv = A.out_v[o];
                 if (w2 < 1e-4f && w3 < 1e-4f) {
@@ -1030,12 +1136,13 @@ __device__ void guided_pixel(const GuidedArgs &A, int ox, int oy, uint32_t k) {
     A.out_v[o] = (float)vv;
 }
 
+template <bool RADIAL, bool TEMPORAL, int SEL, bool CEDGE>
 __global__ __launch_bounds__(kBlock) void k_guided(GuidedArgs A) {
     const int ow = A.ax.n, oh = A.ay.n;
     const uint32_t total = A.indices ? A.row_off[oh] : (uint32_t)ow * (uint32_t)oh;
     for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < total; k += gridDim.x * kBlock) {
         const uint32_t idx = A.indices ? A.indices[k] : k;
-        guided_pixel(A, (int)(idx % (uint32_t)ow), (int)(idx / (uint32_t)ow), k);
+        guided_pixel<RADIAL, TEMPORAL, SEL, CEDGE>(A, (int)(idx % (uint32_t)ow), (int)(idx / (uint32_t)ow), k);
     }
 }
 
@@ -1976,7 +2083,7 @@ hipError_t launch_to_float(const RawPlane *planes, int count, int type, hipStrea
         mw = max(mw, planes[i].w);
         mh = max(mh, planes[i].h);
     }
-    const dim3 g = rows_grid(mw, mh, count);
+    const dim3 g(1, (unsigned)mh, (unsigned)count);
     if (type == RAW_U8)
         hipLaunchKernelGGL(k_to_float<uint8_t>, g, dim3(kBlock), 0, s, b);
     else if (type == RAW_U16)
@@ -1994,7 +2101,7 @@ hipError_t launch_from_float(const RawOut *planes, int count, int type, hipStrea
         mw = max(mw, planes[i].w);
         mh = max(mh, planes[i].h);
     }
-    const dim3 g = rows_grid(mw, mh, count);
+    const dim3 g(1, (unsigned)mh, (unsigned)count);
     if (type == RAW_U8)
         hipLaunchKernelGGL(k_from_float<uint8_t>, g, dim3(kBlock), 0, s, b);
     else if (type == RAW_U16)
@@ -2080,8 +2187,25 @@ hipError_t launch_guided(const float *u, const float *v, int cw, int ch, const f
                  sel, tmp};
     const size_t n = (size_t)ax.n * ay.n;
     const unsigned blocks = (unsigned)std::min<size_t>(8192, (n + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(k_guided, dim3(blocks), dim3(kBlock), 0, s, A);
-    return hipGetLastError();
+    const bool radial = gp.radial != 0, temporal = tmp.n > 0, cedge = gp.cedge != 0;
+#define LGCR_GUIDED(R, T, S_, C) \
+    if (radial == R && temporal == T && sel.mode == S_ && cedge == C) { \
+        hipLaunchKernelGGL((k_guided<R, T, S_, C>), dim3(blocks), dim3(kBlock), 0, s, A); \
+        return hipGetLastError(); \
+    }
+#define LGCR_GUIDED_SEL(R, T, C) LGCR_GUIDED(R, T, SEL_NONE, C) LGCR_GUIDED(R, T, SEL_MAPS, C) \
+    LGCR_GUIDED(R, T, SEL_FLOAT, C)
+    LGCR_GUIDED_SEL(false, false, false)
+    LGCR_GUIDED_SEL(false, false, true)
+    LGCR_GUIDED_SEL(true, false, false)
+    LGCR_GUIDED_SEL(true, false, true)
+    LGCR_GUIDED(false, true, SEL_NONE, false)
+    LGCR_GUIDED(false, true, SEL_NONE, true)
+    LGCR_GUIDED(true, true, SEL_NONE, false)
+    LGCR_GUIDED(true, true, SEL_NONE, true)
+#undef LGCR_GUIDED_SEL
+#undef LGCR_GUIDED
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_plain_radial(const float *u, const float *v, int cw, int ch, const AxisDev &ax,
@@ -2192,6 +2316,24 @@ hipError_t launch_chroma_consistency(const float *cu, const float *cv, const flo
                                      const int16_t *mvy, int bw, int bh, float *conf, hipStream_t s) {
     hipLaunchKernelGGL(k_chroma_consistency, dim3((unsigned)(bw * bh)), dim3(kBlock), 0, s, cu, cv, nu, nv, cw, ch,
                        luma_w, luma_h, block, mvx, mvy, bw, conf);
+    return hipGetLastError();
+}
+
+hipError_t launch_bp_pack(float *out_u, float *out_v, int ow, int oh, const float *u, const float *v, int cw,
+                          int ch, float bp, const RawOut *planes, int type, int keep_float, hipStream_t s) {
+    OutBatch b{};
+    b.p[0] = planes[0];
+    b.p[1] = planes[1];
+    const dim3 g = rows_grid(cw, ch);
+    if (type == RAW_U8)
+        hipLaunchKernelGGL(k_bp_pack<uint8_t>, g, dim3(kBlock), 0, s, out_u, out_v, ow, oh, u, v, cw, ch, bp, b,
+                           keep_float);
+    else if (type == RAW_U16)
+        hipLaunchKernelGGL(k_bp_pack<uint16_t>, g, dim3(kBlock), 0, s, out_u, out_v, ow, oh, u, v, cw, ch, bp, b,
+                           keep_float);
+    else
+        hipLaunchKernelGGL(k_bp_pack<float>, g, dim3(kBlock), 0, s, out_u, out_v, ow, oh, u, v, cw, ch, bp, b,
+                           keep_float);
     return hipGetLastError();
 }
 

@@ -1029,18 +1029,20 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
                                      w->bm_guide, w->bm_scratch, s));
         HIP_TRY(mark(LGCR_STAGE_APPLY_COLLABORATIVE_FILTER));
     }
-    // BackProject, same-size 4:2:0 only (plugin.cpp:374-375, 683-689)
-    if (p.bp > 0.0 && c->same && c->cw * 2 == c->sw && c->ch * 2 == c->sh) {
-        HIP_TRY(launch_back_project(w->out_u, w->out_v, c->ow, c->oh, w->u, w->v, c->cw, c->ch, float(p.bp), s));
-        HIP_TRY(mark(LGCR_STAGE_BACK_PROJECT));
-    }
-    // ConvertOutput (plugin.cpp:691-711)
+    // BackProject, same-size 4:2:0 only (plugin.cpp:374-375, 683-689), fused
+    // with ConvertOutput (plugin.cpp:691-711)
     const float offs = is_float ? 0.0f : float(0.5 * oscale);
     RawOut ro[2] = {
         {w->out_u, out[1], out_stride[1], c->ow, c->oh, float(oscale), offs, 0, hi},
         {w->out_v, out[2], out_stride[2], c->ow, c->oh, float(oscale), offs, 0, hi},
     };
-    HIP_TRY(launch_from_float(ro, 2, c->raw_type, s));
+    if (p.bp > 0.0 && c->same && c->cw * 2 == c->sw && c->ch * 2 == c->sh) {
+        HIP_TRY(launch_bp_pack(w->out_u, w->out_v, c->ow, c->oh, w->u, w->v, c->cw, c->ch, float(p.bp), ro,
+                               c->raw_type, c->capture ? 1 : 0, s));
+        HIP_TRY(mark(LGCR_STAGE_BACK_PROJECT));
+    } else {
+        HIP_TRY(launch_from_float(ro, 2, c->raw_type, s));
+    }
     HIP_TRY(mark(LGCR_STAGE_CONVERT_OUTPUT));
     return LGCR_OK;
 }
