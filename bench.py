@@ -54,6 +54,33 @@ def cpu_baseline(frames, params_kw, bits, seconds, threads):
     return sum(done) / el, sum(done), el
 
 
+def serving(args):
+    """Clips of mixed resolution / format / filter with Poisson frame
+    arrivals: continuous batching vs one blocking call per frame, at the
+    offered rate and saturated (all requests at t=0)."""
+    from lgcr_amd import abi, serve
+    dev = abi.Device(0)
+    specs = serve.make_mix(args.serve_clips, args.serve_frames, seed=0)
+    clips = [serve.ResidentClip(dev, s, max_inflight=4, threads=min(16, os.cpu_count() or 1)) for s in specs]
+    resident = sum(c.src_bytes + c.out_bytes for c in clips)
+    out = {"clips": len(specs), "frames_per_clip": args.serve_frames, "offered_fps": args.serve_rate,
+           "mix": {k: sum(1 for s in specs if s.kind == k) for k in ("recon", "trecon")},
+           "resident_frames_gb": round(resident / 2**30, 3)}
+    offered = serve.poisson_requests(specs, args.serve_rate, seed=0)
+    saturated = serve.poisson_requests(specs, float("inf"), seed=0)
+    for name, reqs in (("offered", offered), ("saturated", saturated)):
+        for mode in ("continuous", "per_frame"):
+            st = serve.serve(clips, reqs, mode=mode)
+            out[f"{name}_{mode}"] = {"fps": round(st["fps"], 1), "p50_ms": round(st["p50_ms"], 3),
+                                     "p99_ms": round(st["p99_ms"], 3)}
+            out["peak_hbm_gb"] = round(max(out.get("peak_hbm_gb", 0.0),
+                                           st["peak_reserved_gb"] + resident / 2**30), 3)
+    for c in clips:
+        c.release()
+    dev.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +97,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-clips", type=int, default=16)
+    ap.add_argument("--serve-frames", type=int, default=24)
+    ap.add_argument("--serve-rate", type=float, default=600.0)
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -162,6 +193,11 @@ def main():
     }
     clip.close()
     dev.close()
+    del dev_in, dev_out, fin, fout
+    torch.cuda.empty_cache()
+
+    if not args.no_serve:
+        result["serving"] = serving(args)
 
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

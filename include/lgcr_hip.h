@@ -35,7 +35,8 @@ typedef enum lgcr_status {
     LGCR_BACKEND_FAILURE = 3,   /* StageStatus::BackendFailure */
     LGCR_OUT_OF_MEMORY = 4,
     LGCR_REJECTED = 5,          /* admission refused (HBM budget) */
-    LGCR_INVALID_ARGUMENT = 6
+    LGCR_INVALID_ARGUMENT = 6,
+    LGCR_BUSY = 7               /* no free workspace (submit) / not finished (query) */
 } lgcr_status;
 
 /* Resampling kernels, same order as lgcr::Kernel (src/lgcr.h:46). */
@@ -181,6 +182,27 @@ lgcr_status lgcr_trecon_create(lgcr_device *dev, const lgcr_params *p, const lgc
 lgcr_status lgcr_trecon_frame(lgcr_clip *clip, const lgcr_frame *in, const lgcr_frame *nbrs,
                               int n_nbrs, double shift_x, double shift_y, lgcr_frame *out,
                               lgcr_stats *stats);
+
+/* Asynchronous submission for serving loops: the frame is queued on a free
+ * workspace of the clip and the call returns at once with a ticket.  When
+ * every workspace of the clip is busy (and no new one fits the device
+ * budget) the call returns LGCR_BUSY without queuing anything, so a
+ * scheduler can submit another clip's request instead.  Frames must be
+ * device resident.  The workspace stays owned by the ticket until
+ * lgcr_ticket_wait.  This is the non-blocking split of lgcr_recon_frame /
+ * lgcr_trecon_frame (getFrame's arAllFramesReady branch,
+ * src/plugin.cpp:188-721, 740-919), which VapourSynth's own frame
+ * scheduler drives with blocking calls. */
+typedef struct lgcr_ticket lgcr_ticket;
+lgcr_status lgcr_recon_submit(lgcr_clip *clip, const lgcr_frame *in, double shift_x, double shift_y,
+                              const lgcr_frame *out, lgcr_ticket **ticket);
+lgcr_status lgcr_trecon_submit(lgcr_clip *clip, const lgcr_frame *in, const lgcr_frame *nbrs, int n_nbrs,
+                               double shift_x, double shift_y, const lgcr_frame *out, lgcr_ticket **ticket);
+/* LGCR_OK when the ticket's frame is complete, LGCR_BUSY while it runs. */
+lgcr_status lgcr_ticket_query(const lgcr_ticket *ticket);
+/* Block until the frame is complete, return its workspace to the clip and
+ * free the ticket. */
+lgcr_status lgcr_ticket_wait(lgcr_ticket *ticket);
 
 /* Sharpen (sharpenCreate / sharpenGetFrame, src/plugin.cpp:14-111): the
  * fields of SharpenData (src/lgcr.h:968-976).  The clip keeps its input

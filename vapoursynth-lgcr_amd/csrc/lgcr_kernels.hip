@@ -24,6 +24,19 @@ __device__ __forceinline__ float at(const float *p, int w, int h, int x, int y) 
     return p[(size_t)clampi(y, 0, h - 1) * w + clampi(x, 0, w - 1)];
 }
 
+// Correctly rounded 1/y for y >= 1: hardware reciprocal refined by one FMA
+// Newton step.  On gfx950 this equals the IEEE quotient 1.0f / y for every
+// float y in [1, 2^126) (checked exhaustively on the device); larger, inf and
+// NaN arguments take the division.
+__device__ __forceinline__ float recip_ge1(float y) {
+    if (y < 0x1p126f) {
+        const float r = __builtin_amdgcn_rcpf(y);
+        const float e = __builtin_fmaf(-y, r, 1.0f);
+        return __builtin_fmaf(e, r, r);
+    }
+    return 1.0f / y;
+}
+
 // Bilinear sample at double coordinates with clamped taps (lgcr.h:467-475).
 __device__ float bilinear(const float *p, int w, int h, double x, double y) {
     const int x0 = (int)floor(x), y0 = (int)floor(y);
@@ -821,7 +834,7 @@ __device__ __forceinline__ bool sel_compressed(const uint32_t *row_off, const ui
 
 // Feature switches are template parameters so each launch carries only the
 // code (and registers) of the features it uses.
-template <bool RADIAL, bool TEMPORAL, int SEL, bool CEDGE>
+template <bool RADIAL, bool TEMPORAL, int SEL, bool CEDGE, int SUP>
 __device__ void guided_pixel(const GuidedArgs &A, int ox, int oy, uint32_t k) {
     const AxisDev &ax = A.ax, &ay = A.ay;
     const GuideParams &gp = A.gp;
@@ -879,15 +892,16 @@ __device__ void guided_pixel(const GuidedArgs &A, int ox, int oy, uint32_t k) {
 
     // --- pass 1: window luma range + chroma hull (recon.cpp:304-363)
     const int tx0 = ax.start[ox], ty0 = ay.start[oy];
+    const int SX = SUP ? SUP : ax.sup, SY = SUP ? SUP : ay.sup;  // taps per axis
     const float *amx = ax.am + (size_t)ox * ax.sup, *amy = ay.am + (size_t)oy * ay.sup;
     const float *wxp = ax.w + (size_t)ox * ax.sup, *wyp = ay.w + (size_t)oy * ay.sup;
     float max_dl = 0.0f;
     float mnu = 1e30f, mxu = -1e30f, mnv = 1e30f, mxv = -1e30f;
-    for (int j = 0; j < ay.sup; ++j) {
+    _Pragma("unroll") for (int j = 0; j < SY; ++j) {
         if (amy[j] == 0.0f)
             continue;
         const size_t row = (size_t)clampi(ty0 + j, 0, ch - 1) * cw;
-        for (int i = 0; i < ax.sup; ++i) {
+        _Pragma("unroll") for (int i = 0; i < SX; ++i) {
             if (amx[i] == 0.0f)
                 continue;
             const size_t k = row + clampi(tx0 + i, 0, cw - 1);
@@ -912,9 +926,9 @@ __device__ void guided_pixel(const GuidedArgs &A, int ox, int oy, uint32_t k) {
         if (rc > 1e-6f) {
             float gmax = 0.0f;
             const float anx = fabsf(nxv), any = fabsf(nyv);
-            for (int j = 0; j < ay.sup; ++j) {
+            _Pragma("unroll") for (int j = 0; j < SY; ++j) {
                 const int ty = clampi(ty0 + j, 0, ch - 1), ty1 = min(ty + 1, ch - 1);
-                for (int i = 0; i < ax.sup; ++i) {
+                _Pragma("unroll") for (int i = 0; i < SX; ++i) {
                     const int tx = clampi(tx0 + i, 0, cw - 1), tx1 = min(tx + 1, cw - 1);
                     const float c0u = A.u[(size_t)ty * cw + tx], c0v = A.v[(size_t)ty * cw + tx];
                     const float gu = fmx(fabsf(A.u[(size_t)ty * cw + tx1] - c0u) * anx,
@@ -941,14 +955,14 @@ __device__ void guided_pixel(const GuidedArgs &A, int ox, int oy, uint32_t k) {
         if (rescue > 0.0f) {
             float mxX = 0, ssX = 0, mxY = 0, ssY = 0;
             if (!RADIAL)
-            for (int i = 0; i < ax.sup; ++i)
+            _Pragma("unroll") for (int i = 0; i < SX; ++i)
                 if (amx[i] > 0.0f) {
                     const float w = wxp[i];
                     mxX = fmx(mxX, fabsf(w));
                     ssX += w * w;
                 }
             if (!RADIAL)
-            for (int j = 0; j < ay.sup; ++j)
+            _Pragma("unroll") for (int j = 0; j < SY; ++j)
                 if (amy[j] > 0.0f) {
                     const float w = wyp[j];
                     mxY = fmx(mxY, fabsf(w));
@@ -960,13 +974,13 @@ __device__ void guided_pixel(const GuidedArgs &A, int ox, int oy, uint32_t k) {
             } else {
                 const float *base = radial_at(A.rd, ox, oy);
                 float mx = 0, ss = 0;
-                for (int j = 0; j < ay.sup; ++j) {
+                _Pragma("unroll") for (int j = 0; j < SY; ++j) {
                     if (amy[j] == 0.0f)
                         continue;
-                    for (int i = 0; i < ax.sup; ++i) {
+                    _Pragma("unroll") for (int i = 0; i < SX; ++i) {
                         if (amx[i] == 0.0f)
                             continue;
-                        const float w = radial_weight(A.rd, base, ax.sup, i, j, tx0, ty0, scx, scy);
+                        const float w = radial_weight(A.rd, base, SX, i, j, tx0, ty0, scx, scy);
                         mx = fmx(mx, fabsf(w));
                         ss += w * w;
                     }
@@ -983,26 +997,26 @@ __device__ void guided_pixel(const GuidedArgs &A, int ox, int oy, uint32_t k) {
     // --- tap loop (recon.cpp:581-627)
     const float *rbase = RADIAL ? radial_at(A.rd, ox, oy) : nullptr;
     double acc_u = 0.0, acc_v = 0.0, wsum = 0.0, sum_abs = 0.0;
-    for (int j = 0; j < ay.sup; ++j) {
+    _Pragma("unroll") for (int j = 0; j < SY; ++j) {
         if (amy[j] == 0.0f)
             continue;
         const size_t row = (size_t)clampi(ty0 + j, 0, ch - 1) * cw;
         const float wy = wyp[j];
         const float dy = ((float)(ty0 + j) - scy) * gp.frh;
-        for (int i = 0; i < ax.sup; ++i) {
+        _Pragma("unroll") for (int i = 0; i < SX; ++i) {
             if (amx[i] == 0.0f)
                 continue;
             const size_t k = row + clampi(tx0 + i, 0, cw - 1);
-            const float wx = RADIAL ? radial_weight(A.rd, rbase, ax.sup, i, j, tx0, ty0, scx, scy) : wxp[i] * wy;
+            const float wx = RADIAL ? radial_weight(A.rd, rbase, SX, i, j, tx0, ty0, scx, scy) : wxp[i] * wy;
             float w;
             if (guided) {
                 const float dx = ((float)(tx0 + i) - scx) * gp.frw;
                 const float dL = A.lc[k] - L0;
                 const float dp = dx * nxv + dy * nyv;
                 const float dt = -dx * nyv + dy * nxv;
-                const float sim_raw = 1.0f / (1.0f + dL * dL * inv_s2);
+                const float sim_raw = recip_ge1(1.0f + dL * dL * inv_s2);
                 const float sim = 1.0f - guide_fade * (1.0f - sim_raw);
-                const float bump = 1.0f / (1.0f + dp * dp * inv_g2 + dt * dt * inv_g2p);
+                const float bump = recip_ge1(1.0f + dp * dp * inv_g2 + dt * dt * inv_g2p);
                 w = sim * (wx + rescue * bump);
             } else {
                 w = wx;
@@ -1136,13 +1150,13 @@ __device__ void guided_pixel(const GuidedArgs &A, int ox, int oy, uint32_t k) {
     A.out_v[o] = (float)vv;
 }
 
-template <bool RADIAL, bool TEMPORAL, int SEL, bool CEDGE>
+template <bool RADIAL, bool TEMPORAL, int SEL, bool CEDGE, int SUP>
 __global__ __launch_bounds__(kBlock) void k_guided(GuidedArgs A) {
     const int ow = A.ax.n, oh = A.ay.n;
     const uint32_t total = A.indices ? A.row_off[oh] : (uint32_t)ow * (uint32_t)oh;
     for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < total; k += gridDim.x * kBlock) {
         const uint32_t idx = A.indices ? A.indices[k] : k;
-        guided_pixel<RADIAL, TEMPORAL, SEL, CEDGE>(A, (int)(idx % (uint32_t)ow), (int)(idx / (uint32_t)ow), k);
+        guided_pixel<RADIAL, TEMPORAL, SEL, CEDGE, SUP>(A, (int)(idx % (uint32_t)ow), (int)(idx / (uint32_t)ow), k);
     }
 }
 
@@ -2188,9 +2202,15 @@ hipError_t launch_guided(const float *u, const float *v, int cw, int ch, const f
     const size_t n = (size_t)ax.n * ay.n;
     const unsigned blocks = (unsigned)std::min<size_t>(8192, (n + kBlock - 1) / kBlock);
     const bool radial = gp.radial != 0, temporal = tmp.n > 0, cedge = gp.cedge != 0;
+    // 6 taps per axis (Lanczos 3 / Spline36 at chroma upsampling) get fully
+    // unrolled tap loops
+    const int sup6 = (!radial && !cedge && ax.sup == 6 && ay.sup == 6) ? 6 : 0;
 #define LGCR_GUIDED(R, T, S_, C) \
     if (radial == R && temporal == T && sel.mode == S_ && cedge == C) { \
-        hipLaunchKernelGGL((k_guided<R, T, S_, C>), dim3(blocks), dim3(kBlock), 0, s, A); \
+        if (!R && !C && sup6 == 6) \
+            hipLaunchKernelGGL((k_guided<R, T, S_, C, (R || C) ? 0 : 6>), dim3(blocks), dim3(kBlock), 0, s, A); \
+        else \
+            hipLaunchKernelGGL((k_guided<R, T, S_, C, 0>), dim3(blocks), dim3(kBlock), 0, s, A); \
         return hipGetLastError(); \
     }
 #define LGCR_GUIDED_SEL(R, T, C) LGCR_GUIDED(R, T, SEL_NONE, C) LGCR_GUIDED(R, T, SEL_MAPS, C) \

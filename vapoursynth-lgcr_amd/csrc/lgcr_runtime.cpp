@@ -390,6 +390,7 @@ struct Workspace {
     // stage -1 opens a frame.  Events come from a per-workspace pool.
     std::vector<std::pair<int, hipEvent_t>> marks;
     std::vector<hipEvent_t> pool;
+    hipEvent_t done = nullptr;  // completion of a submitted frame
     bool has_ws = false;  // last frame built a sparse workset
 };
 
@@ -735,6 +736,8 @@ void destroy_workspace(lgcr_clip *c, Workspace *w) {
         hipEventDestroy(m.second);
     for (auto ev : w->pool)
         hipEventDestroy(ev);
+    if (w->done)
+        hipEventDestroy(w->done);
     if (w->base) {
         hipFree(w->base);
         c->dev->unreserve(w->bytes);
@@ -766,6 +769,28 @@ lgcr_status acquire(lgcr_clip *c, int *slot) {
         }
         c->ws_cv.wait(lock);
     }
+}
+
+// Non-blocking acquire: an idle workspace, or a new one while the clip is
+// below max_inflight and the device budget admits it; LGCR_BUSY otherwise.
+lgcr_status try_acquire(lgcr_clip *c, int *slot) {
+    std::unique_lock<std::mutex> lock(c->ws_mutex);
+    if (!c->idle.empty()) {
+        *slot = c->idle.back();
+        c->idle.pop_back();
+        return LGCR_OK;
+    }
+    if ((int)c->spaces.size() < c->max_inflight) {
+        std::unique_ptr<Workspace> w;
+        if (create_workspace(c, w) == LGCR_OK) {
+            c->spaces.push_back(std::move(w));
+            *slot = (int)c->spaces.size() - 1;
+            return LGCR_OK;
+        }
+        if (!c->spaces.empty())
+            c->max_inflight = (int)c->spaces.size();
+    }
+    return LGCR_BUSY;
 }
 
 void release(lgcr_clip *c, int slot) {
@@ -1362,6 +1387,17 @@ static lgcr_status create_clip(lgcr_device *dev, const lgcr_params *params, cons
     c->ws_bytes = w->bytes;
     c->spaces.push_back(std::move(w));
     c->idle.push_back(0);
+    // the remaining workspaces up front, as far as the budget admits them,
+    // so steady-state frames never allocate (allocation can stall streams)
+    while ((int)c->spaces.size() < c->max_inflight) {
+        std::unique_ptr<Workspace> more;
+        if (create_workspace(c.get(), more) != LGCR_OK) {
+            c->max_inflight = (int)c->spaces.size();
+            break;
+        }
+        c->spaces.push_back(std::move(more));
+        c->idle.push_back((int)c->spaces.size() - 1);
+    }
     *out = c.release();
     return LGCR_OK;
 }
@@ -1684,6 +1720,152 @@ lgcr_status lgcr_recon_batch(lgcr_clip *c, int n, const lgcr_frame *in, lgcr_fra
     if (timed)
         add_timing(c, us, nst);
     return result;
+}
+
+// ---------------------------------------------------------------------------
+// asynchronous submission (serving loops)
+// ---------------------------------------------------------------------------
+
+struct lgcr_ticket {
+    lgcr_clip *clip;
+    int slot;
+};
+
+namespace {
+
+lgcr_status frame_siting(lgcr_clip *c, double shift_x, double shift_y, Geometry **g) {
+    double sx = std::isnan(shift_x) ? c->p.shift_x : shift_x;
+    double sy = std::isnan(shift_y) ? c->p.shift_y : shift_y;
+    if (c->f.sub_sampling_w == 0)
+        sx = 0.0;
+    if (c->f.sub_sampling_h == 0)
+        sy = 0.0;
+    return get_geometry(c, sx, sy, g);
+}
+
+lgcr_status finish_submit(lgcr_clip *c, int slot, lgcr_status st, lgcr_ticket **ticket) {
+    Workspace *w = c->spaces[slot].get();
+    if (st == LGCR_OK && !w->done) {
+        const hipError_t e = hipEventCreateWithFlags(&w->done, hipEventDisableTiming);
+        if (e != hipSuccess)
+            st = fail(LGCR_BACKEND_FAILURE, "hipEventCreate failed: %s", hipGetErrorString(e));
+    }
+    if (st == LGCR_OK) {
+        const hipError_t e = hipEventRecord(w->done, w->stream);
+        if (e != hipSuccess)
+            st = fail(LGCR_BACKEND_FAILURE, "hipEventRecord failed: %s", hipGetErrorString(e));
+    }
+    if (st != LGCR_OK) {
+        hipStreamSynchronize(w->stream);
+        release(c, slot);
+        return st;
+    }
+    *ticket = new lgcr_ticket{c, slot};
+    return LGCR_OK;
+}
+
+}  // namespace
+
+lgcr_status lgcr_recon_submit(lgcr_clip *c, const lgcr_frame *in, double shift_x, double shift_y,
+                              const lgcr_frame *out, lgcr_ticket **ticket) {
+    if (!c || !in || !out || !ticket)
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_recon_submit: NULL argument");
+    *ticket = nullptr;
+    if (c->temporal || c->sharpen)
+        return fail(LGCR_INVALID_CONTEXT, "lgcr_recon_submit: not a Recon clip");
+    if (in->memory != LGCR_MEMORY_DEVICE || out->memory != LGCR_MEMORY_DEVICE)
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_recon_submit: frames must be device resident");
+    for (int i = 0; i < 3; ++i)
+        if (!in->data[i] || ((i > 0 || !c->same) && !out->data[i]))
+            return fail(LGCR_INVALID_ARGUMENT, "lgcr_recon_submit: plane %d is NULL", i);
+    Geometry *g = nullptr;
+    lgcr_status st = frame_siting(c, shift_x, shift_y, &g);
+    if (st != LGCR_OK)
+        return st;
+    int slot = -1;
+    st = try_acquire(c, &slot);
+    if (st != LGCR_OK)
+        return st;
+    Workspace *w = c->spaces[slot].get();
+    const hipError_t he = hipSetDevice(c->dev->ordinal);
+    if (he != hipSuccess)
+        return finish_submit(c, slot, fail(LGCR_BACKEND_FAILURE, "hipSetDevice failed"), ticket);
+    w->marks.clear();
+    void *dout[3] = {out->data[0], out->data[1], out->data[2]};
+    st = enqueue_frame(c, w, *g, in->data, in->stride, dout, out->stride, false);
+    if (st == LGCR_OK && c->same && dout[0] && dout[0] != in->data[0]) {
+        const hipError_t e = hipMemcpy2DAsync(dout[0], out->stride[0], in->data[0], in->stride[0],
+                                              size_t(c->sw) * c->bps, c->sh, hipMemcpyDeviceToDevice, w->stream);
+        if (e != hipSuccess)
+            st = fail(LGCR_BACKEND_FAILURE, "luma copy failed: %s", hipGetErrorString(e));
+    }
+    return finish_submit(c, slot, st, ticket);
+}
+
+lgcr_status lgcr_trecon_submit(lgcr_clip *c, const lgcr_frame *in, const lgcr_frame *nbrs, int n_nbrs,
+                               double shift_x, double shift_y, const lgcr_frame *out, lgcr_ticket **ticket) {
+    if (!c || !in || !out || !ticket || (n_nbrs > 0 && !nbrs) || n_nbrs < 0)
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_trecon_submit: bad arguments");
+    *ticket = nullptr;
+    if (!c->temporal)
+        return fail(LGCR_INVALID_CONTEXT, "lgcr_trecon_submit: not a TRecon clip");
+    if (n_nbrs > 2 * c->p.trad)
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_trecon_submit: %d neighbours exceed 2*trad", n_nbrs);
+    if (in->memory != LGCR_MEMORY_DEVICE || out->memory != LGCR_MEMORY_DEVICE)
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_trecon_submit: frames must be device resident");
+    for (int k = 0; k < n_nbrs; ++k)
+        if (nbrs[k].memory != LGCR_MEMORY_DEVICE)
+            return fail(LGCR_INVALID_ARGUMENT, "lgcr_trecon_submit: neighbours must be device resident");
+    for (int i = 0; i < 3; ++i)
+        if (!in->data[i] || !out->data[i])
+            return fail(LGCR_INVALID_ARGUMENT, "lgcr_trecon_submit: plane %d is NULL", i);
+    Geometry *g = nullptr;
+    lgcr_status st = frame_siting(c, shift_x, shift_y, &g);
+    if (st != LGCR_OK)
+        return st;
+    int slot = -1;
+    st = try_acquire(c, &slot);
+    if (st != LGCR_OK)
+        return st;
+    Workspace *w = c->spaces[slot].get();
+    const hipError_t he = hipSetDevice(c->dev->ordinal);
+    if (he != hipSuccess)
+        return finish_submit(c, slot, fail(LGCR_BACKEND_FAILURE, "hipSetDevice failed"), ticket);
+    std::vector<const void *> nd(size_t(3) * n_nbrs);
+    std::vector<ptrdiff_t> ns(size_t(3) * n_nbrs);
+    for (int k = 0; k < n_nbrs; ++k)
+        for (int i = 0; i < 3; ++i) {
+            nd[3 * k + i] = nbrs[k].data[i];
+            ns[3 * k + i] = nbrs[k].stride[i];
+        }
+    w->marks.clear();
+    void *dout[3] = {out->data[0], out->data[1], out->data[2]};
+    st = enqueue_trecon(c, w, *g, in->data, in->stride, n_nbrs, nd.data(), ns.data(), dout, out->stride, false);
+    return finish_submit(c, slot, st, ticket);
+}
+
+lgcr_status lgcr_ticket_query(const lgcr_ticket *t) {
+    if (!t)
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_ticket_query: NULL ticket");
+    const hipError_t e = hipEventQuery(t->clip->spaces[t->slot]->done);
+    if (e == hipSuccess)
+        return LGCR_OK;
+    if (e == hipErrorNotReady)
+        return LGCR_BUSY;
+    return fail(LGCR_BACKEND_FAILURE, "hipEventQuery failed: %s", hipGetErrorString(e));
+}
+
+lgcr_status lgcr_ticket_wait(lgcr_ticket *t) {
+    if (!t)
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_ticket_wait: NULL ticket");
+    lgcr_clip *c = t->clip;
+    Workspace *w = c->spaces[t->slot].get();
+    const hipError_t e = hipEventSynchronize(w->done);
+    release(c, t->slot);
+    delete t;
+    if (e != hipSuccess)
+        return fail(LGCR_BACKEND_FAILURE, "hipEventSynchronize failed: %s", hipGetErrorString(e));
+    return LGCR_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -16,7 +16,7 @@ import numpy as np
 from . import paths
 
 LGCR_OK, LGCR_UNSUPPORTED, LGCR_INVALID_CONTEXT, LGCR_BACKEND_FAILURE = 0, 1, 2, 3
-LGCR_OUT_OF_MEMORY, LGCR_REJECTED, LGCR_INVALID_ARGUMENT = 4, 5, 6
+LGCR_OUT_OF_MEMORY, LGCR_REJECTED, LGCR_INVALID_ARGUMENT, LGCR_BUSY = 4, 5, 6, 7
 MEMORY_HOST, MEMORY_DEVICE = 0, 1
 KERNELS = {"bilinear": 0, "bicubic": 1, "lanczos": 2, "spline16": 3, "spline36": 4, "jinc": 5}
 STAGE_COUNT = 19
@@ -32,6 +32,7 @@ SYMBOLS = [
     "lgcr_clip_workspace_bytes", "lgcr_recon_frame", "lgcr_recon_batch", "lgcr_clip_set_timing",
     "lgcr_clip_stage_times", "lgcr_clip_fetch", "lgcr_params_default_trecon", "lgcr_trecon_create",
     "lgcr_trecon_frame", "lgcr_sharpen_params_default", "lgcr_sharpen_create", "lgcr_sharpen_frame",
+    "lgcr_recon_submit", "lgcr_trecon_submit", "lgcr_ticket_query", "lgcr_ticket_wait",
 ]
 STAGE_NAMES = [
     "ConvertInput", "ResampleLuma", "BuildGuideMaps", "BuildTrustMask", "BuildMutualGate",
@@ -115,6 +116,12 @@ def lib():
         L.lgcr_clip_set_timing.argtypes = [vp, C.c_int]
         L.lgcr_clip_stage_times.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
         L.lgcr_clip_fetch.argtypes = [vp, C.c_int, vp, C.c_size_t, C.POINTER(C.c_size_t)]
+        L.lgcr_recon_submit.argtypes = [vp, C.POINTER(Frame), C.c_double, C.c_double, C.POINTER(Frame),
+                                        C.POINTER(vp)]
+        L.lgcr_trecon_submit.argtypes = [vp, C.POINTER(Frame), C.POINTER(Frame), C.c_int, C.c_double, C.c_double,
+                                         C.POINTER(Frame), C.POINTER(vp)]
+        L.lgcr_ticket_query.argtypes = [vp]
+        L.lgcr_ticket_wait.argtypes = [vp]
         L.lgcr_sharpen_params_default.argtypes = [C.POINTER(SharpenParams)]
         L.lgcr_sharpen_create.argtypes = [vp, C.POINTER(SharpenParams), C.POINTER(Format), C.c_int, C.POINTER(vp)]
         L.lgcr_sharpen_frame.argtypes = [vp, C.POINTER(Frame), C.POINTER(Frame)]
