@@ -106,11 +106,14 @@ struct RawOut {
 // Kernel launchers (lgcr_kernels.hip).  All are asynchronous on `s`.
 hipError_t launch_to_float(const RawPlane *planes, int count, int type, hipStream_t s);
 hipError_t launch_from_float(const RawOut *planes, int count, int type, hipStream_t s);
-hipError_t launch_sobel_tensor(const float *y, int w, int h, float *jxx, float *jxy, float *jyy,
-                               uint8_t *seed, float threshold, hipStream_t s);
+// structure tensor written interleaved with the luma: yt = {Y, jxx, jxy, jyy}
+hipError_t launch_sobel_tensor(const float *y, int w, int h, float4 *yt, uint8_t *seed, float threshold,
+                               hipStream_t s);
+// footprint luma; with luv also the interleaved chroma guide {lc, U, V, 0}
 hipError_t launch_lc_map(const float *y, int w, int h, float *lc, int cw, int ch,
                          const int *xb, const int *xe, const int *yb, const int *ye,
-                         hipStream_t s);
+                         hipStream_t s, const float *u = nullptr, const float *v = nullptr,
+                         float4 *luv = nullptr);
 hipError_t launch_dilate(const uint8_t *src, uint8_t *tmp, uint8_t *dst, int w, int h,
                          int radius, hipStream_t s);
 hipError_t launch_workset(const uint8_t *mask, int w, int h, uint32_t *row_offsets,
@@ -132,7 +135,7 @@ hipError_t launch_plain_radial(const float *u, const float *v, int cw, int ch,
                                float *out_u, float *out_v, int clamp_hull, float margin,
                                hipStream_t s);
 hipError_t launch_guided(const float *u, const float *v, int cw, int ch, const float *y,
-                         int sw, int sh, const float *jxx, const float *jxy, const float *jyy,
+                         int sw, int sh, const float4 *yt, const float4 *luv,
                          const float *lc, const float *ms, const AxisDev &ax, const AxisDev &ay,
                          const RadialDev &rd, const GuideParams &gp, const uint32_t *row_offsets,
                          const uint32_t *indices, float *out_u, float *out_v,
@@ -171,7 +174,7 @@ hipError_t launch_detail_sub(const float *y, float *rec, size_t n, hipStream_t s
 hipError_t launch_nyquist(float *a, float *b, int w, int h, int xlevels, int ylevels,
                           float **result, hipStream_t s);
 hipError_t launch_detail_transfer(const uint32_t *row_offsets, const uint32_t *indices,
-                                  const float *jxx, const float *jxy, const float *jyy, int sw,
+                                  const float4 *yt, int sw,
                                   int sh, const AxisDev &ax, const AxisDev &ay, int cw, int ch,
                                   const AffineDev &m, const float *ms, const float *detail,
                                   float ms_strength, float qgate, float strength, float stretch,

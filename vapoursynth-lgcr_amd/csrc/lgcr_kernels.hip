@@ -7,6 +7,7 @@
 #include "lgcr_internal.h"
 
 #include <algorithm>
+#include <climits>
 
 namespace lgcr_hip {
 namespace {
@@ -61,6 +62,33 @@ __device__ float bilinear_fast(const float *p, int w, int h, int x0, float fx, i
     const float *r1 = r0 + w;
     const float a = r0[0], b = r0[1], c = r1[0], d = r1[1];
     return a + (b - a) * fx + (c - a) * fy + (a - b - c + d) * fx * fy;
+}
+
+// bilinear_fast on every component of an interleaved float4 plane.
+__device__ float4 bilinear_fast4(const float4 *p, int w, int h, int x0, float fx, int y0, float fy) {
+    float4 r;
+    if (w == 1) {
+        const float4 a = p[(size_t)y0 * w];
+        if (h == 1)
+            return a;
+        const float4 c = p[(size_t)(y0 + 1) * w];
+        r.x = a.x + (c.x - a.x) * fy; r.y = a.y + (c.y - a.y) * fy;
+        r.z = a.z + (c.z - a.z) * fy; r.w = a.w + (c.w - a.w) * fy;
+        return r;
+    }
+    const float4 *r0 = p + (size_t)y0 * w + x0;
+    const float4 a = r0[0], b = r0[1];
+    if (h == 1) {
+        r.x = a.x + (b.x - a.x) * fx; r.y = a.y + (b.y - a.y) * fx;
+        r.z = a.z + (b.z - a.z) * fx; r.w = a.w + (b.w - a.w) * fx;
+        return r;
+    }
+    const float4 c = r0[w], d = r0[w + 1];
+    r.x = a.x + (b.x - a.x) * fx + (c.x - a.x) * fy + (a.x - b.x - c.x + d.x) * fx * fy;
+    r.y = a.y + (b.y - a.y) * fx + (c.y - a.y) * fy + (a.y - b.y - c.y + d.y) * fx * fy;
+    r.z = a.z + (b.z - a.z) * fx + (c.z - a.z) * fy + (a.z - b.z - c.z + d.z) * fx * fy;
+    r.w = a.w + (b.w - a.w) * fx + (c.w - a.w) * fy + (a.w - b.w - c.w + d.w) * fx * fy;
+    return r;
 }
 
 struct Dir { float coherence, nx, ny; };
@@ -255,8 +283,7 @@ __global__ __launch_bounds__(kBlock) void k_bp_pack(float *ou, float *ov, int ow
 constexpr int TW = 64, TH = 4;
 
 __global__ __launch_bounds__(TW * TH) void k_sobel_tensor(const float *__restrict__ y, int w, int h,
-                                                         float *jxx, float *jxy, float *jyy,
-                                                         uint8_t *seed, float thr) {
+                                                         float4 *yt, uint8_t *seed, float thr) {
     __shared__ float sy[TH + 4][TW + 4];
     __shared__ float gx[TH + 2][TW + 2], gy[TH + 2][TW + 2];
     __shared__ float vxx[TH][TW + 2], vxy[TH][TW + 2], vyy[TH][TW + 2];
@@ -311,20 +338,19 @@ __global__ __launch_bounds__(TW * TH) void k_sobel_tensor(const float *__restric
         oyy = ninth * ((vyy[ly][c - 1] + vyy[ly][c]) + vyy[ly][c + 1]);
     }
     const size_t o = (size_t)r * w + x;
-    jxx[o] = oxx;
-    jxy[o] = oxy;
-    jyy[o] = oyy;
+    yt[o] = make_float4(sy[ly + 2][threadIdx.x + 2], oxx, oxy, oyy);
     if (seed)
         seed[o] = oxx + oyy > thr;
 }
 
 // ---------------------------------------------------------------------------
-// footprint luma map (buildLcMap, maps.cpp:425-499)
+// footprint luma map (buildLcMap, maps.cpp:425-499); with luv it also writes
+// the interleaved chroma guide {lc, U, V, 0} the guided pass gathers from
 // ---------------------------------------------------------------------------
 
 __global__ __launch_bounds__(kBlock) void k_lc_map(const float *__restrict__ y, int w, float *lc, int cw,
                                                    int ch, const int *xb, const int *xe, const int *yb,
-                                                   const int *ye) {
+                                                   const int *ye, const float *u, const float *v, float4 *luv) {
     const int cx = blockIdx.x * kBlock + threadIdx.x, cy = blockIdx.y;
     if (cx >= cw || cy >= ch)
         return;
@@ -333,7 +359,11 @@ __global__ __launch_bounds__(kBlock) void k_lc_map(const float *__restrict__ y, 
     for (int j = y0; j < y1; ++j)
         for (int i = x0; i < x1; ++i)
             sum += y[(size_t)j * w + i];
-    lc[(size_t)cy * cw + cx] = (float)(sum / ((double)(x1 - x0) * (y1 - y0)));
+    const size_t i = (size_t)cy * cw + cx;
+    const float l = (float)(sum / ((double)(x1 - x0) * (y1 - y0)));
+    lc[i] = l;
+    if (luv)
+        luv[i] = make_float4(l, u[i], v[i], 0.0f);
 }
 
 // ---------------------------------------------------------------------------
@@ -773,7 +803,9 @@ struct GuidedArgs {
     int cw, ch;
     const float *y;
     int sw, sh;
-    const float *jxx, *jxy, *jyy, *lc, *ms;
+    const float4 *yt;    // {Y, jxx, jxy, jyy} per luma pixel
+    const float4 *luv;   // {lc, U, V, 0} per chroma sample
+    const float *lc, *ms;
     AxisDev ax, ay;
     RadialDev rd;
     GuideParams gp;
@@ -849,10 +881,9 @@ __device__ void guided_pixel(const GuidedArgs &A, int ox, int oy, uint32_t k) {
     if (guided) {
         const int bxi = ax.lb_i0[ox], byi = ay.lb_i0[oy];
         const float bxf = ax.lb_f[ox], byf = ay.lb_f[oy];
-        L0 = bilinear_fast(A.y, sw, sh, bxi, bxf, byi, byf);
-        const float jxx = bilinear_fast(A.jxx, sw, sh, bxi, bxf, byi, byf);
-        const float jxy = bilinear_fast(A.jxy, sw, sh, bxi, bxf, byi, byf);
-        const float jyy = bilinear_fast(A.jyy, sw, sh, bxi, bxf, byi, byf);
+        const float4 t = bilinear_fast4(A.yt, sw, sh, bxi, bxf, byi, byf);
+        L0 = t.x;
+        const float jxx = t.y, jxy = t.z, jyy = t.w;
         const Dir d = principal_dir(jxx, jxy, jyy);
         coherence = d.coherence;
         nxv = d.nx;
@@ -904,10 +935,10 @@ __device__ void guided_pixel(const GuidedArgs &A, int ox, int oy, uint32_t k) {
         _Pragma("unroll") for (int i = 0; i < SX; ++i) {
             if (amx[i] == 0.0f)
                 continue;
-            const size_t k = row + clampi(tx0 + i, 0, cw - 1);
+            const float4 q = A.luv[row + clampi(tx0 + i, 0, cw - 1)];
             if (guided)
-                max_dl = fmx(max_dl, fabsf(A.lc[k] - L0));
-            const float a = A.u[k], b = A.v[k];
+                max_dl = fmx(max_dl, fabsf(q.x - L0));
+            const float a = q.y, b = q.z;
             mnu = fmn(mnu, a); mxu = fmx(mxu, a);
             mnv = fmn(mnv, b); mxv = fmx(mxv, b);
         }
@@ -1006,12 +1037,12 @@ __device__ void guided_pixel(const GuidedArgs &A, int ox, int oy, uint32_t k) {
         _Pragma("unroll") for (int i = 0; i < SX; ++i) {
             if (amx[i] == 0.0f)
                 continue;
-            const size_t k = row + clampi(tx0 + i, 0, cw - 1);
+            const float4 q = A.luv[row + clampi(tx0 + i, 0, cw - 1)];
             const float wx = RADIAL ? radial_weight(A.rd, rbase, SX, i, j, tx0, ty0, scx, scy) : wxp[i] * wy;
             float w;
             if (guided) {
                 const float dx = ((float)(tx0 + i) - scx) * gp.frw;
-                const float dL = A.lc[k] - L0;
+                const float dL = q.x - L0;
                 const float dp = dx * nxv + dy * nyv;
                 const float dt = -dx * nyv + dy * nxv;
                 const float sim_raw = recip_ge1(1.0f + dL * dL * inv_s2);
@@ -1021,8 +1052,8 @@ __device__ void guided_pixel(const GuidedArgs &A, int ox, int oy, uint32_t k) {
             } else {
                 w = wx;
             }
-            acc_u += (double)w * A.u[k];
-            acc_v += (double)w * A.v[k];
+            acc_u += (double)w * q.y;
+            acc_v += (double)w * q.z;
             wsum += w;
             sum_abs += fabsf(w);
         }
@@ -1436,8 +1467,8 @@ __global__ __launch_bounds__(kBlock) void k_nyq(const float *src, float *dst, in
     dst[(size_t)y * w + x] = 0.25f * a + 0.5f * b + 0.25f * c;
 }
 
-__global__ __launch_bounds__(kBlock) void k_detail_transfer(WorkList L, const float *jxx, const float *jxy,
-                                                            const float *jyy, int sw, int sh, AxisDev ax, AxisDev ay,
+__global__ __launch_bounds__(kBlock) void k_detail_transfer(WorkList L, const float4 *yt, int sw, int sh,
+                                                            AxisDev ax, AxisDev ay,
                                                             int cw, int ch, AffineDev m, const float *ms,
                                                             const float *detail, float mss, float qs, float st,
                                                             float stretch, float ar, int clamp_hull, float *out_u,
@@ -1449,9 +1480,8 @@ __global__ __launch_bounds__(kBlock) void k_detail_transfer(WorkList L, const fl
         const int ox = (int)(idx % (uint32_t)ow), oy = (int)(idx / (uint32_t)ow);
         const int lxi = ax.lb_i0[ox], lyi = ay.lb_i0[oy];
         const float lxf = ax.lb_f[ox], lyf = ay.lb_f[oy];
-        const Dir d = principal_dir(bilinear_fast(jxx, sw, sh, lxi, lxf, lyi, lyf),
-                                    bilinear_fast(jxy, sw, sh, lxi, lxf, lyi, lyf),
-                                    bilinear_fast(jyy, sw, sh, lxi, lxf, lyi, lyf));
+        const float4 t = bilinear_fast4(yt, sw, sh, lxi, lxf, lyi, lyf);
+        const Dir d = principal_dir(t.y, t.z, t.w);
         const int cxi = ax.cb_i0[ox], cyi = ay.cb_i0[oy];
         const float cxf = ax.cb_f[ox], cyf = ay.cb_f[oy];
         const float q = bilinear_fast(m.g, cw, ch, cxi, cxf, cyi, cyf);
@@ -2125,17 +2155,19 @@ hipError_t launch_from_float(const RawOut *planes, int count, int type, hipStrea
     return hipGetLastError();
 }
 
-hipError_t launch_sobel_tensor(const float *y, int w, int h, float *jxx, float *jxy, float *jyy,
-                               uint8_t *seed, float threshold, hipStream_t s) {
+hipError_t launch_sobel_tensor(const float *y, int w, int h, float4 *yt, uint8_t *seed, float threshold,
+                               hipStream_t s) {
     const dim3 g((unsigned)((w + TW - 1) / TW), (unsigned)((h + TH - 1) / TH));
-    hipLaunchKernelGGL(k_sobel_tensor, g, dim3(TW, TH), 0, s, y, w, h, jxx, jxy, jyy, seed, threshold);
+    hipLaunchKernelGGL(k_sobel_tensor, g, dim3(TW, TH), 0, s, y, w, h, yt, seed, threshold);
     return hipGetLastError();
 }
 
 hipError_t launch_lc_map(const float *y, int w, int h, float *lc, int cw, int ch, const int *xb,
-                         const int *xe, const int *yb, const int *ye, hipStream_t s) {
+                         const int *xe, const int *yb, const int *ye, hipStream_t s, const float *u, const float *v,
+                         float4 *luv) {
     (void)h;
-    hipLaunchKernelGGL(k_lc_map, rows_grid(cw, ch), dim3(kBlock), 0, s, y, w, lc, cw, ch, xb, xe, yb, ye);
+    hipLaunchKernelGGL(k_lc_map, rows_grid(cw, ch), dim3(kBlock), 0, s, y, w, lc, cw, ch, xb, xe, yb, ye, u, v,
+                       luv);
     return hipGetLastError();
 }
 
@@ -2193,11 +2225,11 @@ hipError_t launch_plain_bilinear(const float *u, const float *v, int cw, int ch,
 }
 
 hipError_t launch_guided(const float *u, const float *v, int cw, int ch, const float *y, int sw, int sh,
-                         const float *jxx, const float *jxy, const float *jyy, const float *lc, const float *ms,
+                         const float4 *yt, const float4 *luv, const float *lc, const float *ms,
                          const AxisDev &ax, const AxisDev &ay, const RadialDev &rd, const GuideParams &gp,
                          const uint32_t *row_offsets, const uint32_t *indices, float *out_u, float *out_v,
                          const SelectorDev &sel, const TemporalDev &tmp, hipStream_t s) {
-    GuidedArgs A{u, v, cw, ch, y, sw, sh, jxx, jxy, jyy, lc, ms, ax, ay, rd, gp, row_offsets, indices, out_u, out_v,
+    GuidedArgs A{u, v, cw, ch, y, sw, sh, yt, luv, lc, ms, ax, ay, rd, gp, row_offsets, indices, out_u, out_v,
                  sel, tmp};
     const size_t n = (size_t)ax.n * ay.n;
     const unsigned blocks = (unsigned)std::min<size_t>(8192, (n + kBlock - 1) / kBlock);
@@ -2308,15 +2340,15 @@ hipError_t launch_nyquist(float *a, float *b, int w, int h, int xlevels, int yle
     return hipGetLastError();
 }
 
-hipError_t launch_detail_transfer(const uint32_t *row_offsets, const uint32_t *indices, const float *jxx,
-                                  const float *jxy, const float *jyy, int sw, int sh, const AxisDev &ax,
+hipError_t launch_detail_transfer(const uint32_t *row_offsets, const uint32_t *indices, const float4 *yt,
+                                  int sw, int sh, const AxisDev &ax,
                                   const AxisDev &ay, int cw, int ch, const AffineDev &m, const float *ms,
                                   const float *detail, float ms_strength, float qgate, float strength, float stretch,
                                   float ar, int clamp_hull, float *out_u, float *out_v, hipStream_t s) {
     const size_t n = (size_t)ax.n * ay.n;
     const WorkList L{row_offsets, indices, ax.n, ay.n};
     hipLaunchKernelGGL(k_detail_transfer, dim3((unsigned)std::min<size_t>(8192, (n + kBlock - 1) / kBlock)),
-                       dim3(kBlock), 0, s, L, jxx, jxy, jyy, sw, sh, ax, ay, cw, ch, m, ms, detail, ms_strength,
+                       dim3(kBlock), 0, s, L, yt, sw, sh, ax, ay, cw, ch, m, ms, detail, ms_strength,
                        qgate, strength, stretch, ar, clamp_hull, out_u, out_v);
     return hipGetLastError();
 }

@@ -5,6 +5,7 @@
 #include "lgcr_internal.h"
 
 #include <algorithm>
+#include <functional>
 #include <atomic>
 #include <cmath>
 #include <condition_variable>
@@ -362,7 +363,8 @@ struct Workspace {
     void *raw_in[3] = {nullptr, nullptr, nullptr};
     void *raw_out[3] = {nullptr, nullptr, nullptr};
     float *y = nullptr, *u = nullptr, *v = nullptr;
-    float *jxx = nullptr, *jxy = nullptr, *jyy = nullptr;
+    float4 *yt = nullptr;   // {Y, jxx, jxy, jyy}
+    float4 *luv = nullptr;  // {lc, U, V, 0}
     float *lc = nullptr, *ms = nullptr, *grad = nullptr, *hbuf = nullptr;
     float *out_u = nullptr, *out_v = nullptr, *plain_u = nullptr, *plain_v = nullptr;
     float *luma_tmp = nullptr, *luma_out = nullptr;
@@ -640,9 +642,8 @@ void layout(const lgcr_clip *c, Workspace *w, Carver &cv) {
     w->y = cv.take<float>(ns);
     w->u = cv.take<float>(nc);
     w->v = cv.take<float>(nc);
-    w->jxx = cv.take<float>(ns);
-    w->jxy = cv.take<float>(ns);
-    w->jyy = cv.take<float>(ns);
+    w->yt = cv.take<float4>(ns);
+    w->luv = cv.take<float4>(nc);
     w->lc = cv.take<float>(nc);
     w->ms = cv.take<float>(nc);
     w->grad = cv.take<float>(6 * nc);
@@ -889,9 +890,10 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
     if (guided) {
         // BuildGuideMaps (plugin.cpp:324-334, maps.cpp:252-276)
         const double thr = p.algo == 4 ? 0.25 * p.sigma : p.sigma;
-        HIP_TRY(launch_sobel_tensor(w->y, c->sw, c->sh, w->jxx, w->jxy, w->jyy, p.sparse ? w->seed : nullptr,
+        HIP_TRY(launch_sobel_tensor(w->y, c->sw, c->sh, w->yt, p.sparse ? w->seed : nullptr,
                                     float(thr * thr), s));
-        HIP_TRY(launch_lc_map(w->y, c->sw, c->sh, w->lc, c->cw, c->ch, g.xb, g.xe, g.yb, g.ye, s));
+        HIP_TRY(launch_lc_map(w->y, c->sw, c->sh, w->lc, c->cw, c->ch, g.xb, g.xe, g.yb, g.ye, s, w->u, w->v,
+                              w->luv));
         HIP_TRY(mark(LGCR_STAGE_BUILD_GUIDE_MAPS));
         if (p.sparse) {
             // BuildTrustMask + sparse workset (plugin.cpp:335-355)
@@ -980,7 +982,7 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
                 sel.w3p = w->selp + 3 * no;
             }
         }
-        HIP_TRY(launch_guided(w->u, w->v, c->cw, c->ch, w->y, c->sw, c->sh, w->jxx, w->jxy, w->jyy, w->lc,
+        HIP_TRY(launch_guided(w->u, w->v, c->cw, c->ch, w->y, c->sw, c->sh, w->yt, w->luv, w->lc,
                               gate, g.ax, g.ay, g.rd, gp, w->row_off, idx, w->out_u, w->out_v, sel,
                               TemporalDev{}, s));
         HIP_TRY(mark(LGCR_STAGE_APPLY_GUIDED_CORRECTION));
@@ -1042,7 +1044,7 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
         }
         HIP_TRY(mark(LGCR_STAGE_BUILD_DETAIL));
         // ApplyDetailTransfer (plugin.cpp:607-625)
-        HIP_TRY(launch_detail_transfer(w->row_off, w->has_ws ? w->indices : nullptr, w->jxx, w->jxy, w->jyy, c->sw,
+        HIP_TRY(launch_detail_transfer(w->row_off, w->has_ws ? w->indices : nullptr, w->yt, c->sw,
                                        c->sh, g.ax, g.ay, c->cw, c->ch, af, p.ms > 0.0 ? w->ms : nullptr, det,
                                        float(p.ms), float(p.qgate), float(p.strength), float(p.stretch), margin,
                                        clamp_hull, w->out_u, w->out_v, s));
@@ -1128,10 +1130,12 @@ lgcr_status enqueue_trecon(lgcr_clip *c, Workspace *w, const Geometry &g, const 
     }
     if (nn > 0)
         HIP_TRY(hipMemcpyAsync(w->nbr_dev, w->nbr_host.data(), sizeof(NbrDev) * nn, hipMemcpyHostToDevice, s));
+    // the guided pass gathers U/V (and lc) from the interleaved plane even
+    // at strength 0
+    HIP_TRY(launch_lc_map(w->y, c->sw, c->sh, w->lc, c->cw, c->ch, g.xb, g.xe, g.yb, g.ye, s, w->u, w->v, w->luv));
     if (guided) {
-        HIP_TRY(launch_sobel_tensor(w->y, c->sw, c->sh, w->jxx, w->jxy, w->jyy, p.sparse ? w->seed : nullptr,
+        HIP_TRY(launch_sobel_tensor(w->y, c->sw, c->sh, w->yt, p.sparse ? w->seed : nullptr,
                                     float(p.sigma * p.sigma), s));
-        HIP_TRY(launch_lc_map(w->y, c->sw, c->sh, w->lc, c->cw, c->ch, g.xb, g.xe, g.yb, g.ye, s));
         if (p.ms > 0.0)
             HIP_TRY(launch_mutual_gate(w->lc, w->u, w->v, c->cw, c->ch, w->grad, nullptr, w->counters, w->ms, s));
         HIP_TRY(mark(LGCR_STAGE_BUILD_GUIDE_MAPS));
@@ -1182,7 +1186,7 @@ lgcr_status enqueue_trecon(lgcr_clip *c, Workspace *w, const Geometry &g, const 
     td.kp2 = p.kp2;
     td.inv_wx = float(std::min(1.0, double(c->ow) / c->cw));
     td.inv_wy = float(std::min(1.0, double(c->oh) / c->ch));
-    HIP_TRY(launch_guided(w->u, w->v, c->cw, c->ch, w->y, c->sw, c->sh, w->jxx, w->jxy, w->jyy, w->lc,
+    HIP_TRY(launch_guided(w->u, w->v, c->cw, c->ch, w->y, c->sw, c->sh, w->yt, w->luv, w->lc,
                           gp.ms_on ? w->ms : nullptr, g.ax, g.ay, g.rd, gp, w->row_off,
                           w->has_ws ? w->indices : nullptr, w->out_u, w->out_v, SelectorDev{}, td, s));
     HIP_TRY(mark(LGCR_STAGE_APPLY_GUIDED_CORRECTION));
@@ -1398,6 +1402,9 @@ static lgcr_status create_clip(lgcr_device *dev, const lgcr_params *params, cons
         c->spaces.push_back(std::move(more));
         c->idle.push_back((int)c->spaces.size() - 1);
     }
+    // idle is used as a stack: workspace 0 (the one lgcr_clip_fetch reads)
+    // serves a lone caller
+    std::sort(c->idle.begin(), c->idle.end(), std::greater<int>());
     *out = c.release();
     return LGCR_OK;
 }
@@ -2095,9 +2102,19 @@ lgcr_status lgcr_clip_fetch(lgcr_clip *c, int what, void *dst, size_t bytes, siz
     case LGCR_FETCH_Y: src = w->y; size = ns * 4; break;
     case LGCR_FETCH_U: src = w->u; size = nc * 4; break;
     case LGCR_FETCH_V: src = w->v; size = nc * 4; break;
-    case LGCR_FETCH_JXX: src = w->jxx; size = ns * 4; break;
-    case LGCR_FETCH_JXY: src = w->jxy; size = ns * 4; break;
-    case LGCR_FETCH_JYY: src = w->jyy; size = ns * 4; break;
+    case LGCR_FETCH_JXX:
+    case LGCR_FETCH_JXY:
+    case LGCR_FETCH_JYY: {
+        // one component of the interleaved {Y, jxx, jxy, jyy} plane
+        const int comp = 1 + (what - LGCR_FETCH_JXX);
+        size = ns * 4;
+        if (needed)
+            *needed = size;
+        if (dst && bytes >= size)
+            HIP_TRY(hipMemcpy2D(dst, 4, reinterpret_cast<const float *>(w->yt) + comp, 16, 4, ns,
+                                hipMemcpyDeviceToHost));
+        return LGCR_OK;
+    }
     case LGCR_FETCH_LC: src = w->lc; size = nc * 4; break;
     case LGCR_FETCH_MS: src = w->ms; size = nc * 4; break;
     case LGCR_FETCH_SEED: src = w->seed; size = ns; break;
