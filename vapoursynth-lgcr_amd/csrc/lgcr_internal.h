@@ -107,19 +107,21 @@ struct RawOut {
 hipError_t launch_to_float(const RawPlane *planes, int count, int type, hipStream_t s);
 hipError_t launch_from_float(const RawOut *planes, int count, int type, hipStream_t s);
 // structure tensor written interleaved with the luma: yt = {Y, jxx, jxy, jyy}
-hipError_t launch_sobel_tensor(const float *y, int w, int h, float4 *yt, uint8_t *seed, float threshold,
+// seed: one bit per pixel, rows of (w + 63) / 64 words
+hipError_t launch_sobel_tensor(const float *y, int w, int h, float4 *yt, uint64_t *seed, float threshold,
                                hipStream_t s);
+// trust mask (dilation of the seed bits by radius) and the sparse workset:
+// per-output-row counts, row offsets, row-major indices, chroma mask and
+// its population in counters[0]
+hipError_t launch_trust_workset(const uint64_t *seed, uint64_t *tmp, uint64_t *mask, int w, int h, int radius,
+                                uint32_t *row_counts, uint32_t *row_offsets, uint32_t *indices, uint8_t *chroma_mask,
+                                int cw, int ch, const AxisDev &ax, const AxisDev &ay, uint32_t *counters,
+                                hipStream_t s);
 // footprint luma; with luv also the interleaved chroma guide {lc, U, V, 0}
 hipError_t launch_lc_map(const float *y, int w, int h, float *lc, int cw, int ch,
                          const int *xb, const int *xe, const int *yb, const int *ye,
                          hipStream_t s, const float *u = nullptr, const float *v = nullptr,
                          float4 *luv = nullptr);
-hipError_t launch_dilate(const uint8_t *src, uint8_t *tmp, uint8_t *dst, int w, int h,
-                         int radius, hipStream_t s);
-hipError_t launch_workset(const uint8_t *mask, int w, int h, uint32_t *row_offsets,
-                          uint32_t *indices, uint8_t *chroma_mask, int cw, int ch,
-                          const AxisDev &ax, const AxisDev &ay, uint32_t *counters,
-                          hipStream_t s);
 hipError_t launch_mutual_gate(const float *lc, const float *u, const float *v, int w, int h,
                               float *grad, const uint8_t *chroma_mask,
                               const uint32_t *counters, float *gate, hipStream_t s);

@@ -283,7 +283,7 @@ __global__ __launch_bounds__(kBlock) void k_bp_pack(float *ou, float *ov, int ow
 constexpr int TW = 64, TH = 4;
 
 __global__ __launch_bounds__(TW * TH) void k_sobel_tensor(const float *__restrict__ y, int w, int h,
-                                                         float4 *yt, uint8_t *seed, float thr) {
+                                                         float4 *yt, uint64_t *seed, int w64, float thr) {
     __shared__ float sy[TH + 4][TW + 4];
     __shared__ float gx[TH + 2][TW + 2], gy[TH + 2][TW + 2];
     __shared__ float vxx[TH][TW + 2], vxy[TH][TW + 2], vyy[TH][TW + 2];
@@ -317,12 +317,12 @@ __global__ __launch_bounds__(TW * TH) void k_sobel_tensor(const float *__restric
     }
     __syncthreads();
     const int x = x0 + threadIdx.x, r = y0 + threadIdx.y;
-    if (x >= w || r >= h)
-        return;
+    const bool inside = x < w && r < h;
     const int ly = threadIdx.y;
     const int c = x - (x0 - 1);  // column of x in the V arrays
-    float oxx, oxy, oyy;
-    if (x == 0) {
+    float oxx = 0.0f, oxy = 0.0f, oyy = 0.0f;
+    if (!inside) {
+    } else if (x == 0) {
         const int c1 = (w > 1 ? 1 : 0) - (x0 - 1);
         oxx = (2.0f * vxx[ly][c] + vxx[ly][c1]) / 9.0f;
         oxy = (2.0f * vxy[ly][c] + vxy[ly][c1]) / 9.0f;
@@ -337,10 +337,15 @@ __global__ __launch_bounds__(TW * TH) void k_sobel_tensor(const float *__restric
         oxy = ninth * ((vxy[ly][c - 1] + vxy[ly][c]) + vxy[ly][c + 1]);
         oyy = ninth * ((vyy[ly][c - 1] + vyy[ly][c]) + vyy[ly][c + 1]);
     }
+    // trust seed (fused in buildGuideMaps, maps.cpp:252-276) as one bit per
+    // pixel: a wave is one 64-pixel tile row, its ballot one mask word
+    const uint64_t bits = __ballot(inside && oxx + oyy > thr);
+    if (!inside)
+        return;
     const size_t o = (size_t)r * w + x;
     yt[o] = make_float4(sy[ly + 2][threadIdx.x + 2], oxx, oxy, oyy);
-    if (seed)
-        seed[o] = oxx + oyy > thr;
+    if (seed && threadIdx.x == 0)
+        seed[(size_t)r * w64 + blockIdx.x] = bits;
 }
 
 // ---------------------------------------------------------------------------
@@ -367,72 +372,12 @@ __global__ __launch_bounds__(kBlock) void k_lc_map(const float *__restrict__ y, 
 }
 
 // ---------------------------------------------------------------------------
-// trust-mask dilation (buildTrustMask, maps.cpp:295-339): a pixel is set when
-// any seed lies within Chebyshev distance r.  Horizontal pass: one row per
-// block staged in LDS, running counts over 8-pixel runs; vertical pass:
-// running counts down 16-row column segments.
+// trust mask and sparse workset (buildTrustMask, maps.cpp:280-340;
+// buildSparseWorkset, maps.cpp:342-422) on bit planes: row-major list of
+// active output pixels via per-row counts, a per-row prefix and an ordered
+// per-row compaction (wave ballots), then the chroma mask of the bilinear
+// footprints and its population.
 // ---------------------------------------------------------------------------
-
-constexpr int kDilRun = 8;
-
-__global__ __launch_bounds__(kBlock) void k_dilate_h(const uint8_t *src, uint8_t *dst, int w, int r) {
-    extern __shared__ uint8_t row[];  // w + 2r, zero padded
-    const int y = blockIdx.x;
-    const uint8_t *s = src + (size_t)y * w;
-    for (int i = threadIdx.x; i < w + 2 * r; i += kBlock) {
-        const int x = i - r;
-        row[i] = (x >= 0 && x < w) ? (s[x] != 0) : 0;
-    }
-    __syncthreads();
-    for (int x0 = threadIdx.x * kDilRun; x0 < w; x0 += kBlock * kDilRun) {
-        int cnt = 0;
-        for (int i = 0; i <= 2 * r; ++i)  // window [x0 - r, x0 + r]
-            cnt += row[x0 + i];
-        const int x1 = min(w, x0 + kDilRun);
-        for (int x = x0; x < x1; ++x) {
-            dst[(size_t)y * w + x] = cnt != 0;
-            if (x + 1 < x1)  // slide to [x + 1 - r, x + 1 + r]; LDS is zero padded
-                cnt += row[x + 2 * r + 1] - row[x];
-        }
-    }
-}
-
-constexpr int kDilRows = 16;
-
-__global__ __launch_bounds__(kBlock) void k_dilate_v(const uint8_t *src, uint8_t *dst, int w, int h, int r) {
-    const int x = blockIdx.x * kBlock + threadIdx.x;
-    const int y0 = blockIdx.y * kDilRows;
-    if (x >= w)
-        return;
-    int cnt = 0;
-    for (int j = y0 - r; j <= y0 + r; ++j)
-        if (j >= 0 && j < h)
-            cnt += src[(size_t)j * w + x] != 0;
-    const int y1 = min(h, y0 + kDilRows);
-    for (int y = y0; y < y1; ++y) {
-        dst[(size_t)y * w + x] = cnt != 0;
-        const int rem = y - r, add = y + r + 1;
-        if (rem >= 0)
-            cnt -= src[(size_t)rem * w + x] != 0;
-        if (add < h)
-            cnt += src[(size_t)add * w + x] != 0;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// sparse workset (buildSparseWorkset, maps.cpp:342-422): row-major list of
-// active output pixels via per-row counts, an exclusive scan over rows and an
-// ordered per-row compaction (wave ballots), then the chroma mask of the
-// bilinear footprints and its population.
-// ---------------------------------------------------------------------------
-
-struct MaskMap { const uint8_t *mask; int mw, mh, ow, oh; };
-
-__device__ __forceinline__ bool out_active(const MaskMap &m, int x, int y) {
-    const int mx = min(m.mw - 1, (int)((int64_t)x * m.mw / m.ow));
-    const int my = min(m.mh - 1, (int)((int64_t)y * m.mh / m.oh));
-    return m.mask[(size_t)my * m.mw + mx] != 0;
-}
 
 __device__ __forceinline__ int block_sum(int v, int *scratch) {
     // wave64 reduction then across the 4 waves of the block
@@ -450,51 +395,114 @@ __device__ __forceinline__ int block_sum(int v, int *scratch) {
     return t;  // valid in thread 0
 }
 
-__global__ __launch_bounds__(kBlock) void k_row_count(MaskMap m, uint32_t *row_count) {
+// ---------------------------------------------------------------------------
+// trust mask on bit planes: one bit per luma pixel, 64 per word, rows of
+// w64 words.  Horizontal pass: each output word ORs the row shifted by every
+// offset in [-r, r]; vertical pass: OR over rows y-r..y+r, with the row
+// population when the mask has the output size.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint64_t row_bits(const uint64_t *row, int w64, long pos) {
+    // 64 bits starting at bit `pos` of the row; zeros outside it
+    const long q = pos >> 6;
+    const int sft = (int)(pos & 63);
+    const uint64_t lo = (q >= 0 && q < w64) ? row[q] : 0ull;
+    if (sft == 0)
+        return lo;
+    const uint64_t hi = (q + 1 >= 0 && q + 1 < w64) ? row[q + 1] : 0ull;
+    return (lo >> sft) | (hi << (64 - sft));
+}
+
+__global__ __launch_bounds__(kBlock) void k_dilate_hbits(const uint64_t *in, uint64_t *out, int w, int w64, int h,
+                                                         int r, uint8_t *zero8, size_t n_zero8, uint32_t *counters) {
+    const int wx = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
+    // clear the chroma mask and the counters the workset fills next
+    const size_t tid = ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * kBlock + threadIdx.x;
+    const size_t nthreads = (size_t)gridDim.x * gridDim.y * kBlock;
+    for (size_t i = tid; i < n_zero8; i += nthreads)
+        zero8[i] = 0;
+    if (tid < 8)
+        counters[tid] = 0;
+    if (wx >= w64)
+        return;
+    const uint64_t *row = in + (size_t)y * w64;
+    uint64_t acc = 0;
+    for (int d = -r; d <= r; ++d)
+        acc |= row_bits(row, w64, (long)wx * 64 + d);
+    const int valid = w - wx * 64;   // pixels of this word inside the row
+    if (valid < 64)
+        acc &= (1ull << valid) - 1ull;
+    out[(size_t)y * w64 + wx] = acc;
+}
+
+__global__ __launch_bounds__(64) void k_dilate_vbits(const uint64_t *in, uint64_t *out, int w64, int h, int r,
+                                                     uint32_t *row_count) {
+    const int y = blockIdx.x;
+    const int j0 = max(0, y - r), j1 = min(h - 1, y + r);
+    int cnt = 0;
+    for (int wx = threadIdx.x; wx < w64; wx += 64) {
+        uint64_t acc = 0;
+        for (int j = j0; j <= j1; ++j)
+            acc |= in[(size_t)j * w64 + wx];
+        out[(size_t)y * w64 + wx] = acc;
+        cnt += __popcll(acc);
+    }
+    if (row_count) {
+        for (int o = 32; o > 0; o >>= 1)
+            cnt += __shfl_xor(cnt, o, 64);
+        if (threadIdx.x == 0)
+            row_count[y] = (uint32_t)cnt;
+    }
+}
+
+// Output-pixel view of a bit mask at mask resolution (nearest mapping of
+// buildSparseWorkset, maps.cpp:342-360).
+struct MaskBits { const uint64_t *bits; int mw, mh, w64, ow, oh; };
+
+__device__ __forceinline__ bool out_active_bits(const MaskBits &m, int x, int y) {
+    const int mx = min(m.mw - 1, (int)((int64_t)x * m.mw / m.ow));
+    const int my = min(m.mh - 1, (int)((int64_t)y * m.mh / m.oh));
+    return (m.bits[(size_t)my * m.w64 + (mx >> 6)] >> (mx & 63)) & 1ull;
+}
+
+__global__ __launch_bounds__(kBlock) void k_row_count_bits(MaskBits m, uint32_t *row_count) {
     __shared__ int scratch[kBlock / 64];
     const int y = blockIdx.x;
     int c = 0;
     for (int x = threadIdx.x; x < m.ow; x += kBlock)
-        c += out_active(m, x, y);
+        c += out_active_bits(m, x, y);
     const int t = block_sum(c, scratch);
     if (threadIdx.x == 0)
         row_count[y] = (uint32_t)t;
 }
 
-// Exclusive scan of n row counts in place; off[n] receives the total.
-__global__ __launch_bounds__(1024) void k_scan_rows(uint32_t *off, int n) {
-    __shared__ uint32_t part[1024];
-    const int per = (n + 1023) / 1024;
-    const int b = threadIdx.x * per, e = min(n, b + per);
-    uint32_t s = 0;
-    for (int i = b; i < e; ++i)
-        s += off[i];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int d = 1; d < 1024; d <<= 1) {
-        const uint32_t v = threadIdx.x >= (unsigned)d ? part[threadIdx.x - d] : 0;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
-    }
-    uint32_t run = part[threadIdx.x] - s;  // exclusive prefix of this chunk
-    for (int i = b; i < e; ++i) {
-        const uint32_t c = off[i];
-        off[i] = run;
-        run += c;
-    }
-    if (threadIdx.x == 1023)
-        off[n] = part[1023];
-}
-
-__global__ __launch_bounds__(kBlock) void k_compact(MaskMap m, const uint32_t *off, uint32_t *indices) {
+// One block per output row: exclusive prefix of the row counts (summed
+// here, so no separate scan pass), ordered ballot compaction of the row's
+// active pixels, and the 2x2 bilinear chroma footprints of each.
+__global__ __launch_bounds__(kBlock) void k_compact_bits(MaskBits m, const uint32_t *row_count, uint32_t *off,
+                                                         uint32_t *indices, const int *cx_i0, const int *cy_i0,
+                                                         uint8_t *cmask, int cw, int ch) {
+    __shared__ int scratch[kBlock / 64];
     __shared__ int wave_cnt[kBlock / 64];
     const int y = blockIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t base = off[y];
+    int part = 0;
+    for (int i = threadIdx.x; i < y; i += kBlock)
+        part += (int)row_count[i];
+    const int prefix = block_sum(part, scratch);
+    __shared__ uint32_t base_s;
+    if (threadIdx.x == 0) {
+        base_s = (uint32_t)prefix;
+        off[y] = (uint32_t)prefix;
+        if (y == m.oh - 1)
+            off[m.oh] = (uint32_t)prefix + row_count[y];
+    }
+    __syncthreads();
+    uint32_t base = base_s;
+    const int cy0 = cy_i0[y], cy1 = min(cy0 + 1, ch - 1);
     for (int x0 = 0; x0 < m.ow; x0 += kBlock) {
         const int x = x0 + threadIdx.x;
-        const bool a = x < m.ow && out_active(m, x, y);
+        const bool a = x < m.ow && out_active_bits(m, x, y);
         const uint64_t bal = __ballot(a);
         const int before = __popcll(bal & ((1ull << lane) - 1ull));
         if (lane == 0)
@@ -505,29 +513,42 @@ __global__ __launch_bounds__(kBlock) void k_compact(MaskMap m, const uint32_t *o
             prior += i < wave ? wave_cnt[i] : 0;
             total += wave_cnt[i];
         }
-        if (a)
+        if (a) {
             indices[base + prior + before] = (uint32_t)((size_t)y * m.ow + x);
+            const int cx0 = cx_i0[x], cx1 = min(cx0 + 1, cw - 1);
+            cmask[(size_t)cy0 * cw + cx0] = 1;
+            cmask[(size_t)cy0 * cw + cx1] = 1;
+            cmask[(size_t)cy1 * cw + cx0] = 1;
+            cmask[(size_t)cy1 * cw + cx1] = 1;
+        }
         base += total;
         __syncthreads();
     }
 }
 
+// population of a 0/1 byte plane, 16 bytes per thread step
+__global__ __launch_bounds__(kBlock) void k_count_u8x16(const uint8_t *p, size_t n, uint32_t *counter) {
+    __shared__ int scratch[kBlock / 64];
+    int c = 0;
+    const size_t n16 = n / 16;
+    const uint4 *q = reinterpret_cast<const uint4 *>(p);
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n16; i += (size_t)gridDim.x * kBlock) {
+        const uint4 v = q[i];
+        // bytes are 0 or 1: the byte sums fit in each byte lane
+        const uint32_t s4 = v.x + v.y + v.z + v.w;
+        c += (int)((s4 & 0xff) + ((s4 >> 8) & 0xff) + ((s4 >> 16) & 0xff) + (s4 >> 24));
+    }
+    if (blockIdx.x == 0)
+        for (size_t i = n16 * 16 + threadIdx.x; i < n; i += kBlock)
+            c += p[i] != 0;
+    const int t = block_sum(c, scratch);
+    if (threadIdx.x == 0 && t)
+        atomicAdd(counter, (uint32_t)t);
+}
+
 __global__ __launch_bounds__(kBlock) void k_fill_u8(uint8_t *p, size_t n, uint8_t v) {
     for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
         p[i] = v;
-}
-
-__global__ __launch_bounds__(kBlock) void k_chroma_mark(MaskMap m, const int *cx_i0, const int *cy_i0,
-                                                         uint8_t *cmask, int cw, int ch) {
-    const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
-    if (x >= m.ow || !out_active(m, x, y))
-        return;
-    const int cy0 = cy_i0[y], cy1 = min(cy0 + 1, ch - 1);
-    const int cx0 = cx_i0[x], cx1 = min(cx0 + 1, cw - 1);
-    cmask[(size_t)cy0 * cw + cx0] = 1;
-    cmask[(size_t)cy0 * cw + cx1] = 1;
-    cmask[(size_t)cy1 * cw + cx0] = 1;
-    cmask[(size_t)cy1 * cw + cx1] = 1;
 }
 
 __global__ __launch_bounds__(kBlock) void k_count_u8(const uint8_t *p, size_t n, uint32_t *counter) {
@@ -2155,10 +2176,10 @@ hipError_t launch_from_float(const RawOut *planes, int count, int type, hipStrea
     return hipGetLastError();
 }
 
-hipError_t launch_sobel_tensor(const float *y, int w, int h, float4 *yt, uint8_t *seed, float threshold,
+hipError_t launch_sobel_tensor(const float *y, int w, int h, float4 *yt, uint64_t *seed, float threshold,
                                hipStream_t s) {
     const dim3 g((unsigned)((w + TW - 1) / TW), (unsigned)((h + TH - 1) / TH));
-    hipLaunchKernelGGL(k_sobel_tensor, g, dim3(TW, TH), 0, s, y, w, h, yt, seed, threshold);
+    hipLaunchKernelGGL(k_sobel_tensor, g, dim3(TW, TH), 0, s, y, w, h, yt, seed, (w + 63) / 64, threshold);
     return hipGetLastError();
 }
 
@@ -2171,32 +2192,24 @@ hipError_t launch_lc_map(const float *y, int w, int h, float *lc, int cw, int ch
     return hipGetLastError();
 }
 
-hipError_t launch_dilate(const uint8_t *src, uint8_t *tmp, uint8_t *dst, int w, int h, int radius,
-                         hipStream_t s) {
-    const size_t lds = (size_t)w + 2 * (size_t)radius;
-    if (lds > 60 * 1024)
-        return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_dilate_h, dim3((unsigned)h), dim3(kBlock), lds, s, src, tmp, w, radius);
-    hipLaunchKernelGGL(k_dilate_v, dim3((unsigned)((w + kBlock - 1) / kBlock), (unsigned)((h + kDilRows - 1) / kDilRows)),
-                       dim3(kBlock), 0, s, tmp, dst, w, h, radius);
-    return hipGetLastError();
-}
-
-hipError_t launch_workset(const uint8_t *mask, int w, int h, uint32_t *row_offsets, uint32_t *indices,
-                          uint8_t *chroma_mask, int cw, int ch, const AxisDev &ax, const AxisDev &ay,
-                          uint32_t *counters, hipStream_t s) {
-    const MaskMap m{mask, w, h, ax.n, ay.n};
-    hipLaunchKernelGGL(k_row_count, dim3((unsigned)ay.n), dim3(kBlock), 0, s, m, row_offsets);
-    hipLaunchKernelGGL(k_scan_rows, dim3(1), dim3(1024), 0, s, row_offsets, ay.n);
-    hipLaunchKernelGGL(k_compact, dim3((unsigned)ay.n), dim3(kBlock), 0, s, m, row_offsets, indices);
+hipError_t launch_trust_workset(const uint64_t *seed, uint64_t *tmp, uint64_t *mask, int w, int h, int radius,
+                                uint32_t *row_counts, uint32_t *row_offsets, uint32_t *indices, uint8_t *chroma_mask,
+                                int cw, int ch, const AxisDev &ax, const AxisDev &ay, uint32_t *counters,
+                                hipStream_t s) {
+    const int w64 = (w + 63) / 64;
     const size_t nc = (size_t)cw * ch;
-    hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)std::min<size_t>(1024, (nc + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
-                       chroma_mask, nc, (uint8_t)0);
-    hipLaunchKernelGGL(k_chroma_mark, rows_grid(ax.n, ay.n), dim3(kBlock), 0, s, m, ax.cb_i0, ay.cb_i0,
-                       chroma_mask, cw, ch);
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, counters, 4);
-    hipLaunchKernelGGL(k_count_u8, dim3((unsigned)std::min<size_t>(512, (nc + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
-                       chroma_mask, nc, counters);
+    hipLaunchKernelGGL(k_dilate_hbits, dim3((unsigned)((w64 + kBlock - 1) / kBlock), (unsigned)h), dim3(kBlock), 0, s,
+                       seed, tmp, w, w64, h, radius, chroma_mask, nc, counters);
+    const bool same = ax.n == w && ay.n == h;
+    hipLaunchKernelGGL(k_dilate_vbits, dim3((unsigned)h), dim3(64), 0, s, tmp, mask, w64, h, radius,
+                       same ? row_counts : nullptr);
+    const MaskBits m{mask, w, h, w64, ax.n, ay.n};
+    if (!same)
+        hipLaunchKernelGGL(k_row_count_bits, dim3((unsigned)ay.n), dim3(kBlock), 0, s, m, row_counts);
+    hipLaunchKernelGGL(k_compact_bits, dim3((unsigned)ay.n), dim3(kBlock), 0, s, m, row_counts, row_offsets, indices,
+                       ax.cb_i0, ay.cb_i0, chroma_mask, cw, ch);
+    hipLaunchKernelGGL(k_count_u8x16, dim3((unsigned)std::min<size_t>(256, (nc / 16 + kBlock - 1) / kBlock + 1)),
+                       dim3(kBlock), 0, s, chroma_mask, nc, counters);
     return hipGetLastError();
 }
 

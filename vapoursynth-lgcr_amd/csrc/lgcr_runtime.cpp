@@ -368,8 +368,9 @@ struct Workspace {
     float *lc = nullptr, *ms = nullptr, *grad = nullptr, *hbuf = nullptr;
     float *out_u = nullptr, *out_v = nullptr, *plain_u = nullptr, *plain_v = nullptr;
     float *luma_tmp = nullptr, *luma_out = nullptr;
-    uint8_t *seed = nullptr, *mask = nullptr, *mtmp = nullptr, *cmask = nullptr;
-    uint32_t *row_off = nullptr, *indices = nullptr, *counters = nullptr;
+    uint64_t *seed = nullptr, *mask = nullptr, *mtmp = nullptr;  // bit planes, rows of w64 words
+    uint8_t *cmask = nullptr;
+    uint32_t *row_off = nullptr, *row_cnt = nullptr, *indices = nullptr, *counters = nullptr;
     // algo 4
     float *ls = nullptr, *mm = nullptr, *lgf = nullptr, *sel = nullptr, *selp = nullptr;
     uint8_t *roi = nullptr;
@@ -656,11 +657,13 @@ void layout(const lgcr_clip *c, Workspace *w, Carver &cv) {
         w->luma_tmp = cv.take<float>(size_t(c->ow) * c->sh);
         w->luma_out = cv.take<float>(no);
     }
-    w->seed = cv.take<uint8_t>(ns);
-    w->mask = cv.take<uint8_t>(ns);
-    w->mtmp = cv.take<uint8_t>(ns);
+    const size_t nbits = size_t((c->sw + 63) / 64) * c->sh;
+    w->seed = cv.take<uint64_t>(nbits);
+    w->mask = cv.take<uint64_t>(nbits);
+    w->mtmp = cv.take<uint64_t>(nbits);
     w->cmask = cv.take<uint8_t>(nc);
     w->row_off = cv.take<uint32_t>(size_t(c->oh) + 1);
+    w->row_cnt = cv.take<uint32_t>(size_t(c->oh) + 1);
     w->indices = cv.take<uint32_t>(no);
     w->counters = cv.take<uint32_t>(8);
     if (c->p.algo == 4 && c->p.strength > 0.0) {
@@ -897,9 +900,8 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
         HIP_TRY(mark(LGCR_STAGE_BUILD_GUIDE_MAPS));
         if (p.sparse) {
             // BuildTrustMask + sparse workset (plugin.cpp:335-355)
-            HIP_TRY(launch_dilate(w->seed, w->mtmp, w->mask, c->sw, c->sh, c->dil, s));
-            HIP_TRY(launch_workset(w->mask, c->sw, c->sh, w->row_off, w->indices, w->cmask, c->cw, c->ch, g.ax,
-                                   g.ay, w->counters, s));
+            HIP_TRY(launch_trust_workset(w->seed, w->mtmp, w->mask, c->sw, c->sh, c->dil, w->row_cnt, w->row_off,
+                                         w->indices, w->cmask, c->cw, c->ch, g.ax, g.ay, w->counters, s));
             w->has_ws = true;
             HIP_TRY(mark(LGCR_STAGE_BUILD_TRUST_MASK));
         }
@@ -1148,9 +1150,8 @@ lgcr_status enqueue_trecon(lgcr_clip *c, Workspace *w, const Geometry &g, const 
     const float margin = float(std::max(0.0, p.ar_margin));
     w->has_ws = false;
     if (p.sparse && guided) {
-        HIP_TRY(launch_dilate(w->seed, w->mtmp, w->mask, c->sw, c->sh, c->dil, s));
-        HIP_TRY(launch_workset(w->mask, c->sw, c->sh, w->row_off, w->indices, w->cmask, c->cw, c->ch, g.ax, g.ay,
-                               w->counters, s));
+        HIP_TRY(launch_trust_workset(w->seed, w->mtmp, w->mask, c->sw, c->sh, c->dil, w->row_cnt, w->row_off,
+                                     w->indices, w->cmask, c->cw, c->ch, g.ax, g.ay, w->counters, s));
         w->has_ws = true;
         HIP_TRY(launch_plain_separable(w->u, w->v, c->cw, c->ch, g.ax, g.ay, w->hbuf, w->out_u, w->out_v, clamp_hull,
                                        margin, s));
@@ -2117,8 +2118,24 @@ lgcr_status lgcr_clip_fetch(lgcr_clip *c, int what, void *dst, size_t bytes, siz
     }
     case LGCR_FETCH_LC: src = w->lc; size = nc * 4; break;
     case LGCR_FETCH_MS: src = w->ms; size = nc * 4; break;
-    case LGCR_FETCH_SEED: src = w->seed; size = ns; break;
-    case LGCR_FETCH_MASK: src = w->mask; size = ns; break;
+    case LGCR_FETCH_SEED:
+    case LGCR_FETCH_MASK: {
+        // bit planes expanded to one byte per pixel
+        const int w64 = (c->sw + 63) / 64;
+        size = ns;
+        if (needed)
+            *needed = size;
+        if (dst && bytes >= size) {
+            std::vector<uint64_t> words(size_t(w64) * c->sh);
+            HIP_TRY(hipMemcpy(words.data(), what == LGCR_FETCH_SEED ? w->seed : w->mask,
+                              words.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
+            uint8_t *o = static_cast<uint8_t *>(dst);
+            for (int yy = 0; yy < c->sh; ++yy)
+                for (int xx = 0; xx < c->sw; ++xx)
+                    o[size_t(yy) * c->sw + xx] = (words[size_t(yy) * w64 + (xx >> 6)] >> (xx & 63)) & 1u;
+        }
+        return LGCR_OK;
+    }
     case LGCR_FETCH_CHROMA_MASK: src = w->cmask; size = nc; break;
     case LGCR_FETCH_INDICES: {
         uint32_t count = 0;
