@@ -571,32 +571,54 @@ __global__ void k_zero_u32(uint32_t *p, int n) {
 // mutual-structure gate (buildMutualGate, maps.cpp:831-948)
 // ---------------------------------------------------------------------------
 
+// Sobel gradients of lc, U and V interleaved per sample: g4 = {lc_x, lc_y,
+// U_x, U_y}, g2 = {V_x, V_y}, so a gate tap is two loads.
 __global__ __launch_bounds__(kBlock) void k_grad3(const float *lc, const float *u, const float *v,
-                                                  int w, int h, float *g) {
+                                                  int w, int h, float4 *g4, float2 *g2) {
     const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
     if (x >= w)
         return;
-    const size_t n = (size_t)w * h, i = (size_t)y * w + x;
-    float a, b;
-    sobel(lc, w, h, x, y, a, b); g[i] = a; g[n + i] = b;
-    sobel(u, w, h, x, y, a, b); g[2 * n + i] = a; g[3 * n + i] = b;
-    sobel(v, w, h, x, y, a, b); g[4 * n + i] = a; g[5 * n + i] = b;
+    const size_t i = (size_t)y * w + x;
+    float a, b, c, d, e, f;
+    sobel(lc, w, h, x, y, a, b);
+    sobel(u, w, h, x, y, c, d);
+    sobel(v, w, h, x, y, e, f);
+    g4[i] = make_float4(a, b, c, d);
+    g2[i] = make_float2(e, f);
 }
 
 // SobelRowRing::sampleProjected (maps.cpp:782-791) at bilinearPoint
 // (maps.cpp:175-186).
-__device__ __forceinline__ float projected(const float *gx, const float *gy, int w, int h,
-                                           int x0, int x1, int y0, int y1, float fx, float fy,
-                                           float nx, float ny) {
-    const float a = gx[(size_t)y0 * w + x0] * nx + gy[(size_t)y0 * w + x0] * ny;
-    const float b = gx[(size_t)y0 * w + x1] * nx + gy[(size_t)y0 * w + x1] * ny;
-    const float c = gx[(size_t)y1 * w + x0] * nx + gy[(size_t)y1 * w + x0] * ny;
-    const float d = gx[(size_t)y1 * w + x1] * nx + gy[(size_t)y1 * w + x1] * ny;
-    return a + (b - a) * fx + (c - a) * fy + (a - b - c + d) * fx * fy;
+// Bilinear sample of the three gradients projected on n (the Y, U and V
+// results of sampleProjected at one point).
+struct Proj3 { float y, u, v; };
+__device__ __forceinline__ Proj3 projected3(const float4 *g4, const float2 *g2, int w, int x0, int x1, int y0,
+                                            int y1, float fx, float fy, float nx, float ny) {
+    const size_t i00 = (size_t)y0 * w + x0, i01 = (size_t)y0 * w + x1, i10 = (size_t)y1 * w + x0,
+                 i11 = (size_t)y1 * w + x1;
+    const float4 A = g4[i00], B = g4[i01], C = g4[i10], D = g4[i11];
+    const float2 A2 = g2[i00], B2 = g2[i01], C2 = g2[i10], D2 = g2[i11];
+    Proj3 r;
+    {
+        const float a = A.x * nx + A.y * ny, b = B.x * nx + B.y * ny;
+        const float c = C.x * nx + C.y * ny, d = D.x * nx + D.y * ny;
+        r.y = a + (b - a) * fx + (c - a) * fy + (a - b - c + d) * fx * fy;
+    }
+    {
+        const float a = A.z * nx + A.w * ny, b = B.z * nx + B.w * ny;
+        const float c = C.z * nx + C.w * ny, d = D.z * nx + D.w * ny;
+        r.u = a + (b - a) * fx + (c - a) * fy + (a - b - c + d) * fx * fy;
+    }
+    {
+        const float a = A2.x * nx + A2.y * ny, b = B2.x * nx + B2.y * ny;
+        const float c = C2.x * nx + C2.y * ny, d = D2.x * nx + D2.y * ny;
+        r.v = a + (b - a) * fx + (c - a) * fy + (a - b - c + d) * fx * fy;
+    }
+    return r;
 }
 
-__global__ __launch_bounds__(kBlock) void k_gate(const float *__restrict__ g, int w, int h,
-                                                 const uint8_t *cmask, const uint32_t *counters,
+__global__ __launch_bounds__(kBlock) void k_gate(const float4 *__restrict__ g4, const float2 *__restrict__ g2,
+                                                 int w, int h, const uint8_t *cmask, const uint32_t *counters,
                                                  float *gate) {
     const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
     if (x >= w)
@@ -608,12 +630,12 @@ __global__ __launch_bounds__(kBlock) void k_gate(const float *__restrict__ g, in
         *out = 0.0f;
         return;
     }
-    const float *lx = g, *ly = g + n, *ux = g + 2 * n, *uy = g + 3 * n, *vx = g + 4 * n, *vy = g + 5 * n;
     float sxx = 0, sxy = 0, syy = 0;
     for (int dy = -1; dy <= 1; ++dy)
         for (int dx = -1; dx <= 1; ++dx) {
-            const float a = at(lx, w, h, x + dx, y + dy);
-            const float b = at(ly, w, h, x + dx, y + dy);
+            const float4 t = g4[(size_t)clampi(y + dy, 0, h - 1) * w + clampi(x + dx, 0, w - 1)];
+            const float a = t.x;
+            const float b = t.y;
             sxx += a * a;
             sxy += a * b;
             syy += b * b;
@@ -632,9 +654,10 @@ __global__ __launch_bounds__(kBlock) void k_gate(const float *__restrict__ g, in
         const int x0 = clampi(rx, 0, w - 1), x1 = clampi(rx + 1, 0, w - 1);
         const int y0 = clampi(ry, 0, h - 1), y1 = clampi(ry + 1, 0, h - 1);
         const float fx = (float)(px - rx), fy = (float)(py - ry);
-        const float gY = fabsf(projected(lx, ly, w, h, x0, x1, y0, y1, fx, fy, nx, ny));
-        const float au = projected(ux, uy, w, h, x0, x1, y0, y1, fx, fy, nx, ny);
-        const float av = projected(vx, vy, w, h, x0, x1, y0, y1, fx, fy, nx, ny);
+        const Proj3 pr = projected3(g4, g2, w, x0, x1, y0, y1, fx, fy, nx, ny);
+        const float gY = fabsf(pr.y);
+        const float au = pr.u;
+        const float av = pr.v;
         const float gC = sqrtf(au * au + av * av);
         sumY += gY; sqY += gY * gY; momY += (float)k * gY;
         sumC += gC; sqC += gC * gC; momC += (float)k * gC;
@@ -679,13 +702,10 @@ __global__ __launch_bounds__(kBlock) void k_plain_h(const float *__restrict__ u,
         mnu = fmn(mnu, a); mxu = fmx(mxu, a);
         mnv = fmn(mnv, b); mxv = fmx(mxv, b);
     }
+    // interleaved: {vu, vv} then {min u, max u, min v, max v}
     const size_t n = (size_t)ax.n * ch, k = (size_t)sy * ax.n + ox;
-    hb[k] = (float)vu;
-    hb[n + k] = (float)vv;
-    hb[2 * n + k] = mnu;
-    hb[3 * n + k] = mxu;
-    hb[4 * n + k] = mnv;
-    hb[5 * n + k] = mxv;
+    reinterpret_cast<float2 *>(hb)[k] = make_float2((float)vu, (float)vv);
+    reinterpret_cast<float4 *>(hb + ((2 * n + 3) & ~(size_t)3))[k] = make_float4(mnu, mxu, mnv, mxv);
 }
 
 __global__ __launch_bounds__(kBlock) void k_plain_v(const float *__restrict__ u, const float *__restrict__ v,
@@ -706,10 +726,12 @@ __global__ __launch_bounds__(kBlock) void k_plain_v(const float *__restrict__ u,
         if (am[j] == 0.0f)
             continue;
         const size_t k = (size_t)clampi(start + j, 0, ch - 1) * ow + ox;
-        vu += (double)wt[j] * hb[k];
-        vv += (double)wt[j] * hb[n + k];
-        mnu = fmn(mnu, hb[2 * n + k]); mxu = fmx(mxu, hb[3 * n + k]);
-        mnv = fmn(mnv, hb[4 * n + k]); mxv = fmx(mxv, hb[5 * n + k]);
+        const float2 val = reinterpret_cast<const float2 *>(hb)[k];
+        const float4 hull = reinterpret_cast<const float4 *>(hb + ((2 * n + 3) & ~(size_t)3))[k];
+        vu += (double)wt[j] * val.x;
+        vv += (double)wt[j] * val.y;
+        mnu = fmn(mnu, hull.x); mxu = fmx(mxu, hull.y);
+        mnv = fmn(mnv, hull.z); mxv = fmx(mxv, hull.w);
     }
     const double wsum = (double)ax.wsum[ox] * ay.wsum[oy];
     const double asum = (double)ax.wabs[ox] * ay.wabs[oy];
@@ -2216,8 +2238,11 @@ hipError_t launch_trust_workset(const uint64_t *seed, uint64_t *tmp, uint64_t *m
 hipError_t launch_mutual_gate(const float *lc, const float *u, const float *v, int w, int h, float *grad,
                               const uint8_t *chroma_mask, const uint32_t *counters, float *gate,
                               hipStream_t s) {
-    hipLaunchKernelGGL(k_grad3, rows_grid(w, h), dim3(kBlock), 0, s, lc, u, v, w, h, grad);
-    hipLaunchKernelGGL(k_gate, rows_grid(w, h), dim3(kBlock), 0, s, grad, w, h, chroma_mask, counters, gate);
+    const size_t n = (size_t)w * h;
+    float4 *g4 = reinterpret_cast<float4 *>(grad);
+    float2 *g2 = reinterpret_cast<float2 *>(grad + 4 * n);  // 16-byte aligned: 4n floats
+    hipLaunchKernelGGL(k_grad3, rows_grid(w, h), dim3(kBlock), 0, s, lc, u, v, w, h, g4, g2);
+    hipLaunchKernelGGL(k_gate, rows_grid(w, h), dim3(kBlock), 0, s, g4, g2, w, h, chroma_mask, counters, gate);
     return hipGetLastError();
 }
 

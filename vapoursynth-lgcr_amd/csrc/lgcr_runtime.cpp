@@ -648,7 +648,7 @@ void layout(const lgcr_clip *c, Workspace *w, Carver &cv) {
     w->lc = cv.take<float>(nc);
     w->ms = cv.take<float>(nc);
     w->grad = cv.take<float>(6 * nc);
-    w->hbuf = cv.take<float>(6 * size_t(std::max(c->ow, c->sw)) * c->ch);
+    w->hbuf = cv.take<float>(6 * size_t(std::max(c->ow, c->sw)) * c->ch + 4);
     w->out_u = cv.take<float>(no);
     w->out_v = cv.take<float>(no);
     w->plain_u = cv.take<float>(no);
