@@ -141,7 +141,12 @@ hipError_t launch_guided(const float *u, const float *v, int cw, int ch, const f
                          const float *lc, const float *ms, const AxisDev &ax, const AxisDev &ay,
                          const RadialDev &rd, const GuideParams &gp, const uint32_t *row_offsets,
                          const uint32_t *indices, float *out_u, float *out_v,
-                         const SelectorDev &sel, const TemporalDev &tmp, hipStream_t s);
+                         const SelectorDev &sel, const TemporalDev &tmp, hipStream_t s,
+                         const uint64_t *mask_bits = nullptr, int mask_w = 0, int mask_h = 0,
+                         int tiles_fit = 0);
+// whether every 8x8 output tile's chroma footprint (6 taps per axis) fits
+// the LDS window of the tiled guided pass
+int guided_tiles_fit(const int *xs, int ow, const int *ys, int oh);
 // TRecon motion search and chroma consistency (algos.cpp:683-807)
 hipError_t launch_block_match(const float *cur, const float *nbr, int w, int h, int block,
                               int search, float tsad, int16_t *mvx, int16_t *mvy, float *conf,

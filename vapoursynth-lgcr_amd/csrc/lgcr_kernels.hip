@@ -857,6 +857,8 @@ struct GuidedArgs {
     float *out_u, *out_v;
     SelectorDev sel;
     TemporalDev tmp;
+    const uint64_t *mbits;   // trust mask bits at mask resolution (tile kernel), nullptr = dense
+    int mw, mh, mw64;
 };
 
 // double-precision closed-form kernels for the temporal tap weight
@@ -909,8 +911,15 @@ __device__ __forceinline__ bool sel_compressed(const uint32_t *row_off, const ui
 
 // Feature switches are template parameters so each launch carries only the
 // code (and registers) of the features it uses.
-template <bool RADIAL, bool TEMPORAL, int SEL, bool CEDGE, int SUP>
-__device__ void guided_pixel(const GuidedArgs &A, int ox, int oy, uint32_t k) {
+// WIN: the pass-1 and tap loops read the chroma guide from an LDS window of
+// kTWin x kTWin samples whose origin is (wx0, wy0), filled with luv at
+// clamped coordinates (so it holds exactly what the global reads return).
+constexpr int kTile = 8, kTWin = 16;
+
+template <bool RADIAL, bool TEMPORAL, int SEL, bool CEDGE, int SUP, bool WIN = false>
+__device__ __attribute__((always_inline)) void guided_pixel(const GuidedArgs &A, int ox, int oy, uint32_t k,
+                                                            const float4 *win = nullptr, int wx0 = 0,
+                                                            int wy0 = 0) {
     const AxisDev &ax = A.ax, &ay = A.ay;
     const GuideParams &gp = A.gp;
     const int cw = A.cw, ch = A.ch, sw = A.sw, sh = A.sh;
@@ -978,7 +987,8 @@ __device__ void guided_pixel(const GuidedArgs &A, int ox, int oy, uint32_t k) {
         _Pragma("unroll") for (int i = 0; i < SX; ++i) {
             if (amx[i] == 0.0f)
                 continue;
-            const float4 q = A.luv[row + clampi(tx0 + i, 0, cw - 1)];
+            const float4 q = WIN ? win[(ty0 + j - wy0) * kTWin + (tx0 + i - wx0)]
+                                 : A.luv[row + clampi(tx0 + i, 0, cw - 1)];
             if (guided)
                 max_dl = fmx(max_dl, fabsf(q.x - L0));
             const float a = q.y, b = q.z;
@@ -1080,7 +1090,8 @@ __device__ void guided_pixel(const GuidedArgs &A, int ox, int oy, uint32_t k) {
         _Pragma("unroll") for (int i = 0; i < SX; ++i) {
             if (amx[i] == 0.0f)
                 continue;
-            const float4 q = A.luv[row + clampi(tx0 + i, 0, cw - 1)];
+            const float4 q = WIN ? win[(ty0 + j - wy0) * kTWin + (tx0 + i - wx0)]
+                                 : A.luv[row + clampi(tx0 + i, 0, cw - 1)];
             const float wx = RADIAL ? radial_weight(A.rd, rbase, SX, i, j, tx0, ty0, scx, scy) : wxp[i] * wy;
             float w;
             if (guided) {
@@ -1234,6 +1245,47 @@ __global__ __launch_bounds__(kBlock) void k_guided(GuidedArgs A) {
     }
 }
 
+
+// Hot path of the guided pass (6 taps per axis, separable, no cedge, no
+// compressed selector): one wave per 8x8 output tile, lanes on the tile's
+// pixels that are active in the trust mask, the tile's chroma-guide
+// footprint staged in LDS.  The host enables it when every tile's footprint
+// fits kTWin x kTWin.
+template <bool TEMPORAL, int SEL>
+__global__ __launch_bounds__(kBlock) void k_guided_tiles(GuidedArgs A, int tiles_x, int tiles) {
+    __shared__ float4 wins[kBlock / 64][kTWin * kTWin];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int t = blockIdx.x * (kBlock / 64) + wave;
+    if (t >= tiles)
+        return;  // whole wave
+    const int ow = A.ax.n, oh = A.ay.n, cw = A.cw, ch = A.ch;
+    const int tx = t % tiles_x, ty = t / tiles_x;
+    const int ox = tx * kTile + (lane & (kTile - 1)), oy = ty * kTile + (lane / kTile);
+    bool act = ox < ow && oy < oh;
+    if (act && A.mbits) {
+        const int mx = min(A.mw - 1, (int)((int64_t)ox * A.mw / ow));
+        const int my = min(A.mh - 1, (int)((int64_t)oy * A.mh / oh));
+        act = (A.mbits[(size_t)my * A.mw64 + (mx >> 6)] >> (mx & 63)) & 1ull;
+    }
+    if (__ballot(act) == 0)
+        return;  // no active pixel in the tile (uniform)
+    int x_lo = INT_MAX, x_hi = INT_MIN, y_lo = INT_MAX, y_hi = INT_MIN;
+    for (int c = 0; c < kTile; ++c) {
+        const int sx = A.ax.start[min(tx * kTile + c, ow - 1)], sy = A.ay.start[min(ty * kTile + c, oh - 1)];
+        x_lo = min(x_lo, sx); x_hi = max(x_hi, sx);
+        y_lo = min(y_lo, sy); y_hi = max(y_hi, sy);
+    }
+    const int nc = x_hi - x_lo + 6, nr = y_hi - y_lo + 6;
+    float4 *wb = wins[wave];
+    for (int e = lane; e < nc * nr; e += 64) {
+        const int r = e / nc, c = e - r * nc;
+        wb[r * kTWin + c] = A.luv[(size_t)clampi(y_lo + r, 0, ch - 1) * cw + clampi(x_lo + c, 0, cw - 1)];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    if (act)
+        guided_pixel<false, TEMPORAL, SEL, false, 6, true>(A, ox, oy, 0u, wb, x_lo, y_lo);
+}
 
 // ---------------------------------------------------------------------------
 // algo 4: LGF regression maps and selector blend (algos.cpp:241-593)
@@ -2266,9 +2318,21 @@ hipError_t launch_guided(const float *u, const float *v, int cw, int ch, const f
                          const float4 *yt, const float4 *luv, const float *lc, const float *ms,
                          const AxisDev &ax, const AxisDev &ay, const RadialDev &rd, const GuideParams &gp,
                          const uint32_t *row_offsets, const uint32_t *indices, float *out_u, float *out_v,
-                         const SelectorDev &sel, const TemporalDev &tmp, hipStream_t s) {
+                         const SelectorDev &sel, const TemporalDev &tmp, hipStream_t s, const uint64_t *mask_bits,
+                         int mask_w, int mask_h, int tiles_fit) {
     GuidedArgs A{u, v, cw, ch, y, sw, sh, yt, luv, lc, ms, ax, ay, rd, gp, row_offsets, indices, out_u, out_v,
-                 sel, tmp};
+                 sel, tmp, indices ? mask_bits : nullptr, mask_w, mask_h, (mask_w + 63) / 64};
+    if (tiles_fit && !gp.radial && !gp.cedge && ax.sup == 6 && ay.sup == 6 && sel.mode != SEL_FLOAT) {
+        const int tiles_x = (ax.n + kTile - 1) / kTile, tiles = tiles_x * ((ay.n + kTile - 1) / kTile);
+        const dim3 g((unsigned)((tiles + kBlock / 64 - 1) / (kBlock / 64)));
+        if (tmp.n > 0)
+            hipLaunchKernelGGL((k_guided_tiles<true, SEL_NONE>), g, dim3(kBlock), 0, s, A, tiles_x, tiles);
+        else if (sel.mode == SEL_MAPS)
+            hipLaunchKernelGGL((k_guided_tiles<false, SEL_MAPS>), g, dim3(kBlock), 0, s, A, tiles_x, tiles);
+        else
+            hipLaunchKernelGGL((k_guided_tiles<false, SEL_NONE>), g, dim3(kBlock), 0, s, A, tiles_x, tiles);
+        return hipGetLastError();
+    }
     const size_t n = (size_t)ax.n * ay.n;
     const unsigned blocks = (unsigned)std::min<size_t>(8192, (n + kBlock - 1) / kBlock);
     const bool radial = gp.radial != 0, temporal = tmp.n > 0, cedge = gp.cedge != 0;
@@ -2462,6 +2526,23 @@ hipError_t launch_sharpen(const float *src, float *dst, int w, int h, const floa
                   p.clamp_hull};
     hipLaunchKernelGGL(k_sharpen, rows_grid(w, h), dim3(kBlock), 0, s, A);
     return hipGetLastError();
+}
+
+int guided_tiles_fit(const int *xs, int ow, const int *ys, int oh) {
+    auto fit = [](const int *st, int n) {
+        for (int t0 = 0; t0 < n; t0 += kTile) {
+            int lo = INT_MAX, hi = INT_MIN;
+            for (int c = 0; c < kTile; ++c) {
+                const int v = st[std::min(t0 + c, n - 1)];
+                lo = std::min(lo, v);
+                hi = std::max(hi, v);
+            }
+            if (hi - lo + 6 > kTWin)
+                return false;
+        }
+        return true;
+    };
+    return fit(xs, ow) && fit(ys, oh);
 }
 
 size_t bm_scratch_bytes(int w, int h) {

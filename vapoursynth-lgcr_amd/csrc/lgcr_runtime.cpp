@@ -353,6 +353,7 @@ struct Geometry {
     WeightsDev yc_bx, yc_by, yc_cx, yc_cy;
     AxisDev sax, say;
     RadialDev srd;
+    int tiles_fit = 0;  // tiled guided pass usable (chroma footprints of 8x8 tiles fit LDS)
 };
 
 struct Workspace {
@@ -517,6 +518,7 @@ lgcr_status get_geometry(lgcr_clip *c, double sx, double sy, Geometry **out) {
     auto g = std::make_unique<Geometry>();
     g->sx = sx;
     g->sy = sy;
+    g->tiles_fit = guided_tiles_fit(ax.start.data(), ax.n, ay.start.data(), ay.n);
     HIP_TRY(hipSetDevice(c->dev->ordinal));
     HIP_TRY(hipMalloc(&g->dev, blob.host.size()));
     HIP_TRY(hipMemcpy(g->dev, blob.host.data(), blob.host.size(), hipMemcpyHostToDevice));
@@ -986,7 +988,7 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
         }
         HIP_TRY(launch_guided(w->u, w->v, c->cw, c->ch, w->y, c->sw, c->sh, w->yt, w->luv, w->lc,
                               gate, g.ax, g.ay, g.rd, gp, w->row_off, idx, w->out_u, w->out_v, sel,
-                              TemporalDev{}, s));
+                              TemporalDev{}, s, w->mask, c->sw, c->sh, g.tiles_fit));
         HIP_TRY(mark(LGCR_STAGE_APPLY_GUIDED_CORRECTION));
         if (algo4 && is_float) {
             HIP_TRY(launch_roi_mark(w->row_off, idx, c->ow, c->oh, sel, g.ax, g.ay, w->roi, c->cw, c->ch,
@@ -1189,7 +1191,8 @@ lgcr_status enqueue_trecon(lgcr_clip *c, Workspace *w, const Geometry &g, const 
     td.inv_wy = float(std::min(1.0, double(c->oh) / c->ch));
     HIP_TRY(launch_guided(w->u, w->v, c->cw, c->ch, w->y, c->sw, c->sh, w->yt, w->luv, w->lc,
                           gp.ms_on ? w->ms : nullptr, g.ax, g.ay, g.rd, gp, w->row_off,
-                          w->has_ws ? w->indices : nullptr, w->out_u, w->out_v, SelectorDev{}, td, s));
+                          w->has_ws ? w->indices : nullptr, w->out_u, w->out_v, SelectorDev{}, td, s, w->mask,
+                          c->sw, c->sh, g.tiles_fit));
     HIP_TRY(mark(LGCR_STAGE_APPLY_GUIDED_CORRECTION));
     // collaborative filter after the temporal pass (plugin.cpp:898-899)
     if (p.bm) {
