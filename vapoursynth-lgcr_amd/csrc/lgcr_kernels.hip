@@ -38,6 +38,25 @@ __device__ __forceinline__ float recip_ge1(float y) {
     return 1.0f / y;
 }
 
+// XCD-aware block order.  Workgroups are dispatched round-robin over the 8
+// XCDs, each with its own L2; remapping the linear block id gives every XCD
+// a contiguous range of tiles, so halos shared by neighbouring tiles hit the
+// same L2.  Bijective for any block count.
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ unsigned xcd_linear() {
+    const unsigned nb = gridDim.x * gridDim.y;
+    const unsigned lin = blockIdx.y * gridDim.x + blockIdx.x;
+    const unsigned xcd = lin % kXcds, slot = lin / kXcds;
+    const unsigned q = nb / kXcds, r = nb % kXcds;
+    return xcd * q + min(xcd, r) + slot;
+}
+// (x, y) block coordinates of the remapped order for a 2D grid
+__device__ __forceinline__ void xcd_block2(unsigned &bx, unsigned &by) {
+    const unsigned l = xcd_linear();
+    bx = l % gridDim.x;
+    by = l / gridDim.x;
+}
+
 // Bilinear sample at double coordinates with clamped taps (lgcr.h:467-475).
 __device__ float bilinear(const float *p, int w, int h, double x, double y) {
     const int x0 = (int)floor(x), y0 = (int)floor(y);
@@ -287,7 +306,9 @@ __global__ __launch_bounds__(TW * TH) void k_sobel_tensor(const float *__restric
     __shared__ float sy[TH + 4][TW + 4];
     __shared__ float gx[TH + 2][TW + 2], gy[TH + 2][TW + 2];
     __shared__ float vxx[TH][TW + 2], vxy[TH][TW + 2], vyy[TH][TW + 2];
-    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+    unsigned bx, by;
+    xcd_block2(bx, by);
+    const int x0 = bx * TW, y0 = by * TH;
     const int tid = threadIdx.y * TW + threadIdx.x;
     for (int i = tid; i < (TH + 4) * (TW + 4); i += TW * TH) {
         const int ly = i / (TW + 4), lx = i % (TW + 4);
@@ -345,7 +366,7 @@ __global__ __launch_bounds__(TW * TH) void k_sobel_tensor(const float *__restric
     const size_t o = (size_t)r * w + x;
     yt[o] = make_float4(sy[ly + 2][threadIdx.x + 2], oxx, oxy, oyy);
     if (seed && threadIdx.x == 0)
-        seed[(size_t)r * w64 + blockIdx.x] = bits;
+        seed[(size_t)r * w64 + bx] = bits;
 }
 
 // ---------------------------------------------------------------------------
@@ -575,7 +596,9 @@ __global__ void k_zero_u32(uint32_t *p, int n) {
 // U_x, U_y}, g2 = {V_x, V_y}, so a gate tap is two loads.
 __global__ __launch_bounds__(kBlock) void k_grad3(const float *lc, const float *u, const float *v,
                                                   int w, int h, float4 *g4, float2 *g2) {
-    const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
+    unsigned bx, by;
+    xcd_block2(bx, by);
+    const int x = bx * kBlock + threadIdx.x, y = by;
     if (x >= w)
         return;
     const size_t i = (size_t)y * w + x;
@@ -683,7 +706,9 @@ __global__ __launch_bounds__(kBlock) void k_gate(const float4 *__restrict__ g4, 
 
 __global__ __launch_bounds__(kBlock) void k_plain_h(const float *__restrict__ u, const float *__restrict__ v,
                                                     int cw, int ch, AxisDev ax, float *hb) {
-    const int ox = blockIdx.x * kBlock + threadIdx.x, sy = blockIdx.y;
+    unsigned bx, by;
+    xcd_block2(bx, by);
+    const int ox = bx * kBlock + threadIdx.x, sy = by;
     if (ox >= ax.n)
         return;
     const int start = ax.start[ox];
@@ -712,7 +737,9 @@ __global__ __launch_bounds__(kBlock) void k_plain_v(const float *__restrict__ u,
                                                     int cw, int ch, AxisDev ax, AxisDev ay,
                                                     const float *__restrict__ hb, float *out_u,
                                                     float *out_v, int clamp_hull, float margin) {
-    const int ox = blockIdx.x * kBlock + threadIdx.x, oy = blockIdx.y;
+    unsigned bx, by;
+    xcd_block2(bx, by);
+    const int ox = bx * kBlock + threadIdx.x, oy = by;
     if (ox >= ax.n)
         return;
     const int ow = ax.n;

@@ -853,7 +853,7 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
             ev = w->pool.back();
             w->pool.pop_back();
         } else {
-            const hipError_t e = hipEventCreate(&ev);
+            const hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableSystemFence);
             if (e != hipSuccess)
                 return e;
         }
@@ -1096,7 +1096,7 @@ lgcr_status enqueue_trecon(lgcr_clip *c, Workspace *w, const Geometry &g, const 
             ev = w->pool.back();
             w->pool.pop_back();
         } else {
-            const hipError_t e = hipEventCreate(&ev);
+            const hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableSystemFence);
             if (e != hipSuccess)
                 return e;
         }
