@@ -1,0 +1,416 @@
+"""Host-side mirror of the LGCR VapourSynth plugin surface (src/plugin.cpp)
+over the MI355X backend.
+
+``core.lgcr.Recon``, ``core.lgcr.TRecon`` and ``core.lgcr.Sharpen`` take the
+reference's arguments with the same names, defaults, ranges and error
+messages (lgcrCreate, plugin.cpp:1012-1158; tReconCreate, plugin.cpp:921-990;
+sharpenCreate, plugin.cpp:85-111) and return clips whose frames are
+computed by ``liblgcr_hip.so``.  Clips and frames are small in-memory
+stand-ins for VapourSynth's (VapourSynth itself is not a dependency):
+planar numpy planes plus a property dict.  Per frame the filters apply the
+reference's property rules: H.273 ``_ChromaLocation`` siting on subsampled
+axes unless ``loc`` is given (plugin.cpp:157-175), the prop removed from 4:4:4
+outputs (plugin.cpp:268-271, 840-841), the source luma plane shared by
+same-size Recon (plugin.cpp:265-269), ``_LGCR_*`` profile props when
+``LGCR_PROFILE`` is set (plugin.cpp:114-155).
+"""
+from __future__ import annotations
+
+import math
+import os
+import threading
+from dataclasses import dataclass, field, replace
+
+import numpy as np
+
+from . import abi
+
+YUV = "YUV"
+INTEGER, FLOAT = 0, 1
+
+
+class Error(Exception):
+    """Filter creation or frame errors (vs.Error in VapourSynth)."""
+
+
+@dataclass(frozen=True)
+class VideoFormat:
+    color_family: str
+    sample_type: int
+    bits_per_sample: int
+    subsampling_w: int
+    subsampling_h: int
+
+    @property
+    def bytes_per_sample(self):
+        return 4 if self.sample_type == FLOAT else (1 if self.bits_per_sample <= 8 else 2)
+
+    @property
+    def dtype(self):
+        return np.float32 if self.sample_type == FLOAT else (np.uint8 if self.bytes_per_sample == 1 else np.uint16)
+
+
+def query_video_format(color_family, sample_type, bits, ss_w, ss_h):
+    return VideoFormat(color_family, sample_type, bits, ss_w, ss_h)
+
+
+@dataclass
+class VideoFrame:
+    format: VideoFormat
+    planes: list
+    props: dict = field(default_factory=dict)
+
+    @property
+    def width(self):
+        return self.planes[0].shape[1]
+
+    @property
+    def height(self):
+        return self.planes[0].shape[0]
+
+    def __getitem__(self, p):
+        return self.planes[p]
+
+
+class VideoNode:
+    """A clip: constant format and size, frames produced on request."""
+
+    def __init__(self, fmt, width, height, num_frames, get):
+        self.format, self.width, self.height, self.num_frames = fmt, width, height, num_frames
+        self._get = get
+
+    def get_frame(self, n):
+        if not 0 <= n < self.num_frames:
+            raise Error(f"requested frame {n} is out of range")
+        return self._get(n)
+
+
+def clip_from_planes(frames, fmt: VideoFormat, props=None):
+    """Source clip over host planes: frames is a list of (Y, U, V) arrays;
+    props a dict or a list of per-frame dicts."""
+    h, w = frames[0][0].shape
+
+    def get(n):
+        p = props[n] if isinstance(props, list) else (props or {})
+        return VideoFrame(fmt, list(frames[n]), dict(p))
+
+    return VideoNode(fmt, w, h, len(frames), get)
+
+
+# ---------------------------------------------------------------------------
+# argument handling shared by the filters
+# ---------------------------------------------------------------------------
+
+def _check_args(fn, given, allowed):
+    for k in given:
+        if k not in allowed:
+            raise Error(f"{fn}: argument {k} is not expected")
+
+
+def _get(args, key, default):
+    v = args.get(key)
+    return default if v is None else v
+
+
+def _device():
+    with _Device.lock:
+        if _Device.dev is None:
+            _Device.dev = abi.Device(0)
+        return _Device.dev
+
+
+class _Device:
+    lock = threading.Lock()
+    dev = None
+
+
+def _profiling():
+    v = os.environ.get("LGCR_PROFILE", "")
+    return bool(v) and v[0] != "0"
+
+
+def _siting(props, fmt, shift_x, shift_y, loc_set):
+    """applyFrameChromaSiting (plugin.cpp:157-175)."""
+    if fmt.subsampling_w == 0:
+        shift_x = 0.0
+    if fmt.subsampling_h == 0:
+        shift_y = 0.0
+    if loc_set or "_ChromaLocation" not in props:
+        return shift_x, shift_y
+    cl = int(props["_ChromaLocation"])
+    if fmt.subsampling_w > 0:
+        shift_x = -0.5 if cl in (0, 2, 4) else 0.0
+    if fmt.subsampling_h > 0:
+        shift_y = -0.5 if cl in (2, 3) else (0.5 if cl in (4, 5) else 0.0)
+    return shift_x, shift_y
+
+
+def _abi_format(fmt, w, h):
+    return abi.Format(fmt.sample_type, fmt.bits_per_sample, fmt.bytes_per_sample, fmt.subsampling_w,
+                      fmt.subsampling_h, w, h)
+
+
+# frame-property names of the stages (stageName, src/pipeline.h:222-244), in
+# the backend's LGCR_STAGE_* order
+PROP_STAGE_NAMES = ["input_conversion", "luma_resample", "buildGuideMaps", "buildTrustMask", "buildMutualGate",
+                    "plainChroma", "buildLGF", "buildAffineMaps", "buildDetail", "reconstructChroma",
+                    "selectorBlend", "detailTransfer", "collaborative_chroma", "downsample_base",
+                    "downsample_guide", "downsample_candidate_score", "downsample_output", "backProject",
+                    "output_conversion"]
+
+
+def _profile_props(props, st, fmt, sw, sh, ow, oh, bm):
+    """writeProfile (plugin.cpp:126-155) from the backend's per-frame stats.
+    Conversion work follows the reference's counting (plugin.cpp:250-253,
+    708-710): float input planes are viewed, not converted, and float output
+    is converted only with bm."""
+    is_float = fmt.sample_type == FLOAT
+    cw, ch = sw >> fmt.subsampling_w, sh >> fmt.subsampling_h
+    pixels = [int(v) for v in st.stage_pixels]
+    pixels[0] = 0 if is_float else sw * sh + 2 * cw * ch
+    pixels[18] = 2 * ow * oh if (not is_float or bm) else 0
+    for i, name in enumerate(PROP_STAGE_NAMES):
+        props[f"_LGCR_{name}_us"] = float(st.stage_us[i])
+        props[f"_LGCR_{name}_pixels"] = pixels[i]
+        props[f"_LGCR_{name}_taps"] = 0
+    props["_LGCR_total_us"] = float(st.total_us)
+    props["_LGCR_output_pixels"] = int(st.output_pixels)
+    props["_LGCR_taps_visited"] = 0
+    props["_LGCR_sparse_active_ratio"] = (float(st.active_output_pixels) / st.sparse_total_pixels
+                                          if st.sparse_total_pixels else 1.0)
+
+
+# ---------------------------------------------------------------------------
+# Recon (lgcrCreate / lgcrGetFrame)
+# ---------------------------------------------------------------------------
+
+RECON_ARGS = ("width", "height", "kernel", "taps", "algo", "b", "c", "strength", "sigma", "sratio", "sdb",
+              "stretch", "gsigma", "rescue", "ridge", "cedge", "ar", "reg", "loc", "sparse", "bp", "ms", "qgate",
+              "bm")
+
+
+def recon_params(clip: VideoNode, **args):
+    """lgcrCreate's argument parsing and validation (plugin.cpp:1012-1142);
+    returns the backend parameters."""
+    _check_args("Recon", args, RECON_ARGS)
+    fmt = clip.format
+    if fmt.color_family != YUV or (fmt.sample_type == FLOAT and fmt.bits_per_sample != 32):
+        raise Error("LGCR: input must be planar YUV (8-16 bit int or 32 bit float)")
+    if fmt.subsampling_w > 1 or fmt.subsampling_h > 1:
+        raise Error("LGCR: only 4:4:4 / 4:2:2 / 4:2:0 input is supported")
+    if clip.width == 0 or clip.height == 0:
+        raise Error("LGCR: constant input dimensions required")
+    p = abi.Params()
+    abi.lib().lgcr_params_default(p)
+    p.out_w = int(_get(args, "width", clip.width))
+    p.out_h = int(_get(args, "height", clip.height))
+    if p.out_w <= 0 or p.out_h <= 0:
+        raise Error("LGCR: invalid width/height")
+    k = _get(args, "kernel", "lanczos")
+    if isinstance(k, bytes):
+        k = k.decode()
+    p.radial = 0
+    if k == "bilinear":
+        p.kernel, p.support = abi.KERNELS["bilinear"], 1.0
+    elif k == "bicubic":
+        p.kernel, p.support = abi.KERNELS["bicubic"], 2.0
+        p.kp1 = float(_get(args, "b", 0.0))
+        p.kp2 = float(_get(args, "c", 0.6))
+    elif k in ("lanczos", "jinc"):
+        taps = int(_get(args, "taps", 3))
+        if taps < 1 or taps > 64:
+            raise Error(f"LGCR: {k} taps must be in the range 1..64")
+        p.kernel = abi.KERNELS[k]
+        p.kp1 = float(taps)
+        p.support = float(taps)
+        p.radial = 1 if k == "jinc" else 0
+    elif k == "spline16":
+        p.kernel, p.support = abi.KERNELS["spline16"], 2.0
+    elif k == "spline36":
+        p.kernel, p.support = abi.KERNELS["spline36"], 3.0
+    else:
+        raise Error('LGCR: kernel must be "bilinear", "bicubic", "lanczos", "spline16", "spline36" or "jinc"')
+    p.strength = float(_get(args, "strength", 0.8))
+    p.sigma = float(_get(args, "sigma", 0.01))
+    p.sratio = float(_get(args, "sratio", 0.15))
+    p.algo = int(_get(args, "algo", 2))
+    if p.algo not in (2, 4, 6):
+        raise Error("LGCR: algo must be 2 (guided), 4 (selector) or 6 (detail transfer)")
+    p.sdb = float(_get(args, "sdb", 3.0))
+    p.stretch = float(_get(args, "stretch", 1.0))
+    p.gsigma = float(_get(args, "gsigma", 2.5))
+    p.rescue = float(_get(args, "rescue", 1.0))
+    p.ridge = 1 if int(_get(args, "ridge", 1)) != 0 else 0
+    p.cedge = 1 if int(_get(args, "cedge", 0)) != 0 else 0
+    p.ms = float(_get(args, "ms", 1.0))
+    p.qgate = float(_get(args, "qgate", 1.0))
+    p.ar_margin = float(_get(args, "ar", 0.0))
+    p.reg = float(_get(args, "reg", 0.005))
+    loc = args.get("loc")
+    p.shift_x, p.shift_y, p.loc_set = -0.5, 0.0, 0
+    if loc is not None:
+        if isinstance(loc, bytes):
+            loc = loc.decode()
+        if loc == "left":
+            p.shift_x = -0.5
+        elif loc == "center":
+            p.shift_x = 0.0
+        else:
+            raise Error('LGCR: loc must be "left" or "center"')
+        p.loc_set = 1
+    p.sparse = 1 if int(_get(args, "sparse", 1)) != 0 else 0
+    p.bm = 1 if int(_get(args, "bm", 0)) != 0 else 0
+    p.bp = float(_get(args, "bp", 0.0))
+    if fmt.subsampling_w == 0:
+        p.shift_x = 0.0
+    if fmt.subsampling_h == 0:
+        p.shift_y = 0.0
+    if (p.strength < 0.0 or p.strength > 1.0 or p.sigma <= 0.0 or p.sratio <= 0.0 or p.sdb <= 0.0
+            or p.stretch < 0.0 or p.gsigma <= 0.0 or p.reg <= 0.0 or p.bp < 0.0 or p.bp > 1.0 or p.ms < 0.0
+            or p.ms > 1.0 or p.qgate < 0.0 or p.qgate > 1.0 or p.rescue < 0.0 or p.rescue > 1.0):
+        raise Error("LGCR: invalid parameter range (need 0<=strength/rescue<=1, 0<=bp/ms/qgate<=1, "
+                    "sigma/sratio/sdb/gsigma/reg>0, stretch>=0)")
+    return p
+
+
+def Recon(clip: VideoNode, **args) -> VideoNode:
+    p = recon_params(clip, **args)
+    fmt_in = clip.format
+    fmt_out = replace(fmt_in, subsampling_w=0, subsampling_h=0)
+    backend = abi.Clip(_device(), p, _abi_format(fmt_in, clip.width, clip.height))
+    ow, oh = p.out_w, p.out_h
+    same = ow == clip.width and oh == clip.height
+    shift_x, shift_y, loc_set = p.shift_x, p.shift_y, bool(p.loc_set)
+
+    def get(n):
+        src = clip.get_frame(n)
+        sx, sy = _siting(src.props, fmt_in, shift_x, shift_y, loc_set)
+        profile = _profiling()
+        res = backend.recon_host(*src.planes, shift_x=sx, shift_y=sy, stats=profile, share_luma=same)
+        out, st = res if profile else (res, None)
+        props = dict(src.props)
+        props.pop("_ChromaLocation", None)
+        if profile:
+            _profile_props(props, st, fmt_in, clip.width, clip.height, ow, oh, bool(p.bm))
+        return VideoFrame(fmt_out, [src.planes[0] if same else out[0], out[1], out[2]], props)
+
+    node = VideoNode(fmt_out, ow, oh, clip.num_frames, get)
+    node._backend = backend
+    return node
+
+
+# ---------------------------------------------------------------------------
+# TRecon (tReconCreate / tReconGetFrame)
+# ---------------------------------------------------------------------------
+
+TRECON_ARGS = ("strength", "sigma", "sratio", "sdb", "gsigma", "stretch", "ar", "ridge", "sparse", "ms", "trad",
+               "tsearch", "tsad", "bm")
+
+
+def temporal_frame_numbers(n, n_last, trad):
+    """plugin.cpp:729-738: n-trad..n+trad clamped, centre and duplicates
+    removed."""
+    out = []
+    for k in range(-trad, trad + 1):
+        fn = min(max(n + k, 0), n_last)
+        if fn != n and fn not in out:
+            out.append(fn)
+    return out
+
+
+def trecon_params(clip: VideoNode, **args):
+    """tReconCreate's parsing and validation (plugin.cpp:921-976)."""
+    _check_args("TRecon", args, TRECON_ARGS)
+    fmt = clip.format
+    if (fmt.color_family != YUV or (fmt.sample_type == FLOAT and fmt.bits_per_sample != 32)
+            or fmt.subsampling_w > 1 or fmt.subsampling_h > 1):
+        raise Error("TRecon: planar YUV 444/422/420, 8-16 bit int or 32 bit float")
+    if clip.width == 0 or clip.height == 0 or clip.num_frames == 0:
+        raise Error("TRecon: constant dimensions and frame count required")
+    p = abi.Params()
+    abi.lib().lgcr_params_default_trecon(p)
+    p.strength = float(_get(args, "strength", 0.8))
+    p.sigma = float(_get(args, "sigma", 0.01))
+    p.sratio = float(_get(args, "sratio", 0.15))
+    p.sdb = float(_get(args, "sdb", 3.0))
+    p.gsigma = float(_get(args, "gsigma", 2.5))
+    p.stretch = float(_get(args, "stretch", 1.0))
+    p.ar_margin = float(_get(args, "ar", 0.0))
+    p.ms = float(_get(args, "ms", 1.0))
+    p.trad = int(_get(args, "trad", 1))
+    p.tsearch = int(_get(args, "tsearch", 6))
+    p.tsad = float(_get(args, "tsad", 0.02))
+    if (p.strength < 0.0 or p.strength > 1.0 or p.sigma <= 0.0 or p.sratio <= 0.0 or p.sdb <= 0.0
+            or p.stretch < 0.0 or p.gsigma <= 0.0 or p.ms < 0.0 or p.ms > 1.0 or p.trad < 0 or p.trad > 8
+            or p.tsearch < 0 or p.tsearch > 64 or p.tsad <= 0.0):
+        raise Error("TRecon: invalid parameter range (need 0<=strength<=1, "
+                    "sigma/sratio/sdb/gsigma/tsad>0, stretch>=0, 0<=trad<=8, 0<=tsearch<=64)")
+    p.ridge = 1 if int(_get(args, "ridge", 1)) != 0 else 0
+    p.bm = 1 if int(_get(args, "bm", 0)) != 0 else 0
+    p.sparse = 1 if int(_get(args, "sparse", 0)) != 0 else 0
+    p.algo, p.kernel, p.kp1, p.support, p.radial = 2, abi.KERNELS["lanczos"], 3.0, 3.0, 0
+    p.shift_x, p.shift_y, p.loc_set = -0.5, 0.0, 0
+    return p
+
+
+def TRecon(clip: VideoNode, **args) -> VideoNode:
+    p = trecon_params(clip, **args)
+    fmt_in = clip.format
+    fmt_out = replace(fmt_in, subsampling_w=0, subsampling_h=0)
+    backend = abi.Clip(_device(), p, _abi_format(fmt_in, clip.width, clip.height), temporal=True)
+    n_last = clip.num_frames - 1
+    trad = p.trad
+
+    def get(n):
+        src = clip.get_frame(n)
+        sx, sy = _siting(src.props, fmt_in, -0.5, 0.0, False)
+        nbrs = [clip.get_frame(k).planes for k in temporal_frame_numbers(n, n_last, trad)]
+        out = backend.trecon_host(src.planes, nbrs, shift_x=sx, shift_y=sy)
+        props = dict(src.props)
+        props.pop("_ChromaLocation", None)
+        return VideoFrame(fmt_out, out, props)
+
+    node = VideoNode(fmt_out, clip.width, clip.height, clip.num_frames, get)
+    node._backend = backend
+    return node
+
+
+# ---------------------------------------------------------------------------
+# Sharpen (sharpenCreate / sharpenGetFrame)
+# ---------------------------------------------------------------------------
+
+SHARPEN_ARGS = ("alpha", "sigma", "sratio", "gspatial", "ar")
+
+
+def Sharpen(clip: VideoNode, **args) -> VideoNode:
+    _check_args("Sharpen", args, SHARPEN_ARGS)
+    fmt = clip.format
+    if fmt.color_family != YUV or (fmt.sample_type == FLOAT and fmt.bits_per_sample != 32):
+        raise Error("LGCR Sharpen: planar YUV only")
+    sp = abi.sharpen_params(alpha=float(_get(args, "alpha", 0.3)), sigma=float(_get(args, "sigma", 0.01)),
+                            sratio=float(_get(args, "sratio", 0.15)), gspatial=float(_get(args, "gspatial", 1.2)),
+                            ar=float(_get(args, "ar", 0.0)))
+    if sp.sigma <= 0.0 or sp.sratio <= 0.0 or sp.gspatial <= 0.0:
+        raise Error("LGCR Sharpen: sigma/sratio/gspatial must be > 0")
+    backend = abi.SharpenClip(_device(), sp, _abi_format(fmt, clip.width, clip.height))
+
+    def get(n):
+        src = clip.get_frame(n)
+        return VideoFrame(fmt, backend.sharpen_host(*src.planes), dict(src.props))
+
+    node = VideoNode(fmt, clip.width, clip.height, clip.num_frames, get)
+    node._backend = backend
+    return node
+
+
+class _Namespace:
+    Recon = staticmethod(Recon)
+    TRecon = staticmethod(TRecon)
+    Sharpen = staticmethod(Sharpen)
+
+
+class _Core:
+    lgcr = _Namespace()
+
+
+core = _Core()
