@@ -24,6 +24,10 @@ struct AxisDev {
     const float *cb_f = nullptr;
     const int *lb_i0 = nullptr;   // lpos on the source luma grid
     const float *lb_f = nullptr;
+    // sup <= 6: one 64-byte record per output, 4 float4:
+    // {start, pos, lpos, active-tap bits}, {lb_i0, lb_f, cb_i0, cb_f},
+    // {w0, w1, w2, w3}, {w4, w5, 0, 0} (ints stored by bit pattern)
+    const float4 *rec = nullptr;
 };
 
 // Separable polyphase weight table (WeightTable, src/lgcr.h:601-606).

@@ -25,16 +25,18 @@ __device__ __forceinline__ float at(const float *p, int w, int h, int x, int y) 
     return p[(size_t)clampi(y, 0, h - 1) * w + clampi(x, 0, w - 1)];
 }
 
-// Correctly rounded 1/y for y >= 1: hardware reciprocal refined by one FMA
-// Newton step.  On gfx950 this equals the IEEE quotient 1.0f / y for every
-// float y in [1, 2^126) (checked exhaustively on the device); larger, inf and
-// NaN arguments take the division.
+// Correctly rounded 1/y for y >= 1.  rcp_nr is the hardware reciprocal
+// refined by one FMA Newton step; on gfx950 it equals the IEEE quotient
+// 1.0f / y for every float y in [1, 2^126) (checked exhaustively on the
+// device).  recip_ge1 sends larger, inf and NaN arguments to the division.
+__device__ __forceinline__ float rcp_nr(float y) {
+    const float r = __builtin_amdgcn_rcpf(y);
+    const float e = __builtin_fmaf(-y, r, 1.0f);
+    return __builtin_fmaf(e, r, r);
+}
 __device__ __forceinline__ float recip_ge1(float y) {
-    if (y < 0x1p126f) {
-        const float r = __builtin_amdgcn_rcpf(y);
-        const float e = __builtin_fmaf(-y, r, 1.0f);
-        return __builtin_fmaf(e, r, r);
-    }
+    if (y < 0x1p126f)
+        return rcp_nr(y);
     return 1.0f / y;
 }
 
@@ -943,24 +945,167 @@ __device__ __forceinline__ bool sel_compressed(const uint32_t *row_off, const ui
 // clamped coordinates (so it holds exactly what the global reads return).
 constexpr int kTile = 8, kTWin = 16;
 
+// Per-output-pixel axis data: first tap, positions, bilinear indices, the
+// tap weights and active-tap flags.  6-tap axes read everything from one
+// 64-byte record (AxisDev::rec) and keep it in registers; other supports
+// read the weights from the tables.
+template <int SUP> struct PixAxis {
+    int start, lb_i0, cb_i0;
+    float pos, lpos, lb_f, cb_f;
+    const float *w, *am;
+    __device__ __forceinline__ bool on(int i) const { return am[i] != 0.0f; }
+    __device__ __forceinline__ float wt(int i) const { return w[i]; }
+};
+template <> struct PixAxis<6> {
+    int start, lb_i0, cb_i0;
+    float pos, lpos, lb_f, cb_f;
+    uint32_t bits;
+    float w[6];
+    __device__ __forceinline__ bool on(int i) const { return (bits >> i) & 1u; }
+    __device__ __forceinline__ float wt(int i) const { return w[i]; }
+};
+
+template <int SUP> __device__ __forceinline__ PixAxis<SUP> pix_axis(const AxisDev &a, int o) {
+    PixAxis<SUP> p;
+    if constexpr (SUP == 6) {
+        const float4 *r = a.rec + 4 * (size_t)o;
+        const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+        p.start = __float_as_int(r0.x);
+        p.pos = r0.y;
+        p.lpos = r0.z;
+        p.bits = __float_as_uint(r0.w);
+        p.lb_i0 = __float_as_int(r1.x);
+        p.lb_f = r1.y;
+        p.cb_i0 = __float_as_int(r1.z);
+        p.cb_f = r1.w;
+        p.w[0] = r2.x; p.w[1] = r2.y; p.w[2] = r2.z; p.w[3] = r2.w;
+        p.w[4] = r3.x; p.w[5] = r3.y;
+    } else {
+        p.start = a.start[o];
+        p.pos = a.pos[o];
+        p.lpos = a.lpos[o];
+        p.lb_i0 = a.lb_i0[o];
+        p.lb_f = a.lb_f[o];
+        p.cb_i0 = a.cb_i0[o];
+        p.cb_f = a.cb_f[o];
+        p.w = a.w + (size_t)o * a.sup;
+        p.am = a.am + (size_t)o * a.sup;
+    }
+    return p;
+}
+
+// Per-pixel inputs of the weighted tap sum.
+struct TapIn {
+    float L0, nxv, nyv, scx, scy, inv_s2, guide_fade, inv_g2, inv_g2p, rescue;
+    int tx0, ty0;
+};
+struct TapAcc {
+    double u = 0.0, v = 0.0, wsum = 0.0, sabs = 0.0;
+};
+
+// Guided tap sum (recon.cpp:581-627).  With fixed 6-tap supports the loops
+// are straight-line code: inactive taps are computed and their weight is
+// replaced by an exact zero, which leaves every accumulator unchanged.
+// FAST: the caller has shown every reciprocal argument of the active taps
+// lies in [1, 2^126), where rcp_nr is exact.  GUIDED: strength > 0.
+template <bool RADIAL, int SUP, bool WIN, bool FAST, bool GUIDED>
+__device__ __forceinline__ void guided_taps(const GuidedArgs &A, const PixAxis<SUP> &px, const PixAxis<SUP> &py,
+                                            int ox, int oy, const TapIn &t, const float4 *win, int wx0, int wy0,
+                                            TapAcc &acc) {
+    const GuideParams &gp = A.gp;
+    const int cw = A.cw, ch = A.ch;
+    const int SX = SUP ? SUP : A.ax.sup, SY = SUP ? SUP : A.ay.sup;
+    const float *rbase = RADIAL ? radial_at(A.rd, ox, oy) : nullptr;
+    // Window offset of the first tap.  The empty asm hides its value so the
+    // compiler reloads the window here instead of keeping all 36 samples of
+    // the first pass live in registers.
+    int woff = (t.ty0 - wy0) * kTWin + (t.tx0 - wx0);
+    int goff = 0;  // the same for the global chroma guide
+    if (WIN)
+        __asm__ volatile("" : "+v"(woff));
+    else if (SUP)
+        __asm__ volatile("" : "+s"(goff));
+    PixAxis<SUP> qx = px, qy = py;
+    if constexpr (SUP == 6) {
+        // likewise the active-tap masks are rebuilt here, not kept from the
+        // first pass
+        __asm__ volatile("" : "+v"(qx.bits), "+v"(qy.bits));
+    }
+    constexpr int NH = SUP ? SUP : 1;
+    float dxn[NH], dxt[NH];
+    if constexpr (SUP != 0) {
+        _Pragma("unroll") for (int i = 0; i < SUP; ++i) {
+            const float dx = ((float)(t.tx0 + i) - t.scx) * gp.frw;
+            dxn[i] = dx * t.nxv;
+            dxt[i] = -dx * t.nyv;
+        }
+    }
+    _Pragma("unroll") for (int j = 0; j < SY; ++j) {
+        const bool onj = qy.on(j);
+        if (!SUP && !onj)
+            continue;
+        const size_t row = (size_t)clampi(t.ty0 + j, 0, ch - 1) * cw;
+        const float wy = qy.wt(j);
+        const float dy = ((float)(t.ty0 + j) - t.scy) * gp.frh;
+        const float dyn = dy * t.nyv, dyt = dy * t.nxv;
+        _Pragma("unroll") for (int i = 0; i < SX; ++i) {
+            const bool on = onj && qx.on(i);
+            if (!SUP && !on)
+                continue;
+            const float4 q = WIN ? win[woff + j * kTWin + i] : A.luv[row + goff + clampi(t.tx0 + i, 0, cw - 1)];
+            const float wx = RADIAL ? radial_weight(A.rd, rbase, SX, i, j, t.tx0, t.ty0, t.scx, t.scy)
+                                    : qx.wt(i) * wy;
+            float w;
+            if (GUIDED) {
+                float dp, dt;
+                if constexpr (SUP != 0) {
+                    dp = dxn[i] + dyn;
+                    dt = dxt[i] + dyt;
+                } else {
+                    const float dx = ((float)(t.tx0 + i) - t.scx) * gp.frw;
+                    dp = dx * t.nxv + dyn;
+                    dt = -dx * t.nyv + dyt;
+                }
+                const float dL = q.x - t.L0;
+                const float ys = 1.0f + dL * dL * t.inv_s2;
+                const float yb = 1.0f + dp * dp * t.inv_g2 + dt * dt * t.inv_g2p;
+                const float sim_raw = FAST ? rcp_nr(ys) : recip_ge1(ys);
+                const float sim = 1.0f - t.guide_fade * (1.0f - sim_raw);
+                const float bump = FAST ? rcp_nr(yb) : recip_ge1(yb);
+                w = sim * (wx + t.rescue * bump);
+            } else {
+                w = wx;
+            }
+            if (SUP)
+                w = on ? w : 0.0f;
+            acc.u += (double)w * q.y;
+            acc.v += (double)w * q.z;
+            acc.wsum += w;
+            acc.sabs += fabsf(w);
+        }
+        // one row of taps in flight at a time keeps the register footprint
+        // of the unrolled loop small
+        if (SUP)
+            __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
 template <bool RADIAL, bool TEMPORAL, int SEL, bool CEDGE, int SUP, bool WIN = false>
 __device__ __attribute__((always_inline)) void guided_pixel(const GuidedArgs &A, int ox, int oy, uint32_t k,
                                                             const float4 *win = nullptr, int wx0 = 0,
                                                             int wy0 = 0) {
-    const AxisDev &ax = A.ax, &ay = A.ay;
     const GuideParams &gp = A.gp;
     const int cw = A.cw, ch = A.ch, sw = A.sw, sh = A.sh;
     const bool guided = gp.strength > 0.0f;
+    const PixAxis<SUP> px = pix_axis<SUP>(A.ax, ox), py = pix_axis<SUP>(A.ay, oy);
 
     // --- guide metadata (recon.cpp:155-218)
     float L0 = 0.0f, nxv = 0.0f, nyv = 0.0f, coherence = 0.0f, min_diff = 0.0f, ridge = 0.0f;
     float phase = 0.0f, mfade = 1.0f;
     bool has_dir = false;
-    const float scx = ax.pos[ox], scy = ay.pos[oy];
+    const float scx = px.pos, scy = py.pos;
     if (guided) {
-        const int bxi = ax.lb_i0[ox], byi = ay.lb_i0[oy];
-        const float bxf = ax.lb_f[ox], byf = ay.lb_f[oy];
-        const float4 t = bilinear_fast4(A.yt, sw, sh, bxi, bxf, byi, byf);
+        const float4 t = bilinear_fast4(A.yt, sw, sh, px.lb_i0, px.lb_f, py.lb_i0, py.lb_f);
         L0 = t.x;
         const float jxx = t.y, jxy = t.z, jyy = t.w;
         const Dir d = principal_dir(jxx, jxy, jyy);
@@ -968,7 +1113,7 @@ __device__ __attribute__((always_inline)) void guided_pixel(const GuidedArgs &A,
         nxv = d.nx;
         nyv = d.ny;
         has_dir = jxx + jyy > 1e-6f;
-        const float lx = ax.lpos[ox], ly = ay.lpos[oy];
+        const float lx = px.lpos, ly = py.lpos;
         if (has_dir) {
             const float sp = gp.sp_out;
             const float ya = bilinear(A.y, sw, sh, lx - sp * nxv, ly - sp * nyv);
@@ -995,34 +1140,45 @@ __device__ __attribute__((always_inline)) void guided_pixel(const GuidedArgs &A,
         const float phx = fmn(fx, 1.0f - fx), phy = fmn(fy, 1.0f - fy);
         phase = (1.0f - 2.0f * phx) * (1.0f - 2.0f * phy);
         if (gp.ms_on) {
-            const float gate = bilinear_fast(A.ms, cw, ch, ax.cb_i0[ox], ax.cb_f[ox], ay.cb_i0[oy], ay.cb_f[oy]);
+            const float gate = bilinear_fast(A.ms, cw, ch, px.cb_i0, px.cb_f, py.cb_i0, py.cb_f);
             mfade = 1.0f - gp.ms * (1.0f - gate);
         }
     }
 
     // --- pass 1: window luma range + chroma hull (recon.cpp:304-363)
-    const int tx0 = ax.start[ox], ty0 = ay.start[oy];
-    const int SX = SUP ? SUP : ax.sup, SY = SUP ? SUP : ay.sup;  // taps per axis
-    const float *amx = ax.am + (size_t)ox * ax.sup, *amy = ay.am + (size_t)oy * ay.sup;
-    const float *wxp = ax.w + (size_t)ox * ax.sup, *wyp = ay.w + (size_t)oy * ay.sup;
+    const int tx0 = px.start, ty0 = py.start;
+    const int SX = SUP ? SUP : A.ax.sup, SY = SUP ? SUP : A.ay.sup;  // taps per axis
     float max_dl = 0.0f;
     float mnu = 1e30f, mxu = -1e30f, mnv = 1e30f, mxv = -1e30f;
     _Pragma("unroll") for (int j = 0; j < SY; ++j) {
-        if (amy[j] == 0.0f)
+        const bool onj = py.on(j);
+        if (!SUP && !onj)
             continue;
         const size_t row = (size_t)clampi(ty0 + j, 0, ch - 1) * cw;
         _Pragma("unroll") for (int i = 0; i < SX; ++i) {
-            if (amx[i] == 0.0f)
+            const bool on = onj && px.on(i);
+            if (!SUP && !on)
                 continue;
             const float4 q = WIN ? win[(ty0 + j - wy0) * kTWin + (tx0 + i - wx0)]
                                  : A.luv[row + clampi(tx0 + i, 0, cw - 1)];
-            if (guided)
-                max_dl = fmx(max_dl, fabsf(q.x - L0));
+            const float m = fmx(max_dl, fabsf(q.x - L0));
+            max_dl = on ? m : max_dl;
             const float a = q.y, b = q.z;
-            mnu = fmn(mnu, a); mxu = fmx(mxu, a);
-            mnv = fmn(mnv, b); mxv = fmx(mxv, b);
+            const float nmu = fmn(mnu, a), nxu = fmx(mxu, a), nmv = fmn(mnv, b), nxv2 = fmx(mxv, b);
+            mnu = on ? nmu : mnu;
+            mxu = on ? nxu : mxu;
+            mnv = on ? nmv : mnv;
+            mxv = on ? nxv2 : mxv;
         }
+        if (SUP)
+            __builtin_amdgcn_sched_barrier(0);
     }
+    if (!guided)
+        max_dl = 0.0f;
+    // The hull is read only at the end; the empty asm makes the compiler
+    // finish it here rather than keep every window sample live until then.
+    if (SUP)
+        __asm__ volatile("" ::"v"(mnu), "v"(mxu), "v"(mnv), "v"(mxv), "v"(max_dl));
 
     // --- adaptive sigma and fades (recon.cpp:369-417)
     const float t2 = fclamp(min_diff / (0.1f * max_dl + 1e-6f), 0.0f, 1.0f);
@@ -1064,32 +1220,36 @@ __device__ __attribute__((always_inline)) void guided_pixel(const GuidedArgs &A,
         const float sim0 = 1.0f / (1.0f + dl0 * dl0 * inv_s2);
         rescue = gp.rescue * guide_fade * phase * phase * (1.0f - sim0);
         if (rescue > 0.0f) {
-            float mxX = 0, ssX = 0, mxY = 0, ssY = 0;
-            if (!RADIAL)
-            _Pragma("unroll") for (int i = 0; i < SX; ++i)
-                if (amx[i] > 0.0f) {
-                    const float w = wxp[i];
-                    mxX = fmx(mxX, fabsf(w));
-                    ssX += w * w;
-                }
-            if (!RADIAL)
-            _Pragma("unroll") for (int j = 0; j < SY; ++j)
-                if (amy[j] > 0.0f) {
-                    const float w = wyp[j];
-                    mxY = fmx(mxY, fabsf(w));
-                    ssY += w * w;
-                }
             float kappa;
             if (!RADIAL) {
+                float mxX = 0, ssX = 0, mxY = 0, ssY = 0;
+                _Pragma("unroll") for (int i = 0; i < SX; ++i) {
+                    const bool on = px.on(i);
+                    if (!SUP && !on)
+                        continue;
+                    const float w = px.wt(i);
+                    const float m = fmx(mxX, fabsf(w)), s2 = ssX + w * w;
+                    mxX = on ? m : mxX;
+                    ssX = on ? s2 : ssX;
+                }
+                _Pragma("unroll") for (int j = 0; j < SY; ++j) {
+                    const bool on = py.on(j);
+                    if (!SUP && !on)
+                        continue;
+                    const float w = py.wt(j);
+                    const float m = fmx(mxY, fabsf(w)), s2 = ssY + w * w;
+                    mxY = on ? m : mxY;
+                    ssY = on ? s2 : ssY;
+                }
                 kappa = (mxX * mxX * mxY * mxY) / (ssX * ssY + 1e-12f);
             } else {
                 const float *base = radial_at(A.rd, ox, oy);
                 float mx = 0, ss = 0;
                 _Pragma("unroll") for (int j = 0; j < SY; ++j) {
-                    if (amy[j] == 0.0f)
+                    if (!py.on(j))
                         continue;
                     _Pragma("unroll") for (int i = 0; i < SX; ++i) {
-                        if (amx[i] == 0.0f)
+                        if (!px.on(i))
                             continue;
                         const float w = radial_weight(A.rd, base, SX, i, j, tx0, ty0, scx, scy);
                         mx = fmx(mx, fabsf(w));
@@ -1106,43 +1266,32 @@ __device__ __attribute__((always_inline)) void guided_pixel(const GuidedArgs &A,
     const float inv_g2p = inv_g2 / (aniso * aniso);
 
     // --- tap loop (recon.cpp:581-627)
-    const float *rbase = RADIAL ? radial_at(A.rd, ox, oy) : nullptr;
-    double acc_u = 0.0, acc_v = 0.0, wsum = 0.0, sum_abs = 0.0;
-    _Pragma("unroll") for (int j = 0; j < SY; ++j) {
-        if (amy[j] == 0.0f)
-            continue;
-        const size_t row = (size_t)clampi(ty0 + j, 0, ch - 1) * cw;
-        const float wy = wyp[j];
-        const float dy = ((float)(ty0 + j) - scy) * gp.frh;
-        _Pragma("unroll") for (int i = 0; i < SX; ++i) {
-            if (amx[i] == 0.0f)
-                continue;
-            const float4 q = WIN ? win[(ty0 + j - wy0) * kTWin + (tx0 + i - wx0)]
-                                 : A.luv[row + clampi(tx0 + i, 0, cw - 1)];
-            const float wx = RADIAL ? radial_weight(A.rd, rbase, SX, i, j, tx0, ty0, scx, scy) : wxp[i] * wy;
-            float w;
-            if (guided) {
-                const float dx = ((float)(tx0 + i) - scx) * gp.frw;
-                const float dL = q.x - L0;
-                const float dp = dx * nxv + dy * nyv;
-                const float dt = -dx * nyv + dy * nxv;
-                const float sim_raw = recip_ge1(1.0f + dL * dL * inv_s2);
-                const float sim = 1.0f - guide_fade * (1.0f - sim_raw);
-                const float bump = recip_ge1(1.0f + dp * dp * inv_g2 + dt * dt * inv_g2p);
-                w = sim * (wx + rescue * bump);
-            } else {
-                w = wx;
-            }
-            acc_u += (double)w * q.y;
-            acc_v += (double)w * q.z;
-            wsum += w;
-            sum_abs += fabsf(w);
-        }
+    const TapIn ti{L0, nxv, nyv, scx, scy, inv_s2, guide_fade, inv_g2, inv_g2p, rescue, tx0, ty0};
+    // Every active tap's reciprocal arguments are bounded by the window luma
+    // range (max_dl) and the largest projected offset; when both bounds are
+    // below 2^126 for the whole wave the branch-free reciprocal is exact.
+    bool fast = true;
+    if (guided) {
+        const float ysm = 1.0f + max_dl * max_dl * inv_s2;
+        const float dxm = fmx(fabsf((float)tx0 - scx), fabsf((float)(tx0 + SX - 1) - scx)) * gp.frw;
+        const float dym = fmx(fabsf((float)ty0 - scy), fabsf((float)(ty0 + SY - 1) - scy)) * gp.frh;
+        const float b = (dxm + dym) * (fabsf(nxv) + fabsf(nyv));
+        const float ybm = 1.0f + 2.0f * b * b * (inv_g2 + inv_g2p);
+        fast = ysm < 0x1p120f && ybm < 0x1p120f;
     }
+    TapAcc acc;
+    if (!guided)
+        guided_taps<RADIAL, SUP, WIN, true, false>(A, px, py, ox, oy, ti, win, wx0, wy0, acc);
+    else if (__all(fast))
+        guided_taps<RADIAL, SUP, WIN, true, true>(A, px, py, ox, oy, ti, win, wx0, wy0, acc);
+    else  // rare: exact reciprocals, weights from the tables, compact loop
+        guided_taps<RADIAL, 0, WIN, false, true>(A, pix_axis<0>(A.ax, ox), pix_axis<0>(A.ay, oy), ox, oy, ti,
+                                                 win, wx0, wy0, acc);
+    double acc_u = acc.u, acc_v = acc.v, wsum = acc.wsum, sum_abs = acc.sabs;
     // --- temporal taps of TRecon (recon.cpp:642-745)
     if (TEMPORAL && guided && A.tmp.n > 0) {
         const TemporalDev &T = A.tmp;
-        const int lxi = clampi((int)ax.lpos[ox], 0, sw - 1), lyi = clampi((int)ay.lpos[oy], 0, sh - 1);
+        const int lxi = clampi((int)px.lpos, 0, sw - 1), lyi = clampi((int)py.lpos, 0, sh - 1);
         const float frw = gp.frw, frh = gp.frh;
         for (int q = 0; q < T.n; ++q) {
             const NbrDev nb = T.nbrs[q];
@@ -1212,7 +1361,7 @@ __device__ __attribute__((always_inline)) void guided_pixel(const GuidedArgs &A,
         vu = dclamp(vu, (double)(mnu - gp.margin), (double)(mxu + gp.margin));
         vv = dclamp(vv, (double)(mnv - gp.margin), (double)(mxv + gp.margin));
     }
-    const size_t o = (size_t)oy * ax.n + ox;
+    const size_t o = (size_t)oy * A.ax.n + ox;
     const SelectorDev &S = A.sel;
     if (SEL != SEL_NONE) {
         // algo-4 selector weights (recon.cpp:762-811)
@@ -1221,7 +1370,7 @@ __device__ __attribute__((always_inline)) void guided_pixel(const GuidedArgs &A,
         const float w2 = hedgy * (1.0f - diag) * S.strength;
         const float w3 = hedgy * diag * S.strength;
         if (SEL == SEL_FLOAT) {
-            if (sel_compressed(A.row_off, A.indices, ax.n, ay.n)) {
+            if (sel_compressed(A.row_off, A.indices, A.ax.n, A.ay.n)) {
                 const float bu = A.out_u[o], bv = A.out_v[o];
                 if (w2 < 1e-4f && w3 < 1e-4f) {
                     S.du[k] = 0.0f;
@@ -1242,8 +1391,8 @@ __device__ __attribute__((always_inline)) void guided_pixel(const GuidedArgs &A,
         }
         if (w2 >= 1e-4f || w3 >= 1e-4f) {
             const size_t nc = (size_t)cw * ch;
-            const int cx = ax.cb_i0[ox], cy = ay.cb_i0[oy];
-            const float cxf = ax.cb_f[ox], cyf = ay.cb_f[oy];
+            const int cx = px.cb_i0, cy = py.cb_i0;
+            const float cxf = px.cb_f, cyf = py.cb_f;
             const float bu = A.out_u[o], bv = A.out_v[o];
             const float au = bilinear_fast(S.lgf, cw, ch, cx, cxf, cy, cyf);
             const float bbu = bilinear_fast(S.lgf + nc, cw, ch, cx, cxf, cy, cyf);
@@ -1263,7 +1412,7 @@ __device__ __attribute__((always_inline)) void guided_pixel(const GuidedArgs &A,
 }
 
 template <bool RADIAL, bool TEMPORAL, int SEL, bool CEDGE, int SUP>
-__global__ __launch_bounds__(kBlock) void k_guided(GuidedArgs A) {
+__global__ __launch_bounds__(kBlock, SUP == 6 ? (TEMPORAL ? 2 : 4) : 1) void k_guided(GuidedArgs A) {
     const int ow = A.ax.n, oh = A.ay.n;
     const uint32_t total = A.indices ? A.row_off[oh] : (uint32_t)ow * (uint32_t)oh;
     for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < total; k += gridDim.x * kBlock) {
@@ -1279,7 +1428,7 @@ __global__ __launch_bounds__(kBlock) void k_guided(GuidedArgs A) {
 // footprint staged in LDS.  The host enables it when every tile's footprint
 // fits kTWin x kTWin.
 template <bool TEMPORAL, int SEL>
-__global__ __launch_bounds__(kBlock) void k_guided_tiles(GuidedArgs A, int tiles_x, int tiles) {
+__global__ __launch_bounds__(kBlock, TEMPORAL ? 4 : 6) void k_guided_tiles(GuidedArgs A, int tiles_x, int tiles) {
     __shared__ float4 wins[kBlock / 64][kTWin * kTWin];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int t = blockIdx.x * (kBlock / 64) + wave;
@@ -2349,7 +2498,8 @@ hipError_t launch_guided(const float *u, const float *v, int cw, int ch, const f
                          int mask_w, int mask_h, int tiles_fit) {
     GuidedArgs A{u, v, cw, ch, y, sw, sh, yt, luv, lc, ms, ax, ay, rd, gp, row_offsets, indices, out_u, out_v,
                  sel, tmp, indices ? mask_bits : nullptr, mask_w, mask_h, (mask_w + 63) / 64};
-    if (tiles_fit && !gp.radial && !gp.cedge && ax.sup == 6 && ay.sup == 6 && sel.mode != SEL_FLOAT) {
+    if (tiles_fit && !gp.radial && !gp.cedge && ax.sup == 6 && ay.sup == 6 && ax.rec && ay.rec &&
+        sel.mode != SEL_FLOAT) {
         const int tiles_x = (ax.n + kTile - 1) / kTile, tiles = tiles_x * ((ay.n + kTile - 1) / kTile);
         const dim3 g((unsigned)((tiles + kBlock / 64 - 1) / (kBlock / 64)));
         if (tmp.n > 0)
@@ -2365,7 +2515,7 @@ hipError_t launch_guided(const float *u, const float *v, int cw, int ch, const f
     const bool radial = gp.radial != 0, temporal = tmp.n > 0, cedge = gp.cedge != 0;
     // 6 taps per axis (Lanczos 3 / Spline36 at chroma upsampling) get fully
     // unrolled tap loops
-    const int sup6 = (!radial && !cedge && ax.sup == 6 && ay.sup == 6) ? 6 : 0;
+    const int sup6 = (!radial && !cedge && ax.sup == 6 && ay.sup == 6 && ax.rec && ay.rec) ? 6 : 0;
 #define LGCR_GUIDED(R, T, S_, C) \
     if (radial == R && temporal == T && sel.mode == S_ && cedge == C) { \
         if (!R && !C && sup6 == 6) \

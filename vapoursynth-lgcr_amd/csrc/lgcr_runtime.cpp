@@ -139,6 +139,37 @@ struct HostAxis {
     std::vector<float> w, am, wsum, wabs, pos, lpos, cb_f, lb_f;
 };
 
+// Per-output records of a support <= 6 axis (AxisDev::rec); empty otherwise.
+std::vector<float> pack_axis(const HostAxis &h) {
+    std::vector<float> r;
+    if (h.sup > 6)
+        return r;
+    r.assign((size_t)h.n * 16, 0.0f);
+    auto ibits = [](int v) {
+        float f;
+        std::memcpy(&f, &v, sizeof(f));
+        return f;
+    };
+    for (int o = 0; o < h.n; ++o) {
+        float *d = r.data() + (size_t)o * 16;
+        uint32_t bits = 0;
+        for (int t = 0; t < h.sup; ++t) {
+            if (h.am[(size_t)o * h.sup + t] != 0.0f)
+                bits |= 1u << t;
+            d[8 + t] = h.w.empty() ? 0.0f : h.w[(size_t)o * h.sup + t];
+        }
+        d[0] = ibits(h.start[o]);
+        d[1] = h.pos[o];
+        d[2] = h.lpos[o];
+        d[3] = ibits((int)bits);
+        d[4] = ibits(h.lb_i0[o]);
+        d[5] = h.lb_f[o];
+        d[6] = ibits(h.cb_i0[o]);
+        d[7] = h.cb_f[o];
+    }
+    return r;
+}
+
 void bilin_axis(const std::vector<float> &pos, int src_n, std::vector<int> &i0, std::vector<float> &f) {
     const size_t n = pos.size();
     i0.assign(n, 0);
@@ -459,14 +490,14 @@ lgcr_status get_geometry(lgcr_clip *c, double sx, double sy, Geometry **out) {
     footprint(c->cw, c->rw, sx, c->sw, xb, xe);
     footprint(c->ch, c->rh, sy, c->sh, yb, ye);
     Blob blob;
-    size_t o[2][11];
+    size_t o[2][12];
     const HostAxis *axes[2] = {&ax, &ay};
     for (int a = 0; a < 2; ++a) {
         const HostAxis &h = *axes[a];
         o[a][0] = blob.add(h.start); o[a][1] = blob.add(h.w); o[a][2] = blob.add(h.am);
         o[a][3] = blob.add(h.wsum); o[a][4] = blob.add(h.wabs); o[a][5] = blob.add(h.pos);
         o[a][6] = blob.add(h.lpos); o[a][7] = blob.add(h.cb_i0); o[a][8] = blob.add(h.cb_f);
-        o[a][9] = blob.add(h.lb_i0); o[a][10] = blob.add(h.lb_f);
+        o[a][9] = blob.add(h.lb_i0); o[a][10] = blob.add(h.lb_f); o[a][11] = blob.add(pack_axis(h));
     }
     const size_t oxb = blob.add(xb), oxe = blob.add(xe), oyb = blob.add(yb), oye = blob.add(ye);
     std::vector<float> plx(c->cw), ply(c->ch);
@@ -499,7 +530,7 @@ lgcr_status get_geometry(lgcr_clip *c, double sx, double sy, Geometry **out) {
     const bool src_axes = a6 && !c->same;
     HostAxis sax, say;
     HostRadial shr;
-    size_t os[2][11] = {};
+    size_t os[2][12] = {};
     if (src_axes) {
         sax = build_axis(c->sw, c->sw, c->rw, sx, c->p);
         say = build_axis(c->sh, c->sh, c->rh, sy, c->p);
@@ -511,7 +542,7 @@ lgcr_status get_geometry(lgcr_clip *c, double sx, double sy, Geometry **out) {
             os[a][0] = blob.add(h.start); os[a][1] = blob.add(h.w); os[a][2] = blob.add(h.am);
             os[a][3] = blob.add(h.wsum); os[a][4] = blob.add(h.wabs); os[a][5] = blob.add(h.pos);
             os[a][6] = blob.add(h.lpos); os[a][7] = blob.add(h.cb_i0); os[a][8] = blob.add(h.cb_f);
-            os[a][9] = blob.add(h.lb_i0); os[a][10] = blob.add(h.lb_f);
+            os[a][9] = blob.add(h.lb_i0); os[a][10] = blob.add(h.lb_f); os[a][11] = blob.add(pack_axis(h));
         }
     }
     const size_t osrx = blob.add(shr.xphase), osry = blob.add(shr.yphase), osrw = blob.add(shr.w);
@@ -539,6 +570,7 @@ lgcr_status get_geometry(lgcr_clip *c, double sx, double sy, Geometry **out) {
         d.cb_f = reinterpret_cast<const float *>(base + o[a][8]);
         d.lb_i0 = reinterpret_cast<const int *>(base + o[a][9]);
         d.lb_f = reinterpret_cast<const float *>(base + o[a][10]);
+        d.rec = d.sup <= 6 ? reinterpret_cast<const float4 *>(base + o[a][11]) : nullptr;
     }
     g->xb = reinterpret_cast<const int *>(base + oxb);
     g->xe = reinterpret_cast<const int *>(base + oxe);
@@ -585,6 +617,7 @@ lgcr_status get_geometry(lgcr_clip *c, double sx, double sy, Geometry **out) {
             d.cb_f = reinterpret_cast<const float *>(base + os[a][8]);
             d.lb_i0 = reinterpret_cast<const int *>(base + os[a][9]);
             d.lb_f = reinterpret_cast<const float *>(base + os[a][10]);
+            d.rec = d.sup <= 6 ? reinterpret_cast<const float4 *>(base + os[a][11]) : nullptr;
         }
         g->srd = g->rd;
         g->srd.valid = shr.valid ? 1 : 0;
