@@ -21,8 +21,10 @@ __device__ __forceinline__ float fmx(float a, float b) { return a < b ? b : a; }
 __device__ __forceinline__ float fclamp(float v, float lo, float hi) { return v < lo ? lo : (hi < v ? hi : v); }
 __device__ __forceinline__ double dclamp(double v, double lo, double hi) { return v < lo ? lo : (hi < v ? hi : v); }
 
+// Planes hold fewer than 2^31 samples (checked at clip creation), so sample
+// offsets are 32-bit.
 __device__ __forceinline__ float at(const float *p, int w, int h, int x, int y) {
-    return p[(size_t)clampi(y, 0, h - 1) * w + clampi(x, 0, w - 1)];
+    return p[(uint32_t)(clampi(y, 0, h - 1) * w + clampi(x, 0, w - 1))];
 }
 
 // Correctly rounded 1/y for y >= 1.  rcp_nr is the hardware reciprocal
@@ -63,6 +65,18 @@ __device__ __forceinline__ void xcd_block2(unsigned &bx, unsigned &by) {
 __device__ float bilinear(const float *p, int w, int h, double x, double y) {
     const int x0 = (int)floor(x), y0 = (int)floor(y);
     const float fx = (float)(x - x0), fy = (float)(y - y0);
+    const float a = at(p, w, h, x0, y0), b = at(p, w, h, x0 + 1, y0);
+    const float c = at(p, w, h, x0, y0 + 1), d = at(p, w, h, x0 + 1, y0 + 1);
+    return a + (b - a) * fx + (c - a) * fy + (a - b - c + d) * fx * fy;
+}
+
+// The same sample for a coordinate that is a float value: floorf and the
+// float subtraction give exactly the integer part and the rounded fraction
+// the double path computes.
+__device__ __forceinline__ float bilinear_f(const float *p, int w, int h, float x, float y) {
+    const float flx = floorf(x), fly = floorf(y);
+    const int x0 = (int)flx, y0 = (int)fly;
+    const float fx = x - flx, fy = y - fly;
     const float a = at(p, w, h, x0, y0), b = at(p, w, h, x0 + 1, y0);
     const float c = at(p, w, h, x0, y0 + 1), d = at(p, w, h, x0 + 1, y0 + 1);
     return a + (b - a) * fx + (c - a) * fy + (a - b - c + d) * fx * fy;
@@ -133,14 +147,6 @@ __device__ Dir principal_dir(float jxx, float jxy, float jyy) {
     return d;
 }
 
-// 3x3 Sobel / 8 with clamped taps (maps.cpp:9-20).
-__device__ __forceinline__ void sobel(const float *p, int w, int h, int x, int y, float &gx, float &gy) {
-    const float tl = at(p, w, h, x - 1, y - 1), t = at(p, w, h, x, y - 1), tr = at(p, w, h, x + 1, y - 1);
-    const float l = at(p, w, h, x - 1, y), r = at(p, w, h, x + 1, y);
-    const float bl = at(p, w, h, x - 1, y + 1), b = at(p, w, h, x, y + 1), br = at(p, w, h, x + 1, y + 1);
-    gx = (tr + 2.0f * r + br - tl - 2.0f * l - bl) * 0.125f;
-    gy = (bl + 2.0f * b + br - tl - 2.0f * t - tr) * 0.125f;
-}
 
 // ---------------------------------------------------------------------------
 // format conversion (planeToFloat / floatToPlane, lgcr.h:986-1013)
@@ -295,15 +301,17 @@ __global__ __launch_bounds__(kBlock) void k_bp_pack(float *ou, float *ov, int ow
 // ---------------------------------------------------------------------------
 // structure tensor + fused trust seed (sobelTensor3x3, maps.cpp:64-168)
 //
-// One 64x4 output tile per block.  LDS holds the luma tile with a 2-pixel
-// halo, the Sobel gradients (1-pixel halo) evaluated at clamped centres, and
-// the vertical 3-row sums, so every border rule of the reference falls out
-// of the clamped centres.
+// One 64x8 output tile per block of 64x4 threads (each thread steps down
+// the tile by 4 rows).  LDS holds the luma tile with a 2-pixel halo, the
+// Sobel gradients (1-pixel halo) evaluated at clamped centres, and the
+// vertical 3-row sums, so every border rule of the reference falls out of
+// the clamped centres; tiles whose halo lies inside the image skip the
+// clamping.
 // ---------------------------------------------------------------------------
 
-constexpr int TW = 64, TH = 4;
+constexpr int TW = 64, TH = 8, TR = 4;
 
-__global__ __launch_bounds__(TW * TH) void k_sobel_tensor(const float *__restrict__ y, int w, int h,
+__global__ __launch_bounds__(TW * TR) void k_sobel_tensor(const float *__restrict__ y, int w, int h,
                                                          float4 *yt, uint64_t *seed, int w64, float thr) {
     __shared__ float sy[TH + 4][TW + 4];
     __shared__ float gx[TH + 2][TW + 2], gy[TH + 2][TW + 2];
@@ -311,26 +319,34 @@ __global__ __launch_bounds__(TW * TH) void k_sobel_tensor(const float *__restric
     unsigned bx, by;
     xcd_block2(bx, by);
     const int x0 = bx * TW, y0 = by * TH;
-    const int tid = threadIdx.y * TW + threadIdx.x;
-    for (int i = tid; i < (TH + 4) * (TW + 4); i += TW * TH) {
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int tid = ty * TW + tx;
+    for (int i = tid; i < (TH + 4) * (TW + 4); i += TW * TR) {
         const int ly = i / (TW + 4), lx = i % (TW + 4);
-        sy[ly][lx] = y[(size_t)clampi(y0 - 2 + ly, 0, h - 1) * w + clampi(x0 - 2 + lx, 0, w - 1)];
+        sy[ly][lx] = y[(uint32_t)(clampi(y0 - 2 + ly, 0, h - 1) * w + clampi(x0 - 2 + lx, 0, w - 1))];
     }
     __syncthreads();
-    for (int i = tid; i < (TH + 2) * (TW + 2); i += TW * TH) {
+    const bool interior = x0 >= 2 && x0 + TW + 1 < w && y0 >= 2 && y0 + TH + 1 < h;
+    for (int i = tid; i < (TH + 2) * (TW + 2); i += TW * TR) {
         const int ly = i / (TW + 2), lx = i % (TW + 2);
-        const int cx = clampi(x0 - 1 + lx, 0, w - 1), cy = clampi(y0 - 1 + ly, 0, h - 1);
-        // tap coordinates clamped to the image, then made tile-relative
-        const int xl = clampi(cx - 1, 0, w - 1) - (x0 - 2), xc = cx - (x0 - 2), xr = clampi(cx + 1, 0, w - 1) - (x0 - 2);
-        const int yt = clampi(cy - 1, 0, h - 1) - (y0 - 2), yc = cy - (y0 - 2), yb = clampi(cy + 1, 0, h - 1) - (y0 - 2);
-        const float tl = sy[yt][xl], t = sy[yt][xc], tr = sy[yt][xr];
+        int xl, xc, xr, yu, yc, yd;
+        if (interior) {
+            xl = lx; xc = lx + 1; xr = lx + 2;
+            yu = ly; yc = ly + 1; yd = ly + 2;
+        } else {
+            const int cx = clampi(x0 - 1 + lx, 0, w - 1), cy = clampi(y0 - 1 + ly, 0, h - 1);
+            // tap coordinates clamped to the image, then made tile-relative
+            xl = clampi(cx - 1, 0, w - 1) - (x0 - 2); xc = cx - (x0 - 2); xr = clampi(cx + 1, 0, w - 1) - (x0 - 2);
+            yu = clampi(cy - 1, 0, h - 1) - (y0 - 2); yc = cy - (y0 - 2); yd = clampi(cy + 1, 0, h - 1) - (y0 - 2);
+        }
+        const float tl = sy[yu][xl], t = sy[yu][xc], tr = sy[yu][xr];
         const float l = sy[yc][xl], r = sy[yc][xr];
-        const float bl = sy[yb][xl], b = sy[yb][xc], br = sy[yb][xr];
+        const float bl = sy[yd][xl], b = sy[yd][xc], br = sy[yd][xr];
         gx[ly][lx] = (tr + 2.0f * r + br - tl - 2.0f * l - bl) * 0.125f;
         gy[ly][lx] = (bl + 2.0f * b + br - tl - 2.0f * t - tr) * 0.125f;
     }
     __syncthreads();
-    for (int i = tid; i < TH * (TW + 2); i += TW * TH) {
+    for (int i = tid; i < TH * (TW + 2); i += TW * TR) {
         const int ly = i / (TW + 2), lx = i % (TW + 2);
         const float a0 = gx[ly][lx], a1 = gx[ly + 1][lx], a2 = gx[ly + 2][lx];
         const float b0 = gy[ly][lx], b1 = gy[ly + 1][lx], b2 = gy[ly + 2][lx];
@@ -339,36 +355,38 @@ __global__ __launch_bounds__(TW * TH) void k_sobel_tensor(const float *__restric
         vyy[ly][lx] = b0 * b0 + b1 * b1 + b2 * b2;
     }
     __syncthreads();
-    const int x = x0 + threadIdx.x, r = y0 + threadIdx.y;
-    const bool inside = x < w && r < h;
-    const int ly = threadIdx.y;
-    const int c = x - (x0 - 1);  // column of x in the V arrays
-    float oxx = 0.0f, oxy = 0.0f, oyy = 0.0f;
-    if (!inside) {
-    } else if (x == 0) {
-        const int c1 = (w > 1 ? 1 : 0) - (x0 - 1);
-        oxx = (2.0f * vxx[ly][c] + vxx[ly][c1]) / 9.0f;
-        oxy = (2.0f * vxy[ly][c] + vxy[ly][c1]) / 9.0f;
-        oyy = (2.0f * vyy[ly][c] + vyy[ly][c1]) / 9.0f;
-    } else if (x == w - 1) {
-        oxx = (vxx[ly][c - 1] + 2.0f * vxx[ly][c]) / 9.0f;
-        oxy = (vxy[ly][c - 1] + 2.0f * vxy[ly][c]) / 9.0f;
-        oyy = (vyy[ly][c - 1] + 2.0f * vyy[ly][c]) / 9.0f;
-    } else {
-        const float ninth = 1.0f / 9.0f;
-        oxx = ninth * ((vxx[ly][c - 1] + vxx[ly][c]) + vxx[ly][c + 1]);
-        oxy = ninth * ((vxy[ly][c - 1] + vxy[ly][c]) + vxy[ly][c + 1]);
-        oyy = ninth * ((vyy[ly][c - 1] + vyy[ly][c]) + vyy[ly][c + 1]);
+    const int x = x0 + tx;
+    const int c = tx + 1;  // column of x in the V arrays
+    for (int ly = ty; ly < TH; ly += TR) {
+        const int r = y0 + ly;
+        const bool inside = x < w && r < h;
+        float oxx = 0.0f, oxy = 0.0f, oyy = 0.0f;
+        if (!inside) {
+        } else if (x == 0) {
+            const int c1 = (w > 1 ? 1 : 0) - (x0 - 1);
+            oxx = (2.0f * vxx[ly][c] + vxx[ly][c1]) / 9.0f;
+            oxy = (2.0f * vxy[ly][c] + vxy[ly][c1]) / 9.0f;
+            oyy = (2.0f * vyy[ly][c] + vyy[ly][c1]) / 9.0f;
+        } else if (x == w - 1) {
+            oxx = (vxx[ly][c - 1] + 2.0f * vxx[ly][c]) / 9.0f;
+            oxy = (vxy[ly][c - 1] + 2.0f * vxy[ly][c]) / 9.0f;
+            oyy = (vyy[ly][c - 1] + 2.0f * vyy[ly][c]) / 9.0f;
+        } else {
+            const float ninth = 1.0f / 9.0f;
+            oxx = ninth * ((vxx[ly][c - 1] + vxx[ly][c]) + vxx[ly][c + 1]);
+            oxy = ninth * ((vxy[ly][c - 1] + vxy[ly][c]) + vxy[ly][c + 1]);
+            oyy = ninth * ((vyy[ly][c - 1] + vyy[ly][c]) + vyy[ly][c + 1]);
+        }
+        // trust seed (fused in buildGuideMaps, maps.cpp:252-276) as one bit
+        // per pixel: a wave is one 64-pixel tile row, its ballot one word
+        const uint64_t bits = __ballot(inside && oxx + oyy > thr);
+        if (r >= h)
+            break;
+        if (seed && tx == 0)
+            seed[(size_t)r * w64 + bx] = bits;
+        if (inside)
+            yt[(uint32_t)r * (uint32_t)w + (uint32_t)x] = make_float4(sy[ly + 2][tx + 2], oxx, oxy, oyy);
     }
-    // trust seed (fused in buildGuideMaps, maps.cpp:252-276) as one bit per
-    // pixel: a wave is one 64-pixel tile row, its ballot one mask word
-    const uint64_t bits = __ballot(inside && oxx + oyy > thr);
-    if (!inside)
-        return;
-    const size_t o = (size_t)r * w + x;
-    yt[o] = make_float4(sy[ly + 2][threadIdx.x + 2], oxx, oxy, oyy);
-    if (seed && threadIdx.x == 0)
-        seed[(size_t)r * w64 + bx] = bits;
 }
 
 // ---------------------------------------------------------------------------
@@ -475,6 +493,115 @@ __global__ __launch_bounds__(64) void k_dilate_vbits(const uint64_t *in, uint64_
             cnt += __shfl_xor(cnt, o, 64);
         if (threadIdx.x == 0)
             row_count[y] = (uint32_t)cnt;
+    }
+}
+
+// Trust mask without the index list, for the tiled guided pass (it reads
+// the mask bits) at output size == mask size: one thread per mask word in
+// both dilation passes, then one thread per chroma sample for the chroma
+// mask.
+//
+// Horizontal pass for radius < 64: a word and its two neighbours hold every
+// bit within the radius, so the OR of the 2r+1 shifted rows is formed in
+// registers (the same bits row_bits returns).
+__global__ __launch_bounds__(kBlock) void k_hdilate_words(const uint64_t *in, uint64_t *out, int w, int w64, int h,
+                                                          int r, uint32_t *counters) {
+    const int idx = blockIdx.x * kBlock + threadIdx.x;
+    if (idx < 8)
+        counters[idx] = 0;
+    if (idx >= w64 * h)
+        return;
+    const int y = idx / w64, wx = idx - y * w64;
+    const uint64_t *row = in + (uint32_t)(y * w64);
+    const uint64_t lo = wx > 0 ? row[wx - 1] : 0ull, mid = row[wx], hi = wx + 1 < w64 ? row[wx + 1] : 0ull;
+    uint64_t acc = mid;
+    for (int d = 1; d <= r; ++d)
+        acc |= (mid << d) | (lo >> (64 - d)) | (mid >> d) | (hi << (64 - d));
+    const int valid = w - wx * 64;  // pixels of this word inside the row
+    if (valid < 64)
+        acc &= (1ull << valid) - 1ull;
+    out[idx] = acc;
+}
+
+// Per-block population counts go to partial sums that k_finish_counts adds
+// up: one same-address atomic per block would serialise across the XCDs.
+constexpr int kCountBlocks = 256;
+
+// Vertical pass (grid-stride, at most kCountBlocks blocks); also counts the
+// active pixels (active output pixels, the mask having the output size).
+__global__ __launch_bounds__(kBlock) void k_vdilate_words(const uint64_t *in, uint64_t *out, int w64, int h, int r,
+                                                          uint32_t *partial) {
+    __shared__ int scratch[kBlock / 64];
+    int cnt = 0;
+    for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < w64 * h; idx += gridDim.x * kBlock) {
+        const int y = idx / w64, wx = idx - y * w64;
+        const int j0 = max(0, y - r), j1 = min(h - 1, y + r);
+        uint64_t acc = 0;
+        for (int j = j0; j <= j1; ++j)
+            acc |= in[(uint32_t)(j * w64 + wx)];
+        out[idx] = acc;
+        cnt += __popcll(acc);
+    }
+    const int t = block_sum(cnt, scratch);
+    if (threadIdx.x == 0)
+        partial[blockIdx.x] = (uint32_t)t;
+}
+
+// Chroma mask: sample (cx, cy) is marked when an active output pixel has it
+// in its 2x2 bilinear footprint (buildSparseWorkset, maps.cpp:342-360).  The
+// host passes, per chroma column and row, the contiguous range of output
+// columns / rows whose footprint contains it ({first, last}; first > last
+// for none).  Writes every sample (grid-stride, at most kCountBlocks blocks)
+// and counts the marked ones.
+__global__ __launch_bounds__(kBlock) void k_chroma_mark(const uint64_t *mask, int w64, int cw, int ch,
+                                                        const int2 *xr, const int2 *yr, uint8_t *cmask,
+                                                        uint32_t *partial) {
+    __shared__ int scratch[kBlock / 64];
+    int cnt = 0;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < cw * ch; i += gridDim.x * kBlock) {
+        const int cy = i / cw, cx = i - cy * cw;
+        const int2 xs = xr[cx], ys = yr[cy];
+        uint64_t any = 0;
+        if (xs.x <= xs.y) {
+            const int wa = xs.x >> 6, wb = xs.y >> 6;
+            const uint64_t ma = ~0ull << (xs.x & 63);
+            const uint64_t mb = (xs.y & 63) == 63 ? ~0ull : (1ull << ((xs.y & 63) + 1)) - 1ull;
+            for (int y = ys.x; y <= ys.y; ++y) {
+                const uint64_t *row = mask + (uint32_t)(y * w64);
+                if (wa == wb) {
+                    any |= row[wa] & ma & mb;
+                } else {
+                    any |= row[wa] & ma;
+                    for (int wd = wa + 1; wd < wb; ++wd)
+                        any |= row[wd];
+                    any |= row[wb] & mb;
+                }
+            }
+        }
+        const int hit = any != 0ull;
+        cmask[(uint32_t)i] = (uint8_t)hit;
+        cnt += hit;
+    }
+    const int t = block_sum(cnt, scratch);
+    if (threadIdx.x == 0)
+        partial[blockIdx.x] = (uint32_t)t;
+}
+
+// counters[0] = sum of the chroma-mark partials, counters[2] = sum of the
+// vertical-pass partials (one block).
+__global__ __launch_bounds__(kBlock) void k_finish_counts(const uint32_t *pc, int nc, const uint32_t *pa, int na,
+                                                          uint32_t *counters) {
+    __shared__ int scratch[kBlock / 64];
+    int a = 0, b = 0;
+    for (int i = threadIdx.x; i < nc; i += kBlock)
+        a += (int)pc[i];
+    for (int i = threadIdx.x; i < na; i += kBlock)
+        b += (int)pa[i];
+    const int ta = block_sum(a, scratch);
+    const int tb = block_sum(b, scratch);
+    if (threadIdx.x == 0) {
+        counters[0] = (uint32_t)ta;
+        counters[2] = (uint32_t)tb;
     }
 }
 
@@ -594,73 +721,84 @@ __global__ void k_zero_u32(uint32_t *p, int n) {
 // mutual-structure gate (buildMutualGate, maps.cpp:831-948)
 // ---------------------------------------------------------------------------
 
-// Sobel gradients of lc, U and V interleaved per sample: g4 = {lc_x, lc_y,
-// U_x, U_y}, g2 = {V_x, V_y}, so a gate tap is two loads.
-__global__ __launch_bounds__(kBlock) void k_grad3(const float *lc, const float *u, const float *v,
-                                                  int w, int h, float4 *g4, float2 *g2) {
+// Mutual-structure gate with the gradient pass fused in: a block owns a
+// 32x8 tile of chroma samples, stages lc/U/V for the tile plus a 5-sample
+// apron in LDS, computes the three Sobel gradients for the tile plus a
+// 4-sample apron (every sample the 3x3 tensor and the seven projected
+// bilinear samples can touch, as |k n| <= 3 + rounding) and evaluates the
+// gate from there.  LDS entries are addressed by clamped image coordinates,
+// so each lookup returns exactly the value k_grad3 + k_gate read.
+constexpr int kGateTW = 32, kGateTH = 8, kGateHalo = 4;
+constexpr int kGateGW = kGateTW + 2 * kGateHalo, kGateGH = kGateTH + 2 * kGateHalo;
+constexpr int kGateSW = kGateGW + 2, kGateSH = kGateGH + 2;
+
+// 3x3 Sobel / 8 (maps.cpp:9-20) from an LDS tile; the caller passes clamped
+// neighbour indices.
+__device__ __forceinline__ void sobel_lds(const float (*p)[kGateSW], int xm, int xc, int xp, int ym, int yc, int yp,
+                                          float &gx, float &gy) {
+    const float tl = p[ym][xm], t = p[ym][xc], tr = p[ym][xp];
+    const float l = p[yc][xm], r = p[yc][xp];
+    const float bl = p[yp][xm], b = p[yp][xc], br = p[yp][xp];
+    gx = (tr + 2.0f * r + br - tl - 2.0f * l - bl) * 0.125f;
+    gy = (bl + 2.0f * b + br - tl - 2.0f * t - tr) * 0.125f;
+}
+
+__global__ __launch_bounds__(kBlock) void k_gate_fused(const float *__restrict__ lc, const float *__restrict__ u,
+                                                       const float *__restrict__ v, int w, int h,
+                                                       const uint8_t *cmask, const uint32_t *counters,
+                                                       float *gate) {
+    __shared__ float src[3][kGateSH][kGateSW];
+    __shared__ float grd[6][kGateGH][kGateGW];
     unsigned bx, by;
     xcd_block2(bx, by);
-    const int x = bx * kBlock + threadIdx.x, y = by;
-    if (x >= w)
-        return;
-    const size_t i = (size_t)y * w + x;
-    float a, b, c, d, e, f;
-    sobel(lc, w, h, x, y, a, b);
-    sobel(u, w, h, x, y, c, d);
-    sobel(v, w, h, x, y, e, f);
-    g4[i] = make_float4(a, b, c, d);
-    g2[i] = make_float2(e, f);
-}
-
-// SobelRowRing::sampleProjected (maps.cpp:782-791) at bilinearPoint
-// (maps.cpp:175-186).
-// Bilinear sample of the three gradients projected on n (the Y, U and V
-// results of sampleProjected at one point).
-struct Proj3 { float y, u, v; };
-__device__ __forceinline__ Proj3 projected3(const float4 *g4, const float2 *g2, int w, int x0, int x1, int y0,
-                                            int y1, float fx, float fy, float nx, float ny) {
-    const size_t i00 = (size_t)y0 * w + x0, i01 = (size_t)y0 * w + x1, i10 = (size_t)y1 * w + x0,
-                 i11 = (size_t)y1 * w + x1;
-    const float4 A = g4[i00], B = g4[i01], C = g4[i10], D = g4[i11];
-    const float2 A2 = g2[i00], B2 = g2[i01], C2 = g2[i10], D2 = g2[i11];
-    Proj3 r;
-    {
-        const float a = A.x * nx + A.y * ny, b = B.x * nx + B.y * ny;
-        const float c = C.x * nx + C.y * ny, d = D.x * nx + D.y * ny;
-        r.y = a + (b - a) * fx + (c - a) * fy + (a - b - c + d) * fx * fy;
-    }
-    {
-        const float a = A.z * nx + A.w * ny, b = B.z * nx + B.w * ny;
-        const float c = C.z * nx + C.w * ny, d = D.z * nx + D.w * ny;
-        r.u = a + (b - a) * fx + (c - a) * fy + (a - b - c + d) * fx * fy;
-    }
-    {
-        const float a = A2.x * nx + A2.y * ny, b = B2.x * nx + B2.y * ny;
-        const float c = C2.x * nx + C2.y * ny, d = D2.x * nx + D2.y * ny;
-        r.v = a + (b - a) * fx + (c - a) * fy + (a - b - c + d) * fx * fy;
-    }
-    return r;
-}
-
-__global__ __launch_bounds__(kBlock) void k_gate(const float4 *__restrict__ g4, const float2 *__restrict__ g2,
-                                                 int w, int h, const uint8_t *cmask, const uint32_t *counters,
-                                                 float *gate) {
-    const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
-    if (x >= w)
-        return;
+    const int tx0 = (int)bx * kGateTW, ty0 = (int)by * kGateTH;
+    const int gx0 = tx0 - kGateHalo, gy0 = ty0 - kGateHalo, sx0 = gx0 - 1, sy0 = gy0 - 1;
+    const int x = tx0 + (int)(threadIdx.x % kGateTW), y = ty0 + (int)(threadIdx.x / kGateTW);
     const size_t n = (size_t)w * h;
     const bool dense = !cmask || (uint64_t)counters[0] * 2 >= (uint64_t)n;
-    float *out = gate + (size_t)y * w + x;
-    if (!dense && cmask[(size_t)y * w + x] == 0) {
-        *out = 0.0f;
+    const bool inside = x < w && y < h;
+    const bool need = inside && (dense || cmask[(size_t)y * w + x] != 0);
+    if (!__syncthreads_or(need)) {
+        if (inside)
+            gate[(size_t)y * w + x] = 0.0f;
         return;
     }
+    for (int e = threadIdx.x; e < kGateSH * kGateSW; e += kBlock) {
+        const int r = e / kGateSW, c = e - r * kGateSW;
+        const size_t k = (size_t)clampi(sy0 + r, 0, h - 1) * w + clampi(sx0 + c, 0, w - 1);
+        src[0][r][c] = lc[k];
+        src[1][r][c] = u[k];
+        src[2][r][c] = v[k];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kGateGH * kGateGW; e += kBlock) {
+        const int r = e / kGateGW, c = e - r * kGateGW;
+        const int P = clampi(gx0 + c, 0, w - 1), Q = clampi(gy0 + r, 0, h - 1);
+        const int xm = clampi(P - 1, 0, w - 1) - sx0, xc = P - sx0, xp = clampi(P + 1, 0, w - 1) - sx0;
+        const int ym = clampi(Q - 1, 0, h - 1) - sy0, yc = Q - sy0, yp = clampi(Q + 1, 0, h - 1) - sy0;
+        _Pragma("unroll") for (int q = 0; q < 3; ++q) {
+            float gx, gy;
+            sobel_lds(src[q], xm, xc, xp, ym, yc, yp, gx, gy);
+            grd[2 * q][r][c] = gx;
+            grd[2 * q + 1][r][c] = gy;
+        }
+    }
+    __syncthreads();
+    if (!need) {
+        if (inside)
+            gate[(size_t)y * w + x] = 0.0f;
+        return;
+    }
+    float *out = gate + (size_t)y * w + x;
+    // LDS column / row of a clamped image coordinate (bounded for any input)
+    auto cx = [&](int X) { return clampi(clampi(X, 0, w - 1) - gx0, 0, kGateGW - 1); };
+    auto cy = [&](int Y) { return clampi(clampi(Y, 0, h - 1) - gy0, 0, kGateGH - 1); };
     float sxx = 0, sxy = 0, syy = 0;
     for (int dy = -1; dy <= 1; ++dy)
         for (int dx = -1; dx <= 1; ++dx) {
-            const float4 t = g4[(size_t)clampi(y + dy, 0, h - 1) * w + clampi(x + dx, 0, w - 1)];
-            const float a = t.x;
-            const float b = t.y;
+            const int r = cy(y + dy), c = cx(x + dx);
+            const float a = grd[0][r][c];
+            const float b = grd[1][r][c];
             sxx += a * a;
             sxy += a * b;
             syy += b * b;
@@ -676,13 +814,19 @@ __global__ __launch_bounds__(kBlock) void k_gate(const float4 *__restrict__ g4, 
         const double px = (float)x + (float)k * nx;
         const double py = (float)y + (float)k * ny;
         const int rx = (int)floor(px), ry = (int)floor(py);
-        const int x0 = clampi(rx, 0, w - 1), x1 = clampi(rx + 1, 0, w - 1);
-        const int y0 = clampi(ry, 0, h - 1), y1 = clampi(ry + 1, 0, h - 1);
+        const int c0 = cx(rx), c1 = cx(rx + 1), r0 = cy(ry), r1 = cy(ry + 1);
         const float fx = (float)(px - rx), fy = (float)(py - ry);
-        const Proj3 pr = projected3(g4, g2, w, x0, x1, y0, y1, fx, fy, nx, ny);
-        const float gY = fabsf(pr.y);
-        const float au = pr.u;
-        const float av = pr.v;
+        float res[3];
+        _Pragma("unroll") for (int q = 0; q < 3; ++q) {
+            const float a = grd[2 * q][r0][c0] * nx + grd[2 * q + 1][r0][c0] * ny;
+            const float b = grd[2 * q][r0][c1] * nx + grd[2 * q + 1][r0][c1] * ny;
+            const float c = grd[2 * q][r1][c0] * nx + grd[2 * q + 1][r1][c0] * ny;
+            const float dd = grd[2 * q][r1][c1] * nx + grd[2 * q + 1][r1][c1] * ny;
+            res[q] = a + (b - a) * fx + (c - a) * fy + (a - b - c + dd) * fx * fy;
+        }
+        const float gY = fabsf(res[0]);
+        const float au = res[1];
+        const float av = res[2];
         const float gC = sqrtf(au * au + av * av);
         sumY += gY; sqY += gY * gY; momY += (float)k * gY;
         sumC += gC; sqC += gC * gC; momC += (float)k * gC;
@@ -700,12 +844,65 @@ __global__ __launch_bounds__(kBlock) void k_gate(const float4 *__restrict__ g4, 
     *out = (tw * tw * (3.0f - 2.0f * tw)) * (tp * tp * (3.0f - 2.0f * tp));
 }
 
+// Per-output-pixel axis data: first tap, positions, bilinear indices, the
+// tap weights and active-tap flags.  6-tap axes read everything from one
+// 64-byte record (AxisDev::rec) and keep it in registers; other supports
+// read the weights from the tables.
+template <int SUP> struct PixAxis {
+    int start, lb_i0, cb_i0;
+    float pos, lpos, lb_f, cb_f;
+    const float *w, *am;
+    __device__ __forceinline__ bool on(int i) const { return am[i] != 0.0f; }
+    __device__ __forceinline__ float wt(int i) const { return w[i]; }
+};
+template <> struct PixAxis<6> {
+    int start, lb_i0, cb_i0;
+    float pos, lpos, lb_f, cb_f;
+    uint32_t bits;
+    float w[6];
+    __device__ __forceinline__ bool on(int i) const { return (bits >> i) & 1u; }
+    __device__ __forceinline__ float wt(int i) const { return w[i]; }
+};
+
+template <int SUP> __device__ __forceinline__ PixAxis<SUP> pix_axis(const AxisDev &a, int o) {
+    PixAxis<SUP> p;
+    if constexpr (SUP == 6) {
+        const float4 *r = a.rec + 4 * (size_t)o;
+        const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+        p.start = __float_as_int(r0.x);
+        p.pos = r0.y;
+        p.lpos = r0.z;
+        p.bits = __float_as_uint(r0.w);
+        p.lb_i0 = __float_as_int(r1.x);
+        p.lb_f = r1.y;
+        p.cb_i0 = __float_as_int(r1.z);
+        p.cb_f = r1.w;
+        p.w[0] = r2.x; p.w[1] = r2.y; p.w[2] = r2.z; p.w[3] = r2.w;
+        p.w[4] = r3.x; p.w[5] = r3.y;
+    } else {
+        p.start = a.start[o];
+        p.pos = a.pos[o];
+        p.lpos = a.lpos[o];
+        p.lb_i0 = a.lb_i0[o];
+        p.lb_f = a.lb_f[o];
+        p.cb_i0 = a.cb_i0[o];
+        p.cb_f = a.cb_f[o];
+        p.w = a.w + (size_t)o * a.sup;
+        p.am = a.am + (size_t)o * a.sup;
+    }
+    return p;
+}
+
 // ---------------------------------------------------------------------------
 // plain chroma base (plainChromaSeparable, recon.cpp:1151-1461): horizontal
 // pass per source chroma row with 1D hulls, vertical pass per output pixel
 // with the separable normalisation, cancellation fallback and 2D hull clamp.
 // ---------------------------------------------------------------------------
 
+// SUP == 6: the 6 taps are straight-line code with every load issued up
+// front (clamped indices are always in bounds); an inactive tap adds an
+// exact zero and leaves the hull alone, so the sums equal the skipping loop.
+template <int SUP>
 __global__ __launch_bounds__(kBlock) void k_plain_h(const float *__restrict__ u, const float *__restrict__ v,
                                                     int cw, int ch, AxisDev ax, float *hb) {
     unsigned bx, by;
@@ -713,21 +910,42 @@ __global__ __launch_bounds__(kBlock) void k_plain_h(const float *__restrict__ u,
     const int ox = bx * kBlock + threadIdx.x, sy = by;
     if (ox >= ax.n)
         return;
-    const int start = ax.start[ox];
-    const float *wt = ax.w + (size_t)ox * ax.sup;
-    const float *am = ax.am + (size_t)ox * ax.sup;
     const float *su = u + (size_t)sy * cw, *sv = v + (size_t)sy * cw;
     double vu = 0.0, vv = 0.0;
     float mnu = 1e30f, mxu = -1e30f, mnv = 1e30f, mxv = -1e30f;
-    for (int i = 0; i < ax.sup; ++i) {
-        if (am[i] == 0.0f)
-            continue;
-        const int x = clampi(start + i, 0, cw - 1);
-        const float a = su[x], b = sv[x];
-        vu += (double)wt[i] * a;
-        vv += (double)wt[i] * b;
-        mnu = fmn(mnu, a); mxu = fmx(mxu, a);
-        mnv = fmn(mnv, b); mxv = fmx(mxv, b);
+    if constexpr (SUP == 6) {
+        const PixAxis<6> px = pix_axis<6>(ax, ox);
+        float a[6], b[6];
+        _Pragma("unroll") for (int i = 0; i < 6; ++i) {
+            const int x = clampi(px.start + i, 0, cw - 1);
+            a[i] = su[x];
+            b[i] = sv[x];
+        }
+        _Pragma("unroll") for (int i = 0; i < 6; ++i) {
+            const bool on = px.on(i);
+            const float w = on ? px.w[i] : 0.0f;
+            vu += (double)w * a[i];
+            vv += (double)w * b[i];
+            const float nmu = fmn(mnu, a[i]), nxu = fmx(mxu, a[i]), nmv = fmn(mnv, b[i]), nxv = fmx(mxv, b[i]);
+            mnu = on ? nmu : mnu;
+            mxu = on ? nxu : mxu;
+            mnv = on ? nmv : mnv;
+            mxv = on ? nxv : mxv;
+        }
+    } else {
+        const int start = ax.start[ox];
+        const float *wt = ax.w + (size_t)ox * ax.sup;
+        const float *am = ax.am + (size_t)ox * ax.sup;
+        for (int i = 0; i < ax.sup; ++i) {
+            if (am[i] == 0.0f)
+                continue;
+            const int x = clampi(start + i, 0, cw - 1);
+            const float a = su[x], b = sv[x];
+            vu += (double)wt[i] * a;
+            vv += (double)wt[i] * b;
+            mnu = fmn(mnu, a); mxu = fmx(mxu, a);
+            mnv = fmn(mnv, b); mxv = fmx(mxv, b);
+        }
     }
     // interleaved: {vu, vv} then {min u, max u, min v, max v}
     const size_t n = (size_t)ax.n * ch, k = (size_t)sy * ax.n + ox;
@@ -735,6 +953,7 @@ __global__ __launch_bounds__(kBlock) void k_plain_h(const float *__restrict__ u,
     reinterpret_cast<float4 *>(hb + ((2 * n + 3) & ~(size_t)3))[k] = make_float4(mnu, mxu, mnv, mxv);
 }
 
+template <int SUP>
 __global__ __launch_bounds__(kBlock) void k_plain_v(const float *__restrict__ u, const float *__restrict__ v,
                                                     int cw, int ch, AxisDev ax, AxisDev ay,
                                                     const float *__restrict__ hb, float *out_u,
@@ -746,21 +965,45 @@ __global__ __launch_bounds__(kBlock) void k_plain_v(const float *__restrict__ u,
         return;
     const int ow = ax.n;
     const size_t n = (size_t)ow * ch;
+    const float2 *hv = reinterpret_cast<const float2 *>(hb);
+    const float4 *hh = reinterpret_cast<const float4 *>(hb + ((2 * n + 3) & ~(size_t)3));
     const int start = ay.start[oy];
     const float *wt = ay.w + (size_t)oy * ay.sup;
     const float *am = ay.am + (size_t)oy * ay.sup;
     double vu = 0.0, vv = 0.0;
     float mnu = 1e30f, mxu = -1e30f, mnv = 1e30f, mxv = -1e30f;
-    for (int j = 0; j < ay.sup; ++j) {
-        if (am[j] == 0.0f)
-            continue;
-        const size_t k = (size_t)clampi(start + j, 0, ch - 1) * ow + ox;
-        const float2 val = reinterpret_cast<const float2 *>(hb)[k];
-        const float4 hull = reinterpret_cast<const float4 *>(hb + ((2 * n + 3) & ~(size_t)3))[k];
-        vu += (double)wt[j] * val.x;
-        vv += (double)wt[j] * val.y;
-        mnu = fmn(mnu, hull.x); mxu = fmx(mxu, hull.y);
-        mnv = fmn(mnv, hull.z); mxv = fmx(mxv, hull.w);
+    if constexpr (SUP == 6) {
+        float2 val[6];
+        float4 hull[6];
+        _Pragma("unroll") for (int j = 0; j < 6; ++j) {
+            const size_t k = (size_t)clampi(start + j, 0, ch - 1) * ow + ox;
+            val[j] = hv[k];
+            hull[j] = hh[k];
+        }
+        _Pragma("unroll") for (int j = 0; j < 6; ++j) {
+            const bool on = am[j] != 0.0f;   // uniform per block row
+            const float w = on ? wt[j] : 0.0f;
+            vu += (double)w * val[j].x;
+            vv += (double)w * val[j].y;
+            const float nmu = fmn(mnu, hull[j].x), nxu = fmx(mxu, hull[j].y);
+            const float nmv = fmn(mnv, hull[j].z), nxv = fmx(mxv, hull[j].w);
+            mnu = on ? nmu : mnu;
+            mxu = on ? nxu : mxu;
+            mnv = on ? nmv : mnv;
+            mxv = on ? nxv : mxv;
+        }
+    } else {
+        for (int j = 0; j < ay.sup; ++j) {
+            if (am[j] == 0.0f)
+                continue;
+            const size_t k = (size_t)clampi(start + j, 0, ch - 1) * ow + ox;
+            const float2 val = hv[k];
+            const float4 hull = hh[k];
+            vu += (double)wt[j] * val.x;
+            vv += (double)wt[j] * val.y;
+            mnu = fmn(mnu, hull.x); mxu = fmx(mxu, hull.y);
+            mnv = fmn(mnv, hull.z); mxv = fmx(mxv, hull.w);
+        }
     }
     const double wsum = (double)ax.wsum[ox] * ay.wsum[oy];
     const double asum = (double)ax.wabs[ox] * ay.wabs[oy];
@@ -778,6 +1021,99 @@ __global__ __launch_bounds__(kBlock) void k_plain_v(const float *__restrict__ u,
     const size_t o = (size_t)oy * ow + ox;
     out_u[o] = (float)vu;
     out_v[o] = (float)vv;
+}
+
+// Both passes in one kernel for 6-tap axes: a block owns 64 output columns
+// x 16 output rows, computes the horizontal pass of the source rows those
+// outputs read into LDS (each element exactly as k_plain_h<6> computes it)
+// and runs the vertical pass from there.  The host enables it when every
+// tile reads at most kPlainRows source rows.
+constexpr int kPlainCols = 64, kPlainTileRows = 16, kPlainRows = 24;
+
+__global__ __launch_bounds__(kBlock) void k_plain_fused(const float *__restrict__ u, const float *__restrict__ v,
+                                                        int cw, int ch, AxisDev ax, AxisDev ay, float *out_u,
+                                                        float *out_v, int clamp_hull, float margin) {
+    __shared__ float2 sval[kPlainRows][kPlainCols];
+    __shared__ float4 shull[kPlainRows][kPlainCols];
+    unsigned bx, by;
+    xcd_block2(bx, by);
+    const int c = threadIdx.x & (kPlainCols - 1), g = threadIdx.x / kPlainCols;
+    const int ow = ax.n, oh = ay.n;
+    const int ox = (int)bx * kPlainCols + c, oy0 = (int)by * kPlainTileRows;
+    const int r0 = ay.start[oy0];
+    const int nr = ay.start[min(oy0 + kPlainTileRows, oh) - 1] + 6 - r0;
+    // columns past the right edge repeat the last one in the first pass
+    const PixAxis<6> px = pix_axis<6>(ax, min(ox, ow - 1));
+    int xs[6];
+    _Pragma("unroll") for (int i = 0; i < 6; ++i) xs[i] = clampi(px.start + i, 0, cw - 1);
+    for (int t = g; t < nr; t += kBlock / kPlainCols) {
+        const size_t row = (size_t)clampi(r0 + t, 0, ch - 1) * cw;
+        float a[6], b[6];
+        _Pragma("unroll") for (int i = 0; i < 6; ++i) {
+            a[i] = u[row + xs[i]];
+            b[i] = v[row + xs[i]];
+        }
+        double vu = 0.0, vv = 0.0;
+        float mnu = 1e30f, mxu = -1e30f, mnv = 1e30f, mxv = -1e30f;
+        _Pragma("unroll") for (int i = 0; i < 6; ++i) {
+            const bool on = px.on(i);
+            const float w = on ? px.w[i] : 0.0f;
+            vu += (double)w * a[i];
+            vv += (double)w * b[i];
+            const float nmu = fmn(mnu, a[i]), nxu = fmx(mxu, a[i]), nmv = fmn(mnv, b[i]), nxv = fmx(mxv, b[i]);
+            mnu = on ? nmu : mnu;
+            mxu = on ? nxu : mxu;
+            mnv = on ? nmv : mnv;
+            mxv = on ? nxv : mxv;
+        }
+        sval[t][c] = make_float2((float)vu, (float)vv);
+        shull[t][c] = make_float4(mnu, mxu, mnv, mxv);
+    }
+    __syncthreads();
+    if (ox >= ow)
+        return;
+    const float axw = ax.wsum[ox], axa = ax.wabs[ox], axp = ax.pos[ox];
+    constexpr int kPer = kPlainTileRows / (kBlock / kPlainCols);
+    for (int k = 0; k < kPer; ++k) {
+        const int oy = oy0 + g * kPer + k;
+        if (oy >= oh)
+            break;
+        const int t0 = ay.start[oy] - r0;
+        const float *wt = ay.w + (size_t)oy * 6;
+        const float *am = ay.am + (size_t)oy * 6;
+        double vu = 0.0, vv = 0.0;
+        float mnu = 1e30f, mxu = -1e30f, mnv = 1e30f, mxv = -1e30f;
+        _Pragma("unroll") for (int j = 0; j < 6; ++j) {
+            const bool on = am[j] != 0.0f;
+            const float w = on ? wt[j] : 0.0f;
+            const float2 val = sval[t0 + j][c];
+            const float4 hull = shull[t0 + j][c];
+            vu += (double)w * val.x;
+            vv += (double)w * val.y;
+            const float nmu = fmn(mnu, hull.x), nxu = fmx(mxu, hull.y);
+            const float nmv = fmn(mnv, hull.z), nxv = fmx(mxv, hull.w);
+            mnu = on ? nmu : mnu;
+            mxu = on ? nxu : mxu;
+            mnv = on ? nmv : mnv;
+            mxv = on ? nxv : mxv;
+        }
+        const double wsum = (double)axw * ay.wsum[oy];
+        const double asum = (double)axa * ay.wabs[oy];
+        if (wsum > 1e-6 && wsum >= 0.05 * asum) {
+            vu /= wsum;
+            vv /= wsum;
+        } else {
+            vu = bilinear(u, cw, ch, axp, ay.pos[oy]);
+            vv = bilinear(v, cw, ch, axp, ay.pos[oy]);
+        }
+        if (clamp_hull) {
+            vu = dclamp(vu, (double)(mnu - margin), (double)(mxu + margin));
+            vv = dclamp(vv, (double)(mnv - margin), (double)(mxv + margin));
+        }
+        const size_t o = (size_t)oy * ow + ox;
+        out_u[o] = (float)vu;
+        out_v[o] = (float)vv;
+    }
 }
 
 // plainChromaBilinear (recon.cpp:1031-1149)
@@ -943,56 +1279,7 @@ __device__ __forceinline__ bool sel_compressed(const uint32_t *row_off, const ui
 // WIN: the pass-1 and tap loops read the chroma guide from an LDS window of
 // kTWin x kTWin samples whose origin is (wx0, wy0), filled with luv at
 // clamped coordinates (so it holds exactly what the global reads return).
-constexpr int kTile = 8, kTWin = 16;
-
-// Per-output-pixel axis data: first tap, positions, bilinear indices, the
-// tap weights and active-tap flags.  6-tap axes read everything from one
-// 64-byte record (AxisDev::rec) and keep it in registers; other supports
-// read the weights from the tables.
-template <int SUP> struct PixAxis {
-    int start, lb_i0, cb_i0;
-    float pos, lpos, lb_f, cb_f;
-    const float *w, *am;
-    __device__ __forceinline__ bool on(int i) const { return am[i] != 0.0f; }
-    __device__ __forceinline__ float wt(int i) const { return w[i]; }
-};
-template <> struct PixAxis<6> {
-    int start, lb_i0, cb_i0;
-    float pos, lpos, lb_f, cb_f;
-    uint32_t bits;
-    float w[6];
-    __device__ __forceinline__ bool on(int i) const { return (bits >> i) & 1u; }
-    __device__ __forceinline__ float wt(int i) const { return w[i]; }
-};
-
-template <int SUP> __device__ __forceinline__ PixAxis<SUP> pix_axis(const AxisDev &a, int o) {
-    PixAxis<SUP> p;
-    if constexpr (SUP == 6) {
-        const float4 *r = a.rec + 4 * (size_t)o;
-        const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
-        p.start = __float_as_int(r0.x);
-        p.pos = r0.y;
-        p.lpos = r0.z;
-        p.bits = __float_as_uint(r0.w);
-        p.lb_i0 = __float_as_int(r1.x);
-        p.lb_f = r1.y;
-        p.cb_i0 = __float_as_int(r1.z);
-        p.cb_f = r1.w;
-        p.w[0] = r2.x; p.w[1] = r2.y; p.w[2] = r2.z; p.w[3] = r2.w;
-        p.w[4] = r3.x; p.w[5] = r3.y;
-    } else {
-        p.start = a.start[o];
-        p.pos = a.pos[o];
-        p.lpos = a.lpos[o];
-        p.lb_i0 = a.lb_i0[o];
-        p.lb_f = a.lb_f[o];
-        p.cb_i0 = a.cb_i0[o];
-        p.cb_f = a.cb_f[o];
-        p.w = a.w + (size_t)o * a.sup;
-        p.am = a.am + (size_t)o * a.sup;
-    }
-    return p;
-}
+constexpr int kTile = 8, kTWin = 16, kTStride = kTWin + 1;  // padded rows spread LDS banks
 
 // Per-pixel inputs of the weighted tap sum.
 struct TapIn {
@@ -1019,7 +1306,7 @@ __device__ __forceinline__ void guided_taps(const GuidedArgs &A, const PixAxis<S
     // Window offset of the first tap.  The empty asm hides its value so the
     // compiler reloads the window here instead of keeping all 36 samples of
     // the first pass live in registers.
-    int woff = (t.ty0 - wy0) * kTWin + (t.tx0 - wx0);
+    int woff = (t.ty0 - wy0) * kTStride + (t.tx0 - wx0);
     int goff = 0;  // the same for the global chroma guide
     if (WIN)
         __asm__ volatile("" : "+v"(woff));
@@ -1052,7 +1339,7 @@ __device__ __forceinline__ void guided_taps(const GuidedArgs &A, const PixAxis<S
             const bool on = onj && qx.on(i);
             if (!SUP && !on)
                 continue;
-            const float4 q = WIN ? win[woff + j * kTWin + i] : A.luv[row + goff + clampi(t.tx0 + i, 0, cw - 1)];
+            const float4 q = WIN ? win[woff + j * kTStride + i] : A.luv[row + goff + clampi(t.tx0 + i, 0, cw - 1)];
             const float wx = RADIAL ? radial_weight(A.rd, rbase, SX, i, j, t.tx0, t.ty0, t.scx, t.scy)
                                     : qx.wt(i) * wy;
             float w;
@@ -1116,16 +1403,16 @@ __device__ __attribute__((always_inline)) void guided_pixel(const GuidedArgs &A,
         const float lx = px.lpos, ly = py.lpos;
         if (has_dir) {
             const float sp = gp.sp_out;
-            const float ya = bilinear(A.y, sw, sh, lx - sp * nxv, ly - sp * nyv);
-            const float yb = bilinear(A.y, sw, sh, lx + sp * nxv, ly + sp * nyv);
+            const float ya = bilinear_f(A.y, sw, sh, lx - sp * nxv, ly - sp * nyv);
+            const float yb = bilinear_f(A.y, sw, sh, lx + sp * nxv, ly + sp * nyv);
             min_diff = fmn(fabsf(ya - L0), fabsf(yb - L0));
             if (gp.ridge) {
                 const float dl = 1.5f * sp;
                 const float dl2 = 2.0f * dl;
-                const float ym2 = bilinear(A.y, sw, sh, lx - dl2 * nxv, ly - dl2 * nyv);
-                const float ym1 = bilinear(A.y, sw, sh, lx - dl * nxv, ly - dl * nyv);
-                const float yp1 = bilinear(A.y, sw, sh, lx + dl * nxv, ly + dl * nyv);
-                const float yp2 = bilinear(A.y, sw, sh, lx + dl2 * nxv, ly + dl2 * nyv);
+                const float ym2 = bilinear_f(A.y, sw, sh, lx - dl2 * nxv, ly - dl2 * nyv);
+                const float ym1 = bilinear_f(A.y, sw, sh, lx - dl * nxv, ly - dl * nyv);
+                const float yp1 = bilinear_f(A.y, sw, sh, lx + dl * nxv, ly + dl * nyv);
+                const float yp2 = bilinear_f(A.y, sw, sh, lx + dl2 * nxv, ly + dl2 * nyv);
                 const float df[4] = {ym1 - ym2, L0 - ym1, yp1 - L0, yp2 - yp1};
                 for (int q = 0; q < 3; ++q) {
                     const float a = df[q], b = df[q + 1];
@@ -1159,7 +1446,7 @@ __device__ __attribute__((always_inline)) void guided_pixel(const GuidedArgs &A,
             const bool on = onj && px.on(i);
             if (!SUP && !on)
                 continue;
-            const float4 q = WIN ? win[(ty0 + j - wy0) * kTWin + (tx0 + i - wx0)]
+            const float4 q = WIN ? win[(ty0 + j - wy0) * kTStride + (tx0 + i - wx0)]
                                  : A.luv[row + clampi(tx0 + i, 0, cw - 1)];
             const float m = fmx(max_dl, fabsf(q.x - L0));
             max_dl = on ? m : max_dl;
@@ -1429,8 +1716,10 @@ __global__ __launch_bounds__(kBlock, SUP == 6 ? (TEMPORAL ? 2 : 4) : 1) void k_g
 // fits kTWin x kTWin.
 template <bool TEMPORAL, int SEL>
 __global__ __launch_bounds__(kBlock, TEMPORAL ? 4 : 6) void k_guided_tiles(GuidedArgs A, int tiles_x, int tiles) {
-    __shared__ float4 wins[kBlock / 64][kTWin * kTWin];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __shared__ float4 wins[kBlock / 64][kTWin * kTStride];
+    const int lane = threadIdx.x & 63;
+    // wave-uniform tile index, held in scalar registers
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int t = blockIdx.x * (kBlock / 64) + wave;
     if (t >= tiles)
         return;  // whole wave
@@ -1439,23 +1728,26 @@ __global__ __launch_bounds__(kBlock, TEMPORAL ? 4 : 6) void k_guided_tiles(Guide
     const int ox = tx * kTile + (lane & (kTile - 1)), oy = ty * kTile + (lane / kTile);
     bool act = ox < ow && oy < oh;
     if (act && A.mbits) {
-        const int mx = min(A.mw - 1, (int)((int64_t)ox * A.mw / ow));
-        const int my = min(A.mh - 1, (int)((int64_t)oy * A.mh / oh));
-        act = (A.mbits[(size_t)my * A.mw64 + (mx >> 6)] >> (mx & 63)) & 1ull;
+        // nearest mask sample (buildSparseWorkset, maps.cpp:342-360); the
+        // mask has the output size unless the clip resizes
+        const bool same = A.mw == ow && A.mh == oh;
+        const int mx = same ? ox : min(A.mw - 1, (int)((int64_t)ox * A.mw / ow));
+        const int my = same ? oy : min(A.mh - 1, (int)((int64_t)oy * A.mh / oh));
+        act = (A.mbits[(uint32_t)(my * A.mw64 + (mx >> 6))] >> (mx & 63)) & 1ull;
     }
     if (__ballot(act) == 0)
         return;  // no active pixel in the tile (uniform)
-    int x_lo = INT_MAX, x_hi = INT_MIN, y_lo = INT_MAX, y_hi = INT_MIN;
+    int x_lo = INT_MAX, y_lo = INT_MAX;
     for (int c = 0; c < kTile; ++c) {
-        const int sx = A.ax.start[min(tx * kTile + c, ow - 1)], sy = A.ay.start[min(ty * kTile + c, oh - 1)];
-        x_lo = min(x_lo, sx); x_hi = max(x_hi, sx);
-        y_lo = min(y_lo, sy); y_hi = max(y_hi, sy);
+        x_lo = min(x_lo, A.ax.start[min(tx * kTile + c, ow - 1)]);
+        y_lo = min(y_lo, A.ay.start[min(ty * kTile + c, oh - 1)]);
     }
-    const int nc = x_hi - x_lo + 6, nr = y_hi - y_lo + 6;
+    // the whole kTWin x kTWin window from (x_lo, y_lo); the host checked that
+    // every tile's taps lie inside it (guided_tiles_fit)
     float4 *wb = wins[wave];
-    for (int e = lane; e < nc * nr; e += 64) {
-        const int r = e / nc, c = e - r * nc;
-        wb[r * kTWin + c] = A.luv[(size_t)clampi(y_lo + r, 0, ch - 1) * cw + clampi(x_lo + c, 0, cw - 1)];
+    _Pragma("unroll") for (int e = lane; e < kTWin * kTWin; e += 64) {
+        const int r = e / kTWin, c = e % kTWin;
+        wb[r * kTStride + c] = A.luv[(uint32_t)(clampi(y_lo + r, 0, ch - 1) * cw + clampi(x_lo + c, 0, cw - 1))];
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
@@ -2429,7 +2721,7 @@ hipError_t launch_from_float(const RawOut *planes, int count, int type, hipStrea
 hipError_t launch_sobel_tensor(const float *y, int w, int h, float4 *yt, uint64_t *seed, float threshold,
                                hipStream_t s) {
     const dim3 g((unsigned)((w + TW - 1) / TW), (unsigned)((h + TH - 1) / TH));
-    hipLaunchKernelGGL(k_sobel_tensor, g, dim3(TW, TH), 0, s, y, w, h, yt, seed, (w + 63) / 64, threshold);
+    hipLaunchKernelGGL(k_sobel_tensor, g, dim3(TW, TR), 0, s, y, w, h, yt, seed, (w + 63) / 64, threshold);
     return hipGetLastError();
 }
 
@@ -2463,23 +2755,53 @@ hipError_t launch_trust_workset(const uint64_t *seed, uint64_t *tmp, uint64_t *m
     return hipGetLastError();
 }
 
-hipError_t launch_mutual_gate(const float *lc, const float *u, const float *v, int w, int h, float *grad,
+hipError_t launch_trust_mask_tiles(const uint64_t *seed, uint64_t *tmp, uint64_t *mask, int w, int h, int radius,
+                                   uint8_t *chroma_mask, int cw, int ch, const int2 *xr, const int2 *yr,
+                                   uint32_t *counters, hipStream_t s) {
+    const int w64 = (w + 63) / 64;
+    const unsigned blocks = (unsigned)((w64 * h + kBlock - 1) / kBlock);
+    const int nv = (int)std::min<unsigned>(blocks, kCountBlocks);
+    const int nm = (int)std::min<long>(((long)cw * ch + kBlock - 1) / kBlock, kCountBlocks);
+    uint32_t *pa = counters + 8, *pc = counters + 8 + kCountBlocks;
+    hipLaunchKernelGGL(k_hdilate_words, dim3(blocks), dim3(kBlock), 0, s, seed, tmp, w, w64, h, radius, counters);
+    hipLaunchKernelGGL(k_vdilate_words, dim3((unsigned)nv), dim3(kBlock), 0, s, tmp, mask, w64, h, radius, pa);
+    hipLaunchKernelGGL(k_chroma_mark, dim3((unsigned)nm), dim3(kBlock), 0, s, mask, w64, cw, ch, xr, yr, chroma_mask,
+                       pc);
+    hipLaunchKernelGGL(k_finish_counts, dim3(1), dim3(kBlock), 0, s, pc, nm, pa, nv, counters);
+    return hipGetLastError();
+}
+
+bool guided_tiled(int tiles_fit, int radial, int cedge, const AxisDev &ax, const AxisDev &ay, int sel_mode) {
+    return tiles_fit && !radial && !cedge && ax.sup == 6 && ay.sup == 6 && ax.rec && ay.rec && sel_mode != SEL_FLOAT;
+}
+
+hipError_t launch_mutual_gate(const float *lc, const float *u, const float *v, int w, int h,
                               const uint8_t *chroma_mask, const uint32_t *counters, float *gate,
                               hipStream_t s) {
-    const size_t n = (size_t)w * h;
-    float4 *g4 = reinterpret_cast<float4 *>(grad);
-    float2 *g2 = reinterpret_cast<float2 *>(grad + 4 * n);  // 16-byte aligned: 4n floats
-    hipLaunchKernelGGL(k_grad3, rows_grid(w, h), dim3(kBlock), 0, s, lc, u, v, w, h, g4, g2);
-    hipLaunchKernelGGL(k_gate, rows_grid(w, h), dim3(kBlock), 0, s, g4, g2, w, h, chroma_mask, counters, gate);
+    const dim3 g((unsigned)((w + kGateTW - 1) / kGateTW), (unsigned)((h + kGateTH - 1) / kGateTH));
+    hipLaunchKernelGGL(k_gate_fused, g, dim3(kBlock), 0, s, lc, u, v, w, h, chroma_mask, counters, gate);
     return hipGetLastError();
 }
 
 hipError_t launch_plain_separable(const float *u, const float *v, int cw, int ch, const AxisDev &ax,
                                   const AxisDev &ay, float *hbuf, float *out_u, float *out_v, int clamp_hull,
                                   float margin, hipStream_t s) {
-    hipLaunchKernelGGL(k_plain_h, rows_grid(ax.n, ch), dim3(kBlock), 0, s, u, v, cw, ch, ax, hbuf);
-    hipLaunchKernelGGL(k_plain_v, rows_grid(ax.n, ay.n), dim3(kBlock), 0, s, u, v, cw, ch, ax, ay, hbuf, out_u,
-                       out_v, clamp_hull, margin);
+    if (ax.sup == 6 && ay.sup == 6 && ax.rec && ay.tile_rows16 > 0 && ay.tile_rows16 <= kPlainRows) {
+        const dim3 g((unsigned)((ax.n + kPlainCols - 1) / kPlainCols), (unsigned)((ay.n + kPlainTileRows - 1) / kPlainTileRows));
+        hipLaunchKernelGGL(k_plain_fused, g, dim3(kBlock), 0, s, u, v, cw, ch, ax, ay, out_u, out_v, clamp_hull,
+                           margin);
+        return hipGetLastError();
+    }
+    if (ax.sup == 6 && ax.rec)
+        hipLaunchKernelGGL(k_plain_h<6>, rows_grid(ax.n, ch), dim3(kBlock), 0, s, u, v, cw, ch, ax, hbuf);
+    else
+        hipLaunchKernelGGL(k_plain_h<0>, rows_grid(ax.n, ch), dim3(kBlock), 0, s, u, v, cw, ch, ax, hbuf);
+    if (ay.sup == 6)
+        hipLaunchKernelGGL(k_plain_v<6>, rows_grid(ax.n, ay.n), dim3(kBlock), 0, s, u, v, cw, ch, ax, ay, hbuf,
+                           out_u, out_v, clamp_hull, margin);
+    else
+        hipLaunchKernelGGL(k_plain_v<0>, rows_grid(ax.n, ay.n), dim3(kBlock), 0, s, u, v, cw, ch, ax, ay, hbuf,
+                           out_u, out_v, clamp_hull, margin);
     return hipGetLastError();
 }
 
@@ -2498,8 +2820,7 @@ hipError_t launch_guided(const float *u, const float *v, int cw, int ch, const f
                          int mask_w, int mask_h, int tiles_fit) {
     GuidedArgs A{u, v, cw, ch, y, sw, sh, yt, luv, lc, ms, ax, ay, rd, gp, row_offsets, indices, out_u, out_v,
                  sel, tmp, indices ? mask_bits : nullptr, mask_w, mask_h, (mask_w + 63) / 64};
-    if (tiles_fit && !gp.radial && !gp.cedge && ax.sup == 6 && ay.sup == 6 && ax.rec && ay.rec &&
-        sel.mode != SEL_FLOAT) {
+    if (guided_tiled(tiles_fit, gp.radial, gp.cedge, ax, ay, sel.mode)) {
         const int tiles_x = (ax.n + kTile - 1) / kTile, tiles = tiles_x * ((ay.n + kTile - 1) / kTile);
         const dim3 g((unsigned)((tiles + kBlock / 64 - 1) / (kBlock / 64)));
         if (tmp.n > 0)

@@ -5,6 +5,7 @@
 #include "lgcr_internal.h"
 
 #include <algorithm>
+#include <climits>
 #include <functional>
 #include <atomic>
 #include <cmath>
@@ -138,6 +139,57 @@ struct HostAxis {
     std::vector<int> start, cb_i0, lb_i0;
     std::vector<float> w, am, wsum, wabs, pos, lpos, cb_f, lb_f;
 };
+
+// Source rows read by the widest group of 16 consecutive outputs
+// (AxisDev::tile_rows16); INT_MAX when the first taps are not monotone.
+int tile_rows16(const HostAxis &h) {
+    for (int o = 1; o < h.n; ++o)
+        if (h.start[o] < h.start[o - 1])
+            return INT_MAX;
+    int m = 0;
+    for (int o = 0; o < h.n; o += 16) {
+        const int last = std::min(o + 16, h.n) - 1;
+        m = std::max(m, h.start[last] + h.sup - h.start[o]);
+    }
+    return m;
+}
+
+// For each chroma sample along an axis, the range of outputs whose bilinear
+// footprint {cb_i0, min(cb_i0 + 1, n - 1)} contains it, as {first, last}
+// ({1, 0} for none).  ok is cleared when a range is not contiguous.
+std::vector<int> footprint_ranges(const HostAxis &h, int n, bool &ok) {
+    std::vector<int> r(2 * size_t(n));
+    for (int k = 0; k < n; ++k) {
+        r[2 * k] = INT_MAX;
+        r[2 * k + 1] = INT_MIN;
+    }
+    auto touch = [&](int k, int o) {
+        r[2 * k] = std::min(r[2 * k], o);
+        r[2 * k + 1] = std::max(r[2 * k + 1], o);
+    };
+    for (int o = 0; o < h.n; ++o) {
+        const int c0 = h.cb_i0[o], c1 = std::min(c0 + 1, n - 1);
+        if (c0 < 0 || c0 >= n) {
+            ok = false;
+            return r;
+        }
+        touch(c0, o);
+        touch(c1, o);
+    }
+    for (int k = 0; k < n; ++k) {
+        if (r[2 * k] > r[2 * k + 1]) {
+            r[2 * k] = 1;
+            r[2 * k + 1] = 0;
+            continue;
+        }
+        for (int o = r[2 * k]; o <= r[2 * k + 1]; ++o) {
+            const int c0 = h.cb_i0[o], c1 = std::min(c0 + 1, n - 1);
+            if (c0 != k && c1 != k)
+                ok = false;
+        }
+    }
+    return r;
+}
 
 // Per-output records of a support <= 6 axis (AxisDev::rec); empty otherwise.
 std::vector<float> pack_axis(const HostAxis &h) {
@@ -385,6 +437,10 @@ struct Geometry {
     AxisDev sax, say;
     RadialDev srd;
     int tiles_fit = 0;  // tiled guided pass usable (chroma footprints of 8x8 tiles fit LDS)
+    // per chroma column / row: range of output columns / rows whose bilinear
+    // footprint contains it (cm_ok: every range contiguous)
+    const int2 *cm_x = nullptr, *cm_y = nullptr;
+    bool cm_ok = false;
 };
 
 struct Workspace {
@@ -397,7 +453,7 @@ struct Workspace {
     float *y = nullptr, *u = nullptr, *v = nullptr;
     float4 *yt = nullptr;   // {Y, jxx, jxy, jyy}
     float4 *luv = nullptr;  // {lc, U, V, 0}
-    float *lc = nullptr, *ms = nullptr, *grad = nullptr, *hbuf = nullptr;
+    float *lc = nullptr, *ms = nullptr, *hbuf = nullptr;
     float *out_u = nullptr, *out_v = nullptr, *plain_u = nullptr, *plain_v = nullptr;
     float *luma_tmp = nullptr, *luma_out = nullptr;
     uint64_t *seed = nullptr, *mask = nullptr, *mtmp = nullptr;  // bit planes, rows of w64 words
@@ -427,6 +483,7 @@ struct Workspace {
     std::vector<hipEvent_t> pool;
     hipEvent_t done = nullptr;  // completion of a submitted frame
     bool has_ws = false;  // last frame built a sparse workset
+    bool ws_indices = false;  // ... with the compacted index list
 };
 
 }  // namespace
@@ -500,6 +557,8 @@ lgcr_status get_geometry(lgcr_clip *c, double sx, double sy, Geometry **out) {
         o[a][9] = blob.add(h.lb_i0); o[a][10] = blob.add(h.lb_f); o[a][11] = blob.add(pack_axis(h));
     }
     const size_t oxb = blob.add(xb), oxe = blob.add(xe), oyb = blob.add(yb), oye = blob.add(ye);
+    bool cm_ok = true;
+    const size_t ocmx = blob.add(footprint_ranges(ax, c->cw, cm_ok)), ocmy = blob.add(footprint_ranges(ay, c->ch, cm_ok));
     std::vector<float> plx(c->cw), ply(c->ch);
     for (int x = 0; x < c->cw; ++x)
         plx[x] = float((x + 0.5) * c->rw - 0.5 + sx);
@@ -571,7 +630,11 @@ lgcr_status get_geometry(lgcr_clip *c, double sx, double sy, Geometry **out) {
         d.lb_i0 = reinterpret_cast<const int *>(base + o[a][9]);
         d.lb_f = reinterpret_cast<const float *>(base + o[a][10]);
         d.rec = d.sup <= 6 ? reinterpret_cast<const float4 *>(base + o[a][11]) : nullptr;
+        d.tile_rows16 = tile_rows16(*axes[a]);
     }
+    g->cm_x = reinterpret_cast<const int2 *>(base + ocmx);
+    g->cm_y = reinterpret_cast<const int2 *>(base + ocmy);
+    g->cm_ok = cm_ok;
     g->xb = reinterpret_cast<const int *>(base + oxb);
     g->xe = reinterpret_cast<const int *>(base + oxe);
     g->yb = reinterpret_cast<const int *>(base + oyb);
@@ -618,6 +681,7 @@ lgcr_status get_geometry(lgcr_clip *c, double sx, double sy, Geometry **out) {
             d.lb_i0 = reinterpret_cast<const int *>(base + os[a][9]);
             d.lb_f = reinterpret_cast<const float *>(base + os[a][10]);
             d.rec = d.sup <= 6 ? reinterpret_cast<const float4 *>(base + os[a][11]) : nullptr;
+            d.tile_rows16 = tile_rows16(*sa[a]);
         }
         g->srd = g->rd;
         g->srd.valid = shr.valid ? 1 : 0;
@@ -682,7 +746,6 @@ void layout(const lgcr_clip *c, Workspace *w, Carver &cv) {
     w->luv = cv.take<float4>(nc);
     w->lc = cv.take<float>(nc);
     w->ms = cv.take<float>(nc);
-    w->grad = cv.take<float>(6 * nc);
     w->hbuf = cv.take<float>(6 * size_t(std::max(c->ow, c->sw)) * c->ch + 4);
     w->out_u = cv.take<float>(no);
     w->out_v = cv.take<float>(no);
@@ -700,7 +763,7 @@ void layout(const lgcr_clip *c, Workspace *w, Carver &cv) {
     w->row_off = cv.take<uint32_t>(size_t(c->oh) + 1);
     w->row_cnt = cv.take<uint32_t>(size_t(c->oh) + 1);
     w->indices = cv.take<uint32_t>(no);
-    w->counters = cv.take<uint32_t>(8);
+    w->counters = cv.take<uint32_t>(kCounterWords);
     if (c->p.algo == 4 && c->p.strength > 0.0) {
         w->ls = cv.take<float>(nc);
         w->mm = cv.take<float>(4 * nc);
@@ -873,6 +936,21 @@ void add_timing(lgcr_clip *c, const double *us, const uint64_t *n) {
 
 // Enqueue one frame on workspace `w`.  `in` / `out` planes are device
 // pointers (already staged for host frames).
+// BuildTrustMask + sparse workset.  When the tiled guided pass consumes it
+// (it reads the mask bits) at output size == luma size, only the mask, the
+// chroma mask and the counters are built; otherwise also the compacted index
+// list the other passes visit.
+hipError_t trust_workset(lgcr_clip *c, Workspace *w, const Geometry &g, bool tiled, hipStream_t s) {
+    const bool fast = tiled && c->same && g.cm_ok && !c->capture && c->dil < 64;
+    w->has_ws = true;
+    w->ws_indices = !fast;
+    if (fast)
+        return launch_trust_mask_tiles(w->seed, w->mtmp, w->mask, c->sw, c->sh, c->dil, w->cmask, c->cw, c->ch,
+                                       g.cm_x, g.cm_y, w->counters, s);
+    return launch_trust_workset(w->seed, w->mtmp, w->mask, c->sw, c->sh, c->dil, w->row_cnt, w->row_off, w->indices,
+                                w->cmask, c->cw, c->ch, g.ax, g.ay, w->counters, s);
+}
+
 lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const void *const in[3],
                           const ptrdiff_t in_stride[3], void *const out[3], const ptrdiff_t out_stride[3],
                           bool timed) {
@@ -935,14 +1013,14 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
         HIP_TRY(mark(LGCR_STAGE_BUILD_GUIDE_MAPS));
         if (p.sparse) {
             // BuildTrustMask + sparse workset (plugin.cpp:335-355)
-            HIP_TRY(launch_trust_workset(w->seed, w->mtmp, w->mask, c->sw, c->sh, c->dil, w->row_cnt, w->row_off,
-                                         w->indices, w->cmask, c->cw, c->ch, g.ax, g.ay, w->counters, s));
-            w->has_ws = true;
+            const int sel_mode = p.algo == 4 ? (is_float ? SEL_FLOAT : SEL_MAPS) : SEL_NONE;
+            HIP_TRY(trust_workset(c, w, g, p.algo != 6 && guided_tiled(g.tiles_fit, p.radial, p.cedge, g.ax, g.ay,
+                                                                        sel_mode), s));
             HIP_TRY(mark(LGCR_STAGE_BUILD_TRUST_MASK));
         }
         if (p.ms > 0.0) {
             // BuildMutualGate (plugin.cpp:356-365)
-            HIP_TRY(launch_mutual_gate(w->lc, w->u, w->v, c->cw, c->ch, w->grad, w->has_ws ? w->cmask : nullptr,
+            HIP_TRY(launch_mutual_gate(w->lc, w->u, w->v, c->cw, c->ch, w->has_ws ? w->cmask : nullptr,
                                        w->counters, w->ms, s));
             HIP_TRY(mark(LGCR_STAGE_BUILD_MUTUAL_GATE));
         }
@@ -1174,7 +1252,7 @@ lgcr_status enqueue_trecon(lgcr_clip *c, Workspace *w, const Geometry &g, const 
         HIP_TRY(launch_sobel_tensor(w->y, c->sw, c->sh, w->yt, p.sparse ? w->seed : nullptr,
                                     float(p.sigma * p.sigma), s));
         if (p.ms > 0.0)
-            HIP_TRY(launch_mutual_gate(w->lc, w->u, w->v, c->cw, c->ch, w->grad, nullptr, w->counters, w->ms, s));
+            HIP_TRY(launch_mutual_gate(w->lc, w->u, w->v, c->cw, c->ch, nullptr, w->counters, w->ms, s));
         HIP_TRY(mark(LGCR_STAGE_BUILD_GUIDE_MAPS));
     }
     const double oscale = is_float ? 1.0 : double((1 << c->f.bits_per_sample) - 1);
@@ -1185,9 +1263,7 @@ lgcr_status enqueue_trecon(lgcr_clip *c, Workspace *w, const Geometry &g, const 
     const float margin = float(std::max(0.0, p.ar_margin));
     w->has_ws = false;
     if (p.sparse && guided) {
-        HIP_TRY(launch_trust_workset(w->seed, w->mtmp, w->mask, c->sw, c->sh, c->dil, w->row_cnt, w->row_off,
-                                     w->indices, w->cmask, c->cw, c->ch, g.ax, g.ay, w->counters, s));
-        w->has_ws = true;
+        HIP_TRY(trust_workset(c, w, g, guided_tiled(g.tiles_fit, p.radial, p.cedge, g.ax, g.ay, SEL_NONE), s));
         HIP_TRY(launch_plain_separable(w->u, w->v, c->cw, c->ch, g.ax, g.ay, w->hbuf, w->out_u, w->out_v, clamp_hull,
                                        margin, s));
         HIP_TRY(mark(LGCR_STAGE_BUILD_BASE_CHROMA));
@@ -1387,6 +1463,9 @@ static lgcr_status create_clip(lgcr_device *dev, const lgcr_params *params, cons
     c->max_inflight = max_inflight > 0 ? max_inflight : 4;
     if (size_t(c->sw) + 2 * size_t(c->dil) > 60 * 1024)
         return fail(LGCR_UNSUPPORTED, "LGCR HIP backend: frame width %d too large", c->sw);
+    // kernels index planes with 32-bit sample offsets
+    if (size_t(c->sw) * c->sh > size_t(INT_MAX) || size_t(c->ow) * c->oh > size_t(INT_MAX))
+        return fail(LGCR_UNSUPPORTED, "LGCR HIP backend: planes of 2^31 or more samples are not supported");
     if (const char *cap = std::getenv("LGCR_CAPTURE"))
         c->capture = cap[0] && cap[0] != '0';
     HIP_TRY(hipSetDevice(dev->ordinal));
@@ -1685,9 +1764,12 @@ lgcr_status lgcr_recon_frame(lgcr_clip *c, const lgcr_frame *in, double shift_x,
         stats->stage_pixels[LGCR_STAGE_CONVERT_OUTPUT] = 2 * no;
         stats->output_pixels = uint64_t(c->ow) * c->oh;
         if (w->has_ws) {
-            uint32_t cnt[1] = {0}, act = 0;
-            HIP_TRY(hipMemcpy(&act, w->row_off + c->oh, sizeof(uint32_t), hipMemcpyDeviceToHost));
-            HIP_TRY(hipMemcpy(cnt, w->counters, sizeof(uint32_t), hipMemcpyDeviceToHost));
+            uint32_t cnt[3] = {0, 0, 0}, act = 0;
+            HIP_TRY(hipMemcpy(cnt, w->counters, sizeof(cnt), hipMemcpyDeviceToHost));
+            if (w->ws_indices)
+                HIP_TRY(hipMemcpy(&act, w->row_off + c->oh, sizeof(uint32_t), hipMemcpyDeviceToHost));
+            else
+                act = cnt[2];
             stats->active_output_pixels = act;
             stats->active_chroma_pixels = cnt[0];
             stats->sparse_total_pixels = uint64_t(c->sw) * c->sh;
@@ -2177,7 +2259,7 @@ lgcr_status lgcr_clip_fetch(lgcr_clip *c, int what, void *dst, size_t bytes, siz
         uint32_t count = 0;
         HIP_TRY(hipMemcpy(&count, w->row_off + c->oh, sizeof(count), hipMemcpyDeviceToHost));
         src = w->indices;
-        size = w->has_ws ? size_t(count) * 4 : 0;
+        size = w->has_ws && w->ws_indices ? size_t(count) * 4 : 0;
         break;
     }
     case LGCR_FETCH_PLAIN_U: src = w->plain_u; size = no * 4; break;
