@@ -26,7 +26,7 @@ struct AxisDev {
     const float *lb_f = nullptr;
     // sup <= 6: one 64-byte record per output, 4 float4:
     // {start, pos, lpos, active-tap bits}, {lb_i0, lb_f, cb_i0, cb_f},
-    // {w0, w1, w2, w3}, {w4, w5, 0, 0} (ints stored by bit pattern)
+    // {w0, w1, w2, w3}, {w4, w5, wsum, wabs} (ints stored by bit pattern)
     const float4 *rec = nullptr;
     // largest number of source rows/columns read by 16 consecutive outputs
     // (start[o + 15] + sup - start[o] over o = 0, 16, 32, ...)
@@ -129,8 +129,9 @@ hipError_t launch_lc_map(const float *y, int w, int h, float *lc, int cw, int ch
                          const int *xb, const int *xe, const int *yb, const int *ye,
                          hipStream_t s, const float *u = nullptr, const float *v = nullptr,
                          float4 *luv = nullptr);
+// count / ncount: the marked chroma samples are count[0] + ... + count[ncount - 1]
 hipError_t launch_mutual_gate(const float *lc, const float *u, const float *v, int w, int h,
-                              const uint8_t *chroma_mask, const uint32_t *counters, float *gate,
+                              const uint8_t *chroma_mask, const uint32_t *count, int ncount, float *gate,
                               hipStream_t s);
 hipError_t launch_plain_separable(const float *u, const float *v, int cw, int ch,
                                   const AxisDev &ax, const AxisDev &ay, float *hbuf,
@@ -150,23 +151,27 @@ hipError_t launch_guided(const float *u, const float *v, int cw, int ch, const f
                          const uint32_t *indices, float *out_u, float *out_v,
                          const SelectorDev &sel, const TemporalDev &tmp, hipStream_t s,
                          const uint64_t *mask_bits = nullptr, int mask_w = 0, int mask_h = 0,
-                         int tiles_fit = 0);
+                         int tiles_fit = 0, int plain_in_tiles = 0);
 // whether every 8x8 output tile's chroma footprint (6 taps per axis) fits
 // the LDS window of the tiled guided pass
 int guided_tiles_fit(const int *xs, int ow, const int *ys, int oh);
 // true when launch_guided takes the tiled pass, which reads the trust mask
 // bits instead of the compacted index list
-constexpr int kCounterWords = 8 + 2 * 256;
 bool guided_tiled(int tiles_fit, int radial, int cedge, const AxisDev &ax, const AxisDev &ay, int sel_mode);
 // Trust mask for the tiled pass at output size == luma size: dilated mask
-// bits, chroma mask and counters (counters[0] marked chroma samples,
-// counters[2] active output pixels; counters has kCounterWords entries, the
-// ones past 8 hold partial sums), no index list.  xr / yr: per chroma
-// column / row, the range of output columns / rows whose bilinear footprint
-// contains it.
-hipError_t launch_trust_mask_tiles(const uint64_t *seed, uint64_t *tmp, uint64_t *mask, int w, int h, int radius,
-                                   uint8_t *chroma_mask, int cw, int ch, const int2 *xr, const int2 *yr,
-                                   uint32_t *counters, hipStream_t s);
+// bits and chroma mask, no index list, one block per band of cband chroma
+// rows.  band[b] = {first owned mask row, end of owned rows, first and last
+// mask row the band reads}; the owned ranges partition the mask rows.
+// Counters: counters[kCntChromaWords + b] marked chroma samples of band b,
+// counters[kCntChromaWords + kMaxBands + b] active pixels of its mask rows.
+// xr / yr: per chroma column / row, the range of output columns / rows whose
+// bilinear footprint contains it.
+constexpr int kCntChromaWords = 64, kMaxBands = 4096, kMaxCband = 64;
+constexpr int kCounterWords = kCntChromaWords + 2 * kMaxBands;
+size_t trust_fused_lds(int w, int radius, int rows, int cw);
+hipError_t launch_trust_mask_tiles(const uint64_t *seed, uint64_t *mask, int w, int h, int radius, const int4 *band,
+                                   int nbands, int cband, int max_rows, uint8_t *chroma_mask, int cw, int ch,
+                                   const int2 *xr, const int2 *yr, uint32_t *counters, hipStream_t s);
 // TRecon motion search and chroma consistency (algos.cpp:683-807)
 hipError_t launch_block_match(const float *cur, const float *nbr, int w, int h, int block,
                               int search, float tsad, int16_t *mvx, int16_t *mvy, float *conf,

@@ -42,6 +42,12 @@ __device__ __forceinline__ float recip_ge1(float y) {
     return 1.0f / y;
 }
 
+// acc + (double)a * b.  A product of two floats is exact in double, so the
+// fused form rounds once, exactly like the separate multiply and add.
+__device__ __forceinline__ double dacc(double acc, float a, float b) {
+    return __builtin_fma((double)a, (double)b, acc);
+}
+
 // XCD-aware block order.  Workgroups are dispatched round-robin over the 8
 // XCDs, each with its own L2; remapping the linear block id gives every XCD
 // a contiguous range of tiles, so halos shared by neighbouring tiles hit the
@@ -497,77 +503,87 @@ __global__ __launch_bounds__(64) void k_dilate_vbits(const uint64_t *in, uint64_
 }
 
 // Trust mask without the index list, for the tiled guided pass (it reads
-// the mask bits) at output size == mask size: one thread per mask word in
-// both dilation passes, then one thread per chroma sample for the chroma
-// mask.
+// the mask bits) at output size == mask size, in one kernel.  Block b owns
+// a band of chroma rows and the mask rows band[b].x .. band[b].y - 1 (the
+// host partitions the mask rows); it stages the seed rows its mask rows
+// band[b].z .. band[b].w read (r above and below, zero outside the image),
+// dilates them in LDS, stores and counts its own mask rows, and marks its
+// chroma rows from the LDS rows.  Each block writes
+// its two counts to its own partial slot: a shared counter would serialise
+// the atomics of every block across the XCDs.
 //
-// Horizontal pass for radius < 64: a word and its two neighbours hold every
-// bit within the radius, so the OR of the 2r+1 shifted rows is formed in
-// registers (the same bits row_bits returns).
-__global__ __launch_bounds__(kBlock) void k_hdilate_words(const uint64_t *in, uint64_t *out, int w, int w64, int h,
-                                                          int r, uint32_t *counters) {
-    const int idx = blockIdx.x * kBlock + threadIdx.x;
-    if (idx < 8)
-        counters[idx] = 0;
-    if (idx >= w64 * h)
-        return;
-    const int y = idx / w64, wx = idx - y * w64;
-    const uint64_t *row = in + (uint32_t)(y * w64);
-    const uint64_t lo = wx > 0 ? row[wx - 1] : 0ull, mid = row[wx], hi = wx + 1 < w64 ? row[wx + 1] : 0ull;
-    uint64_t acc = mid;
-    for (int d = 1; d <= r; ++d)
-        acc |= (mid << d) | (lo >> (64 - d)) | (mid >> d) | (hi << (64 - d));
-    const int valid = w - wx * 64;  // pixels of this word inside the row
-    if (valid < 64)
-        acc &= (1ull << valid) - 1ull;
-    out[idx] = acc;
-}
-
-// Per-block population counts go to partial sums that k_finish_counts adds
-// up: one same-address atomic per block would serialise across the XCDs.
-constexpr int kCountBlocks = 256;
-
-// Vertical pass (grid-stride, at most kCountBlocks blocks); also counts the
-// active pixels (active output pixels, the mask having the output size).
-__global__ __launch_bounds__(kBlock) void k_vdilate_words(const uint64_t *in, uint64_t *out, int w64, int h, int r,
-                                                          uint32_t *partial) {
-    __shared__ int scratch[kBlock / 64];
-    int cnt = 0;
-    for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < w64 * h; idx += gridDim.x * kBlock) {
-        const int y = idx / w64, wx = idx - y * w64;
-        const int j0 = max(0, y - r), j1 = min(h - 1, y + r);
-        uint64_t acc = 0;
-        for (int j = j0; j <= j1; ++j)
-            acc |= in[(uint32_t)(j * w64 + wx)];
-        out[idx] = acc;
-        cnt += __popcll(acc);
-    }
-    const int t = block_sum(cnt, scratch);
-    if (threadIdx.x == 0)
-        partial[blockIdx.x] = (uint32_t)t;
-}
-
-// Chroma mask: sample (cx, cy) is marked when an active output pixel has it
-// in its 2x2 bilinear footprint (buildSparseWorkset, maps.cpp:342-360).  The
-// host passes, per chroma column and row, the contiguous range of output
-// columns / rows whose footprint contains it ({first, last}; first > last
-// for none).  Writes every sample (grid-stride, at most kCountBlocks blocks)
-// and counts the marked ones.
-__global__ __launch_bounds__(kBlock) void k_chroma_mark(const uint64_t *mask, int w64, int cw, int ch,
+// Horizontal dilation for radius < 64: a word and its two neighbours hold
+// every bit within the radius, so the OR of the 2r+1 shifted rows is formed
+// in registers.
+__global__ __launch_bounds__(kBlock) void k_trust_fused(const uint64_t *seed, uint64_t *mask, int w, int w64, int h,
+                                                        int r, const int4 *band, int cband, int cw, int ch,
                                                         const int2 *xr, const int2 *yr, uint8_t *cmask,
-                                                        uint32_t *partial) {
+                                                        uint32_t *counters) {
+    extern __shared__ uint64_t tf_lds[];
     __shared__ int scratch[kBlock / 64];
-    int cnt = 0;
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < cw * ch; i += gridDim.x * kBlock) {
-        const int cy = i / cw, cx = i - cy * cw;
-        const int2 xs = xr[cx], ys = yr[cy];
+    __shared__ int2 ys_band[kMaxCband];
+    const int4 bd = band[blockIdx.x];
+    const int rows = bd.w - bd.z + 1, srows = rows + 2 * r, ws = w64 + 2;
+    uint64_t *raw = tf_lds;                 // srows x ws seed rows, zero outside the image
+    uint64_t *vd = raw + srows * ws;        // rows x ws vertical ORs, one zero word each side
+    uint64_t *mk = vd + rows * ws;          // rows x w64 (mask rows bd.z ..)
+    int2 *xs_l = reinterpret_cast<int2 *>(mk + rows * w64);  // cw column ranges
+    const int s0 = bd.z - r;
+    const int cy0 = blockIdx.x * cband, ncy = min(cband, ch - cy0);
+    for (int i = threadIdx.x; i < cw; i += kBlock)
+        xs_l[i] = xr[i];
+    if (threadIdx.x < ncy)
+        ys_band[threadIdx.x] = yr[cy0 + threadIdx.x];
+    if (blockIdx.x == 0 && threadIdx.x < kCntChromaWords)
+        counters[threadIdx.x] = 0;  // per-frame counters of the other passes
+    for (int i = threadIdx.x; i < srows * ws; i += kBlock) {
+        const int k = i / ws, c = i - k * ws, sy = s0 + k, wx = c - 1;
+        raw[i] = (sy >= 0 && sy < h && wx >= 0 && wx < w64) ? seed[(uint32_t)(sy * w64 + wx)] : 0ull;
+    }
+    __syncthreads();
+    // the square dilation is separable in either order: vertical OR of the
+    // 2r+1 seed rows first (only the band's rows), then the horizontal pass
+    for (int i = threadIdx.x; i < rows * ws; i += kBlock) {
+        const int k = i / ws;
+        uint64_t acc = 0;
+        for (int j = 0; j <= 2 * r; ++j)
+            acc |= raw[(k + j) * ws + (i - k * ws)];
+        vd[i] = acc;
+    }
+    __syncthreads();
+    int act = 0;
+    for (int i = threadIdx.x; i < rows * w64; i += kBlock) {
+        const int k = i / w64, wx = i - k * w64, y = bd.z + k;
+        const uint64_t *q = vd + k * ws + wx;
+        const uint64_t lo = q[0], mid = q[1], hi = q[2];
+        uint64_t acc = mid;
+        for (int d = 1; d <= r; ++d)
+            acc |= (mid << d) | (lo >> (64 - d)) | (mid >> d) | (hi << (64 - d));
+        const int valid = w - wx * 64;  // pixels of this word inside the row
+        if (valid < 64)
+            acc &= (1ull << valid) - 1ull;
+        mk[i] = acc;
+        if (y >= bd.x && y < bd.y) {
+            mask[(uint32_t)(y * w64 + wx)] = acc;
+            act += __popcll(acc);
+        }
+    }
+    __syncthreads();
+    // chroma marks (buildSparseWorkset, maps.cpp:342-360): sample (cx, cy)
+    // is marked when an active output pixel has it in its 2x2 bilinear
+    // footprint; xr / yr give, per chroma column / row, the contiguous range
+    // of output columns / rows whose footprint contains it
+    int marked = 0;
+    for (int i = threadIdx.x; i < ncy * cw; i += kBlock) {
+        const int k = i / cw, cx = i - k * cw, cy = cy0 + k;
+        const int2 xs = xs_l[cx], ys = ys_band[k];
         uint64_t any = 0;
-        if (xs.x <= xs.y) {
+        if (xs.x <= xs.y && ys.x <= ys.y) {
             const int wa = xs.x >> 6, wb = xs.y >> 6;
             const uint64_t ma = ~0ull << (xs.x & 63);
             const uint64_t mb = (xs.y & 63) == 63 ? ~0ull : (1ull << ((xs.y & 63) + 1)) - 1ull;
             for (int y = ys.x; y <= ys.y; ++y) {
-                const uint64_t *row = mask + (uint32_t)(y * w64);
+                const uint64_t *row = mk + (y - bd.z) * w64;
                 if (wa == wb) {
                     any |= row[wa] & ma & mb;
                 } else {
@@ -579,29 +595,14 @@ __global__ __launch_bounds__(kBlock) void k_chroma_mark(const uint64_t *mask, in
             }
         }
         const int hit = any != 0ull;
-        cmask[(uint32_t)i] = (uint8_t)hit;
-        cnt += hit;
+        cmask[(uint32_t)(cy * cw + cx)] = (uint8_t)hit;
+        marked += hit;
     }
-    const int t = block_sum(cnt, scratch);
-    if (threadIdx.x == 0)
-        partial[blockIdx.x] = (uint32_t)t;
-}
-
-// counters[0] = sum of the chroma-mark partials, counters[2] = sum of the
-// vertical-pass partials (one block).
-__global__ __launch_bounds__(kBlock) void k_finish_counts(const uint32_t *pc, int nc, const uint32_t *pa, int na,
-                                                          uint32_t *counters) {
-    __shared__ int scratch[kBlock / 64];
-    int a = 0, b = 0;
-    for (int i = threadIdx.x; i < nc; i += kBlock)
-        a += (int)pc[i];
-    for (int i = threadIdx.x; i < na; i += kBlock)
-        b += (int)pa[i];
-    const int ta = block_sum(a, scratch);
-    const int tb = block_sum(b, scratch);
+    const int ta = block_sum(act, scratch);
+    const int tm = block_sum(marked, scratch);
     if (threadIdx.x == 0) {
-        counters[0] = (uint32_t)ta;
-        counters[2] = (uint32_t)tb;
+        counters[kCntChromaWords + blockIdx.x] = (uint32_t)tm;
+        counters[kCntChromaWords + kMaxBands + blockIdx.x] = (uint32_t)ta;
     }
 }
 
@@ -745,17 +746,32 @@ __device__ __forceinline__ void sobel_lds(const float (*p)[kGateSW], int xm, int
 
 __global__ __launch_bounds__(kBlock) void k_gate_fused(const float *__restrict__ lc, const float *__restrict__ u,
                                                        const float *__restrict__ v, int w, int h,
-                                                       const uint8_t *cmask, const uint32_t *counters,
+                                                       const uint8_t *cmask, const uint32_t *count, int ncount,
                                                        float *gate) {
     __shared__ float src[3][kGateSH][kGateSW];
     __shared__ float grd[6][kGateGH][kGateGW];
+    __shared__ int s_dense;
     unsigned bx, by;
     xcd_block2(bx, by);
     const int tx0 = (int)bx * kGateTW, ty0 = (int)by * kGateTH;
     const int gx0 = tx0 - kGateHalo, gy0 = ty0 - kGateHalo, sx0 = gx0 - 1, sy0 = gy0 - 1;
     const int x = tx0 + (int)(threadIdx.x % kGateTW), y = ty0 + (int)(threadIdx.x / kGateTW);
     const size_t n = (size_t)w * h;
-    const bool dense = !cmask || (uint64_t)counters[0] * 2 >= (uint64_t)n;
+    // dense unless at most half the chroma samples are marked (the marked
+    // count is the sum of count[0..ncount))
+    if (threadIdx.x < 64) {
+        uint32_t c = 0u;
+        if (cmask) {
+            _Pragma("unroll 8") for (int i = threadIdx.x; i < ncount; i += 64)
+                c += count[i];
+        }
+        for (int o = 32; o > 0; o >>= 1)
+            c += __shfl_xor(c, o, 64);
+        if (threadIdx.x == 0)
+            s_dense = !cmask || (uint64_t)c * 2 >= (uint64_t)n;
+    }
+    __syncthreads();
+    const bool dense = s_dense != 0;
     const bool inside = x < w && y < h;
     const bool need = inside && (dense || cmask[(size_t)y * w + x] != 0);
     if (!__syncthreads_or(need)) {
@@ -860,6 +876,7 @@ template <> struct PixAxis<6> {
     float pos, lpos, lb_f, cb_f;
     uint32_t bits;
     float w[6];
+    float wsum, wabs;  // sum and absolute sum of the active weights
     __device__ __forceinline__ bool on(int i) const { return (bits >> i) & 1u; }
     __device__ __forceinline__ float wt(int i) const { return w[i]; }
 };
@@ -879,6 +896,8 @@ template <int SUP> __device__ __forceinline__ PixAxis<SUP> pix_axis(const AxisDe
         p.cb_f = r1.w;
         p.w[0] = r2.x; p.w[1] = r2.y; p.w[2] = r2.z; p.w[3] = r2.w;
         p.w[4] = r3.x; p.w[5] = r3.y;
+        p.wsum = r3.z;
+        p.wabs = r3.w;
     } else {
         p.start = a.start[o];
         p.pos = a.pos[o];
@@ -924,8 +943,8 @@ __global__ __launch_bounds__(kBlock) void k_plain_h(const float *__restrict__ u,
         _Pragma("unroll") for (int i = 0; i < 6; ++i) {
             const bool on = px.on(i);
             const float w = on ? px.w[i] : 0.0f;
-            vu += (double)w * a[i];
-            vv += (double)w * b[i];
+            vu = dacc(vu, w, a[i]);
+            vv = dacc(vv, w, b[i]);
             const float nmu = fmn(mnu, a[i]), nxu = fmx(mxu, a[i]), nmv = fmn(mnv, b[i]), nxv = fmx(mxv, b[i]);
             mnu = on ? nmu : mnu;
             mxu = on ? nxu : mxu;
@@ -941,8 +960,8 @@ __global__ __launch_bounds__(kBlock) void k_plain_h(const float *__restrict__ u,
                 continue;
             const int x = clampi(start + i, 0, cw - 1);
             const float a = su[x], b = sv[x];
-            vu += (double)wt[i] * a;
-            vv += (double)wt[i] * b;
+            vu = dacc(vu, wt[i], a);
+            vv = dacc(vv, wt[i], b);
             mnu = fmn(mnu, a); mxu = fmx(mxu, a);
             mnv = fmn(mnv, b); mxv = fmx(mxv, b);
         }
@@ -983,8 +1002,8 @@ __global__ __launch_bounds__(kBlock) void k_plain_v(const float *__restrict__ u,
         _Pragma("unroll") for (int j = 0; j < 6; ++j) {
             const bool on = am[j] != 0.0f;   // uniform per block row
             const float w = on ? wt[j] : 0.0f;
-            vu += (double)w * val[j].x;
-            vv += (double)w * val[j].y;
+            vu = dacc(vu, w, val[j].x);
+            vv = dacc(vv, w, val[j].y);
             const float nmu = fmn(mnu, hull[j].x), nxu = fmx(mxu, hull[j].y);
             const float nmv = fmn(mnv, hull[j].z), nxv = fmx(mxv, hull[j].w);
             mnu = on ? nmu : mnu;
@@ -999,8 +1018,8 @@ __global__ __launch_bounds__(kBlock) void k_plain_v(const float *__restrict__ u,
             const size_t k = (size_t)clampi(start + j, 0, ch - 1) * ow + ox;
             const float2 val = hv[k];
             const float4 hull = hh[k];
-            vu += (double)wt[j] * val.x;
-            vv += (double)wt[j] * val.y;
+            vu = dacc(vu, wt[j], val.x);
+            vv = dacc(vv, wt[j], val.y);
             mnu = fmn(mnu, hull.x); mxu = fmx(mxu, hull.y);
             mnv = fmn(mnv, hull.z); mxv = fmx(mxv, hull.w);
         }
@@ -1058,8 +1077,8 @@ __global__ __launch_bounds__(kBlock) void k_plain_fused(const float *__restrict_
         _Pragma("unroll") for (int i = 0; i < 6; ++i) {
             const bool on = px.on(i);
             const float w = on ? px.w[i] : 0.0f;
-            vu += (double)w * a[i];
-            vv += (double)w * b[i];
+            vu = dacc(vu, w, a[i]);
+            vv = dacc(vv, w, b[i]);
             const float nmu = fmn(mnu, a[i]), nxu = fmx(mxu, a[i]), nmv = fmn(mnv, b[i]), nxv = fmx(mxv, b[i]);
             mnu = on ? nmu : mnu;
             mxu = on ? nxu : mxu;
@@ -1088,8 +1107,8 @@ __global__ __launch_bounds__(kBlock) void k_plain_fused(const float *__restrict_
             const float w = on ? wt[j] : 0.0f;
             const float2 val = sval[t0 + j][c];
             const float4 hull = shull[t0 + j][c];
-            vu += (double)w * val.x;
-            vv += (double)w * val.y;
+            vu = dacc(vu, w, val.x);
+            vv = dacc(vv, w, val.y);
             const float nmu = fmn(mnu, hull.x), nxu = fmx(mxu, hull.y);
             const float nmv = fmn(mnv, hull.z), nxv = fmx(mxv, hull.w);
             mnu = on ? nmu : mnu;
@@ -1176,11 +1195,11 @@ __global__ __launch_bounds__(kBlock) void k_plain_radial(const float *__restrict
             const float w = radial_weight(rd, base, ax.sup, i, j, tx0, ty0, scx, scy);
             const size_t k = row + clampi(tx0 + i, 0, cw - 1);
             const float a = u[k], b = v[k];
-            au += (double)w * a;
+            au = dacc(au, w, a);
             ws += w;
             sa += fabsf(w);
             mnu = fmn(mnu, a); mxu = fmx(mxu, a);
-            av += (double)w * b;
+            av = dacc(av, w, b);
             mnv = fmn(mnv, b); mxv = fmx(mxv, b);
         }
     }
@@ -1365,8 +1384,8 @@ __device__ __forceinline__ void guided_taps(const GuidedArgs &A, const PixAxis<S
             }
             if (SUP)
                 w = on ? w : 0.0f;
-            acc.u += (double)w * q.y;
-            acc.v += (double)w * q.z;
+            acc.u = dacc(acc.u, w, q.y);
+            acc.v = dacc(acc.v, w, q.z);
             acc.wsum += w;
             acc.sabs += fabsf(w);
         }
@@ -1628,8 +1647,8 @@ __device__ __attribute__((always_inline)) void guided_pixel(const GuidedArgs &A,
             const float w = tconf * sim * kw * nw;
             const size_t tk = (size_t)tny * cw + tnx;
             const float nu = nb.u[tk], nv = nb.v[tk];
-            acc_u += (double)w * nu;
-            acc_v += (double)w * nv;
+            acc_u = dacc(acc_u, w, nu);
+            acc_v = dacc(acc_v, w, nv);
             wsum += w;
             sum_abs += fabsf(w);
             mnu = fmn(mnu, nu); mxu = fmx(mxu, nu);
@@ -1714,9 +1733,18 @@ __global__ __launch_bounds__(kBlock, SUP == 6 ? (TEMPORAL ? 2 : 4) : 1) void k_g
 // pixels that are active in the trust mask, the tile's chroma-guide
 // footprint staged in LDS.  The host enables it when every tile's footprint
 // fits kTWin x kTWin.
-template <bool TEMPORAL, int SEL>
-__global__ __launch_bounds__(kBlock, TEMPORAL ? 4 : 6) void k_guided_tiles(GuidedArgs A, int tiles_x, int tiles) {
+//
+// PLAIN: the wave also computes the plain separable base of its tile from
+// the same window (every element as k_plain_fused computes it: horizontal
+// pass of the 16 window rows into LDS, vertical pass per pixel) and stores
+// it where the guided pass does not overwrite it; with a selector the
+// guided pass reads it back as its base.
+template <bool TEMPORAL, int SEL, bool PLAIN>
+__global__ __launch_bounds__(kBlock, TEMPORAL ? 4 : (PLAIN ? 5 : 6)) void k_guided_tiles(GuidedArgs A, int tiles_x,
+                                                                                       int tiles) {
     __shared__ float4 wins[kBlock / 64][kTWin * kTStride];
+    __shared__ float2 hval[PLAIN ? kBlock / 64 : 1][kTWin][kTile];
+    __shared__ float4 hhull[PLAIN ? kBlock / 64 : 1][kTWin][kTile];
     const int lane = threadIdx.x & 63;
     // wave-uniform tile index, held in scalar registers
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1726,7 +1754,8 @@ __global__ __launch_bounds__(kBlock, TEMPORAL ? 4 : 6) void k_guided_tiles(Guide
     const int ow = A.ax.n, oh = A.ay.n, cw = A.cw, ch = A.ch;
     const int tx = t % tiles_x, ty = t / tiles_x;
     const int ox = tx * kTile + (lane & (kTile - 1)), oy = ty * kTile + (lane / kTile);
-    bool act = ox < ow && oy < oh;
+    const bool inside = ox < ow && oy < oh;
+    bool act = inside;
     if (act && A.mbits) {
         // nearest mask sample (buildSparseWorkset, maps.cpp:342-360); the
         // mask has the output size unless the clip resizes
@@ -1735,7 +1764,8 @@ __global__ __launch_bounds__(kBlock, TEMPORAL ? 4 : 6) void k_guided_tiles(Guide
         const int my = same ? oy : min(A.mh - 1, (int)((int64_t)oy * A.mh / oh));
         act = (A.mbits[(uint32_t)(my * A.mw64 + (mx >> 6))] >> (mx & 63)) & 1ull;
     }
-    if (__ballot(act) == 0)
+    const bool any = __ballot(act) != 0;
+    if (!PLAIN && !any)
         return;  // no active pixel in the tile (uniform)
     int x_lo = INT_MAX, y_lo = INT_MAX;
     for (int c = 0; c < kTile; ++c) {
@@ -1751,6 +1781,77 @@ __global__ __launch_bounds__(kBlock, TEMPORAL ? 4 : 6) void k_guided_tiles(Guide
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
+    if constexpr (PLAIN) {
+        // horizontal pass: lane -> column lane % 8, window rows lane / 8 and
+        // lane / 8 + 8 (columns past the right edge repeat the last one)
+        const int c8 = lane & (kTile - 1);
+        const PixAxis<6> pc = pix_axis<6>(A.ax, min(ox, ow - 1));
+        const int col0 = pc.start - x_lo;
+        _Pragma("unroll") for (int k = 0; k < 2; ++k) {
+            const int r = (lane >> 3) + 8 * k;
+            double vu = 0.0, vv = 0.0;
+            float mnu = 1e30f, mxu = -1e30f, mnv = 1e30f, mxv = -1e30f;
+            _Pragma("unroll") for (int i = 0; i < 6; ++i) {
+                const float4 q = wb[r * kTStride + col0 + i];
+                const float a = q.y, b = q.z;
+                const bool on = pc.on(i);
+                const float w = on ? pc.w[i] : 0.0f;
+                vu = dacc(vu, w, a);
+                vv = dacc(vv, w, b);
+                const float nmu = fmn(mnu, a), nxu = fmx(mxu, a), nmv = fmn(mnv, b), nxv = fmx(mxv, b);
+                mnu = on ? nmu : mnu;
+                mxu = on ? nxu : mxu;
+                mnv = on ? nmv : mnv;
+                mxv = on ? nxv : mxv;
+            }
+            hval[wave][r][c8] = make_float2((float)vu, (float)vv);
+            hhull[wave][r][c8] = make_float4(mnu, mxu, mnv, mxv);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        // vertical pass, normalisation, fallback and hull clamp
+        // (plainChromaSeparable, recon.cpp:1151-1461)
+        if (inside) {
+            const PixAxis<6> pr = pix_axis<6>(A.ay, oy);
+            const int t0 = pr.start - y_lo;
+            double vu = 0.0, vv = 0.0;
+            float mnu = 1e30f, mxu = -1e30f, mnv = 1e30f, mxv = -1e30f;
+            _Pragma("unroll") for (int j = 0; j < 6; ++j) {
+                const bool on = pr.on(j);
+                const float w = on ? pr.w[j] : 0.0f;
+                const float2 val = hval[wave][t0 + j][c8];
+                const float4 hull = hhull[wave][t0 + j][c8];
+                vu = dacc(vu, w, val.x);
+                vv = dacc(vv, w, val.y);
+                const float nmu = fmn(mnu, hull.x), nxu = fmx(mxu, hull.y);
+                const float nmv = fmn(mnv, hull.z), nxv = fmx(mxv, hull.w);
+                mnu = on ? nmu : mnu;
+                mxu = on ? nxu : mxu;
+                mnv = on ? nmv : mnv;
+                mxv = on ? nxv : mxv;
+            }
+            const double wsum = (double)pc.wsum * pr.wsum;
+            const double asum = (double)pc.wabs * pr.wabs;
+            if (wsum > 1e-6 && wsum >= 0.05 * asum) {
+                vu /= wsum;
+                vv /= wsum;
+            } else {
+                vu = bilinear_f(A.u, cw, ch, pc.pos, pr.pos);
+                vv = bilinear_f(A.v, cw, ch, pc.pos, pr.pos);
+            }
+            if (A.gp.clamp_hull) {
+                vu = dclamp(vu, (double)(mnu - A.gp.margin), (double)(mxu + A.gp.margin));
+                vv = dclamp(vv, (double)(mnv - A.gp.margin), (double)(mxv + A.gp.margin));
+            }
+            if (!act || SEL != SEL_NONE) {
+                const uint32_t o = (uint32_t)(oy * ow + ox);
+                A.out_u[o] = (float)vu;
+                A.out_v[o] = (float)vv;
+            }
+        }
+        if (!any)
+            return;
+    }
     if (act)
         guided_pixel<false, TEMPORAL, SEL, false, 6, true>(A, ox, oy, 0u, wb, x_lo, y_lo);
 }
@@ -2170,7 +2271,7 @@ __global__ __launch_bounds__(kMeThreads) void k_block_match(const float *__restr
     for (int t = 0; t < nx * ny; ++t) {
         const float v = sc[t];
         sum += v;
-        sum2 += (double)v * v;
+        sum2 = dacc(sum2, v, v);
     }
     const float npix = (float)(nx * ny);
     const float sad0 = s_sad0;
@@ -2284,7 +2385,7 @@ __global__ __launch_bounds__(kBlock) void k_resample_h(const float *src, int sw,
     const int s = t.start[x];
     double acc = 0.0;
     for (int j = 0; j < t.sup; ++j)
-        acc += (double)wp[j] * src[(size_t)y * sw + clampi(s + j, 0, sw - 1)];
+        acc = dacc(acc, wp[j], src[(size_t)y * sw + clampi(s + j, 0, sw - 1)]);
     dst[(size_t)y * t.n + x] = (float)acc;
 }
 
@@ -2296,7 +2397,7 @@ __global__ __launch_bounds__(kBlock) void k_resample_v(const float *src, int sw,
     const int s = t.start[y];
     double acc = 0.0;
     for (int j = 0; j < t.sup; ++j)
-        acc += (double)wp[j] * src[(size_t)clampi(s + j, 0, sh - 1) * sw + x];
+        acc = dacc(acc, wp[j], src[(size_t)clampi(s + j, 0, sh - 1) * sw + x]);
     dst[(size_t)y * sw + x] = (float)acc;
 }
 
@@ -2318,7 +2419,7 @@ __global__ __launch_bounds__(kBlock) void k_resample_radial(const float *src, in
             const int tx = clampi(x0 + i, 0, sw - 1);
             const float dx = (float)((x0 + i - cx) * inv_w);
             const float w = lut_lookup(lut, lut_n, sqrtf(dx * dx + dy * dy) * lut_scale);
-            acc += (double)w * src[(size_t)ty * sw + tx];
+            acc = dacc(acc, w, src[(size_t)ty * sw + tx]);
             wsum += w;
         }
     }
@@ -2755,19 +2856,17 @@ hipError_t launch_trust_workset(const uint64_t *seed, uint64_t *tmp, uint64_t *m
     return hipGetLastError();
 }
 
-hipError_t launch_trust_mask_tiles(const uint64_t *seed, uint64_t *tmp, uint64_t *mask, int w, int h, int radius,
-                                   uint8_t *chroma_mask, int cw, int ch, const int2 *xr, const int2 *yr,
-                                   uint32_t *counters, hipStream_t s) {
+size_t trust_fused_lds(int w, int radius, int rows, int cw) {
+    const size_t w64 = (size_t)(w + 63) / 64, srows = (size_t)rows + 2 * (size_t)radius;
+    return (srows * (w64 + 2) + (size_t)rows * (w64 + 2) + (size_t)rows * w64) * 8 + (size_t)cw * 8;
+}
+
+hipError_t launch_trust_mask_tiles(const uint64_t *seed, uint64_t *mask, int w, int h, int radius, const int4 *band,
+                                   int nbands, int cband, int max_rows, uint8_t *chroma_mask, int cw, int ch,
+                                   const int2 *xr, const int2 *yr, uint32_t *counters, hipStream_t s) {
     const int w64 = (w + 63) / 64;
-    const unsigned blocks = (unsigned)((w64 * h + kBlock - 1) / kBlock);
-    const int nv = (int)std::min<unsigned>(blocks, kCountBlocks);
-    const int nm = (int)std::min<long>(((long)cw * ch + kBlock - 1) / kBlock, kCountBlocks);
-    uint32_t *pa = counters + 8, *pc = counters + 8 + kCountBlocks;
-    hipLaunchKernelGGL(k_hdilate_words, dim3(blocks), dim3(kBlock), 0, s, seed, tmp, w, w64, h, radius, counters);
-    hipLaunchKernelGGL(k_vdilate_words, dim3((unsigned)nv), dim3(kBlock), 0, s, tmp, mask, w64, h, radius, pa);
-    hipLaunchKernelGGL(k_chroma_mark, dim3((unsigned)nm), dim3(kBlock), 0, s, mask, w64, cw, ch, xr, yr, chroma_mask,
-                       pc);
-    hipLaunchKernelGGL(k_finish_counts, dim3(1), dim3(kBlock), 0, s, pc, nm, pa, nv, counters);
+    hipLaunchKernelGGL(k_trust_fused, dim3((unsigned)nbands), dim3(kBlock), trust_fused_lds(w, radius, max_rows, cw), s,
+                       seed, mask, w, w64, h, radius, band, cband, cw, ch, xr, yr, chroma_mask, counters);
     return hipGetLastError();
 }
 
@@ -2776,10 +2875,10 @@ bool guided_tiled(int tiles_fit, int radial, int cedge, const AxisDev &ax, const
 }
 
 hipError_t launch_mutual_gate(const float *lc, const float *u, const float *v, int w, int h,
-                              const uint8_t *chroma_mask, const uint32_t *counters, float *gate,
+                              const uint8_t *chroma_mask, const uint32_t *count, int ncount, float *gate,
                               hipStream_t s) {
     const dim3 g((unsigned)((w + kGateTW - 1) / kGateTW), (unsigned)((h + kGateTH - 1) / kGateTH));
-    hipLaunchKernelGGL(k_gate_fused, g, dim3(kBlock), 0, s, lc, u, v, w, h, chroma_mask, counters, gate);
+    hipLaunchKernelGGL(k_gate_fused, g, dim3(kBlock), 0, s, lc, u, v, w, h, chroma_mask, count, ncount, gate);
     return hipGetLastError();
 }
 
@@ -2817,18 +2916,30 @@ hipError_t launch_guided(const float *u, const float *v, int cw, int ch, const f
                          const AxisDev &ax, const AxisDev &ay, const RadialDev &rd, const GuideParams &gp,
                          const uint32_t *row_offsets, const uint32_t *indices, float *out_u, float *out_v,
                          const SelectorDev &sel, const TemporalDev &tmp, hipStream_t s, const uint64_t *mask_bits,
-                         int mask_w, int mask_h, int tiles_fit) {
+                         int mask_w, int mask_h, int tiles_fit, int plain_in_tiles) {
     GuidedArgs A{u, v, cw, ch, y, sw, sh, yt, luv, lc, ms, ax, ay, rd, gp, row_offsets, indices, out_u, out_v,
                  sel, tmp, indices ? mask_bits : nullptr, mask_w, mask_h, (mask_w + 63) / 64};
     if (guided_tiled(tiles_fit, gp.radial, gp.cedge, ax, ay, sel.mode)) {
         const int tiles_x = (ax.n + kTile - 1) / kTile, tiles = tiles_x * ((ay.n + kTile - 1) / kTile);
         const dim3 g((unsigned)((tiles + kBlock / 64 - 1) / (kBlock / 64)));
-        if (tmp.n > 0)
-            hipLaunchKernelGGL((k_guided_tiles<true, SEL_NONE>), g, dim3(kBlock), 0, s, A, tiles_x, tiles);
-        else if (sel.mode == SEL_MAPS)
-            hipLaunchKernelGGL((k_guided_tiles<false, SEL_MAPS>), g, dim3(kBlock), 0, s, A, tiles_x, tiles);
-        else
-            hipLaunchKernelGGL((k_guided_tiles<false, SEL_NONE>), g, dim3(kBlock), 0, s, A, tiles_x, tiles);
+#define LGCR_TILES(T, S_, P) hipLaunchKernelGGL((k_guided_tiles<T, S_, P>), g, dim3(kBlock), 0, s, A, tiles_x, tiles)
+        if (tmp.n > 0) {
+            if (plain_in_tiles)
+                LGCR_TILES(true, SEL_NONE, true);
+            else
+                LGCR_TILES(true, SEL_NONE, false);
+        } else if (sel.mode == SEL_MAPS) {
+            if (plain_in_tiles)
+                LGCR_TILES(false, SEL_MAPS, true);
+            else
+                LGCR_TILES(false, SEL_MAPS, false);
+        } else {
+            if (plain_in_tiles)
+                LGCR_TILES(false, SEL_NONE, true);
+            else
+                LGCR_TILES(false, SEL_NONE, false);
+        }
+#undef LGCR_TILES
         return hipGetLastError();
     }
     const size_t n = (size_t)ax.n * ay.n;

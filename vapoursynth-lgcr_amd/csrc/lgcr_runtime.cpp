@@ -191,6 +191,43 @@ std::vector<int> footprint_ranges(const HostAxis &h, int n, bool &ok) {
     return r;
 }
 
+// Bands of the fused trust-mask pass: cband chroma rows per band, the mask
+// rows partitioned into consecutive owned ranges, and the rows each band
+// reads (its owned rows and every output row of its chroma rows' ranges).
+struct TrustBands {
+    std::vector<int> band;  // 4 ints per band
+    int n = 0, cband = 1, max_rows = 0;
+};
+TrustBands trust_bands(const std::vector<int> &yr, int ch, int h) {
+    TrustBands t;
+    t.cband = std::max(1, (ch + 767) / 768);
+    t.n = (ch + t.cband - 1) / t.cband;
+    std::vector<int> own(size_t(t.n) + 1, 0);
+    for (int b = 1; b < t.n; ++b) {
+        const int cy0 = b * t.cband;
+        const int lo = yr[2 * cy0] <= yr[2 * cy0 + 1] ? yr[2 * cy0] : own[b - 1];
+        own[b] = std::min(std::max(lo, own[b - 1]), h);
+    }
+    own[t.n] = h;
+    t.band.resize(size_t(t.n) * 4);
+    for (int b = 0; b < t.n; ++b) {
+        int lo = own[b], hi = own[b + 1] - 1;
+        for (int cy = b * t.cband; cy < std::min(ch, (b + 1) * t.cband); ++cy)
+            if (yr[2 * cy] <= yr[2 * cy + 1]) {
+                lo = std::min(lo, yr[2 * cy]);
+                hi = std::max(hi, yr[2 * cy + 1]);
+            }
+        lo = std::max(lo, 0);
+        hi = std::min(std::max(hi, lo), h - 1);
+        t.band[4 * b] = own[b];
+        t.band[4 * b + 1] = own[b + 1];
+        t.band[4 * b + 2] = lo;
+        t.band[4 * b + 3] = hi;
+        t.max_rows = std::max(t.max_rows, hi - lo + 1);
+    }
+    return t;
+}
+
 // Per-output records of a support <= 6 axis (AxisDev::rec); empty otherwise.
 std::vector<float> pack_axis(const HostAxis &h) {
     std::vector<float> r;
@@ -210,6 +247,8 @@ std::vector<float> pack_axis(const HostAxis &h) {
                 bits |= 1u << t;
             d[8 + t] = h.w.empty() ? 0.0f : h.w[(size_t)o * h.sup + t];
         }
+        d[14] = h.wsum[o];
+        d[15] = h.wabs[o];
         d[0] = ibits(h.start[o]);
         d[1] = h.pos[o];
         d[2] = h.lpos[o];
@@ -441,6 +480,10 @@ struct Geometry {
     // footprint contains it (cm_ok: every range contiguous)
     const int2 *cm_x = nullptr, *cm_y = nullptr;
     bool cm_ok = false;
+    // bands of the fused trust-mask pass (launch_trust_mask_tiles)
+    const int4 *tb = nullptr;
+    int tb_n = 0, tb_c = 0, tb_rows = 0;
+    bool tb_ok = false;
 };
 
 struct Workspace {
@@ -558,7 +601,12 @@ lgcr_status get_geometry(lgcr_clip *c, double sx, double sy, Geometry **out) {
     }
     const size_t oxb = blob.add(xb), oxe = blob.add(xe), oyb = blob.add(yb), oye = blob.add(ye);
     bool cm_ok = true;
-    const size_t ocmx = blob.add(footprint_ranges(ax, c->cw, cm_ok)), ocmy = blob.add(footprint_ranges(ay, c->ch, cm_ok));
+    const std::vector<int> cmy = footprint_ranges(ay, c->ch, cm_ok);
+    const size_t ocmx = blob.add(footprint_ranges(ax, c->cw, cm_ok)), ocmy = blob.add(cmy);
+    TrustBands tbands;
+    if (cm_ok && c->same)
+        tbands = trust_bands(cmy, c->ch, c->oh);
+    const size_t otb = blob.add(tbands.band);
     std::vector<float> plx(c->cw), ply(c->ch);
     for (int x = 0; x < c->cw; ++x)
         plx[x] = float((x + 0.5) * c->rw - 0.5 + sx);
@@ -635,6 +683,12 @@ lgcr_status get_geometry(lgcr_clip *c, double sx, double sy, Geometry **out) {
     g->cm_x = reinterpret_cast<const int2 *>(base + ocmx);
     g->cm_y = reinterpret_cast<const int2 *>(base + ocmy);
     g->cm_ok = cm_ok;
+    g->tb = reinterpret_cast<const int4 *>(base + otb);
+    g->tb_n = tbands.n;
+    g->tb_c = tbands.cband;
+    g->tb_rows = tbands.max_rows;
+    g->tb_ok = cm_ok && c->same && tbands.n > 0 && tbands.n <= kMaxBands &&
+               tbands.cband <= kMaxCband && trust_fused_lds(c->sw, c->dil, tbands.max_rows, c->cw) <= 64 * 1024;
     g->xb = reinterpret_cast<const int *>(base + oxb);
     g->xe = reinterpret_cast<const int *>(base + oxe);
     g->yb = reinterpret_cast<const int *>(base + oyb);
@@ -941,12 +995,12 @@ void add_timing(lgcr_clip *c, const double *us, const uint64_t *n) {
 // chroma mask and the counters are built; otherwise also the compacted index
 // list the other passes visit.
 hipError_t trust_workset(lgcr_clip *c, Workspace *w, const Geometry &g, bool tiled, hipStream_t s) {
-    const bool fast = tiled && c->same && g.cm_ok && !c->capture && c->dil < 64;
+    const bool fast = tiled && g.tb_ok && !c->capture && c->dil < 64;
     w->has_ws = true;
     w->ws_indices = !fast;
     if (fast)
-        return launch_trust_mask_tiles(w->seed, w->mtmp, w->mask, c->sw, c->sh, c->dil, w->cmask, c->cw, c->ch,
-                                       g.cm_x, g.cm_y, w->counters, s);
+        return launch_trust_mask_tiles(w->seed, w->mask, c->sw, c->sh, c->dil, g.tb, g.tb_n, g.tb_c, g.tb_rows,
+                                       w->cmask, c->cw, c->ch, g.cm_x, g.cm_y, w->counters, s);
     return launch_trust_workset(w->seed, w->mtmp, w->mask, c->sw, c->sh, c->dil, w->row_cnt, w->row_off, w->indices,
                                 w->cmask, c->cw, c->ch, g.ax, g.ay, w->counters, s);
 }
@@ -1003,6 +1057,9 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
 
     w->has_ws = false;
     const bool guided = p.strength > 0.0;
+    const bool algo4 = p.algo == 4 && guided;
+    const int sel_mode = algo4 ? (is_float ? SEL_FLOAT : SEL_MAPS) : SEL_NONE;
+    const bool tiled = guided && p.algo != 6 && guided_tiled(g.tiles_fit, p.radial, p.cedge, g.ax, g.ay, sel_mode);
     if (guided) {
         // BuildGuideMaps (plugin.cpp:324-334, maps.cpp:252-276)
         const double thr = p.algo == 4 ? 0.25 * p.sigma : p.sigma;
@@ -1013,15 +1070,16 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
         HIP_TRY(mark(LGCR_STAGE_BUILD_GUIDE_MAPS));
         if (p.sparse) {
             // BuildTrustMask + sparse workset (plugin.cpp:335-355)
-            const int sel_mode = p.algo == 4 ? (is_float ? SEL_FLOAT : SEL_MAPS) : SEL_NONE;
-            HIP_TRY(trust_workset(c, w, g, p.algo != 6 && guided_tiled(g.tiles_fit, p.radial, p.cedge, g.ax, g.ay,
-                                                                        sel_mode), s));
+            HIP_TRY(trust_workset(c, w, g, tiled, s));
             HIP_TRY(mark(LGCR_STAGE_BUILD_TRUST_MASK));
         }
         if (p.ms > 0.0) {
             // BuildMutualGate (plugin.cpp:356-365)
+            // marked chroma count: one word, or one partial per band (trust_workset)
+            const bool bands = w->has_ws && !w->ws_indices;
             HIP_TRY(launch_mutual_gate(w->lc, w->u, w->v, c->cw, c->ch, w->has_ws ? w->cmask : nullptr,
-                                       w->counters, w->ms, s));
+                                       bands ? w->counters + kCntChromaWords : w->counters, bands ? g.tb_n : 1,
+                                       w->ms, s));
             HIP_TRY(mark(LGCR_STAGE_BUILD_MUTUAL_GATE));
         }
     }
@@ -1037,8 +1095,10 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
         return launch_plain_separable(w->u, w->v, c->cw, c->ch, g.ax, g.ay, w->hbuf, w->out_u, w->out_v,
                                       clamp_hull, margin, s);
     };
-    const bool algo4 = p.algo == 4 && guided;
-    if (!guided || w->has_ws || algo4 || p.algo == 6) {
+    // the tiled guided pass computes the plain base of its tiles itself
+    // unless the stage buffers are captured
+    const bool plain_in_tiles = tiled && (w->has_ws || algo4) && !c->capture;
+    if ((!guided || w->has_ws || algo4 || p.algo == 6) && !plain_in_tiles) {
         // BuildBaseChroma (plugin.cpp:396-402, 627-648)
         HIP_TRY(plain());
         if (c->capture && guided) {
@@ -1099,7 +1159,7 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
         }
         HIP_TRY(launch_guided(w->u, w->v, c->cw, c->ch, w->y, c->sw, c->sh, w->yt, w->luv, w->lc,
                               gate, g.ax, g.ay, g.rd, gp, w->row_off, idx, w->out_u, w->out_v, sel,
-                              TemporalDev{}, s, w->mask, c->sw, c->sh, g.tiles_fit));
+                              TemporalDev{}, s, w->mask, c->sw, c->sh, g.tiles_fit, plain_in_tiles));
         HIP_TRY(mark(LGCR_STAGE_APPLY_GUIDED_CORRECTION));
         if (algo4 && is_float) {
             HIP_TRY(launch_roi_mark(w->row_off, idx, c->ow, c->oh, sel, g.ax, g.ay, w->roi, c->cw, c->ch,
@@ -1252,7 +1312,7 @@ lgcr_status enqueue_trecon(lgcr_clip *c, Workspace *w, const Geometry &g, const 
         HIP_TRY(launch_sobel_tensor(w->y, c->sw, c->sh, w->yt, p.sparse ? w->seed : nullptr,
                                     float(p.sigma * p.sigma), s));
         if (p.ms > 0.0)
-            HIP_TRY(launch_mutual_gate(w->lc, w->u, w->v, c->cw, c->ch, nullptr, w->counters, w->ms, s));
+            HIP_TRY(launch_mutual_gate(w->lc, w->u, w->v, c->cw, c->ch, nullptr, w->counters, 1, w->ms, s));
         HIP_TRY(mark(LGCR_STAGE_BUILD_GUIDE_MAPS));
     }
     const double oscale = is_float ? 1.0 : double((1 << c->f.bits_per_sample) - 1);
@@ -1262,10 +1322,14 @@ lgcr_status enqueue_trecon(lgcr_clip *c, Workspace *w, const Geometry &g, const 
     const int clamp_hull = p.ar_margin >= 0.0;
     const float margin = float(std::max(0.0, p.ar_margin));
     w->has_ws = false;
+    const bool tiled = guided_tiled(g.tiles_fit, p.radial, p.cedge, g.ax, g.ay, SEL_NONE);
+    bool plain_in_tiles = false;
     if (p.sparse && guided) {
-        HIP_TRY(trust_workset(c, w, g, guided_tiled(g.tiles_fit, p.radial, p.cedge, g.ax, g.ay, SEL_NONE), s));
-        HIP_TRY(launch_plain_separable(w->u, w->v, c->cw, c->ch, g.ax, g.ay, w->hbuf, w->out_u, w->out_v, clamp_hull,
-                                       margin, s));
+        HIP_TRY(trust_workset(c, w, g, tiled, s));
+        plain_in_tiles = tiled && !c->capture;
+        if (!plain_in_tiles)
+            HIP_TRY(launch_plain_separable(w->u, w->v, c->cw, c->ch, g.ax, g.ay, w->hbuf, w->out_u, w->out_v,
+                                           clamp_hull, margin, s));
         HIP_TRY(mark(LGCR_STAGE_BUILD_BASE_CHROMA));
     }
     GuideParams gp;
@@ -1301,7 +1365,7 @@ lgcr_status enqueue_trecon(lgcr_clip *c, Workspace *w, const Geometry &g, const 
     HIP_TRY(launch_guided(w->u, w->v, c->cw, c->ch, w->y, c->sw, c->sh, w->yt, w->luv, w->lc,
                           gp.ms_on ? w->ms : nullptr, g.ax, g.ay, g.rd, gp, w->row_off,
                           w->has_ws ? w->indices : nullptr, w->out_u, w->out_v, SelectorDev{}, td, s, w->mask,
-                          c->sw, c->sh, g.tiles_fit));
+                          c->sw, c->sh, g.tiles_fit, plain_in_tiles));
     HIP_TRY(mark(LGCR_STAGE_APPLY_GUIDED_CORRECTION));
     // collaborative filter after the temporal pass (plugin.cpp:898-899)
     if (p.bm) {
@@ -1764,12 +1828,18 @@ lgcr_status lgcr_recon_frame(lgcr_clip *c, const lgcr_frame *in, double shift_x,
         stats->stage_pixels[LGCR_STAGE_CONVERT_OUTPUT] = 2 * no;
         stats->output_pixels = uint64_t(c->ow) * c->oh;
         if (w->has_ws) {
-            uint32_t cnt[3] = {0, 0, 0}, act = 0;
-            HIP_TRY(hipMemcpy(cnt, w->counters, sizeof(cnt), hipMemcpyDeviceToHost));
-            if (w->ws_indices)
+            uint32_t cnt[1] = {0}, act = 0;
+            if (w->ws_indices) {
+                HIP_TRY(hipMemcpy(cnt, w->counters, sizeof(cnt), hipMemcpyDeviceToHost));
                 HIP_TRY(hipMemcpy(&act, w->row_off + c->oh, sizeof(uint32_t), hipMemcpyDeviceToHost));
-            else
-                act = cnt[2];
+            } else {
+                std::vector<uint32_t> words(kCounterWords);
+                HIP_TRY(hipMemcpy(words.data(), w->counters, words.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+                for (int b = 0; b < g->tb_n; ++b) {
+                    cnt[0] += words[kCntChromaWords + b];
+                    act += words[kCntChromaWords + kMaxBands + b];
+                }
+            }
             stats->active_output_pixels = act;
             stats->active_chroma_pixels = cnt[0];
             stats->sparse_total_pixels = uint64_t(c->sw) * c->sh;
