@@ -20,17 +20,57 @@ METRIC = "Recon 1080p420 10-bit fps, HBM GB/s (% peak); serving p50/p99 frame la
 HBM_PEAK_GBS = 8000.0
 
 
-def measured_traffic(workload_key):
-    """HBM bytes per launch of the dominant kernel from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the same workload, or None."""
-    path = os.path.join(ROOT, "profiles", f"{workload_key}_hbm_traffic.md")
+VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9   # non-packed FP32 / FP64 VALU lane-ops per s
+
+
+def profile_figure(name, key):
+    """`key = N` from a committed rocprofv3 summary under profiles/, or None."""
+    path = os.path.join(ROOT, "profiles", name)
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_launch"):
+            if line.startswith(key):
                 return int(line.split("=")[1])
     except OSError:
         pass
     return None
+
+
+def measured_traffic(workload_key):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the same workload, or None."""
+    return profile_figure(f"{workload_key}_hbm_traffic.md", "traffic_bytes_per_launch")
+
+
+def measured_valu(workload_key):
+    """VALU lane-ops (SQ_INSTS_VALU x 64) per launch of the dominant kernel
+    from the committed rocprofv3 counter summary, or None."""
+    return profile_figure(f"{workload_key}_valu.md", "valu_lane_ops_per_launch")
+
+
+def host_path(clip, frames, n, threads):
+    """Frames/s through host planes (upload, Recon, download per frame;
+    pageable numpy buffers as the plugin hands them over), `threads` frames
+    in flight."""
+    import threading
+    nxt = [0]
+    lock = threading.Lock()
+
+    def work():
+        while True:
+            with lock:
+                k = nxt[0]
+                nxt[0] += 1
+            if k >= n:
+                return
+            clip.recon_host(*frames[k % len(frames)])
+
+    ts = [threading.Thread(target=work) for _ in range(threads)]
+    t0 = time.perf_counter()
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    return n / (time.perf_counter() - t0)
 
 
 def algorithmic_bytes(w, h, cw, ch, bps):
@@ -114,6 +154,7 @@ def main():
     ap.add_argument("--serve-frames", type=int, default=24)
     ap.add_argument("--serve-rate", type=float, default=600.0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-host-path", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -127,10 +168,11 @@ def main():
 
     W, H, bits = args.width, args.height, args.bits
     cw, ch = W // 2, H // 2
-    bps = 1 if bits <= 8 else 2
+    bps = 4 if bits == 32 else (1 if bits <= 8 else 2)   # bits 32: YUV420PS
     frames = synth.clip(args.frames, W, H, bits=bits, seed=0, scene_per_frame=True,
                         threads=min(16, os.cpu_count() or 1))
-    tdt = torch.uint8 if bps == 1 else torch.int16
+    tdt = {1: torch.uint8, 2: torch.int16, 4: torch.float32}[bps]
+    np_dt = {1: np.uint8, 2: np.uint16, 4: np.float32}[bps]
 
     def to_dev(a):
         return torch.from_numpy(a.view(np.int16) if bps == 2 else a).to("cuda")
@@ -143,7 +185,7 @@ def main():
 
     dev = abi.Device(0)
     params = abi.default_params(algo=args.algo, bp=args.bp, strength=args.strength)
-    clip = abi.Clip(dev, params, abi.make_format(np.uint8 if bps == 1 else np.uint16, W, H, bits=bits),
+    clip = abi.Clip(dev, params, abi.make_format(np_dt, W, H, bits=None if bits == 32 else bits),
                     max_inflight=args.inflight)
     for _ in range(args.warmup):
         clip.recon_batch(fin, fout)
@@ -171,9 +213,14 @@ def main():
     achieved = per_frame_bytes / (dom_avg_us * 1e-6) / 1e9
     stage_avg = {k: round(v[0] / max(1, v[1]), 2) for k, v in stages.items()}
 
-    traffic = None
+    traffic = valu = None
     if (W, H, bits, args.algo) == (1920, 1080, 10, 2) and dom_name == "ApplyGuidedCorrection":
         traffic = measured_traffic("recon_1080p420p10_algo2")
+        valu = measured_valu("recon_1080p420p10_algo2")
+    # attainable time of the dominant kernel: the larger of its HBM time and
+    # its VALU time at the non-packed lane-op peak
+    t_min = max(per_frame_bytes / (HBM_PEAK_GBS * 1e9), (valu or 0) / VALU_PEAK_LANE_OPS)
+    fmt_name = "PS" if bits == 32 else f"P{bits}"
     result = {
         "metric": METRIC,
         "value": round(fps, 2),
@@ -187,7 +234,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 (f64 accumulators)",
         "data": "synthetic (polygon scenes, seed 0, one scene per frame)",
-        "config": {"workload": f"Recon {W}x{H} YUV420P{bits} -> YUV444P{bits}, algo {args.algo}, "
+        "config": {"workload": f"Recon {W}x{H} YUV420{fmt_name} -> YUV444{fmt_name}, algo {args.algo}, "
                                f"lanczos3, strength {args.strength}, sparse 1, bp {args.bp}",
                    "frames_per_step": args.frames, "inflight_workspaces": args.inflight},
         "end_to_end_hbm_gbs": round(per_frame_bytes * fps / 1e9, 2),
@@ -203,9 +250,21 @@ def main():
             "traffic": traffic,
             "bytes_per_launch": per_frame_bytes,
             "avg_launch_us": round(dom_avg_us, 2),
+            "valu_lane_ops_per_launch": valu,
+            "valu_peak_lane_ops_per_s": VALU_PEAK_LANE_OPS,
+            "attainable_frac": round(t_min / (dom_avg_us * 1e-6), 4) if valu else None,
         },
     }
     clip.close()
+    if not args.no_host_path and bits != 32:
+        hclip = abi.Clip(dev, params, abi.make_format(np_dt, W, H, bits=bits), max_inflight=args.inflight)
+        host_path(hclip, frames, 8, args.inflight)  # warm-up
+        n_host = min(len(frames), 96)
+        result["host_buffers"] = {
+            "fps": round(host_path(hclip, frames, n_host, args.inflight), 1), "frames": n_host,
+            "threads": args.inflight,
+            "note": "host planes in and out per frame (PCIe-inclusive; the VapourSynth plugin boundary)"}
+        hclip.close()
     dev.close()
     del dev_in, dev_out, fin, fout
     torch.cuda.empty_cache()

@@ -525,6 +525,8 @@ struct Workspace {
     std::vector<std::pair<int, hipEvent_t>> marks;
     std::vector<hipEvent_t> pool;
     hipEvent_t done = nullptr;  // completion of a submitted frame
+    uint8_t *pin = nullptr;  // pinned staging for host planes (allocated on first use)
+    size_t pin_bytes = 0;
     bool has_ws = false;  // last frame built a sparse workset
     bool ws_indices = false;  // ... with the compacted index list
 };
@@ -900,7 +902,74 @@ void destroy_workspace(lgcr_clip *c, Workspace *w) {
     }
     if (w->nbr_raw)
         hipFree(w->nbr_raw);
+    if (w->pin)
+        hipHostFree(w->pin);
 }
+
+// Host planes go through the workspace's pinned staging buffer: the
+// caller's rows (pageable memory, any stride) are copied into it on the CPU
+// and each plane moves with one contiguous DMA; downloads land there and are
+// copied out to the caller's rows after the stream has drained.
+struct HostXfer {
+    Workspace *w;
+    hipStream_t s;
+    size_t cur = 0;
+    struct Pending {
+        const uint8_t *pin;
+        void *dst;
+        ptrdiff_t stride;
+        size_t rb;
+        int rows;
+    };
+    std::vector<Pending> down;
+
+    static size_t span(size_t rb, int rows) { return (rb * size_t(rows) + 255) & ~size_t(255); }
+
+    // reserve staging for `bytes` (the sum of span() of every plane moved)
+    hipError_t reserve(size_t bytes) {
+        if (w->pin_bytes >= bytes)
+            return hipSuccess;
+        if (w->pin)
+            hipHostFree(w->pin);
+        w->pin = nullptr;
+        w->pin_bytes = 0;
+        const hipError_t e = hipHostMalloc(reinterpret_cast<void **>(&w->pin), bytes, hipHostMallocDefault);
+        if (e == hipSuccess)
+            w->pin_bytes = bytes;
+        return e;
+    }
+    hipError_t in(void *dev, const void *src, ptrdiff_t stride, size_t rb, int rows) {
+        uint8_t *p = w->pin + cur;
+        cur += span(rb, rows);
+        const uint8_t *q = static_cast<const uint8_t *>(src);
+        if (stride == ptrdiff_t(rb)) {
+            std::memcpy(p, q, rb * size_t(rows));
+        } else {
+            for (int r = 0; r < rows; ++r)
+                std::memcpy(p + rb * size_t(r), q + stride * r, rb);
+        }
+        return hipMemcpyAsync(dev, p, rb * size_t(rows), hipMemcpyHostToDevice, s);
+    }
+    hipError_t out(void *dst, ptrdiff_t stride, const void *dev, size_t rb, int rows) {
+        uint8_t *p = w->pin + cur;
+        cur += span(rb, rows);
+        down.push_back({p, dst, stride, rb, rows});
+        return hipMemcpyAsync(p, dev, rb * size_t(rows), hipMemcpyDeviceToHost, s);
+    }
+    // after the stream has drained
+    void finish() {
+        for (const Pending &d : down) {
+            uint8_t *q = static_cast<uint8_t *>(d.dst);
+            if (d.stride == ptrdiff_t(d.rb)) {
+                std::memcpy(q, d.pin, d.rb * size_t(d.rows));
+            } else {
+                for (int r = 0; r < d.rows; ++r)
+                    std::memcpy(q + d.stride * r, d.pin + d.rb * size_t(r), d.rb);
+            }
+        }
+        down.clear();
+    }
+};
 
 lgcr_status acquire(lgcr_clip *c, int *slot) {
     std::unique_lock<std::mutex> lock(c->ws_mutex);
@@ -1646,11 +1715,18 @@ lgcr_status lgcr_trecon_frame(lgcr_clip *c, const lgcr_frame *in, const lgcr_fra
     ptrdiff_t dos[3];
     std::vector<const void *> nd(size_t(3) * n_nbrs);
     std::vector<ptrdiff_t> ns(size_t(3) * n_nbrs);
+    HostXfer hx{w, w->stream};
+    if (host) {
+        size_t bytes = 0;
+        for (int i = 0; i < 3; ++i)
+            bytes += HostXfer::span(size_t(pw[i]) * c->bps, ph[i]) * size_t(1 + n_nbrs) +
+                     HostXfer::span(size_t(c->ow) * c->bps, c->oh);
+        HIP_TRY(hx.reserve(bytes));
+    }
     for (int i = 0; i < 3; ++i) {
         const size_t rb = size_t(pw[i]) * c->bps;
         if (host) {
-            HIP_TRY(hipMemcpy2DAsync(w->raw_in[i], rb, in->data[i], in->stride[i], rb, ph[i], hipMemcpyHostToDevice,
-                                     w->stream));
+            HIP_TRY(hx.in(w->raw_in[i], in->data[i], in->stride[i], rb, ph[i]));
             din[i] = w->raw_in[i];
             dis[i] = ptrdiff_t(rb);
             dout[i] = w->raw_out[i];
@@ -1672,8 +1748,7 @@ lgcr_status lgcr_trecon_frame(lgcr_clip *c, const lgcr_frame *in, const lgcr_fra
             const size_t rb = size_t(pw[i]) * c->bps;
             if (host) {
                 uint8_t *dst = static_cast<uint8_t *>(w->nbr_raw) + off;
-                HIP_TRY(hipMemcpy2DAsync(dst, rb, nbrs[k].data[i], nbrs[k].stride[i], rb, ph[i], hipMemcpyHostToDevice,
-                                         w->stream));
+                HIP_TRY(hx.in(dst, nbrs[k].data[i], nbrs[k].stride[i], rb, ph[i]));
                 nd[3 * k + i] = dst;
                 ns[3 * k + i] = ptrdiff_t(rb);
                 off += rb * ph[i];
@@ -1689,12 +1764,10 @@ lgcr_status lgcr_trecon_frame(lgcr_clip *c, const lgcr_frame *in, const lgcr_fra
     if (st != LGCR_OK)
         return st;
     if (host)
-        for (int i = 0; i < 3; ++i) {
-            const size_t rb = size_t(c->ow) * c->bps;
-            HIP_TRY(hipMemcpy2DAsync(out->data[i], out->stride[i], w->raw_out[i], rb, rb, c->oh, hipMemcpyDeviceToHost,
-                                     w->stream));
-        }
+        for (int i = 0; i < 3; ++i)
+            HIP_TRY(hx.out(out->data[i], out->stride[i], w->raw_out[i], size_t(c->ow) * c->bps, c->oh));
     HIP_TRY(hipStreamSynchronize(w->stream));
+    hx.finish();
     double us[LGCR_STAGE_COUNT] = {};
     uint64_t nst[LGCR_STAGE_COUNT] = {};
     if (timed) {
@@ -1768,11 +1841,17 @@ lgcr_status lgcr_recon_frame(lgcr_clip *c, const lgcr_frame *in, double shift_x,
     void *dout[3];
     ptrdiff_t dout_stride[3];
     const bool host = in->memory == LGCR_MEMORY_HOST;
+    HostXfer hx{w, w->stream};
+    if (host) {
+        size_t bytes = 0;
+        for (int i = 0; i < 3; ++i)
+            bytes += HostXfer::span(size_t(pw[i]) * c->bps, ph[i]) + HostXfer::span(size_t(c->ow) * c->bps, c->oh);
+        HIP_TRY(hx.reserve(bytes));
+    }
     for (int i = 0; i < 3; ++i) {
         if (host) {
             const size_t rb = size_t(pw[i]) * c->bps;
-            HIP_TRY(hipMemcpy2DAsync(w->raw_in[i], rb, in->data[i], in->stride[i], rb, ph[i], hipMemcpyHostToDevice,
-                                     w->stream));
+            HIP_TRY(hx.in(w->raw_in[i], in->data[i], in->stride[i], rb, ph[i]));
             din[i] = w->raw_in[i];
             din_stride[i] = ptrdiff_t(rb);
             dout[i] = w->raw_out[i];
@@ -1793,11 +1872,8 @@ lgcr_status lgcr_recon_frame(lgcr_clip *c, const lgcr_frame *in, double shift_x,
     if (st != LGCR_OK)
         return st;
     if (host) {
-        for (int i = c->same ? 1 : 0; i < 3; ++i) {
-            const size_t rb = size_t(c->ow) * c->bps;
-            HIP_TRY(hipMemcpy2DAsync(out->data[i], out->stride[i], w->raw_out[i], rb, rb, c->oh,
-                                     hipMemcpyDeviceToHost, w->stream));
-        }
+        for (int i = c->same ? 1 : 0; i < 3; ++i)
+            HIP_TRY(hx.out(out->data[i], out->stride[i], w->raw_out[i], size_t(c->ow) * c->bps, c->oh));
         if (c->same && out->data[0] && out->data[0] != in->data[0]) {
             const size_t rb = size_t(c->sw) * c->bps;
             for (int yy = 0; yy < c->sh; ++yy)
@@ -1810,6 +1886,7 @@ lgcr_status lgcr_recon_frame(lgcr_clip *c, const lgcr_frame *in, double shift_x,
                                  hipMemcpyDeviceToDevice, w->stream));
     }
     HIP_TRY(hipStreamSynchronize(w->stream));
+    hx.finish();
     double us[LGCR_STAGE_COUNT] = {};
     uint64_t nst[LGCR_STAGE_COUNT] = {};
     if (timed) {
@@ -2210,11 +2287,17 @@ lgcr_status lgcr_sharpen_frame(lgcr_clip *c, const lgcr_frame *in, lgcr_frame *o
     void *dout[3];
     ptrdiff_t dout_stride[3];
     const bool host = in->memory == LGCR_MEMORY_HOST;
+    HostXfer hx{w, w->stream};
+    if (host) {
+        size_t bytes = 0;
+        for (int i = 0; i < 3; ++i)
+            bytes += 2 * HostXfer::span(size_t(pw[i]) * c->bps, ph[i]);
+        HIP_TRY(hx.reserve(bytes));
+    }
     for (int i = 0; i < 3; ++i) {
         const size_t rb = size_t(pw[i]) * c->bps;
         if (host) {
-            HIP_TRY(hipMemcpy2DAsync(w->raw_in[i], rb, in->data[i], in->stride[i], rb, ph[i], hipMemcpyHostToDevice,
-                                     w->stream));
+            HIP_TRY(hx.in(w->raw_in[i], in->data[i], in->stride[i], rb, ph[i]));
             din[i] = w->raw_in[i];
             din_stride[i] = ptrdiff_t(rb);
             dout[i] = w->raw_out[i];
@@ -2230,12 +2313,10 @@ lgcr_status lgcr_sharpen_frame(lgcr_clip *c, const lgcr_frame *in, lgcr_frame *o
     if (st != LGCR_OK)
         return st;
     if (host)
-        for (int i = 0; i < 3; ++i) {
-            const size_t rb = size_t(pw[i]) * c->bps;
-            HIP_TRY(hipMemcpy2DAsync(out->data[i], out->stride[i], w->raw_out[i], rb, rb, ph[i],
-                                     hipMemcpyDeviceToHost, w->stream));
-        }
+        for (int i = 0; i < 3; ++i)
+            HIP_TRY(hx.out(out->data[i], out->stride[i], w->raw_out[i], size_t(pw[i]) * c->bps, ph[i]));
     HIP_TRY(hipStreamSynchronize(w->stream));
+    hx.finish();
     return LGCR_OK;
 }
 
