@@ -227,6 +227,15 @@ lgcr_status lgcr_sharpen_frame(lgcr_clip *clip, const lgcr_frame *in, lgcr_frame
 lgcr_status lgcr_clip_set_timing(lgcr_clip *clip, int enable);
 lgcr_status lgcr_clip_stage_times(lgcr_clip *clip, double *us, uint64_t *launches);
 
+/* Steady-state frame graphs (off by default).  When enabled, device-resident
+ * Recon frames (lgcr_recon_frame, lgcr_recon_batch, lgcr_recon_submit)
+ * replay a hipGraph of the frame's kernels captured once per workspace and
+ * geometry; the frames' plane pointers reach it through a per-workspace
+ * table in device memory.  Stage timing and stage capture use the direct
+ * launches.  No counterpart in the reference (its frames run stage by stage
+ * on the host). */
+lgcr_status lgcr_clip_set_graphs(lgcr_clip *clip, int enable);
+
 /* Intermediates of the last frame processed on workspace 0 of a clip, for
  * parity checks against the stage functions of src/lgcr.h:555-962.
  * Copies up to `bytes` into host memory `dst` and stores the full size in

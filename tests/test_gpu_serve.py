@@ -13,26 +13,31 @@ pytestmark = pytest.mark.gpu
 SMALL = [(320, 180), (256, 144), (384, 216)]
 
 
-def _mix(device, seed, n_clips=6, frames=5):
+def _mix(device, seed, n_clips=6, frames=5, graphs=False):
     specs = serve.make_mix(n_clips, frames, seed=seed, resolutions=SMALL)
-    return specs, [serve.ResidentClip(device, s, max_inflight=2, threads=4) for s in specs]
+    return specs, [serve.ResidentClip(device, s, max_inflight=2, threads=4, graphs=graphs) for s in specs]
 
 
 def _alone(clip, n):
-    """Serve frame n of `clip` by itself on fresh backend state."""
+    """Serve frame n of `clip` by itself on fresh backend state, with direct
+    kernel launches (no frame graph)."""
     clip.release()
     assert clip.admit()
+    clip.clip.set_graphs(False)
     clip.run_blocking(n)
     out = clip.output_host(0, n)
     clip.release()
     return out
 
 
+@pytest.mark.parametrize("capture", [1, 0], ids=["capture", "graphs"])
 @pytest.mark.parametrize("budget_mb", [0, 24])
-def test_mix_equals_alone(device, budget_mb):
+def test_mix_equals_alone(device, monkeypatch, budget_mb, capture):
+    # with stage capture off the mix's Recon frames replay frame graphs
+    monkeypatch.setenv("LGCR_CAPTURE", str(capture))
     dev = abi.Device(0, budget_mb << 20) if budget_mb else device
     try:
-        specs, clips = _mix(dev, seed=5)
+        specs, clips = _mix(dev, seed=5, graphs=not capture)
         reqs = serve.poisson_requests(specs, 600.0, seed=5)
         got = {}
         st = serve.serve(clips, reqs, mode="continuous", keep=got)

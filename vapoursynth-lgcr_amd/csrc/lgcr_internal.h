@@ -93,12 +93,31 @@ struct SelectorDev {
 // Sample formats of raw planes.
 enum RawType : int { RAW_U8 = 0, RAW_U16 = 1, RAW_F32 = 2 };
 
+// Frame planes of a captured frame graph, read on the device: the graph's
+// conversion kernels take plane `slot` of entry `counter` and the graph's
+// last node advances `counter`, so one graph replays over a table of frames
+// queued in stream order on one workspace.
+struct IoSlot {
+    const void *in[3];
+    ptrdiff_t in_stride[3];
+    void *out[3];
+    ptrdiff_t out_stride[3];
+};
+constexpr int kIoSlots = 1024;
+struct IoTable {
+    uint32_t counter;
+    uint32_t pad[63];
+    IoSlot slot[kIoSlots];
+};
+
 struct RawPlane {
     const void *data;
     ptrdiff_t stride;   // bytes
     int w, h;
     float *dst;         // converted f32, stride w
     double scale, offset;
+    const IoTable *io;  // non-null: data/stride are in[slot] of io's current entry
+    int slot;
 };
 
 struct RawOut {
@@ -108,6 +127,8 @@ struct RawOut {
     int w, h;
     float scale, offset;
     int lo, hi;
+    const IoTable *io;  // non-null: data/stride are out[slot] of io's current entry
+    int slot;
 };
 
 // Kernel launchers (lgcr_kernels.hip).  All are asynchronous on `s`.
@@ -231,6 +252,8 @@ hipError_t launch_collaborative(const float *guide, int gw, int gh, float *u, fl
 // 4:2:0); keep_float also stores the corrected float planes.
 hipError_t launch_bp_pack(float *out_u, float *out_v, int ow, int oh, const float *u, const float *v, int cw,
                           int ch, float bp, const RawOut *planes, int type, int keep_float, hipStream_t s);
+// io->counter += 1 (the last node of a frame graph)
+hipError_t launch_io_advance(IoTable *io, hipStream_t s);
 hipError_t launch_back_project(float *out_u, float *out_v, int ow, int oh, const float *u,
                                const float *v, int cw, int ch, float bp, hipStream_t s);
 hipError_t launch_resample_h(const float *src, int sw, int sh, float *dst,

@@ -161,6 +161,32 @@ __device__ Dir principal_dir(float jxx, float jxy, float jyy) {
 struct InBatch { RawPlane p[3]; };
 struct OutBatch { RawOut p[3]; };
 
+// plane address and stride of a conversion: by value, or from the frame
+// table of a captured frame graph
+__device__ __forceinline__ const char *plane_in(const RawPlane &P, ptrdiff_t &stride) {
+    if (P.io) {
+        const IoSlot &e = P.io->slot[P.io->counter];
+        stride = e.in_stride[P.slot];
+        return static_cast<const char *>(e.in[P.slot]);
+    }
+    stride = P.stride;
+    return static_cast<const char *>(P.data);
+}
+__device__ __forceinline__ char *plane_out(const RawOut &P, ptrdiff_t &stride) {
+    if (P.io) {
+        const IoSlot &e = P.io->slot[P.io->counter];
+        stride = e.out_stride[P.slot];
+        return static_cast<char *>(e.out[P.slot]);
+    }
+    stride = P.stride;
+    return static_cast<char *>(P.data);
+}
+
+__global__ void k_io_advance(IoTable *io) {
+    if (threadIdx.x == 0)
+        io->counter += 1;
+}
+
 // One block per row and plane; each thread converts columns x, x+256, ...
 // four at a time so several loads are in flight per lane.
 constexpr int kConvUnroll = 4;
@@ -171,7 +197,9 @@ __global__ __launch_bounds__(kBlock) void k_to_float(InBatch b) {
     const int y = blockIdx.y;
     if (y >= P.h)
         return;
-    const T *row = reinterpret_cast<const T *>(static_cast<const char *>(P.data) + P.stride * y);
+    ptrdiff_t stride;
+    const char *base = plane_in(P, stride);
+    const T *row = reinterpret_cast<const T *>(base + stride * y);
     float *dst = P.dst + (size_t)y * P.w;
     const double scale = P.scale, offset = P.offset;
     for (int x0 = threadIdx.x; x0 < P.w; x0 += kBlock * kConvUnroll) {
@@ -195,7 +223,9 @@ __global__ __launch_bounds__(kBlock) void k_copy_f32(InBatch b) {
     const int y = blockIdx.y;
     if (y >= P.h)
         return;
-    const float *row = reinterpret_cast<const float *>(static_cast<const char *>(P.data) + P.stride * y);
+    ptrdiff_t stride;
+    const char *base = plane_in(P, stride);
+    const float *row = reinterpret_cast<const float *>(base + stride * y);
     float *dst = P.dst + (size_t)y * P.w;
     for (int x0 = threadIdx.x; x0 < P.w; x0 += kBlock * kConvUnroll) {
         float v[kConvUnroll];
@@ -224,7 +254,9 @@ __global__ __launch_bounds__(kBlock) void k_from_float(OutBatch b) {
     const int y = blockIdx.y;
     if (y >= P.h)
         return;
-    T *row = reinterpret_cast<T *>(static_cast<char *>(P.data) + P.stride * y);
+    ptrdiff_t stride;
+    char *base = plane_out(P, stride);
+    T *row = reinterpret_cast<T *>(base + stride * y);
     const float *src = P.src + (size_t)y * P.w;
     for (int x0 = threadIdx.x; x0 < P.w; x0 += kBlock * kConvUnroll) {
         float v[kConvUnroll];
@@ -247,7 +279,9 @@ __global__ __launch_bounds__(kBlock) void k_from_float_f32(OutBatch b) {
     const int y = blockIdx.y;
     if (y >= P.h)
         return;
-    float *row = reinterpret_cast<float *>(static_cast<char *>(P.data) + P.stride * y);
+    ptrdiff_t stride;
+    char *base = plane_out(P, stride);
+    float *row = reinterpret_cast<float *>(base + stride * y);
     const float *src = P.src + (size_t)y * P.w;
     for (int x0 = threadIdx.x; x0 < P.w; x0 += kBlock * kConvUnroll) {
         float v[kConvUnroll];
@@ -293,8 +327,10 @@ __global__ __launch_bounds__(kBlock) void k_bp_pack(float *ou, float *ov, int ow
             o[(size_t)yb * ow + xa] = q01;
             o[(size_t)yb * ow + xb] = q11;
         }
-        T *ra = reinterpret_cast<T *>(static_cast<char *>(P.data) + P.stride * ya);
-        T *rb = reinterpret_cast<T *>(static_cast<char *>(P.data) + P.stride * yb);
+        ptrdiff_t stride;
+        char *base = plane_out(P, stride);
+        T *ra = reinterpret_cast<T *>(base + stride * ya);
+        T *rb = reinterpret_cast<T *>(base + stride * yb);
         if constexpr (sizeof(T) == 4) {
             ra[xa] = q00; ra[xb] = q10; rb[xa] = q01; rb[xb] = q11;
         } else {
@@ -3097,6 +3133,11 @@ hipError_t launch_bp_pack(float *out_u, float *out_v, int ow, int oh, const floa
     else
         hipLaunchKernelGGL(k_bp_pack<float>, g, dim3(kBlock), 0, s, out_u, out_v, ow, oh, u, v, cw, ch, bp, b,
                            keep_float);
+    return hipGetLastError();
+}
+
+hipError_t launch_io_advance(IoTable *io, hipStream_t s) {
+    hipLaunchKernelGGL(k_io_advance, dim3(1), dim3(64), 0, s, io);
     return hipGetLastError();
 }
 

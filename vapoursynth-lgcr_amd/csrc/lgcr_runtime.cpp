@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstddef>
 #include <functional>
 #include <atomic>
 #include <cmath>
@@ -527,6 +528,12 @@ struct Workspace {
     hipEvent_t done = nullptr;  // completion of a submitted frame
     uint8_t *pin = nullptr;  // pinned staging for host planes (allocated on first use)
     size_t pin_bytes = 0;
+    // captured steady-state frame graph (device-resident Recon frames) and
+    // the frame table its conversion kernels read
+    IoTable *io_dev = nullptr;
+    IoTable *io_host = nullptr;  // pinned
+    hipGraphExec_t gexec = nullptr;
+    const void *gkey = nullptr;  // geometry the graph was captured for
     bool has_ws = false;  // last frame built a sparse workset
     bool ws_indices = false;  // ... with the compacted index list
 };
@@ -543,6 +550,7 @@ struct lgcr_clip {
     int dil = 0;
     bool same = true;
     bool capture = false;
+    std::atomic<bool> graphs{false};  // replay captured frame graphs for device-resident Recon frames
     bool temporal = false;  // TRecon clip (neighbour buffers)
     bool sharpen = false;   // Sharpen clip
     lgcr_sharpen_params sp{};
@@ -904,6 +912,12 @@ void destroy_workspace(lgcr_clip *c, Workspace *w) {
         hipFree(w->nbr_raw);
     if (w->pin)
         hipHostFree(w->pin);
+    if (w->gexec)
+        hipGraphExecDestroy(w->gexec);
+    if (w->io_dev)
+        hipFree(w->io_dev);
+    if (w->io_host)
+        hipHostFree(w->io_host);
 }
 
 // Host planes go through the workspace's pinned staging buffer: the
@@ -1076,7 +1090,7 @@ hipError_t trust_workset(lgcr_clip *c, Workspace *w, const Geometry &g, bool til
 
 lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const void *const in[3],
                           const ptrdiff_t in_stride[3], void *const out[3], const ptrdiff_t out_stride[3],
-                          bool timed) {
+                          bool timed, IoTable *io = nullptr) {
     const lgcr_params &p = c->p;
     hipStream_t s = w->stream;
     auto mark = [&](int stage) -> hipError_t {
@@ -1101,9 +1115,9 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
     HIP_TRY(mark(-1));
     // ConvertInput (plugin.cpp:210-253)
     RawPlane rp[3] = {
-        {in[0], in_stride[0], c->sw, c->sh, w->y, yscale, 0.0},
-        {in[1], in_stride[1], c->cw, c->ch, w->u, yscale, coff},
-        {in[2], in_stride[2], c->cw, c->ch, w->v, yscale, coff},
+        {in[0], in_stride[0], c->sw, c->sh, w->y, yscale, 0.0, io, 0},
+        {in[1], in_stride[1], c->cw, c->ch, w->u, yscale, coff, io, 1},
+        {in[2], in_stride[2], c->cw, c->ch, w->v, yscale, coff, io, 2},
     };
     HIP_TRY(launch_to_float(rp, 3, c->raw_type, s));
     HIP_TRY(mark(LGCR_STAGE_CONVERT_INPUT));
@@ -1119,7 +1133,7 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
             HIP_TRY(launch_resample_h(w->y, c->sw, c->sh, w->luma_tmp, c->wh, s));
             HIP_TRY(launch_resample_v(w->luma_tmp, c->ow, c->sh, w->luma_out, c->wv, s));
         }
-        RawOut ro{w->luma_out, out[0], out_stride[0], c->ow, c->oh, float(oscale), 0.0f, 0, hi};
+        RawOut ro{w->luma_out, out[0], out_stride[0], c->ow, c->oh, float(oscale), 0.0f, 0, hi, io, 0};
         HIP_TRY(launch_from_float(&ro, 1, c->raw_type, s));
         HIP_TRY(mark(LGCR_STAGE_RESAMPLE_LUMA));
     }
@@ -1304,8 +1318,8 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
     // with ConvertOutput (plugin.cpp:691-711)
     const float offs = is_float ? 0.0f : float(0.5 * oscale);
     RawOut ro[2] = {
-        {w->out_u, out[1], out_stride[1], c->ow, c->oh, float(oscale), offs, 0, hi},
-        {w->out_v, out[2], out_stride[2], c->ow, c->oh, float(oscale), offs, 0, hi},
+        {w->out_u, out[1], out_stride[1], c->ow, c->oh, float(oscale), offs, 0, hi, io, 1},
+        {w->out_v, out[2], out_stride[2], c->ow, c->oh, float(oscale), offs, 0, hi, io, 2},
     };
     if (p.bp > 0.0 && c->same && c->cw * 2 == c->sw && c->ch * 2 == c->sh) {
         HIP_TRY(launch_bp_pack(w->out_u, w->out_v, c->ow, c->oh, w->u, w->v, c->cw, c->ch, float(p.bp), ro,
@@ -1315,6 +1329,72 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
         HIP_TRY(launch_from_float(ro, 2, c->raw_type, s));
     }
     HIP_TRY(mark(LGCR_STAGE_CONVERT_OUTPUT));
+    if (io)
+        HIP_TRY(launch_io_advance(io, s));
+    return LGCR_OK;
+}
+
+// Device-resident Recon frames through the workspace's captured frame graph:
+// the frames' plane pointers go into the workspace's frame table (one
+// upload, ordered on the stream before the replays), the graph is captured
+// on first use for this geometry, then replayed once per frame; each replay
+// reads the table entry its predecessor advanced to.  Every frame of one
+// call must be queued on this workspace and the previous call's replays
+// must have completed (the callers synchronise the stream at their end).
+// Not used while stage timing or stage capture is on.
+lgcr_status enqueue_frames_graph(lgcr_clip *c, Workspace *w, const Geometry &g, const lgcr_frame *in,
+                                 const lgcr_frame *out, const int *idx, int m) {
+    hipStream_t s = w->stream;
+    if (m > kIoSlots)
+        return fail(LGCR_INVALID_ARGUMENT, "enqueue_frames_graph: %d frames exceed the frame table", m);
+    const size_t head = offsetof(IoTable, slot);
+    if (!w->io_dev) {
+        HIP_TRY(hipMalloc(reinterpret_cast<void **>(&w->io_dev), sizeof(IoTable)));
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&w->io_host), sizeof(IoTable), hipHostMallocDefault));
+    }
+    IoTable *t = w->io_host;
+    t->counter = 0;
+    for (int k = 0; k < m; ++k) {
+        const lgcr_frame &fi = in[idx[k]], &fo = out[idx[k]];
+        IoSlot &e = t->slot[k];
+        for (int i = 0; i < 3; ++i) {
+            e.in[i] = fi.data[i];
+            e.in_stride[i] = fi.stride[i];
+            e.out[i] = fo.data[i];
+            e.out_stride[i] = fo.stride[i];
+        }
+    }
+    HIP_TRY(hipMemcpyAsync(w->io_dev, t, head + sizeof(IoSlot) * size_t(m), hipMemcpyHostToDevice, s));
+    if (!w->gexec || w->gkey != &g) {
+        if (w->gexec) {
+            HIP_TRY(hipGraphExecDestroy(w->gexec));
+            w->gexec = nullptr;
+        }
+        const void *din[3] = {nullptr, nullptr, nullptr};
+        const ptrdiff_t dst[3] = {0, 0, 0};
+        void *dout[3] = {nullptr, nullptr, nullptr};
+        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const lgcr_status st = enqueue_frame(c, w, g, din, dst, dout, dst, false, w->io_dev);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &graph);
+        if (st != LGCR_OK) {
+            if (graph)
+                hipGraphDestroy(graph);
+            return st;
+        }
+        HIP_TRY(e);
+        const hipError_t ei = hipGraphInstantiate(&w->gexec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        HIP_TRY(ei);
+        w->gkey = &g;
+    }
+    for (int k = 0; k < m; ++k) {
+        HIP_TRY(hipGraphLaunch(w->gexec, s));
+        const lgcr_frame &fi = in[idx[k]], &fo = out[idx[k]];
+        if (c->same && fo.data[0] && fo.data[0] != fi.data[0])
+            HIP_TRY(hipMemcpy2DAsync(fo.data[0], fo.stride[0], fi.data[0], fi.stride[0], size_t(c->sw) * c->bps, c->sh,
+                                     hipMemcpyDeviceToDevice, s));
+    }
     return LGCR_OK;
 }
 
@@ -1868,6 +1948,14 @@ lgcr_status lgcr_recon_frame(lgcr_clip *c, const lgcr_frame *in, double shift_x,
     }
     const bool timed = stats != nullptr || c->timing.load();
     w->marks.clear();
+    if (!host && !timed && !c->capture && c->graphs.load()) {
+        const int idx0 = 0;
+        st = enqueue_frames_graph(c, w, *g, in, out, &idx0, 1);
+        if (st != LGCR_OK)
+            return st;
+        HIP_TRY(hipStreamSynchronize(w->stream));
+        return LGCR_OK;
+    }
     st = enqueue_frame(c, w, *g, din, din_stride, dout, dout_stride, timed);
     if (st != LGCR_OK)
         return st;
@@ -1967,7 +2055,16 @@ lgcr_status lgcr_recon_batch(lgcr_clip *c, int n, const lgcr_frame *in, lgcr_fra
     const bool timed = c->timing.load();
     for (int s : slots)
         c->spaces[s]->marks.clear();
-    for (int k = 0; k < n && result == LGCR_OK; ++k) {
+    const int ns = (int)slots.size();
+    const bool graphs = c->graphs.load() && !timed && !c->capture && (n + ns - 1) / ns <= kIoSlots;
+    for (int j = 0; graphs && j < ns && result == LGCR_OK; ++j) {
+        std::vector<int> idx;
+        for (int k = j; k < n; k += ns)
+            idx.push_back(k);
+        if (!idx.empty())
+            result = enqueue_frames_graph(c, c->spaces[slots[j]].get(), *g, in, out, idx.data(), (int)idx.size());
+    }
+    for (int k = 0; !graphs && k < n && result == LGCR_OK; ++k) {
         Workspace *w = c->spaces[slots[k % slots.size()]].get();
         void *dout[3] = {out[k].data[0], out[k].data[1], out[k].data[2]};
         result = enqueue_frame(c, w, *g, in[k].data, in[k].stride, dout, out[k].stride, timed);
@@ -2064,6 +2161,11 @@ lgcr_status lgcr_recon_submit(lgcr_clip *c, const lgcr_frame *in, double shift_x
     if (he != hipSuccess)
         return finish_submit(c, slot, fail(LGCR_BACKEND_FAILURE, "hipSetDevice failed"), ticket);
     w->marks.clear();
+    if (c->graphs.load() && !c->capture) {
+        const int idx0 = 0;
+        st = enqueue_frames_graph(c, w, *g, in, out, &idx0, 1);
+        return finish_submit(c, slot, st, ticket);
+    }
     void *dout[3] = {out->data[0], out->data[1], out->data[2]};
     st = enqueue_frame(c, w, *g, in->data, in->stride, dout, out->stride, false);
     if (st == LGCR_OK && c->same && dout[0] && dout[0] != in->data[0]) {
@@ -2317,6 +2419,13 @@ lgcr_status lgcr_sharpen_frame(lgcr_clip *c, const lgcr_frame *in, lgcr_frame *o
             HIP_TRY(hx.out(out->data[i], out->stride[i], w->raw_out[i], size_t(pw[i]) * c->bps, ph[i]));
     HIP_TRY(hipStreamSynchronize(w->stream));
     hx.finish();
+    return LGCR_OK;
+}
+
+lgcr_status lgcr_clip_set_graphs(lgcr_clip *c, int enable) {
+    if (!c)
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_clip_set_graphs: NULL clip");
+    c->graphs.store(enable != 0);
     return LGCR_OK;
 }
 

@@ -33,6 +33,7 @@ SYMBOLS = [
     "lgcr_clip_stage_times", "lgcr_clip_fetch", "lgcr_params_default_trecon", "lgcr_trecon_create",
     "lgcr_trecon_frame", "lgcr_sharpen_params_default", "lgcr_sharpen_create", "lgcr_sharpen_frame",
     "lgcr_recon_submit", "lgcr_trecon_submit", "lgcr_ticket_query", "lgcr_ticket_wait",
+    "lgcr_clip_set_graphs",
 ]
 STAGE_NAMES = [
     "ConvertInput", "ResampleLuma", "BuildGuideMaps", "BuildTrustMask", "BuildMutualGate",
@@ -114,6 +115,7 @@ def lib():
         L.lgcr_trecon_frame.argtypes = [vp, C.POINTER(Frame), C.POINTER(Frame), C.c_int, C.c_double, C.c_double,
                                         C.POINTER(Frame), C.POINTER(Stats)]
         L.lgcr_clip_set_timing.argtypes = [vp, C.c_int]
+        L.lgcr_clip_set_graphs.argtypes = [vp, C.c_int]
         L.lgcr_clip_stage_times.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
         L.lgcr_clip_fetch.argtypes = [vp, C.c_int, vp, C.c_size_t, C.POINTER(C.c_size_t)]
         L.lgcr_recon_submit.argtypes = [vp, C.POINTER(Frame), C.c_double, C.c_double, C.POINTER(Frame),
@@ -290,6 +292,9 @@ class Clip:
 
     def set_timing(self, enable=True):
         check(lib().lgcr_clip_set_timing(self.handle, int(enable)))
+
+    def set_graphs(self, enable=True):
+        check(lib().lgcr_clip_set_graphs(self.handle, int(enable)))
 
     def stage_times(self):
         """{stage name: (total_us, launches)} accumulated since set_timing."""

@@ -99,9 +99,10 @@ class ResidentClip:
     """A clip's source frames in HBM, its output buffers and its backend
     state (created on admission)."""
 
-    def __init__(self, device, spec: ClipSpec, max_inflight=4, threads=8):
+    def __init__(self, device, spec: ClipSpec, max_inflight=4, threads=8, graphs=False):
         import torch
         self.spec = spec
+        self.graphs = graphs  # Recon frames replay captured frame graphs
         self.device = device
         ssw, ssh, bits = FORMATS[spec.fmt]
         self.ssw, self.ssh, self.bits = ssw, ssh, bits
@@ -150,6 +151,8 @@ class ResidentClip:
             if e.status == abi.LGCR_REJECTED:
                 return False
             raise
+        if self.graphs:
+            self.clip.set_graphs(True)
         return True
 
     def release(self):
