@@ -156,6 +156,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-host-path", action="store_true")
     ap.add_argument("--graphs", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -189,6 +190,8 @@ def main():
     clip = abi.Clip(dev, params, abi.make_format(np_dt, W, H, bits=None if bits == 32 else bits),
                     max_inflight=args.inflight)
     clip.set_graphs(args.graphs)
+    # tile shapes of the tunable kernels for this frame size (autotuner)
+    tuning = clip.autotune_device(fin[0]) if not args.no_autotune else clip.tuning()
     for _ in range(args.warmup):
         clip.recon_batch(fin, fout)
     torch.cuda.synchronize()
@@ -241,6 +244,7 @@ def main():
         "warmup": args.warmup,
         "ms_per_step": round(elapsed / args.steps * 1000.0, 3),
         "frame_graphs": args.graphs,
+        "tuning": {"tensor_tile_rows": tuning[0], "gate_tile_cols": tuning[1], "autotuned": not args.no_autotune},
         "timed_stage_pass_fps": round(n / (t3 - t2), 2),
         "higher_is_better": True,
         "scaling": "weak",

@@ -236,6 +236,24 @@ lgcr_status lgcr_clip_stage_times(lgcr_clip *clip, double *us, uint64_t *launche
  * on the host). */
 lgcr_status lgcr_clip_set_graphs(lgcr_clip *clip, int enable);
 
+/* Tile shapes of the kernels whose best shape depends on the frame size.
+ * No counterpart in the reference (its CPU stages have no tiles). */
+typedef struct lgcr_tuning {
+    int tensor_tile_rows; /* Sobel tensor tiles of 64 x {4, 8, 16} rows */
+    int gate_tile_cols;   /* mutual-gate tiles of {16x16, 32x8, 64x4} (by columns) */
+} lgcr_tuning;
+lgcr_status lgcr_clip_get_tuning(const lgcr_clip *clip, lgcr_tuning *tuning);
+/* LGCR_INVALID_ARGUMENT for a shape outside the candidates. */
+lgcr_status lgcr_clip_set_tuning(lgcr_clip *clip, const lgcr_tuning *tuning);
+/* Autotuner: runs `frame` (host or device planes of this Recon clip) once,
+ * times every candidate shape of each tunable kernel on that frame's
+ * intermediates with HIP events, and keeps the fastest.  The choice is
+ * remembered in this process for (width, height, subsampling, sample type,
+ * bits, algo): clips created later with the same key start with it.  Every
+ * candidate computes identical results.  Call it (and lgcr_clip_set_tuning)
+ * while no frame of the clip is in flight. */
+lgcr_status lgcr_clip_autotune(lgcr_clip *clip, const lgcr_frame *frame);
+
 /* Intermediates of the last frame processed on workspace 0 of a clip, for
  * parity checks against the stage functions of src/lgcr.h:555-962.
  * Copies up to `bytes` into host memory `dst` and stores the full size in

@@ -136,8 +136,9 @@ hipError_t launch_to_float(const RawPlane *planes, int count, int type, hipStrea
 hipError_t launch_from_float(const RawOut *planes, int count, int type, hipStream_t s);
 // structure tensor written interleaved with the luma: yt = {Y, jxx, jxy, jyy}
 // seed: one bit per pixel, rows of (w + 63) / 64 words
+// tile_rows: 4, 8 or 16 (tile 64 x tile_rows)
 hipError_t launch_sobel_tensor(const float *y, int w, int h, float4 *yt, uint64_t *seed, float threshold,
-                               hipStream_t s);
+                               hipStream_t s, int tile_rows = 8);
 // trust mask (dilation of the seed bits by radius) and the sparse workset:
 // per-output-row counts, row offsets, row-major indices, chroma mask and
 // its population in counters[0]
@@ -151,9 +152,10 @@ hipError_t launch_lc_map(const float *y, int w, int h, float *lc, int cw, int ch
                          hipStream_t s, const float *u = nullptr, const float *v = nullptr,
                          float4 *luv = nullptr);
 // count / ncount: the marked chroma samples are count[0] + ... + count[ncount - 1]
+// tile_cols: 16, 32 or 64 (tiles 16x16, 32x8, 64x4)
 hipError_t launch_mutual_gate(const float *lc, const float *u, const float *v, int w, int h,
                               const uint8_t *chroma_mask, const uint32_t *count, int ncount, float *gate,
-                              hipStream_t s);
+                              hipStream_t s, int tile_cols = 32);
 hipError_t launch_plain_separable(const float *u, const float *v, int cw, int ch,
                                   const AxisDev &ax, const AxisDev &ay, float *hbuf,
                                   float *out_u, float *out_v, int clamp_hull, float margin,

@@ -343,16 +343,17 @@ __global__ __launch_bounds__(kBlock) void k_bp_pack(float *ou, float *ov, int ow
 // ---------------------------------------------------------------------------
 // structure tensor + fused trust seed (sobelTensor3x3, maps.cpp:64-168)
 //
-// One 64x8 output tile per block of 64x4 threads (each thread steps down
-// the tile by 4 rows).  LDS holds the luma tile with a 2-pixel halo, the
+// One 64xTH output tile per block of 64x4 threads (each thread steps down
+// the tile by 4 rows; TH is tuned per clip, 8 by default).  LDS holds the luma tile with a 2-pixel halo, the
 // Sobel gradients (1-pixel halo) evaluated at clamped centres, and the
 // vertical 3-row sums, so every border rule of the reference falls out of
 // the clamped centres; tiles whose halo lies inside the image skip the
 // clamping.
 // ---------------------------------------------------------------------------
 
-constexpr int TW = 64, TH = 8, TR = 4;
+constexpr int TW = 64, TR = 4;
 
+template <int TH>
 __global__ __launch_bounds__(TW * TR) void k_sobel_tensor(const float *__restrict__ y, int w, int h,
                                                          float4 *yt, uint64_t *seed, int w64, float thr) {
     __shared__ float sy[TH + 4][TW + 4];
@@ -765,13 +766,12 @@ __global__ void k_zero_u32(uint32_t *p, int n) {
 // bilinear samples can touch, as |k n| <= 3 + rounding) and evaluates the
 // gate from there.  LDS entries are addressed by clamped image coordinates,
 // so each lookup returns exactly the value k_grad3 + k_gate read.
-constexpr int kGateTW = 32, kGateTH = 8, kGateHalo = 4;
-constexpr int kGateGW = kGateTW + 2 * kGateHalo, kGateGH = kGateTH + 2 * kGateHalo;
-constexpr int kGateSW = kGateGW + 2, kGateSH = kGateGH + 2;
+constexpr int kGateHalo = 4;
 
 // 3x3 Sobel / 8 (maps.cpp:9-20) from an LDS tile; the caller passes clamped
 // neighbour indices.
-__device__ __forceinline__ void sobel_lds(const float (*p)[kGateSW], int xm, int xc, int xp, int ym, int yc, int yp,
+template <int SW>
+__device__ __forceinline__ void sobel_lds(const float (*p)[SW], int xm, int xc, int xp, int ym, int yc, int yp,
                                           float &gx, float &gy) {
     const float tl = p[ym][xm], t = p[ym][xc], tr = p[ym][xp];
     const float l = p[yc][xm], r = p[yc][xp];
@@ -780,10 +780,14 @@ __device__ __forceinline__ void sobel_lds(const float (*p)[kGateSW], int xm, int
     gy = (bl + 2.0f * b + br - tl - 2.0f * t - tr) * 0.125f;
 }
 
+template <int kGateTW, int kGateTH>
 __global__ __launch_bounds__(kBlock) void k_gate_fused(const float *__restrict__ lc, const float *__restrict__ u,
                                                        const float *__restrict__ v, int w, int h,
                                                        const uint8_t *cmask, const uint32_t *count, int ncount,
                                                        float *gate) {
+    static_assert(kGateTW * kGateTH == kBlock, "one thread per tile sample");
+    constexpr int kGateGW = kGateTW + 2 * kGateHalo, kGateGH = kGateTH + 2 * kGateHalo;
+    constexpr int kGateSW = kGateGW + 2, kGateSH = kGateGH + 2;
     __shared__ float src[3][kGateSH][kGateSW];
     __shared__ float grd[6][kGateGH][kGateGW];
     __shared__ int s_dense;
@@ -830,7 +834,7 @@ __global__ __launch_bounds__(kBlock) void k_gate_fused(const float *__restrict__
         const int ym = clampi(Q - 1, 0, h - 1) - sy0, yc = Q - sy0, yp = clampi(Q + 1, 0, h - 1) - sy0;
         _Pragma("unroll") for (int q = 0; q < 3; ++q) {
             float gx, gy;
-            sobel_lds(src[q], xm, xc, xp, ym, yc, yp, gx, gy);
+            sobel_lds<kGateSW>(src[q], xm, xc, xp, ym, yc, yp, gx, gy);
             grd[2 * q][r][c] = gx;
             grd[2 * q + 1][r][c] = gy;
         }
@@ -2856,9 +2860,17 @@ hipError_t launch_from_float(const RawOut *planes, int count, int type, hipStrea
 }
 
 hipError_t launch_sobel_tensor(const float *y, int w, int h, float4 *yt, uint64_t *seed, float threshold,
-                               hipStream_t s) {
-    const dim3 g((unsigned)((w + TW - 1) / TW), (unsigned)((h + TH - 1) / TH));
-    hipLaunchKernelGGL(k_sobel_tensor, g, dim3(TW, TR), 0, s, y, w, h, yt, seed, (w + 63) / 64, threshold);
+                               hipStream_t s, int tile_rows) {
+#define LGCR_SOBEL(TH_)                                                                                    \
+    hipLaunchKernelGGL(k_sobel_tensor<TH_>, dim3((unsigned)((w + TW - 1) / TW), (unsigned)((h + TH_ - 1) / TH_)), \
+                       dim3(TW, TR), 0, s, y, w, h, yt, seed, (w + 63) / 64, threshold)
+    if (tile_rows == 4)
+        LGCR_SOBEL(4);
+    else if (tile_rows == 16)
+        LGCR_SOBEL(16);
+    else
+        LGCR_SOBEL(8);
+#undef LGCR_SOBEL
     return hipGetLastError();
 }
 
@@ -2912,9 +2924,17 @@ bool guided_tiled(int tiles_fit, int radial, int cedge, const AxisDev &ax, const
 
 hipError_t launch_mutual_gate(const float *lc, const float *u, const float *v, int w, int h,
                               const uint8_t *chroma_mask, const uint32_t *count, int ncount, float *gate,
-                              hipStream_t s) {
-    const dim3 g((unsigned)((w + kGateTW - 1) / kGateTW), (unsigned)((h + kGateTH - 1) / kGateTH));
-    hipLaunchKernelGGL(k_gate_fused, g, dim3(kBlock), 0, s, lc, u, v, w, h, chroma_mask, count, ncount, gate);
+                              hipStream_t s, int tile_cols) {
+#define LGCR_GATE(TW_, TH_)                                                                                        \
+    hipLaunchKernelGGL((k_gate_fused<TW_, TH_>), dim3((unsigned)((w + TW_ - 1) / TW_), (unsigned)((h + TH_ - 1) / TH_)), \
+                       dim3(kBlock), 0, s, lc, u, v, w, h, chroma_mask, count, ncount, gate)
+    if (tile_cols == 64)
+        LGCR_GATE(64, 4);
+    else if (tile_cols == 16)
+        LGCR_GATE(16, 16);
+    else
+        LGCR_GATE(32, 8);
+#undef LGCR_GATE
     return hipGetLastError();
 }
 

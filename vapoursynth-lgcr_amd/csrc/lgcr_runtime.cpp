@@ -5,6 +5,7 @@
 #include "lgcr_internal.h"
 
 #include <algorithm>
+#include <array>
 #include <climits>
 #include <cstddef>
 #include <functional>
@@ -551,6 +552,7 @@ struct lgcr_clip {
     bool same = true;
     bool capture = false;
     std::atomic<bool> graphs{false};  // replay captured frame graphs for device-resident Recon frames
+    lgcr_tuning tune{8, 32};          // tile shapes (lgcr_clip_autotune)
     bool temporal = false;  // TRecon clip (neighbour buffers)
     bool sharpen = false;   // Sharpen clip
     lgcr_sharpen_params sp{};
@@ -985,6 +987,29 @@ struct HostXfer {
     }
 };
 
+// Autotuned tile shapes, remembered per (width, height, subsampling, sample
+// type, bits, algo) for the life of the process.
+using TuneKey = std::array<int, 7>;
+std::mutex g_tune_mutex;
+std::map<TuneKey, lgcr_tuning> g_tuned;
+
+TuneKey tune_key(const lgcr_clip *c) {
+    return {c->sw, c->sh, c->f.sub_sampling_w, c->f.sub_sampling_h, c->f.sample_type, c->f.bits_per_sample,
+            c->p.algo};
+}
+
+void tuned_lookup(lgcr_clip *c) {
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    const auto it = g_tuned.find(tune_key(c));
+    if (it != g_tuned.end())
+        c->tune = it->second;
+}
+
+bool tuning_valid(const lgcr_tuning &t) {
+    return (t.tensor_tile_rows == 4 || t.tensor_tile_rows == 8 || t.tensor_tile_rows == 16) &&
+           (t.gate_tile_cols == 16 || t.gate_tile_cols == 32 || t.gate_tile_cols == 64);
+}
+
 lgcr_status acquire(lgcr_clip *c, int *slot) {
     std::unique_lock<std::mutex> lock(c->ws_mutex);
     for (;;) {
@@ -1147,7 +1172,7 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
         // BuildGuideMaps (plugin.cpp:324-334, maps.cpp:252-276)
         const double thr = p.algo == 4 ? 0.25 * p.sigma : p.sigma;
         HIP_TRY(launch_sobel_tensor(w->y, c->sw, c->sh, w->yt, p.sparse ? w->seed : nullptr,
-                                    float(thr * thr), s));
+                                    float(thr * thr), s, c->tune.tensor_tile_rows));
         HIP_TRY(launch_lc_map(w->y, c->sw, c->sh, w->lc, c->cw, c->ch, g.xb, g.xe, g.yb, g.ye, s, w->u, w->v,
                               w->luv));
         HIP_TRY(mark(LGCR_STAGE_BUILD_GUIDE_MAPS));
@@ -1162,7 +1187,7 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
             const bool bands = w->has_ws && !w->ws_indices;
             HIP_TRY(launch_mutual_gate(w->lc, w->u, w->v, c->cw, c->ch, w->has_ws ? w->cmask : nullptr,
                                        bands ? w->counters + kCntChromaWords : w->counters, bands ? g.tb_n : 1,
-                                       w->ms, s));
+                                       w->ms, s, c->tune.gate_tile_cols));
             HIP_TRY(mark(LGCR_STAGE_BUILD_MUTUAL_GATE));
         }
     }
@@ -1459,9 +1484,10 @@ lgcr_status enqueue_trecon(lgcr_clip *c, Workspace *w, const Geometry &g, const 
     HIP_TRY(launch_lc_map(w->y, c->sw, c->sh, w->lc, c->cw, c->ch, g.xb, g.xe, g.yb, g.ye, s, w->u, w->v, w->luv));
     if (guided) {
         HIP_TRY(launch_sobel_tensor(w->y, c->sw, c->sh, w->yt, p.sparse ? w->seed : nullptr,
-                                    float(p.sigma * p.sigma), s));
+                                    float(p.sigma * p.sigma), s, c->tune.tensor_tile_rows));
         if (p.ms > 0.0)
-            HIP_TRY(launch_mutual_gate(w->lc, w->u, w->v, c->cw, c->ch, nullptr, w->counters, 1, w->ms, s));
+            HIP_TRY(launch_mutual_gate(w->lc, w->u, w->v, c->cw, c->ch, nullptr, w->counters, 1, w->ms, s,
+                                       c->tune.gate_tile_cols));
         HIP_TRY(mark(LGCR_STAGE_BUILD_GUIDE_MAPS));
     }
     const double oscale = is_float ? 1.0 : double((1 << c->f.bits_per_sample) - 1);
@@ -1681,6 +1707,7 @@ static lgcr_status create_clip(lgcr_device *dev, const lgcr_params *params, cons
         return fail(LGCR_UNSUPPORTED, "LGCR HIP backend: planes of 2^31 or more samples are not supported");
     if (const char *cap = std::getenv("LGCR_CAPTURE"))
         c->capture = cap[0] && cap[0] != '0';
+    tuned_lookup(c.get());
     HIP_TRY(hipSetDevice(dev->ordinal));
     if (!c->same && !p.radial) {
         const HostWeights th = build_weights(c->sw, c->ow, p.kernel, p.kp1, p.kp2, p.support, 0.0);
@@ -2419,6 +2446,147 @@ lgcr_status lgcr_sharpen_frame(lgcr_clip *c, const lgcr_frame *in, lgcr_frame *o
             HIP_TRY(hx.out(out->data[i], out->stride[i], w->raw_out[i], size_t(pw[i]) * c->bps, ph[i]));
     HIP_TRY(hipStreamSynchronize(w->stream));
     hx.finish();
+    return LGCR_OK;
+}
+
+lgcr_status lgcr_clip_get_tuning(const lgcr_clip *c, lgcr_tuning *t) {
+    if (!c || !t)
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_clip_get_tuning: NULL argument");
+    *t = c->tune;
+    return LGCR_OK;
+}
+
+lgcr_status lgcr_clip_set_tuning(lgcr_clip *c, const lgcr_tuning *t) {
+    if (!c || !t)
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_clip_set_tuning: NULL argument");
+    if (!tuning_valid(*t))
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_clip_set_tuning: tensor tile rows must be 4, 8 or 16 and gate "
+                                           "tile columns 16, 32 or 64");
+    c->tune = *t;
+    return LGCR_OK;
+}
+
+lgcr_status lgcr_clip_autotune(lgcr_clip *c, const lgcr_frame *frame) {
+    if (!c || !frame)
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_clip_autotune: NULL argument");
+    if (c->temporal || c->sharpen)
+        return fail(LGCR_UNSUPPORTED, "lgcr_clip_autotune: Recon clips only");
+    for (int i = 0; i < 3; ++i)
+        if (!frame->data[i])
+            return fail(LGCR_INVALID_ARGUMENT, "lgcr_clip_autotune: plane %d is NULL", i);
+    Geometry *g = nullptr;
+    lgcr_status st = get_geometry(c, c->f.sub_sampling_w ? c->p.shift_x : 0.0, c->f.sub_sampling_h ? c->p.shift_y : 0.0,
+                                  &g);
+    if (st != LGCR_OK)
+        return st;
+    int slot = -1;
+    st = acquire(c, &slot);
+    if (st != LGCR_OK)
+        return st;
+    Workspace *w = c->spaces[slot].get();
+    struct Releaser {
+        lgcr_clip *c;
+        int slot;
+        ~Releaser() { release(c, slot); }
+    } rel{c, slot};
+    HIP_TRY(hipSetDevice(c->dev->ordinal));
+    hipStream_t s = w->stream;
+    // one frame through the clip (into the workspace's own output planes) so
+    // the tunable kernels have this frame's intermediates to run on
+    const int pw[3] = {c->sw, c->cw, c->cw}, ph[3] = {c->sh, c->ch, c->ch};
+    const void *din[3];
+    ptrdiff_t dis[3];
+    void *dout[3];
+    ptrdiff_t dos[3];
+    HostXfer hx{w, s};
+    const bool host = frame->memory == LGCR_MEMORY_HOST;
+    if (host) {
+        size_t bytes = 0;
+        for (int i = 0; i < 3; ++i)
+            bytes += HostXfer::span(size_t(pw[i]) * c->bps, ph[i]);
+        HIP_TRY(hx.reserve(bytes));
+    }
+    for (int i = 0; i < 3; ++i) {
+        const size_t rb = size_t(pw[i]) * c->bps;
+        if (host) {
+            HIP_TRY(hx.in(w->raw_in[i], frame->data[i], frame->stride[i], rb, ph[i]));
+            din[i] = w->raw_in[i];
+            dis[i] = ptrdiff_t(rb);
+        } else {
+            din[i] = frame->data[i];
+            dis[i] = frame->stride[i];
+        }
+        dout[i] = w->raw_out[i];
+        dos[i] = ptrdiff_t(size_t(c->ow) * c->bps);
+    }
+    if (c->same)
+        dout[0] = nullptr;
+    w->marks.clear();
+    st = enqueue_frame(c, w, *g, din, dis, dout, dos, false);
+    if (st != LGCR_OK)
+        return st;
+    HIP_TRY(hipStreamSynchronize(s));
+    const lgcr_params &p = c->p;
+    const bool guided = p.strength > 0.0;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    struct Events {
+        hipEvent_t a, b;
+        ~Events() {
+            hipEventDestroy(a);
+            hipEventDestroy(b);
+        }
+    } evs{e0, e1};
+    constexpr int kReps = 8;
+    auto time_it = [&](auto &&launch, float &ms) -> hipError_t {
+        hipError_t e = launch();  // warm-up
+        if (e != hipSuccess)
+            return e;
+        if ((e = hipEventRecord(e0, s)) != hipSuccess)
+            return e;
+        for (int r = 0; r < kReps; ++r)
+            if ((e = launch()) != hipSuccess)
+                return e;
+        if ((e = hipEventRecord(e1, s)) != hipSuccess)
+            return e;
+        if ((e = hipEventSynchronize(e1)) != hipSuccess)
+            return e;
+        return hipEventElapsedTime(&ms, e0, e1);
+    };
+    lgcr_tuning best = c->tune;
+    if (guided) {
+        // the tensor pass rewrites this frame's tensor and seed with the same values
+        const double thr = p.algo == 4 ? 0.25 * p.sigma : p.sigma;
+        float best_ms = 1e30f;
+        for (int rows : {4, 8, 16}) {
+            float ms = 0.0f;
+            HIP_TRY(time_it([&] { return launch_sobel_tensor(w->y, c->sw, c->sh, w->yt, p.sparse ? w->seed : nullptr,
+                                                            float(thr * thr), s, rows); }, ms));
+            if (ms < best_ms) {
+                best_ms = ms;
+                best.tensor_tile_rows = rows;
+            }
+        }
+        if (p.ms > 0.0) {
+            const bool bands = w->has_ws && !w->ws_indices;
+            best_ms = 1e30f;
+            for (int cols : {16, 32, 64}) {
+                float ms = 0.0f;
+                HIP_TRY(time_it([&] { return launch_mutual_gate(w->lc, w->u, w->v, c->cw, c->ch,
+                                                                w->has_ws ? w->cmask : nullptr,
+                                                                bands ? w->counters + kCntChromaWords : w->counters,
+                                                                bands ? g->tb_n : 1, w->ms, s, cols); }, ms));
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best.gate_tile_cols = cols;
+                }
+            }
+        }
+    }
+    c->tune = best;
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    g_tuned[tune_key(c)] = best;
     return LGCR_OK;
 }
 

@@ -33,7 +33,7 @@ SYMBOLS = [
     "lgcr_clip_stage_times", "lgcr_clip_fetch", "lgcr_params_default_trecon", "lgcr_trecon_create",
     "lgcr_trecon_frame", "lgcr_sharpen_params_default", "lgcr_sharpen_create", "lgcr_sharpen_frame",
     "lgcr_recon_submit", "lgcr_trecon_submit", "lgcr_ticket_query", "lgcr_ticket_wait",
-    "lgcr_clip_set_graphs",
+    "lgcr_clip_set_graphs", "lgcr_clip_get_tuning", "lgcr_clip_set_tuning", "lgcr_clip_autotune",
 ]
 STAGE_NAMES = [
     "ConvertInput", "ResampleLuma", "BuildGuideMaps", "BuildTrustMask", "BuildMutualGate",
@@ -66,6 +66,10 @@ class Format(C.Structure):
     _fields_ = [("sample_type", C.c_int), ("bits_per_sample", C.c_int), ("bytes_per_sample", C.c_int),
                 ("sub_sampling_w", C.c_int), ("sub_sampling_h", C.c_int), ("width", C.c_int),
                 ("height", C.c_int)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("tensor_tile_rows", C.c_int), ("gate_tile_cols", C.c_int)]
 
 
 class Frame(C.Structure):
@@ -116,6 +120,9 @@ def lib():
                                         C.POINTER(Frame), C.POINTER(Stats)]
         L.lgcr_clip_set_timing.argtypes = [vp, C.c_int]
         L.lgcr_clip_set_graphs.argtypes = [vp, C.c_int]
+        L.lgcr_clip_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
+        L.lgcr_clip_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
+        L.lgcr_clip_autotune.argtypes = [vp, C.POINTER(Frame)]
         L.lgcr_clip_stage_times.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
         L.lgcr_clip_fetch.argtypes = [vp, C.c_int, vp, C.c_size_t, C.POINTER(C.c_size_t)]
         L.lgcr_recon_submit.argtypes = [vp, C.POINTER(Frame), C.c_double, C.c_double, C.POINTER(Frame),
@@ -295,6 +302,28 @@ class Clip:
 
     def set_graphs(self, enable=True):
         check(lib().lgcr_clip_set_graphs(self.handle, int(enable)))
+
+    def tuning(self):
+        t = Tuning()
+        check(lib().lgcr_clip_get_tuning(self.handle, C.byref(t)))
+        return t.tensor_tile_rows, t.gate_tile_cols
+
+    def set_tuning(self, tensor_tile_rows, gate_tile_cols):
+        t = Tuning(tensor_tile_rows, gate_tile_cols)
+        check(lib().lgcr_clip_set_tuning(self.handle, C.byref(t)))
+
+    def autotune_host(self, y, u, v):
+        """Autotune on one frame of host planes; returns the chosen shapes."""
+        f, keep = host_frame((y, u, v))
+        check(lib().lgcr_clip_autotune(self.handle, C.byref(f)))
+        return self.tuning()
+
+    def autotune_device(self, planes):
+        """Autotune on one frame of device planes ((ptr, stride) x 3)."""
+        f = Frame((C.c_void_p * 3)(*[p for p, _ in planes]), (C.c_ssize_t * 3)(*[s for _, s in planes]),
+                  MEMORY_DEVICE)
+        check(lib().lgcr_clip_autotune(self.handle, C.byref(f)))
+        return self.tuning()
 
     def stage_times(self):
         """{stage name: (total_us, launches)} accumulated since set_timing."""
