@@ -267,6 +267,8 @@ enum {
     LGCR_FETCH_INDICES,                                     /* u32 active output indices */
     LGCR_FETCH_PLAIN_U, LGCR_FETCH_PLAIN_V,                 /* plain base, f32 */
     LGCR_FETCH_OUT_U, LGCR_FETCH_OUT_V,                     /* result before packing, f32 */
+    LGCR_FETCH_MV,      /* TRecon, first neighbour: int16 mvx[nb] then mvy[nb], nb = 16x16 luma blocks */
+    LGCR_FETCH_CONF,    /* TRecon, first neighbour: f32 block confidence after the chroma check */
     LGCR_FETCH_COUNT
 };
 lgcr_status lgcr_clip_fetch(lgcr_clip *clip, int what, void *dst, size_t bytes, size_t *needed);

@@ -145,6 +145,8 @@ def lib():
         i16p = C.POINTER(C.c_int16)
         L.or_block_match.argtypes = [fp, fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, i16p, i16p, fp,
                                      C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.or_chroma_consistency.argtypes = [fp, fp, fp, fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, i16p, i16p,
+                                            C.c_int, C.c_int, fp]
         _lib = L
     return _lib
 

@@ -1,5 +1,7 @@
 """GPU parity for TRecon (motion-compensated temporal taps) against the
 oracle, through the C ABI."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -70,3 +72,44 @@ def test_trecon_1080p_p10(device):
         cnt, mx = compare_planes(ref[i], got[i], False)
         print(f"trecon-1080p {name}: mismatches={cnt} max_abs={mx:g}")
         assert cnt == 0
+
+
+# (w, h, tsearch): the window kernel holds the block and its search window in
+# LDS up to 64 KiB (tsearch <= 54 at 16x16 blocks); tsearch 60 takes the
+# global-memory kernel.  98x62 has partial blocks on the right and bottom.
+ME_CASES = [(160, 96, 0), (160, 96, 2), (160, 96, 6), (98, 62, 6), (160, 96, 20), (128, 80, 60)]
+
+
+@pytest.mark.parametrize("w,h,search", ME_CASES, ids=[f"{w}x{h}-s{s}" for w, h, s in ME_CASES])
+def test_trecon_motion_vectors(device, w, h, search):
+    """Block vectors and confidences of the first neighbour, bit-exact against
+    the oracle's blockMatch + applyChromaConsistency (algos.cpp:683-807).
+    Float frames, so the converted planes are the input planes."""
+    frames = [synth.clip_frame(w, h, bits=32, seed=5, frame=k, motion=True, polygons=10) for k in range(3)]
+    tsad = 0.02
+    clip = abi.Clip(device, abi.trecon_params(tsearch=search, tsad=tsad), abi.make_format(np.float32, w, h),
+                    temporal=True)
+    try:
+        clip.trecon_host(frames[1], [frames[0], frames[2]])
+        mv = clip.fetch("mv", np.int16)
+        conf = clip.fetch("conf")
+    finally:
+        clip.close()
+    L = oracle.lib()
+    bw, bh = (w + 15) // 16, (h + 15) // 16
+    nb = bw * bh
+    rmv = np.zeros(2 * nb, np.int16)
+    rconf = np.zeros(nb, np.float32)
+    i16 = lambda a: a.ctypes.data_as(C.POINTER(C.c_int16))
+    cy, cu, cv = (np.ascontiguousarray(p, np.float32) for p in frames[1])
+    ny, nu, nv = (np.ascontiguousarray(p, np.float32) for p in frames[0])
+    obw, obh = C.c_int(), C.c_int()
+    L.or_block_match(oracle.fptr(cy), oracle.fptr(ny), w, h, 16, search, tsad, i16(rmv), i16(rmv[nb:]),
+                     oracle.fptr(rconf), C.byref(obw), C.byref(obh))
+    L.or_chroma_consistency(oracle.fptr(cu), oracle.fptr(cv), oracle.fptr(nu), oracle.fptr(nv), w // 2, h // 2, w, h,
+                            16, i16(rmv), i16(rmv[nb:]), bw, bh, oracle.fptr(rconf))
+    moving = int(np.count_nonzero(rmv[:nb]) + np.count_nonzero(rmv[nb:]))
+    print(f"{w}x{h} s{search}: blocks={nb} nonzero components={moving} "
+          f"mv mismatches={int(np.count_nonzero(mv != rmv))} conf mismatches={int(np.count_nonzero(conf != rconf))}")
+    assert np.array_equal(mv, rmv)
+    assert np.array_equal(conf.view(np.uint32), rconf.view(np.uint32))

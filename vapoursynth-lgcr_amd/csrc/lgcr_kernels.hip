@@ -2332,9 +2332,149 @@ __global__ __launch_bounds__(kMeThreads) void k_block_match(const float *__restr
     conf[idx] = match_q * uniq;
 }
 
+// The same search with the block's search window in LDS: one wave per block.
+// The window holds every neighbour sample a candidate placement inside the
+// image reads (rows y0-search .. y0+bh-1+search, columns likewise; samples
+// outside the image are clamped copies no valid placement reads).  Each lane
+// scores kMeCand candidates per pass, walking the block's samples once in
+// the reference's (j, i) order with one float accumulator per candidate, so
+// every SAD rounds exactly as the scalar loop; the argmin is a wave
+// reduction on (sad, |dx|+|dy|, scan index).
+constexpr int kMeCand = 3;
+constexpr int kMeLdsMax = 64 * 1024;
+
+__device__ __forceinline__ bool me_better(float s2, int n2, int i2, float s1, int n1, int i1) {
+    if (i2 == 0x7fffffff)
+        return false;
+    if (i1 == 0x7fffffff || s2 < s1)
+        return true;
+    return s2 == s1 && (n2 < n1 || (n2 == n1 && i2 < i1));
+}
+
+__global__ __launch_bounds__(64) void k_block_match_win(const float *__restrict__ cur, const float *__restrict__ nbr,
+                                                        int w, int h, int block, int search, float tsad,
+                                                        int16_t *mvx, int16_t *mvy, float *conf) {
+    extern __shared__ float me_lds[];
+    __shared__ float s_sad0;
+    const int lane = threadIdx.x;
+    const int bw = (w + block - 1) / block;
+    const int bx = blockIdx.x % bw, by = blockIdx.x / bw;
+    const int x0 = bx * block, y0 = by * block;
+    const int bw_ = min(block, w - x0), bh_ = min(block, h - y0);
+    const int nx = (bw_ + 1) / 2, ny = (bh_ + 1) / 2;
+    const int half = (block + 1) / 2;
+    const int ws = block + 2 * search;
+    float *sc = me_lds;                // nx*ny block samples
+    float *win = me_lds + half * half; // ws x ws window
+    for (int t = lane; t < nx * ny; t += 64)
+        sc[t] = at(cur, w, h, x0 + 2 * (t % nx), y0 + 2 * (t / nx));
+    // four loads in flight per lane before the first LDS store
+    for (int t0 = 0; t0 < ws * ws; t0 += 256) {
+        float r[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int t = min(t0 + lane + 64 * k, ws * ws - 1);
+            r[k] = at(nbr, w, h, x0 - search + t % ws, y0 - search + t / ws);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (t0 + lane + 64 * k < ws * ws)
+                win[t0 + lane + 64 * k] = r[k];
+    }
+    if (lane == 0)
+        s_sad0 = 1e30f;
+    __syncthreads();
+    const int side = 2 * search + 1, ncand = side * side;
+    float best = 1e30f;
+    int bidx = 0x7fffffff, bnorm = 0x7fffffff;
+    for (int base = 0; base < ncand; base += 64 * kMeCand) {
+        float acc[kMeCand];
+        int off[kMeCand];
+#pragma unroll
+        for (int k = 0; k < kMeCand; ++k) {
+            const int c = base + lane + 64 * k;
+            const int dy = c / side - search, dx = c % side - search;
+            const bool ok = c < ncand && x0 + dx >= 0 && y0 + dy >= 0 && x0 + dx + bw_ <= w && y0 + dy + bh_ <= h;
+            off[k] = ok ? (dy + search) * ws + (dx + search) : 0;
+            acc[k] = ok ? 0.0f : -1.0f;
+        }
+        for (int j = 0; j < ny; ++j) {
+            const float *srow = sc + j * nx;
+            const int orow = 2 * j * ws;
+            for (int i = 0; i < nx; ++i) {
+                const float v = srow[i];
+#pragma unroll
+                for (int k = 0; k < kMeCand; ++k)
+                    acc[k] += fabsf(v - win[off[k] + orow + 2 * i]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kMeCand; ++k) {
+            const int c = base + lane + 64 * k;
+            const int dy = c / side - search, dx = c % side - search;
+            const bool ok = c < ncand && x0 + dx >= 0 && y0 + dy >= 0 && x0 + dx + bw_ <= w && y0 + dy + bh_ <= h;
+            if (!ok)
+                continue;
+            if (dx == 0 && dy == 0)
+                s_sad0 = acc[k];
+            const int norm = abs(dx) + abs(dy);
+            if (me_better(acc[k], norm, c, best, bnorm, bidx)) {
+                best = acc[k];
+                bidx = c;
+                bnorm = norm;
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const float s2 = __shfl_xor(best, o);
+        const int i2 = __shfl_xor(bidx, o);
+        const int n2 = __shfl_xor(bnorm, o);
+        if (me_better(s2, n2, i2, best, bnorm, bidx)) {
+            best = s2;
+            bidx = i2;
+            bnorm = n2;
+        }
+    }
+    __syncthreads();
+    if (lane != 0)
+        return;
+    float bsad = best;
+    int bdx = 0, bdy = 0;
+    if (bidx != 0x7fffffff) {
+        bdy = bidx / side - search;
+        bdx = bidx % side - search;
+    }
+    double sum = 0.0, sum2 = 0.0;
+    for (int t = 0; t < nx * ny; ++t) {
+        const float v = sc[t];
+        sum += v;
+        sum2 = dacc(sum2, v, v);
+    }
+    const float npix = (float)(nx * ny);
+    const float sad0 = s_sad0;
+    if ((bdx != 0 || bdy != 0) && sad0 - bsad <= 0.10f * tsad * npix) {
+        bdx = 0;
+        bdy = 0;
+        bsad = sad0;
+    }
+    const int idx = blockIdx.x;
+    mvx[idx] = (int16_t)bdx;
+    mvy[idx] = (int16_t)bdy;
+    const float sad_px = bsad / npix;
+    const double vv = sum2 / npix - (sum / npix) * (sum / npix);
+    const float var = (float)(0.0 < vv ? vv : 0.0);
+    const float inv_t2 = 1.0f / (tsad * tsad);
+    const float match_q = 1.0f / (1.0f + sad_px * sad_px * inv_t2);
+    const float uniq = var / (var + 0.25f * tsad * tsad);
+    conf[idx] = match_q * uniq;
+}
+
 constexpr int kCcMax = 1024;
 
-__global__ __launch_bounds__(kBlock) void k_chroma_consistency(const float *cu, const float *cv, const float *nu,
+constexpr int kCcThreads = 64;  // one wave per block
+
+__global__ __launch_bounds__(kCcThreads) void k_chroma_consistency(const float *cu, const float *cv, const float *nu,
                                                                const float *nv, int cw, int ch, int luma_w,
                                                                int luma_h, int block, const int16_t *mvx,
                                                                const int16_t *mvy, int bw, float *conf) {
@@ -2353,7 +2493,7 @@ __global__ __launch_bounds__(kBlock) void k_chroma_consistency(const float *cu, 
     const int nxc = cx1 - cx0, n = nxc * (cy1 - cy0);
     if (n > kCcMax)
         return;  // the host rejects block sizes that could overflow
-    for (int t = threadIdx.x; t < n; t += kBlock) {
+    for (int t = threadIdx.x; t < n; t += kCcThreads) {
         const int x = cx0 + t % nxc, y = cy0 + t / nxc;
         const int qx = clampi((int)llround(x + mx), 0, cw - 1), qy = clampi((int)llround(y + my), 0, ch - 1);
         const size_t a = (size_t)y * cw + x, b = (size_t)qy * cw + qx;
@@ -2362,7 +2502,7 @@ __global__ __launch_bounds__(kBlock) void k_chroma_consistency(const float *cu, 
     __syncthreads();
     // order statistics by rank: element t has rank #{smaller} + #{equal, lower index}
     const int mid = n / 2;
-    for (int t = threadIdx.x; t < n; t += kBlock) {
+    for (int t = threadIdx.x; t < n; t += kCcThreads) {
         const float d = delta[t];
         int rank = 0;
         for (int k = 0; k < n; ++k) {
@@ -3125,15 +3265,21 @@ hipError_t launch_block_match(const float *cur, const float *nbr, int w, int h, 
     if (block > 64)
         return hipErrorInvalidValue;  // LDS block cache holds 32x32 subsampled samples
     const int bw = (w + block - 1) / block, bh = (h + block - 1) / block;
-    hipLaunchKernelGGL(k_block_match, dim3((unsigned)(bw * bh)), dim3(kMeThreads), 0, s, cur, nbr, w, h, block, search,
-                       tsad, mvx, mvy, conf);
+    const int half = (block + 1) / 2, ws = block + 2 * search;
+    const size_t lds = sizeof(float) * ((size_t)half * half + (size_t)ws * ws);
+    if (lds <= (size_t)kMeLdsMax)
+        hipLaunchKernelGGL(k_block_match_win, dim3((unsigned)(bw * bh)), dim3(64), lds, s, cur, nbr, w, h, block,
+                           search, tsad, mvx, mvy, conf);
+    else
+        hipLaunchKernelGGL(k_block_match, dim3((unsigned)(bw * bh)), dim3(kMeThreads), 0, s, cur, nbr, w, h, block,
+                           search, tsad, mvx, mvy, conf);
     return hipGetLastError();
 }
 
 hipError_t launch_chroma_consistency(const float *cu, const float *cv, const float *nu, const float *nv, int cw,
                                      int ch, int luma_w, int luma_h, int block, const int16_t *mvx,
                                      const int16_t *mvy, int bw, int bh, float *conf, hipStream_t s) {
-    hipLaunchKernelGGL(k_chroma_consistency, dim3((unsigned)(bw * bh)), dim3(kBlock), 0, s, cu, cv, nu, nv, cw, ch,
+    hipLaunchKernelGGL(k_chroma_consistency, dim3((unsigned)(bw * bh)), dim3(kCcThreads), 0, s, cu, cv, nu, nv, cw, ch,
                        luma_w, luma_h, block, mvx, mvy, bw, conf);
     return hipGetLastError();
 }

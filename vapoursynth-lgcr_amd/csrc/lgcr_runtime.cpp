@@ -2694,6 +2694,15 @@ lgcr_status lgcr_clip_fetch(lgcr_clip *c, int what, void *dst, size_t bytes, siz
     case LGCR_FETCH_PLAIN_V: src = w->plain_v; size = no * 4; break;
     case LGCR_FETCH_OUT_U: src = w->out_u; size = no * 4; break;
     case LGCR_FETCH_OUT_V: src = w->out_v; size = no * 4; break;
+    case LGCR_FETCH_MV:
+    case LGCR_FETCH_CONF: {
+        const size_t nb = size_t((c->sw + 15) / 16) * ((c->sh + 15) / 16);
+        if (w->nb.empty() || !w->nb[0].mv || !w->nb[0].conf)
+            return fail(LGCR_INVALID_ARGUMENT, "lgcr_clip_fetch: no neighbour state (not a TRecon clip)");
+        src = what == LGCR_FETCH_MV ? static_cast<const void *>(w->nb[0].mv) : w->nb[0].conf;
+        size = what == LGCR_FETCH_MV ? nb * 2 * sizeof(int16_t) : nb * sizeof(float);
+        break;
+    }
     default:
         return fail(LGCR_INVALID_ARGUMENT, "lgcr_clip_fetch: unknown id %d", what);
     }

@@ -8,6 +8,7 @@ when the library is missing.
 from __future__ import annotations
 
 import ctypes as C
+import importlib.util
 import math
 import os
 
@@ -23,7 +24,7 @@ STAGE_COUNT = 19
 
 FETCH = {name: i for i, name in enumerate([
     "y", "u", "v", "jxx", "jxy", "jyy", "lc", "ms", "seed", "mask", "chroma_mask", "indices",
-    "plain_u", "plain_v", "out_u", "out_v"])}
+    "plain_u", "plain_v", "out_u", "out_v", "mv", "conf"])}
 
 # exported symbols, in header order
 SYMBOLS = [
@@ -98,6 +99,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(paths.HIP_LIB):
             raise RuntimeError(f"{paths.HIP_LIB} is not built; run __graft_entry__.build()")
+        # One HIP runtime per process: torch ships its own libamdhip64.so.7.
+        # Loaded first, it satisfies this library's dependency on that soname,
+        # so device pointers, streams and events of both live in one runtime;
+        # loaded after /opt/rocm's copy, torch would start a second runtime
+        # that finds no device.
+        if importlib.util.find_spec("torch") is not None:
+            import torch  # noqa: F401
         L = C.CDLL(paths.HIP_LIB)
         vp = C.c_void_p
         L.lgcr_abi_version.restype = C.c_int

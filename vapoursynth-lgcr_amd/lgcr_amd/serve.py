@@ -122,6 +122,9 @@ class ResidentClip:
         self.free_out = list(range(max_inflight))
         self.clip = None
         self.rejected_at = -1     # release generation of the last refused admission
+        # relative device work of one frame (1080p Recon = 1): TRecon also
+        # converts and block-matches its two neighbours
+        self.cost = spec.width * spec.height / (1920 * 1080) * (4.0 if self.temporal else 1.0)
         self.src_bytes = sum(t.numel() * t.element_size() for f in self.src for t in f)
         self.out_bytes = sum(t.numel() * t.element_size() for f in self.out for t in f)
 
@@ -225,16 +228,20 @@ def _percentiles(lat):
     return float(np.percentile(a, 50) * 1e3), float(np.percentile(a, 99) * 1e3)
 
 
-def serve(clips, requests, mode="continuous", keep=None, poll_sleep=20e-6, max_inflight=8):
+def serve(clips, requests, mode="continuous", keep=None, poll_sleep=20e-6, max_inflight=8, lat_out=None,
+          max_work=None):
     """Serve `requests` (sorted by arrival) on `clips` (ResidentClip list).
 
     mode "continuous": non-blocking submissions across clips (at most
-    `max_inflight` frames on the device at once, oldest request first),
+    `max_inflight` frames on the device at once and, with `max_work`, at
+    most that much relative work (`ResidentClip.cost`) unless the device is
+    idle; oldest request first),
     completions polled; mode "per_frame": one blocking call at a time.
     A clip whose frames are all served keeps its device state until another
     clip is refused admission (releasing frees memory but synchronises the
     device).  keep: optional dict filled with {(clip, frame): host planes}
-    for correctness checks.  Returns a stats dict (fps, p50/p99 latency in
+    for correctness checks.  lat_out: optional list that receives
+    (request, latency in s) per served frame.  Returns a stats dict (fps, p50/p99 latency in
     ms, peak reserved HBM)."""
     remaining = [0] * len(clips)
     for r in requests:
@@ -258,6 +265,8 @@ def serve(clips, requests, mode="continuous", keep=None, poll_sleep=20e-6, max_i
                     raise RuntimeError("per-frame serving: clip does not fit the device budget alone")
             c.run_blocking(r.frame)
             lat.append(time.perf_counter() - t0 - r.t)
+            if lat_out is not None:
+                lat_out.append((r, lat[-1]))
             peak = max(peak, dev.reserved())
             if keep is not None:
                 keep[(r.clip, r.frame)] = c.output_host(0, r.frame)
@@ -269,6 +278,7 @@ def serve(clips, requests, mode="continuous", keep=None, poll_sleep=20e-6, max_i
 
     pending = deque()
     inflight = []   # (ticket, request, slot)
+    work = 0.0      # relative work of the frames in flight
     gen = 0         # bumped whenever a clip releases its device state
     while i < len(requests) or pending or inflight:
         now = time.perf_counter() - t0
@@ -282,7 +292,10 @@ def serve(clips, requests, mode="continuous", keep=None, poll_sleep=20e-6, max_i
             c = clips[r.clip]
             if c.poll(tk):
                 c.finish(tk, slot)
+                work -= c.cost
                 lat.append(time.perf_counter() - t0 - r.t)
+                if lat_out is not None:
+                    lat_out.append((r, lat[-1]))
                 if keep is not None:
                     keep[(r.clip, r.frame)] = c.output_host(slot, r.frame)
                 remaining[r.clip] -= 1
@@ -297,11 +310,18 @@ def serve(clips, requests, mode="continuous", keep=None, poll_sleep=20e-6, max_i
         # submit queued requests in arrival order; a busy clip does not block
         # the others (continuous batching), but a clip's frames stay in order
         blocked = set()
+        over_budget = False
         rest = deque()
         while pending:
             r = pending.popleft()
             c = clips[r.clip]
-            if len(inflight) >= max_inflight:
+            if len(inflight) >= max_inflight or over_budget:
+                rest.append(r)
+                continue
+            if max_work is not None and inflight and work + c.cost > max_work:
+                # the oldest request that does not fit holds the later ones
+                # back, so small frames cannot starve a large one
+                over_budget = True
                 rest.append(r)
                 continue
             if r.clip in blocked or (c.clip is None and c.rejected_at == gen):
@@ -329,6 +349,7 @@ def serve(clips, requests, mode="continuous", keep=None, poll_sleep=20e-6, max_i
                 rest.append(r)
                 continue
             inflight.append((res[0], r, res[1]))
+            work += c.cost
             progressed = True
         pending = rest
         peak = max(peak, dev.reserved())
