@@ -155,11 +155,11 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serve-clips", type=int, default=16)
-    ap.add_argument("--serve-frames", type=int, default=24)
+    ap.add_argument("--serve-frames", type=int, default=48)
     ap.add_argument("--serve-rate", default="600,1500", help="offered frame rates (comma-separated)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-poll-us", type=float, default=20.0)
-    ap.add_argument("--serve-inflight", type=int, default=8)
+    ap.add_argument("--serve-inflight", type=int, default=4)
     ap.add_argument("--serve-graphs", action="store_true")
     ap.add_argument("--no-host-path", action="store_true")
     ap.add_argument("--graphs", action="store_true")

@@ -228,7 +228,7 @@ def _percentiles(lat):
     return float(np.percentile(a, 50) * 1e3), float(np.percentile(a, 99) * 1e3)
 
 
-def serve(clips, requests, mode="continuous", keep=None, poll_sleep=20e-6, max_inflight=8, lat_out=None,
+def serve(clips, requests, mode="continuous", keep=None, poll_sleep=20e-6, max_inflight=4, lat_out=None,
           max_work=None):
     """Serve `requests` (sorted by arrival) on `clips` (ResidentClip list).
 
