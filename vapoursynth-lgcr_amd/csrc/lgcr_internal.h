@@ -69,9 +69,23 @@ struct NbrDev {
 };
 
 // Temporal-tap context of the guided pass (recon.cpp:642-745).
+// Temporal tap of one output column (row) for one integer block vector:
+// the neighbour chroma sample it lands on, the residual offset in luma
+// units and the separable kernel weight of that offset (recon.cpp:662-707).
+struct TempTap {
+    int tn;
+    float dd;
+    double k;
+};
+
 struct TemporalDev {
     const NbrDev *nbrs = nullptr;   // device array
     int n = 0;
+    // per-clip tables over (output position, vector + search): TempTap for
+    // x and y, and |sin(pi * frac(v / r))| for x then y
+    const TempTap *tx = nullptr, *ty = nullptr;
+    const float *nsin = nullptr;
+    int search = 0;
     float shift_x = 0.0f, shift_y = 0.0f;  // unused placeholders for alignment
     double sx = 0.0, sy = 0.0;      // frame siting (luma units)
     double rw = 1.0, rh = 1.0;

@@ -1646,11 +1646,16 @@ __device__ __attribute__((always_inline)) void guided_pixel(const GuidedArgs &A,
             const float tconf = nb.conf[bi];
             if (tconf < 0.05f)
                 continue;
-            const float mx = (float)nb.mvx[bi], my = (float)nb.mvy[bi];
-            const int tnx = clampi((int)lroundf(scx + mx / frw), 0, cw - 1);
-            const int tny = clampi((int)lroundf(scy + my / frh), 0, ch - 1);
-            const float ddx = ((float)tnx - scx) * frw - mx;
-            const float ddy = ((float)tny - scy) * frh - my;
+            const int ivx = nb.mvx[bi], ivy = nb.mvy[bi];
+            const float mx = (float)ivx, my = (float)ivy;
+            // target sample, residual and kernel weight per (output column,
+            // vector x) and (output row, vector y), tabulated per clip
+            const int nvec = 2 * T.search + 1;
+            const int jx = clampi(ivx + T.search, 0, nvec - 1), jy = clampi(ivy + T.search, 0, nvec - 1);
+            const TempTap ex = T.tx[(size_t)ox * nvec + jx];
+            const TempTap ey = T.ty[(size_t)oy * nvec + jy];
+            const int tnx = ex.tn, tny = ey.tn;
+            const float ddx = ex.dd, ddy = ey.dd;
             const bool stat = (fabsf(mx) + fabsf(my) <= 1.0f) && tconf > 0.5f;
             float dL;
             if (stat) {
@@ -1674,14 +1679,11 @@ __device__ __attribute__((always_inline)) void guided_pixel(const GuidedArgs &A,
                 const float dxc = ddx / frw * T.inv_wx, dyc = ddy / frh * T.inv_wy;
                 kw = lut_lookup(A.rd.lut, A.rd.lut_n, sqrtf(dxc * dxc + dyc * dyc) * A.rd.lut_scale);
             } else {
-                kw = (float)(dkernel(T.kernel, fabsf(ddx) / T.rw * T.inv_wx, T.kp1, T.kp2) *
-                             dkernel(T.kernel, fabsf(ddy) / T.rh * T.inv_wy, T.kp1, T.kp2));
+                kw = (float)(ex.k * ey.k);
             }
             float nw = 1.0f;
             if (!stat) {
-                const float fx = mx / frw, fy = my / frh;
-                const float px = fx - floorf(fx), py = fy - floorf(fy);
-                nw = fmx(fabsf(sinf(3.14159265358979323846f * px)), fabsf(sinf(3.14159265358979323846f * py)));
+                nw = fmx(T.nsin[jx], T.nsin[nvec + jy]);
                 nw *= 1.0f - ss_ramp;
             }
             const float w = tconf * sim * kw * nw;

@@ -486,7 +486,44 @@ struct Geometry {
     const int4 *tb = nullptr;
     int tb_n = 0, tb_c = 0, tb_rows = 0;
     bool tb_ok = false;
+    // TRecon temporal taps per (output column / row, vector)
+    const TempTap *ttx = nullptr, *tty = nullptr;
+    const float *tsin = nullptr;
 };
+
+// Temporal taps of one axis for every output position and every integer
+// vector v in [-search, search], evaluated exactly as the per-pixel code of
+// recon.cpp:662-707 does: target sample lround(pos + v / r) clamped, residual
+// (target - pos) * r - v, weight kernel(|residual| / r * inv_w) in double.
+static std::vector<TempTap> temporal_taps(const HostAxis &h, int cn, double r, double inv_w, int search,
+                                   const lgcr_params &p) {
+    const int nv = 2 * search + 1;
+    const float fr = float(r), finv = float(inv_w);
+    std::vector<TempTap> t(size_t(h.n) * nv);
+    for (int o = 0; o < h.n; ++o) {
+        const float pos = h.pos[o];
+        for (int v = -search; v <= search; ++v) {
+            const float m = float(v);
+            const int tn = std::min(std::max(int(lroundf(pos + m / fr)), 0), cn - 1);
+            const float dd = (float(tn) - pos) * fr - m;
+            const double k = kernel_eval(p.kernel, double(std::fabs(dd)) / r * double(finv), p.kp1, p.kp2);
+            t[size_t(o) * nv + (v + search)] = TempTap{tn, dd, k};
+        }
+    }
+    return t;
+}
+
+// |sin(pi * frac(v / r))| per integer vector, x then y (recon.cpp:720-727)
+static std::vector<float> temporal_sines(double rw, double rh, int search) {
+    std::vector<float> s;
+    for (const double r : {rw, rh})
+        for (int v = -search; v <= search; ++v) {
+            const float f = float(v) / float(r);
+            const float fr = f - std::floor(f);
+            s.push_back(std::fabs(std::sin(float(M_PI) * fr)));
+        }
+    return s;
+}
 
 struct Workspace {
     hipStream_t stream = nullptr;
@@ -665,6 +702,14 @@ lgcr_status get_geometry(lgcr_clip *c, double sx, double sy, Geometry **out) {
         }
     }
     const size_t osrx = blob.add(shr.xphase), osry = blob.add(shr.yphase), osrw = blob.add(shr.w);
+    std::vector<TempTap> ttx, tty;
+    std::vector<float> tsin;
+    if (c->temporal) {
+        ttx = temporal_taps(ax, c->cw, c->rw, std::min(1.0, double(c->ow) / c->cw), c->p.tsearch, c->p);
+        tty = temporal_taps(ay, c->ch, c->rh, std::min(1.0, double(c->oh) / c->ch), c->p.tsearch, c->p);
+        tsin = temporal_sines(c->rw, c->rh, c->p.tsearch);
+    }
+    const size_t otx = blob.add(ttx), oty = blob.add(tty), otsin = blob.add(tsin);
     auto g = std::make_unique<Geometry>();
     g->sx = sx;
     g->sy = sy;
@@ -691,6 +736,11 @@ lgcr_status get_geometry(lgcr_clip *c, double sx, double sy, Geometry **out) {
         d.lb_f = reinterpret_cast<const float *>(base + o[a][10]);
         d.rec = d.sup <= 6 ? reinterpret_cast<const float4 *>(base + o[a][11]) : nullptr;
         d.tile_rows16 = tile_rows16(*axes[a]);
+    }
+    if (c->temporal) {
+        g->ttx = reinterpret_cast<const TempTap *>(base + otx);
+        g->tty = reinterpret_cast<const TempTap *>(base + oty);
+        g->tsin = reinterpret_cast<const float *>(base + otsin);
     }
     g->cm_x = reinterpret_cast<const int2 *>(base + ocmx);
     g->cm_y = reinterpret_cast<const int2 *>(base + ocmy);
@@ -1528,6 +1578,10 @@ lgcr_status enqueue_trecon(lgcr_clip *c, Workspace *w, const Geometry &g, const 
     TemporalDev td;
     td.nbrs = w->nbr_dev;
     td.n = nn;
+    td.tx = g.ttx;
+    td.ty = g.tty;
+    td.nsin = g.tsin;
+    td.search = p.tsearch;
     td.sx = g.sx;
     td.sy = g.sy;
     td.rw = c->rw;
