@@ -75,9 +75,12 @@ def test_trecon_1080p_p10(device):
 
 
 # (w, h, tsearch): the window kernel holds the block and its search window in
-# LDS up to 64 KiB (tsearch <= 54 at 16x16 blocks); tsearch 60 takes the
-# global-memory kernel.  98x62 has partial blocks on the right and bottom.
-ME_CASES = [(160, 96, 0), (160, 96, 2), (160, 96, 6), (98, 62, 6), (160, 96, 20), (128, 80, 60)]
+# LDS, four blocks per workgroup up to tsearch 23 (at 16x16 blocks), one up to
+# 54; tsearch 60 takes the global-memory kernel.  98x62 has partial blocks on
+# the right and bottom; 160x96 has 60 blocks, so the last four-block group is
+# full, 98x62 has 28, 128x80 40 and 144x80 45 (a partial last group).
+ME_CASES = [(160, 96, 0), (160, 96, 2), (160, 96, 6), (98, 62, 6), (144, 80, 6), (160, 96, 20), (160, 96, 30),
+            (128, 80, 60)]
 
 
 @pytest.mark.parametrize("w,h,search", ME_CASES, ids=[f"{w}x{h}-s{s}" for w, h, s in ME_CASES])

@@ -2334,14 +2334,15 @@ __global__ __launch_bounds__(kMeThreads) void k_block_match(const float *__restr
     conf[idx] = match_q * uniq;
 }
 
-// The same search with the block's search window in LDS: one wave per block.
-// The window holds every neighbour sample a candidate placement inside the
-// image reads (rows y0-search .. y0+bh-1+search, columns likewise; samples
-// outside the image are clamped copies no valid placement reads).  Each lane
-// scores kMeCand candidates per pass, walking the block's samples once in
-// the reference's (j, i) order with one float accumulator per candidate, so
-// every SAD rounds exactly as the scalar loop; the argmin is a wave
-// reduction on (sad, |dx|+|dy|, scan index).
+// The same search with the block's search window in LDS: one wave per block,
+// WAVES blocks per workgroup.  The window holds every neighbour sample a
+// candidate placement inside the image reads (rows y0-search ..
+// y0+bh-1+search, columns likewise; samples outside the image are clamped
+// copies no valid placement reads).  Each lane scores kMeCand candidates per
+// pass, walking the block's samples once in the reference's (j, i) order
+// with one float accumulator per candidate, so every SAD rounds exactly as
+// the scalar loop; the argmin is a wave reduction on (sad, |dx|+|dy|, scan
+// index).
 constexpr int kMeCand = 3;
 constexpr int kMeLdsMax = 64 * 1024;
 
@@ -2353,73 +2354,114 @@ __device__ __forceinline__ bool me_better(float s2, int n2, int i2, float s1, in
     return s2 == s1 && (n2 < n1 || (n2 == n1 && i2 < i1));
 }
 
-__global__ __launch_bounds__(64) void k_block_match_win(const float *__restrict__ cur, const float *__restrict__ nbr,
-                                                        int w, int h, int block, int search, float tsad,
-                                                        int16_t *mvx, int16_t *mvy, float *conf) {
+// SAD of kMeCand candidates over an NX x NY block with compile-time loop
+// bounds, so the window reads of a row use immediate LDS offsets.
+template <int NX, int NY>
+__device__ __forceinline__ void me_sad_fixed(const float *sc, const float *win, int ws, const int (&off)[kMeCand],
+                                             float (&acc)[kMeCand]) {
+#pragma unroll
+    for (int j = 0; j < NY; ++j) {
+        const float *wr = win + 2 * j * ws;
+#pragma unroll
+        for (int i = 0; i < NX; ++i) {
+            const float v = sc[j * NX + i];
+#pragma unroll
+            for (int k = 0; k < kMeCand; ++k)
+                acc[k] += fabsf(v - wr[off[k] + 2 * i]);
+        }
+    }
+}
+
+template <int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_block_match_win(const float *__restrict__ cur,
+                                                                const float *__restrict__ nbr, int w, int h, int block,
+                                                                int search, float tsad, int16_t *mvx, int16_t *mvy,
+                                                                float *conf) {
     extern __shared__ float me_lds[];
-    __shared__ float s_sad0;
-    const int lane = threadIdx.x;
-    const int bw = (w + block - 1) / block;
-    const int bx = blockIdx.x % bw, by = blockIdx.x / bw;
+    __shared__ float s_sad0[WAVES], s_best[WAVES];
+    __shared__ int s_bidx[WAVES];
+    // wave-uniform: the block geometry below lives in scalar registers
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int bw = (w + block - 1) / block, nblocks = bw * ((h + block - 1) / block);
+    // a wave past the last block repeats the last one; its result is not
+    // stored, and every wave reaches the barriers
+    const int blk = min((int)blockIdx.x * WAVES + wave, nblocks - 1);
+    const int bx = blk % bw, by = blk / bw;
     const int x0 = bx * block, y0 = by * block;
     const int bw_ = min(block, w - x0), bh_ = min(block, h - y0);
     const int nx = (bw_ + 1) / 2, ny = (bh_ + 1) / 2;
     const int half = (block + 1) / 2;
     const int ws = block + 2 * search;
-    float *sc = me_lds;                // nx*ny block samples
-    float *win = me_lds + half * half; // ws x ws window
+    const int region = half * half + ws * ws;
+    float *sc = me_lds + wave * region;  // nx*ny block samples
+    float *win = sc + half * half;       // ws x ws window
     for (int t = lane; t < nx * ny; t += 64)
         sc[t] = at(cur, w, h, x0 + 2 * (t % nx), y0 + 2 * (t / nx));
-    // four loads in flight per lane before the first LDS store
-    for (int t0 = 0; t0 < ws * ws; t0 += 256) {
-        float r[4];
+    {
+        // window element t = r * ws + q; (r, q) advance by 64 per step
+        const int dq = 64 % ws, dr = 64 / ws, n = ws * ws;
+        int q = lane % ws, r = lane / ws;
+        for (int t0 = 0; t0 < n; t0 += 256) {
+            float v[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int t = min(t0 + lane + 64 * k, ws * ws - 1);
-            r[k] = at(nbr, w, h, x0 - search + t % ws, y0 - search + t / ws);
+            for (int k = 0; k < 4; ++k) {
+                v[k] = at(nbr, w, h, x0 - search + q, y0 - search + r);  // clamped: in bounds for any (q, r)
+                q += dq;
+                r += dr;
+                if (q >= ws) {
+                    q -= ws;
+                    ++r;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (t0 + lane + 64 * k < n)
+                    win[t0 + lane + 64 * k] = v[k];
         }
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if (t0 + lane + 64 * k < ws * ws)
-                win[t0 + lane + 64 * k] = r[k];
     }
     if (lane == 0)
-        s_sad0 = 1e30f;
+        s_sad0[wave] = 1e30f;
     __syncthreads();
     const int side = 2 * search + 1, ncand = side * side;
     float best = 1e30f;
     int bidx = 0x7fffffff, bnorm = 0x7fffffff;
+    const bool full = nx == 8 && ny == 8;
     for (int base = 0; base < ncand; base += 64 * kMeCand) {
         float acc[kMeCand];
-        int off[kMeCand];
+        int off[kMeCand], cdx[kMeCand], cdy[kMeCand];
+        bool cok[kMeCand];
 #pragma unroll
         for (int k = 0; k < kMeCand; ++k) {
             const int c = base + lane + 64 * k;
-            const int dy = c / side - search, dx = c % side - search;
-            const bool ok = c < ncand && x0 + dx >= 0 && y0 + dy >= 0 && x0 + dx + bw_ <= w && y0 + dy + bh_ <= h;
-            off[k] = ok ? (dy + search) * ws + (dx + search) : 0;
-            acc[k] = ok ? 0.0f : -1.0f;
+            cdy[k] = c / side - search;
+            cdx[k] = c - (cdy[k] + search) * side - search;
+            cok[k] = c < ncand && x0 + cdx[k] >= 0 && y0 + cdy[k] >= 0 && x0 + cdx[k] + bw_ <= w &&
+                     y0 + cdy[k] + bh_ <= h;
+            off[k] = cok[k] ? (cdy[k] + search) * ws + (cdx[k] + search) : 0;
+            acc[k] = 0.0f;
         }
-        for (int j = 0; j < ny; ++j) {
-            const float *srow = sc + j * nx;
-            const int orow = 2 * j * ws;
-            for (int i = 0; i < nx; ++i) {
-                const float v = srow[i];
+        if (full) {
+            me_sad_fixed<8, 8>(sc, win, ws, off, acc);
+        } else {
+            for (int j = 0; j < ny; ++j) {
+                const float *srow = sc + j * nx;
+                const float *wr = win + 2 * j * ws;
+                for (int i = 0; i < nx; ++i) {
+                    const float v = srow[i];
 #pragma unroll
-                for (int k = 0; k < kMeCand; ++k)
-                    acc[k] += fabsf(v - win[off[k] + orow + 2 * i]);
+                    for (int k = 0; k < kMeCand; ++k)
+                        acc[k] += fabsf(v - wr[off[k] + 2 * i]);
+                }
             }
         }
 #pragma unroll
         for (int k = 0; k < kMeCand; ++k) {
-            const int c = base + lane + 64 * k;
-            const int dy = c / side - search, dx = c % side - search;
-            const bool ok = c < ncand && x0 + dx >= 0 && y0 + dy >= 0 && x0 + dx + bw_ <= w && y0 + dy + bh_ <= h;
-            if (!ok)
+            if (!cok[k])
                 continue;
-            if (dx == 0 && dy == 0)
-                s_sad0 = acc[k];
-            const int norm = abs(dx) + abs(dy);
+            if (cdx[k] == 0 && cdy[k] == 0)
+                s_sad0[wave] = acc[k];
+            const int c = base + lane + 64 * k;
+            const int norm = abs(cdx[k]) + abs(cdy[k]);
             if (me_better(acc[k], norm, c, best, bnorm, bidx)) {
                 best = acc[k];
                 bidx = c;
@@ -2438,38 +2480,49 @@ __global__ __launch_bounds__(64) void k_block_match_win(const float *__restrict_
             bnorm = n2;
         }
     }
+    if (lane == 0) {
+        s_best[wave] = best;
+        s_bidx[wave] = bidx;
+    }
     __syncthreads();
-    if (lane != 0)
+    // lane k of wave 0 finishes block k of the workgroup: one sequential
+    // pass over the block's samples (double sums in the reference's order)
+    // per block instead of one per wave
+    if (wave != 0 || lane >= WAVES || (int)blockIdx.x * WAVES + lane >= nblocks)
         return;
-    float bsad = best;
+    const int fb = (int)blockIdx.x * WAVES + lane;
+    const int fx0 = (fb % bw) * block, fy0 = (fb / bw) * block;
+    const int fnx = (min(block, w - fx0) + 1) / 2, fny = (min(block, h - fy0) + 1) / 2;
+    const float *fsc = me_lds + lane * region;
+    float bsad = s_best[lane];
+    const int fidx = s_bidx[lane];
     int bdx = 0, bdy = 0;
-    if (bidx != 0x7fffffff) {
-        bdy = bidx / side - search;
-        bdx = bidx % side - search;
+    if (fidx != 0x7fffffff) {
+        bdy = fidx / side - search;
+        bdx = fidx % side - search;
     }
     double sum = 0.0, sum2 = 0.0;
-    for (int t = 0; t < nx * ny; ++t) {
-        const float v = sc[t];
+    for (int t = 0; t < fnx * fny; ++t) {
+        const float v = fsc[t];
         sum += v;
         sum2 = dacc(sum2, v, v);
     }
-    const float npix = (float)(nx * ny);
-    const float sad0 = s_sad0;
+    const float npix = (float)(fnx * fny);
+    const float sad0 = s_sad0[lane];
     if ((bdx != 0 || bdy != 0) && sad0 - bsad <= 0.10f * tsad * npix) {
         bdx = 0;
         bdy = 0;
         bsad = sad0;
     }
-    const int idx = blockIdx.x;
-    mvx[idx] = (int16_t)bdx;
-    mvy[idx] = (int16_t)bdy;
+    mvx[fb] = (int16_t)bdx;
+    mvy[fb] = (int16_t)bdy;
     const float sad_px = bsad / npix;
     const double vv = sum2 / npix - (sum / npix) * (sum / npix);
     const float var = (float)(0.0 < vv ? vv : 0.0);
     const float inv_t2 = 1.0f / (tsad * tsad);
     const float match_q = 1.0f / (1.0f + sad_px * sad_px * inv_t2);
     const float uniq = var / (var + 0.25f * tsad * tsad);
-    conf[idx] = match_q * uniq;
+    conf[fb] = match_q * uniq;
 }
 
 constexpr int kCcMax = 1024;
@@ -3269,9 +3322,13 @@ hipError_t launch_block_match(const float *cur, const float *nbr, int w, int h, 
     const int bw = (w + block - 1) / block, bh = (h + block - 1) / block;
     const int half = (block + 1) / 2, ws = block + 2 * search;
     const size_t lds = sizeof(float) * ((size_t)half * half + (size_t)ws * ws);
-    if (lds <= (size_t)kMeLdsMax)
-        hipLaunchKernelGGL(k_block_match_win, dim3((unsigned)(bw * bh)), dim3(64), lds, s, cur, nbr, w, h, block,
+    const unsigned nb = (unsigned)(bw * bh);
+    if (4 * lds <= (size_t)kMeLdsMax)
+        hipLaunchKernelGGL(k_block_match_win<4>, dim3((nb + 3) / 4), dim3(256), 4 * lds, s, cur, nbr, w, h, block,
                            search, tsad, mvx, mvy, conf);
+    else if (lds <= (size_t)kMeLdsMax)
+        hipLaunchKernelGGL(k_block_match_win<1>, dim3(nb), dim3(64), lds, s, cur, nbr, w, h, block, search, tsad,
+                           mvx, mvy, conf);
     else
         hipLaunchKernelGGL(k_block_match, dim3((unsigned)(bw * bh)), dim3(kMeThreads), 0, s, cur, nbr, w, h, block,
                            search, tsad, mvx, mvy, conf);
