@@ -789,7 +789,10 @@ __global__ __launch_bounds__(kBlock) void k_gate_fused(const float *__restrict__
     constexpr int kGateGW = kGateTW + 2 * kGateHalo, kGateGH = kGateTH + 2 * kGateHalo;
     constexpr int kGateSW = kGateGW + 2, kGateSH = kGateGH + 2;
     __shared__ float src[3][kGateSH][kGateSW];
-    __shared__ float grd[6][kGateGH][kGateGW];
+    // gradients per sample, interleaved so a bilinear corner is one 16-byte
+    // and one 8-byte read: {gx, gy} of lc and U, then of V
+    __shared__ float4 gyu[kGateGH][kGateGW];
+    __shared__ float2 gvv[kGateGH][kGateGW];
     __shared__ int s_dense;
     unsigned bx, by;
     xcd_block2(bx, by);
@@ -832,12 +835,11 @@ __global__ __launch_bounds__(kBlock) void k_gate_fused(const float *__restrict__
         const int P = clampi(gx0 + c, 0, w - 1), Q = clampi(gy0 + r, 0, h - 1);
         const int xm = clampi(P - 1, 0, w - 1) - sx0, xc = P - sx0, xp = clampi(P + 1, 0, w - 1) - sx0;
         const int ym = clampi(Q - 1, 0, h - 1) - sy0, yc = Q - sy0, yp = clampi(Q + 1, 0, h - 1) - sy0;
-        _Pragma("unroll") for (int q = 0; q < 3; ++q) {
-            float gx, gy;
-            sobel_lds<kGateSW>(src[q], xm, xc, xp, ym, yc, yp, gx, gy);
-            grd[2 * q][r][c] = gx;
-            grd[2 * q + 1][r][c] = gy;
-        }
+        float g[6];
+        _Pragma("unroll") for (int q = 0; q < 3; ++q)
+            sobel_lds<kGateSW>(src[q], xm, xc, xp, ym, yc, yp, g[2 * q], g[2 * q + 1]);
+        gyu[r][c] = make_float4(g[0], g[1], g[2], g[3]);
+        gvv[r][c] = make_float2(g[4], g[5]);
     }
     __syncthreads();
     if (!need) {
@@ -853,8 +855,9 @@ __global__ __launch_bounds__(kBlock) void k_gate_fused(const float *__restrict__
     for (int dy = -1; dy <= 1; ++dy)
         for (int dx = -1; dx <= 1; ++dx) {
             const int r = cy(y + dy), c = cx(x + dx);
-            const float a = grd[0][r][c];
-            const float b = grd[1][r][c];
+            const float4 g = gyu[r][c];
+            const float a = g.x;
+            const float b = g.y;
             sxx += a * a;
             sxy += a * b;
             syy += b * b;
@@ -872,12 +875,18 @@ __global__ __launch_bounds__(kBlock) void k_gate_fused(const float *__restrict__
         const int rx = (int)floor(px), ry = (int)floor(py);
         const int c0 = cx(rx), c1 = cx(rx + 1), r0 = cy(ry), r1 = cy(ry + 1);
         const float fx = (float)(px - rx), fy = (float)(py - ry);
+        const float4 p00 = gyu[r0][c0], p01 = gyu[r0][c1], p10 = gyu[r1][c0], p11 = gyu[r1][c1];
+        const float2 q00 = gvv[r0][c0], q01 = gvv[r0][c1], q10 = gvv[r1][c0], q11 = gvv[r1][c1];
+        const float gxs[3][4] = {{p00.x, p01.x, p10.x, p11.x}, {p00.z, p01.z, p10.z, p11.z},
+                                 {q00.x, q01.x, q10.x, q11.x}};
+        const float gys[3][4] = {{p00.y, p01.y, p10.y, p11.y}, {p00.w, p01.w, p10.w, p11.w},
+                                 {q00.y, q01.y, q10.y, q11.y}};
         float res[3];
         _Pragma("unroll") for (int q = 0; q < 3; ++q) {
-            const float a = grd[2 * q][r0][c0] * nx + grd[2 * q + 1][r0][c0] * ny;
-            const float b = grd[2 * q][r0][c1] * nx + grd[2 * q + 1][r0][c1] * ny;
-            const float c = grd[2 * q][r1][c0] * nx + grd[2 * q + 1][r1][c0] * ny;
-            const float dd = grd[2 * q][r1][c1] * nx + grd[2 * q + 1][r1][c1] * ny;
+            const float a = gxs[q][0] * nx + gys[q][0] * ny;
+            const float b = gxs[q][1] * nx + gys[q][1] * ny;
+            const float c = gxs[q][2] * nx + gys[q][2] * ny;
+            const float dd = gxs[q][3] * nx + gys[q][3] * ny;
             res[q] = a + (b - a) * fx + (c - a) * fy + (a - b - c + dd) * fx * fy;
         }
         const float gY = fabsf(res[0]);
