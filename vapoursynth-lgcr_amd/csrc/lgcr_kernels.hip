@@ -1790,6 +1790,51 @@ __global__ __launch_bounds__(kBlock, SUP == 6 ? (TEMPORAL ? 2 : 4) : 1) void k_g
 // pass of the 16 window rows into LDS, vertical pass per pixel) and stores
 // it where the guided pass does not overwrite it; with a selector the
 // guided pass reads it back as its base.
+// One horizontal plain sum (plainChromaSeparable's row pass,
+// recon.cpp:1151-1461) over six window samples: weighted U/V in double and
+// the 1-D hulls of the active taps.  ALL: every tap is active, so the hull
+// needs no masks (inactive taps carry zero weight in the axis records, so
+// the sums need none either way).
+template <bool ALL>
+__device__ __forceinline__ void plain_h_row(const float4 *row, const PixAxis<6> &pc, float2 &val, float4 &hull) {
+    double vu = 0.0, vv = 0.0;
+    float mnu = 1e30f, mxu = -1e30f, mnv = 1e30f, mxv = -1e30f;
+    _Pragma("unroll") for (int i = 0; i < 6; ++i) {
+        const float4 q = row[i];
+        const float a = q.y, b = q.z;
+        vu = dacc(vu, pc.w[i], a);
+        vv = dacc(vv, pc.w[i], b);
+        const float nmu = fmn(mnu, a), nxu = fmx(mxu, a), nmv = fmn(mnv, b), nxv = fmx(mxv, b);
+        const bool on = ALL || pc.on(i);
+        mnu = on ? nmu : mnu;
+        mxu = on ? nxu : mxu;
+        mnv = on ? nmv : mnv;
+        mxv = on ? nxv : mxv;
+    }
+    val = make_float2((float)vu, (float)vv);
+    hull = make_float4(mnu, mxu, mnv, mxv);
+}
+
+// The vertical pass over six rows of horizontal results (column stride
+// kTile in LDS).
+template <bool ALL>
+__device__ __forceinline__ void plain_v_col(const float2 *val, const float4 *hull, const PixAxis<6> &pr, double &vu,
+                                            double &vv, float &mnu, float &mxu, float &mnv, float &mxv) {
+    _Pragma("unroll") for (int j = 0; j < 6; ++j) {
+        const float2 v = val[j * kTile];
+        const float4 h = hull[j * kTile];
+        vu = dacc(vu, pr.w[j], v.x);
+        vv = dacc(vv, pr.w[j], v.y);
+        const float nmu = fmn(mnu, h.x), nxu = fmx(mxu, h.y);
+        const float nmv = fmn(mnv, h.z), nxv = fmx(mxv, h.w);
+        const bool on = ALL || pr.on(j);
+        mnu = on ? nmu : mnu;
+        mxu = on ? nxu : mxu;
+        mnv = on ? nmv : mnv;
+        mxv = on ? nxv : mxv;
+    }
+}
+
 template <bool TEMPORAL, int SEL, bool PLAIN>
 __global__ __launch_bounds__(kBlock, TEMPORAL ? 4 : (PLAIN ? 5 : 6)) void k_guided_tiles(GuidedArgs A, int tiles_x,
                                                                                        int tiles) {
@@ -1838,49 +1883,31 @@ __global__ __launch_bounds__(kBlock, TEMPORAL ? 4 : (PLAIN ? 5 : 6)) void k_guid
         const int c8 = lane & (kTile - 1);
         const PixAxis<6> pc = pix_axis<6>(A.ax, min(ox, ow - 1));
         const int col0 = pc.start - x_lo;
-        _Pragma("unroll") for (int k = 0; k < 2; ++k) {
-            const int r = (lane >> 3) + 8 * k;
-            double vu = 0.0, vv = 0.0;
-            float mnu = 1e30f, mxu = -1e30f, mnv = 1e30f, mxv = -1e30f;
-            _Pragma("unroll") for (int i = 0; i < 6; ++i) {
-                const float4 q = wb[r * kTStride + col0 + i];
-                const float a = q.y, b = q.z;
-                const bool on = pc.on(i);
-                const float w = on ? pc.w[i] : 0.0f;
-                vu = dacc(vu, w, a);
-                vv = dacc(vv, w, b);
-                const float nmu = fmn(mnu, a), nxu = fmx(mxu, a), nmv = fmn(mnv, b), nxv = fmx(mxv, b);
-                mnu = on ? nmu : mnu;
-                mxu = on ? nxu : mxu;
-                mnv = on ? nmv : mnv;
-                mxv = on ? nxv : mxv;
-            }
-            hval[wave][r][c8] = make_float2((float)vu, (float)vv);
-            hhull[wave][r][c8] = make_float4(mnu, mxu, mnv, mxv);
+        // inactive taps carry zero weight in the axis records; when every
+        // lane's six taps are active (the interior) the hulls need no masks
+        if (__all(pc.bits == 0x3fu)) {
+            _Pragma("unroll") for (int k = 0; k < 2; ++k)
+                plain_h_row<true>(wb + ((lane >> 3) + 8 * k) * kTStride + col0, pc, hval[wave][(lane >> 3) + 8 * k][c8],
+                                  hhull[wave][(lane >> 3) + 8 * k][c8]);
+        } else {
+            _Pragma("unroll") for (int k = 0; k < 2; ++k)
+                plain_h_row<false>(wb + ((lane >> 3) + 8 * k) * kTStride + col0, pc,
+                                   hval[wave][(lane >> 3) + 8 * k][c8], hhull[wave][(lane >> 3) + 8 * k][c8]);
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         __builtin_amdgcn_wave_barrier();
         // vertical pass, normalisation, fallback and hull clamp
         // (plainChromaSeparable, recon.cpp:1151-1461)
+        const PixAxis<6> pr = pix_axis<6>(A.ay, min(oy, oh - 1));
+        const bool vall = __all(pr.bits == 0x3fu);
         if (inside) {
-            const PixAxis<6> pr = pix_axis<6>(A.ay, oy);
             const int t0 = pr.start - y_lo;
             double vu = 0.0, vv = 0.0;
             float mnu = 1e30f, mxu = -1e30f, mnv = 1e30f, mxv = -1e30f;
-            _Pragma("unroll") for (int j = 0; j < 6; ++j) {
-                const bool on = pr.on(j);
-                const float w = on ? pr.w[j] : 0.0f;
-                const float2 val = hval[wave][t0 + j][c8];
-                const float4 hull = hhull[wave][t0 + j][c8];
-                vu = dacc(vu, w, val.x);
-                vv = dacc(vv, w, val.y);
-                const float nmu = fmn(mnu, hull.x), nxu = fmx(mxu, hull.y);
-                const float nmv = fmn(mnv, hull.z), nxv = fmx(mxv, hull.w);
-                mnu = on ? nmu : mnu;
-                mxu = on ? nxu : mxu;
-                mnv = on ? nmv : mnv;
-                mxv = on ? nxv : mxv;
-            }
+            if (vall)
+                plain_v_col<true>(&hval[wave][t0][c8], &hhull[wave][t0][c8], pr, vu, vv, mnu, mxu, mnv, mxv);
+            else
+                plain_v_col<false>(&hval[wave][t0][c8], &hhull[wave][t0][c8], pr, vu, vv, mnu, mxu, mnv, mxv);
             const double wsum = (double)pc.wsum * pr.wsum;
             const double asum = (double)pc.wabs * pr.wabs;
             if (wsum > 1e-6 && wsum >= 0.05 * asum) {

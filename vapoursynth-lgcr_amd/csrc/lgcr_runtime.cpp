@@ -245,9 +245,12 @@ std::vector<float> pack_axis(const HostAxis &h) {
         float *d = r.data() + (size_t)o * 16;
         uint32_t bits = 0;
         for (int t = 0; t < h.sup; ++t) {
-            if (h.am[(size_t)o * h.sup + t] != 0.0f)
+            const bool on = h.am[(size_t)o * h.sup + t] != 0.0f;
+            if (on)
                 bits |= 1u << t;
-            d[8 + t] = h.w.empty() ? 0.0f : h.w[(size_t)o * h.sup + t];
+            // every consumer masks inactive taps; a zero weight lets the
+            // plain pass drop the mask from its sums
+            d[8 + t] = h.w.empty() || !on ? 0.0f : h.w[(size_t)o * h.sup + t];
         }
         d[14] = h.wsum[o];
         d[15] = h.wabs[o];
