@@ -32,6 +32,13 @@ CASES = [
     ("p8-420-upscale", 96, 64, 1, 1, 8, {"width": 160, "height": 120}),
     ("p8-420-bm", 96, 64, 1, 1, 8, {"bm": 1, "cn": 3.0}),
     ("p8-420-sigma", 96, 64, 1, 1, 8, {"sigma": 0.002}),
+    # back-projection inside the guided tiles: every sample type, a frame
+    # whose width and height are not multiples of the 8x8 tile, and 4:2:2
+    # where bp does not apply (packing only)
+    ("p8-420-bp", 96, 64, 1, 1, 8, {"bp": 1.0}),
+    ("ps-420-bp", 96, 64, 1, 1, 32, {"bp": 0.5}),
+    ("p16-420-bp-ragged", 100, 58, 1, 1, 16, {"bp": 0.7}),
+    ("p8-422-bp", 96, 64, 1, 0, 8, {"bp": 0.5}),
 ]
 
 

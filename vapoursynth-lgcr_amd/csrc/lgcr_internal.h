@@ -188,7 +188,11 @@ hipError_t launch_guided(const float *u, const float *v, int cw, int ch, const f
                          const uint32_t *indices, float *out_u, float *out_v,
                          const SelectorDev &sel, const TemporalDev &tmp, hipStream_t s,
                          const uint64_t *mask_bits = nullptr, int mask_w = 0, int mask_h = 0,
-                         int tiles_fit = 0, int plain_in_tiles = 0);
+                         int tiles_fit = 0, int plain_in_tiles = 0,
+                         // non-null: the tiled pass also back-projects
+                         // (bp_on) and writes these two output planes of
+                         // out_type itself (out_u/out_v are not written)
+                         const RawOut *fused_out = nullptr, int out_type = 0, float bp = 0.0f, int bp_on = 0);
 // whether every 8x8 output tile's chroma footprint (6 taps per axis) fits
 // the LDS window of the tiled guided pass
 int guided_tiles_fit(const int *xs, int ow, const int *ys, int oh);

@@ -1292,49 +1292,12 @@ struct GuidedArgs {
     TemporalDev tmp;
     const uint64_t *mbits;   // trust mask bits at mask resolution (tile kernel), nullptr = dense
     int mw, mh, mw64;
+    // tile kernel with fused output: back-projection (bp_on) and packing
+    // straight into the output planes
+    RawOut po_u, po_v;
+    float bp;
+    int bp_on;
 };
-
-// double-precision closed-form kernels for the temporal tap weight
-// (lgcr.h:53-95; the host builds every other table)
-__device__ double dsinc(double x) {
-    if (fabs(x) < 1e-8)
-        return 1.0;
-    x *= 3.14159265358979323846;
-    return sin(x) / x;
-}
-
-__device__ double dkernel(int k, double x, double p1, double p2) {
-    x = fabs(x);
-    switch (k) {
-    case 0:
-        return x < 1.0 ? 1.0 - x : 0.0;
-    case 1: {
-        const double b = p1, c = p2;
-        if (x < 1.0)
-            return ((12 - 9 * b - 6 * c) * x * x * x + (-18 + 12 * b + 6 * c) * x * x + (6 - 2 * b)) / 6.0;
-        if (x < 2.0)
-            return ((-b - 6 * c) * x * x * x + (6 * b + 30 * c) * x * x + (-12 * b - 48 * c) * x + (8 * b + 24 * c)) / 6.0;
-        return 0.0;
-    }
-    case 2:
-        return x < p1 ? dsinc(x) * dsinc(x / p1) : 0.0;
-    case 3:
-        if (x < 1.0)
-            return ((x - 9.0 / 5.0) * x - 1.0 / 5.0) * x + 1.0;
-        if (x < 2.0)
-            return ((-1.0 / 3.0 * (x - 1.0) + 4.0 / 5.0) * (x - 1.0) - 7.0 / 15.0) * (x - 1.0);
-        return 0.0;
-    case 4:
-        if (x < 1.0)
-            return ((13.0 / 11.0 * x - 453.0 / 209.0) * x - 3.0 / 209.0) * x + 1.0;
-        if (x < 2.0)
-            return ((-6.0 / 11.0 * (x - 1.0) + 270.0 / 209.0) * (x - 1.0) - 156.0 / 209.0) * (x - 1.0);
-        if (x < 3.0)
-            return ((1.0 / 11.0 * (x - 2.0) - 45.0 / 209.0) * (x - 2.0) + 26.0 / 209.0) * (x - 2.0);
-        return 0.0;
-    }
-    return 0.0;
-}
 
 // True when the float selector keeps compressed per-active arrays: a sparse
 // workset that is not in dense fallback (plugin.cpp:455-456).
@@ -1448,7 +1411,7 @@ __device__ __forceinline__ void guided_taps(const GuidedArgs &A, const PixAxis<S
 template <bool RADIAL, bool TEMPORAL, int SEL, bool CEDGE, int SUP, bool WIN = false>
 __device__ __attribute__((always_inline)) void guided_pixel(const GuidedArgs &A, int ox, int oy, uint32_t k,
                                                             const float4 *win = nullptr, int wx0 = 0,
-                                                            int wy0 = 0) {
+                                                            int wy0 = 0, float2 *res = nullptr) {
     const GuideParams &gp = A.gp;
     const int cw = A.cw, ch = A.ch, sw = A.sw, sh = A.sh;
     const bool guided = gp.strength > 0.0f;
@@ -1764,6 +1727,10 @@ __device__ __attribute__((always_inline)) void guided_pixel(const GuidedArgs &A,
             vv = A.out_v[o];
         }
     }
+    if (res) {
+        *res = make_float2((float)vu, (float)vv);
+        return;
+    }
     A.out_u[o] = (float)vu;
     A.out_v[o] = (float)vv;
 }
@@ -1835,9 +1802,13 @@ __device__ __forceinline__ void plain_v_col(const float2 *val, const float4 *hul
     }
 }
 
-template <bool TEMPORAL, int SEL, bool PLAIN>
+// OUTT: 0 = results to out_u/out_v as floats; otherwise the tile also
+// applies back-projection (A.bp_on) and writes the output planes itself
+// (1 = 8-bit, 2 = 16-bit, 3 = float samples).
+template <bool TEMPORAL, int SEL, bool PLAIN, int OUTT = 0>
 __global__ __launch_bounds__(kBlock, TEMPORAL ? 4 : (PLAIN ? 5 : 6)) void k_guided_tiles(GuidedArgs A, int tiles_x,
                                                                                        int tiles) {
+    static_assert(OUTT == 0 || (PLAIN && !TEMPORAL && SEL == SEL_NONE), "fused output needs every pixel's result");
     __shared__ float4 wins[kBlock / 64][kTWin * kTStride];
     __shared__ float2 hval[PLAIN ? kBlock / 64 : 1][kTWin][kTile];
     __shared__ float4 hhull[PLAIN ? kBlock / 64 : 1][kTWin][kTile];
@@ -1851,6 +1822,7 @@ __global__ __launch_bounds__(kBlock, TEMPORAL ? 4 : (PLAIN ? 5 : 6)) void k_guid
     const int tx = t % tiles_x, ty = t / tiles_x;
     const int ox = tx * kTile + (lane & (kTile - 1)), oy = ty * kTile + (lane / kTile);
     const bool inside = ox < ow && oy < oh;
+    float2 res = make_float2(0.0f, 0.0f);  // this pixel's result (fused output)
     bool act = inside;
     if (act && A.mbits) {
         // nearest mask sample (buildSparseWorkset, maps.cpp:342-360); the
@@ -1921,17 +1893,60 @@ __global__ __launch_bounds__(kBlock, TEMPORAL ? 4 : (PLAIN ? 5 : 6)) void k_guid
                 vu = dclamp(vu, (double)(mnu - A.gp.margin), (double)(mxu + A.gp.margin));
                 vv = dclamp(vv, (double)(mnv - A.gp.margin), (double)(mxv + A.gp.margin));
             }
-            if (!act || SEL != SEL_NONE) {
+            if (OUTT != 0) {
+                res = make_float2((float)vu, (float)vv);
+            } else if (!act || SEL != SEL_NONE) {
                 const uint32_t o = (uint32_t)(oy * ow + ox);
                 A.out_u[o] = (float)vu;
                 A.out_v[o] = (float)vv;
             }
         }
-        if (!any)
+        if (OUTT == 0 && !any)
             return;
     }
     if (act)
-        guided_pixel<false, TEMPORAL, SEL, false, 6, true>(A, ox, oy, 0u, wb, x_lo, y_lo);
+        guided_pixel<false, TEMPORAL, SEL, false, 6, true>(A, ox, oy, 0u, wb, x_lo, y_lo,
+                                                           OUTT != 0 ? &res : nullptr);
+    if constexpr (OUTT != 0) {
+        // back-projection (backProject, algos.cpp:663-677): same-size 4:2:0
+        // with an even output, so the 2x2 block of chroma sample (ox/2,
+        // oy/2) is lanes l, l^1, l^8, l^9 of this tile; every lane forms
+        // 0.25 * (p00 + p10 + p01 + p11) in that order, then its own sample
+        float fu = res.x, fv = res.y;
+        if (A.bp_on) {
+            const float xu = __shfl_xor(fu, 1), yu = __shfl_xor(fu, 8), du = __shfl_xor(fu, 9);
+            const float xv = __shfl_xor(fv, 1), yv = __shfl_xor(fv, 8), dv = __shfl_xor(fv, 9);
+            const bool ex = lane & 1, ey = (lane >> 3) & 1;
+            // p00 p10 p01 p11 seen from this lane's corner
+            const float u00 = ex ? (ey ? du : xu) : (ey ? yu : fu);
+            const float u10 = ex ? (ey ? yu : fu) : (ey ? du : xu);
+            const float u01 = ex ? (ey ? xu : du) : (ey ? fu : yu);
+            const float u11 = ex ? (ey ? fu : yu) : (ey ? xu : du);
+            const float v00 = ex ? (ey ? dv : xv) : (ey ? yv : fv);
+            const float v10 = ex ? (ey ? yv : fv) : (ey ? dv : xv);
+            const float v01 = ex ? (ey ? xv : dv) : (ey ? fv : yv);
+            const float v11 = ex ? (ey ? fv : yv) : (ey ? xv : dv);
+            if (inside) {
+                const uint32_t ci = (uint32_t)((oy >> 1) * cw + (ox >> 1));
+                fu += A.bp * (A.u[ci] - 0.25f * (u00 + u10 + u01 + u11));
+                fv += A.bp * (A.v[ci] - 0.25f * (v00 + v10 + v01 + v11));
+            }
+        }
+        if (inside) {
+            ptrdiff_t su, sv;
+            char *bu = plane_out(A.po_u, su), *bv = plane_out(A.po_v, sv);
+            if constexpr (OUTT == 1) {
+                reinterpret_cast<uint8_t *>(bu + su * oy)[ox] = pack_sample<uint8_t>(fu, A.po_u);
+                reinterpret_cast<uint8_t *>(bv + sv * oy)[ox] = pack_sample<uint8_t>(fv, A.po_v);
+            } else if constexpr (OUTT == 2) {
+                reinterpret_cast<uint16_t *>(bu + su * oy)[ox] = pack_sample<uint16_t>(fu, A.po_u);
+                reinterpret_cast<uint16_t *>(bv + sv * oy)[ox] = pack_sample<uint16_t>(fv, A.po_v);
+            } else {
+                reinterpret_cast<float *>(bu + su * oy)[ox] = fu;
+                reinterpret_cast<float *>(bv + sv * oy)[ox] = fv;
+            }
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -3203,13 +3218,31 @@ hipError_t launch_guided(const float *u, const float *v, int cw, int ch, const f
                          const AxisDev &ax, const AxisDev &ay, const RadialDev &rd, const GuideParams &gp,
                          const uint32_t *row_offsets, const uint32_t *indices, float *out_u, float *out_v,
                          const SelectorDev &sel, const TemporalDev &tmp, hipStream_t s, const uint64_t *mask_bits,
-                         int mask_w, int mask_h, int tiles_fit, int plain_in_tiles) {
+                         int mask_w, int mask_h, int tiles_fit, int plain_in_tiles, const RawOut *fused_out,
+                         int out_type, float bp, int bp_on) {
     GuidedArgs A{u, v, cw, ch, y, sw, sh, yt, luv, lc, ms, ax, ay, rd, gp, row_offsets, indices, out_u, out_v,
                  sel, tmp, indices ? mask_bits : nullptr, mask_w, mask_h, (mask_w + 63) / 64};
     if (guided_tiled(tiles_fit, gp.radial, gp.cedge, ax, ay, sel.mode)) {
         const int tiles_x = (ax.n + kTile - 1) / kTile, tiles = tiles_x * ((ay.n + kTile - 1) / kTile);
         const dim3 g((unsigned)((tiles + kBlock / 64 - 1) / (kBlock / 64)));
 #define LGCR_TILES(T, S_, P) hipLaunchKernelGGL((k_guided_tiles<T, S_, P>), g, dim3(kBlock), 0, s, A, tiles_x, tiles)
+        if (fused_out) {
+            // the caller checked: plain base in the tiles, no selector, no
+            // temporal taps, and bp_on only for same-size 4:2:0 with even sizes
+            if (!plain_in_tiles || tmp.n > 0 || sel.mode != SEL_NONE || (bp_on && ((ax.n | ay.n) & 1)))
+                return hipErrorInvalidValue;
+            A.po_u = fused_out[0];
+            A.po_v = fused_out[1];
+            A.bp = bp;
+            A.bp_on = bp_on;
+            if (out_type == RAW_U8)
+                hipLaunchKernelGGL((k_guided_tiles<false, SEL_NONE, true, 1>), g, dim3(kBlock), 0, s, A, tiles_x, tiles);
+            else if (out_type == RAW_U16)
+                hipLaunchKernelGGL((k_guided_tiles<false, SEL_NONE, true, 2>), g, dim3(kBlock), 0, s, A, tiles_x, tiles);
+            else
+                hipLaunchKernelGGL((k_guided_tiles<false, SEL_NONE, true, 3>), g, dim3(kBlock), 0, s, A, tiles_x, tiles);
+            return hipGetLastError();
+        }
         if (tmp.n > 0) {
             if (plain_in_tiles)
                 LGCR_TILES(true, SEL_NONE, true);

@@ -1202,6 +1202,10 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
 
     const double oscale = is_float ? 1.0 : double((1 << c->f.bits_per_sample) - 1);
     const int hi = is_float ? 0 : (1 << c->f.bits_per_sample) - 1;
+    const float out_offset = is_float ? 0.0f : float(0.5 * oscale);
+    // back-projection applies to same-size 4:2:0 only (plugin.cpp:374-375)
+    const bool bp_same = p.bp > 0.0 && c->same && c->cw * 2 == c->sw && c->ch * 2 == c->sh;
+    bool fused_out = false;  // output written by the guided tiles
     // ResampleLuma (plugin.cpp:273-316)
     if (!c->same) {
         if (p.radial) {
@@ -1318,9 +1322,18 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
                 sel.w3p = w->selp + 3 * no;
             }
         }
+        // the tiles also back-project and pack the output when every pixel's
+        // result is final there (no selector, bm or detail pass after it)
+        fused_out = plain_in_tiles && !algo4 && !p.bm &&
+                    guided_tiled(g.tiles_fit, gp.radial, gp.cedge, g.ax, g.ay, SEL_NONE);
+        const RawOut fo[2] = {
+            {nullptr, out[1], out_stride[1], c->ow, c->oh, float(oscale), out_offset, 0, hi, io, 1},
+            {nullptr, out[2], out_stride[2], c->ow, c->oh, float(oscale), out_offset, 0, hi, io, 2},
+        };
         HIP_TRY(launch_guided(w->u, w->v, c->cw, c->ch, w->y, c->sw, c->sh, w->yt, w->luv, w->lc,
                               gate, g.ax, g.ay, g.rd, gp, w->row_off, idx, w->out_u, w->out_v, sel,
-                              TemporalDev{}, s, w->mask, c->sw, c->sh, g.tiles_fit, plain_in_tiles));
+                              TemporalDev{}, s, w->mask, c->sw, c->sh, g.tiles_fit, plain_in_tiles,
+                              fused_out ? fo : nullptr, c->raw_type, float(p.bp), bp_same ? 1 : 0));
         HIP_TRY(mark(LGCR_STAGE_APPLY_GUIDED_CORRECTION));
         if (algo4 && is_float) {
             HIP_TRY(launch_roi_mark(w->row_off, idx, c->ow, c->oh, sel, g.ax, g.ay, w->roi, c->cw, c->ch,
@@ -1393,13 +1406,16 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
         HIP_TRY(mark(LGCR_STAGE_APPLY_COLLABORATIVE_FILTER));
     }
     // BackProject, same-size 4:2:0 only (plugin.cpp:374-375, 683-689), fused
-    // with ConvertOutput (plugin.cpp:691-711)
-    const float offs = is_float ? 0.0f : float(0.5 * oscale);
+    // with ConvertOutput (plugin.cpp:691-711); done inside the guided tiles
+    // when fused_out
     RawOut ro[2] = {
-        {w->out_u, out[1], out_stride[1], c->ow, c->oh, float(oscale), offs, 0, hi, io, 1},
-        {w->out_v, out[2], out_stride[2], c->ow, c->oh, float(oscale), offs, 0, hi, io, 2},
+        {w->out_u, out[1], out_stride[1], c->ow, c->oh, float(oscale), out_offset, 0, hi, io, 1},
+        {w->out_v, out[2], out_stride[2], c->ow, c->oh, float(oscale), out_offset, 0, hi, io, 2},
     };
-    if (p.bp > 0.0 && c->same && c->cw * 2 == c->sw && c->ch * 2 == c->sh) {
+    if (fused_out) {
+        if (bp_same)
+            HIP_TRY(mark(LGCR_STAGE_BACK_PROJECT));
+    } else if (bp_same) {
         HIP_TRY(launch_bp_pack(w->out_u, w->out_v, c->ow, c->oh, w->u, w->v, c->cw, c->ch, float(p.bp), ro,
                                c->raw_type, c->capture ? 1 : 0, s));
         HIP_TRY(mark(LGCR_STAGE_BACK_PROJECT));
