@@ -1910,26 +1910,19 @@ __global__ __launch_bounds__(kBlock, TEMPORAL ? 4 : (PLAIN ? 5 : 6)) void k_guid
     if constexpr (OUTT != 0) {
         // back-projection (backProject, algos.cpp:663-677): same-size 4:2:0
         // with an even output, so the 2x2 block of chroma sample (ox/2,
-        // oy/2) is lanes l, l^1, l^8, l^9 of this tile; every lane forms
-        // 0.25 * (p00 + p10 + p01 + p11) in that order, then its own sample
+        // oy/2) is lanes l, l^1, l^8, l^9 of this tile.  The block's top-left
+        // lane forms p00 + p10 + p01 + p11 in that order and the other three
+        // read its sum.
         float fu = res.x, fv = res.y;
         if (A.bp_on) {
-            const float xu = __shfl_xor(fu, 1), yu = __shfl_xor(fu, 8), du = __shfl_xor(fu, 9);
-            const float xv = __shfl_xor(fv, 1), yv = __shfl_xor(fv, 8), dv = __shfl_xor(fv, 9);
-            const bool ex = lane & 1, ey = (lane >> 3) & 1;
-            // p00 p10 p01 p11 seen from this lane's corner
-            const float u00 = ex ? (ey ? du : xu) : (ey ? yu : fu);
-            const float u10 = ex ? (ey ? yu : fu) : (ey ? du : xu);
-            const float u01 = ex ? (ey ? xu : du) : (ey ? fu : yu);
-            const float u11 = ex ? (ey ? fu : yu) : (ey ? xu : du);
-            const float v00 = ex ? (ey ? dv : xv) : (ey ? yv : fv);
-            const float v10 = ex ? (ey ? yv : fv) : (ey ? dv : xv);
-            const float v01 = ex ? (ey ? xv : dv) : (ey ? fv : yv);
-            const float v11 = ex ? (ey ? fv : yv) : (ey ? xv : dv);
+            float su = ((fu + __shfl_xor(fu, 1)) + __shfl_xor(fu, 8)) + __shfl_xor(fu, 9);
+            float sv = ((fv + __shfl_xor(fv, 1)) + __shfl_xor(fv, 8)) + __shfl_xor(fv, 9);
+            su = __shfl(su, lane & ~9);
+            sv = __shfl(sv, lane & ~9);
             if (inside) {
                 const uint32_t ci = (uint32_t)((oy >> 1) * cw + (ox >> 1));
-                fu += A.bp * (A.u[ci] - 0.25f * (u00 + u10 + u01 + u11));
-                fv += A.bp * (A.v[ci] - 0.25f * (v00 + v10 + v01 + v11));
+                fu += A.bp * (A.u[ci] - 0.25f * su);
+                fv += A.bp * (A.v[ci] - 0.25f * sv);
             }
         }
         if (inside) {
