@@ -89,20 +89,27 @@ def test_bench_frame_no_capture(no_capture):
         clip.close()
 
 
-@pytest.mark.parametrize("fmt", ["420p10", "444ps"])
-def test_trecon_no_capture(no_capture, fmt):
-    ssw, ssh, bits = (1, 1, 10) if fmt == "420p10" else (0, 0, 32)
-    frames = [frame(96, 64, ss_w=ssw, ss_h=ssh, bits=bits, seed=5, frame=k, motion=True) for k in range(3)]
-    tp = abi.trecon_params(sparse=1)
-    clip = abi.Clip(no_capture, tp, abi.make_format(frames[0][0].dtype, 96, 64, ssw, ssh,
+TRECON_FORMATS = {"420p8": (1, 1, 8), "420p10": (1, 1, 10), "422p16": (1, 0, 16), "444ps": (0, 0, 32)}
+# sparse 1: plain base in the tiles; sparse 0 (the TRecon default): the dense
+# temporal tiles write the packed output themselves
+TRECON_CASES = [("420p10", 1, 96, 64), ("444ps", 1, 96, 64), ("420p8", 0, 96, 64), ("420p10", 0, 96, 64),
+                ("422p16", 0, 98, 62), ("444ps", 0, 96, 64)]
+
+
+@pytest.mark.parametrize("fmt,sparse,w,h", TRECON_CASES, ids=[f"{f}-sparse{s}-{w}x{h}" for f, s, w, h in TRECON_CASES])
+def test_trecon_no_capture(no_capture, fmt, sparse, w, h):
+    ssw, ssh, bits = TRECON_FORMATS[fmt]
+    frames = [frame(w, h, ss_w=ssw, ss_h=ssh, bits=bits, seed=5, frame=k, motion=True) for k in range(3)]
+    tp = abi.trecon_params(sparse=sparse)
+    clip = abi.Clip(no_capture, tp, abi.make_format(frames[0][0].dtype, w, h, ssw, ssh,
                                                     None if bits in (8, 16, 32) else bits), temporal=True)
     try:
         got = clip.trecon_host(frames[1], [frames[0], frames[2]])
-        ref = oracle.trecon_frame(oracle.trecon_params(sparse=1), frames[1], [frames[0], frames[2]],
+        ref = oracle.trecon_frame(oracle.trecon_params(sparse=sparse), frames[1], [frames[0], frames[2]],
                                   bits=None if bits in (8, 16, 32) else bits, ss_w=ssw, ss_h=ssh)
         for i in range(3):
             n, mx = compare_planes(ref[i], got[i], bits == 32)
-            print(f"trecon {fmt} plane {i}: mismatches={n} max_abs={mx:g}")
+            print(f"trecon {fmt} sparse {sparse} {w}x{h} plane {i}: mismatches={n} max_abs={mx:g}")
             assert n == 0
     finally:
         clip.close()

@@ -1808,7 +1808,9 @@ __device__ __forceinline__ void plain_v_col(const float2 *val, const float4 *hul
 template <bool TEMPORAL, int SEL, bool PLAIN, int OUTT = 0>
 __global__ __launch_bounds__(kBlock, TEMPORAL ? 4 : (PLAIN ? 5 : 6)) void k_guided_tiles(GuidedArgs A, int tiles_x,
                                                                                        int tiles) {
-    static_assert(OUTT == 0 || (PLAIN && !TEMPORAL && SEL == SEL_NONE), "fused output needs every pixel's result");
+    // fused output needs every inside pixel's result in the tile: the plain
+    // base in the tiles (Recon), or a dense temporal pass (TRecon, no mask)
+    static_assert(OUTT == 0 || (SEL == SEL_NONE && PLAIN != TEMPORAL), "fused output needs every pixel's result");
     __shared__ float4 wins[kBlock / 64][kTWin * kTStride];
     __shared__ float2 hval[PLAIN ? kBlock / 64 : 1][kTWin][kTile];
     __shared__ float4 hhull[PLAIN ? kBlock / 64 : 1][kTWin][kTile];
@@ -3220,20 +3222,30 @@ hipError_t launch_guided(const float *u, const float *v, int cw, int ch, const f
         const dim3 g((unsigned)((tiles + kBlock / 64 - 1) / (kBlock / 64)));
 #define LGCR_TILES(T, S_, P) hipLaunchKernelGGL((k_guided_tiles<T, S_, P>), g, dim3(kBlock), 0, s, A, tiles_x, tiles)
         if (fused_out) {
-            // the caller checked: plain base in the tiles, no selector, no
-            // temporal taps, and bp_on only for same-size 4:2:0 with even sizes
-            if (!plain_in_tiles || tmp.n > 0 || sel.mode != SEL_NONE || (bp_on && ((ax.n | ay.n) & 1)))
+            // the caller checked: no selector, and either the plain base in
+            // the tiles without temporal taps (bp_on only for same-size 4:2:0
+            // with even sizes) or a dense temporal pass without bp
+            const bool recon = plain_in_tiles && tmp.n == 0 && !(bp_on && ((ax.n | ay.n) & 1));
+            const bool trecon = !plain_in_tiles && tmp.n > 0 && !A.mbits && !bp_on;
+            if (sel.mode != SEL_NONE || !(recon || trecon))
                 return hipErrorInvalidValue;
             A.po_u = fused_out[0];
             A.po_v = fused_out[1];
             A.bp = bp;
             A.bp_on = bp_on;
-            if (out_type == RAW_U8)
-                hipLaunchKernelGGL((k_guided_tiles<false, SEL_NONE, true, 1>), g, dim3(kBlock), 0, s, A, tiles_x, tiles);
-            else if (out_type == RAW_U16)
-                hipLaunchKernelGGL((k_guided_tiles<false, SEL_NONE, true, 2>), g, dim3(kBlock), 0, s, A, tiles_x, tiles);
-            else
-                hipLaunchKernelGGL((k_guided_tiles<false, SEL_NONE, true, 3>), g, dim3(kBlock), 0, s, A, tiles_x, tiles);
+#define LGCR_FUSED(T, P)                                                                                          \
+    if (out_type == RAW_U8)                                                                                     \
+        hipLaunchKernelGGL((k_guided_tiles<T, SEL_NONE, P, 1>), g, dim3(kBlock), 0, s, A, tiles_x, tiles);      \
+    else if (out_type == RAW_U16)                                                                               \
+        hipLaunchKernelGGL((k_guided_tiles<T, SEL_NONE, P, 2>), g, dim3(kBlock), 0, s, A, tiles_x, tiles);      \
+    else                                                                                                        \
+        hipLaunchKernelGGL((k_guided_tiles<T, SEL_NONE, P, 3>), g, dim3(kBlock), 0, s, A, tiles_x, tiles);
+            if (recon) {
+                LGCR_FUSED(false, true)
+            } else {
+                LGCR_FUSED(true, false)
+            }
+#undef LGCR_FUSED
             return hipGetLastError();
         }
         if (tmp.n > 0) {

@@ -1610,10 +1610,18 @@ lgcr_status enqueue_trecon(lgcr_clip *c, Workspace *w, const Geometry &g, const 
     td.kp2 = p.kp2;
     td.inv_wx = float(std::min(1.0, double(c->ow) / c->cw));
     td.inv_wy = float(std::min(1.0, double(c->oh) / c->ch));
+    // a dense temporal pass leaves every pixel's result in its tile, which
+    // then packs the output itself (no bm after it, no capture)
+    const float offs = is_float ? 0.0f : float(0.5 * oscale);
+    RawOut ro[2] = {
+        {w->out_u, out[1], out_stride[1], c->ow, c->oh, float(oscale), offs, 0, hi},
+        {w->out_v, out[2], out_stride[2], c->ow, c->oh, float(oscale), offs, 0, hi},
+    };
+    const bool fused = tiled && nn > 0 && !plain_in_tiles && !w->has_ws && !p.bm && !c->capture;
     HIP_TRY(launch_guided(w->u, w->v, c->cw, c->ch, w->y, c->sw, c->sh, w->yt, w->luv, w->lc,
                           gp.ms_on ? w->ms : nullptr, g.ax, g.ay, g.rd, gp, w->row_off,
                           w->has_ws ? w->indices : nullptr, w->out_u, w->out_v, SelectorDev{}, td, s, w->mask,
-                          c->sw, c->sh, g.tiles_fit, plain_in_tiles));
+                          c->sw, c->sh, g.tiles_fit, plain_in_tiles, fused ? ro : nullptr, c->raw_type, 0.0f, 0));
     HIP_TRY(mark(LGCR_STAGE_APPLY_GUIDED_CORRECTION));
     // collaborative filter after the temporal pass (plugin.cpp:898-899)
     if (p.bm) {
@@ -1621,12 +1629,8 @@ lgcr_status enqueue_trecon(lgcr_clip *c, Workspace *w, const Geometry &g, const 
                                      w->bm_guide, w->bm_scratch, s));
         HIP_TRY(mark(LGCR_STAGE_APPLY_COLLABORATIVE_FILTER));
     }
-    const float offs = is_float ? 0.0f : float(0.5 * oscale);
-    RawOut ro[2] = {
-        {w->out_u, out[1], out_stride[1], c->ow, c->oh, float(oscale), offs, 0, hi},
-        {w->out_v, out[2], out_stride[2], c->ow, c->oh, float(oscale), offs, 0, hi},
-    };
-    HIP_TRY(launch_from_float(ro, 2, c->raw_type, s));
+    if (!fused)
+        HIP_TRY(launch_from_float(ro, 2, c->raw_type, s));
     HIP_TRY(mark(LGCR_STAGE_CONVERT_OUTPUT));
     return LGCR_OK;
 }
