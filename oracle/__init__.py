@@ -343,3 +343,58 @@ def trecon_frame(params, center, nbrs, bits=None, ss_w=1, ss_h=1):
     if rc != 0:
         raise RuntimeError(f"oracle trecon_frame failed ({rc})")
     return outs
+
+
+class DownsampleParams(C.Structure):
+    _fields_ = [("quality", C.c_int), ("kernel", C.c_int), ("strength", C.c_float), ("shift_x", C.c_float),
+                ("shift_y", C.c_float)]
+
+
+DOWNSAMPLE_KERNELS = {"spline36": 0, "lanczos3": 1, "binomial": 2}
+# loc -> (H.273 _ChromaLocation, shift x, shift y) (downsample.cpp:875-893)
+DOWNSAMPLE_LOCATIONS = {"left": (0, -0.5, 0.0), "center": (1, 0.0, 0.0), "topleft": (2, -0.5, -0.5),
+                        "top": (3, 0.0, -0.5), "bottomleft": (4, -0.5, 0.5), "bottom": (5, 0.0, 0.5)}
+
+
+def downsample_params(quality=0, kernel="spline36", strength=1.0, loc="left"):
+    _, sx, sy = DOWNSAMPLE_LOCATIONS[loc]
+    return DownsampleParams(int(quality), DOWNSAMPLE_KERNELS[kernel], float(strength), sx, sy)
+
+
+def _ds_call(fn, params, y, u, v, bits, outs):
+    is_float = y.dtype == np.float32
+    if bits is None:
+        bits = 32 if is_float else (8 if y.dtype == np.uint8 else 16)
+    fmt = FrameFormat(1 if is_float else 0, bits, y.dtype.itemsize, 0, 0, y.shape[1], y.shape[0])
+    ins = [np.ascontiguousarray(a) for a in (y, u, v)]
+    return fmt, ins, fn(C.byref(params), C.byref(fmt), (C.c_void_p * 3)(*[a.ctypes.data for a in ins]),
+                        (C.c_ssize_t * 3)(*[a.strides[0] for a in ins]), *outs)
+
+
+def downsample_frame(params, y, u, v, bits=None):
+    """Downsample 4:4:4 -> 4:2:0 (downsampleGetFrame, downsample.cpp:725-765):
+    returns (Y, U, V) with Y the source plane and U, V at half size in the
+    source sample type."""
+    h, w = y.shape
+    ou = np.zeros((h // 2, w // 2), y.dtype)
+    ov = np.zeros((h // 2, w // 2), y.dtype)
+    L = lib()
+    L.or_downsample_frame.restype = C.c_int
+    outs = ((C.c_void_p * 3)(None, ou.ctypes.data, ov.ctypes.data), (C.c_ssize_t * 3)(0, ou.strides[0], ov.strides[0]))
+    _, _, rc = _ds_call(L.or_downsample_frame, params, y, u, v, bits, outs)
+    if rc != 0:
+        raise RuntimeError(f"oracle downsample_frame failed ({rc})")
+    return y, ou, ov
+
+
+def downsample_base(params, y, u, v, bits=None):
+    """The plain separable baseline (BaselineRing) as float U, V planes."""
+    h, w = y.shape
+    bu = np.zeros((h // 2, w // 2), np.float32)
+    bv = np.zeros((h // 2, w // 2), np.float32)
+    L = lib()
+    L.or_downsample_base.restype = C.c_int
+    _, _, rc = _ds_call(L.or_downsample_base, params, y, u, v, bits, (fptr(bu), fptr(bv)))
+    if rc != 0:
+        raise RuntimeError(f"oracle downsample_base failed ({rc})")
+    return bu, bv

@@ -2643,3 +2643,465 @@ int or_trecon_frame(const or_params *pin, const or_frame_format *f,
     free(nbrs);
     return 0;
 }
+
+/* ------------------------------------------------------------------------
+ * Downsample 4:4:4 -> 4:2:0 (downsample.cpp), scalar build: every sum is a
+ * float accumulation in the order written there.
+ * ---------------------------------------------------------------------- */
+
+typedef struct ds_plane {
+    const uint8_t *data;
+    ptrdiff_t stride;
+    int w, h, type; /* type: 0 u8, 1 u16, 2 f32 */
+    float scale, offset;
+} ds_plane;
+
+/* SourcePlane::at (downsample.cpp:101-106): clamped sample * scale + offset */
+static float ds_at(const ds_plane *p, int x, int y) {
+    x = clampi(x, 0, p->w - 1);
+    y = clampi(y, 0, p->h - 1);
+    const uint8_t *row = p->data + (ptrdiff_t)y * p->stride;
+    float raw;
+    if (p->type == 0)
+        raw = (float)row[x];
+    else if (p->type == 1)
+        raw = (float)((const uint16_t *)row)[x];
+    else
+        raw = ((const float *)row)[x];
+    return raw * p->scale + p->offset;
+}
+
+/* buildBinomialWeights (downsample.cpp:52-84) and buildDownsampleWeights
+ * (downsample.cpp:86-92): spline36 / lanczos3 go through buildWeights at
+ * half size (support 3, widened to 6 by the 2x decimation) */
+static int ds_weights(int src_n, int kernel, float shift, or_weights *t) {
+    if (kernel != 2)
+        return or_build_weights(src_n, src_n / 2, kernel == 0 ? 4 : 2, 3.0, 0.0, 3.0, (double)shift, t);
+    const int n = src_n / 2;
+    const int integer_phase = fabsf(fabsf(shift) - 0.5f) < 1e-6f;
+    t->n = n;
+    t->sup = integer_phase ? 5 : 4;
+    t->start = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    t->w = (float *)malloc(sizeof(float) * (size_t)(n > 0 ? n : 1) * (size_t)t->sup);
+    if (!t->start || !t->w)
+        return -1;
+    static const float iw[5] = {1.0f / 16.0f, 4.0f / 16.0f, 6.0f / 16.0f, 4.0f / 16.0f, 1.0f / 16.0f};
+    static const float hw[4] = {1.0f / 8.0f, 3.0f / 8.0f, 3.0f / 8.0f, 1.0f / 8.0f};
+    for (int i = 0; i < n; ++i) {
+        const float center = 2.0f * (float)i + 0.5f + shift;
+        if (integer_phase) {
+            t->start[i] = (int)lroundf(center) - 2;
+            memcpy(t->w + (size_t)i * 5, iw, sizeof(iw));
+        } else {
+            t->start[i] = (int)floorf(center) - 1;
+            memcpy(t->w + (size_t)i * 4, hw, sizeof(hw));
+        }
+    }
+    return 0;
+}
+
+/* BaselineRing (downsample.cpp:129-261), scalar branch: horizontal float
+ * sums of clamped samples per source row, then vertical float sums over the
+ * clamped rows of each output row */
+static int ds_baseline(const ds_plane *u, const ds_plane *v, const or_weights *xw, const or_weights *yw,
+                       float *bu, float *bv) {
+    const int cw = xw->n, ch = yw->n, h = u->h;
+    float *ru = (float *)malloc(sizeof(float) * (size_t)cw * (size_t)h);
+    float *rv = (float *)malloc(sizeof(float) * (size_t)cw * (size_t)h);
+    if (!ru || !rv) {
+        free(ru);
+        free(rv);
+        return -1;
+    }
+    for (int sy = 0; sy < h; ++sy)
+        for (int x = 0; x < cw; ++x) {
+            const float *wt = xw->w + (size_t)x * xw->sup;
+            float su = 0.0f, sv = 0.0f;
+            for (int t = 0; t < xw->sup; ++t) {
+                const int sx = xw->start[x] + t;
+                su += wt[t] * ds_at(u, sx, sy);
+                sv += wt[t] * ds_at(v, sx, sy);
+            }
+            ru[(size_t)sy * cw + x] = su;
+            rv[(size_t)sy * cw + x] = sv;
+        }
+    for (int oy = 0; oy < ch; ++oy) {
+        float *ou = bu + (size_t)oy * cw, *ov = bv + (size_t)oy * cw;
+        for (int x = 0; x < cw; ++x) {
+            ou[x] = 0.0f;
+            ov[x] = 0.0f;
+        }
+        const float *wy = yw->w + (size_t)oy * yw->sup;
+        for (int t = 0; t < yw->sup; ++t) {
+            const int sy = clampi(yw->start[oy] + t, 0, h - 1);
+            const float w = wy[t];
+            for (int x = 0; x < cw; ++x) {
+                ou[x] += w * ru[(size_t)sy * cw + x];
+                ov[x] += w * rv[(size_t)sy * cw + x];
+            }
+        }
+    }
+    free(ru);
+    free(rv);
+    return 0;
+}
+
+typedef struct ds_dir { float x, y; } ds_dir;
+static const ds_dir ds_dirs[8] = {
+    {1.0f, 0.0f}, {0.9238795f, 0.3826834f}, {0.7071068f, 0.7071068f}, {0.3826834f, 0.9238795f},
+    {0.0f, 1.0f}, {-0.3826834f, 0.9238795f}, {-0.7071068f, 0.7071068f}, {-0.9238795f, 0.3826834f},
+};
+
+typedef struct ds_stats { int primary, secondary; float mix, gate; int isotropic; } ds_stats;
+
+static float ds_clampf(float v, float lo, float hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+/* classify (downsample.cpp:281-385) */
+static ds_stats ds_classify(const ds_plane *yp, const ds_plane *up, const ds_plane *vp, float cx, float cy,
+                            int quality) {
+    ds_stats st = {0, 1, 0.0f, 0.0f, 1};
+    const int ix = (int)floorf(cx + 0.5f), iy = (int)floorf(cy + 0.5f);
+    float jxx = 0.0f, jxy = 0.0f, jyy = 0.0f, cgx = 0.0f, cgy = 0.0f;
+    if (quality == 0) {
+        const float tl = ds_at(yp, ix - 1, iy - 1), tc = ds_at(yp, ix, iy - 1), tr = ds_at(yp, ix + 1, iy - 1);
+        const float ml = ds_at(yp, ix - 1, iy), mr = ds_at(yp, ix + 1, iy);
+        const float bl = ds_at(yp, ix - 1, iy + 1), bc = ds_at(yp, ix, iy + 1), br = ds_at(yp, ix + 1, iy + 1);
+        cgx = (tr + 2.0f * mr + br - tl - 2.0f * ml - bl) * 0.125f;
+        cgy = (bl + 2.0f * bc + br - tl - 2.0f * tc - tr) * 0.125f;
+        jxx = cgx * cgx;
+        jxy = cgx * cgy;
+        jyy = cgy * cgy;
+    } else {
+        for (int k = -1; k <= 1; ++k) {
+            const float gx = 0.5f * (ds_at(yp, ix + 1, iy + k) - ds_at(yp, ix - 1, iy + k));
+            const float gy = 0.5f * (ds_at(yp, ix, iy + k + 1) - ds_at(yp, ix, iy + k - 1));
+            jxx += gx * gx;
+            jxy += gx * gy;
+            jyy += gy * gy;
+            if (k == 0) {
+                cgx = gx;
+                cgy = gy;
+            }
+        }
+    }
+    const float trace = jxx + jyy;
+    if (trace < 1.0e-6f)
+        return st;
+    const float diff = jxx - jyy;
+    const float disc2 = diff * diff + 4.0f * jxy * jxy;
+    const float coh2 = disc2 / (trace * trace + 1e-12f);
+    float best = -1.0f, second = -1.0f;
+    const int step = quality == 0 ? 2 : 1;
+    for (int d = 0; d < 8; d += step) {
+        const ds_dir n = ds_dirs[d];
+        const float resp = n.x * n.x * jxx + 2.0f * n.x * n.y * jxy + n.y * n.y * jyy;
+        if (resp > best) {
+            second = best;
+            st.secondary = st.primary;
+            best = resp;
+            st.primary = d;
+        } else if (resp > second) {
+            second = resp;
+            st.secondary = d;
+        }
+    }
+    st.isotropic = coh2 < (quality == 0 ? 0.20f : 0.10f);
+    if (quality > 0 && best > 1e-12f) {
+        const float adj = 0.8535534f;
+        const float ratio = ds_clampf(second / best, 0.0f, 1.0f);
+        st.mix = 0.5f * ds_clampf((ratio - adj) / (1.0f - adj), 0.0f, 1.0f);
+    }
+    const float ugx = 0.5f * (ds_at(up, ix + 1, iy) - ds_at(up, ix - 1, iy));
+    const float ugy = 0.5f * (ds_at(up, ix, iy + 1) - ds_at(up, ix, iy - 1));
+    const float vgx = 0.5f * (ds_at(vp, ix + 1, iy) - ds_at(vp, ix - 1, iy));
+    const float vgy = 0.5f * (ds_at(vp, ix, iy + 1) - ds_at(vp, ix, iy - 1));
+    const float cen = ugx * ugx + ugy * ugy + vgx * vgx + vgy * vgy;
+    const ds_dir nrm = ds_dirs[st.primary];
+    const float un = ugx * nrm.x + ugy * nrm.y, vn = vgx * nrm.x + vgy * nrm.y;
+    const float aligned = un * un + vn * vn;
+    const int nsx = (int)lroundf(nrm.x), nsy = (int)lroundf(nrm.y);
+    const float cu = ds_at(up, ix, iy), cv = ds_at(vp, ix, iy);
+    const float curu = ds_at(up, ix + nsx, iy + nsy) - 2.0f * cu + ds_at(up, ix - nsx, iy - nsy);
+    const float curv = ds_at(vp, ix + nsx, iy + nsy) - 2.0f * cv + ds_at(vp, ix - nsx, iy - nsy);
+    const float curE = curu * curu + curv * curv;
+    const float sharp = curE / (curE + cen + 1.0e-12f);
+    const float luma_g = trace / (trace + 3.0e-5f);
+    const float conf_g = coh2 / (coh2 + 0.08f);
+    const float chroma_g = cen / (cen + 2.0e-5f);
+    const float dir_g = aligned / (cen + 1.0e-12f);
+    const float centerE = cgx * cgx + cgy * cgy;
+    const float pos_g = centerE / (centerE + 1.0e-5f);
+    st.gate = ds_clampf(luma_g * conf_g * chroma_g * dir_g * pos_g * sharp, 0.0f, 1.0f);
+    if (quality == 0)
+        st.gate *= 0.985f;
+    else if (quality == 2)
+        st.gate = fminf(1.0f, st.gate * 1.01f);
+    if (sharp < (quality == 0 ? 0.20f : 0.12f))
+        st.gate = 0.0f;
+    return st;
+}
+
+typedef struct ds_env { ds_dir normal; int radius; float aniso; int isotropic; } ds_env;
+typedef struct ds_acc { float w, u, v; } ds_acc;
+
+/* spatialWeight (downsample.cpp:402-411) */
+static float ds_spatial(const ds_env *e, float dx, float dy) {
+    const float r2 = dx * dx + dy * dy;
+    if (e->isotropic)
+        return 1.0f / (1.0f + 0.42f * r2);
+    const float nd = dx * e->normal.x + dy * e->normal.y;
+    const float td = -dx * e->normal.y + dy * e->normal.x;
+    return 1.0f / (1.0f + (0.9f + e->aniso) * nd * nd + 0.16f * td * td);
+}
+
+/* accumulateGuided (downsample.cpp:413-458) */
+static void ds_accumulate(const ds_plane *yp, const ds_plane *up, const ds_plane *vp, float cx, float cy,
+                          const ds_env *env, int n, ds_acc *acc) {
+    int maxr = 0;
+    for (int i = 0; i < n; ++i) {
+        acc[i].w = acc[i].u = acc[i].v = 0.0f;
+        if (env[i].radius > maxr)
+            maxr = env[i].radius;
+    }
+    const int x0 = (int)ceilf(cx - (float)maxr), x1 = (int)floorf(cx + (float)maxr);
+    const int y0 = (int)ceilf(cy - (float)maxr), y1 = (int)floorf(cy + (float)maxr);
+    const int cx0 = (int)floorf(cx), cy0 = (int)floorf(cy);
+    const float fx = cx - (float)cx0, fy = cy - (float)cy0;
+    const float ya = ds_at(yp, cx0, cy0), yb = ds_at(yp, cx0 + 1, cy0);
+    const float yc = ds_at(yp, cx0, cy0 + 1), yd = ds_at(yp, cx0 + 1, cy0 + 1);
+    const float cl = ya + (yb - ya) * fx + (yc - ya) * fy + (ya - yb - yc + yd) * fx * fy;
+    const float rs2 = 2.25e-4f;
+    for (int sy = y0; sy <= y1; ++sy)
+        for (int sx = x0; sx <= x1; ++sx) {
+            const float dx = (float)sx - cx, dy = (float)sy - cy;
+            const float delta = ds_at(yp, sx, sy) - cl;
+            const float rw = rs2 / (rs2 + delta * delta);
+            const float su = ds_at(up, sx, sy), sv = ds_at(vp, sx, sy);
+            for (int i = 0; i < n; ++i) {
+                if (fabsf(dx) > (float)env[i].radius || fabsf(dy) > (float)env[i].radius)
+                    continue;
+                const float w = rw * ds_spatial(&env[i], dx, dy);
+                acc[i].w += w;
+                acc[i].u += w * su;
+                acc[i].v += w * sv;
+            }
+        }
+}
+
+static float ds_norm(float sum, float w, float fallback) { return w > 1e-12f ? sum / w : fallback; }
+
+/* writeRow (downsample.cpp:117-126): integers round with the chroma offset */
+static void ds_store(const or_frame_format *f, void *out, ptrdiff_t stride, int x, int y, float v) {
+    uint8_t *row = (uint8_t *)out + (ptrdiff_t)y * stride;
+    if (f->sample_type == 1) {
+        ((float *)row)[x] = v;
+        return;
+    }
+    const int maxv = (1 << f->bits) - 1;
+    const float scale = (float)maxv, offset = 0.5f * scale;
+    const int q = clampi((int)(v * scale + offset + 0.5f), 0, maxv);
+    if (f->bytes_per_sample == 1)
+        row[x] = (uint8_t)q;
+    else
+        ((uint16_t *)row)[x] = (uint16_t)q;
+}
+
+static void ds_planes(const or_frame_format *f, const void *const in[3], const ptrdiff_t in_stride[3],
+                      ds_plane p[3]) {
+    const int is_float = f->sample_type == 1;
+    const int maxv = is_float ? 0 : (1 << f->bits) - 1;
+    const float scale = is_float ? 1.0f : 1.0f / (float)maxv;
+    const int type = is_float ? 2 : (f->bytes_per_sample == 1 ? 0 : 1);
+    for (int i = 0; i < 3; ++i) {
+        p[i].data = (const uint8_t *)in[i];
+        p[i].stride = in_stride[i];
+        p[i].w = f->width;
+        p[i].h = f->height;
+        p[i].type = type;
+        p[i].scale = scale;
+        p[i].offset = (i > 0 && !is_float) ? -0.5f : 0.0f;
+    }
+}
+
+/* generateCandidateRow (downsample.cpp:526-568) for every output row:
+ * gate and four candidates (planes c * nc + i) per chroma sample */
+static void ds_candidates(const or_downsample_params *d, const ds_plane *p, int cw, int ch, const float *bu,
+                          const float *bv, float *gate, float *cu, float *cv) {
+    const size_t nc = (size_t)cw * (size_t)ch;
+    for (int y = 0; y < ch; ++y) {
+        const float cy = 2.0f * (float)y + 0.5f + d->shift_y;
+        for (int x = 0; x < cw; ++x) {
+            const size_t i = (size_t)y * cw + x;
+            const float cx = 2.0f * (float)x + 0.5f + d->shift_x;
+            const ds_stats st = ds_classify(&p[0], &p[1], &p[2], cx, cy, 2);
+            gate[i] = st.gate > 1.0e-4f ? st.gate : 0.0f;
+            if (gate[i] == 0.0f) {
+                for (int c = 0; c < 4; ++c) {
+                    cu[(size_t)c * nc + i] = bu[i];
+                    cv[(size_t)c * nc + i] = bv[i];
+                }
+                continue;
+            }
+            const ds_env env[5] = {{ds_dirs[st.primary], 3, 2.2f, st.isotropic},
+                                   {ds_dirs[st.secondary], 3, 2.2f, st.isotropic},
+                                   {ds_dirs[st.primary], 4, 2.7f, st.isotropic},
+                                   {ds_dirs[st.secondary], 4, 2.7f, st.isotropic},
+                                   {ds_dirs[st.primary], 4, 0.0f, 1}};
+            ds_acc a[5];
+            ds_accumulate(&p[0], &p[1], &p[2], cx, cy, env, 5, a);
+            const float u0 = ds_norm(a[0].u, a[0].w, bu[i]), v0 = ds_norm(a[0].v, a[0].w, bv[i]);
+            const float u1 = ds_norm(a[1].u, a[1].w, bu[i]), v1 = ds_norm(a[1].v, a[1].w, bv[i]);
+            cu[i] = u0 + st.mix * (u1 - u0);
+            cv[i] = v0 + st.mix * (v1 - v0);
+            for (int c = 1; c < 4; ++c) {
+                cu[(size_t)c * nc + i] = ds_norm(a[c + 1].u, a[c + 1].w, bu[i]);
+                cv[(size_t)c * nc + i] = ds_norm(a[c + 1].v, a[c + 1].w, bv[i]);
+            }
+        }
+    }
+}
+
+/* test access: the quality-2 gates and candidates of the next frame */
+static struct { float *gate, *cu, *cv; } g_ds_debug;
+void or_downsample_debug(float *gate, float *cu, float *cv) {
+    g_ds_debug.gate = gate;
+    g_ds_debug.cu = cu;
+    g_ds_debug.cv = cv;
+}
+
+int or_downsample_base(const or_downsample_params *d, const or_frame_format *f, const void *const in[3],
+                       const ptrdiff_t in_stride[3], float *base_u, float *base_v) {
+    ds_plane p[3];
+    ds_planes(f, in, in_stride, p);
+    or_weights xw = {0}, yw = {0};
+    int rc = ds_weights(f->width, d->kernel, d->shift_x, &xw);
+    if (!rc)
+        rc = ds_weights(f->height, d->kernel, d->shift_y, &yw);
+    if (!rc)
+        rc = ds_baseline(&p[1], &p[2], &xw, &yw, base_u, base_v);
+    or_free_weights(&xw);
+    or_free_weights(&yw);
+    return rc;
+}
+
+/* processFrame (downsample.cpp:628-722) with makeGuidedPixel
+ * (downsample.cpp:466-507), generateCandidateRow (downsample.cpp:526-568)
+ * and scoreCandidateRow (downsample.cpp:576-626); the three-row candidate
+ * ring is whole planes here (rows outside the frame clamp, as the ring's
+ * lookup does) */
+int or_downsample_frame(const or_downsample_params *d, const or_frame_format *f, const void *const in[3],
+                        const ptrdiff_t in_stride[3], void *const out[3], const ptrdiff_t out_stride[3]) {
+    const int w = f->width, h = f->height, cw = w / 2, ch = h / 2;
+    const size_t nc = (size_t)cw * (size_t)ch;
+    ds_plane p[3];
+    ds_planes(f, in, in_stride, p);
+    float *bu = (float *)malloc(sizeof(float) * (nc ? nc : 1));
+    float *bv = (float *)malloc(sizeof(float) * (nc ? nc : 1));
+    if (!bu || !bv || or_downsample_base(d, f, in, in_stride, bu, bv)) {
+        free(bu);
+        free(bv);
+        return -1;
+    }
+    const float s = d->strength;
+    if (s == 0.0f || d->quality < 2) {
+        for (int y = 0; y < ch; ++y) {
+            const float cy = 2.0f * (float)y + 0.5f + d->shift_y;
+            for (int x = 0; x < cw; ++x) {
+                const size_t i = (size_t)y * cw + x;
+                float ru = bu[i], rv = bv[i];
+                if (s != 0.0f) {
+                    const float cx = 2.0f * (float)x + 0.5f + d->shift_x;
+                    const ds_stats st = ds_classify(&p[0], &p[1], &p[2], cx, cy, d->quality);
+                    float gu = bu[i], gv = bv[i], gate = st.gate;
+                    if (gate <= 1.0e-4f) {
+                        gate = 0.0f;
+                    } else if (d->quality == 0) {
+                        const ds_env env = {ds_dirs[st.primary], 2, 1.8f, st.isotropic};
+                        ds_acc a;
+                        ds_accumulate(&p[0], &p[1], &p[2], cx, cy, &env, 1, &a);
+                        gu = ds_norm(a.u, a.w, bu[i]);
+                        gv = ds_norm(a.v, a.w, bv[i]);
+                    } else {
+                        const ds_env env[2] = {{ds_dirs[st.primary], 3, 2.2f, st.isotropic},
+                                               {ds_dirs[st.secondary], 3, 2.2f, st.isotropic}};
+                        ds_acc a[2];
+                        ds_accumulate(&p[0], &p[1], &p[2], cx, cy, env, 2, a);
+                        const float u0 = ds_norm(a[0].u, a[0].w, bu[i]), v0 = ds_norm(a[0].v, a[0].w, bv[i]);
+                        const float u1 = ds_norm(a[1].u, a[1].w, bu[i]), v1 = ds_norm(a[1].v, a[1].w, bv[i]);
+                        gu = u0 + st.mix * (u1 - u0);
+                        gv = v0 + st.mix * (v1 - v0);
+                    }
+                    ru = bu[i] + s * gate * (gu - bu[i]);
+                    rv = bv[i] + s * gate * (gv - bv[i]);
+                }
+                ds_store(f, out[1], out_stride[1], x, y, ru);
+                ds_store(f, out[2], out_stride[2], x, y, rv);
+            }
+        }
+        free(bu);
+        free(bv);
+        return 0;
+    }
+    /* quality 2: gate and four candidates per chroma sample, then the
+     * loopback score over the sample's 2x2 source block */
+    float *gate = (float *)malloc(sizeof(float) * nc);
+    float *cu = (float *)malloc(sizeof(float) * nc * 4), *cv = (float *)malloc(sizeof(float) * nc * 4);
+    if (!gate || !cu || !cv) {
+        free(bu); free(bv); free(gate); free(cu); free(cv);
+        return -1;
+    }
+    ds_candidates(d, p, cw, ch, bu, bv, gate, cu, cv);
+    if (g_ds_debug.gate) {
+        memcpy(g_ds_debug.gate, gate, sizeof(float) * nc);
+        memcpy(g_ds_debug.cu, cu, sizeof(float) * nc * 4);
+        memcpy(g_ds_debug.cv, cv, sizeof(float) * nc * 4);
+    }
+    for (int y = 0; y < ch; ++y)
+        for (int x = 0; x < cw; ++x) {
+            const size_t i = (size_t)y * cw + x;
+            float ru = bu[i], rv = bv[i];
+            if (gate[i] != 0.0f) {
+                float score[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+                for (int hy = 2 * y; hy <= 2 * y + 1; ++hy)
+                    for (int hx = 2 * x; hx <= 2 * x + 1; ++hx) {
+                        const float gx = 0.5f * (ds_at(&p[0], hx + 1, hy) - ds_at(&p[0], hx - 1, hy));
+                        const float gy = 0.5f * (ds_at(&p[0], hx, hy + 1) - ds_at(&p[0], hx, hy - 1));
+                        const float e2 = gx * gx + gy * gy;
+                        const float ew = 0.125f + e2 / (e2 + 1.0e-5f);
+                        const float lx = ((float)hx - 0.5f - d->shift_x) * 0.5f;
+                        const float ly = ((float)hy - 0.5f - d->shift_y) * 0.5f;
+                        const float su = ds_at(&p[1], hx, hy), sv = ds_at(&p[2], hx, hy);
+                        const int rx0 = (int)floorf(lx), ry0 = (int)floorf(ly);
+                        const int x0 = clampi(rx0, 0, cw - 1), x1 = clampi(rx0 + 1, 0, cw - 1);
+                        const int r0 = clampi(ry0, 0, ch - 1), r1 = clampi(ry0 + 1, 0, ch - 1);
+                        const float fx = lx - (float)rx0, fy = ly - (float)ry0;
+                        const float wts[4] = {(1.0f - fx) * (1.0f - fy), fx * (1.0f - fy), (1.0f - fx) * fy, fx * fy};
+                        const size_t idx[4] = {(size_t)r0 * cw + x0, (size_t)r0 * cw + x1, (size_t)r1 * cw + x0,
+                                               (size_t)r1 * cw + x1};
+                        for (int c = 0; c < 4; ++c) {
+                            float recu = 0.0f, recv = 0.0f;
+                            for (int k = 0; k < 4; ++k) {
+                                const size_t j = idx[k];
+                                float vu = bu[j], vv = bv[j];
+                                vu += gate[j] * (cu[(size_t)c * nc + j] - vu);
+                                vv += gate[j] * (cv[(size_t)c * nc + j] - vv);
+                                recu += wts[k] * vu;
+                                recv += wts[k] * vv;
+                            }
+                            const float du = recu - su, dv = recv - sv;
+                            score[c] += ew * (du * du + dv * dv);
+                        }
+                    }
+                int best = 0;
+                for (int c = 1; c < 4; ++c)
+                    if (score[c] < score[best])
+                        best = c;
+                ru = bu[i] + s * gate[i] * (cu[(size_t)best * nc + i] - bu[i]);
+                rv = bv[i] + s * gate[i] * (cv[(size_t)best * nc + i] - bv[i]);
+            }
+            ds_store(f, out[1], out_stride[1], x, y, ru);
+            ds_store(f, out[2], out_stride[2], x, y, rv);
+        }
+    free(bu); free(bv); free(gate); free(cu); free(cv);
+    return 0;
+}

@@ -202,6 +202,20 @@ void or_chroma_consistency(const float *cu, const float *cv, const float *nu, co
                            int cw, int ch, int luma_w, int luma_h, int block, const int16_t *mvx,
                            const int16_t *mvy, int bw, int bh, float *conf);
 
+/* Downsample 4:4:4 -> 4:2:0 (downsample.cpp).  kernel: 0 spline36,
+ * 1 lanczos3, 2 binomial; shifts are the output siting in source units.
+ * out[1], out[2] receive U and V at (width/2, height/2); out[0] is unused
+ * (the plugin shares the source luma). */
+typedef struct or_downsample_params { int quality, kernel; float strength, shift_x, shift_y; } or_downsample_params;
+int or_downsample_frame(const or_downsample_params *d, const or_frame_format *f, const void *const in[3],
+                        const ptrdiff_t in_stride[3], void *const out[3], const ptrdiff_t out_stride[3]);
+/* test access: the next quality-2 frame also copies its gates (nc) and
+ * candidates (4 * nc, candidate-major) here; NULL turns it off */
+void or_downsample_debug(float *gate, float *cand_u, float *cand_v);
+/* the plain separable baseline alone (float U, V at the output size) */
+int or_downsample_base(const or_downsample_params *d, const or_frame_format *f, const void *const in[3],
+                       const ptrdiff_t in_stride[3], float *base_u, float *base_v);
+
 /* intermediates of the last or_recon_frame on this thread */
 typedef struct or_debug {
     int sw, sh, cw, ch, ow, oh;
