@@ -219,6 +219,31 @@ lgcr_status lgcr_sharpen_create(lgcr_device *dev, const lgcr_sharpen_params *p, 
                                 int max_inflight, lgcr_clip **out);
 lgcr_status lgcr_sharpen_frame(lgcr_clip *clip, const lgcr_frame *in, lgcr_frame *out);
 
+/* Downsample (downsampleCreate / downsampleGetFrame, src/downsample.cpp:
+ * 725-906): the fields of DownsampleData (src/lgcr.h:611-623).  Input is
+ * planar YUV 4:4:4 (8-16 bit integer or 32-bit float) with even width and
+ * height; the output is 4:2:0 of the same sample type.  out->data[0] may
+ * be NULL: the plugin shares the source luma plane (downsample.cpp:737-742);
+ * otherwise Y is copied.  U and V of `out` are (width/2) x (height/2).
+ * Destroy with lgcr_clip_destroy. */
+enum {
+    LGCR_DS_KERNEL_SPLINE36 = 0, LGCR_DS_KERNEL_LANCZOS3 = 1, LGCR_DS_KERNEL_BINOMIAL = 2
+};
+typedef struct lgcr_downsample_params {
+    int quality;              /* 0 fast, 1 balanced, 2 high */
+    int kernel;               /* LGCR_DS_KERNEL_* */
+    double strength;          /* guided blend, 0..1 (float in the filter) */
+    int chroma_location;      /* H.273 _ChromaLocation written on the output:
+                                 0 left, 1 center, 2 topleft, 3 top,
+                                 4 bottomleft, 5 bottom (downsample.cpp:875-893) */
+} lgcr_downsample_params;
+void lgcr_downsample_params_default(lgcr_downsample_params *p);
+lgcr_status lgcr_downsample_create(lgcr_device *dev, const lgcr_downsample_params *p, const lgcr_format *fmt,
+                                   int max_inflight, lgcr_clip **out);
+/* stats (may be NULL): DownsampleBase / Guide / CandidateScore / Output
+ * device times, as writeDownsampleProfile reports them (downsample.cpp:31-49) */
+lgcr_status lgcr_downsample_frame(lgcr_clip *clip, const lgcr_frame *in, lgcr_frame *out, lgcr_stats *stats);
+
 /* Per-stage device time accumulated while timing is on: the nanosecond
  * counters of PipelineMetrics (src/pipeline.h:315-336), here measured with
  * HIP events recorded around each stage on the workspace stream that runs

@@ -1,0 +1,130 @@
+"""GPU parity for Downsample (downsample.cpp) against the oracle, through the
+C ABI and the plugin surface."""
+import numpy as np
+import pytest
+
+import oracle
+from lgcr_amd import abi, plugin, synth
+
+pytestmark = pytest.mark.gpu
+
+DT = {8: np.uint8, 10: np.uint16, 16: np.uint16, 32: np.float32}
+
+
+def frame444(w, h, bits, seed=3):
+    return synth.clip_frame(w, h, ss_w=0, ss_h=0, bits=bits, seed=seed, polygons=14)
+
+
+def run_gpu(device, planes, bits, **kw):
+    y = planes[0]
+    fmt = abi.make_format(y.dtype, y.shape[1], y.shape[0], 0, 0, bits if bits == 10 else None)
+    clip = abi.DownsampleClip(device, abi.downsample_params(**kw), fmt)
+    try:
+        return clip.downsample_host(*planes)
+    finally:
+        clip.close()
+
+
+def compare(tag, ref, got):
+    worst = 0
+    for i, name in enumerate("YUV"):
+        a, b = np.asarray(ref[i]), np.asarray(got[i])
+        assert a.shape == b.shape, (tag, name, a.shape, b.shape)
+        n = int(np.count_nonzero(a != b))
+        mx = float(np.max(np.abs(a.astype(np.float64) - b.astype(np.float64)))) if a.size else 0.0
+        print(f"{tag} {name}: mismatches={n} max_abs={mx:g}")
+        worst = max(worst, n)
+    assert worst == 0
+
+
+CASES = []
+for bits in (8, 10, 16, 32):
+    for kernel in ("spline36", "lanczos3", "binomial"):
+        for quality in (0, 1, 2):
+            CASES.append((bits, kernel, quality, "left", 1.0, 96, 64))
+CASES += [
+    (8, "spline36", 0, "center", 1.0, 96, 64), (8, "spline36", 1, "topleft", 1.0, 96, 64),
+    (10, "lanczos3", 2, "top", 1.0, 96, 64), (32, "binomial", 2, "bottomleft", 1.0, 96, 64),
+    (32, "spline36", 1, "bottom", 0.5, 96, 64), (16, "binomial", 0, "center", 0.0, 96, 64),
+    (8, "spline36", 2, "left", 0.3, 98, 62), (32, "lanczos3", 1, "topleft", 1.0, 100, 58),
+    (8, "binomial", 2, "bottom", 1.0, 2, 2), (10, "spline36", 0, "left", 1.0, 6, 4),
+    (32, "spline36", 2, "left", 1.0, 4, 6),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=["-".join(map(str, c)) for c in CASES])
+def test_downsample_matrix(device, case):
+    bits, kernel, quality, loc, strength, w, h = case
+    planes = frame444(w, h, bits)
+    ref = oracle.downsample_frame(oracle.downsample_params(quality=quality, kernel=kernel, strength=strength,
+                                                           loc=loc), *planes, bits=None if bits != 10 else 10)
+    got = run_gpu(device, planes, bits, quality=quality, kernel=kernel, strength=strength, loc=loc)
+    compare("-".join(map(str, case)), ref, got)
+
+
+@pytest.mark.parametrize("quality", [0, 2])
+def test_downsample_1080p(device, quality):
+    """A 1920x1080 10-bit 4:4:4 frame."""
+    planes = frame444(1920, 1080, 10, seed=8)
+    ref = oracle.downsample_frame(oracle.downsample_params(quality=quality), *planes, bits=10)
+    got = run_gpu(device, planes, 10, quality=quality)
+    compare(f"1080p-q{quality}", ref, got)
+
+
+def test_downsample_device_frames(device):
+    """Device-resident planes: out Y NULL (shared) and U, V at half size."""
+    import torch
+    planes = frame444(128, 72, 16, seed=5)
+    ref = oracle.downsample_frame(oracle.downsample_params(quality=1, kernel="lanczos3"), *planes)
+    din = [torch.from_numpy(p.view(np.int16)).to("cuda") for p in planes]
+    dout = [torch.empty((36, 64), dtype=torch.int16, device="cuda") for _ in range(2)]
+    clip = abi.DownsampleClip(device, abi.downsample_params(quality=1, kernel="lanczos3"),
+                              abi.make_format(np.uint16, 128, 72, 0, 0))
+    try:
+        clip.downsample_device([(t.data_ptr(), t.stride(0) * 2) for t in din],
+                               [(0, 0)] + [(t.data_ptr(), t.stride(0) * 2) for t in dout])
+        torch.cuda.synchronize()
+    finally:
+        clip.close()
+    for i in (1, 2):
+        got = dout[i - 1].cpu().numpy().view(np.uint16)
+        assert np.array_equal(got, ref[i])
+
+
+def test_downsample_plugin_surface(device, monkeypatch):
+    """core.lgcr.Downsample: 4:2:0 output of the same sample type, Y shared
+    with the source (test/test_downsample.py:60-81), _ChromaLocation from
+    loc, source props kept, _LGCR_* profile props under LGCR_PROFILE."""
+    monkeypatch.setenv("LGCR_PROFILE", "1")
+    fmt = plugin.query_video_format(plugin.YUV, plugin.FLOAT, 32, 0, 0)
+    planes = frame444(32, 24, 32)
+    clip = plugin.clip_from_planes([planes], fmt, props={"LGCRDownsampleTest": 73})
+    for loc, value in abi.DS_LOCATIONS.items():
+        f = plugin.core.lgcr.Downsample(clip, loc=loc, quality=2).get_frame(0)
+        assert f.format.subsampling_w == 1 and f.format.subsampling_h == 1
+        assert f.format.sample_type == plugin.FLOAT and f.format.bits_per_sample == 32
+        assert f.props["_ChromaLocation"] == value and f.props["LGCRDownsampleTest"] == 73
+        assert f[0] is planes[0] or np.shares_memory(f[0], planes[0])
+        ref = oracle.downsample_frame(oracle.downsample_params(quality=2, loc=loc), *planes)
+        assert np.array_equal(f[1], ref[1]) and np.array_equal(f[2], ref[2])
+        assert f.props["_LGCR_downsample_base_pixels"] == 2 * 16 * 12
+        assert f.props["_LGCR_downsample_candidate_score_taps"] == 16 * 12 * 16
+        assert f.props["_LGCR_total_us"] > 0.0
+
+
+def test_downsample_constant_formats(device):
+    """test/test_downsample.py:84-113 on the device: constants stay
+    constant for every format, kernel and quality."""
+    for bits in (8, 10, 16, 32):
+        if bits == 32:
+            color = [0.375, 0.125, -0.20]
+        else:
+            m = (1 << bits) - 1
+            color = [round(0.375 * m), round(0.625 * m), round(0.30 * m)]
+        planes = [np.full((26, 34), c, DT[bits]) for c in color]
+        for kernel in ("spline36", "lanczos3", "binomial"):
+            for quality in (0, 1, 2):
+                y, u, v = run_gpu(device, planes, bits, kernel=kernel, quality=quality)
+                assert np.array_equal(y, planes[0])
+                for p, c in ((u, color[1]), (v, color[2])):
+                    assert float(np.max(np.abs(p.astype(np.float64) - c))) <= (2e-7 if bits == 32 else 0)

@@ -98,3 +98,32 @@ def test_siting_rules():
     assert s({"_ChromaLocation": 2}, fmt444, -0.5, 0.0, False) == (0.0, 0.0)
     assert s({}, fmt420, -0.5, 0.0, False) == (-0.5, 0.0)
     assert plugin.temporal_frame_numbers(0, 4, 2) == [1, 2]
+
+
+def test_downsample_argument_errors():
+    """downsampleCreate's checks and messages (downsample.cpp:816-893;
+    test/test_downsample.py:341-362), raised before any device work."""
+    f444 = plugin.query_video_format(plugin.YUV, plugin.FLOAT, 32, 0, 0)
+    f420 = plugin.query_video_format(plugin.YUV, plugin.FLOAT, 32, 1, 1)
+    rgb = plugin.query_video_format("RGB", plugin.FLOAT, 32, 0, 0)
+    p444 = [np.zeros((16, 16), np.float32)] * 3
+    odd = plugin.clip_from_planes([[np.zeros((14, 17), np.float32)] * 3], f444)
+    sub = plugin.clip_from_planes([[np.zeros((16, 16), np.float32), np.zeros((8, 8), np.float32),
+                                    np.zeros((8, 8), np.float32)]], f420)
+    valid = plugin.clip_from_planes([p444], f444)
+    cases = [
+        (lambda: plugin.core.lgcr.Downsample(plugin.clip_from_planes([p444], rgb)), "input must be planar YUV444"),
+        (lambda: plugin.core.lgcr.Downsample(sub), "input must be planar YUV444"),
+        (lambda: plugin.core.lgcr.Downsample(odd), "constant even input width and height are required"),
+        (lambda: plugin.core.lgcr.Downsample(valid, quality=-1), "quality must be 0, 1, or 2"),
+        (lambda: plugin.core.lgcr.Downsample(valid, quality=3), "quality must be 0, 1, or 2"),
+        (lambda: plugin.core.lgcr.Downsample(valid, kernel="lanczos"), "kernel must be"),
+        (lambda: plugin.core.lgcr.Downsample(valid, strength=-0.01), "strength must be in the range 0..1"),
+        (lambda: plugin.core.lgcr.Downsample(valid, strength=1.01), "strength must be in the range 0..1"),
+        (lambda: plugin.core.lgcr.Downsample(valid, strength=float("nan")), "strength must be in the range 0..1"),
+        (lambda: plugin.core.lgcr.Downsample(valid, loc="middle"), "loc must be left, center"),
+        (lambda: plugin.core.lgcr.Downsample(valid, sigma=1.0), "argument sigma is not expected"),
+    ]
+    for call, msg in cases:
+        with pytest.raises(plugin.Error, match=msg):
+            call()

@@ -145,6 +145,26 @@ struct RawOut {
     int slot;
 };
 
+// Downsample 4:4:4 -> 4:2:0 (downsample.cpp).  Source planes are read raw
+// (type RAW_*) and converted per sample as SourcePlane::at does.
+struct DsPlane {
+    const void *data;
+    ptrdiff_t stride;  // bytes
+    float scale, offset;
+};
+struct DsArgs {
+    DsPlane y, u, v;
+    int type;          // RAW_U8, RAW_U16 or RAW_F32
+    int w, h, cw, ch;  // source (4:4:4) and output chroma sizes
+    WeightsDev xw, yw; // baseline taps (start may be negative: loads clamp)
+    float *tmp_u, *tmp_v;    // horizontal pass, cw x h
+    float *base_u, *base_v;  // baseline, cw x ch
+    float *gate, *cand_u, *cand_v;  // quality 2: gate (cw x ch), 4 candidate planes each
+    int quality;
+    float strength, shift_x, shift_y;
+    RawOut out_u, out_v;     // packed U, V (src unused)
+};
+
 // Kernel launchers (lgcr_kernels.hip).  All are asynchronous on `s`.
 hipError_t launch_to_float(const RawPlane *planes, int count, int type, hipStream_t s);
 hipError_t launch_from_float(const RawOut *planes, int count, int type, hipStream_t s);
@@ -181,6 +201,13 @@ hipError_t launch_plain_radial(const float *u, const float *v, int cw, int ch,
                                const AxisDev &ax, const AxisDev &ay, const RadialDev &rd,
                                float *out_u, float *out_v, int clamp_hull, float margin,
                                hipStream_t s);
+// Downsample stages: the separable baseline; then either the packed
+// baseline (strength 0), the guided pass of quality 0/1 with packing, or the
+// quality-2 candidates followed by the scored, packed result.
+hipError_t launch_ds_base(const DsArgs &a, hipStream_t s);
+hipError_t launch_ds_output(const DsArgs &a, hipStream_t s);
+hipError_t launch_ds_candidates(const DsArgs &a, hipStream_t s);
+hipError_t launch_ds_score(const DsArgs &a, hipStream_t s);
 hipError_t launch_guided(const float *u, const float *v, int cw, int ch, const float *y,
                          int sw, int sh, const float4 *yt, const float4 *luv,
                          const float *lc, const float *ms, const AxisDev &ax, const AxisDev &ay,

@@ -3058,6 +3058,355 @@ __global__ __launch_bounds__(kBlock) void k_sharpen(SharpenArgs A) {
 
 inline dim3 rows_grid(int w, int h, int z = 1) { return dim3((unsigned)((w + kBlock - 1) / kBlock), (unsigned)h, (unsigned)z); }
 
+// ---------------------------------------------------------------------------
+// Downsample 4:4:4 -> 4:2:0 (downsample.cpp).  Every value is the float
+// expression of the scalar build in its order; samples are read raw and
+// converted per read (SourcePlane::at, downsample.cpp:101-106).
+// ---------------------------------------------------------------------------
+
+template <typename T>
+__device__ __forceinline__ float ds_at(const DsPlane &p, int w, int h, int x, int y) {
+    x = clampi(x, 0, w - 1);
+    y = clampi(y, 0, h - 1);
+    const T *row = reinterpret_cast<const T *>(static_cast<const char *>(p.data) + p.stride * (ptrdiff_t)y);
+    return (float)row[x] * p.scale + p.offset;
+}
+
+// horizontal baseline pass (BaselineRing::ensureRow, downsample.cpp:200-229,
+// scalar branch): one thread per (output column, source row)
+template <typename T> __global__ __launch_bounds__(kBlock) void k_ds_hpass(DsArgs a) {
+    const int x = blockIdx.x * kBlock + threadIdx.x, sy = blockIdx.y;
+    if (x >= a.cw)
+        return;
+    const float *wt = a.xw.w + (size_t)x * a.xw.sup;
+    const int st = a.xw.start[x];
+    float su = 0.0f, sv = 0.0f;
+    for (int t = 0; t < a.xw.sup; ++t) {
+        su += wt[t] * ds_at<T>(a.u, a.w, a.h, st + t, sy);
+        sv += wt[t] * ds_at<T>(a.v, a.w, a.h, st + t, sy);
+    }
+    a.tmp_u[(size_t)sy * a.cw + x] = su;
+    a.tmp_v[(size_t)sy * a.cw + x] = sv;
+}
+
+// vertical baseline pass (BaselineRing::makeRow, downsample.cpp:139-170)
+__global__ __launch_bounds__(kBlock) void k_ds_vpass(DsArgs a) {
+    const int x = blockIdx.x * kBlock + threadIdx.x, oy = blockIdx.y;
+    if (x >= a.cw)
+        return;
+    const float *wy = a.yw.w + (size_t)oy * a.yw.sup;
+    float bu = 0.0f, bv = 0.0f;
+    for (int t = 0; t < a.yw.sup; ++t) {
+        const int sy = clampi(a.yw.start[oy] + t, 0, a.h - 1);
+        bu += wy[t] * a.tmp_u[(size_t)sy * a.cw + x];
+        bv += wy[t] * a.tmp_v[(size_t)sy * a.cw + x];
+    }
+    a.base_u[(size_t)oy * a.cw + x] = bu;
+    a.base_v[(size_t)oy * a.cw + x] = bv;
+}
+
+__device__ __forceinline__ float2 ds_dir(int d) {
+    switch (d) {
+    case 0: return make_float2(1.0f, 0.0f);
+    case 1: return make_float2(0.9238795f, 0.3826834f);
+    case 2: return make_float2(0.7071068f, 0.7071068f);
+    case 3: return make_float2(0.3826834f, 0.9238795f);
+    case 4: return make_float2(0.0f, 1.0f);
+    case 5: return make_float2(-0.3826834f, 0.9238795f);
+    case 6: return make_float2(-0.7071068f, 0.7071068f);
+    default: return make_float2(-0.9238795f, 0.3826834f);
+    }
+}
+
+struct DsStats {
+    int primary = 0, secondary = 1;
+    float mix = 0.0f, gate = 0.0f;
+    bool isotropic = true;
+};
+
+// classify (downsample.cpp:281-385)
+template <typename T>
+__device__ DsStats ds_classify(const DsArgs &a, float cx, float cy, int quality) {
+    DsStats st;
+    const int w = a.w, h = a.h;
+    const int ix = (int)floorf(cx + 0.5f), iy = (int)floorf(cy + 0.5f);
+    float jxx = 0.0f, jxy = 0.0f, jyy = 0.0f, cgx = 0.0f, cgy = 0.0f;
+    if (quality == 0) {
+        const float tl = ds_at<T>(a.y, w, h, ix - 1, iy - 1), tc = ds_at<T>(a.y, w, h, ix, iy - 1);
+        const float tr = ds_at<T>(a.y, w, h, ix + 1, iy - 1);
+        const float ml = ds_at<T>(a.y, w, h, ix - 1, iy), mr = ds_at<T>(a.y, w, h, ix + 1, iy);
+        const float bl = ds_at<T>(a.y, w, h, ix - 1, iy + 1), bc = ds_at<T>(a.y, w, h, ix, iy + 1);
+        const float br = ds_at<T>(a.y, w, h, ix + 1, iy + 1);
+        cgx = (tr + 2.0f * mr + br - tl - 2.0f * ml - bl) * 0.125f;
+        cgy = (bl + 2.0f * bc + br - tl - 2.0f * tc - tr) * 0.125f;
+        jxx = cgx * cgx;
+        jxy = cgx * cgy;
+        jyy = cgy * cgy;
+    } else {
+        for (int k = -1; k <= 1; ++k) {
+            const float gx = 0.5f * (ds_at<T>(a.y, w, h, ix + 1, iy + k) - ds_at<T>(a.y, w, h, ix - 1, iy + k));
+            const float gy = 0.5f * (ds_at<T>(a.y, w, h, ix, iy + k + 1) - ds_at<T>(a.y, w, h, ix, iy + k - 1));
+            jxx += gx * gx;
+            jxy += gx * gy;
+            jyy += gy * gy;
+            if (k == 0) {
+                cgx = gx;
+                cgy = gy;
+            }
+        }
+    }
+    const float trace = jxx + jyy;
+    if (trace < 1.0e-6f)
+        return st;
+    const float diff = jxx - jyy;
+    const float disc2 = diff * diff + 4.0f * jxy * jxy;
+    const float coh2 = disc2 / (trace * trace + 1e-12f);
+    float best = -1.0f, second = -1.0f;
+    const int step = quality == 0 ? 2 : 1;
+    for (int d = 0; d < 8; d += step) {
+        const float2 n = ds_dir(d);
+        const float resp = n.x * n.x * jxx + 2.0f * n.x * n.y * jxy + n.y * n.y * jyy;
+        if (resp > best) {
+            second = best;
+            st.secondary = st.primary;
+            best = resp;
+            st.primary = d;
+        } else if (resp > second) {
+            second = resp;
+            st.secondary = d;
+        }
+    }
+    st.isotropic = coh2 < (quality == 0 ? 0.20f : 0.10f);
+    if (quality > 0 && best > 1e-12f) {
+        const float adj = 0.8535534f;
+        const float ratio = fclamp(second / best, 0.0f, 1.0f);
+        st.mix = 0.5f * fclamp((ratio - adj) / (1.0f - adj), 0.0f, 1.0f);
+    }
+    const float ugx = 0.5f * (ds_at<T>(a.u, w, h, ix + 1, iy) - ds_at<T>(a.u, w, h, ix - 1, iy));
+    const float ugy = 0.5f * (ds_at<T>(a.u, w, h, ix, iy + 1) - ds_at<T>(a.u, w, h, ix, iy - 1));
+    const float vgx = 0.5f * (ds_at<T>(a.v, w, h, ix + 1, iy) - ds_at<T>(a.v, w, h, ix - 1, iy));
+    const float vgy = 0.5f * (ds_at<T>(a.v, w, h, ix, iy + 1) - ds_at<T>(a.v, w, h, ix, iy - 1));
+    const float cen = ugx * ugx + ugy * ugy + vgx * vgx + vgy * vgy;
+    const float2 nrm = ds_dir(st.primary);
+    const float un = ugx * nrm.x + ugy * nrm.y, vn = vgx * nrm.x + vgy * nrm.y;
+    const float aligned = un * un + vn * vn;
+    const int nsx = (int)lroundf(nrm.x), nsy = (int)lroundf(nrm.y);
+    const float cu = ds_at<T>(a.u, w, h, ix, iy), cv = ds_at<T>(a.v, w, h, ix, iy);
+    const float curu = ds_at<T>(a.u, w, h, ix + nsx, iy + nsy) - 2.0f * cu + ds_at<T>(a.u, w, h, ix - nsx, iy - nsy);
+    const float curv = ds_at<T>(a.v, w, h, ix + nsx, iy + nsy) - 2.0f * cv + ds_at<T>(a.v, w, h, ix - nsx, iy - nsy);
+    const float curE = curu * curu + curv * curv;
+    const float sharp = curE / (curE + cen + 1.0e-12f);
+    const float luma_g = trace / (trace + 3.0e-5f);
+    const float conf_g = coh2 / (coh2 + 0.08f);
+    const float chroma_g = cen / (cen + 2.0e-5f);
+    const float dir_g = aligned / (cen + 1.0e-12f);
+    const float centerE = cgx * cgx + cgy * cgy;
+    const float pos_g = centerE / (centerE + 1.0e-5f);
+    st.gate = fclamp(luma_g * conf_g * chroma_g * dir_g * pos_g * sharp, 0.0f, 1.0f);
+    if (quality == 0)
+        st.gate *= 0.985f;
+    else if (quality == 2)
+        st.gate = fminf(1.0f, st.gate * 1.01f);
+    if (sharp < (quality == 0 ? 0.20f : 0.12f))
+        st.gate = 0.0f;
+    return st;
+}
+
+struct DsEnv {
+    float nx, ny;
+    int radius;
+    float aniso;
+    bool iso;
+};
+
+// spatialWeight (downsample.cpp:402-411)
+__device__ __forceinline__ float ds_spatial(const DsEnv &e, float dx, float dy) {
+    const float r2 = dx * dx + dy * dy;
+    if (e.iso)
+        return 1.0f / (1.0f + 0.42f * r2);
+    const float nd = dx * e.nx + dy * e.ny;
+    const float td = -dx * e.ny + dy * e.nx;
+    return 1.0f / (1.0f + (0.9f + e.aniso) * nd * nd + 0.16f * td * td);
+}
+
+// accumulateGuided (downsample.cpp:413-458) over N envelopes
+template <typename T, int N>
+__device__ void ds_accumulate(const DsArgs &a, float cx, float cy, const DsEnv (&env)[N], float (&aw)[N],
+                              float (&au)[N], float (&av)[N]) {
+    const int w = a.w, h = a.h;
+    int maxr = 0;
+    _Pragma("unroll") for (int i = 0; i < N; ++i) {
+        aw[i] = au[i] = av[i] = 0.0f;
+        maxr = max(maxr, env[i].radius);
+    }
+    const int x0 = (int)ceilf(cx - (float)maxr), x1 = (int)floorf(cx + (float)maxr);
+    const int y0 = (int)ceilf(cy - (float)maxr), y1 = (int)floorf(cy + (float)maxr);
+    const int cx0 = (int)floorf(cx), cy0 = (int)floorf(cy);
+    const float fx = cx - (float)cx0, fy = cy - (float)cy0;
+    const float ya = ds_at<T>(a.y, w, h, cx0, cy0), yb = ds_at<T>(a.y, w, h, cx0 + 1, cy0);
+    const float yc = ds_at<T>(a.y, w, h, cx0, cy0 + 1), yd = ds_at<T>(a.y, w, h, cx0 + 1, cy0 + 1);
+    const float cl = ya + (yb - ya) * fx + (yc - ya) * fy + (ya - yb - yc + yd) * fx * fy;
+    const float rs2 = 2.25e-4f;
+    for (int sy = y0; sy <= y1; ++sy)
+        for (int sx = x0; sx <= x1; ++sx) {
+            const float dx = (float)sx - cx, dy = (float)sy - cy;
+            const float delta = ds_at<T>(a.y, w, h, sx, sy) - cl;
+            const float rw = rs2 / (rs2 + delta * delta);
+            const float su = ds_at<T>(a.u, w, h, sx, sy), sv = ds_at<T>(a.v, w, h, sx, sy);
+            _Pragma("unroll") for (int i = 0; i < N; ++i) {
+                if (fabsf(dx) > (float)env[i].radius || fabsf(dy) > (float)env[i].radius)
+                    continue;
+                const float wgt = rw * ds_spatial(env[i], dx, dy);
+                aw[i] += wgt;
+                au[i] += wgt * su;
+                av[i] += wgt * sv;
+            }
+        }
+}
+
+__device__ __forceinline__ float ds_norm(float sum, float wgt, float fallback) {
+    return wgt > 1e-12f ? sum / wgt : fallback;
+}
+
+// writeRow (downsample.cpp:117-126)
+template <typename T> __device__ __forceinline__ void ds_store(const RawOut &P, int x, int y, float v) {
+    T *row = reinterpret_cast<T *>(static_cast<char *>(P.data) + P.stride * (ptrdiff_t)y);
+    if constexpr (sizeof(T) == 4)
+        row[x] = v;
+    else
+        row[x] = pack_sample<T>(v, P);
+}
+
+// packed baseline (strength 0) or the guided result of quality 0/1
+// (processFrame, downsample.cpp:686-712, with makeGuidedPixel,
+// downsample.cpp:466-507); one thread per chroma sample
+template <typename T> __global__ __launch_bounds__(kBlock) void k_ds_output(DsArgs a) {
+    const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
+    if (x >= a.cw)
+        return;
+    const size_t i = (size_t)y * a.cw + x;
+    const float bu = a.base_u[i], bv = a.base_v[i];
+    float ru = bu, rv = bv;
+    if (a.strength != 0.0f) {
+        const float cx = 2.0f * (float)x + 0.5f + a.shift_x, cy = 2.0f * (float)y + 0.5f + a.shift_y;
+        const DsStats st = ds_classify<T>(a, cx, cy, a.quality);
+        float gu = bu, gv = bv, gate = st.gate;
+        if (gate <= 1.0e-4f) {
+            gate = 0.0f;
+        } else if (a.quality == 0) {
+            const float2 n = ds_dir(st.primary);
+            const DsEnv env[1] = {{n.x, n.y, 2, 1.8f, st.isotropic}};
+            float aw[1], au[1], av[1];
+            ds_accumulate<T, 1>(a, cx, cy, env, aw, au, av);
+            gu = ds_norm(au[0], aw[0], bu);
+            gv = ds_norm(av[0], aw[0], bv);
+        } else {
+            const float2 n0 = ds_dir(st.primary), n1 = ds_dir(st.secondary);
+            const DsEnv env[2] = {{n0.x, n0.y, 3, 2.2f, st.isotropic}, {n1.x, n1.y, 3, 2.2f, st.isotropic}};
+            float aw[2], au[2], av[2];
+            ds_accumulate<T, 2>(a, cx, cy, env, aw, au, av);
+            const float u0 = ds_norm(au[0], aw[0], bu), v0 = ds_norm(av[0], aw[0], bv);
+            const float u1 = ds_norm(au[1], aw[1], bu), v1 = ds_norm(av[1], aw[1], bv);
+            gu = u0 + st.mix * (u1 - u0);
+            gv = v0 + st.mix * (v1 - v0);
+        }
+        ru = bu + a.strength * gate * (gu - bu);
+        rv = bv + a.strength * gate * (gv - bv);
+    }
+    ds_store<T>(a.out_u, x, y, ru);
+    ds_store<T>(a.out_v, x, y, rv);
+}
+
+// quality 2: gate and the four candidates of every chroma sample
+// (generateCandidateRow, downsample.cpp:526-568)
+template <typename T> __global__ __launch_bounds__(kBlock) void k_ds_candidates(DsArgs a) {
+    const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
+    if (x >= a.cw)
+        return;
+    const size_t nc = (size_t)a.cw * a.ch, i = (size_t)y * a.cw + x;
+    const float bu = a.base_u[i], bv = a.base_v[i];
+    const float cx = 2.0f * (float)x + 0.5f + a.shift_x, cy = 2.0f * (float)y + 0.5f + a.shift_y;
+    const DsStats st = ds_classify<T>(a, cx, cy, 2);
+    const float gate = st.gate > 1.0e-4f ? st.gate : 0.0f;
+    a.gate[i] = gate;
+    if (gate == 0.0f) {
+        _Pragma("unroll") for (int c = 0; c < 4; ++c) {
+            a.cand_u[c * nc + i] = bu;
+            a.cand_v[c * nc + i] = bv;
+        }
+        return;
+    }
+    const float2 n0 = ds_dir(st.primary), n1 = ds_dir(st.secondary);
+    const DsEnv env[5] = {{n0.x, n0.y, 3, 2.2f, st.isotropic}, {n1.x, n1.y, 3, 2.2f, st.isotropic},
+                          {n0.x, n0.y, 4, 2.7f, st.isotropic}, {n1.x, n1.y, 4, 2.7f, st.isotropic},
+                          {n0.x, n0.y, 4, 0.0f, true}};
+    float aw[5], au[5], av[5];
+    ds_accumulate<T, 5>(a, cx, cy, env, aw, au, av);
+    const float u0 = ds_norm(au[0], aw[0], bu), v0 = ds_norm(av[0], aw[0], bv);
+    const float u1 = ds_norm(au[1], aw[1], bu), v1 = ds_norm(av[1], aw[1], bv);
+    a.cand_u[i] = u0 + st.mix * (u1 - u0);
+    a.cand_v[i] = v0 + st.mix * (v1 - v0);
+    _Pragma("unroll") for (int c = 1; c < 4; ++c) {
+        a.cand_u[c * nc + i] = ds_norm(au[c + 1], aw[c + 1], bu);
+        a.cand_v[c * nc + i] = ds_norm(av[c + 1], aw[c + 1], bv);
+    }
+}
+
+// quality 2: loopback score over the sample's 2x2 source block, best
+// candidate, blend and packing (scoreCandidateRow, downsample.cpp:576-626;
+// rows outside the frame clamp as the three-row ring's lookup does)
+template <typename T> __global__ __launch_bounds__(kBlock) void k_ds_score(DsArgs a) {
+    const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
+    if (x >= a.cw)
+        return;
+    const int cw = a.cw, ch = a.ch, w = a.w, h = a.h;
+    const size_t nc = (size_t)cw * ch, i = (size_t)y * cw + x;
+    const float bu = a.base_u[i], bv = a.base_v[i];
+    const float gate = a.gate[i];
+    float ru = bu, rv = bv;
+    if (gate != 0.0f) {
+        float score[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int hy = 2 * y; hy <= 2 * y + 1; ++hy)
+            for (int hx = 2 * x; hx <= 2 * x + 1; ++hx) {
+                const float gx = 0.5f * (ds_at<T>(a.y, w, h, hx + 1, hy) - ds_at<T>(a.y, w, h, hx - 1, hy));
+                const float gy = 0.5f * (ds_at<T>(a.y, w, h, hx, hy + 1) - ds_at<T>(a.y, w, h, hx, hy - 1));
+                const float e2 = gx * gx + gy * gy;
+                const float ew = 0.125f + e2 / (e2 + 1.0e-5f);
+                const float lx = ((float)hx - 0.5f - a.shift_x) * 0.5f;
+                const float ly = ((float)hy - 0.5f - a.shift_y) * 0.5f;
+                const float su = ds_at<T>(a.u, w, h, hx, hy), sv = ds_at<T>(a.v, w, h, hx, hy);
+                const int rx0 = (int)floorf(lx), ry0 = (int)floorf(ly);
+                const int x0 = clampi(rx0, 0, cw - 1), x1 = clampi(rx0 + 1, 0, cw - 1);
+                const int r0 = clampi(ry0, 0, ch - 1), r1 = clampi(ry0 + 1, 0, ch - 1);
+                const float fx = lx - (float)rx0, fy = ly - (float)ry0;
+                const float wts[4] = {(1.0f - fx) * (1.0f - fy), fx * (1.0f - fy), (1.0f - fx) * fy, fx * fy};
+                const size_t idx[4] = {(size_t)r0 * cw + x0, (size_t)r0 * cw + x1, (size_t)r1 * cw + x0,
+                                       (size_t)r1 * cw + x1};
+                _Pragma("unroll") for (int c = 0; c < 4; ++c) {
+                    float recu = 0.0f, recv = 0.0f;
+                    _Pragma("unroll") for (int k = 0; k < 4; ++k) {
+                        const size_t j = idx[k];
+                        float vu = a.base_u[j], vv = a.base_v[j];
+                        vu += a.gate[j] * (a.cand_u[c * nc + j] - vu);
+                        vv += a.gate[j] * (a.cand_v[c * nc + j] - vv);
+                        recu += wts[k] * vu;
+                        recv += wts[k] * vv;
+                    }
+                    const float du = recu - su, dv = recv - sv;
+                    score[c] += ew * (du * du + dv * dv);
+                }
+            }
+        int best = 0;
+        _Pragma("unroll") for (int c = 1; c < 4; ++c)
+            if (score[c] < score[best])
+                best = c;
+        ru = bu + a.strength * gate * (a.cand_u[best * nc + i] - bu);
+        rv = bv + a.strength * gate * (a.cand_v[best * nc + i] - bv);
+    }
+    ds_store<T>(a.out_u, x, y, ru);
+    ds_store<T>(a.out_v, x, y, rv);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -3539,5 +3888,45 @@ hipError_t launch_collaborative(const float *guide, int gw, int gh, float *u, fl
     hipLaunchKernelGGL(k_bm_aggregate, rows_grid(w, h), dim3(kBlock), 0, s, A);
     return hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// Downsample launchers
+// ---------------------------------------------------------------------------
+
+#define LGCR_DS_TYPED(K, grid)                                                                  \
+    if (a.type == RAW_U8)                                                                       \
+        hipLaunchKernelGGL(K<uint8_t>, grid, dim3(kBlock), 0, s, a);                            \
+    else if (a.type == RAW_U16)                                                                 \
+        hipLaunchKernelGGL(K<uint16_t>, grid, dim3(kBlock), 0, s, a);                           \
+    else                                                                                        \
+        hipLaunchKernelGGL(K<float>, grid, dim3(kBlock), 0, s, a);
+
+hipError_t launch_ds_base(const DsArgs &a, hipStream_t s) {
+    const unsigned gx = (unsigned)((a.cw + kBlock - 1) / kBlock);
+    LGCR_DS_TYPED(k_ds_hpass, dim3(gx, (unsigned)a.h))
+    hipLaunchKernelGGL(k_ds_vpass, dim3(gx, (unsigned)a.ch), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_ds_output(const DsArgs &a, hipStream_t s) {
+    const dim3 g((unsigned)((a.cw + kBlock - 1) / kBlock), (unsigned)a.ch);
+    LGCR_DS_TYPED(k_ds_output, g)
+    return hipGetLastError();
+}
+
+hipError_t launch_ds_candidates(const DsArgs &a, hipStream_t s) {
+    const dim3 g((unsigned)((a.cw + kBlock - 1) / kBlock), (unsigned)a.ch);
+    LGCR_DS_TYPED(k_ds_candidates, g)
+    return hipGetLastError();
+}
+
+hipError_t launch_ds_score(const DsArgs &a, hipStream_t s) {
+    const dim3 g((unsigned)((a.cw + kBlock - 1) / kBlock), (unsigned)a.ch);
+    LGCR_DS_TYPED(k_ds_score, g)
+    return hipGetLastError();
+}
+
+#undef LGCR_DS_TYPED
+
 
 }  // namespace lgcr_hip

@@ -541,6 +541,9 @@ struct Workspace {
     float *lc = nullptr, *ms = nullptr, *hbuf = nullptr;
     float *out_u = nullptr, *out_v = nullptr, *plain_u = nullptr, *plain_v = nullptr;
     float *luma_tmp = nullptr, *luma_out = nullptr;
+    // Downsample: horizontal pass (2 x cw x h), baseline (2 x cw x ch),
+    // quality-2 gate and candidates (9 x cw x ch)
+    float *ds_tmp = nullptr, *ds_base = nullptr, *ds_cand = nullptr;
     uint64_t *seed = nullptr, *mask = nullptr, *mtmp = nullptr;  // bit planes, rows of w64 words
     uint8_t *cmask = nullptr;
     uint32_t *row_off = nullptr, *row_cnt = nullptr, *indices = nullptr, *counters = nullptr;
@@ -595,6 +598,11 @@ struct lgcr_clip {
     lgcr_tuning tune{8, 32};          // tile shapes (lgcr_clip_autotune)
     bool temporal = false;  // TRecon clip (neighbour buffers)
     bool sharpen = false;   // Sharpen clip
+    bool downsample = false;  // Downsample clip
+    lgcr_downsample_params dsp{};
+    float ds_shift_x = -0.5f, ds_shift_y = 0.0f, ds_strength = 1.0f;
+    void *ds_tab = nullptr;  // baseline taps
+    WeightsDev ds_xw, ds_yw;
     lgcr_sharpen_params sp{};
     void *sharpen_tab = nullptr;  // luma footprints of the chroma samples
     const int *sxb = nullptr, *sxe = nullptr, *syb = nullptr, *sye = nullptr;
@@ -837,6 +845,19 @@ struct Carver {
 
 void layout(const lgcr_clip *c, Workspace *w, Carver &cv) {
     const size_t ns = size_t(c->sw) * c->sh, nc = size_t(c->cw) * c->ch, no = size_t(c->ow) * c->oh;
+    if (c->downsample) {
+        // raw 4:4:4 in, raw Y + 4:2:0 chroma out, float intermediates
+        for (int i = 0; i < 3; ++i)
+            w->raw_in[i] = cv.take<uint8_t>(ns * c->bps);
+        w->raw_out[0] = cv.take<uint8_t>(ns * c->bps);
+        w->raw_out[1] = cv.take<uint8_t>(nc * c->bps);
+        w->raw_out[2] = cv.take<uint8_t>(nc * c->bps);
+        w->ds_tmp = cv.take<float>(2 * size_t(c->cw) * c->sh);
+        w->ds_base = cv.take<float>(2 * nc);
+        if (c->dsp.quality == 2 && c->ds_strength != 0.0f)
+            w->ds_cand = cv.take<float>(9 * nc);
+        return;
+    }
     if (c->sharpen) {
         // converted planes, sharpened planes, footprint luma
         const size_t nps[3] = {ns, nc, nc};
@@ -1984,6 +2005,8 @@ void lgcr_clip_destroy(lgcr_clip *c) {
         hipFree(c->weights_dev);
     if (c->sharpen_tab)
         hipFree(c->sharpen_tab);
+    if (c->ds_tab)
+        hipFree(c->ds_tab);
     delete c;
 }
 
@@ -2526,6 +2549,271 @@ lgcr_status lgcr_sharpen_frame(lgcr_clip *c, const lgcr_frame *in, lgcr_frame *o
     return LGCR_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Downsample (downsampleCreate / downsampleGetFrame, src/downsample.cpp:725-906)
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// buildBinomialWeights (downsample.cpp:52-84): [1 4 6 4 1]/16 centred on the
+// nearest sample at integer phase (shift +-0.5), else [1 3 3 1]/8
+HostWeights binomial_weights(int src_n, float shift) {
+    HostWeights t;
+    t.n = src_n / 2;
+    const bool integer_phase = std::fabs(std::fabs(shift) - 0.5f) < 1e-6f;
+    t.sup = integer_phase ? 5 : 4;
+    t.start.resize(t.n);
+    t.w.resize(size_t(t.n) * t.sup);
+    static const float iw[5] = {1.0f / 16.0f, 4.0f / 16.0f, 6.0f / 16.0f, 4.0f / 16.0f, 1.0f / 16.0f};
+    static const float hw[4] = {1.0f / 8.0f, 3.0f / 8.0f, 3.0f / 8.0f, 1.0f / 8.0f};
+    for (int i = 0; i < t.n; ++i) {
+        const float center = 2.0f * float(i) + 0.5f + shift;
+        if (integer_phase) {
+            t.start[i] = int(lroundf(center)) - 2;
+            std::copy(iw, iw + 5, t.w.begin() + size_t(i) * 5);
+        } else {
+            t.start[i] = int(std::floor(center)) - 1;
+            std::copy(hw, hw + 4, t.w.begin() + size_t(i) * 4);
+        }
+    }
+    return t;
+}
+
+// buildDownsampleWeights (downsample.cpp:86-92): spline36 / lanczos3 through
+// the shared weight builder at half size (support 3, widened by the 2x)
+HostWeights downsample_weights(int src_n, int kernel, float shift) {
+    if (kernel == LGCR_DS_KERNEL_BINOMIAL)
+        return binomial_weights(src_n, shift);
+    return build_weights(src_n, src_n / 2, kernel == LGCR_DS_KERNEL_SPLINE36 ? LGCR_KERNEL_SPLINE36 : LGCR_KERNEL_LANCZOS,
+                         3.0, 0.0, 3.0, double(shift));
+}
+
+lgcr_status enqueue_downsample(lgcr_clip *c, Workspace *w, const void *const in[3], const ptrdiff_t in_stride[3],
+                               void *const out[3], const ptrdiff_t out_stride[3], bool timed) {
+    hipStream_t s = w->stream;
+    auto mark = [&](int stage) -> hipError_t {
+        if (!timed)
+            return hipSuccess;
+        hipEvent_t ev = nullptr;
+        if (!w->pool.empty()) {
+            ev = w->pool.back();
+            w->pool.pop_back();
+        } else {
+            const hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableSystemFence);
+            if (e != hipSuccess)
+                return e;
+        }
+        w->marks.emplace_back(stage, ev);
+        return hipEventRecord(ev, s);
+    };
+    const bool is_float = c->f.sample_type == 1;
+    const int maxv = is_float ? 0 : (1 << c->f.bits_per_sample) - 1;
+    // processFrame (downsample.cpp:630-654): scale 1/max, chroma offset -0.5
+    const float scale = is_float ? 1.0f : 1.0f / float(maxv);
+    const float coff = is_float ? 0.0f : -0.5f;
+    const size_t nc = size_t(c->cw) * c->ch;
+    DsArgs a{};
+    a.y = DsPlane{in[0], in_stride[0], scale, 0.0f};
+    a.u = DsPlane{in[1], in_stride[1], scale, coff};
+    a.v = DsPlane{in[2], in_stride[2], scale, coff};
+    a.type = c->raw_type;
+    a.w = c->sw;
+    a.h = c->sh;
+    a.cw = c->cw;
+    a.ch = c->ch;
+    a.xw = c->ds_xw;
+    a.yw = c->ds_yw;
+    a.tmp_u = w->ds_tmp;
+    a.tmp_v = w->ds_tmp + size_t(c->cw) * c->sh;
+    a.base_u = w->ds_base;
+    a.base_v = w->ds_base + nc;
+    a.gate = w->ds_cand;
+    a.cand_u = w->ds_cand ? w->ds_cand + nc : nullptr;
+    a.cand_v = w->ds_cand ? w->ds_cand + 5 * nc : nullptr;
+    a.quality = c->dsp.quality;
+    a.strength = c->ds_strength;
+    a.shift_x = c->ds_shift_x;
+    a.shift_y = c->ds_shift_y;
+    // writeRow (downsample.cpp:117-126): int(v * max + max / 2 + 0.5), clamped
+    const float oscale = is_float ? 1.0f : float(maxv), ooff = is_float ? 0.0f : 0.5f * float(maxv);
+    a.out_u = RawOut{nullptr, out[1], out_stride[1], c->cw, c->ch, oscale, ooff, 0, maxv};
+    a.out_v = RawOut{nullptr, out[2], out_stride[2], c->cw, c->ch, oscale, ooff, 0, maxv};
+    HIP_TRY(mark(-1));
+    HIP_TRY(launch_ds_base(a, s));
+    HIP_TRY(mark(LGCR_STAGE_DOWNSAMPLE_BASE));
+    if (a.strength != 0.0f && a.quality == 2) {
+        HIP_TRY(launch_ds_candidates(a, s));
+        HIP_TRY(mark(LGCR_STAGE_DOWNSAMPLE_GUIDE));
+        HIP_TRY(launch_ds_score(a, s));
+        HIP_TRY(mark(LGCR_STAGE_DOWNSAMPLE_CANDIDATE_SCORE));
+    } else {
+        HIP_TRY(launch_ds_output(a, s));
+        HIP_TRY(mark(a.strength != 0.0f ? LGCR_STAGE_DOWNSAMPLE_GUIDE : LGCR_STAGE_DOWNSAMPLE_OUTPUT));
+    }
+    if (out[0])
+        HIP_TRY(hipMemcpy2DAsync(out[0], size_t(out_stride[0]), in[0], size_t(in_stride[0]),
+                                 size_t(c->sw) * c->bps, size_t(c->sh), hipMemcpyDeviceToDevice, s));
+    return LGCR_OK;
+}
+
+}  // namespace
+
+void lgcr_downsample_params_default(lgcr_downsample_params *p) {
+    if (!p)
+        return;
+    // downsampleCreate defaults (downsample.cpp:851-893)
+    p->quality = 0;
+    p->kernel = LGCR_DS_KERNEL_SPLINE36;
+    p->strength = 1.0;
+    p->chroma_location = 0;
+}
+
+lgcr_status lgcr_downsample_create(lgcr_device *dev, const lgcr_downsample_params *params, const lgcr_format *fmt,
+                                   int max_inflight, lgcr_clip **out) {
+    if (!dev || !params || !fmt || !out)
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_downsample_create: NULL argument");
+    *out = nullptr;
+    // downsampleCreate validation, in its order (downsample.cpp:825-893)
+    const bool integer = fmt->sample_type == 0 && fmt->bits_per_sample >= 8 && fmt->bits_per_sample <= 16;
+    const bool floating = fmt->sample_type == 1 && fmt->bits_per_sample == 32;
+    if (fmt->sub_sampling_w != 0 || fmt->sub_sampling_h != 0 || (!integer && !floating))
+        return fail(LGCR_INVALID_ARGUMENT, "Downsample: input must be planar YUV444 (8-16 bit integer or float32)");
+    if (fmt->width <= 0 || fmt->height <= 0 || (fmt->width & 1) || (fmt->height & 1))
+        return fail(LGCR_INVALID_ARGUMENT, "Downsample: constant even input width and height are required");
+    if (params->quality < 0 || params->quality > 2)
+        return fail(LGCR_INVALID_ARGUMENT, "Downsample: quality must be 0, 1, or 2");
+    if (params->kernel < LGCR_DS_KERNEL_SPLINE36 || params->kernel > LGCR_DS_KERNEL_BINOMIAL)
+        return fail(LGCR_INVALID_ARGUMENT, "Downsample: kernel must be \"spline36\", \"lanczos3\", or \"binomial\"");
+    const float strength = float(params->strength);
+    if (!std::isfinite(strength) || strength < 0.0f || strength > 1.0f)
+        return fail(LGCR_INVALID_ARGUMENT, "Downsample: strength must be in the range 0..1");
+    static const float sx[6] = {-0.5f, 0.0f, -0.5f, 0.0f, -0.5f, 0.0f};
+    static const float sy[6] = {0.0f, 0.0f, -0.5f, -0.5f, 0.5f, 0.5f};
+    if (params->chroma_location < 0 || params->chroma_location > 5)
+        return fail(LGCR_INVALID_ARGUMENT, "Downsample: loc must be left, center, topleft, top, bottomleft, or bottom");
+    auto c = std::make_unique<lgcr_clip>();
+    c->dev = dev;
+    c->downsample = true;
+    c->dsp = *params;
+    c->ds_strength = strength;
+    c->ds_shift_x = sx[params->chroma_location];
+    c->ds_shift_y = sy[params->chroma_location];
+    c->f = *fmt;
+    c->sw = c->ow = fmt->width;
+    c->sh = c->oh = fmt->height;
+    c->cw = fmt->width / 2;
+    c->ch = fmt->height / 2;
+    c->rw = c->rh = 2.0;
+    c->raw_type = fmt->sample_type == 1 ? RAW_F32 : (fmt->bytes_per_sample == 1 ? RAW_U8 : RAW_U16);
+    c->bps = fmt->sample_type == 1 ? 4 : fmt->bytes_per_sample;
+    c->max_inflight = max_inflight > 0 ? max_inflight : 4;
+    HIP_TRY(hipSetDevice(dev->ordinal));
+    const HostWeights xw = downsample_weights(c->sw, params->kernel, c->ds_shift_x);
+    const HostWeights yw = downsample_weights(c->sh, params->kernel, c->ds_shift_y);
+    Blob b;
+    const size_t o0 = b.add(xw.start), o1 = b.add(xw.w), o2 = b.add(yw.start), o3 = b.add(yw.w);
+    HIP_TRY(hipMalloc(&c->ds_tab, b.host.size()));
+    HIP_TRY(hipMemcpy(c->ds_tab, b.host.data(), b.host.size(), hipMemcpyHostToDevice));
+    const uint8_t *base = static_cast<const uint8_t *>(c->ds_tab);
+    c->ds_xw = WeightsDev{xw.n, xw.sup, reinterpret_cast<const int *>(base + o0),
+                          reinterpret_cast<const float *>(base + o1)};
+    c->ds_yw = WeightsDev{yw.n, yw.sup, reinterpret_cast<const int *>(base + o2),
+                          reinterpret_cast<const float *>(base + o3)};
+    std::unique_ptr<Workspace> w;
+    const lgcr_status st = create_workspace(c.get(), w);
+    if (st != LGCR_OK) {
+        lgcr_clip_destroy(c.release());
+        return st;
+    }
+    c->ws_bytes = w->bytes;
+    c->spaces.push_back(std::move(w));
+    c->idle.push_back(0);
+    *out = c.release();
+    return LGCR_OK;
+}
+
+lgcr_status lgcr_downsample_frame(lgcr_clip *c, const lgcr_frame *in, lgcr_frame *out, lgcr_stats *stats) {
+    if (!c || !in || !out)
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_downsample_frame: NULL argument");
+    if (!c->downsample)
+        return fail(LGCR_INVALID_CONTEXT, "lgcr_downsample_frame: clip was not created by lgcr_downsample_create");
+    if (in->memory != out->memory)
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_downsample_frame: input and output must be in the same memory");
+    for (int i = 0; i < 3; ++i)
+        if (!in->data[i] || (i > 0 && !out->data[i]))
+            return fail(LGCR_INVALID_ARGUMENT, "lgcr_downsample_frame: plane %d is NULL", i);
+    int slot = -1;
+    lgcr_status st = acquire(c, &slot);
+    if (st != LGCR_OK)
+        return st;
+    Workspace *w = c->spaces[slot].get();
+    struct Releaser {
+        lgcr_clip *c;
+        int slot;
+        ~Releaser() { release(c, slot); }
+    } rel{c, slot};
+    HIP_TRY(hipSetDevice(c->dev->ordinal));
+    const size_t rs = size_t(c->sw) * c->bps, rc = size_t(c->cw) * c->bps;
+    const void *din[3];
+    ptrdiff_t din_stride[3];
+    void *dout[3];
+    ptrdiff_t dout_stride[3];
+    const bool host = in->memory == LGCR_MEMORY_HOST;
+    HostXfer hx{w, w->stream};
+    if (host) {
+        HIP_TRY(hx.reserve(3 * HostXfer::span(rs, c->sh) + 2 * HostXfer::span(rc, c->ch)));
+        for (int i = 0; i < 3; ++i) {
+            HIP_TRY(hx.in(w->raw_in[i], in->data[i], in->stride[i], rs, c->sh));
+            din[i] = w->raw_in[i];
+            din_stride[i] = ptrdiff_t(rs);
+        }
+        dout[0] = nullptr;  // luma is copied on the host below
+        dout_stride[0] = 0;
+        for (int i = 1; i < 3; ++i) {
+            dout[i] = w->raw_out[i];
+            dout_stride[i] = ptrdiff_t(rc);
+        }
+    } else {
+        for (int i = 0; i < 3; ++i) {
+            din[i] = in->data[i];
+            din_stride[i] = in->stride[i];
+            dout[i] = out->data[i];
+            dout_stride[i] = out->stride[i];
+        }
+    }
+    const bool timed = stats != nullptr || c->timing.load();
+    w->marks.clear();
+    st = enqueue_downsample(c, w, din, din_stride, dout, dout_stride, timed);
+    if (st != LGCR_OK)
+        return st;
+    if (host) {
+        for (int i = 1; i < 3; ++i)
+            HIP_TRY(hx.out(out->data[i], out->stride[i], w->raw_out[i], rc, c->ch));
+        if (out->data[0] && out->data[0] != in->data[0])
+            for (int y = 0; y < c->sh; ++y)
+                std::memcpy(static_cast<uint8_t *>(out->data[0]) + out->stride[0] * y,
+                            static_cast<const uint8_t *>(in->data[0]) + in->stride[0] * y, rs);
+    }
+    HIP_TRY(hipStreamSynchronize(w->stream));
+    hx.finish();
+    double us[LGCR_STAGE_COUNT] = {};
+    uint64_t nst[LGCR_STAGE_COUNT] = {};
+    if (timed) {
+        HIP_TRY(collect_timeline(w, us, nst));
+        if (c->timing.load())
+            add_timing(c, us, nst);
+    }
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        for (int i = 0; i < LGCR_STAGE_COUNT; ++i) {
+            stats->stage_us[i] = us[i];
+            stats->total_us += us[i];
+        }
+        stats->output_pixels = uint64_t(c->sw) * c->sh + 2 * uint64_t(c->cw) * c->ch;
+    }
+    return LGCR_OK;
+}
+
 lgcr_status lgcr_clip_get_tuning(const lgcr_clip *c, lgcr_tuning *t) {
     if (!c || !t)
         return fail(LGCR_INVALID_ARGUMENT, "lgcr_clip_get_tuning: NULL argument");
@@ -2546,7 +2834,7 @@ lgcr_status lgcr_clip_set_tuning(lgcr_clip *c, const lgcr_tuning *t) {
 lgcr_status lgcr_clip_autotune(lgcr_clip *c, const lgcr_frame *frame) {
     if (!c || !frame)
         return fail(LGCR_INVALID_ARGUMENT, "lgcr_clip_autotune: NULL argument");
-    if (c->temporal || c->sharpen)
+    if (c->temporal || c->sharpen || c->downsample)
         return fail(LGCR_UNSUPPORTED, "lgcr_clip_autotune: Recon clips only");
     for (int i = 0; i < 3; ++i)
         if (!frame->data[i])

@@ -35,6 +35,7 @@ SYMBOLS = [
     "lgcr_trecon_frame", "lgcr_sharpen_params_default", "lgcr_sharpen_create", "lgcr_sharpen_frame",
     "lgcr_recon_submit", "lgcr_trecon_submit", "lgcr_ticket_query", "lgcr_ticket_wait",
     "lgcr_clip_set_graphs", "lgcr_clip_get_tuning", "lgcr_clip_set_tuning", "lgcr_clip_autotune",
+    "lgcr_downsample_params_default", "lgcr_downsample_create", "lgcr_downsample_frame",
 ]
 STAGE_NAMES = [
     "ConvertInput", "ResampleLuma", "BuildGuideMaps", "BuildTrustMask", "BuildMutualGate",
@@ -61,6 +62,15 @@ class Params(C.Structure):
 class SharpenParams(C.Structure):
     _fields_ = [("alpha", C.c_double), ("sigma", C.c_double), ("sratio", C.c_double), ("gspatial", C.c_double),
                 ("ar_margin", C.c_double)]
+
+
+class DownsampleParams(C.Structure):
+    _fields_ = [("quality", C.c_int), ("kernel", C.c_int), ("strength", C.c_double), ("chroma_location", C.c_int)]
+
+
+DS_KERNELS = {"spline36": 0, "lanczos3": 1, "binomial": 2}
+# loc -> H.273 _ChromaLocation (downsample.cpp:875-893)
+DS_LOCATIONS = {"left": 0, "center": 1, "topleft": 2, "top": 3, "bottomleft": 4, "bottom": 5}
 
 
 class Format(C.Structure):
@@ -142,6 +152,10 @@ def lib():
         L.lgcr_sharpen_params_default.argtypes = [C.POINTER(SharpenParams)]
         L.lgcr_sharpen_create.argtypes = [vp, C.POINTER(SharpenParams), C.POINTER(Format), C.c_int, C.POINTER(vp)]
         L.lgcr_sharpen_frame.argtypes = [vp, C.POINTER(Frame), C.POINTER(Frame)]
+        L.lgcr_downsample_params_default.argtypes = [C.POINTER(DownsampleParams)]
+        L.lgcr_downsample_create.argtypes = [vp, C.POINTER(DownsampleParams), C.POINTER(Format), C.c_int,
+                                             C.POINTER(vp)]
+        L.lgcr_downsample_frame.argtypes = [vp, C.POINTER(Frame), C.POINTER(Frame), C.POINTER(Stats)]
         _lib = L
     return _lib
 
@@ -213,6 +227,18 @@ def sharpen_params(**kw) -> SharpenParams:
         kw["ar_margin"] = kw.pop("ar")
     for k, v in kw.items():
         setattr(p, k, v)
+    return p
+
+
+def downsample_params(quality=0, kernel="spline36", strength=1.0, loc="left") -> DownsampleParams:
+    """Downsample arguments (downsample.cpp:851-893) as the C struct; kernel
+    and loc take the plugin's names."""
+    p = DownsampleParams()
+    lib().lgcr_downsample_params_default(C.byref(p))
+    p.quality = int(quality)
+    p.kernel = DS_KERNELS.get(kernel, -1) if isinstance(kernel, str) else int(kernel)
+    p.strength = float(strength)
+    p.chroma_location = DS_LOCATIONS.get(loc, -1) if isinstance(loc, str) else int(loc)
     return p
 
 
@@ -376,6 +402,46 @@ class SharpenClip:
         fout = Frame((C.c_void_p * 3)(*[p for p, _ in planes_out]),
                      (C.c_ssize_t * 3)(*[s for _, s in planes_out]), MEMORY_DEVICE)
         check(lib().lgcr_sharpen_frame(self.handle, C.byref(fin), C.byref(fout)))
+
+    def close(self):
+        if self.handle:
+            lib().lgcr_clip_destroy(self.handle)
+            self.handle = None
+
+
+class DownsampleClip:
+    """One Downsample instance: 4:4:4 frames in, 4:2:0 frames out."""
+
+    def __init__(self, device: Device, params: DownsampleParams, fmt: Format, max_inflight=4):
+        h = C.c_void_p()
+        check(lib().lgcr_downsample_create(device.handle, C.byref(params), C.byref(fmt), max_inflight,
+                                           C.byref(h)))
+        self.handle = h
+        self.fmt = fmt
+
+    def downsample_host(self, y, u, v, stats=False, copy_luma=True):
+        """(Y, U, V) 4:4:4 host planes -> (Y, U, V) 4:2:0; with copy_luma
+        False the output Y is the input array itself (plane sharing)."""
+        fin, keep = host_frame((y, u, v))
+        h, w = y.shape
+        ou = np.empty((h // 2, w // 2), y.dtype)
+        ov = np.empty((h // 2, w // 2), y.dtype)
+        oy = np.empty_like(y) if copy_luma else None
+        fout = Frame((C.c_void_p * 3)(oy.ctypes.data if copy_luma else None, ou.ctypes.data, ov.ctypes.data),
+                     (C.c_ssize_t * 3)(oy.strides[0] if copy_luma else 0, ou.strides[0], ov.strides[0]),
+                     MEMORY_HOST)
+        st = Stats() if stats else None
+        check(lib().lgcr_downsample_frame(self.handle, C.byref(fin), C.byref(fout), C.byref(st) if stats else None))
+        out = (oy if copy_luma else y, ou, ov)
+        return (out, st) if stats else out
+
+    def downsample_device(self, planes_in, planes_out):
+        """Device planes ((ptr, byte stride) x 3); planes_out[0] may be (0, 0)."""
+        fin = Frame((C.c_void_p * 3)(*[p for p, _ in planes_in]), (C.c_ssize_t * 3)(*[s for _, s in planes_in]),
+                    MEMORY_DEVICE)
+        fout = Frame((C.c_void_p * 3)(*[p or None for p, _ in planes_out]),
+                     (C.c_ssize_t * 3)(*[s for _, s in planes_out]), MEMORY_DEVICE)
+        check(lib().lgcr_downsample_frame(self.handle, C.byref(fin), C.byref(fout), None))
 
     def close(self):
         if self.handle:

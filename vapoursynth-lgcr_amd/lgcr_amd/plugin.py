@@ -1,10 +1,11 @@
 """Host-side mirror of the LGCR VapourSynth plugin surface (src/plugin.cpp)
 over the MI355X backend.
 
-``core.lgcr.Recon``, ``core.lgcr.TRecon`` and ``core.lgcr.Sharpen`` take the
-reference's arguments with the same names, defaults, ranges and error
-messages (lgcrCreate, plugin.cpp:1012-1158; tReconCreate, plugin.cpp:921-990;
-sharpenCreate, plugin.cpp:85-111) and return clips whose frames are
+``core.lgcr.Recon``, ``core.lgcr.TRecon``, ``core.lgcr.Sharpen`` and
+``core.lgcr.Downsample`` take the reference's arguments with the same names,
+defaults, ranges and error messages (lgcrCreate, plugin.cpp:1012-1158;
+tReconCreate, plugin.cpp:921-990; sharpenCreate, plugin.cpp:85-111;
+downsampleCreate, downsample.cpp:816-906) and return clips whose frames are
 computed by ``liblgcr_hip.so``.  Clips and frames are small in-memory
 stand-ins for VapourSynth's (VapourSynth itself is not a dependency):
 planar numpy planes plus a property dict.  Per frame the filters apply the
@@ -403,10 +404,96 @@ def Sharpen(clip: VideoNode, **args) -> VideoNode:
     return node
 
 
+# ---------------------------------------------------------------------------
+# Downsample (downsampleCreate / downsampleGetFrame, downsample.cpp:725-906)
+# ---------------------------------------------------------------------------
+
+DOWNSAMPLE_ARGS = ("quality", "kernel", "strength", "loc")
+
+
+def _ds_profile_props(props, st, w, h, quality, kernel, strength, loc):
+    """writeDownsampleProfile (downsample.cpp:31-49) with the work counts of
+    processFrame (downsample.cpp:767-789) and the backend's stage times."""
+    cw, ch = w // 2, h // 2
+    pixels = 2 * cw * ch
+    if kernel == "binomial":
+        # 5 taps at integer phase (shift +-0.5), else 4 (downsample.cpp:52-58)
+        xsup = 5 if loc in ("left", "topleft", "bottomleft") else 4
+        ysup = 4 if loc in ("left", "center") else 5
+    else:
+        xsup = ysup = 12
+    base_taps = 2 * (h * cw * xsup + ch * cw * ysup)
+    work = {13: (pixels, base_taps), 16: (pixels, 0)}
+    if strength > 0.0:
+        diameter = 2 * (quality + 2) + 1
+        candidates = (1, 2, 5)[quality]
+        work[14] = (pixels, cw * ch * diameter * diameter * candidates)
+        if quality == 2:
+            work[15] = (pixels, cw * ch * 16)
+    for i, name in enumerate(PROP_STAGE_NAMES):
+        px, taps = work.get(i, (0, 0))
+        props[f"_LGCR_{name}_us"] = float(st.stage_us[i])
+        props[f"_LGCR_{name}_pixels"] = px
+        props[f"_LGCR_{name}_taps"] = taps
+    props["_LGCR_total_us"] = float(st.total_us)
+    props["_LGCR_output_pixels"] = w * h + pixels
+    props["_LGCR_taps_visited"] = base_taps
+
+
+def Downsample(clip: VideoNode, **args) -> VideoNode:
+    """core.lgcr.Downsample(clip, quality=0, kernel="spline36", strength=1.0,
+    loc="left"): planar YUV 4:4:4 to 4:2:0 with the same sample type; the
+    output's Y plane is the source's (plane sharing) and ``_ChromaLocation``
+    is set from ``loc``."""
+    _check_args("Downsample", args, DOWNSAMPLE_ARGS)
+    fmt = clip.format
+    integer = fmt.sample_type == INTEGER and 8 <= fmt.bits_per_sample <= 16
+    floating = fmt.sample_type == FLOAT and fmt.bits_per_sample == 32
+    if fmt.color_family != YUV or fmt.subsampling_w != 0 or fmt.subsampling_h != 0 or not (integer or floating):
+        raise Error("Downsample: input must be planar YUV444 (8-16 bit integer or float32)")
+    if clip.width <= 0 or clip.height <= 0 or clip.width % 2 or clip.height % 2:
+        raise Error("Downsample: constant even input width and height are required")
+    quality = int(_get(args, "quality", 0))
+    if quality < 0 or quality > 2:
+        raise Error("Downsample: quality must be 0, 1, or 2")
+    kernel = _get(args, "kernel", "spline36")
+    if isinstance(kernel, bytes):
+        kernel = kernel.decode()
+    if kernel not in abi.DS_KERNELS:
+        raise Error('Downsample: kernel must be "spline36", "lanczos3", or "binomial"')
+    strength = float(np.float32(_get(args, "strength", 1.0)))
+    if not math.isfinite(strength) or strength < 0.0 or strength > 1.0:
+        raise Error("Downsample: strength must be in the range 0..1")
+    loc = _get(args, "loc", "left")
+    if isinstance(loc, bytes):
+        loc = loc.decode()
+    if loc not in abi.DS_LOCATIONS:
+        raise Error("Downsample: loc must be left, center, topleft, top, bottomleft, or bottom")
+    dp = abi.downsample_params(quality=quality, kernel=kernel, strength=strength, loc=loc)
+    backend = abi.DownsampleClip(_device(), dp, _abi_format(fmt, clip.width, clip.height))
+    fmt_out = replace(fmt, subsampling_w=1, subsampling_h=1)
+
+    def get(n):
+        src = clip.get_frame(n)
+        profile = _profiling()
+        res = backend.downsample_host(*src.planes, stats=profile, copy_luma=False)
+        out, st = res if profile else (res, None)
+        props = dict(src.props)
+        props["_ChromaLocation"] = abi.DS_LOCATIONS[loc]
+        if profile:
+            _ds_profile_props(props, st, clip.width, clip.height, quality, kernel, strength, loc)
+        return VideoFrame(fmt_out, list(out), props)
+
+    node = VideoNode(fmt_out, clip.width, clip.height, clip.num_frames, get)
+    node._backend = backend
+    return node
+
+
 class _Namespace:
     Recon = staticmethod(Recon)
     TRecon = staticmethod(TRecon)
     Sharpen = staticmethod(Sharpen)
+    Downsample = staticmethod(Downsample)
 
 
 class _Core:
