@@ -107,6 +107,48 @@ def cpu_baseline(frames, params_kw, bits, seconds, threads):
     return sum(done) / el, sum(done), el
 
 
+def downsample(args):
+    """Downsample 1920x1080 YUV444P10 -> YUV420P10 (spline36, left siting,
+    strength 1) at each quality on device-resident frames, one frame at a
+    time (each call waits for its frame), beside the oracle on one host
+    thread."""
+    import numpy as np
+    import torch
+    from lgcr_amd import abi, synth
+    W, H, n = 1920, 1080, 32
+    frames = synth.clip(n, W, H, ss_w=0, ss_h=0, bits=10, seed=1, scene_per_frame=True,
+                        threads=min(16, os.cpu_count() or 1))
+    dev = abi.Device(0)
+    din = [[torch.from_numpy(p.view(np.int16)).to("cuda") for p in f] for f in frames]
+    dout = [torch.empty((H // 2, W // 2), dtype=torch.int16, device="cuda") for _ in range(2)]
+    torch.cuda.synchronize()
+    pin = [[(t.data_ptr(), t.stride(0) * 2) for t in f] for f in din]
+    pout = [(0, 0)] + [(t.data_ptr(), t.stride(0) * 2) for t in dout]
+    out = {"workload": "Downsample 1920x1080 YUV444P10 -> YUV420P10, spline36, left, strength 1",
+           "frames": n, "inflight": 1}
+    for q in (0, 1, 2):
+        clip = abi.DownsampleClip(dev, abi.downsample_params(quality=q), abi.make_format(np.uint16, W, H, 0, 0, 10))
+        for f in pin[:4]:
+            clip.downsample_device(f, pout)
+        t0 = time.perf_counter()
+        for f in pin:
+            clip.downsample_device(f, pout)
+        el = time.perf_counter() - t0
+        clip.close()
+        out[f"q{q}_fps"] = round(n / el, 1)
+    if not args.no_cpu_baseline:
+        import oracle  # the CPU baseline only
+        t0 = time.perf_counter()
+        for f in frames[:2]:
+            oracle.downsample_frame(oracle.downsample_params(quality=0), *f, bits=10)
+        out["cpu_oracle_q0_fps"] = round(2 / (time.perf_counter() - t0), 2)
+        out["cpu_oracle_note"] = "oracle C restatement, one host thread"
+    del din, dout
+    dev.close()
+    torch.cuda.empty_cache()
+    return out
+
+
 def serving(args):
     """Clips of mixed resolution / format / filter with Poisson frame
     arrivals: continuous batching vs one blocking call per frame, at the
@@ -161,6 +203,7 @@ def main():
     ap.add_argument("--serve-frames", type=int, default=48)
     ap.add_argument("--serve-rate", default="600,1500", help="offered frame rates (comma-separated)")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-downsample", action="store_true")
     ap.add_argument("--serve-poll-us", type=float, default=20.0)
     ap.add_argument("--serve-inflight", type=int, default=4)
     ap.add_argument("--serve-graphs", action="store_true")
@@ -302,6 +345,9 @@ def main():
 
     if not args.no_serve:
         result["serving"] = serving(args)
+
+    if not args.no_downsample:
+        result["downsample"] = downsample(args)
 
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
