@@ -112,6 +112,8 @@ def downsample(args):
     strength 1) at each quality on device-resident frames, one frame at a
     time (each call waits for its frame), beside the oracle on one host
     thread."""
+    import ctypes as C
+
     import numpy as np
     import torch
     from lgcr_amd import abi, synth
@@ -126,6 +128,11 @@ def downsample(args):
     pout = [(0, 0)] + [(t.data_ptr(), t.stride(0) * 2) for t in dout]
     out = {"workload": "Downsample 1920x1080 YUV444P10 -> YUV420P10, spline36, left, strength 1",
            "frames": n, "inflight": 1}
+    # algorithmic bytes: Y, U, V 4:4:4 read once, U, V 4:2:0 written once
+    # (luma is shared, not written)
+    nbytes = 3 * W * H * 2 + 2 * (W // 2) * (H // 2) * 2
+    out["bytes_per_frame"] = nbytes
+    L = abi.lib()
     for q in (0, 1, 2):
         clip = abi.DownsampleClip(dev, abi.downsample_params(quality=q), abi.make_format(np.uint16, W, H, 0, 0, 10))
         for f in pin[:4]:
@@ -134,8 +141,20 @@ def downsample(args):
         for f in pin:
             clip.downsample_device(f, pout)
         el = time.perf_counter() - t0
-        clip.close()
         out[f"q{q}_fps"] = round(n / el, 1)
+        # per-stage device time (HIP events around each stage), separate pass
+        abi.check(L.lgcr_clip_set_timing(clip.handle, 1))
+        for f in pin:
+            clip.downsample_device(f, pout)
+        us = (C.c_double * abi.STAGE_COUNT)()
+        cnt = (C.c_uint64 * abi.STAGE_COUNT)()
+        abi.check(L.lgcr_clip_stage_times(clip.handle, us, cnt))
+        stages = {abi.STAGE_NAMES[i]: round(us[i] / cnt[i], 2) for i in range(abi.STAGE_COUNT) if cnt[i]}
+        dev_us = sum(stages.values())
+        out[f"q{q}_stage_avg_us"] = stages
+        out[f"q{q}_device_us"] = round(dev_us, 2)
+        out[f"q{q}_hbm_frac_of_device_time"] = round(nbytes / (dev_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 5)
+        clip.close()
     if not args.no_cpu_baseline:
         import oracle  # the CPU baseline only
         t0 = time.perf_counter()

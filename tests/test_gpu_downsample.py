@@ -128,3 +128,22 @@ def test_downsample_constant_formats(device):
                 assert np.array_equal(y, planes[0])
                 for p, c in ((u, color[1]), (v, color[2])):
                     assert float(np.max(np.abs(p.astype(np.float64) - c))) <= (2e-7 if bits == 32 else 0)
+
+
+def test_clip_kinds_are_checked(device):
+    """Recon entry points refuse a Downsample clip; stage fetches are for
+    Recon and TRecon clips only."""
+    import ctypes as C
+    planes = frame444(16, 16, 8)
+    clip = abi.DownsampleClip(device, abi.downsample_params(), abi.make_format(np.uint8, 16, 16, 0, 0))
+    try:
+        fin, keep = abi.host_frame(planes)
+        fout, keep2 = abi.host_frame([np.empty_like(p) for p in planes])
+        L = abi.lib()
+        assert L.lgcr_recon_frame(clip.handle, C.byref(fin), float("nan"), float("nan"), C.byref(fout),
+                                  None) == abi.LGCR_INVALID_CONTEXT
+        assert b"not a Recon clip" in L.lgcr_last_error()
+        needed = C.c_size_t()
+        assert L.lgcr_clip_fetch(clip.handle, abi.FETCH["u"], None, 0, C.byref(needed)) == abi.LGCR_UNSUPPORTED
+    finally:
+        clip.close()

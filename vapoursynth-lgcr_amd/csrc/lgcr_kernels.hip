@@ -3073,15 +3073,44 @@ __device__ __forceinline__ float ds_at(const DsPlane &p, int w, int h, int x, in
 }
 
 // horizontal baseline pass (BaselineRing::ensureRow, downsample.cpp:200-229,
-// scalar branch): one thread per (output column, source row)
+// scalar branch): one thread per (output column, source row).  The block's
+// source segment is converted once into LDS (the taps of neighbouring
+// columns overlap six-fold at 2x decimation); starts are non-decreasing in
+// the column, so the segment is [start of the first column, start of the
+// last + sup).
+constexpr int kDsSeg = 2 * kBlock + 32;
+
 template <typename T> __global__ __launch_bounds__(kBlock) void k_ds_hpass(DsArgs a) {
-    const int x = blockIdx.x * kBlock + threadIdx.x, sy = blockIdx.y;
+    __shared__ float seg_u[kDsSeg], seg_v[kDsSeg];
+    const int x0 = blockIdx.x * kBlock, sy = blockIdx.y, sup = a.xw.sup;
+    const int x = x0 + threadIdx.x;
+    const int s0 = a.xw.start[x0], s1 = a.xw.start[min(x0 + kBlock, a.cw) - 1] + sup;
+    if (s1 - s0 <= kDsSeg) {
+        for (int i = threadIdx.x; i < s1 - s0; i += kBlock) {
+            seg_u[i] = ds_at<T>(a.u, a.w, a.h, s0 + i, sy);
+            seg_v[i] = ds_at<T>(a.v, a.w, a.h, s0 + i, sy);
+        }
+        __syncthreads();
+        if (x >= a.cw)
+            return;
+        const float *wt = a.xw.w + (size_t)x * sup;
+        const int o = a.xw.start[x] - s0;
+        float su = 0.0f, sv = 0.0f;
+        for (int t = 0; t < sup; ++t) {
+            su += wt[t] * seg_u[o + t];
+            sv += wt[t] * seg_v[o + t];
+        }
+        a.tmp_u[(size_t)sy * a.cw + x] = su;
+        a.tmp_v[(size_t)sy * a.cw + x] = sv;
+        return;
+    }
+    // wider tables (not produced by the 2x builders): taps read the planes
     if (x >= a.cw)
         return;
-    const float *wt = a.xw.w + (size_t)x * a.xw.sup;
+    const float *wt = a.xw.w + (size_t)x * sup;
     const int st = a.xw.start[x];
     float su = 0.0f, sv = 0.0f;
-    for (int t = 0; t < a.xw.sup; ++t) {
+    for (int t = 0; t < sup; ++t) {
         su += wt[t] * ds_at<T>(a.u, a.w, a.h, st + t, sy);
         sv += wt[t] * ds_at<T>(a.v, a.w, a.h, st + t, sy);
     }
@@ -3124,19 +3153,40 @@ struct DsStats {
     bool isotropic = true;
 };
 
+// Source sample access for the guided passes: straight from the planes
+// (clamped, converted per read) or from a block's LDS tile that holds the
+// same values for every coordinate the block's pixels read.
+template <typename T> struct DsGlobal {
+    const DsArgs &a;
+    __device__ float y(int x, int r) const { return ds_at<T>(a.y, a.w, a.h, x, r); }
+    __device__ float u(int x, int r) const { return ds_at<T>(a.u, a.w, a.h, x, r); }
+    __device__ float v(int x, int r) const { return ds_at<T>(a.v, a.w, a.h, x, r); }
+};
+// a 32x8 block of chroma outputs reads source columns [2*x0-6, 2*x0+70) and
+// rows [2*y0-6, 2*y0+22): window radius <= 4 around centres within 0.5 of
+// 2*x+0.5, plus the +-1 neighbours of the direction test and bilinear centre
+constexpr int kDsTW = 32, kDsTH = 8, kDsHalo = 6;
+constexpr int kDsSW = 2 * kDsTW + 2 * kDsHalo, kDsSH = 2 * kDsTH + 2 * kDsHalo;
+struct DsTile {
+    const float *ty, *tu, *tv;
+    int ox, oy;
+    __device__ float y(int x, int r) const { return ty[(r - oy) * kDsSW + (x - ox)]; }
+    __device__ float u(int x, int r) const { return tu[(r - oy) * kDsSW + (x - ox)]; }
+    __device__ float v(int x, int r) const { return tv[(r - oy) * kDsSW + (x - ox)]; }
+};
+
 // classify (downsample.cpp:281-385)
-template <typename T>
-__device__ DsStats ds_classify(const DsArgs &a, float cx, float cy, int quality) {
+template <typename S>
+__device__ DsStats ds_classify(const S &src, float cx, float cy, int quality) {
     DsStats st;
-    const int w = a.w, h = a.h;
     const int ix = (int)floorf(cx + 0.5f), iy = (int)floorf(cy + 0.5f);
     float jxx = 0.0f, jxy = 0.0f, jyy = 0.0f, cgx = 0.0f, cgy = 0.0f;
     if (quality == 0) {
-        const float tl = ds_at<T>(a.y, w, h, ix - 1, iy - 1), tc = ds_at<T>(a.y, w, h, ix, iy - 1);
-        const float tr = ds_at<T>(a.y, w, h, ix + 1, iy - 1);
-        const float ml = ds_at<T>(a.y, w, h, ix - 1, iy), mr = ds_at<T>(a.y, w, h, ix + 1, iy);
-        const float bl = ds_at<T>(a.y, w, h, ix - 1, iy + 1), bc = ds_at<T>(a.y, w, h, ix, iy + 1);
-        const float br = ds_at<T>(a.y, w, h, ix + 1, iy + 1);
+        const float tl = src.y(ix - 1, iy - 1), tc = src.y(ix, iy - 1);
+        const float tr = src.y(ix + 1, iy - 1);
+        const float ml = src.y(ix - 1, iy), mr = src.y(ix + 1, iy);
+        const float bl = src.y(ix - 1, iy + 1), bc = src.y(ix, iy + 1);
+        const float br = src.y(ix + 1, iy + 1);
         cgx = (tr + 2.0f * mr + br - tl - 2.0f * ml - bl) * 0.125f;
         cgy = (bl + 2.0f * bc + br - tl - 2.0f * tc - tr) * 0.125f;
         jxx = cgx * cgx;
@@ -3144,8 +3194,8 @@ __device__ DsStats ds_classify(const DsArgs &a, float cx, float cy, int quality)
         jyy = cgy * cgy;
     } else {
         for (int k = -1; k <= 1; ++k) {
-            const float gx = 0.5f * (ds_at<T>(a.y, w, h, ix + 1, iy + k) - ds_at<T>(a.y, w, h, ix - 1, iy + k));
-            const float gy = 0.5f * (ds_at<T>(a.y, w, h, ix, iy + k + 1) - ds_at<T>(a.y, w, h, ix, iy + k - 1));
+            const float gx = 0.5f * (src.y(ix + 1, iy + k) - src.y(ix - 1, iy + k));
+            const float gy = 0.5f * (src.y(ix, iy + k + 1) - src.y(ix, iy + k - 1));
             jxx += gx * gx;
             jxy += gx * gy;
             jyy += gy * gy;
@@ -3182,18 +3232,18 @@ __device__ DsStats ds_classify(const DsArgs &a, float cx, float cy, int quality)
         const float ratio = fclamp(second / best, 0.0f, 1.0f);
         st.mix = 0.5f * fclamp((ratio - adj) / (1.0f - adj), 0.0f, 1.0f);
     }
-    const float ugx = 0.5f * (ds_at<T>(a.u, w, h, ix + 1, iy) - ds_at<T>(a.u, w, h, ix - 1, iy));
-    const float ugy = 0.5f * (ds_at<T>(a.u, w, h, ix, iy + 1) - ds_at<T>(a.u, w, h, ix, iy - 1));
-    const float vgx = 0.5f * (ds_at<T>(a.v, w, h, ix + 1, iy) - ds_at<T>(a.v, w, h, ix - 1, iy));
-    const float vgy = 0.5f * (ds_at<T>(a.v, w, h, ix, iy + 1) - ds_at<T>(a.v, w, h, ix, iy - 1));
+    const float ugx = 0.5f * (src.u(ix + 1, iy) - src.u(ix - 1, iy));
+    const float ugy = 0.5f * (src.u(ix, iy + 1) - src.u(ix, iy - 1));
+    const float vgx = 0.5f * (src.v(ix + 1, iy) - src.v(ix - 1, iy));
+    const float vgy = 0.5f * (src.v(ix, iy + 1) - src.v(ix, iy - 1));
     const float cen = ugx * ugx + ugy * ugy + vgx * vgx + vgy * vgy;
     const float2 nrm = ds_dir(st.primary);
     const float un = ugx * nrm.x + ugy * nrm.y, vn = vgx * nrm.x + vgy * nrm.y;
     const float aligned = un * un + vn * vn;
     const int nsx = (int)lroundf(nrm.x), nsy = (int)lroundf(nrm.y);
-    const float cu = ds_at<T>(a.u, w, h, ix, iy), cv = ds_at<T>(a.v, w, h, ix, iy);
-    const float curu = ds_at<T>(a.u, w, h, ix + nsx, iy + nsy) - 2.0f * cu + ds_at<T>(a.u, w, h, ix - nsx, iy - nsy);
-    const float curv = ds_at<T>(a.v, w, h, ix + nsx, iy + nsy) - 2.0f * cv + ds_at<T>(a.v, w, h, ix - nsx, iy - nsy);
+    const float cu = src.u(ix, iy), cv = src.v(ix, iy);
+    const float curu = src.u(ix + nsx, iy + nsy) - 2.0f * cu + src.u(ix - nsx, iy - nsy);
+    const float curv = src.v(ix + nsx, iy + nsy) - 2.0f * cv + src.v(ix - nsx, iy - nsy);
     const float curE = curu * curu + curv * curv;
     const float sharp = curE / (curE + cen + 1.0e-12f);
     const float luma_g = trace / (trace + 3.0e-5f);
@@ -3219,21 +3269,22 @@ struct DsEnv {
     bool iso;
 };
 
-// spatialWeight (downsample.cpp:402-411)
+// spatialWeight (downsample.cpp:402-411).  Both denominators are 1 plus
+// non-negative terms and stay below 200 for windows of radius <= 4, where
+// rcp_nr equals the division 1.0f / y exactly.
 __device__ __forceinline__ float ds_spatial(const DsEnv &e, float dx, float dy) {
     const float r2 = dx * dx + dy * dy;
     if (e.iso)
-        return 1.0f / (1.0f + 0.42f * r2);
+        return rcp_nr(1.0f + 0.42f * r2);
     const float nd = dx * e.nx + dy * e.ny;
     const float td = -dx * e.ny + dy * e.nx;
-    return 1.0f / (1.0f + (0.9f + e.aniso) * nd * nd + 0.16f * td * td);
+    return rcp_nr(1.0f + (0.9f + e.aniso) * nd * nd + 0.16f * td * td);
 }
 
 // accumulateGuided (downsample.cpp:413-458) over N envelopes
-template <typename T, int N>
-__device__ void ds_accumulate(const DsArgs &a, float cx, float cy, const DsEnv (&env)[N], float (&aw)[N],
+template <typename S, int N>
+__device__ void ds_accumulate(const S &src, float cx, float cy, const DsEnv (&env)[N], float (&aw)[N],
                               float (&au)[N], float (&av)[N]) {
-    const int w = a.w, h = a.h;
     int maxr = 0;
     _Pragma("unroll") for (int i = 0; i < N; ++i) {
         aw[i] = au[i] = av[i] = 0.0f;
@@ -3243,16 +3294,16 @@ __device__ void ds_accumulate(const DsArgs &a, float cx, float cy, const DsEnv (
     const int y0 = (int)ceilf(cy - (float)maxr), y1 = (int)floorf(cy + (float)maxr);
     const int cx0 = (int)floorf(cx), cy0 = (int)floorf(cy);
     const float fx = cx - (float)cx0, fy = cy - (float)cy0;
-    const float ya = ds_at<T>(a.y, w, h, cx0, cy0), yb = ds_at<T>(a.y, w, h, cx0 + 1, cy0);
-    const float yc = ds_at<T>(a.y, w, h, cx0, cy0 + 1), yd = ds_at<T>(a.y, w, h, cx0 + 1, cy0 + 1);
+    const float ya = src.y(cx0, cy0), yb = src.y(cx0 + 1, cy0);
+    const float yc = src.y(cx0, cy0 + 1), yd = src.y(cx0 + 1, cy0 + 1);
     const float cl = ya + (yb - ya) * fx + (yc - ya) * fy + (ya - yb - yc + yd) * fx * fy;
     const float rs2 = 2.25e-4f;
     for (int sy = y0; sy <= y1; ++sy)
         for (int sx = x0; sx <= x1; ++sx) {
             const float dx = (float)sx - cx, dy = (float)sy - cy;
-            const float delta = ds_at<T>(a.y, w, h, sx, sy) - cl;
+            const float delta = src.y(sx, sy) - cl;
             const float rw = rs2 / (rs2 + delta * delta);
-            const float su = ds_at<T>(a.u, w, h, sx, sy), sv = ds_at<T>(a.v, w, h, sx, sy);
+            const float su = src.u(sx, sy), sv = src.v(sx, sy);
             _Pragma("unroll") for (int i = 0; i < N; ++i) {
                 if (fabsf(dx) > (float)env[i].radius || fabsf(dy) > (float)env[i].radius)
                     continue;
@@ -3268,6 +3319,21 @@ __device__ __forceinline__ float ds_norm(float sum, float wgt, float fallback) {
     return wgt > 1e-12f ? sum / wgt : fallback;
 }
 
+// Stage the block's source tile (clamped, converted) in LDS.
+template <typename T>
+__device__ DsTile ds_load_tile(const DsArgs &a, float *ty, float *tu, float *tv) {
+    const int ox = (int)blockIdx.x * 2 * kDsTW - kDsHalo, oy = (int)blockIdx.y * 2 * kDsTH - kDsHalo;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < kDsSW * kDsSH; e += kBlock) {
+        const int r = e / kDsSW, c = e - r * kDsSW;
+        ty[e] = ds_at<T>(a.y, a.w, a.h, ox + c, oy + r);
+        tu[e] = ds_at<T>(a.u, a.w, a.h, ox + c, oy + r);
+        tv[e] = ds_at<T>(a.v, a.w, a.h, ox + c, oy + r);
+    }
+    __syncthreads();
+    return DsTile{ty, tu, tv, ox, oy};
+}
+
 // writeRow (downsample.cpp:117-126)
 template <typename T> __device__ __forceinline__ void ds_store(const RawOut &P, int x, int y, float v) {
     T *row = reinterpret_cast<T *>(static_cast<char *>(P.data) + P.stride * (ptrdiff_t)y);
@@ -3281,15 +3347,20 @@ template <typename T> __device__ __forceinline__ void ds_store(const RawOut &P, 
 // (processFrame, downsample.cpp:686-712, with makeGuidedPixel,
 // downsample.cpp:466-507); one thread per chroma sample
 template <typename T> __global__ __launch_bounds__(kBlock) void k_ds_output(DsArgs a) {
-    const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
-    if (x >= a.cw)
+    __shared__ float ty[kDsSW * kDsSH], tu[kDsSW * kDsSH], tv[kDsSW * kDsSH];
+    const int x = blockIdx.x * kDsTW + (int)(threadIdx.x % kDsTW), y = blockIdx.y * kDsTH + (int)(threadIdx.x / kDsTW);
+    const bool guided = a.strength != 0.0f;
+    DsTile src{};
+    if (guided)
+        src = ds_load_tile<T>(a, ty, tu, tv);
+    if (x >= a.cw || y >= a.ch)
         return;
     const size_t i = (size_t)y * a.cw + x;
     const float bu = a.base_u[i], bv = a.base_v[i];
     float ru = bu, rv = bv;
-    if (a.strength != 0.0f) {
+    if (guided) {
         const float cx = 2.0f * (float)x + 0.5f + a.shift_x, cy = 2.0f * (float)y + 0.5f + a.shift_y;
-        const DsStats st = ds_classify<T>(a, cx, cy, a.quality);
+        const DsStats st = ds_classify(src, cx, cy, a.quality);
         float gu = bu, gv = bv, gate = st.gate;
         if (gate <= 1.0e-4f) {
             gate = 0.0f;
@@ -3297,14 +3368,14 @@ template <typename T> __global__ __launch_bounds__(kBlock) void k_ds_output(DsAr
             const float2 n = ds_dir(st.primary);
             const DsEnv env[1] = {{n.x, n.y, 2, 1.8f, st.isotropic}};
             float aw[1], au[1], av[1];
-            ds_accumulate<T, 1>(a, cx, cy, env, aw, au, av);
+            ds_accumulate<DsTile, 1>(src, cx, cy, env, aw, au, av);
             gu = ds_norm(au[0], aw[0], bu);
             gv = ds_norm(av[0], aw[0], bv);
         } else {
             const float2 n0 = ds_dir(st.primary), n1 = ds_dir(st.secondary);
             const DsEnv env[2] = {{n0.x, n0.y, 3, 2.2f, st.isotropic}, {n1.x, n1.y, 3, 2.2f, st.isotropic}};
             float aw[2], au[2], av[2];
-            ds_accumulate<T, 2>(a, cx, cy, env, aw, au, av);
+            ds_accumulate<DsTile, 2>(src, cx, cy, env, aw, au, av);
             const float u0 = ds_norm(au[0], aw[0], bu), v0 = ds_norm(av[0], aw[0], bv);
             const float u1 = ds_norm(au[1], aw[1], bu), v1 = ds_norm(av[1], aw[1], bv);
             gu = u0 + st.mix * (u1 - u0);
@@ -3320,13 +3391,15 @@ template <typename T> __global__ __launch_bounds__(kBlock) void k_ds_output(DsAr
 // quality 2: gate and the four candidates of every chroma sample
 // (generateCandidateRow, downsample.cpp:526-568)
 template <typename T> __global__ __launch_bounds__(kBlock) void k_ds_candidates(DsArgs a) {
-    const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
-    if (x >= a.cw)
+    __shared__ float ty[kDsSW * kDsSH], tu[kDsSW * kDsSH], tv[kDsSW * kDsSH];
+    const int x = blockIdx.x * kDsTW + (int)(threadIdx.x % kDsTW), y = blockIdx.y * kDsTH + (int)(threadIdx.x / kDsTW);
+    const DsTile src = ds_load_tile<T>(a, ty, tu, tv);
+    if (x >= a.cw || y >= a.ch)
         return;
     const size_t nc = (size_t)a.cw * a.ch, i = (size_t)y * a.cw + x;
     const float bu = a.base_u[i], bv = a.base_v[i];
     const float cx = 2.0f * (float)x + 0.5f + a.shift_x, cy = 2.0f * (float)y + 0.5f + a.shift_y;
-    const DsStats st = ds_classify<T>(a, cx, cy, 2);
+    const DsStats st = ds_classify(src, cx, cy, 2);
     const float gate = st.gate > 1.0e-4f ? st.gate : 0.0f;
     a.gate[i] = gate;
     if (gate == 0.0f) {
@@ -3341,7 +3414,7 @@ template <typename T> __global__ __launch_bounds__(kBlock) void k_ds_candidates(
                           {n0.x, n0.y, 4, 2.7f, st.isotropic}, {n1.x, n1.y, 4, 2.7f, st.isotropic},
                           {n0.x, n0.y, 4, 0.0f, true}};
     float aw[5], au[5], av[5];
-    ds_accumulate<T, 5>(a, cx, cy, env, aw, au, av);
+    ds_accumulate<DsTile, 5>(src, cx, cy, env, aw, au, av);
     const float u0 = ds_norm(au[0], aw[0], bu), v0 = ds_norm(av[0], aw[0], bv);
     const float u1 = ds_norm(au[1], aw[1], bu), v1 = ds_norm(av[1], aw[1], bv);
     a.cand_u[i] = u0 + st.mix * (u1 - u0);
@@ -3909,13 +3982,14 @@ hipError_t launch_ds_base(const DsArgs &a, hipStream_t s) {
 }
 
 hipError_t launch_ds_output(const DsArgs &a, hipStream_t s) {
-    const dim3 g((unsigned)((a.cw + kBlock - 1) / kBlock), (unsigned)a.ch);
+    static_assert(kDsTW * kDsTH == kBlock, "one thread per output of the tile");
+    const dim3 g((unsigned)((a.cw + kDsTW - 1) / kDsTW), (unsigned)((a.ch + kDsTH - 1) / kDsTH));
     LGCR_DS_TYPED(k_ds_output, g)
     return hipGetLastError();
 }
 
 hipError_t launch_ds_candidates(const DsArgs &a, hipStream_t s) {
-    const dim3 g((unsigned)((a.cw + kBlock - 1) / kBlock), (unsigned)a.ch);
+    const dim3 g((unsigned)((a.cw + kDsTW - 1) / kDsTW), (unsigned)((a.ch + kDsTH - 1) / kDsTH));
     LGCR_DS_TYPED(k_ds_candidates, g)
     return hipGetLastError();
 }

@@ -2016,6 +2016,8 @@ lgcr_status lgcr_recon_frame(lgcr_clip *c, const lgcr_frame *in, double shift_x,
                              lgcr_frame *out, lgcr_stats *stats) {
     if (!c || !in || !out)
         return fail(LGCR_INVALID_ARGUMENT, "lgcr_recon_frame: NULL argument");
+    if (c->temporal || c->sharpen || c->downsample)
+        return fail(LGCR_INVALID_CONTEXT, "lgcr_recon_frame: not a Recon clip");
     if (in->memory != out->memory)
         return fail(LGCR_INVALID_ARGUMENT, "lgcr_recon_frame: input and output must be in the same memory");
     for (int i = 0; i < 3; ++i)
@@ -2143,6 +2145,8 @@ lgcr_status lgcr_recon_frame(lgcr_clip *c, const lgcr_frame *in, double shift_x,
 lgcr_status lgcr_recon_batch(lgcr_clip *c, int n, const lgcr_frame *in, lgcr_frame *out) {
     if (!c || n < 0 || (n && (!in || !out)))
         return fail(LGCR_INVALID_ARGUMENT, "lgcr_recon_batch: bad arguments");
+    if (c->temporal || c->sharpen || c->downsample)
+        return fail(LGCR_INVALID_CONTEXT, "lgcr_recon_batch: not a Recon clip");
     for (int k = 0; k < n; ++k)
         if (in[k].memory != LGCR_MEMORY_DEVICE || out[k].memory != LGCR_MEMORY_DEVICE)
             return fail(LGCR_INVALID_ARGUMENT, "lgcr_recon_batch: frames must be device resident");
@@ -2268,7 +2272,7 @@ lgcr_status lgcr_recon_submit(lgcr_clip *c, const lgcr_frame *in, double shift_x
     if (!c || !in || !out || !ticket)
         return fail(LGCR_INVALID_ARGUMENT, "lgcr_recon_submit: NULL argument");
     *ticket = nullptr;
-    if (c->temporal || c->sharpen)
+    if (c->temporal || c->sharpen || c->downsample)
         return fail(LGCR_INVALID_CONTEXT, "lgcr_recon_submit: not a Recon clip");
     if (in->memory != LGCR_MEMORY_DEVICE || out->memory != LGCR_MEMORY_DEVICE)
         return fail(LGCR_INVALID_ARGUMENT, "lgcr_recon_submit: frames must be device resident");
@@ -2988,6 +2992,8 @@ lgcr_status lgcr_clip_stage_times(lgcr_clip *c, double *us, uint64_t *launches) 
 lgcr_status lgcr_clip_fetch(lgcr_clip *c, int what, void *dst, size_t bytes, size_t *needed) {
     if (!c)
         return fail(LGCR_INVALID_ARGUMENT, "lgcr_clip_fetch: NULL clip");
+    if (c->sharpen || c->downsample)
+        return fail(LGCR_UNSUPPORTED, "lgcr_clip_fetch: Recon and TRecon clips only");
     int slot = -1;
     {
         // workspace 0 only; wait until it is idle
