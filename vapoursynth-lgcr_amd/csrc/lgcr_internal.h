@@ -157,13 +157,17 @@ struct DsArgs {
     int type;          // RAW_U8, RAW_U16 or RAW_F32
     int w, h, cw, ch;  // source (4:4:4) and output chroma sizes
     WeightsDev xw, yw; // baseline taps (start may be negative: loads clamp)
-    float *tmp_u, *tmp_v;    // horizontal pass, cw x h
     float *base_u, *base_v;  // baseline, cw x ch
     float *gate, *cand_u, *cand_v;  // quality 2: gate (cw x ch), 4 candidate planes each
     int quality;
     float strength, shift_x, shift_y;
     RawOut out_u, out_v;     // packed U, V (src unused)
 };
+
+// baseline kernel: kDsBaseTileW x kDsBaseTileH outputs per block; the
+// tile's source columns (start of its first output to start of its last +
+// support) and source rows must span at most kDsBaseSegW and kDsBaseSegH
+constexpr int kDsBaseTileW = 32, kDsBaseTileH = 16, kDsBaseSegW = 80, kDsBaseSegH = 48;
 
 // Kernel launchers (lgcr_kernels.hip).  All are asynchronous on `s`.
 hipError_t launch_to_float(const RawPlane *planes, int count, int type, hipStream_t s);

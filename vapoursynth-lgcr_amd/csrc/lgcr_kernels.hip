@@ -3072,66 +3072,105 @@ __device__ __forceinline__ float ds_at(const DsPlane &p, int w, int h, int x, in
     return (float)row[x] * p.scale + p.offset;
 }
 
-// horizontal baseline pass (BaselineRing::ensureRow, downsample.cpp:200-229,
-// scalar branch): one thread per (output column, source row).  The block's
-// source segment is converted once into LDS (the taps of neighbouring
-// columns overlap six-fold at 2x decimation); starts are non-decreasing in
-// the column, so the segment is [start of the first column, start of the
-// last + sup).
-constexpr int kDsSeg = 2 * kBlock + 32;
+// Downsample baseline (BaselineRing, downsample.cpp:129-261: ensureRow's
+// horizontal taps, then makeRow's vertical taps over clamped rows) in one
+// kernel, one kDsBaseTileW x kDsBaseTileH block of outputs per block; the
+// reference's row ring becomes the tile's row buffer in LDS.  The horizontal pass runs over the tile's source rows (clamped
+// rows of its vertical taps) in chunks of eight: each chunk's source
+// segment is converted into LDS, the next chunk's raw samples are requested
+// (requesting every chunk up front measured slower), then every thread makes
+// one horizontal sample of both planes into the row buffer; the vertical
+// pass reads the row buffer.  SX / SY are the tap counts (12 for spline36
+// and lanczos3 at 2x, 4 or 5 binomial), each sum runs over the taps in order
+// from 0, and samples convert as SourcePlane::at (downsample.cpp:101-106).
+constexpr int kDbChunk = kBlock / kDsBaseTileW;
+constexpr int kDbLoads = (kDbChunk * kDsBaseSegW + kBlock - 1) / kBlock;
+static_assert(kDbChunk * kDsBaseTileW == kBlock && kDsBaseTileW * kDsBaseTileH == 2 * kBlock,
+              "one horizontal sample per thread per chunk, two outputs per thread");
 
-template <typename T> __global__ __launch_bounds__(kBlock) void k_ds_hpass(DsArgs a) {
-    __shared__ float seg_u[kDsSeg], seg_v[kDsSeg];
-    const int x0 = blockIdx.x * kBlock, sy = blockIdx.y, sup = a.xw.sup;
-    const int x = x0 + threadIdx.x;
-    const int s0 = a.xw.start[x0], s1 = a.xw.start[min(x0 + kBlock, a.cw) - 1] + sup;
-    if (s1 - s0 <= kDsSeg) {
-        for (int i = threadIdx.x; i < s1 - s0; i += kBlock) {
-            seg_u[i] = ds_at<T>(a.u, a.w, a.h, s0 + i, sy);
-            seg_v[i] = ds_at<T>(a.v, a.w, a.h, s0 + i, sy);
-        }
-        __syncthreads();
-        if (x >= a.cw)
-            return;
-        const float *wt = a.xw.w + (size_t)x * sup;
-        const int o = a.xw.start[x] - s0;
-        float su = 0.0f, sv = 0.0f;
-        for (int t = 0; t < sup; ++t) {
-            su += wt[t] * seg_u[o + t];
-            sv += wt[t] * seg_v[o + t];
-        }
-        a.tmp_u[(size_t)sy * a.cw + x] = su;
-        a.tmp_v[(size_t)sy * a.cw + x] = sv;
-        return;
-    }
-    // wider tables (not produced by the 2x builders): taps read the planes
-    if (x >= a.cw)
-        return;
-    const float *wt = a.xw.w + (size_t)x * sup;
-    const int st = a.xw.start[x];
-    float su = 0.0f, sv = 0.0f;
-    for (int t = 0; t < sup; ++t) {
-        su += wt[t] * ds_at<T>(a.u, a.w, a.h, st + t, sy);
-        sv += wt[t] * ds_at<T>(a.v, a.w, a.h, st + t, sy);
-    }
-    a.tmp_u[(size_t)sy * a.cw + x] = su;
-    a.tmp_v[(size_t)sy * a.cw + x] = sv;
+template <typename T>
+__device__ __forceinline__ T ds_raw(const DsPlane &p, int w, int x, int y) {
+    x = clampi(x, 0, w - 1);
+    return reinterpret_cast<const T *>(static_cast<const char *>(p.data) + p.stride * (ptrdiff_t)y)[x];
 }
 
-// vertical baseline pass (BaselineRing::makeRow, downsample.cpp:139-170)
-__global__ __launch_bounds__(kBlock) void k_ds_vpass(DsArgs a) {
-    const int x = blockIdx.x * kBlock + threadIdx.x, oy = blockIdx.y;
-    if (x >= a.cw)
-        return;
-    const float *wy = a.yw.w + (size_t)oy * a.yw.sup;
-    float bu = 0.0f, bv = 0.0f;
-    for (int t = 0; t < a.yw.sup; ++t) {
-        const int sy = clampi(a.yw.start[oy] + t, 0, a.h - 1);
-        bu += wy[t] * a.tmp_u[(size_t)sy * a.cw + x];
-        bv += wy[t] * a.tmp_v[(size_t)sy * a.cw + x];
+template <typename T, int SX, int SY> __global__ __launch_bounds__(kBlock) void k_ds_base(DsArgs a) {
+    __shared__ float seg_u[kDbChunk][kDsBaseSegW], seg_v[kDbChunk][kDsBaseSegW];
+    __shared__ float row_u[kDsBaseSegH][kDsBaseTileW], row_v[kDsBaseSegH][kDsBaseTileW];
+    const int x0 = blockIdx.x * kDsBaseTileW, oy0 = blockIdx.y * kDsBaseTileH;
+    const int xl = min(x0 + kDsBaseTileW, a.cw) - 1, oyl = min(oy0 + kDsBaseTileH, a.ch) - 1;
+    const int sx0 = a.xw.start[x0], nsx = a.xw.start[xl] + SX - sx0;
+    const int r0 = clampi(a.yw.start[oy0], 0, a.h - 1);
+    const int nr = clampi(a.yw.start[oyl] + SY - 1, 0, a.h - 1) - r0 + 1;
+    const int tx = threadIdx.x % kDsBaseTileW, tr = threadIdx.x / kDsBaseTileW;
+    const int x = min(x0 + tx, xl);
+    const float *wt = a.xw.w + (size_t)x * SX;
+    const int o = a.xw.start[x] - sx0;
+    // this thread's slots of a chunk: sample i = threadIdx.x + j * kBlock
+    // is row i / nsx, column i % nsx (row kDbChunk: unused slot)
+    int srow[kDbLoads], scol[kDbLoads];
+#pragma unroll
+    for (int j = 0; j < kDbLoads; ++j) {
+        const int i = threadIdx.x + j * kBlock;
+        srow[j] = i < kDbChunk * nsx ? i / nsx : kDbChunk;
+        scol[j] = i - srow[j] * nsx;
     }
-    a.base_u[(size_t)oy * a.cw + x] = bu;
-    a.base_v[(size_t)oy * a.cw + x] = bv;
+    T ru[kDbLoads], rv[kDbLoads];
+    auto fetch = [&](int c0) {
+#pragma unroll
+        for (int j = 0; j < kDbLoads; ++j)
+            if (srow[j] < kDbChunk && c0 + srow[j] < nr) {
+                ru[j] = ds_raw<T>(a.u, a.w, sx0 + scol[j], r0 + c0 + srow[j]);
+                rv[j] = ds_raw<T>(a.v, a.w, sx0 + scol[j], r0 + c0 + srow[j]);
+            }
+    };
+    fetch(0);
+    float wx[SX];
+#pragma unroll
+    for (int t = 0; t < SX; ++t)
+        wx[t] = wt[t];
+    for (int c0 = 0; c0 < nr; c0 += kDbChunk) {
+        const int rows = min(kDbChunk, nr - c0);
+#pragma unroll
+        for (int j = 0; j < kDbLoads; ++j)
+            if (srow[j] < rows) {
+                seg_u[srow[j]][scol[j]] = (float)ru[j] * a.u.scale + a.u.offset;
+                seg_v[srow[j]][scol[j]] = (float)rv[j] * a.v.scale + a.v.offset;
+            }
+        __syncthreads();
+        if (c0 + kDbChunk < nr)
+            fetch(c0 + kDbChunk);
+        if (tr < rows) {
+            float su = 0.0f, sv = 0.0f;
+#pragma unroll
+            for (int t = 0; t < SX; ++t) {
+                su += wx[t] * seg_u[tr][o + t];
+                sv += wx[t] * seg_v[tr][o + t];
+            }
+            row_u[c0 + tr][tx] = su;
+            row_v[c0 + tr][tx] = sv;
+        }
+        __syncthreads();
+    }
+    const int ox = x0 + tx;
+    if (ox >= a.cw)
+        return;
+    for (int k = 0; k < 2; ++k) {
+        const int oy = oy0 + tr + k * kDbChunk;
+        if (oy > oyl)
+            break;
+        const float *wy = a.yw.w + (size_t)oy * SY;
+        const int sy = a.yw.start[oy] - r0;
+        float bu = 0.0f, bv = 0.0f;
+#pragma unroll
+        for (int t = 0; t < SY; ++t) {
+            const int r = clampi(sy + t, -r0, a.h - 1 - r0);
+            bu += wy[t] * row_u[r][tx];
+            bv += wy[t] * row_v[r][tx];
+        }
+        a.base_u[(size_t)oy * a.cw + ox] = bu;
+        a.base_v[(size_t)oy * a.cw + ox] = bv;
+    }
 }
 
 __device__ __forceinline__ float2 ds_dir(int d) {
@@ -3165,14 +3204,20 @@ template <typename T> struct DsGlobal {
 // a 32x8 block of chroma outputs reads source columns [2*x0-6, 2*x0+70) and
 // rows [2*y0-6, 2*y0+22): window radius <= 4 around centres within 0.5 of
 // 2*x+0.5, plus the +-1 neighbours of the direction test and bilinear centre
-constexpr int kDsTW = 32, kDsTH = 8, kDsHalo = 6;
-constexpr int kDsSW = 2 * kDsTW + 2 * kDsHalo, kDsSH = 2 * kDsTH + 2 * kDsHalo;
-struct DsTile {
+// Guided Downsample tiles: 32 x 8 chroma samples per block, their 64 x 16
+// source samples and a halo of H on every side, converted once into LDS.
+// The window of radius r around cx in {2x, 2x + 0.5}, cy in {2y, 2y + 0.5,
+// 2y + 1} spans source columns 2x - r .. 2x + r and rows 2y - r .. 2y + 1 + r,
+// and the structure tensor reads +-1 columns and +-2 rows around
+// (2x or 2x + 1, 2y or 2y + 1), so H = max(r, 2) covers every read.
+constexpr int kDsTW = 32, kDsTH = 8;
+template <int H> struct DsTile {
+    static constexpr int SW = 2 * kDsTW + 2 * H, SH = 2 * kDsTH + 2 * H, N = SW * SH;
     const float *ty, *tu, *tv;
     int ox, oy;
-    __device__ float y(int x, int r) const { return ty[(r - oy) * kDsSW + (x - ox)]; }
-    __device__ float u(int x, int r) const { return tu[(r - oy) * kDsSW + (x - ox)]; }
-    __device__ float v(int x, int r) const { return tv[(r - oy) * kDsSW + (x - ox)]; }
+    __device__ float y(int x, int r) const { return ty[(r - oy) * SW + (x - ox)]; }
+    __device__ float u(int x, int r) const { return tu[(r - oy) * SW + (x - ox)]; }
+    __device__ float v(int x, int r) const { return tv[(r - oy) * SW + (x - ox)]; }
 };
 
 // classify (downsample.cpp:281-385)
@@ -3320,18 +3365,19 @@ __device__ __forceinline__ float ds_norm(float sum, float wgt, float fallback) {
 }
 
 // Stage the block's source tile (clamped, converted) in LDS.
-template <typename T>
-__device__ DsTile ds_load_tile(const DsArgs &a, float *ty, float *tu, float *tv) {
-    const int ox = (int)blockIdx.x * 2 * kDsTW - kDsHalo, oy = (int)blockIdx.y * 2 * kDsTH - kDsHalo;
+template <typename T, int H>
+__device__ DsTile<H> ds_load_tile(const DsArgs &a, float *ty, float *tu, float *tv) {
+    constexpr int SW = DsTile<H>::SW;
+    const int ox = (int)blockIdx.x * 2 * kDsTW - H, oy = (int)blockIdx.y * 2 * kDsTH - H;
     const int tid = threadIdx.x;
-    for (int e = tid; e < kDsSW * kDsSH; e += kBlock) {
-        const int r = e / kDsSW, c = e - r * kDsSW;
+    for (int e = tid; e < DsTile<H>::N; e += kBlock) {
+        const int r = e / SW, c = e - r * SW;
         ty[e] = ds_at<T>(a.y, a.w, a.h, ox + c, oy + r);
         tu[e] = ds_at<T>(a.u, a.w, a.h, ox + c, oy + r);
         tv[e] = ds_at<T>(a.v, a.w, a.h, ox + c, oy + r);
     }
     __syncthreads();
-    return DsTile{ty, tu, tv, ox, oy};
+    return DsTile<H>{ty, tu, tv, ox, oy};
 }
 
 // writeRow (downsample.cpp:117-126)
@@ -3345,14 +3391,15 @@ template <typename T> __device__ __forceinline__ void ds_store(const RawOut &P, 
 
 // packed baseline (strength 0) or the guided result of quality 0/1
 // (processFrame, downsample.cpp:686-712, with makeGuidedPixel,
-// downsample.cpp:466-507); one thread per chroma sample
-template <typename T> __global__ __launch_bounds__(kBlock) void k_ds_output(DsArgs a) {
-    __shared__ float ty[kDsSW * kDsSH], tu[kDsSW * kDsSH], tv[kDsSW * kDsSH];
+// downsample.cpp:466-507); one thread per chroma sample.  H: 2 for
+// quality 0 (radius 2), 3 for quality 1 (radius 3).
+template <typename T, int H> __global__ __launch_bounds__(kBlock) void k_ds_output(DsArgs a) {
+    __shared__ float ty[DsTile<H>::N], tu[DsTile<H>::N], tv[DsTile<H>::N];
     const int x = blockIdx.x * kDsTW + (int)(threadIdx.x % kDsTW), y = blockIdx.y * kDsTH + (int)(threadIdx.x / kDsTW);
     const bool guided = a.strength != 0.0f;
-    DsTile src{};
+    DsTile<H> src{};
     if (guided)
-        src = ds_load_tile<T>(a, ty, tu, tv);
+        src = ds_load_tile<T, H>(a, ty, tu, tv);
     if (x >= a.cw || y >= a.ch)
         return;
     const size_t i = (size_t)y * a.cw + x;
@@ -3368,14 +3415,14 @@ template <typename T> __global__ __launch_bounds__(kBlock) void k_ds_output(DsAr
             const float2 n = ds_dir(st.primary);
             const DsEnv env[1] = {{n.x, n.y, 2, 1.8f, st.isotropic}};
             float aw[1], au[1], av[1];
-            ds_accumulate<DsTile, 1>(src, cx, cy, env, aw, au, av);
+            ds_accumulate<DsTile<H>, 1>(src, cx, cy, env, aw, au, av);
             gu = ds_norm(au[0], aw[0], bu);
             gv = ds_norm(av[0], aw[0], bv);
         } else {
             const float2 n0 = ds_dir(st.primary), n1 = ds_dir(st.secondary);
             const DsEnv env[2] = {{n0.x, n0.y, 3, 2.2f, st.isotropic}, {n1.x, n1.y, 3, 2.2f, st.isotropic}};
             float aw[2], au[2], av[2];
-            ds_accumulate<DsTile, 2>(src, cx, cy, env, aw, au, av);
+            ds_accumulate<DsTile<H>, 2>(src, cx, cy, env, aw, au, av);
             const float u0 = ds_norm(au[0], aw[0], bu), v0 = ds_norm(av[0], aw[0], bv);
             const float u1 = ds_norm(au[1], aw[1], bu), v1 = ds_norm(av[1], aw[1], bv);
             gu = u0 + st.mix * (u1 - u0);
@@ -3391,9 +3438,10 @@ template <typename T> __global__ __launch_bounds__(kBlock) void k_ds_output(DsAr
 // quality 2: gate and the four candidates of every chroma sample
 // (generateCandidateRow, downsample.cpp:526-568)
 template <typename T> __global__ __launch_bounds__(kBlock) void k_ds_candidates(DsArgs a) {
-    __shared__ float ty[kDsSW * kDsSH], tu[kDsSW * kDsSH], tv[kDsSW * kDsSH];
+    constexpr int H = 5;  // radius 4; one spare column and row measured faster than 4
+    __shared__ float ty[DsTile<H>::N], tu[DsTile<H>::N], tv[DsTile<H>::N];
     const int x = blockIdx.x * kDsTW + (int)(threadIdx.x % kDsTW), y = blockIdx.y * kDsTH + (int)(threadIdx.x / kDsTW);
-    const DsTile src = ds_load_tile<T>(a, ty, tu, tv);
+    const DsTile<H> src = ds_load_tile<T, H>(a, ty, tu, tv);
     if (x >= a.cw || y >= a.ch)
         return;
     const size_t nc = (size_t)a.cw * a.ch, i = (size_t)y * a.cw + x;
@@ -3414,7 +3462,7 @@ template <typename T> __global__ __launch_bounds__(kBlock) void k_ds_candidates(
                           {n0.x, n0.y, 4, 2.7f, st.isotropic}, {n1.x, n1.y, 4, 2.7f, st.isotropic},
                           {n0.x, n0.y, 4, 0.0f, true}};
     float aw[5], au[5], av[5];
-    ds_accumulate<DsTile, 5>(src, cx, cy, env, aw, au, av);
+    ds_accumulate<DsTile<H>, 5>(src, cx, cy, env, aw, au, av);
     const float u0 = ds_norm(au[0], aw[0], bu), v0 = ds_norm(av[0], aw[0], bv);
     const float u1 = ds_norm(au[1], aw[1], bu), v1 = ds_norm(av[1], aw[1], bv);
     a.cand_u[i] = u0 + st.mix * (u1 - u0);
@@ -3974,17 +4022,49 @@ hipError_t launch_collaborative(const float *guide, int gw, int gh, float *u, fl
     else                                                                                        \
         hipLaunchKernelGGL(K<float>, grid, dim3(kBlock), 0, s, a);
 
+template <int SX, int SY> void launch_ds_base_tiled(const DsArgs &a, dim3 g, hipStream_t s) {
+    if (a.type == RAW_U8)
+        hipLaunchKernelGGL((k_ds_base<uint8_t, SX, SY>), g, dim3(kBlock), 0, s, a);
+    else if (a.type == RAW_U16)
+        hipLaunchKernelGGL((k_ds_base<uint16_t, SX, SY>), g, dim3(kBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_ds_base<float, SX, SY>), g, dim3(kBlock), 0, s, a);
+}
+
 hipError_t launch_ds_base(const DsArgs &a, hipStream_t s) {
-    const unsigned gx = (unsigned)((a.cw + kBlock - 1) / kBlock);
-    LGCR_DS_TYPED(k_ds_hpass, dim3(gx, (unsigned)a.h))
-    hipLaunchKernelGGL(k_ds_vpass, dim3(gx, (unsigned)a.ch), dim3(kBlock), 0, s, a);
+    const dim3 g((unsigned)((a.cw + kDsBaseTileW - 1) / kDsBaseTileW), (unsigned)((a.ch + kDsBaseTileH - 1) / kDsBaseTileH));
+    const int sx = a.xw.sup, sy = a.yw.sup;
+    if (sx == 12 && sy == 12)
+        launch_ds_base_tiled<12, 12>(a, g, s);
+    else if (sx == 4 && sy == 4)
+        launch_ds_base_tiled<4, 4>(a, g, s);
+    else if (sx == 4 && sy == 5)
+        launch_ds_base_tiled<4, 5>(a, g, s);
+    else if (sx == 5 && sy == 4)
+        launch_ds_base_tiled<5, 4>(a, g, s);
+    else if (sx == 5 && sy == 5)
+        launch_ds_base_tiled<5, 5>(a, g, s);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
+}
+
+template <int H> void launch_ds_output_h(const DsArgs &a, dim3 g, hipStream_t s) {
+    if (a.type == RAW_U8)
+        hipLaunchKernelGGL((k_ds_output<uint8_t, H>), g, dim3(kBlock), 0, s, a);
+    else if (a.type == RAW_U16)
+        hipLaunchKernelGGL((k_ds_output<uint16_t, H>), g, dim3(kBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_ds_output<float, H>), g, dim3(kBlock), 0, s, a);
 }
 
 hipError_t launch_ds_output(const DsArgs &a, hipStream_t s) {
     static_assert(kDsTW * kDsTH == kBlock, "one thread per output of the tile");
     const dim3 g((unsigned)((a.cw + kDsTW - 1) / kDsTW), (unsigned)((a.ch + kDsTH - 1) / kDsTH));
-    LGCR_DS_TYPED(k_ds_output, g)
+    if (a.strength != 0.0f && a.quality == 1)
+        launch_ds_output_h<3>(a, g, s);
+    else
+        launch_ds_output_h<2>(a, g, s);
     return hipGetLastError();
 }
 

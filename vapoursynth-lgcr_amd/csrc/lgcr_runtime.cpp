@@ -543,7 +543,7 @@ struct Workspace {
     float *luma_tmp = nullptr, *luma_out = nullptr;
     // Downsample: horizontal pass (2 x cw x h), baseline (2 x cw x ch),
     // quality-2 gate and candidates (9 x cw x ch)
-    float *ds_tmp = nullptr, *ds_base = nullptr, *ds_cand = nullptr;
+    float *ds_base = nullptr, *ds_cand = nullptr;
     uint64_t *seed = nullptr, *mask = nullptr, *mtmp = nullptr;  // bit planes, rows of w64 words
     uint8_t *cmask = nullptr;
     uint32_t *row_off = nullptr, *row_cnt = nullptr, *indices = nullptr, *counters = nullptr;
@@ -852,7 +852,6 @@ void layout(const lgcr_clip *c, Workspace *w, Carver &cv) {
         w->raw_out[0] = cv.take<uint8_t>(ns * c->bps);
         w->raw_out[1] = cv.take<uint8_t>(nc * c->bps);
         w->raw_out[2] = cv.take<uint8_t>(nc * c->bps);
-        w->ds_tmp = cv.take<float>(2 * size_t(c->cw) * c->sh);
         w->ds_base = cv.take<float>(2 * nc);
         if (c->dsp.quality == 2 && c->ds_strength != 0.0f)
             w->ds_cand = cv.take<float>(9 * nc);
@@ -2627,8 +2626,6 @@ lgcr_status enqueue_downsample(lgcr_clip *c, Workspace *w, const void *const in[
     a.ch = c->ch;
     a.xw = c->ds_xw;
     a.yw = c->ds_yw;
-    a.tmp_u = w->ds_tmp;
-    a.tmp_v = w->ds_tmp + size_t(c->cw) * c->sh;
     a.base_u = w->ds_base;
     a.base_v = w->ds_base + nc;
     a.gate = w->ds_cand;
@@ -2723,6 +2720,26 @@ lgcr_status lgcr_downsample_create(lgcr_device *dev, const lgcr_downsample_param
                           reinterpret_cast<const float *>(base + o1)};
     c->ds_yw = WeightsDev{yw.n, yw.sup, reinterpret_cast<const int *>(base + o2),
                           reinterpret_cast<const float *>(base + o3)};
+    // the baseline kernel holds a tile's taps in LDS: every tile of outputs
+    // must span at most the LDS tile (starts are non-decreasing, so a tile
+    // spans first start .. last start + support), with 12, 4 or 5 taps.
+    // The 2x tables of even sizes always do.
+    auto fits = [](const HostWeights &t, int tile, int cap) {
+        if (t.n <= 0 || (t.sup != 12 && t.sup != 4 && t.sup != 5))
+            return false;
+        for (int i = 1; i < t.n; ++i)
+            if (t.start[i] < t.start[i - 1])
+                return false;
+        for (int i0 = 0; i0 < t.n; i0 += tile)
+            if (t.start[std::min(i0 + tile, t.n) - 1] + t.sup - t.start[i0] > cap)
+                return false;
+        return true;
+    };
+    if (!fits(xw, kDsBaseTileW, kDsBaseSegW) || !fits(yw, kDsBaseTileH, kDsBaseSegH) ||
+        (xw.sup == 12) != (yw.sup == 12)) {
+        lgcr_clip_destroy(c.release());
+        return fail(LGCR_UNSUPPORTED, "Downsample: baseline taps do not fit the tile kernel");
+    }
     std::unique_ptr<Workspace> w;
     const lgcr_status st = create_workspace(c.get(), w);
     if (st != LGCR_OK) {
