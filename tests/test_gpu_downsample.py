@@ -62,6 +62,39 @@ def test_downsample_matrix(device, case):
     compare("-".join(map(str, case)), ref, got)
 
 
+def dense_edges(w, h, bits, kind):
+    """Concentric rings or diagonal bars, 3-4 samples wide, with chroma that
+    follows the luma: most chroma samples pass the gate (up to 215 of a
+    32x8 block), so the guided kernels' per-block lists run several rounds."""
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    if kind == "rings":
+        s = np.floor(np.sqrt((xx - w / 2) ** 2 + (yy - h / 2) ** 2) / 3) % 2
+    else:
+        s = np.floor((xx + 0.6 * yy) / 4) % 2
+    planes = (0.2 + 0.6 * s, 0.5 + 0.35 * (s - 0.5) * np.cos(xx / 7), 0.5 - 0.3 * (s - 0.5))
+    if bits == 32:
+        return [np.ascontiguousarray(p, dtype=np.float32) for p in planes]
+    m = (1 << bits) - 1
+    return [np.clip(np.round(p * m), 0, m).astype(DT[bits]) for p in planes]
+
+
+DENSE = [(kind, bits, q, w, h) for kind in ("rings", "diag") for bits in (10, 32) for q in (0, 1, 2)
+         for (w, h) in ((128, 64), (130, 66))]
+
+
+@pytest.mark.parametrize("case", DENSE, ids=["-".join(map(str, c)) for c in DENSE])
+def test_downsample_dense_gates(device, case):
+    kind, bits, quality, w, h = case
+    planes = dense_edges(w, h, bits, kind)
+    ref = oracle.downsample_frame(oracle.downsample_params(quality=quality), *planes,
+                                  bits=10 if bits == 10 else None)
+    base = oracle.downsample_frame(oracle.downsample_params(quality=quality, strength=0.0), *planes,
+                                   bits=10 if bits == 10 else None)
+    assert np.mean(ref[1] != base[1]) > 0.3  # the gate passes for most samples
+    got = run_gpu(device, planes, bits, quality=quality)
+    compare("dense-" + "-".join(map(str, case)), ref, got)
+
+
 @pytest.mark.parametrize("quality", [0, 2])
 def test_downsample_1080p(device, quality):
     """A 1920x1080 10-bit 4:4:4 frame."""

@@ -3389,87 +3389,210 @@ template <typename T> __device__ __forceinline__ void ds_store(const RawOut &P, 
         row[x] = pack_sample<T>(v, P);
 }
 
+// Gated samples of a guided tile, compacted in LDS: a sample whose gate
+// passes is appended (tile index, directions, isotropy, mix, gate) and the
+// block's threads then run the window sums over the list, so only as many
+// lanes as there are gated samples walk the windows.  Every sample's
+// arithmetic is unchanged; only the thread that computes it differs.
+// Envelopes are independent sums, so with few gated samples the list runs
+// one lane per (sample, envelope): envelope k sums over its own window in
+// the same row-major order as the all-envelope loop (a tap outside an
+// envelope's radius is skipped there), and the sample's lane finishes from
+// the per-envelope sums in LDS.
+template <bool GATE> struct DsGated {
+    int n;
+    int key[kBlock];      // tile index | primary << 8 | secondary << 11 | isotropic << 14
+    float mix[kBlock];
+    float gate[GATE ? kBlock : 1];
+};
+
+__device__ __forceinline__ int ds_gated_key(int t, const DsStats &st) {
+    return t | st.primary << 8 | st.secondary << 11 | (st.isotropic ? 1 << 14 : 0);
+}
+
+__device__ __forceinline__ void ds_gated_decode(int key, int &x, int &y, int &primary, int &secondary, bool &iso) {
+    const int t = key & 0xff;
+    x = blockIdx.x * kDsTW + t % kDsTW;
+    y = blockIdx.y * kDsTH + t / kDsTW;
+    primary = (key >> 8) & 7;
+    secondary = (key >> 11) & 7;
+    iso = (key >> 14) & 1;
+}
+
 // packed baseline (strength 0) or the guided result of quality 0/1
 // (processFrame, downsample.cpp:686-712, with makeGuidedPixel,
-// downsample.cpp:466-507); one thread per chroma sample.  H: 2 for
+// downsample.cpp:466-507).  Every thread classifies one chroma sample and
+// packs the ungated ones; the gated ones go through the list.  H: 2 for
 // quality 0 (radius 2), 3 for quality 1 (radius 3).
 template <typename T, int H> __global__ __launch_bounds__(kBlock) void k_ds_output(DsArgs a) {
     __shared__ float ty[DsTile<H>::N], tu[DsTile<H>::N], tv[DsTile<H>::N];
-    const int x = blockIdx.x * kDsTW + (int)(threadIdx.x % kDsTW), y = blockIdx.y * kDsTH + (int)(threadIdx.x / kDsTW);
-    const bool guided = a.strength != 0.0f;
-    DsTile<H> src{};
-    if (guided)
-        src = ds_load_tile<T, H>(a, ty, tu, tv);
-    if (x >= a.cw || y >= a.ch)
+    __shared__ DsGated<true> list;
+    const int tid = threadIdx.x;
+    const int x = blockIdx.x * kDsTW + (int)(tid % kDsTW), y = blockIdx.y * kDsTH + (int)(tid / kDsTW);
+    const bool valid = x < a.cw && y < a.ch;
+    if (a.strength == 0.0f) {
+        if (valid) {
+            const size_t i = (size_t)y * a.cw + x;
+            ds_store<T>(a.out_u, x, y, a.base_u[i]);
+            ds_store<T>(a.out_v, x, y, a.base_v[i]);
+        }
         return;
-    const size_t i = (size_t)y * a.cw + x;
-    const float bu = a.base_u[i], bv = a.base_v[i];
-    float ru = bu, rv = bv;
-    if (guided) {
+    }
+    if (tid == 0)
+        list.n = 0;
+    const DsTile<H> src = ds_load_tile<T, H>(a, ty, tu, tv);
+    if (valid) {
+        const size_t i = (size_t)y * a.cw + x;
+        const float bu = a.base_u[i], bv = a.base_v[i];
         const float cx = 2.0f * (float)x + 0.5f + a.shift_x, cy = 2.0f * (float)y + 0.5f + a.shift_y;
         const DsStats st = ds_classify(src, cx, cy, a.quality);
-        float gu = bu, gv = bv, gate = st.gate;
-        if (gate <= 1.0e-4f) {
-            gate = 0.0f;
-        } else if (a.quality == 0) {
-            const float2 n = ds_dir(st.primary);
-            const DsEnv env[1] = {{n.x, n.y, 2, 1.8f, st.isotropic}};
+        if (!(st.gate <= 1.0e-4f)) {
+            const int e = atomicAdd(&list.n, 1);
+            list.key[e] = ds_gated_key(tid, st);
+            list.mix[e] = st.mix;
+            list.gate[e] = st.gate;
+        } else {
+            // gate 0: the blend keeps the baseline (gu = bu)
+            const float gate = 0.0f;
+            ds_store<T>(a.out_u, x, y, bu + a.strength * gate * (bu - bu));
+            ds_store<T>(a.out_v, x, y, bv + a.strength * gate * (bv - bv));
+        }
+    }
+    __syncthreads();
+    const int n = list.n;
+    if constexpr (H == 2) {
+        // quality 0: one envelope, one lane per gated sample
+        for (int e = tid; e < n; e += kBlock) {
+            int gx, gy, primary, secondary;
+            bool iso;
+            ds_gated_decode(list.key[e], gx, gy, primary, secondary, iso);
+            const float gate = list.gate[e];
+            const size_t i = (size_t)gy * a.cw + gx;
+            const float bu = a.base_u[i], bv = a.base_v[i];
+            const float cx = 2.0f * (float)gx + 0.5f + a.shift_x, cy = 2.0f * (float)gy + 0.5f + a.shift_y;
+            const float2 n0 = ds_dir(primary);
+            const DsEnv env[1] = {{n0.x, n0.y, 2, 1.8f, iso}};
             float aw[1], au[1], av[1];
             ds_accumulate<DsTile<H>, 1>(src, cx, cy, env, aw, au, av);
-            gu = ds_norm(au[0], aw[0], bu);
-            gv = ds_norm(av[0], aw[0], bv);
-        } else {
-            const float2 n0 = ds_dir(st.primary), n1 = ds_dir(st.secondary);
-            const DsEnv env[2] = {{n0.x, n0.y, 3, 2.2f, st.isotropic}, {n1.x, n1.y, 3, 2.2f, st.isotropic}};
-            float aw[2], au[2], av[2];
-            ds_accumulate<DsTile<H>, 2>(src, cx, cy, env, aw, au, av);
-            const float u0 = ds_norm(au[0], aw[0], bu), v0 = ds_norm(av[0], aw[0], bv);
-            const float u1 = ds_norm(au[1], aw[1], bu), v1 = ds_norm(av[1], aw[1], bv);
-            gu = u0 + st.mix * (u1 - u0);
-            gv = v0 + st.mix * (v1 - v0);
+            const float gu = ds_norm(au[0], aw[0], bu), gv = ds_norm(av[0], aw[0], bv);
+            ds_store<T>(a.out_u, gx, gy, bu + a.strength * gate * (gu - bu));
+            ds_store<T>(a.out_v, gx, gy, bv + a.strength * gate * (gv - bv));
         }
-        ru = bu + a.strength * gate * (gu - bu);
-        rv = bv + a.strength * gate * (gv - bv);
+    } else {
+        // quality 1: primary and secondary envelopes, one lane each
+        constexpr int NE = 2, PER = 64;
+        __shared__ float sums[PER][NE][3];
+        for (int e0 = 0; e0 < n; e0 += PER) {
+            const int m = min(PER, n - e0);
+            if (tid < m * NE) {
+                const int e = e0 + tid / NE, k = tid % NE;
+                int gx, gy, primary, secondary;
+                bool iso;
+                ds_gated_decode(list.key[e], gx, gy, primary, secondary, iso);
+                const float cx = 2.0f * (float)gx + 0.5f + a.shift_x, cy = 2.0f * (float)gy + 0.5f + a.shift_y;
+                const float2 d = ds_dir(k == 0 ? primary : secondary);
+                const DsEnv env[1] = {{d.x, d.y, 3, 2.2f, iso}};
+                float aw[1], au[1], av[1];
+                ds_accumulate<DsTile<H>, 1>(src, cx, cy, env, aw, au, av);
+                sums[tid / NE][k][0] = aw[0];
+                sums[tid / NE][k][1] = au[0];
+                sums[tid / NE][k][2] = av[0];
+            }
+            __syncthreads();
+            if (tid < m) {
+                const int e = e0 + tid;
+                int gx, gy, primary, secondary;
+                bool iso;
+                ds_gated_decode(list.key[e], gx, gy, primary, secondary, iso);
+                const float gate = list.gate[e], mix = list.mix[e];
+                const size_t i = (size_t)gy * a.cw + gx;
+                const float bu = a.base_u[i], bv = a.base_v[i];
+                const float u0 = ds_norm(sums[tid][0][1], sums[tid][0][0], bu);
+                const float v0 = ds_norm(sums[tid][0][2], sums[tid][0][0], bv);
+                const float u1 = ds_norm(sums[tid][1][1], sums[tid][1][0], bu);
+                const float v1 = ds_norm(sums[tid][1][2], sums[tid][1][0], bv);
+                const float gu = u0 + mix * (u1 - u0), gv = v0 + mix * (v1 - v0);
+                ds_store<T>(a.out_u, gx, gy, bu + a.strength * gate * (gu - bu));
+                ds_store<T>(a.out_v, gx, gy, bv + a.strength * gate * (gv - bv));
+            }
+            __syncthreads();
+        }
     }
-    ds_store<T>(a.out_u, x, y, ru);
-    ds_store<T>(a.out_v, x, y, rv);
 }
 
 // quality 2: gate and the four candidates of every chroma sample
-// (generateCandidateRow, downsample.cpp:526-568)
+// (generateCandidateRow, downsample.cpp:526-568); ungated samples copy the
+// baseline, gated ones go through the list
 template <typename T> __global__ __launch_bounds__(kBlock) void k_ds_candidates(DsArgs a) {
     constexpr int H = 5;  // radius 4; one spare column and row measured faster than 4
     __shared__ float ty[DsTile<H>::N], tu[DsTile<H>::N], tv[DsTile<H>::N];
-    const int x = blockIdx.x * kDsTW + (int)(threadIdx.x % kDsTW), y = blockIdx.y * kDsTH + (int)(threadIdx.x / kDsTW);
+    __shared__ DsGated<false> list;
+    const int tid = threadIdx.x;
+    const int x = blockIdx.x * kDsTW + (int)(tid % kDsTW), y = blockIdx.y * kDsTH + (int)(tid / kDsTW);
+    if (tid == 0)
+        list.n = 0;
     const DsTile<H> src = ds_load_tile<T, H>(a, ty, tu, tv);
-    if (x >= a.cw || y >= a.ch)
-        return;
-    const size_t nc = (size_t)a.cw * a.ch, i = (size_t)y * a.cw + x;
-    const float bu = a.base_u[i], bv = a.base_v[i];
-    const float cx = 2.0f * (float)x + 0.5f + a.shift_x, cy = 2.0f * (float)y + 0.5f + a.shift_y;
-    const DsStats st = ds_classify(src, cx, cy, 2);
-    const float gate = st.gate > 1.0e-4f ? st.gate : 0.0f;
-    a.gate[i] = gate;
-    if (gate == 0.0f) {
-        _Pragma("unroll") for (int c = 0; c < 4; ++c) {
-            a.cand_u[c * nc + i] = bu;
-            a.cand_v[c * nc + i] = bv;
+    const size_t nc = (size_t)a.cw * a.ch;
+    if (x < a.cw && y < a.ch) {
+        const size_t i = (size_t)y * a.cw + x;
+        const float cx = 2.0f * (float)x + 0.5f + a.shift_x, cy = 2.0f * (float)y + 0.5f + a.shift_y;
+        const DsStats st = ds_classify(src, cx, cy, 2);
+        const float gate = st.gate > 1.0e-4f ? st.gate : 0.0f;
+        a.gate[i] = gate;
+        if (gate == 0.0f) {
+            const float bu = a.base_u[i], bv = a.base_v[i];
+            _Pragma("unroll") for (int c = 0; c < 4; ++c) {
+                a.cand_u[c * nc + i] = bu;
+                a.cand_v[c * nc + i] = bv;
+            }
+        } else {
+            const int e = atomicAdd(&list.n, 1);
+            list.key[e] = ds_gated_key(tid, st);
+            list.mix[e] = st.mix;
         }
-        return;
     }
-    const float2 n0 = ds_dir(st.primary), n1 = ds_dir(st.secondary);
-    const DsEnv env[5] = {{n0.x, n0.y, 3, 2.2f, st.isotropic}, {n1.x, n1.y, 3, 2.2f, st.isotropic},
-                          {n0.x, n0.y, 4, 2.7f, st.isotropic}, {n1.x, n1.y, 4, 2.7f, st.isotropic},
-                          {n0.x, n0.y, 4, 0.0f, true}};
-    float aw[5], au[5], av[5];
-    ds_accumulate<DsTile<H>, 5>(src, cx, cy, env, aw, au, av);
-    const float u0 = ds_norm(au[0], aw[0], bu), v0 = ds_norm(av[0], aw[0], bv);
-    const float u1 = ds_norm(au[1], aw[1], bu), v1 = ds_norm(av[1], aw[1], bv);
-    a.cand_u[i] = u0 + st.mix * (u1 - u0);
-    a.cand_v[i] = v0 + st.mix * (v1 - v0);
-    _Pragma("unroll") for (int c = 1; c < 4; ++c) {
-        a.cand_u[c * nc + i] = ds_norm(au[c + 1], aw[c + 1], bu);
-        a.cand_v[c * nc + i] = ds_norm(av[c + 1], aw[c + 1], bv);
+    __syncthreads();
+    const int n = list.n;
+    // envelopes: primary / secondary at radius 3 (aniso 2.2), at radius 4
+    // (2.7), isotropic radius 4
+    constexpr int NE = 5, PER = 32;
+    __shared__ float sums[PER][NE][3];
+    for (int e0 = 0; e0 < n; e0 += PER) {
+        const int m = min(PER, n - e0);
+        if (tid < m * NE) {
+            const int e = e0 + tid / NE, k = tid % NE;
+            int gx, gy, primary, secondary;
+            bool iso;
+            ds_gated_decode(list.key[e], gx, gy, primary, secondary, iso);
+            const float cx = 2.0f * (float)gx + 0.5f + a.shift_x, cy = 2.0f * (float)gy + 0.5f + a.shift_y;
+            const float2 d = ds_dir(k == 1 || k == 3 ? secondary : primary);
+            const DsEnv env[1] = {{d.x, d.y, k < 2 ? 3 : 4, k < 2 ? 2.2f : (k < 4 ? 2.7f : 0.0f), k == 4 || iso}};
+            float aw[1], au[1], av[1];
+            ds_accumulate<DsTile<H>, 1>(src, cx, cy, env, aw, au, av);
+            sums[tid / NE][k][0] = aw[0];
+            sums[tid / NE][k][1] = au[0];
+            sums[tid / NE][k][2] = av[0];
+        }
+        __syncthreads();
+        if (tid < m) {
+            int gx, gy, primary, secondary;
+            bool iso;
+            ds_gated_decode(list.key[e0 + tid], gx, gy, primary, secondary, iso);
+            const size_t i = (size_t)gy * a.cw + gx;
+            const float bu = a.base_u[i], bv = a.base_v[i];
+            const float mix = list.mix[e0 + tid];
+            const float u0 = ds_norm(sums[tid][0][1], sums[tid][0][0], bu);
+            const float v0 = ds_norm(sums[tid][0][2], sums[tid][0][0], bv);
+            const float u1 = ds_norm(sums[tid][1][1], sums[tid][1][0], bu);
+            const float v1 = ds_norm(sums[tid][1][2], sums[tid][1][0], bv);
+            a.cand_u[i] = u0 + mix * (u1 - u0);
+            a.cand_v[i] = v0 + mix * (v1 - v0);
+            _Pragma("unroll") for (int c = 1; c < 4; ++c) {
+                a.cand_u[c * nc + i] = ds_norm(sums[tid][c + 1][1], sums[tid][c + 1][0], bu);
+                a.cand_v[c * nc + i] = ds_norm(sums[tid][c + 1][2], sums[tid][c + 1][0], bv);
+            }
+        }
+        __syncthreads();
     }
 }
 
