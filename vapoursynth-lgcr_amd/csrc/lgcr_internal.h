@@ -167,7 +167,7 @@ struct DsArgs {
 // baseline kernel: kDsBaseTileW x kDsBaseTileH outputs per block; the
 // tile's source columns (start of its first output to start of its last +
 // support) and source rows must span at most kDsBaseSegW and kDsBaseSegH
-constexpr int kDsBaseTileW = 32, kDsBaseTileH = 16, kDsBaseSegW = 80, kDsBaseSegH = 48;
+constexpr int kDsBaseTileW = 32, kDsBaseTileH = 8, kDsBaseSegW = 80, kDsBaseSegH = 32;
 
 // Kernel launchers (lgcr_kernels.hip).  All are asynchronous on `s`.
 hipError_t launch_to_float(const RawPlane *planes, int count, int type, hipStream_t s);

@@ -3085,8 +3085,8 @@ __device__ __forceinline__ float ds_at(const DsPlane &p, int w, int h, int x, in
 // from 0, and samples convert as SourcePlane::at (downsample.cpp:101-106).
 constexpr int kDbChunk = kBlock / kDsBaseTileW;
 constexpr int kDbLoads = (kDbChunk * kDsBaseSegW + kBlock - 1) / kBlock;
-static_assert(kDbChunk * kDsBaseTileW == kBlock && kDsBaseTileW * kDsBaseTileH == 2 * kBlock,
-              "one horizontal sample per thread per chunk, two outputs per thread");
+static_assert(kDbChunk * kDsBaseTileW == kBlock && kDsBaseTileH % kDbChunk == 0,
+              "one horizontal sample per thread per chunk, whole output rows per thread");
 
 template <typename T>
 __device__ __forceinline__ T ds_raw(const DsPlane &p, int w, int x, int y) {
@@ -3155,7 +3155,7 @@ template <typename T, int SX, int SY> __global__ __launch_bounds__(kBlock) void 
     const int ox = x0 + tx;
     if (ox >= a.cw)
         return;
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < kDsBaseTileH / kDbChunk; ++k) {
         const int oy = oy0 + tr + k * kDbChunk;
         if (oy > oyl)
             break;
