@@ -86,7 +86,6 @@ struct TemporalDev {
     const TempTap *tx = nullptr, *ty = nullptr;
     const float *nsin = nullptr;
     int search = 0;
-    float shift_x = 0.0f, shift_y = 0.0f;  // unused placeholders for alignment
     double sx = 0.0, sy = 0.0;      // frame siting (luma units)
     double rw = 1.0, rh = 1.0;
     int kernel = 2;
