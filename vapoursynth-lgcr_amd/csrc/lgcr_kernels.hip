@@ -3598,57 +3598,84 @@ template <typename T> __global__ __launch_bounds__(kBlock) void k_ds_candidates(
 
 // quality 2: loopback score over the sample's 2x2 source block, best
 // candidate, blend and packing (scoreCandidateRow, downsample.cpp:576-626;
-// rows outside the frame clamp as the three-row ring's lookup does)
+// rows outside the frame clamp as the three-row ring's lookup does).  One
+// block per 256 samples of a row: ungated samples pack the baseline, gated
+// ones are listed and scored one lane per (sample, source pixel); the
+// sample's lane adds the four pixels' terms in the scalar order.
 template <typename T> __global__ __launch_bounds__(kBlock) void k_ds_score(DsArgs a) {
-    const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
-    if (x >= a.cw)
-        return;
+    constexpr int PER = kBlock / 4;
+    __shared__ int n_list;
+    __shared__ int list[kBlock];
+    __shared__ float part[PER][4][4];  // [sample][source pixel][candidate]
+    const int tid = threadIdx.x;
+    const int x = blockIdx.x * kBlock + tid, y = blockIdx.y;
     const int cw = a.cw, ch = a.ch, w = a.w, h = a.h;
-    const size_t nc = (size_t)cw * ch, i = (size_t)y * cw + x;
-    const float bu = a.base_u[i], bv = a.base_v[i];
-    const float gate = a.gate[i];
-    float ru = bu, rv = bv;
-    if (gate != 0.0f) {
-        float score[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        for (int hy = 2 * y; hy <= 2 * y + 1; ++hy)
-            for (int hx = 2 * x; hx <= 2 * x + 1; ++hx) {
-                const float gx = 0.5f * (ds_at<T>(a.y, w, h, hx + 1, hy) - ds_at<T>(a.y, w, h, hx - 1, hy));
-                const float gy = 0.5f * (ds_at<T>(a.y, w, h, hx, hy + 1) - ds_at<T>(a.y, w, h, hx, hy - 1));
-                const float e2 = gx * gx + gy * gy;
-                const float ew = 0.125f + e2 / (e2 + 1.0e-5f);
-                const float lx = ((float)hx - 0.5f - a.shift_x) * 0.5f;
-                const float ly = ((float)hy - 0.5f - a.shift_y) * 0.5f;
-                const float su = ds_at<T>(a.u, w, h, hx, hy), sv = ds_at<T>(a.v, w, h, hx, hy);
-                const int rx0 = (int)floorf(lx), ry0 = (int)floorf(ly);
-                const int x0 = clampi(rx0, 0, cw - 1), x1 = clampi(rx0 + 1, 0, cw - 1);
-                const int r0 = clampi(ry0, 0, ch - 1), r1 = clampi(ry0 + 1, 0, ch - 1);
-                const float fx = lx - (float)rx0, fy = ly - (float)ry0;
-                const float wts[4] = {(1.0f - fx) * (1.0f - fy), fx * (1.0f - fy), (1.0f - fx) * fy, fx * fy};
-                const size_t idx[4] = {(size_t)r0 * cw + x0, (size_t)r0 * cw + x1, (size_t)r1 * cw + x0,
-                                       (size_t)r1 * cw + x1};
-                _Pragma("unroll") for (int c = 0; c < 4; ++c) {
-                    float recu = 0.0f, recv = 0.0f;
-                    _Pragma("unroll") for (int k = 0; k < 4; ++k) {
-                        const size_t j = idx[k];
-                        float vu = a.base_u[j], vv = a.base_v[j];
-                        vu += a.gate[j] * (a.cand_u[c * nc + j] - vu);
-                        vv += a.gate[j] * (a.cand_v[c * nc + j] - vv);
-                        recu += wts[k] * vu;
-                        recv += wts[k] * vv;
-                    }
-                    const float du = recu - su, dv = recv - sv;
-                    score[c] += ew * (du * du + dv * dv);
-                }
-            }
-        int best = 0;
-        _Pragma("unroll") for (int c = 1; c < 4; ++c)
-            if (score[c] < score[best])
-                best = c;
-        ru = bu + a.strength * gate * (a.cand_u[best * nc + i] - bu);
-        rv = bv + a.strength * gate * (a.cand_v[best * nc + i] - bv);
+    const size_t nc = (size_t)cw * ch;
+    if (tid == 0)
+        n_list = 0;
+    __syncthreads();
+    if (x < cw) {
+        const size_t i = (size_t)y * cw + x;
+        if (a.gate[i] != 0.0f) {
+            list[atomicAdd(&n_list, 1)] = x;
+        } else {
+            ds_store<T>(a.out_u, x, y, a.base_u[i]);
+            ds_store<T>(a.out_v, x, y, a.base_v[i]);
+        }
     }
-    ds_store<T>(a.out_u, x, y, ru);
-    ds_store<T>(a.out_v, x, y, rv);
+    __syncthreads();
+    const int n = n_list;
+    for (int e0 = 0; e0 < n; e0 += PER) {
+        const int m = min(PER, n - e0);
+        if (tid < 4 * m) {
+            const int sx = list[e0 + tid / 4], k = tid % 4;
+            const int hy = 2 * y + k / 2, hx = 2 * sx + k % 2;
+            const float gx = 0.5f * (ds_at<T>(a.y, w, h, hx + 1, hy) - ds_at<T>(a.y, w, h, hx - 1, hy));
+            const float gy = 0.5f * (ds_at<T>(a.y, w, h, hx, hy + 1) - ds_at<T>(a.y, w, h, hx, hy - 1));
+            const float e2 = gx * gx + gy * gy;
+            const float ew = 0.125f + e2 / (e2 + 1.0e-5f);
+            const float lx = ((float)hx - 0.5f - a.shift_x) * 0.5f;
+            const float ly = ((float)hy - 0.5f - a.shift_y) * 0.5f;
+            const float su = ds_at<T>(a.u, w, h, hx, hy), sv = ds_at<T>(a.v, w, h, hx, hy);
+            const int rx0 = (int)floorf(lx), ry0 = (int)floorf(ly);
+            const int x0 = clampi(rx0, 0, cw - 1), x1 = clampi(rx0 + 1, 0, cw - 1);
+            const int r0 = clampi(ry0, 0, ch - 1), r1 = clampi(ry0 + 1, 0, ch - 1);
+            const float fx = lx - (float)rx0, fy = ly - (float)ry0;
+            const float wts[4] = {(1.0f - fx) * (1.0f - fy), fx * (1.0f - fy), (1.0f - fx) * fy, fx * fy};
+            const size_t idx[4] = {(size_t)r0 * cw + x0, (size_t)r0 * cw + x1, (size_t)r1 * cw + x0,
+                                   (size_t)r1 * cw + x1};
+            _Pragma("unroll") for (int c = 0; c < 4; ++c) {
+                float recu = 0.0f, recv = 0.0f;
+                _Pragma("unroll") for (int q = 0; q < 4; ++q) {
+                    const size_t j = idx[q];
+                    float vu = a.base_u[j], vv = a.base_v[j];
+                    vu += a.gate[j] * (a.cand_u[c * nc + j] - vu);
+                    vv += a.gate[j] * (a.cand_v[c * nc + j] - vv);
+                    recu += wts[q] * vu;
+                    recv += wts[q] * vv;
+                }
+                const float du = recu - su, dv = recv - sv;
+                part[tid / 4][k][c] = ew * (du * du + dv * dv);
+            }
+        }
+        __syncthreads();
+        if (tid < m) {
+            const int sx = list[e0 + tid];
+            const size_t i = (size_t)y * cw + sx;
+            float score[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            _Pragma("unroll") for (int k = 0; k < 4; ++k)
+                _Pragma("unroll") for (int c = 0; c < 4; ++c)
+                    score[c] += part[tid][k][c];
+            int best = 0;
+            _Pragma("unroll") for (int c = 1; c < 4; ++c)
+                if (score[c] < score[best])
+                    best = c;
+            const float bu = a.base_u[i], bv = a.base_v[i], gate = a.gate[i];
+            ds_store<T>(a.out_u, sx, y, bu + a.strength * gate * (a.cand_u[best * nc + i] - bu));
+            ds_store<T>(a.out_v, sx, y, bv + a.strength * gate * (a.cand_v[best * nc + i] - bv));
+        }
+        __syncthreads();
+    }
 }
 
 }  // namespace
