@@ -3192,18 +3192,9 @@ struct DsStats {
     bool isotropic = true;
 };
 
-// Source sample access for the guided passes: straight from the planes
-// (clamped, converted per read) or from a block's LDS tile that holds the
-// same values for every coordinate the block's pixels read.
-template <typename T> struct DsGlobal {
-    const DsArgs &a;
-    __device__ float y(int x, int r) const { return ds_at<T>(a.y, a.w, a.h, x, r); }
-    __device__ float u(int x, int r) const { return ds_at<T>(a.u, a.w, a.h, x, r); }
-    __device__ float v(int x, int r) const { return ds_at<T>(a.v, a.w, a.h, x, r); }
-};
-// a 32x8 block of chroma outputs reads source columns [2*x0-6, 2*x0+70) and
-// rows [2*y0-6, 2*y0+22): window radius <= 4 around centres within 0.5 of
-// 2*x+0.5, plus the +-1 neighbours of the direction test and bilinear centre
+// Source sample access for the guided passes: a block's LDS tile holds, for
+// every coordinate its samples read, the value the planes give there
+// (clamped, converted as SourcePlane::at does).
 // Guided Downsample tiles: 32 x 8 chroma samples per block, their 64 x 16
 // source samples and a halo of H on every side, converted once into LDS.
 // The window of radius r around cx in {2x, 2x + 0.5}, cy in {2y, 2y + 0.5,
