@@ -186,6 +186,29 @@ def downsample(args):
     return out
 
 
+def trecon(args):
+    """TRecon (trad 1: two neighbours, block ME, temporal taps) on
+    device-resident 4:2:0 P10 clips at 1080p and 4K, one frame at a time
+    (each call waits for its frame): milliseconds per frame."""
+    from lgcr_amd import abi, serve
+    dev = abi.Device(0)
+    out = {"workload": "TRecon YUV420P10 -> YUV444P10, lanczos3, trad 1, one frame at a time"}
+    for name, (w, h) in (("1080p", (1920, 1080)), ("2160p", (3840, 2160))):
+        n = 24
+        clip = serve.ResidentClip(dev, serve.ClipSpec(name, "trecon", w, h, "420p10", n, 7), max_inflight=1,
+                                  threads=min(16, os.cpu_count() or 1))
+        clip.admit()
+        for k in range(n):
+            clip.run_blocking(k)
+        t0 = time.perf_counter()
+        for k in range(n):
+            clip.run_blocking(k)
+        out[f"{name}_ms_per_frame"] = round((time.perf_counter() - t0) / n * 1e3, 3)
+        clip.release()
+    dev.close()
+    return out
+
+
 def serving(args):
     """Clips of mixed resolution / format / filter with Poisson frame
     arrivals: continuous batching vs one blocking call per frame, at the
@@ -241,6 +264,7 @@ def main():
     ap.add_argument("--serve-rate", default="600,1500", help="offered frame rates (comma-separated)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-downsample", action="store_true")
+    ap.add_argument("--no-trecon", action="store_true")
     ap.add_argument("--serve-poll-us", type=float, default=20.0)
     ap.add_argument("--serve-inflight", type=int, default=4)
     ap.add_argument("--serve-graphs", action="store_true")
@@ -385,6 +409,9 @@ def main():
 
     if not args.no_downsample:
         result["downsample"] = downsample(args)
+
+    if not args.no_trecon:
+        result["trecon"] = trecon(args)
 
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
