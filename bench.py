@@ -155,23 +155,25 @@ def downsample(args):
         out[f"q{q}_device_us"] = round(dev_us, 2)
         out[f"q{q}_hbm_frac_of_device_time"] = round(nbytes / (dev_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 5)
         clip.close()
-    # host planes in and out per frame (the plugin boundary, PCIe-inclusive),
-    # quality 0, eight frames in flight from eight host threads; luma shared
-    from concurrent.futures import ThreadPoolExecutor
-    clip = abi.DownsampleClip(dev, abi.downsample_params(quality=0), abi.make_format(np.uint16, W, H, 0, 0, 10),
-                              max_inflight=8)
+    if not args.no_host_path:
+        # host planes in and out per frame (the plugin boundary,
+        # PCIe-inclusive), quality 0, eight frames in flight from eight host
+        # threads; luma shared
+        from concurrent.futures import ThreadPoolExecutor
+        clip = abi.DownsampleClip(dev, abi.downsample_params(quality=0),
+                                  abi.make_format(np.uint16, W, H, 0, 0, 10), max_inflight=8)
 
-    def one(f):
-        return clip.downsample_host(*f, copy_luma=False)
+        def one(f):
+            return clip.downsample_host(*f, copy_luma=False)
 
-    with ThreadPoolExecutor(8) as ex:
-        list(ex.map(one, frames[:8]))
-        t0 = time.perf_counter()
-        list(ex.map(one, frames * 2))
-        el = time.perf_counter() - t0
-    out["q0_host_fps"] = round(2 * n / el, 1)
-    out["host_note"] = "host planes in and out per frame, 8 threads (PCIe-inclusive); never the value"
-    clip.close()
+        with ThreadPoolExecutor(8) as ex:
+            list(ex.map(one, frames[:8]))
+            t0 = time.perf_counter()
+            list(ex.map(one, frames * 2))
+            el = time.perf_counter() - t0
+        out["q0_host_fps"] = round(2 * n / el, 1)
+        out["host_note"] = "host planes in and out per frame, 8 threads (PCIe-inclusive); never the value"
+        clip.close()
     if not args.no_cpu_baseline:
         import oracle  # the CPU baseline only
         oracle.downsample_frame(oracle.downsample_params(quality=0), *frames[0], bits=10)  # warm-up
