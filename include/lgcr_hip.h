@@ -145,7 +145,7 @@ uint64_t lgcr_device_reserved(const lgcr_device *dev);
  * src/kernels.cpp:94-131, 411-428) and jinc LUT (plugin.cpp:1071-1076), and
  * reserves the per-frame device workspace (FrameWorkspacePool,
  * src/lgcr.h:431-464; CudaPipeline::allocate, cuda_backend.cpp:108-131).
- * max_inflight bounds the workspaces (0 = 4); admission fails with
+ * max_inflight bounds the workspaces (0 = 4, at most 64); admission fails with
  * LGCR_REJECTED when one workspace does not fit the device budget. */
 lgcr_status lgcr_clip_create(lgcr_device *dev, const lgcr_params *p, const lgcr_format *fmt,
                              int max_inflight, lgcr_clip **out);
@@ -297,6 +297,16 @@ enum {
     LGCR_FETCH_COUNT
 };
 lgcr_status lgcr_clip_fetch(lgcr_clip *clip, int what, void *dst, size_t bytes, size_t *needed);
+
+/* Arithmetic self-check, no reference counterpart.  The guided tap weights
+ * and the Downsample spatial weights take 1/y for y >= 1 as the hardware
+ * reciprocal refined by one FMA Newton step, which the kernels rely on to
+ * equal the IEEE quotient 1.0f / y.  This compares the two, bit for bit, for
+ * every float bit pattern in [lo_bits, hi_bits) on device `ordinal`, and
+ * returns the number of mismatches and the smallest mismatching pattern
+ * (0xffffffff if none). */
+lgcr_status lgcr_check_rcp(int ordinal, uint32_t lo_bits, uint32_t hi_bits, uint64_t *mismatches,
+                           uint32_t *first_bad);
 
 #ifdef __cplusplus
 }

@@ -17,20 +17,44 @@ def run(y, u, v, bits=None, **kw):
     return oracle.downsample_frame(oracle.downsample_params(**kw), y, u, v, bits=bits)
 
 
+def _mirrored_tie(y, loc, cx, cy):
+    """True when the secondary direction of the quality-2 classification at
+    chroma sample (cx, cy) is an exact tie between the mirrored directions
+    +-22.5 degrees: the 3-row luma tensor (downsample.cpp classify, central
+    differences with clamped taps) has jxy == 0 exactly, so
+    n.x^2 jxx + 2 n.x n.y jxy + n.y^2 jyy is equal for (c, s) and (c, -s)."""
+    _, sx, sy = oracle.DOWNSAMPLE_LOCATIONS[loc]
+    h, w = y.shape
+    yd = y.astype(np.float64)
+
+    def at(x, r):
+        return yd[min(max(r, 0), h - 1), min(max(x, 0), w - 1)]
+
+    ix = int(np.floor(2 * cx + 0.5 + sx + 0.5))
+    iy = int(np.floor(2 * cy + 0.5 + sy + 0.5))
+    jxx = jxy = jyy = 0.0
+    for k in (-1, 0, 1):
+        gx = 0.5 * (at(ix + 1, iy + k) - at(ix - 1, iy + k))
+        gy = 0.5 * (at(ix, iy + k + 1) - at(ix, iy + k - 1))
+        jxx, jxy, jyy = jxx + gx * gx, jxy + gx * gy, jyy + gy * gy
+    return jxy == 0.0 and jxx > jyy
+
+
 def test_numpy_reference_vectors():
-    """Agreement with the reference's NumPy model: every sample within 2e-6
-    at qualities 0 and 1 (the reference test's tolerance,
-    test/test_downsample.py:159-176).  At quality 2 the candidates include
-    the secondary direction, and near the frame edge two mirrored
-    directions (+-22.5 degrees) can give equal responses: the float32
-    comparisons of downsample.cpp (and the oracle) and the float64 ones of
-    the NumPy model then break that tie differently, and a few samples take
-    another candidate.  There at most 3 % of the samples may exceed 2e-6
-    and none 1e-4."""
+    """Agreement with the reference's NumPy model.  Qualities 0 and 1: every
+    sample within 2e-6; quality 2 on frame "a" (the frame of the reference's
+    own test, test/test_downsample.py:141-170): every sample within 2e-5,
+    that test's bound.  Frame "b" at quality 2: every sample within 2e-6
+    except where the classification's secondary direction is an exact tie
+    between the mirrored directions +-22.5 degrees (checked per sample from
+    the luma tensor): the float32 comparisons of downsample.cpp (and the
+    oracle) keep the first of two equal responses, while the float64
+    direction table of the NumPy model breaks the tie the other way, so the
+    secondary candidate differs there; those samples stay within 1e-4."""
     g = np.load(GOLDEN)
     n = 0
     worst = {0: 0.0, 1: 0.0, 2: 0.0}
-    flips = 0
+    ties = 0
     for key in g.files:
         if not key.endswith("_u") or key.count("_") < 4:
             continue
@@ -38,17 +62,21 @@ def test_numpy_reference_vectors():
         quality, strength = int(q[1:]), float(s[1:])
         y, u, v = g[f"{name}_y"], g[f"{name}_u"], g[f"{name}_v"]
         _, ou, ov = run(y, u, v, quality=quality, kernel=kernel, strength=strength, loc=loc)
-        err = np.concatenate([np.abs(ou - g[key]).ravel(), np.abs(ov - g[key[:-2] + "_v"]).ravel()])
+        err = np.maximum(np.abs(ou - g[key]), np.abs(ov - g[key[:-2] + "_v"]))
         worst[quality] = max(worst[quality], float(err.max()))
         if quality < 2:
             assert err.max() <= 2e-6, (key, float(err.max()))
+        elif name == "a":
+            assert err.max() <= 2e-5, (key, float(err.max()))
         else:
-            k = int(np.count_nonzero(err > 2e-6))
-            flips += k
-            assert k <= 0.03 * err.size and err.max() <= 1e-4, (key, k, float(err.max()))
+            for cy, cx in np.argwhere(err > 2e-6):
+                assert _mirrored_tie(y, loc, cx, cy), (key, cy, cx, float(err[cy, cx]))
+                assert err[cy, cx] <= 1e-4, (key, cy, cx, float(err[cy, cx]))
+                ties += 1
         n += 1
-    print(f"{n} vector pairs, worst |err| per quality {worst}, quality-2 samples off by > 2e-6: {flips}")
+    print(f"{n} vector pairs, worst |err| per quality {worst}, frame-b samples at a mirrored tie: {ties}")
     assert n == 2 * (len(LOCS) * len(KERNELS) * 3 + 3)
+    assert ties <= 20
 
 
 @pytest.mark.parametrize("bits", [8, 10, 16, 32])

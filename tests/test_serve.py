@@ -132,3 +132,15 @@ def test_per_frame_mode_is_serial():
     st = serve.serve(clips, reqs, mode="per_frame")
     assert st["frames"] == 8 and st["seconds"] >= 0.008
     assert st["p99_ms"] >= st["p50_ms"] > 0
+
+
+@pytest.mark.parametrize("mode", ["continuous", "per_frame"])
+def test_clip_larger_than_budget_raises(mode):
+    # a budget smaller than one clip's workspace: nothing can ever make room,
+    # so both modes refuse instead of waiting forever
+    dev = FakeDevice(budget=2)
+    clips = [FakeClip(dev, 1, slots=1, cost=0.001), FakeClip(dev, 3, slots=1, cost=0.001)]
+    specs = [serve.ClipSpec(f"c{i}", "recon", 64, 64, "420p8", 2, i) for i in range(2)]
+    reqs = serve.poisson_requests(specs, float("inf"))
+    with pytest.raises(RuntimeError, match="does not fit the device budget alone"):
+        serve.serve(clips, reqs, mode=mode)

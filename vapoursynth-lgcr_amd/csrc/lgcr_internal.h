@@ -117,6 +117,8 @@ struct IoSlot {
     ptrdiff_t out_stride[3];
 };
 constexpr int kIoSlots = 1024;
+// Upper bound on the frame workspaces of one clip (frames in flight).
+constexpr int kMaxInflight = 64;
 struct IoTable {
     uint32_t counter;
     uint32_t pad[63];
@@ -169,6 +171,7 @@ struct DsArgs {
 constexpr int kDsBaseTileW = 32, kDsBaseTileH = 8, kDsBaseSegW = 80, kDsBaseSegH = 32;
 
 // Kernel launchers (lgcr_kernels.hip).  All are asynchronous on `s`.
+hipError_t launch_check_rcp(uint32_t lo, uint32_t hi, unsigned long long *out, hipStream_t s);
 hipError_t launch_to_float(const RawPlane *planes, int count, int type, hipStream_t s);
 hipError_t launch_from_float(const RawOut *planes, int count, int type, hipStream_t s);
 // structure tensor written interleaved with the luma: yt = {Y, jxx, jxy, jyy}

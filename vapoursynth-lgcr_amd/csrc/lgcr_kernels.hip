@@ -4223,5 +4223,33 @@ hipError_t launch_ds_score(const DsArgs &a, hipStream_t s) {
 
 #undef LGCR_DS_TYPED
 
+namespace {
+// Compares rcp_nr(y) with the IEEE quotient 1.0f / y, bit for bit, for every
+// float bit pattern in [lo, hi).  out[0] counts mismatches; out[1] keeps the
+// smallest mismatching bit pattern (0xffffffff if none).
+__global__ void k_check_rcp(uint32_t lo, uint32_t hi, unsigned long long *out) {
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    unsigned long long bad = 0;
+    uint32_t first = 0xffffffffu;
+    for (uint64_t b = lo + uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; b < hi; b += stride) {
+        const float y = __uint_as_float((uint32_t)b);
+        const float q = 1.0f / y;
+        const float r = rcp_nr(y);
+        if (__float_as_uint(q) != __float_as_uint(r)) {
+            ++bad;
+            first = min(first, (uint32_t)b);
+        }
+    }
+    if (bad) {
+        atomicAdd(&out[0], bad);
+        atomicMin(&out[1], (unsigned long long)first);
+    }
+}
+}  // namespace
+
+hipError_t launch_check_rcp(uint32_t lo, uint32_t hi, unsigned long long *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_check_rcp, dim3(2048), dim3(kBlock), 0, s, lo, hi, out);
+    return hipGetLastError();
+}
 
 }  // namespace lgcr_hip

@@ -624,6 +624,9 @@ struct lgcr_clip {
 
     std::mutex ws_mutex;
     std::condition_variable ws_cv;
+    // Capacity is reserved for max_inflight at create and max_inflight never
+    // grows afterwards, so push_back (under ws_mutex) never reallocates:
+    // a thread that holds a slot reads spaces[slot] without the lock.
     std::vector<std::unique_ptr<Workspace>> spaces;
     std::vector<int> idle;
 };
@@ -843,6 +846,16 @@ struct Carver {
     }
 };
 
+// Float intermediates of a Downsample frame: baseline U, V and, at quality 2
+// with strength above 0, the gate and four candidates per channel.  Every
+// sample is written by the kernels before it is read.
+void ds_float_layout(const lgcr_clip *c, Workspace *w, Carver &cv) {
+    const size_t nc = size_t(c->cw) * c->ch;
+    w->ds_base = cv.take<float>(2 * nc);
+    if (c->dsp.quality == 2 && c->ds_strength != 0.0f)
+        w->ds_cand = cv.take<float>(9 * nc);
+}
+
 void layout(const lgcr_clip *c, Workspace *w, Carver &cv) {
     const size_t ns = size_t(c->sw) * c->sh, nc = size_t(c->cw) * c->ch, no = size_t(c->ow) * c->oh;
     if (c->downsample) {
@@ -852,9 +865,7 @@ void layout(const lgcr_clip *c, Workspace *w, Carver &cv) {
         w->raw_out[0] = cv.take<uint8_t>(ns * c->bps);
         w->raw_out[1] = cv.take<uint8_t>(nc * c->bps);
         w->raw_out[2] = cv.take<uint8_t>(nc * c->bps);
-        w->ds_base = cv.take<float>(2 * nc);
-        if (c->dsp.quality == 2 && c->ds_strength != 0.0f)
-            w->ds_cand = cv.take<float>(9 * nc);
+        ds_float_layout(c, w, cv);
         return;
     }
     if (c->sharpen) {
@@ -1796,7 +1807,8 @@ static lgcr_status create_clip(lgcr_device *dev, const lgcr_params *params, cons
     c->raw_type = fmt->sample_type == 1 ? RAW_F32 : (fmt->bytes_per_sample == 1 ? RAW_U8 : RAW_U16);
     c->bps = fmt->sample_type == 1 ? 4 : fmt->bytes_per_sample;
     c->dil = int(std::ceil(p.support * std::max(c->rw, c->rh))) + 8;
-    c->max_inflight = max_inflight > 0 ? max_inflight : 4;
+    c->max_inflight = std::min(max_inflight > 0 ? max_inflight : 4, kMaxInflight);
+    c->spaces.reserve(size_t(c->max_inflight));
     if (size_t(c->sw) + 2 * size_t(c->dil) > 60 * 1024)
         return fail(LGCR_UNSUPPORTED, "LGCR HIP backend: frame width %d too large", c->sw);
     // kernels index planes with 32-bit sample offsets
@@ -2462,7 +2474,8 @@ lgcr_status lgcr_sharpen_create(lgcr_device *dev, const lgcr_sharpen_params *par
     c->rh = double(c->sh) / c->ch;
     c->raw_type = fmt->sample_type == 1 ? RAW_F32 : (fmt->bytes_per_sample == 1 ? RAW_U8 : RAW_U16);
     c->bps = fmt->sample_type == 1 ? 4 : fmt->bytes_per_sample;
-    c->max_inflight = max_inflight > 0 ? max_inflight : 4;
+    c->max_inflight = std::min(max_inflight > 0 ? max_inflight : 4, kMaxInflight);
+    c->spaces.reserve(size_t(c->max_inflight));
     HIP_TRY(hipSetDevice(dev->ordinal));
     if (c->cw != c->sw || c->ch != c->sh) {
         // footprint luma map with MPEG-2 siting (plugin.cpp:46-48)
@@ -2707,7 +2720,8 @@ lgcr_status lgcr_downsample_create(lgcr_device *dev, const lgcr_downsample_param
     c->rw = c->rh = 2.0;
     c->raw_type = fmt->sample_type == 1 ? RAW_F32 : (fmt->bytes_per_sample == 1 ? RAW_U8 : RAW_U16);
     c->bps = fmt->sample_type == 1 ? 4 : fmt->bytes_per_sample;
-    c->max_inflight = max_inflight > 0 ? max_inflight : 4;
+    c->max_inflight = std::min(max_inflight > 0 ? max_inflight : 4, kMaxInflight);
+    c->spaces.reserve(size_t(c->max_inflight));
     HIP_TRY(hipSetDevice(dev->ordinal));
     const HostWeights xw = downsample_weights(c->sw, params->kernel, c->ds_shift_x);
     const HostWeights yw = downsample_weights(c->sh, params->kernel, c->ds_shift_y);
@@ -3098,6 +3112,28 @@ lgcr_status lgcr_clip_fetch(lgcr_clip *c, int what, void *dst, size_t bytes, siz
         *needed = size;
     if (dst && bytes && size)
         HIP_TRY(hipMemcpy(dst, src, std::min(bytes, size), hipMemcpyDeviceToHost));
+    return LGCR_OK;
+}
+
+lgcr_status lgcr_check_rcp(int ordinal, uint32_t lo_bits, uint32_t hi_bits, uint64_t *mismatches,
+                           uint32_t *first_bad) {
+    if (!mismatches || !first_bad || hi_bits < lo_bits)
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_check_rcp: bad arguments");
+    HIP_TRY(hipSetDevice(ordinal));
+    unsigned long long *d = nullptr;
+    HIP_TRY(hipMalloc(&d, 2 * sizeof(unsigned long long)));
+    const unsigned long long init[2] = {0ull, 0xffffffffull};
+    hipError_t e = hipMemcpy(d, init, sizeof(init), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = launch_check_rcp(lo_bits, hi_bits, d, nullptr);
+    unsigned long long res[2] = {0, 0};
+    if (e == hipSuccess)
+        e = hipMemcpy(res, d, sizeof(res), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess)
+        return fail(LGCR_BACKEND_FAILURE, "lgcr_check_rcp: %s", hipGetErrorString(e));
+    *mismatches = res[0];
+    *first_bad = (uint32_t)res[1];
     return LGCR_OK;
 }
 

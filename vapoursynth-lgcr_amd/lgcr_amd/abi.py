@@ -36,6 +36,7 @@ SYMBOLS = [
     "lgcr_recon_submit", "lgcr_trecon_submit", "lgcr_ticket_query", "lgcr_ticket_wait",
     "lgcr_clip_set_graphs", "lgcr_clip_get_tuning", "lgcr_clip_set_tuning", "lgcr_clip_autotune",
     "lgcr_downsample_params_default", "lgcr_downsample_create", "lgcr_downsample_frame",
+    "lgcr_check_rcp",
 ]
 STAGE_NAMES = [
     "ConvertInput", "ResampleLuma", "BuildGuideMaps", "BuildTrustMask", "BuildMutualGate",
@@ -156,8 +157,19 @@ def lib():
         L.lgcr_downsample_create.argtypes = [vp, C.POINTER(DownsampleParams), C.POINTER(Format), C.c_int,
                                              C.POINTER(vp)]
         L.lgcr_downsample_frame.argtypes = [vp, C.POINTER(Frame), C.POINTER(Frame), C.POINTER(Stats)]
+        L.lgcr_check_rcp.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64),
+                                     C.POINTER(C.c_uint32)]
         _lib = L
     return _lib
+
+
+def check_rcp(ordinal, lo_bits, hi_bits):
+    """(mismatches, first mismatching bit pattern) of the device reciprocal
+    against the IEEE quotient over float bit patterns [lo_bits, hi_bits)."""
+    n = C.c_uint64(0)
+    first = C.c_uint32(0)
+    check(lib().lgcr_check_rcp(int(ordinal), int(lo_bits), int(hi_bits), C.byref(n), C.byref(first)))
+    return n.value, first.value
 
 
 def check(status):

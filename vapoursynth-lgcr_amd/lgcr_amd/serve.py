@@ -339,6 +339,10 @@ def serve(clips, requests, mode="continuous", keep=None, poll_sleep=20e-6, max_i
                 if freed:
                     gen += 1
                 if not (freed and c.admit()):
+                    if all(o.clip is None for k, o in enumerate(clips) if k != r.clip):
+                        # no other clip holds device state, so no release can
+                        # ever make room for this one
+                        raise RuntimeError("continuous serving: clip does not fit the device budget alone")
                     c.rejected_at = gen
                     blocked.add(r.clip)
                     rest.append(r)
