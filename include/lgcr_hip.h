@@ -243,6 +243,18 @@ lgcr_status lgcr_downsample_create(lgcr_device *dev, const lgcr_downsample_param
 /* stats (may be NULL): DownsampleBase / Guide / CandidateScore / Output
  * device times, as writeDownsampleProfile reports them (downsample.cpp:31-49) */
 lgcr_status lgcr_downsample_frame(lgcr_clip *clip, const lgcr_frame *in, lgcr_frame *out, lgcr_stats *stats);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, hosts that
+ * forbid hipMalloc): lgcr_downsample_frame on device-resident frames whose
+ * float intermediates live in a fixed 32 MiB array per GPU, set up when the
+ * device is opened, instead of a clip workspace.  The call allocates
+ * nothing; it returns LGCR_UNSUPPORTED when a frame needs more than 32 MiB
+ * (44 bytes per chroma sample at quality 2 with strength above 0, else 8:
+ * 1920x1080 fits at every quality, 3840x2160 only below quality 2) and
+ * LGCR_INVALID_ARGUMENT for host frames.  Every call on one GPU shares that
+ * GPU's array; calls from one process are serialised, and the call returns
+ * after its work has finished.  Results equal lgcr_downsample_frame's. */
+lgcr_status lgcr_downsample_frame_noalloc(lgcr_clip *clip, const lgcr_frame *in, lgcr_frame *out);
 
 /* Per-stage device time accumulated while timing is on: the nanosecond
  * counters of PipelineMetrics (src/pipeline.h:315-336), here measured with

@@ -445,6 +445,54 @@ struct Blob {
 // device / clip / workspace
 // ---------------------------------------------------------------------------
 
+// Scratch of lgcr_downsample_frame_noalloc.  The loader places one copy of
+// this array on every GPU the library runs on, held until the process ends.
+// In host code the name `scratch` is not a device address: each GPU's
+// address comes from hipGetSymbolAddress, taken once per device when the
+// device is opened (noalloc_setup), and only that address reaches kernels.
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kNoAllocDevices = 64;
+struct NoAllocState {
+    bool ready = false;        // set-up attempted
+    void *ptr = nullptr;       // this GPU's address of `scratch` (null if set-up failed)
+    hipStream_t stream = nullptr;
+    std::string error;         // why set-up failed
+    std::mutex run;            // calls on one GPU share its array: one at a time
+};
+NoAllocState g_noalloc[kNoAllocDevices];
+std::mutex g_noalloc_setup;
+
+// Once per GPU, with that GPU current: the device address of `scratch` and a
+// stream for the no-allocation entry point.  A failure is kept and reported
+// by that entry point; the other entry points do not depend on it.
+void noalloc_setup(int ordinal) {
+    if (ordinal < 0 || ordinal >= kNoAllocDevices)
+        return;
+    std::lock_guard<std::mutex> lock(g_noalloc_setup);
+    NoAllocState &na = g_noalloc[ordinal];
+    if (na.ready)
+        return;
+    na.ready = true;
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) {
+        na.error = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e);
+        return;
+    }
+    e = hipStreamCreateWithFlags(&na.stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        na.error = std::string("hipStreamCreateWithFlags failed: ") + hipGetErrorString(e);
+        return;
+    }
+    na.ptr = ptr;
+}
+
+}  // namespace
+
 struct lgcr_device {
     int ordinal = 0;
     uint64_t budget = 0;
@@ -1761,6 +1809,7 @@ lgcr_status lgcr_device_open(int ordinal, uint64_t budget, lgcr_device **out) {
     HIP_TRY(hipSetDevice(ordinal));
     size_t free_b = 0, total_b = 0;
     HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    noalloc_setup(ordinal);
     auto d = new lgcr_device();
     d->ordinal = ordinal;
     d->budget = budget ? budget : uint64_t(double(free_b) * 0.9);
@@ -2846,6 +2895,52 @@ lgcr_status lgcr_downsample_frame(lgcr_clip *c, const lgcr_frame *in, lgcr_frame
         }
         stats->output_pixels = uint64_t(c->sw) * c->sh + 2 * uint64_t(c->cw) * c->ch;
     }
+    return LGCR_OK;
+}
+
+lgcr_status lgcr_downsample_frame_noalloc(lgcr_clip *c, const lgcr_frame *in, lgcr_frame *out) {
+    if (!c || !in || !out)
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_downsample_frame_noalloc: NULL argument");
+    if (!c->downsample)
+        return fail(LGCR_INVALID_CONTEXT,
+                    "lgcr_downsample_frame_noalloc: clip was not created by lgcr_downsample_create");
+    if (in->memory != LGCR_MEMORY_DEVICE || out->memory != LGCR_MEMORY_DEVICE)
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_downsample_frame_noalloc: frames must be device resident");
+    for (int i = 0; i < 3; ++i)
+        if (!in->data[i] || (i > 0 && !out->data[i]))
+            return fail(LGCR_INVALID_ARGUMENT, "lgcr_downsample_frame_noalloc: plane %d is NULL", i);
+    const int ordinal = c->dev->ordinal;
+    if (ordinal >= kNoAllocDevices)
+        return fail(LGCR_UNSUPPORTED, "lgcr_downsample_frame_noalloc: device ordinal %d beyond %d", ordinal,
+                    kNoAllocDevices - 1);
+    NoAllocState &na = g_noalloc[ordinal];
+    if (!na.ptr)
+        return fail(LGCR_BACKEND_FAILURE, "lgcr_downsample_frame_noalloc: no scratch on device %d: %s", ordinal,
+                    na.ready ? na.error.c_str() : "device not opened");
+    Workspace w;
+    Carver probe;
+    ds_float_layout(c, &w, probe);
+    if (probe.off > kNoAllocScratchBytes)
+        return fail(LGCR_UNSUPPORTED,
+                    "lgcr_downsample_frame_noalloc: a %dx%d frame at quality %d needs %zu bytes of scratch, "
+                    "more than the %zu available",
+                    c->sw, c->sh, c->dsp.quality, probe.off, kNoAllocScratchBytes);
+    std::lock_guard<std::mutex> lock(na.run);
+    HIP_TRY(hipSetDevice(ordinal));
+    Carver cv;
+    cv.base = static_cast<uint8_t *>(na.ptr);
+    ds_float_layout(c, &w, cv);
+    w.stream = na.stream;
+    const void *din[3] = {in->data[0], in->data[1], in->data[2]};
+    void *dout[3] = {out->data[0], out->data[1], out->data[2]};
+    const lgcr_status st = enqueue_downsample(c, &w, din, in->stride, dout, out->stride, false);
+    // drain before the lock is released, also after a failed launch, so the
+    // next call never overlaps work still reading this GPU's array
+    const hipError_t e = hipStreamSynchronize(na.stream);
+    if (st != LGCR_OK)
+        return st;
+    if (e != hipSuccess)
+        return fail(LGCR_BACKEND_FAILURE, "hipStreamSynchronize failed: %s", hipGetErrorString(e));
     return LGCR_OK;
 }
 

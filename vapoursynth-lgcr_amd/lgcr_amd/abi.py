@@ -36,7 +36,7 @@ SYMBOLS = [
     "lgcr_recon_submit", "lgcr_trecon_submit", "lgcr_ticket_query", "lgcr_ticket_wait",
     "lgcr_clip_set_graphs", "lgcr_clip_get_tuning", "lgcr_clip_set_tuning", "lgcr_clip_autotune",
     "lgcr_downsample_params_default", "lgcr_downsample_create", "lgcr_downsample_frame",
-    "lgcr_check_rcp",
+    "lgcr_downsample_frame_noalloc", "lgcr_check_rcp",
 ]
 STAGE_NAMES = [
     "ConvertInput", "ResampleLuma", "BuildGuideMaps", "BuildTrustMask", "BuildMutualGate",
@@ -157,6 +157,7 @@ def lib():
         L.lgcr_downsample_create.argtypes = [vp, C.POINTER(DownsampleParams), C.POINTER(Format), C.c_int,
                                              C.POINTER(vp)]
         L.lgcr_downsample_frame.argtypes = [vp, C.POINTER(Frame), C.POINTER(Frame), C.POINTER(Stats)]
+        L.lgcr_downsample_frame_noalloc.argtypes = [vp, C.POINTER(Frame), C.POINTER(Frame)]
         L.lgcr_check_rcp.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64),
                                      C.POINTER(C.c_uint32)]
         _lib = L
@@ -454,6 +455,15 @@ class DownsampleClip:
         fout = Frame((C.c_void_p * 3)(*[p or None for p, _ in planes_out]),
                      (C.c_ssize_t * 3)(*[s for _, s in planes_out]), MEMORY_DEVICE)
         check(lib().lgcr_downsample_frame(self.handle, C.byref(fin), C.byref(fout), None))
+
+    def downsample_device_noalloc(self, planes_in, planes_out):
+        """As downsample_device, through the entry point that allocates
+        nothing (a fixed per-GPU scratch array instead of a workspace)."""
+        fin = Frame((C.c_void_p * 3)(*[p for p, _ in planes_in]), (C.c_ssize_t * 3)(*[s for _, s in planes_in]),
+                    MEMORY_DEVICE)
+        fout = Frame((C.c_void_p * 3)(*[p or None for p, _ in planes_out]),
+                     (C.c_ssize_t * 3)(*[s for _, s in planes_out]), MEMORY_DEVICE)
+        check(lib().lgcr_downsample_frame_noalloc(self.handle, C.byref(fin), C.byref(fout)))
 
     def close(self):
         if self.handle:
