@@ -142,6 +142,8 @@ def lib():
         L.or_trecon_frame.argtypes = [C.POINTER(Params), C.POINTER(FrameFormat), C.c_void_p * 3, C.c_ssize_t * 3,
                                       C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_ssize_t),
                                       C.c_void_p * 3, C.c_ssize_t * 3]
+        L.or_affine_debug.argtypes = [fp]
+        L.or_lc_map.restype = None
         i16p = C.POINTER(C.c_int16)
         L.or_block_match.argtypes = [fp, fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, i16p, i16p, fp,
                                      C.POINTER(C.c_int), C.POINTER(C.c_int)]

@@ -1936,6 +1936,11 @@ static void detail_transfer(const or_params *p, const or_debug *D, const affine_
     }
 }
 
+/* test access: the credibility g of the next algo-6 frame's affine maps
+ * (chroma grid) */
+static float *g_affine_debug;
+void or_affine_debug(float *g) { g_affine_debug = g; }
+
 /* algo 6 orchestration (plugin.cpp:573-626) on a plain-filled output */
 static void recon_algo6(const or_params *p, or_debug *D, const or_axis *ax, const or_axis *ay,
                         float *out_u, float *out_v) {
@@ -1949,6 +1954,8 @@ static void recon_algo6(const or_params *p, or_debug *D, const or_axis *ax, cons
     const float *yc[3] = { D->lc, yc1, yc2 };
     const int sparse_roi = D->has_ws && !or_workset_chroma_dense(&D->ws);
     affine_maps_build(p, D->u, D->v, cw, ch, yc, D->has_ws ? D->ws.chroma_mask : NULL, sparse_roi, &m);
+    if (g_affine_debug)
+        memcpy(g_affine_debug, m.g, sizeof(float) * nc);
     float *detail = (float *)malloc(sizeof(float) * (size_t)D->ow * D->oh);
     detail_map(p, D->y, D->sw, D->sh, m.deg, cw, ch, detail);
     detail_transfer(p, D, &m, detail, ax, ay, out_u, out_v);

@@ -212,6 +212,9 @@ int or_downsample_frame(const or_downsample_params *d, const or_frame_format *f,
 /* test access: the next quality-2 frame also copies its gates (nc) and
  * candidates (4 * nc, candidate-major) here; NULL turns it off */
 void or_downsample_debug(float *gate, float *cand_u, float *cand_v);
+/* test access: copy the affine credibility g (chroma grid) of the next
+ * algo-6 Recon frame into g; NULL stops */
+void or_affine_debug(float *g);
 /* the plain separable baseline alone (float U, V at the output size) */
 int or_downsample_base(const or_downsample_params *d, const or_frame_format *f, const void *const in[3],
                        const ptrdiff_t in_stride[3], float *base_u, float *base_v);

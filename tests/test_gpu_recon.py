@@ -166,13 +166,15 @@ def test_config2_frame_1080p_p10(device):
 
 @pytest.mark.parametrize("kernel,taps", [("jinc", 3), ("lanczos", 4)])
 def test_config3_resize_algo6_float(device, kernel, taps):
-    """YUV420PS -> 2x YUV444PS with algo 6 (the 1080p -> 2160p workload at
-    480x270 so the oracle finishes quickly)."""
-    y, u, v = frame(480, 270, bits=32, seed=2)
-    o_out, _, g_out, clip = run_both(y, u, v, device=device, algo=6, kernel=kernel, taps=taps, width=960,
-                                     height=540)
+    """1920x1080 YUV420PS (noise sigma 0.002) -> 3840x2160 YUV444PS with
+    algo 6: every output plane within 1e-5 of the oracle (which takes about
+    10 s for the jinc case on one host core)."""
+    y, u, v = frame(1920, 1080, bits=32, seed=2)
+    o_out, _, g_out, clip = run_both(y, u, v, device=device, algo=6, kernel=kernel, taps=taps, width=3840,
+                                     height=2160)
     clip.close()
-    assert _report(f"resize-{kernel}{taps}-algo6", o_out, g_out, True) == 0
+    assert g_out[1].shape == (2160, 3840)
+    assert _report(f"config3-{kernel}{taps}-algo6", o_out, g_out, True) == 0
 
 
 @pytest.mark.parametrize("algo", [2, 4, 6])

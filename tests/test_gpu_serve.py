@@ -83,3 +83,62 @@ def test_mix_frames_match_oracle(device):
                   f"max_abs={mx:g}")
             assert cnt == 0
     assert seen
+
+
+def _config5_specs(frames=3):
+    """Every resolution of the serving workload with every format, Recon and
+    TRecon alternating so each resolution and each format has both."""
+    specs = []
+    for ri, (w, h) in enumerate(serve.RESOLUTIONS):
+        for fi, fmt in enumerate(serve.FORMATS):
+            kind = "recon" if (ri + fi) % 2 == 0 else "trecon"
+            specs.append(serve.ClipSpec(f"c{ri}{fi}", kind, w, h, fmt, frames, 500 + 10 * ri + fi))
+    return specs
+
+
+def test_config5_mix_at_stated_shapes(device):
+    """720x480 / 1280x720 / 1920x1080 / 3840x2160 clips in P8 4:2:0, P10
+    4:2:0 and P10 4:2:2, Recon and TRecon, served together under Poisson
+    600 frames/s with continuous batching: every frame equals the same frame
+    served alone, and one frame per (filter, format) plus a 4K Recon frame
+    match the oracle."""
+    specs = _config5_specs()
+    clips = [serve.ResidentClip(device, s, max_inflight=2, threads=8) for s in specs]
+    reqs = serve.poisson_requests(specs, 600.0, seed=11)
+    got = {}
+    st = serve.serve(clips, reqs, mode="continuous", keep=got)
+    print(f"config 5 mix, {len(specs)} clips: {st}")
+    assert st["frames"] == len(reqs) == len(got)
+    bad = 0
+    for (ci, n), planes in got.items():
+        ref = _alone(clips[ci], n)
+        bad += sum(int(not np.array_equal(a, b)) for a, b in zip(planes, ref))
+    assert bad == 0
+    checked = set()
+    for ci, c in enumerate(clips):
+        key = (c.spec.kind, c.spec.fmt)
+        big = c.spec.width == 3840 and c.spec.kind == "recon"
+        if key in checked and not big:
+            continue
+        if c.spec.width > 1280 and not big:
+            continue
+        checked.add(key)
+        n = 1
+        src = c.host_src
+        bits = c.bits if c.bits not in (8, 16) else None
+        if c.temporal:
+            ids = serve.temporal_frame_numbers(n, c.spec.frames - 1, 1)
+            ref = oracle.trecon_frame(oracle.trecon_params(), src[n], [src[i] for i in ids], bits=bits,
+                                      ss_w=c.ssw, ss_h=c.ssh)
+        else:
+            y, u, v = src[n]
+            ref = oracle.recon_frame(oracle.make_params(c.spec.width, c.spec.height), y, u, v, bits=bits,
+                                     ss_w=c.ssw, ss_h=c.ssh)
+        for i in range(3):
+            cnt, mx = compare_planes(ref[i], got[(ci, n)][i], False)
+            print(f"{c.spec.kind} {c.spec.width}x{c.spec.height} {c.spec.fmt} plane {i}: mismatches={cnt} "
+                  f"max_abs={mx:g}")
+            assert cnt == 0
+    assert {k for k, _ in checked} == {"recon", "trecon"} and len({f for _, f in checked}) == 3
+    for c in clips:
+        c.release()
