@@ -211,6 +211,80 @@ def trecon(args):
     return out
 
 
+# Non-headline workloads of the benchmark definition, each timed on
+# device-resident frames with eight frames in flight, plus an isolated stage
+# pass: (name, description, width, height, ss_w, ss_h, bits, out_w, out_h,
+# Recon arguments, synthetic-noise arguments, frames)
+WORKLOADS = (
+    ("1080p420p10_algo4", "Recon 1920x1080 YUV420P10 -> YUV444P10, algo 4, lanczos3, bp 0.5",
+     1920, 1080, 1, 1, 10, 1920, 1080, {"algo": 4, "bp": 0.5}, {}, 48),
+    ("1080p420p10_strength0", "Recon 1920x1080 YUV420P10 -> YUV444P10, strength 0 (plain A/B control), bp 0.5",
+     1920, 1080, 1, 1, 10, 1920, 1080, {"strength": 0.0, "bp": 0.5}, {}, 48),
+    ("1080p420ps_to_2160p_jinc3_algo6", "Recon 1920x1080 YUV420PS -> 3840x2160 YUV444PS, jinc 3, algo 6",
+     1920, 1080, 1, 1, 32, 3840, 2160, {"algo": 6, "kernel": "jinc", "taps": 3}, {}, 24),
+    ("1080p420ps_to_2160p_lanczos4_algo6", "Recon 1920x1080 YUV420PS -> 3840x2160 YUV444PS, lanczos 4, algo 6",
+     1920, 1080, 1, 1, 32, 3840, 2160, {"algo": 6, "kernel": "lanczos", "taps": 4}, {}, 24),
+    ("1080p422p8_bm", "Recon 1920x1080 YUV422P8 (noise 3 LSB on all planes) -> YUV444P8, algo 2, bm, bp 0.5",
+     1920, 1080, 1, 0, 8, 1920, 1080, {"bm": 1, "bp": 0.5}, {"luma_noise": 3.0, "chroma_noise": 3.0}, 48),
+)
+
+
+def workloads(args):
+    """fps, per-stage device time and the dominant stage's HBM fraction for
+    each entry of WORKLOADS."""
+    import numpy as np
+    import torch
+    from lgcr_amd import abi, synth
+    dev = abi.Device(0)
+    out = {}
+    for name, desc, W, H, ssw, ssh, bits, ow, oh, kw, noise, n in WORKLOADS:
+        bps = 4 if bits == 32 else (1 if bits <= 8 else 2)
+        np_dt = {1: np.uint8, 2: np.uint16, 4: np.float32}[bps]
+        tdt = {1: torch.uint8, 2: torch.int16, 4: torch.float32}[bps]
+        frames = synth.clip(n, W, H, ss_w=ssw, ss_h=ssh, bits=bits, seed=3, scene_per_frame=True,
+                            threads=min(16, os.cpu_count() or 1), **noise)
+        din = [[torch.from_numpy(p.view(np.int16) if bps == 2 else p).to("cuda") for p in f] for f in frames]
+        same = (ow, oh) == (W, H)
+        dout = [[torch.empty((oh, ow), dtype=tdt, device="cuda") for _ in range(2 if same else 3)]
+                for _ in frames]
+        fin = [[(t.data_ptr(), t.stride(0) * bps) for t in f] for f in din]
+        fout = [([(0, 0)] if same else []) + [(t.data_ptr(), t.stride(0) * bps) for t in o] for o in dout]
+        fmt = abi.make_format(np_dt, W, H, ssw, ssh, bits=None if bits in (8, 16, 32) else bits)
+        params = abi.default_params(width=ow, height=oh, **kw)
+        clip = abi.Clip(dev, params, fmt, max_inflight=8)
+        clip.recon_batch(fin, fout)
+        torch.cuda.synchronize()
+        steps = 2
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            clip.recon_batch(fin, fout)
+        torch.cuda.synchronize()
+        fps = steps * n / (time.perf_counter() - t0)
+        clip.close()
+        sclip = abi.Clip(dev, params, fmt, max_inflight=1)
+        sclip.recon_batch(fin[:4], fout[:4])
+        sclip.set_timing(True)
+        sclip.recon_batch(fin, fout)
+        torch.cuda.synchronize()
+        stages = {k: v[0] / max(1, v[1]) for k, v in sclip.stage_times().items()}
+        sclip.close()
+        cw, ch = W >> ssw, H >> ssh
+        nbytes = bps * (W * H + 2 * cw * ch + 2 * ow * oh + (0 if same else ow * oh))
+        dom = max(stages, key=stages.get)
+        out[name] = {
+            "workload": desc, "frames": n, "inflight": 8, "fps": round(fps, 1),
+            "bytes_per_frame": nbytes, "end_to_end_hbm_gbs": round(nbytes * fps / 1e9, 2),
+            "stage_avg_us": {k: round(v, 2) for k, v in stages.items()},
+            "device_us_per_frame": round(sum(stages.values()), 2),
+            "dominant": {"stage": dom, "avg_us": round(stages[dom], 2),
+                         "hbm_frac": round(nbytes / (stages[dom] * 1e-6) / 1e9 / HBM_PEAK_GBS, 5)},
+        }
+        del din, dout, fin, fout
+        torch.cuda.empty_cache()
+    dev.close()
+    return out
+
+
 def serving(args):
     """Clips of mixed resolution / format / filter with Poisson frame
     arrivals: continuous batching vs one blocking call per frame, at the
@@ -219,7 +293,7 @@ def serving(args):
     dev = abi.Device(0)
     specs = serve.make_mix(args.serve_clips, args.serve_frames, seed=0)
     clips = [serve.ResidentClip(dev, s, max_inflight=4, threads=min(16, os.cpu_count() or 1),
-                                graphs=args.serve_graphs) for s in specs]
+                                graphs=args.serve_graphs, keep_host=False) for s in specs]
     resident = sum(c.src_bytes + c.out_bytes for c in clips)
     rates = [float(r) for r in str(args.serve_rate).split(",") if r]
     out = {"clips": len(specs), "frames_per_clip": args.serve_frames, "offered_fps": rates,
@@ -261,9 +335,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--serve-clips", type=int, default=16)
-    ap.add_argument("--serve-frames", type=int, default=48)
+    ap.add_argument("--serve-clips", type=int, default=64)
+    ap.add_argument("--serve-frames", type=int, default=120)
     ap.add_argument("--serve-rate", default="600,1500", help="offered frame rates (comma-separated)")
+    ap.add_argument("--no-workloads", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-downsample", action="store_true")
     ap.add_argument("--no-trecon", action="store_true")
@@ -405,6 +480,9 @@ def main():
     dev.close()
     del dev_in, dev_out, fin, fout
     torch.cuda.empty_cache()
+
+    if not args.no_workloads:
+        result["workloads"] = workloads(args)
 
     if not args.no_serve:
         result["serving"] = serving(args)

@@ -46,10 +46,28 @@ struct RadialDev {
     int valid = 0, sup_x = 0, sup_y = 0, nx = 0, ny = 0;
     const uint16_t *xphase = nullptr, *yphase = nullptr;
     const float *w = nullptr;
+    // output columns of each x phase (xoff[p] .. xoff[p + 1]) and, per phase
+    // pair, the sum and absolute sum of the active weights in (j, i) order
+    const int *xs = nullptr, *xoff = nullptr;
+    const double *wsa = nullptr;
+    int max_count = 0;
     const float *lut = nullptr;
     int lut_n = 0;
     float lut_scale = 1024.0f;
     float inv_wx = 1.0f, inv_wy = 1.0f;  // per-axis downscale AA factors
+};
+
+// Tables of the radial (jinc) luma resample (resampleRadial,
+// src/kernels.cpp:276-334) when output coordinates fall into few classes of
+// identical tap offsets per axis (2x upscaling: two).  w holds the tap
+// weights of every (y class, x class) pair, sup x sup in (j, i) order as
+// doubles, and wsum their sums in that order; xs lists the output columns
+// of each x class (xoff[c] .. xoff[c + 1]).  valid == 0: no tables.
+struct RadialResampleDev {
+    int valid = 0, ncx = 0, ncy = 0, sup = 0, max_count = 0;
+    const int *x0 = nullptr, *y0 = nullptr, *xs = nullptr, *xoff = nullptr;
+    const uint8_t *ycls = nullptr;
+    const double *w = nullptr, *wsum = nullptr;
 };
 
 // Scalar parameters the kernels read, already narrowed exactly as the
@@ -58,6 +76,9 @@ struct GuideParams {
     float sigma, sratio, sdb, strength, rescue, stretch, inv_gsigma2, ms, margin, sp_out;
     float frw, frh;
     int ridge, cedge, clamp_hull, ms_on, radial;
+    // the planes come from an integer format, so every sample is finite
+    // (the exact tap-loop shortcuts rely on finite weights)
+    int finite = 0;
 };
 
 // One motion-compensated neighbour of TRecon (TemporalNbr, src/lgcr.h:729-736).
@@ -313,6 +334,8 @@ hipError_t launch_resample_h(const float *src, int sw, int sh, float *dst,
                              const WeightsDev &t, hipStream_t s);
 hipError_t launch_resample_v(const float *src, int sw, int sh, float *dst,
                              const WeightsDev &t, hipStream_t s);
+hipError_t launch_resample_radial_tab(const float *src, int sw, int sh, float *dst, int dw, int dh,
+                                      const RadialResampleDev &t, hipStream_t s);
 hipError_t launch_resample_radial(const float *src, int sw, int sh, float *dst, int dw, int dh,
                                   double support, const float *lut, int lut_n, float lut_scale,
                                   hipStream_t s);

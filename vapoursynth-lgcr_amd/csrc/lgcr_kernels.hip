@@ -1269,6 +1269,65 @@ __global__ __launch_bounds__(kBlock) void k_plain_radial(const float *__restrict
     out_v[o] = (float)vv;
 }
 
+// The radial plain base with the phase tables, one x phase per block row:
+// the phase pair's weights, tap activity and weight sums are uniform across
+// the wave (scalar loads), so a tap costs two source loads, two FP64 FMAs
+// and the hull updates.  Same arithmetic and order as k_plain_radial.
+template <int SUP>
+__global__ __launch_bounds__(kBlock) void k_plain_radial_tab(const float *__restrict__ u,
+                                                             const float *__restrict__ v, int cw, int ch,
+                                                             AxisDev ax, AxisDev ay, RadialDev rd, float *out_u,
+                                                             float *out_v, int clamp_hull, float margin) {
+    const int cls = blockIdx.z, oy = blockIdx.y;
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    const int beg = rd.xoff[cls], cnt = rd.xoff[cls + 1] - beg;
+    if (k >= cnt)
+        return;
+    const int SX = SUP ? SUP : ax.sup, SY = SUP ? SUP : ay.sup;
+    const int ox = rd.xs[beg + k];
+    const int pair = (int)rd.yphase[oy] * rd.nx + cls;
+    const float *__restrict__ base = rd.w + (size_t)pair * SX * SY;
+    // tap activity is part of the phase: the class's first column has it
+    const float *__restrict__ amx = ax.am + (size_t)rd.xs[beg] * SX;
+    const float *__restrict__ amy = ay.am + (size_t)oy * SY;
+    const int tx0 = ax.start[ox], ty0 = ay.start[oy];
+    double au = 0.0, av = 0.0;
+    float mnu = 1e30f, mxu = -1e30f, mnv = 1e30f, mxv = -1e30f;
+    for (int j = 0; j < SY; ++j) {
+        if (amy[j] == 0.0f)
+            continue;
+        const size_t row = (size_t)clampi(ty0 + j, 0, ch - 1) * cw;
+        _Pragma("unroll") for (int i = 0; i < SX; ++i) {
+            if (amx[i] == 0.0f)
+                continue;
+            const float w = base[j * SX + i];
+            const size_t q = row + clampi(tx0 + i, 0, cw - 1);
+            const float a = u[q], b = v[q];
+            au = dacc(au, w, a);
+            av = dacc(av, w, b);
+            mnu = fmn(mnu, a); mxu = fmx(mxu, a);
+            mnv = fmn(mnv, b); mxv = fmx(mxv, b);
+        }
+    }
+    const double ws = rd.wsa[2 * pair], sa = rd.wsa[2 * pair + 1];
+    const float scx = ax.pos[ox], scy = ay.pos[oy];
+    double vu, vv;
+    if (ws > 1e-6 && ws >= 0.05 * sa) {
+        vu = au / ws;
+        vv = av / ws;
+    } else {
+        vu = bilinear(u, cw, ch, scx, scy);
+        vv = bilinear(v, cw, ch, scx, scy);
+    }
+    if (clamp_hull) {
+        vu = dclamp(vu, (double)(mnu - margin), (double)(mxu + margin));
+        vv = dclamp(vv, (double)(mnv - margin), (double)(mxv + margin));
+    }
+    const size_t o = (size_t)oy * ax.n + ox;
+    out_u[o] = (float)vu;
+    out_v[o] = (float)vv;
+}
+
 // ---------------------------------------------------------------------------
 // guided reconstruction, algo 2 (reconstructChromaImpl, recon.cpp:14-819):
 // one thread per active output pixel of the compacted workset.
@@ -1326,7 +1385,11 @@ struct TapAcc {
 // replaced by an exact zero, which leaves every accumulator unchanged.
 // FAST: the caller has shown every reciprocal argument of the active taps
 // lies in [1, 2^126), where rcp_nr is exact.  GUIDED: strength > 0.
-template <bool RADIAL, int SUP, bool WIN, bool FAST, bool GUIDED>
+// NORESCUE: the caller has shown rescue == 0 for the pixel.  Then
+// rescue * bump is +0 (bump is finite), wx + 0 == wx and the weight is
+// exactly sim * wx (a zero weight may change sign, which no accumulator
+// sees), so the bump kernel and the projected offsets are not computed.
+template <bool RADIAL, int SUP, bool WIN, bool FAST, bool GUIDED, bool NORESCUE = false>
 __device__ __forceinline__ void guided_taps(const GuidedArgs &A, const PixAxis<SUP> &px, const PixAxis<SUP> &py,
                                             int ox, int oy, const TapIn &t, const float4 *win, int wx0, int wy0,
                                             TapAcc &acc) {
@@ -1351,7 +1414,7 @@ __device__ __forceinline__ void guided_taps(const GuidedArgs &A, const PixAxis<S
     }
     constexpr int NH = SUP ? SUP : 1;
     float dxn[NH], dxt[NH];
-    if constexpr (SUP != 0) {
+    if constexpr (SUP != 0 && !NORESCUE) {
         _Pragma("unroll") for (int i = 0; i < SUP; ++i) {
             const float dx = ((float)(t.tx0 + i) - t.scx) * gp.frw;
             dxn[i] = dx * t.nxv;
@@ -1364,8 +1427,12 @@ __device__ __forceinline__ void guided_taps(const GuidedArgs &A, const PixAxis<S
             continue;
         const size_t row = (size_t)clampi(t.ty0 + j, 0, ch - 1) * cw;
         const float wy = qy.wt(j);
-        const float dy = ((float)(t.ty0 + j) - t.scy) * gp.frh;
-        const float dyn = dy * t.nyv, dyt = dy * t.nxv;
+        float dyn = 0.0f, dyt = 0.0f;
+        if constexpr (!NORESCUE) {
+            const float dy = ((float)(t.ty0 + j) - t.scy) * gp.frh;
+            dyn = dy * t.nyv;
+            dyt = dy * t.nxv;
+        }
         _Pragma("unroll") for (int i = 0; i < SX; ++i) {
             const bool on = onj && qx.on(i);
             if (!SUP && !on)
@@ -1374,7 +1441,13 @@ __device__ __forceinline__ void guided_taps(const GuidedArgs &A, const PixAxis<S
             const float wx = RADIAL ? radial_weight(A.rd, rbase, SX, i, j, t.tx0, t.ty0, t.scx, t.scy)
                                     : qx.wt(i) * wy;
             float w;
-            if (GUIDED) {
+            if (GUIDED && NORESCUE) {
+                const float dL = q.x - t.L0;
+                const float ys = 1.0f + dL * dL * t.inv_s2;
+                const float sim_raw = FAST ? rcp_nr(ys) : recip_ge1(ys);
+                const float sim = 1.0f - t.guide_fade * (1.0f - sim_raw);
+                w = sim * wx;
+            } else if (GUIDED) {
                 float dp, dt;
                 if constexpr (SUP != 0) {
                     dp = dxn[i] + dyn;
@@ -1598,8 +1671,16 @@ __device__ __attribute__((always_inline)) void guided_pixel(const GuidedArgs &A,
         fast = ysm < 0x1p120f && ybm < 0x1p120f;
     }
     TapAcc acc;
-    if (!guided)
+    // guide_fade == 0 makes sim exactly 1 and rescue 0, so every weight is
+    // exactly the base weight: the unguided sum (same order) gives the same
+    // accumulators.  In the flat interior of the dilated trust mask the
+    // mutual gate is 0 and whole waves take this path.  Both shortcuts need
+    // finite samples (a NaN would otherwise reach the weights through the
+    // similarity or the bump kernel).
+    if (!guided || (gp.finite && __all(guide_fade == 0.0f && rescue == 0.0f)))
         guided_taps<RADIAL, SUP, WIN, true, false>(A, px, py, ox, oy, ti, win, wx0, wy0, acc);
+    else if (gp.finite && __all(fast) && __all(rescue == 0.0f))
+        guided_taps<RADIAL, SUP, WIN, true, true, true>(A, px, py, ox, oy, ti, win, wx0, wy0, acc);
     else if (__all(fast))
         guided_taps<RADIAL, SUP, WIN, true, true>(A, px, py, ox, oy, ti, win, wx0, wy0, acc);
     else  // rare: exact reciprocals, weights from the tables, compact loop
@@ -2704,6 +2785,35 @@ __global__ __launch_bounds__(kBlock) void k_resample_radial(const float *src, in
             wsum += w;
         }
     }
+    dst[(size_t)oy * dw + ox] = (float)(wsum > 1e-6 ? acc / wsum : 0.0);
+}
+
+// resampleRadial with class tables: a block row holds output pixels of one
+// x class in one output row, so the pixel's weight table is uniform across
+// the wave and read through the scalar cache; each tap costs one source
+// load, one conversion and one FP64 FMA.  The sums run in the reference's
+// (j, i) order with the same weights, so results equal the per-tap path.
+template <int SUP>
+__global__ __launch_bounds__(kBlock) void k_resample_radial_tab(const float *__restrict__ src, int sw, int sh,
+                                                                float *__restrict__ dst, int dw,
+                                                                RadialResampleDev T) {
+    const int cls = blockIdx.z, oy = blockIdx.y;
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    const int beg = T.xoff[cls], cnt = T.xoff[cls + 1] - beg;
+    if (k >= cnt)
+        return;
+    const int S = SUP ? SUP : T.sup;
+    const int ox = T.xs[beg + k];
+    const int pair = (int)T.ycls[oy] * T.ncx + cls;
+    const double *__restrict__ wt = T.w + (size_t)pair * S * S;
+    const int x0 = T.x0[ox], y0 = T.y0[oy];
+    double acc = 0.0;
+    for (int j = 0; j < S; ++j) {
+        const float *row = src + (size_t)clampi(y0 + j, 0, sh - 1) * sw;
+        _Pragma("unroll") for (int i = 0; i < S; ++i)
+            acc = __builtin_fma(wt[j * S + i], (double)row[clampi(x0 + i, 0, sw - 1)], acc);
+    }
+    const double wsum = T.wsum[pair];
     dst[(size_t)oy * dw + ox] = (float)(wsum > 1e-6 ? acc / wsum : 0.0);
 }
 
@@ -3910,6 +4020,16 @@ hipError_t launch_guided(const float *u, const float *v, int cw, int ch, const f
 hipError_t launch_plain_radial(const float *u, const float *v, int cw, int ch, const AxisDev &ax,
                                const AxisDev &ay, const RadialDev &rd, float *out_u, float *out_v, int clamp_hull,
                                float margin, hipStream_t s) {
+    if (rd.valid && rd.max_count > 0) {
+        const dim3 g((unsigned)((rd.max_count + kBlock - 1) / kBlock), (unsigned)ay.n, (unsigned)rd.nx);
+        if (ax.sup == 6 && ay.sup == 6)
+            hipLaunchKernelGGL(k_plain_radial_tab<6>, g, dim3(kBlock), 0, s, u, v, cw, ch, ax, ay, rd, out_u, out_v,
+                               clamp_hull, margin);
+        else
+            hipLaunchKernelGGL(k_plain_radial_tab<0>, g, dim3(kBlock), 0, s, u, v, cw, ch, ax, ay, rd, out_u, out_v,
+                               clamp_hull, margin);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_plain_radial, rows_grid(ax.n, ay.n), dim3(kBlock), 0, s, u, v, cw, ch, ax, ay, rd, out_u,
                        out_v, clamp_hull, margin);
     return hipGetLastError();
@@ -4068,6 +4188,16 @@ hipError_t launch_resample_v(const float *src, int sw, int sh, float *dst, const
     return hipGetLastError();
 }
 
+hipError_t launch_resample_radial_tab(const float *src, int sw, int sh, float *dst, int dw, int dh,
+                                      const RadialResampleDev &t, hipStream_t s) {
+    const dim3 g((unsigned)((t.max_count + kBlock - 1) / kBlock), (unsigned)dh, (unsigned)t.ncx);
+    if (t.sup == 6)
+        hipLaunchKernelGGL(k_resample_radial_tab<6>, g, dim3(kBlock), 0, s, src, sw, sh, dst, dw, t);
+    else
+        hipLaunchKernelGGL(k_resample_radial_tab<0>, g, dim3(kBlock), 0, s, src, sw, sh, dst, dw, t);
+    return hipGetLastError();
+}
+
 hipError_t launch_resample_radial(const float *src, int sw, int sh, float *dst, int dw, int dh, double support,
                                   const float *lut, int lut_n, float lut_scale, hipStream_t s) {
     const double sx = (double)sw / dw, sy = (double)sh / dh;
@@ -4088,11 +4218,12 @@ hipError_t launch_sharpen(const float *src, float *dst, int w, int h, const floa
     return hipGetLastError();
 }
 
+// Every 8x8 output tile's taps fit a kTWin x kTWin chroma window.
 int guided_tiles_fit(const int *xs, int ow, const int *ys, int oh) {
-    auto fit = [](const int *st, int n) {
-        for (int t0 = 0; t0 < n; t0 += kTile) {
+    auto fit_rows = [](const int *st, int n, int rows) {
+        for (int t0 = 0; t0 < n; t0 += rows) {
             int lo = INT_MAX, hi = INT_MIN;
-            for (int c = 0; c < kTile; ++c) {
+            for (int c = 0; c < rows; ++c) {
                 const int v = st[std::min(t0 + c, n - 1)];
                 lo = std::min(lo, v);
                 hi = std::max(hi, v);
@@ -4102,7 +4233,7 @@ int guided_tiles_fit(const int *xs, int ow, const int *ys, int oh) {
         }
         return true;
     };
-    return fit(xs, ow) && fit(ys, oh);
+    return fit_rows(xs, ow, kTile) && fit_rows(ys, oh, kTile) ? 1 : 0;
 }
 
 size_t bm_scratch_bytes(int w, int h) {

@@ -350,9 +350,11 @@ void footprint(int cn, double r, double shift, int luma_n, std::vector<int> &b, 
 // block per phase pair.  Empty when the phase counts exceed the limits.
 struct HostRadial {
     bool valid = false;
-    int nx = 0, ny = 0;
+    int nx = 0, ny = 0, max_count = 0;
     std::vector<uint16_t> xphase, yphase;
     std::vector<float> w;
+    std::vector<int> xs, xoff;   // output columns of each x phase
+    std::vector<double> wsa;     // per phase pair: sum and absolute sum of the active weights
 };
 
 int collect_phases(const HostAxis &a, std::vector<uint16_t> &index, std::vector<float> &offsets,
@@ -391,6 +393,91 @@ float lut_lookup(const std::vector<float> &lut, float idx) {
     return lut[i0] + (lut[i0 + 1] - lut[i0]) * fr;
 }
 
+// Class tables of the radial luma resample (resampleRadial,
+// src/kernels.cpp:276-334).  Per axis, output coordinates whose tap offsets
+// float((start + i - centre) * invW) are the same floats form one class (at
+// most 64); the weights of a (y class, x class) pair are computed here with
+// the reference's float operations, so they equal the per-tap ones.  Returns
+// false (no tables) when an axis has more classes.
+struct HostRadialResample {
+    int ncx = 0, ncy = 0, sup = 0, max_count = 0;
+    std::vector<int> x0, y0, xs, xoff;
+    std::vector<uint8_t> ycls;
+    std::vector<double> w, wsum;
+};
+
+bool build_radial_resample(int sw, int sh, int dw, int dh, double support, const std::vector<float> &lut,
+                           HostRadialResample &r) {
+    const double sx = double(sw) / dw, sy = double(sh) / dh;
+    const double widen = std::max(1.0, std::max(sx, sy));
+    const float inv_w = float(1.0 / widen);
+    const int radius = int(std::ceil(support * widen));
+    const int sup = 2 * radius;
+    auto classify = [&](int n, double scale, std::vector<int> &start, std::vector<int> &cls,
+                        std::vector<std::vector<float>> &offs) {
+        std::map<std::vector<uint32_t>, int> seen;
+        start.resize(size_t(n));
+        cls.resize(size_t(n));
+        for (int o = 0; o < n; ++o) {
+            const double centre = (o + 0.5) * scale - 0.5;
+            const int s0 = int(std::ceil(centre - radius));
+            start[size_t(o)] = s0;
+            std::vector<float> d(static_cast<size_t>(sup));
+            std::vector<uint32_t> key(static_cast<size_t>(sup));
+            for (int i = 0; i < sup; ++i) {
+                d[size_t(i)] = float((s0 + i - centre) * inv_w);
+                std::memcpy(&key[size_t(i)], &d[size_t(i)], sizeof(float));
+            }
+            auto it = seen.find(key);
+            if (it == seen.end()) {
+                if (offs.size() == 64)
+                    return false;
+                it = seen.emplace(key, int(offs.size())).first;
+                offs.push_back(d);
+            }
+            cls[size_t(o)] = it->second;
+        }
+        return true;
+    };
+    std::vector<int> xc, yc;
+    std::vector<std::vector<float>> ox, oy;
+    if (!classify(dw, sx, r.x0, xc, ox) || !classify(dh, sy, r.y0, yc, oy))
+        return false;
+    r.ncx = int(ox.size());
+    r.ncy = int(oy.size());
+    r.sup = sup;
+    r.ycls.assign(yc.begin(), yc.end());
+    r.xoff.assign(size_t(r.ncx) + 1, 0);
+    for (int c : xc)
+        ++r.xoff[size_t(c) + 1];
+    for (int c = 0; c < r.ncx; ++c) {
+        r.max_count = std::max(r.max_count, r.xoff[size_t(c) + 1]);
+        r.xoff[size_t(c) + 1] += r.xoff[size_t(c)];
+    }
+    r.xs.resize(size_t(dw));
+    std::vector<int> fill(r.xoff.begin(), r.xoff.end() - 1);
+    for (int o = 0; o < dw; ++o)
+        r.xs[size_t(fill[size_t(xc[size_t(o)])]++)] = o;
+    r.w.resize(size_t(r.ncy) * r.ncx * sup * sup);
+    r.wsum.resize(size_t(r.ncy) * r.ncx);
+    for (int a = 0; a < r.ncy; ++a)
+        for (int b = 0; b < r.ncx; ++b) {
+            double wsum = 0.0;
+            double *wp = &r.w[(size_t(a) * r.ncx + b) * sup * sup];
+            for (int j = 0; j < sup; ++j) {
+                const float dy = oy[size_t(a)][size_t(j)];
+                for (int i = 0; i < sup; ++i) {
+                    const float dx = ox[size_t(b)][size_t(i)];
+                    const float w = lut_lookup(lut, std::sqrt(dx * dx + dy * dy) * 1024.0f);
+                    wp[j * sup + i] = double(w);
+                    wsum += w;
+                }
+            }
+            r.wsum[size_t(a) * r.ncx + b] = wsum;
+        }
+    return true;
+}
+
 HostRadial build_radial(const HostAxis &ax, const HostAxis &ay, int src_w, int src_h, const std::vector<float> &lut) {
     HostRadial r;
     std::vector<float> xo, yo;
@@ -418,6 +505,38 @@ HostRadial build_radial(const HostAxis &ax, const HostAxis &ay, int src_w, int s
                     const float dy = (yo[py] + float(j)) * inv_y;
                     w[size_t(j) * ax.sup + i] = lut_lookup(lut, std::sqrt(dx * dx + dy * dy) * 1024.0f);
                 }
+        }
+    // class lists of the phase-uniform plain pass, and each pair's weight
+    // sums over its active taps in the plain pass's (j, i) order
+    // (plainChromaImpl, recon.cpp:970-1005)
+    r.xoff.assign(size_t(nx) + 1, 0);
+    for (uint16_t q : r.xphase)
+        ++r.xoff[size_t(q) + 1];
+    for (int q = 0; q < nx; ++q) {
+        r.max_count = std::max(r.max_count, r.xoff[size_t(q) + 1]);
+        r.xoff[size_t(q) + 1] += r.xoff[size_t(q)];
+    }
+    r.xs.resize(r.xphase.size());
+    std::vector<int> fill(r.xoff.begin(), r.xoff.end() - 1);
+    for (size_t o = 0; o < r.xphase.size(); ++o)
+        r.xs[size_t(fill[r.xphase[o]]++)] = int(o);
+    r.wsa.assign(2 * pairs, 0.0);
+    for (int py = 0; py < ny; ++py)
+        for (int px = 0; px < nx; ++px) {
+            const float *w = &r.w[(size_t(py) * nx + px) * per];
+            double ws = 0.0, sa = 0.0;
+            for (int j = 0; j < ay.sup; ++j) {
+                if (!(ya[py] & (uint64_t(1) << j)))
+                    continue;
+                for (int i = 0; i < ax.sup; ++i) {
+                    if (!(xa[px] & (uint64_t(1) << i)))
+                        continue;
+                    ws += w[size_t(j) * ax.sup + i];
+                    sa += std::fabs(w[size_t(j) * ax.sup + i]);
+                }
+            }
+            r.wsa[2 * (size_t(py) * nx + px)] = ws;
+            r.wsa[2 * (size_t(py) * nx + px) + 1] = sa;
         }
     r.valid = true;
     return r;
@@ -657,6 +776,8 @@ struct lgcr_clip {
     size_t ws_bytes = 0;
     int max_inflight = 4;
     void *lut_dev = nullptr;
+    void *rrs_dev = nullptr;     // class tables of the radial luma resample
+    RadialResampleDev rrs;
     int lut_n = 0;
     std::vector<float> lut;
     void *weights_dev = nullptr;
@@ -732,6 +853,7 @@ lgcr_status get_geometry(lgcr_clip *c, double sx, double sy, Geometry **out) {
     if (c->p.radial)
         hr = build_radial(ax, ay, c->cw, c->ch, c->lut);
     const size_t orx = blob.add(hr.xphase), ory = blob.add(hr.yphase), orw = blob.add(hr.w);
+    const size_t orxs = blob.add(hr.xs), orxo = blob.add(hr.xoff), orwsa = blob.add(hr.wsa);
     const bool a6 = c->p.algo == 6 && c->p.strength > 0.0;
     HostWeights ycw[4];
     size_t oyc[4][2] = {};
@@ -764,6 +886,7 @@ lgcr_status get_geometry(lgcr_clip *c, double sx, double sy, Geometry **out) {
         }
     }
     const size_t osrx = blob.add(shr.xphase), osry = blob.add(shr.yphase), osrw = blob.add(shr.w);
+    const size_t osrxs = blob.add(shr.xs), osrxo = blob.add(shr.xoff), osrwsa = blob.add(shr.wsa);
     std::vector<TempTap> ttx, tty;
     std::vector<float> tsin;
     if (c->temporal) {
@@ -829,6 +952,10 @@ lgcr_status get_geometry(lgcr_clip *c, double sx, double sy, Geometry **out) {
     g->rd.xphase = reinterpret_cast<const uint16_t *>(base + orx);
     g->rd.yphase = reinterpret_cast<const uint16_t *>(base + ory);
     g->rd.w = reinterpret_cast<const float *>(base + orw);
+    g->rd.xs = reinterpret_cast<const int *>(base + orxs);
+    g->rd.xoff = reinterpret_cast<const int *>(base + orxo);
+    g->rd.wsa = reinterpret_cast<const double *>(base + orwsa);
+    g->rd.max_count = hr.max_count;
     g->rd.lut = static_cast<const float *>(c->lut_dev);
     g->rd.lut_n = c->lut_n;
     g->rd.lut_scale = 1024.0f;
@@ -870,6 +997,10 @@ lgcr_status get_geometry(lgcr_clip *c, double sx, double sy, Geometry **out) {
         g->srd.xphase = reinterpret_cast<const uint16_t *>(base + osrx);
         g->srd.yphase = reinterpret_cast<const uint16_t *>(base + osry);
         g->srd.w = reinterpret_cast<const float *>(base + osrw);
+        g->srd.xs = reinterpret_cast<const int *>(base + osrxs);
+        g->srd.xoff = reinterpret_cast<const int *>(base + osrxo);
+        g->srd.wsa = reinterpret_cast<const double *>(base + osrwsa);
+        g->srd.max_count = shr.max_count;
         g->srd.inv_wx = float(std::min(1.0, double(c->sw) / c->cw));
         g->srd.inv_wy = float(std::min(1.0, double(c->sh) / c->ch));
     } else {
@@ -1288,8 +1419,11 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
     // ResampleLuma (plugin.cpp:273-316)
     if (!c->same) {
         if (p.radial) {
-            HIP_TRY(launch_resample_radial(w->y, c->sw, c->sh, w->luma_out, c->ow, c->oh, p.support,
-                                           static_cast<const float *>(c->lut_dev), c->lut_n, 1024.0f, s));
+            if (c->rrs.valid)
+                HIP_TRY(launch_resample_radial_tab(w->y, c->sw, c->sh, w->luma_out, c->ow, c->oh, c->rrs, s));
+            else
+                HIP_TRY(launch_resample_radial(w->y, c->sw, c->sh, w->luma_out, c->ow, c->oh, p.support,
+                                               static_cast<const float *>(c->lut_dev), c->lut_n, 1024.0f, s));
         } else {
             HIP_TRY(launch_resample_h(w->y, c->sw, c->sh, w->luma_tmp, c->wh, s));
             HIP_TRY(launch_resample_v(w->luma_tmp, c->ow, c->sh, w->luma_out, c->wv, s));
@@ -1373,6 +1507,7 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
         gp.clamp_hull = clamp_hull;
         gp.ms_on = p.ms > 0.0;
         gp.radial = p.radial;
+        gp.finite = c->f.sample_type == 1 ? 0 : 1;
         const size_t nc = size_t(c->cw) * c->ch, no = size_t(c->ow) * c->oh;
         const uint32_t *idx = w->has_ws ? w->indices : nullptr;
         SelectorDev sel;
@@ -1462,8 +1597,11 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
         HIP_TRY(launch_nyquist(w->rec, w->nyq, c->sw, c->sh, levels(c->rw), levels(c->rh), &det, s));
         if (!c->same) {
             if (p.radial) {
-                HIP_TRY(launch_resample_radial(det, c->sw, c->sh, w->detail_out, c->ow, c->oh, p.support,
-                                               static_cast<const float *>(c->lut_dev), c->lut_n, 1024.0f, s));
+                if (c->rrs.valid)
+                    HIP_TRY(launch_resample_radial_tab(det, c->sw, c->sh, w->detail_out, c->ow, c->oh, c->rrs, s));
+                else
+                    HIP_TRY(launch_resample_radial(det, c->sw, c->sh, w->detail_out, c->ow, c->oh, p.support,
+                                                   static_cast<const float *>(c->lut_dev), c->lut_n, 1024.0f, s));
             } else {
                 HIP_TRY(launch_resample_h(det, c->sw, c->sh, w->luma_tmp, c->wh, s));
                 HIP_TRY(launch_resample_v(w->luma_tmp, c->ow, c->sh, w->detail_out, c->wv, s));
@@ -1673,6 +1811,7 @@ lgcr_status enqueue_trecon(lgcr_clip *c, Workspace *w, const Geometry &g, const 
     gp.clamp_hull = clamp_hull;
     gp.ms_on = guided && p.ms > 0.0;
     gp.radial = p.radial;
+    gp.finite = c->f.sample_type == 1 ? 0 : 1;
     TemporalDev td;
     td.nbrs = w->nbr_dev;
     td.n = nn;
@@ -1890,6 +2029,43 @@ static lgcr_status create_clip(lgcr_device *dev, const lgcr_params *params, cons
         HIP_TRY(hipMemcpy(c->lut_dev, lut.data(), sizeof(float) * n, hipMemcpyHostToDevice));
         c->lut_n = n;
         c->lut = lut;
+        HostRadialResample rr;
+        if (!c->same && build_radial_resample(c->sw, c->sh, c->ow, c->oh, p.support, lut, rr)) {
+            size_t off = 0;
+            auto place = [&off](size_t bytes) {
+                const size_t at = off;
+                off = (off + bytes + 255) & ~size_t(255);
+                return at;
+            };
+            const size_t a_x0 = place(rr.x0.size() * 4), a_y0 = place(rr.y0.size() * 4);
+            const size_t a_xs = place(rr.xs.size() * 4), a_xo = place(rr.xoff.size() * 4);
+            const size_t a_yc = place(rr.ycls.size()), a_w = place(rr.w.size() * 8);
+            const size_t a_ws = place(rr.wsum.size() * 8);
+            std::vector<uint8_t> blob(off);
+            std::memcpy(&blob[a_x0], rr.x0.data(), rr.x0.size() * 4);
+            std::memcpy(&blob[a_y0], rr.y0.data(), rr.y0.size() * 4);
+            std::memcpy(&blob[a_xs], rr.xs.data(), rr.xs.size() * 4);
+            std::memcpy(&blob[a_xo], rr.xoff.data(), rr.xoff.size() * 4);
+            std::memcpy(&blob[a_yc], rr.ycls.data(), rr.ycls.size());
+            std::memcpy(&blob[a_w], rr.w.data(), rr.w.size() * 8);
+            std::memcpy(&blob[a_ws], rr.wsum.data(), rr.wsum.size() * 8);
+            HIP_TRY(hipMalloc(&c->rrs_dev, blob.size()));
+            HIP_TRY(hipMemcpy(c->rrs_dev, blob.data(), blob.size(), hipMemcpyHostToDevice));
+            const uint8_t *b = static_cast<const uint8_t *>(c->rrs_dev);
+            RadialResampleDev &t = c->rrs;
+            t.valid = 1;
+            t.ncx = rr.ncx;
+            t.ncy = rr.ncy;
+            t.sup = rr.sup;
+            t.max_count = rr.max_count;
+            t.x0 = reinterpret_cast<const int *>(b + a_x0);
+            t.y0 = reinterpret_cast<const int *>(b + a_y0);
+            t.xs = reinterpret_cast<const int *>(b + a_xs);
+            t.xoff = reinterpret_cast<const int *>(b + a_xo);
+            t.ycls = b + a_yc;
+            t.w = reinterpret_cast<const double *>(b + a_w);
+            t.wsum = reinterpret_cast<const double *>(b + a_ws);
+        }
     }
     Geometry *g = nullptr;
     st = get_geometry(c.get(), p.shift_x, p.shift_y, &g);
@@ -2061,6 +2237,8 @@ void lgcr_clip_destroy(lgcr_clip *c) {
         hipFree(kv.second->dev);
     if (c->lut_dev)
         hipFree(c->lut_dev);
+    if (c->rrs_dev)
+        hipFree(c->rrs_dev);
     if (c->weights_dev)
         hipFree(c->weights_dev);
     if (c->sharpen_tab)

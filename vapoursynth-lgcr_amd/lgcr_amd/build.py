@@ -31,7 +31,7 @@ def build_hip(verbose=False):
 
 
 def build_synth():
-    cmd = ["gcc", "-O2", "-Wall", "-fPIC", "-shared", os.path.join(paths.CSRC_DIR, "synth.c"),
+    cmd = ["gcc", "-O3", "-Wall", "-fPIC", "-shared", "-pthread", os.path.join(paths.CSRC_DIR, "synth.c"),
            "-o", paths.SYNTH_LIB + ".tmp", "-lm"]
     subprocess.run(cmd, check=True)
     os.replace(paths.SYNTH_LIB + ".tmp", paths.SYNTH_LIB)

@@ -99,7 +99,7 @@ class ResidentClip:
     """A clip's source frames in HBM, its output buffers and its backend
     state (created on admission)."""
 
-    def __init__(self, device, spec: ClipSpec, max_inflight=4, threads=8, graphs=False):
+    def __init__(self, device, spec: ClipSpec, max_inflight=4, threads=8, graphs=False, keep_host=True):
         import torch
         self.spec = spec
         self.graphs = graphs  # Recon frames replay captured frame graphs
@@ -112,7 +112,9 @@ class ResidentClip:
                             motion=True, threads=threads)
         view = (lambda a: a.view(np.int16)) if self.bps == 2 else (lambda a: a)
         self.src = [[torch.from_numpy(view(p)).to("cuda") for p in f] for f in frames]
-        self.host_src = frames
+        # host copies serve output_host (the shared Recon luma) and checks;
+        # keep_host=False drops them once the frames are resident
+        self.host_src = frames if keep_host else None
         self.temporal = spec.kind == "trecon"
         self.max_inflight = max_inflight
         dt = torch.uint8 if self.bps == 1 else torch.int16
