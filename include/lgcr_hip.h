@@ -116,6 +116,11 @@ typedef struct lgcr_stats {
     uint64_t active_output_pixels;   /* sparse workset size */
     uint64_t active_chroma_pixels;
     uint64_t sparse_total_pixels;
+    /* taps the reference's stages visit for this frame (PipelineMetrics::taps
+     * and tapsVisited, src/pipeline.h:315-336): the plain base
+     * (recon.cpp:1144-1147, 1453-1459) and the guided pass (recon.cpp:253-257) */
+    uint64_t stage_taps[LGCR_STAGE_COUNT];
+    uint64_t taps_visited;
 } lgcr_stats;
 
 typedef struct lgcr_device lgcr_device;
@@ -203,6 +208,12 @@ lgcr_status lgcr_ticket_query(const lgcr_ticket *ticket);
 /* Block until the frame is complete, return its workspace to the clip and
  * free the ticket. */
 lgcr_status lgcr_ticket_wait(lgcr_ticket *ticket);
+/* No reference counterpart (serving loops): wait up to timeout_us
+ * microseconds for any of n tickets to complete, spinning on their
+ * completion events.  LGCR_OK with *index set to a completed ticket,
+ * LGCR_BUSY on timeout (timeout_us <= 0 checks once).  Tickets stay owned
+ * by the caller (retire them with lgcr_ticket_wait). */
+lgcr_status lgcr_ticket_wait_any(const lgcr_ticket *const *tickets, int n, double timeout_us, int *index);
 
 /* Sharpen (sharpenCreate / sharpenGetFrame, src/plugin.cpp:14-111): the
  * fields of SharpenData (src/lgcr.h:968-976).  The clip keeps its input

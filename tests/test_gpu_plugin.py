@@ -319,3 +319,61 @@ def test_trecon_and_sharpen_match_oracle(device):
         cnt, mx = compare_planes(ref[p], sharp[p], False)
         assert cnt == 0, f"Sharpen plane {p}: mismatches={cnt} max_abs={mx:g}"
     assert sharp.format == fmt
+
+
+def _oracle_axis(src_n, dst_n, r, shift, params):
+    import ctypes as C
+    import oracle
+    a = oracle.Axis()
+    assert oracle.lib().or_build_axis(src_n, dst_n, r, shift, C.byref(params), C.byref(a)) == 0
+    act = np.ctypeslib.as_array(a.active, shape=(dst_n,)).astype(np.int64).copy()
+    start = np.ctypeslib.as_array(a.start, shape=(dst_n,)).copy()
+    am = np.ctypeslib.as_array(a.am, shape=(dst_n * a.sup,)).reshape(dst_n, a.sup).copy()
+    sup = a.sup
+    oracle.lib().or_free_axis(C.byref(a))
+    return act, start, am, sup
+
+
+@pytest.mark.parametrize("sparse", [1, 0])
+def test_profile_tap_counts(device, monkeypatch, sparse):
+    """_LGCR_plainChroma_taps, _LGCR_reconstructChroma_taps and
+    _LGCR_taps_visited count what the reference visits: the separable plain
+    base computes each chroma row through a ring of `sup` rows and visits
+    the active horizontal taps per computed row plus the active vertical
+    taps per output pixel (recon.cpp:1453-1459); the guided pass visits
+    activeX * activeY taps per workset pixel (recon.cpp:253-257), here taken
+    from the oracle's workset and axis tables."""
+    import oracle
+    from lgcr_amd import synth
+    monkeypatch.setenv("LGCR_PROFILE", "1")
+    w, h = 128, 72
+    y, u, v = synth.clip_frame(w, h, bits=10, seed=4, polygons=10)
+    fmt = query_video_format(YUV, INTEGER, 10, 1, 1)
+    src = clip_from_planes([(y, u, v)], fmt)
+    f = core.lgcr.Recon(src, sparse=sparse).get_frame(0)
+    params = oracle.make_params(w, h, sparse=sparse)
+    oracle.recon_frame(params, y, u, v, bits=10)
+    dbg = oracle.last_debug()
+    ax, _, _, _ = _oracle_axis(w, w, 2.0, -0.5, params)
+    ay, ystart, yam, ysup = _oracle_axis(h, h, 2.0, 0.0, params)
+    ch = h // 2
+    ring, cached, rows = min(ch, ysup), {}, 0
+    for oy in range(h):
+        for j in range(ysup):
+            if yam[oy, j] == 0.0:
+                continue
+            s = min(max(int(ystart[oy]) + j, 0), ch - 1)
+            if cached.get(s % ring) != s:
+                cached[s % ring] = s
+                rows += 1
+    plain = rows * int(ax.sum()) + w * int(ay.sum())
+    if sparse:
+        idx = dbg["indices"].astype(np.int64)
+        guided = int((ax[idx % w] * ay[idx // w]).sum())
+        assert 0 < guided < int(ax.sum()) * int(ay.sum())
+    else:
+        guided = int(ax.sum()) * int(ay.sum())
+    assert f.props["_LGCR_plainChroma_taps"] == plain
+    assert f.props["_LGCR_reconstructChroma_taps"] == guided
+    assert f.props["_LGCR_taps_visited"] == plain + guided
+    assert f.props["_LGCR_cpu_guided_tap_accumulation_us"] == 0.0

@@ -5,7 +5,7 @@ import time
 
 import pytest
 
-from lgcr_amd import serve
+from lgcr_amd import abi, serve
 
 
 def test_mix_and_poisson_requests():
@@ -144,3 +144,65 @@ def test_clip_larger_than_budget_raises(mode):
     reqs = serve.poisson_requests(specs, float("inf"))
     with pytest.raises(RuntimeError, match="does not fit the device budget alone"):
         serve.serve(clips, reqs, mode=mode)
+
+
+class WaitingClip(FakeClip):
+    """FakeClip with a native-style wait: sleeps until the first of the
+    tickets completes or the timeout passes."""
+    waits = []
+
+    @staticmethod
+    def wait_any(tickets, timeout_s):
+        WaitingClip.waits.append(timeout_s)
+        owner = {t: c for c in WaitingClip.all for t in c.running}
+        ends = [owner[t].running[t] for t in tickets]
+        time.sleep(max(0.0, min(min(ends) - time.perf_counter(), timeout_s)))
+
+
+def test_continuous_uses_clip_wait_any():
+    """The continuous loop blocks in the clips' wait_any (bounded by the next
+    arrival) instead of napping, and still serves every frame in order."""
+    dev = FakeDevice(budget=10)
+    WaitingClip.waits = []
+    clips = [WaitingClip(dev, 3, slots=2) for _ in range(3)]
+    WaitingClip.all = clips
+    reqs = [serve.Request(0.001 * k, k % 3, k // 3) for k in range(12)]
+    keep = {}
+    st = serve.serve(clips, reqs, mode="continuous", keep=keep)
+    assert st["frames"] == 12 and len(keep) == 12
+    assert WaitingClip.waits and all(0.0 <= w <= 5e-3 for w in WaitingClip.waits)
+    for c in clips:
+        assert c.order == sorted(c.order)
+
+
+class FailingClip(FakeClip):
+    """FakeClip whose backend call fails for the frames in `bad`."""
+
+    def __init__(self, *a, bad=(), **k):
+        super().__init__(*a, **k)
+        self.bad = set(bad)
+
+    def submit(self, n):
+        if n in self.bad:
+            raise abi.LgcrError(abi.LGCR_BACKEND_FAILURE, "kernel launch failed")
+        return super().submit(n)
+
+    def run_blocking(self, n, slot=0):
+        if n in self.bad:
+            raise abi.LgcrError(abi.LGCR_BACKEND_FAILURE, "kernel launch failed")
+        super().run_blocking(n, slot)
+
+
+@pytest.mark.parametrize("mode", ["continuous", "per_frame"])
+def test_failed_frame_does_not_stop_serving(mode):
+    """A frame whose backend call fails is reported and skipped; the clip's
+    later frames and the other clips are still served."""
+    dev = FakeDevice(budget=10)
+    clips = [FailingClip(dev, 3, slots=2, bad={1}), FakeClip(dev, 3, slots=2)]
+    reqs = [serve.Request(0.0005 * k, k % 2, k // 2) for k in range(8)]
+    keep, failures = {}, []
+    st = serve.serve(clips, reqs, mode=mode, keep=keep, failures=failures)
+    assert st["frames"] == 7 and st["failed"] == 1
+    assert [(r.clip, r.frame) for r, _ in failures] == [(0, 1)]
+    assert "kernel launch failed" in failures[0][1]
+    assert set(keep) == {(c, f) for c in (0, 1) for f in range(4)} - {(0, 1)}

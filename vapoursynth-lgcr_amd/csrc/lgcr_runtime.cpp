@@ -10,6 +10,7 @@
 #include <cstddef>
 #include <functional>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <condition_variable>
 #include <cstdarg>
@@ -38,6 +39,43 @@ lgcr_status fail(lgcr_status st, const char *fmt, ...) {
     g_last_error = buf;
     return st;
 }
+
+// Failure injection for tests of the error paths.  LGCR_INJECT_FAULT set to
+// "<kind>:<n>" makes the n-th check of that kind since the variable took its
+// current value fail, once: "alloc" checks precede a clip workspace's
+// allocation, "launch" checks precede a frame's first kernel launch.  The
+// variable is read at every check, so setting, changing or clearing it
+// re-arms the hook.
+bool inject_fault(const char *kind) {
+    static std::atomic<bool> seen{false};
+    static std::mutex m;
+    static std::string armed;
+    static long count = 0;
+    const char *v = std::getenv("LGCR_INJECT_FAULT");
+    if ((!v || !*v) && !seen.load(std::memory_order_relaxed))
+        return false;
+    std::lock_guard<std::mutex> lock(m);
+    if (!v || !*v) {
+        armed.clear();
+        seen.store(false, std::memory_order_relaxed);
+        return false;
+    }
+    seen.store(true, std::memory_order_relaxed);
+    if (armed != v) {
+        armed = v;
+        count = 0;
+    }
+    const size_t kl = std::strlen(kind);
+    if (std::strncmp(v, kind, kl) != 0 || v[kl] != ':')
+        return false;
+    return ++count == std::strtol(v + kl + 1, nullptr, 10);
+}
+
+#define INJECT_LAUNCH_FAULT()                                                                 \
+    do {                                                                                      \
+        if (inject_fault("launch"))                                                           \
+            return fail(LGCR_BACKEND_FAILURE, "kernel launch failed: injected by LGCR_INJECT_FAULT"); \
+    } while (0)
 
 #define HIP_TRY(expr)                                                                       \
     do {                                                                                    \
@@ -659,7 +697,42 @@ struct Geometry {
     // TRecon temporal taps per (output column / row, vector)
     const TempTap *ttx = nullptr, *tty = nullptr;
     const float *tsin = nullptr;
+    // work accounting of the reference's profile: active taps per output
+    // column / row and the plain base's taps per frame
+    std::vector<uint8_t> act_x, act_y;
+    uint64_t plain_taps = 0;
 };
+
+// Taps of plainChroma (recon.cpp:1144-1147, 1453-1459): the separable pass
+// computes each chroma row its vertical taps need through a ring of
+// min(ch, sup) rows, in output-row order, and visits the active horizontal
+// taps of every output column per computed row plus the active vertical taps
+// of every output pixel; the radial and bilinear bases visit activeX x activeY.
+uint64_t plain_base_taps(const HostAxis &ax, const HostAxis &ay, int ch, bool separable) {
+    uint64_t sx = 0, sy = 0;
+    for (int o = 0; o < ax.n; ++o)
+        for (int i = 0; i < ax.sup; ++i)
+            sx += ax.am[size_t(o) * ax.sup + i] != 0.0f;
+    for (int o = 0; o < ay.n; ++o)
+        for (int j = 0; j < ay.sup; ++j)
+            sy += ay.am[size_t(o) * ay.sup + j] != 0.0f;
+    if (!separable)
+        return sx * sy;
+    const int ring = std::min(ch, std::max(1, ay.sup));
+    std::vector<int> cached(size_t(ring), -1);
+    uint64_t rows = 0;
+    for (int oy = 0; oy < ay.n; ++oy)
+        for (int j = 0; j < ay.sup; ++j) {
+            if (ay.am[size_t(oy) * ay.sup + j] == 0.0f)
+                continue;
+            const int src = std::clamp(ay.start[size_t(oy)] + j, 0, ch - 1);
+            if (cached[size_t(src % ring)] != src) {
+                cached[size_t(src % ring)] = src;
+                ++rows;
+            }
+        }
+    return rows * sx + uint64_t(ax.n) * sy;
+}
 
 // Temporal taps of one axis for every output position and every integer
 // vector v in [-search, search], evaluated exactly as the per-pixel code of
@@ -899,6 +972,16 @@ lgcr_status get_geometry(lgcr_clip *c, double sx, double sy, Geometry **out) {
     g->sx = sx;
     g->sy = sy;
     g->tiles_fit = guided_tiles_fit(ax.start.data(), ax.n, ay.start.data(), ay.n);
+    auto active_counts = [](const HostAxis &a) {
+        std::vector<uint8_t> n(size_t(a.n), 0);
+        for (int o = 0; o < a.n; ++o)
+            for (int t = 0; t < a.sup; ++t)
+                n[size_t(o)] += a.am[size_t(o) * a.sup + t] != 0.0f;
+        return n;
+    };
+    g->act_x = active_counts(ax);
+    g->act_y = active_counts(ay);
+    g->plain_taps = plain_base_taps(ax, ay, c->ch, !c->p.radial && c->p.kernel != LGCR_KERNEL_BILINEAR);
     HIP_TRY(hipSetDevice(c->dev->ordinal));
     HIP_TRY(hipMalloc(&g->dev, blob.host.size()));
     HIP_TRY(hipMemcpy(g->dev, blob.host.data(), blob.host.size(), hipMemcpyHostToDevice));
@@ -1144,7 +1227,7 @@ lgcr_status create_workspace(lgcr_clip *c, std::unique_ptr<Workspace> &out) {
                     bytes, (unsigned long long)c->dev->reserved.load(), (unsigned long long)c->dev->budget);
     hipError_t e = hipSetDevice(c->dev->ordinal);
     if (e == hipSuccess)
-        e = hipMalloc(&w->base, bytes);
+        e = inject_fault("alloc") ? hipErrorOutOfMemory : hipMalloc(&w->base, bytes);
     if (e != hipSuccess) {
         c->dev->unreserve(bytes);
         return fail(LGCR_OUT_OF_MEMORY, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
@@ -1379,6 +1462,7 @@ hipError_t trust_workset(lgcr_clip *c, Workspace *w, const Geometry &g, bool til
 lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const void *const in[3],
                           const ptrdiff_t in_stride[3], void *const out[3], const ptrdiff_t out_stride[3],
                           bool timed, IoTable *io = nullptr) {
+    INJECT_LAUNCH_FAULT();
     const lgcr_params &p = c->p;
     hipStream_t s = w->stream;
     auto mark = [&](int stage) -> hipError_t {
@@ -1655,6 +1739,7 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
 // Not used while stage timing or stage capture is on.
 lgcr_status enqueue_frames_graph(lgcr_clip *c, Workspace *w, const Geometry &g, const lgcr_frame *in,
                                  const lgcr_frame *out, const int *idx, int m) {
+    INJECT_LAUNCH_FAULT();
     hipStream_t s = w->stream;
     if (m > kIoSlots)
         return fail(LGCR_INVALID_ARGUMENT, "enqueue_frames_graph: %d frames exceed the frame table", m);
@@ -1717,6 +1802,7 @@ lgcr_status enqueue_trecon(lgcr_clip *c, Workspace *w, const Geometry &g, const 
                            const ptrdiff_t in_stride[3], int n_nbrs, const void *const *nbr_data,
                            const ptrdiff_t *nbr_stride, void *const out[3], const ptrdiff_t out_stride[3],
                            bool timed) {
+    INJECT_LAUNCH_FAULT();
     const lgcr_params &p = c->p;
     hipStream_t s = w->stream;
     auto mark = [&](int stage) -> hipError_t {
@@ -2376,6 +2462,43 @@ lgcr_status lgcr_recon_frame(lgcr_clip *c, const lgcr_frame *in, double shift_x,
             stats->active_chroma_pixels = cnt[0];
             stats->sparse_total_pixels = uint64_t(c->sw) * c->sh;
         }
+        // taps of the plain base and of the guided pass (algo 2 and 4 run
+        // reconstructChroma over the workset, or every output without one)
+        stats->stage_taps[LGCR_STAGE_BUILD_BASE_CHROMA] = g->plain_taps;
+        uint64_t guided_taps = 0;
+        if (c->p.strength > 0.0 && c->p.algo != 6) {
+            const size_t ow = size_t(c->ow);
+            if (!w->has_ws) {
+                uint64_t sx = 0, sy = 0;
+                for (uint8_t a : g->act_x)
+                    sx += a;
+                for (uint8_t a : g->act_y)
+                    sy += a;
+                guided_taps = sx * sy;
+            } else if (w->ws_indices) {
+                uint32_t n = 0;
+                HIP_TRY(hipMemcpy(&n, w->row_off + c->oh, sizeof(n), hipMemcpyDeviceToHost));
+                std::vector<uint32_t> idx(n);
+                if (n)
+                    HIP_TRY(hipMemcpy(idx.data(), w->indices, size_t(n) * 4, hipMemcpyDeviceToHost));
+                for (uint32_t k : idx)
+                    guided_taps += uint64_t(g->act_x[k % ow]) * g->act_y[k / ow];
+            } else {
+                // tiled pass at output size == luma size: the mask bits
+                const size_t w64 = size_t((c->sw + 63) / 64);
+                std::vector<uint64_t> bits(w64 * size_t(c->sh));
+                HIP_TRY(hipMemcpy(bits.data(), w->mask, bits.size() * 8, hipMemcpyDeviceToHost));
+                for (int y = 0; y < c->sh; ++y) {
+                    uint64_t row = 0;
+                    for (size_t q = 0; q < w64; ++q)
+                        for (uint64_t b = bits[size_t(y) * w64 + q]; b; b &= b - 1)
+                            row += g->act_x[q * 64 + size_t(__builtin_ctzll(b))];
+                    guided_taps += row * g->act_y[size_t(y)];
+                }
+            }
+        }
+        stats->stage_taps[LGCR_STAGE_APPLY_GUIDED_CORRECTION] = guided_taps;
+        stats->taps_visited = g->plain_taps + guided_taps;
     }
     return LGCR_OK;
 }
@@ -2612,6 +2735,30 @@ lgcr_status lgcr_ticket_wait(lgcr_ticket *t) {
     return LGCR_OK;
 }
 
+lgcr_status lgcr_ticket_wait_any(const lgcr_ticket *const *tickets, int n, double timeout_us, int *index) {
+    if (!tickets || n <= 0 || !index)
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_ticket_wait_any: bad arguments");
+    for (int k = 0; k < n; ++k)
+        if (!tickets[k])
+            return fail(LGCR_INVALID_ARGUMENT, "lgcr_ticket_wait_any: NULL ticket %d", k);
+    const auto t0 = std::chrono::steady_clock::now();
+    for (;;) {
+        for (int k = 0; k < n; ++k) {
+            const hipError_t e = hipEventQuery(tickets[k]->clip->spaces[tickets[k]->slot]->done);
+            if (e == hipSuccess) {
+                *index = k;
+                return LGCR_OK;
+            }
+            if (e != hipErrorNotReady)
+                return fail(LGCR_BACKEND_FAILURE, "hipEventQuery failed: %s", hipGetErrorString(e));
+        }
+        const double waited =
+            std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+        if (waited >= timeout_us)
+            return LGCR_BUSY;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Sharpen (sharpenCreate / sharpenGetFrame, src/plugin.cpp:14-111)
 // ---------------------------------------------------------------------------
@@ -2620,6 +2767,7 @@ namespace {
 
 lgcr_status enqueue_sharpen(lgcr_clip *c, Workspace *w, const void *const in[3], const ptrdiff_t in_stride[3],
                             void *const out[3], const ptrdiff_t out_stride[3]) {
+    INJECT_LAUNCH_FAULT();
     hipStream_t s = w->stream;
     const bool is_float = c->f.sample_type == 1;
     const double yscale = is_float ? 1.0 : 1.0 / double((1 << c->f.bits_per_sample) - 1);
@@ -2833,6 +2981,7 @@ HostWeights downsample_weights(int src_n, int kernel, float shift) {
 
 lgcr_status enqueue_downsample(lgcr_clip *c, Workspace *w, const void *const in[3], const ptrdiff_t in_stride[3],
                                void *const out[3], const ptrdiff_t out_stride[3], bool timed) {
+    INJECT_LAUNCH_FAULT();
     hipStream_t s = w->stream;
     auto mark = [&](int stage) -> hipError_t {
         if (!timed)

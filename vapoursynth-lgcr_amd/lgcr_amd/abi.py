@@ -36,7 +36,7 @@ SYMBOLS = [
     "lgcr_recon_submit", "lgcr_trecon_submit", "lgcr_ticket_query", "lgcr_ticket_wait",
     "lgcr_clip_set_graphs", "lgcr_clip_get_tuning", "lgcr_clip_set_tuning", "lgcr_clip_autotune",
     "lgcr_downsample_params_default", "lgcr_downsample_create", "lgcr_downsample_frame",
-    "lgcr_downsample_frame_noalloc", "lgcr_check_rcp",
+    "lgcr_ticket_wait_any", "lgcr_downsample_frame_noalloc", "lgcr_check_rcp",
 ]
 STAGE_NAMES = [
     "ConvertInput", "ResampleLuma", "BuildGuideMaps", "BuildTrustMask", "BuildMutualGate",
@@ -92,7 +92,8 @@ class Stats(C.Structure):
     _fields_ = [("stage_us", C.c_double * STAGE_COUNT), ("stage_pixels", C.c_uint64 * STAGE_COUNT),
                 ("total_us", C.c_double), ("output_pixels", C.c_uint64),
                 ("active_output_pixels", C.c_uint64), ("active_chroma_pixels", C.c_uint64),
-                ("sparse_total_pixels", C.c_uint64)]
+                ("sparse_total_pixels", C.c_uint64), ("stage_taps", C.c_uint64 * STAGE_COUNT),
+                ("taps_visited", C.c_uint64)]
 
 
 class LgcrError(RuntimeError):
@@ -150,6 +151,7 @@ def lib():
                                          C.POINTER(Frame), C.POINTER(vp)]
         L.lgcr_ticket_query.argtypes = [vp]
         L.lgcr_ticket_wait.argtypes = [vp]
+        L.lgcr_ticket_wait_any.argtypes = [C.POINTER(vp), C.c_int, C.c_double, C.POINTER(C.c_int)]
         L.lgcr_sharpen_params_default.argtypes = [C.POINTER(SharpenParams)]
         L.lgcr_sharpen_create.argtypes = [vp, C.POINTER(SharpenParams), C.POINTER(Format), C.c_int, C.POINTER(vp)]
         L.lgcr_sharpen_frame.argtypes = [vp, C.POINTER(Frame), C.POINTER(Frame)]

@@ -38,6 +38,19 @@ def build_synth():
     return paths.SYNTH_LIB
 
 
+def build_vsplugin():
+    """The VapourSynth 4 plugin glue (liblgcr_vs.so, next to liblgcr_hip.so,
+    which it links with an $ORIGIN rpath).  It holds no device code."""
+    inc = os.path.join(os.path.dirname(paths.ROOT_DIR), "include")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-fPIC", "-shared", f"-I{os.path.join(paths.CSRC_DIR, 'vs4')}",
+           f"-I{inc}", os.path.join(paths.CSRC_DIR, "lgcr_vsplugin.cpp"), f"-L{paths.PKG_DIR}", "-llgcr_hip",
+           "-Wl,-rpath,$ORIGIN", "-o", paths.VS_PLUGIN + ".tmp"]
+    subprocess.run(cmd, check=True)
+    os.replace(paths.VS_PLUGIN + ".tmp", paths.VS_PLUGIN)
+    return paths.VS_PLUGIN
+
+
 def build_all(verbose=False):
     build_synth()
     build_hip(verbose)
+    build_vsplugin()

@@ -6,3 +6,4 @@ ROOT_DIR = os.path.dirname(PKG_DIR)
 CSRC_DIR = os.path.join(ROOT_DIR, "csrc")
 HIP_LIB = os.path.join(PKG_DIR, "liblgcr_hip.so")
 SYNTH_LIB = os.path.join(PKG_DIR, "liblgcr_synth.so")
+VS_PLUGIN = os.path.join(PKG_DIR, "liblgcr_vs.so")   # VapourSynth 4 plugin (VapourSynthPluginInit2)

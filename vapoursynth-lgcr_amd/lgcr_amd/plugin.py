@@ -17,6 +17,7 @@ same-size Recon (plugin.cpp:265-269), ``_LGCR_*`` profile props when
 """
 from __future__ import annotations
 
+import contextlib
 import math
 import os
 import threading
@@ -73,6 +74,17 @@ class VideoFrame:
         return self.planes[p]
 
 
+@contextlib.contextmanager
+def _frame_error(name):
+    """A backend failure while producing a frame is that frame's error
+    (vsapi->setFilterError), prefixed with the filter name; the clip and the
+    process stay usable."""
+    try:
+        yield
+    except abi.LgcrError as e:
+        raise Error(f"{name}: {e}") from e
+
+
 class VideoNode:
     """A clip: constant format and size, frames produced on request."""
 
@@ -111,6 +123,13 @@ def _check_args(fn, given, allowed):
 def _get(args, key, default):
     v = args.get(key)
     return default if v is None else v
+
+
+def _getf(args, key, default):
+    """A float argument read like the reference reads it: a given value
+    narrowed to float (mapGetFloatSaturated), the double default otherwise."""
+    v = args.get(key)
+    return float(default) if v is None else float(np.float32(v))
 
 
 def _device():
@@ -160,6 +179,15 @@ PROP_STAGE_NAMES = ["input_conversion", "luma_resample", "buildGuideMaps", "buil
                     "output_conversion"]
 
 
+# CPU sub-stage timer names (cpuProfileSlotName, src/pipeline.h:282-312)
+CPU_SLOTS = ["guide_sobel", "guide_tensor", "guide_lcMap", "affine_candidate_box", "affine_candidate_bilinear",
+             "affine_candidate_bicubic", "affine_minmax_u", "affine_minmax_v", "affine_window_moments",
+             "affine_finalize_median", "guided_active_rows", "guided_metadata", "guided_tap_accumulation",
+             "guided_normalization_selector", "trust_seed", "trust_dilate", "mutual_gradients", "mutual_gate",
+             "plain_horizontal", "plain_vertical", "lgf_moments", "lgf_finalize", "affine_rolling",
+             "affine_consume", "detail_reconstruct", "detail_nyquist", "detail_transfer"]
+
+
 def _profile_props(props, st, fmt, sw, sh, ow, oh, bm):
     """writeProfile (plugin.cpp:126-155) from the backend's per-frame stats.
     Conversion work follows the reference's counting (plugin.cpp:250-253,
@@ -173,10 +201,14 @@ def _profile_props(props, st, fmt, sw, sh, ow, oh, bm):
     for i, name in enumerate(PROP_STAGE_NAMES):
         props[f"_LGCR_{name}_us"] = float(st.stage_us[i])
         props[f"_LGCR_{name}_pixels"] = pixels[i]
-        props[f"_LGCR_{name}_taps"] = 0
+        props[f"_LGCR_{name}_taps"] = int(st.stage_taps[i])
+    for slot in CPU_SLOTS:
+        # CPU time of the reference's sub-stage timers: the backend computes
+        # on the device, so none is spent on the host
+        props[f"_LGCR_cpu_{slot}_us"] = 0.0
     props["_LGCR_total_us"] = float(st.total_us)
     props["_LGCR_output_pixels"] = int(st.output_pixels)
-    props["_LGCR_taps_visited"] = 0
+    props["_LGCR_taps_visited"] = int(st.taps_visited)
     props["_LGCR_sparse_active_ratio"] = (float(st.active_output_pixels) / st.sparse_total_pixels
                                           if st.sparse_total_pixels else 1.0)
 
@@ -215,8 +247,8 @@ def recon_params(clip: VideoNode, **args):
         p.kernel, p.support = abi.KERNELS["bilinear"], 1.0
     elif k == "bicubic":
         p.kernel, p.support = abi.KERNELS["bicubic"], 2.0
-        p.kp1 = float(_get(args, "b", 0.0))
-        p.kp2 = float(_get(args, "c", 0.6))
+        p.kp1 = _getf(args, "b", 0.0)
+        p.kp2 = _getf(args, "c", 0.6)
     elif k in ("lanczos", "jinc"):
         taps = int(_get(args, "taps", 3))
         if taps < 1 or taps > 64:
@@ -231,22 +263,22 @@ def recon_params(clip: VideoNode, **args):
         p.kernel, p.support = abi.KERNELS["spline36"], 3.0
     else:
         raise Error('LGCR: kernel must be "bilinear", "bicubic", "lanczos", "spline16", "spline36" or "jinc"')
-    p.strength = float(_get(args, "strength", 0.8))
-    p.sigma = float(_get(args, "sigma", 0.01))
-    p.sratio = float(_get(args, "sratio", 0.15))
+    p.strength = _getf(args, "strength", 0.8)
+    p.sigma = _getf(args, "sigma", 0.01)
+    p.sratio = _getf(args, "sratio", 0.15)
     p.algo = int(_get(args, "algo", 2))
     if p.algo not in (2, 4, 6):
         raise Error("LGCR: algo must be 2 (guided), 4 (selector) or 6 (detail transfer)")
-    p.sdb = float(_get(args, "sdb", 3.0))
-    p.stretch = float(_get(args, "stretch", 1.0))
-    p.gsigma = float(_get(args, "gsigma", 2.5))
-    p.rescue = float(_get(args, "rescue", 1.0))
+    p.sdb = _getf(args, "sdb", 3.0)
+    p.stretch = _getf(args, "stretch", 1.0)
+    p.gsigma = _getf(args, "gsigma", 2.5)
+    p.rescue = _getf(args, "rescue", 1.0)
     p.ridge = 1 if int(_get(args, "ridge", 1)) != 0 else 0
     p.cedge = 1 if int(_get(args, "cedge", 0)) != 0 else 0
-    p.ms = float(_get(args, "ms", 1.0))
-    p.qgate = float(_get(args, "qgate", 1.0))
-    p.ar_margin = float(_get(args, "ar", 0.0))
-    p.reg = float(_get(args, "reg", 0.005))
+    p.ms = _getf(args, "ms", 1.0)
+    p.qgate = _getf(args, "qgate", 1.0)
+    p.ar_margin = _getf(args, "ar", 0.0)
+    p.reg = _getf(args, "reg", 0.005)
     loc = args.get("loc")
     p.shift_x, p.shift_y, p.loc_set = -0.5, 0.0, 0
     if loc is not None:
@@ -261,7 +293,7 @@ def recon_params(clip: VideoNode, **args):
         p.loc_set = 1
     p.sparse = 1 if int(_get(args, "sparse", 1)) != 0 else 0
     p.bm = 1 if int(_get(args, "bm", 0)) != 0 else 0
-    p.bp = float(_get(args, "bp", 0.0))
+    p.bp = _getf(args, "bp", 0.0)
     if fmt.subsampling_w == 0:
         p.shift_x = 0.0
     if fmt.subsampling_h == 0:
@@ -287,7 +319,8 @@ def Recon(clip: VideoNode, **args) -> VideoNode:
         src = clip.get_frame(n)
         sx, sy = _siting(src.props, fmt_in, shift_x, shift_y, loc_set)
         profile = _profiling()
-        res = backend.recon_host(*src.planes, shift_x=sx, shift_y=sy, stats=profile, share_luma=same)
+        with _frame_error("LGCR"):
+            res = backend.recon_host(*src.planes, shift_x=sx, shift_y=sy, stats=profile, share_luma=same)
         out, st = res if profile else (res, None)
         props = dict(src.props)
         props.pop("_ChromaLocation", None)
@@ -330,17 +363,17 @@ def trecon_params(clip: VideoNode, **args):
         raise Error("TRecon: constant dimensions and frame count required")
     p = abi.Params()
     abi.lib().lgcr_params_default_trecon(p)
-    p.strength = float(_get(args, "strength", 0.8))
-    p.sigma = float(_get(args, "sigma", 0.01))
-    p.sratio = float(_get(args, "sratio", 0.15))
-    p.sdb = float(_get(args, "sdb", 3.0))
-    p.gsigma = float(_get(args, "gsigma", 2.5))
-    p.stretch = float(_get(args, "stretch", 1.0))
-    p.ar_margin = float(_get(args, "ar", 0.0))
-    p.ms = float(_get(args, "ms", 1.0))
+    p.strength = _getf(args, "strength", 0.8)
+    p.sigma = _getf(args, "sigma", 0.01)
+    p.sratio = _getf(args, "sratio", 0.15)
+    p.sdb = _getf(args, "sdb", 3.0)
+    p.gsigma = _getf(args, "gsigma", 2.5)
+    p.stretch = _getf(args, "stretch", 1.0)
+    p.ar_margin = _getf(args, "ar", 0.0)
+    p.ms = _getf(args, "ms", 1.0)
     p.trad = int(_get(args, "trad", 1))
     p.tsearch = int(_get(args, "tsearch", 6))
-    p.tsad = float(_get(args, "tsad", 0.02))
+    p.tsad = _getf(args, "tsad", 0.02)
     if (p.strength < 0.0 or p.strength > 1.0 or p.sigma <= 0.0 or p.sratio <= 0.0 or p.sdb <= 0.0
             or p.stretch < 0.0 or p.gsigma <= 0.0 or p.ms < 0.0 or p.ms > 1.0 or p.trad < 0 or p.trad > 8
             or p.tsearch < 0 or p.tsearch > 64 or p.tsad <= 0.0):
@@ -366,7 +399,8 @@ def TRecon(clip: VideoNode, **args) -> VideoNode:
         src = clip.get_frame(n)
         sx, sy = _siting(src.props, fmt_in, -0.5, 0.0, False)
         nbrs = [clip.get_frame(k).planes for k in temporal_frame_numbers(n, n_last, trad)]
-        out = backend.trecon_host(src.planes, nbrs, shift_x=sx, shift_y=sy)
+        with _frame_error("TRecon"):
+            out = backend.trecon_host(src.planes, nbrs, shift_x=sx, shift_y=sy)
         props = dict(src.props)
         props.pop("_ChromaLocation", None)
         return VideoFrame(fmt_out, out, props)
@@ -388,16 +422,18 @@ def Sharpen(clip: VideoNode, **args) -> VideoNode:
     fmt = clip.format
     if fmt.color_family != YUV or (fmt.sample_type == FLOAT and fmt.bits_per_sample != 32):
         raise Error("LGCR Sharpen: planar YUV only")
-    sp = abi.sharpen_params(alpha=float(_get(args, "alpha", 0.3)), sigma=float(_get(args, "sigma", 0.01)),
-                            sratio=float(_get(args, "sratio", 0.15)), gspatial=float(_get(args, "gspatial", 1.2)),
-                            ar=float(_get(args, "ar", 0.0)))
+    sp = abi.sharpen_params(alpha=_getf(args, "alpha", 0.3), sigma=_getf(args, "sigma", 0.01),
+                            sratio=_getf(args, "sratio", 0.15), gspatial=_getf(args, "gspatial", 1.2),
+                            ar=_getf(args, "ar", 0.0))
     if sp.sigma <= 0.0 or sp.sratio <= 0.0 or sp.gspatial <= 0.0:
         raise Error("LGCR Sharpen: sigma/sratio/gspatial must be > 0")
     backend = abi.SharpenClip(_device(), sp, _abi_format(fmt, clip.width, clip.height))
 
     def get(n):
         src = clip.get_frame(n)
-        return VideoFrame(fmt, backend.sharpen_host(*src.planes), dict(src.props))
+        with _frame_error("LGCR Sharpen"):
+            planes = backend.sharpen_host(*src.planes)
+        return VideoFrame(fmt, planes, dict(src.props))
 
     node = VideoNode(fmt, clip.width, clip.height, clip.num_frames, get)
     node._backend = backend
@@ -476,7 +512,8 @@ def Downsample(clip: VideoNode, **args) -> VideoNode:
     def get(n):
         src = clip.get_frame(n)
         profile = _profiling()
-        res = backend.downsample_host(*src.planes, stats=profile, copy_luma=False)
+        with _frame_error("Downsample"):
+            res = backend.downsample_host(*src.planes, stats=profile, copy_luma=False)
         out, st = res if profile else (res, None)
         props = dict(src.props)
         props["_ChromaLocation"] = abi.DS_LOCATIONS[loc]

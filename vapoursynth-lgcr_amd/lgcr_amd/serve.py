@@ -196,6 +196,16 @@ class ResidentClip:
         """True when a submitted frame is complete."""
         return abi.lib().lgcr_ticket_query(ticket) == abi.LGCR_OK
 
+    @staticmethod
+    def wait_any(tickets, timeout_s):
+        """Wait up to timeout_s for any of the tickets to complete (spinning
+        on their completion events in native code, GIL released)."""
+        arr = (C.c_void_p * len(tickets))(*tickets)
+        idx = C.c_int(-1)
+        st = abi.lib().lgcr_ticket_wait_any(arr, len(tickets), max(timeout_s, 0.0) * 1e6, C.byref(idx))
+        if st != abi.LGCR_BUSY:
+            abi.check(st)
+
     def finish(self, ticket, slot):
         """Retire a completed ticket: workspace and output slot become free."""
         abi.check(abi.lib().lgcr_ticket_wait(ticket))
@@ -230,8 +240,18 @@ def _percentiles(lat):
     return float(np.percentile(a, 50) * 1e3), float(np.percentile(a, 99) * 1e3)
 
 
+def _wait_inflight(clips, inflight, timeout_s, poll_sleep):
+    """Block until a frame in flight completes or timeout_s passes: the
+    clips' native wait when they have one, else a poll_sleep nap."""
+    waiter = getattr(clips[inflight[0][1].clip], "wait_any", None)
+    if waiter is None:
+        time.sleep(min(poll_sleep, max(timeout_s, 0.0)))
+    else:
+        waiter([tk for tk, _, _ in inflight], timeout_s)
+
+
 def serve(clips, requests, mode="continuous", keep=None, poll_sleep=20e-6, max_inflight=4, lat_out=None,
-          max_work=None):
+          max_work=None, failures=None):
     """Serve `requests` (sorted by arrival) on `clips` (ResidentClip list).
 
     mode "continuous": non-blocking submissions across clips (at most
@@ -243,13 +263,16 @@ def serve(clips, requests, mode="continuous", keep=None, poll_sleep=20e-6, max_i
     clip is refused admission (releasing frees memory but synchronises the
     device).  keep: optional dict filled with {(clip, frame): host planes}
     for correctness checks.  lat_out: optional list that receives
-    (request, latency in s) per served frame.  Returns a stats dict (fps, p50/p99 latency in
+    (request, latency in s) per served frame.  A frame whose backend call
+    fails is not served: it goes to `failures` (a list of (request, message),
+    when given) and the other frames and clips go on.  Returns a stats dict (fps, p50/p99 latency in
     ms, peak reserved HBM)."""
     remaining = [0] * len(clips)
     for r in requests:
         remaining[r.clip] += 1
     dev = clips[0].device if clips else None
     lat, peak = [], 0
+    failed = [] if failures is None else failures
     t0 = time.perf_counter()
     i = 0
     if mode == "per_frame":
@@ -265,7 +288,12 @@ def serve(clips, requests, mode="continuous", keep=None, poll_sleep=20e-6, max_i
                         o.release()
                 if not c.admit():
                     raise RuntimeError("per-frame serving: clip does not fit the device budget alone")
-            c.run_blocking(r.frame)
+            try:
+                c.run_blocking(r.frame)
+            except abi.LgcrError as e:
+                failed.append((r, str(e)))
+                remaining[r.clip] -= 1
+                continue
             lat.append(time.perf_counter() - t0 - r.t)
             if lat_out is not None:
                 lat_out.append((r, lat[-1]))
@@ -275,8 +303,8 @@ def serve(clips, requests, mode="continuous", keep=None, poll_sleep=20e-6, max_i
             remaining[r.clip] -= 1
         el = time.perf_counter() - t0
         p50, p99 = _percentiles(lat)
-        return {"frames": len(lat), "seconds": el, "fps": len(lat) / el, "p50_ms": p50, "p99_ms": p99,
-                "peak_reserved_gb": peak / 2**30}
+        return {"frames": len(lat), "failed": len(failed), "seconds": el, "fps": len(lat) / el, "p50_ms": p50,
+                "p99_ms": p99, "peak_reserved_gb": peak / 2**30}
 
     pending = deque()
     inflight = []   # (ticket, request, slot)
@@ -349,7 +377,14 @@ def serve(clips, requests, mode="continuous", keep=None, poll_sleep=20e-6, max_i
                     blocked.add(r.clip)
                     rest.append(r)
                     continue
-            res = c.submit(r.frame)
+            try:
+                res = c.submit(r.frame)
+            except abi.LgcrError as e:
+                # the frame fails; the clip and the other clips go on
+                failed.append((r, str(e)))
+                remaining[r.clip] -= 1
+                progressed = True
+                continue
             if res is None:
                 blocked.add(r.clip)
                 rest.append(r)
@@ -360,7 +395,11 @@ def serve(clips, requests, mode="continuous", keep=None, poll_sleep=20e-6, max_i
         pending = rest
         peak = max(peak, dev.reserved())
         if not progressed:
-            if inflight or (pending and i >= len(requests)):
+            if inflight:
+                # until a frame completes or the next request arrives
+                nxt = requests[i].t - (time.perf_counter() - t0) if i < len(requests) else 5e-3
+                _wait_inflight(clips, inflight, max(0.0, min(nxt, 5e-3)), poll_sleep)
+            elif pending and i >= len(requests):
                 time.sleep(poll_sleep)
             elif i < len(requests):
                 wait = requests[i].t - (time.perf_counter() - t0)
@@ -368,5 +407,5 @@ def serve(clips, requests, mode="continuous", keep=None, poll_sleep=20e-6, max_i
                     time.sleep(min(wait, 1e-3))
     el = time.perf_counter() - t0
     p50, p99 = _percentiles(lat)
-    return {"frames": len(lat), "seconds": el, "fps": len(lat) / el, "p50_ms": p50, "p99_ms": p99,
-            "peak_reserved_gb": peak / 2**30}
+    return {"frames": len(lat), "failed": len(failed), "seconds": el, "fps": len(lat) / el, "p50_ms": p50,
+            "p99_ms": p99, "peak_reserved_gb": peak / 2**30}
