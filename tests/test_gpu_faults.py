@@ -72,16 +72,36 @@ def test_launch_fault_other_filters(device, monkeypatch, kind):
     assert all(np.array_equal(np.asarray(again[i]), first[i]) for i in range(3))
 
 
-def test_alloc_fault_fails_first_frame(device, monkeypatch):
-    """A workspace allocation that fails makes the frame that needed it fail
-    with an out-of-memory error; the next frame allocates and is served."""
+def test_alloc_fault_at_creation(device, monkeypatch):
+    """A clip's first workspace is allocated when the filter is created: a
+    failed allocation is a creation error, and the next creation works."""
     fa, a = source(80, 48, 2, seed=7)
     monkeypatch.setenv("LGCR_INJECT_FAULT", "alloc:1")
-    ra = core.lgcr.Recon(a)
     with pytest.raises(Error, match=r"^LGCR: .*hipMalloc"):
-        ra.get_frame(0)
+        core.lgcr.Recon(a)
+    ra = core.lgcr.Recon(a)
     assert same(ra.get_frame(0), recon_ref(80, 48, fa[0]))
     assert same(ra.get_frame(1), recon_ref(80, 48, fa[1]))
+
+
+@pytest.mark.parametrize("n", [1, 2, 3])
+def test_alloc_fault_in_serving(device, monkeypatch, n):
+    """A failed workspace allocation while serving (an admission or a clip's
+    extra workspace) delays frames but serves every one, identical to a run
+    without the fault."""
+    specs = serve.make_mix(3, 4, seed=13, resolutions=[(256, 144), (320, 180)], kinds=("recon",))
+    reqs = serve.poisson_requests(specs, 2000.0, seed=13)
+    clean = {}
+    clips = [serve.ResidentClip(device, s, max_inflight=2, threads=4) for s in specs]
+    assert serve.serve(clips, reqs, mode="continuous", keep=clean)["frames"] == len(reqs)
+    for c in clips:
+        c.release()
+    monkeypatch.setenv("LGCR_INJECT_FAULT", f"alloc:{n}")
+    got = {}
+    st = serve.serve(clips, reqs, mode="continuous", keep=got)
+    assert st["frames"] == len(reqs) and st["failed"] == 0
+    for key, planes in got.items():
+        assert all(np.array_equal(x, y) for x, y in zip(planes, clean[key])), key
 
 
 def test_serving_skips_failed_frame(device, monkeypatch):

@@ -706,7 +706,7 @@ void VS_CC downsample_create(const VSMap *in, VSMap *out, void *, VSCore *core, 
         return fail(derr);
     const lgcr_format bf = backend_format(fmt, d->vi_in->width, d->vi_in->height);
     if (lgcr_downsample_create(dev, &dp, &bf, 0, &d->hip) != LGCR_OK)
-        return fail(lgcr_last_error());
+        return fail(std::string("Downsample: ") + lgcr_last_error());
     DownsampleData *data = d.release();
     const VSFilterDependency deps[] = {{data->node, rpGeneral}};
     vsapi->createVideoFilter(out, "Downsample", &data->vi_out, downsample_get_frame, downsample_free,

@@ -76,9 +76,10 @@ class VideoFrame:
 
 @contextlib.contextmanager
 def _frame_error(name):
-    """A backend failure while producing a frame is that frame's error
-    (vsapi->setFilterError), prefixed with the filter name; the clip and the
-    process stay usable."""
+    """A backend failure is the filter's error, prefixed with its name: while
+    producing a frame that frame's error (vsapi->setFilterError), leaving the
+    clip and the process usable; while creating the filter a creation error
+    (vsapi->mapSetError)."""
     try:
         yield
     except abi.LgcrError as e:
@@ -310,7 +311,8 @@ def Recon(clip: VideoNode, **args) -> VideoNode:
     p = recon_params(clip, **args)
     fmt_in = clip.format
     fmt_out = replace(fmt_in, subsampling_w=0, subsampling_h=0)
-    backend = abi.Clip(_device(), p, _abi_format(fmt_in, clip.width, clip.height))
+    with _frame_error("LGCR"):
+        backend = abi.Clip(_device(), p, _abi_format(fmt_in, clip.width, clip.height))
     ow, oh = p.out_w, p.out_h
     same = ow == clip.width and oh == clip.height
     shift_x, shift_y, loc_set = p.shift_x, p.shift_y, bool(p.loc_set)
@@ -391,7 +393,8 @@ def TRecon(clip: VideoNode, **args) -> VideoNode:
     p = trecon_params(clip, **args)
     fmt_in = clip.format
     fmt_out = replace(fmt_in, subsampling_w=0, subsampling_h=0)
-    backend = abi.Clip(_device(), p, _abi_format(fmt_in, clip.width, clip.height), temporal=True)
+    with _frame_error("TRecon"):
+        backend = abi.Clip(_device(), p, _abi_format(fmt_in, clip.width, clip.height), temporal=True)
     n_last = clip.num_frames - 1
     trad = p.trad
 
@@ -427,7 +430,8 @@ def Sharpen(clip: VideoNode, **args) -> VideoNode:
                             ar=_getf(args, "ar", 0.0))
     if sp.sigma <= 0.0 or sp.sratio <= 0.0 or sp.gspatial <= 0.0:
         raise Error("LGCR Sharpen: sigma/sratio/gspatial must be > 0")
-    backend = abi.SharpenClip(_device(), sp, _abi_format(fmt, clip.width, clip.height))
+    with _frame_error("LGCR Sharpen"):
+        backend = abi.SharpenClip(_device(), sp, _abi_format(fmt, clip.width, clip.height))
 
     def get(n):
         src = clip.get_frame(n)
@@ -506,7 +510,8 @@ def Downsample(clip: VideoNode, **args) -> VideoNode:
     if loc not in abi.DS_LOCATIONS:
         raise Error("Downsample: loc must be left, center, topleft, top, bottomleft, or bottom")
     dp = abi.downsample_params(quality=quality, kernel=kernel, strength=strength, loc=loc)
-    backend = abi.DownsampleClip(_device(), dp, _abi_format(fmt, clip.width, clip.height))
+    with _frame_error("Downsample"):
+        backend = abi.DownsampleClip(_device(), dp, _abi_format(fmt, clip.width, clip.height))
     fmt_out = replace(fmt, subsampling_w=1, subsampling_h=1)
 
     def get(n):

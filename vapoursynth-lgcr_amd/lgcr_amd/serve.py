@@ -149,13 +149,20 @@ class ResidentClip:
             return True
         fmt = abi.make_format(np.uint8 if self.bps == 1 else np.uint16, self.spec.width, self.spec.height,
                               self.ssw, self.ssh, self.bits if self.bits not in (8, 16) else None)
-        try:
-            self.clip = abi.Clip(self.device, self.params(), fmt, max_inflight=self.max_inflight,
-                                 temporal=self.temporal)
-        except abi.LgcrError as e:
-            if e.status == abi.LGCR_REJECTED:
-                return False
-            raise
+        for attempt in range(2):
+            try:
+                self.clip = abi.Clip(self.device, self.params(), fmt, max_inflight=self.max_inflight,
+                                     temporal=self.temporal)
+                break
+            except abi.LgcrError as e:
+                # a failed allocation is retried once; refused by the budget
+                # or out of memory again, the clip waits for memory to be
+                # released
+                if e.status == abi.LGCR_OUT_OF_MEMORY and attempt == 0:
+                    continue
+                if e.status in (abi.LGCR_REJECTED, abi.LGCR_OUT_OF_MEMORY):
+                    return False
+                raise
         if self.graphs:
             self.clip.set_graphs(True)
         return True
