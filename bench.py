@@ -23,6 +23,34 @@ HBM_PEAK_GBS = 8000.0
 VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9   # non-packed FP32 / FP64 VALU lane-ops per s
 
 
+# Algorithmic VALU lane-ops of the tiled guided pass (plain base + guided
+# correction), counted from the reference's scalar arithmetic: one op per
+# float/double add, mul, fma, min/max, compare, conversion, sqrt or divide.
+#   per guided tap: range/hull pass 6 (recon.cpp:296-330), weight and
+#     accumulation 27 (recon.cpp:584-620)
+#   per active output pixel: guide metadata 266 (four bilinear guide reads,
+#     tensor direction, normal minimum, ridge probe, phase and mutual gates;
+#     recon.cpp:155-215, lgcr.h:467-538) and adaptive sigma, fades, rescue
+#     with kappa, normalisation and hull clamp 110 (recon.cpp:330-480, 713-724)
+#   per separable plain-base tap and plane: 2 (recon.cpp:1144-1147, 1453-1459)
+OPS_PER_GUIDED_TAP = 33
+OPS_PER_GUIDED_PIXEL = 376
+OPS_PER_PLAIN_TAP_PLANE = 2
+
+
+def algorithmic_valu(clip, frames):
+    """Mean algorithmic lane-ops per frame of the guided tiles over `frames`
+    (host planes), from the backend's per-frame tap and workset counts."""
+    from lgcr_amd import abi
+    total = 0
+    for f in frames:
+        _, st = clip.recon_host(*f, stats=True)
+        total += (OPS_PER_PLAIN_TAP_PLANE * 2 * st.stage_taps[abi.STAGE_BUILD_BASE_CHROMA]
+                  + OPS_PER_GUIDED_TAP * st.stage_taps[abi.STAGE_APPLY_GUIDED_CORRECTION]
+                  + OPS_PER_GUIDED_PIXEL * st.active_output_pixels)
+    return total / max(1, len(frames))
+
+
 def profile_figure(name, key):
     """`key = N` from a committed rocprofv3 summary under profiles/, or None."""
     path = os.path.join(ROOT, "profiles", name)
@@ -429,6 +457,7 @@ def main():
     # attainable time of the dominant kernel: the larger of its HBM time and
     # its VALU time at the non-packed lane-op peak
     t_min = max(per_frame_bytes / (HBM_PEAK_GBS * 1e9), (valu or 0) / VALU_PEAK_LANE_OPS)
+    algo_ops = algorithmic_valu(clip, frames) if dom_name == "ApplyGuidedCorrection" else None
     fmt_name = "PS" if bits == 32 else f"P{bits}"
     result = {
         "metric": METRIC,
@@ -465,6 +494,10 @@ def main():
             "valu_lane_ops_per_launch": valu,
             "valu_peak_lane_ops_per_s": VALU_PEAK_LANE_OPS,
             "attainable_frac": round(t_min / (dom_avg_us * 1e-6), 4) if valu else None,
+            "algorithmic_lane_ops_per_launch": round(algo_ops) if algo_ops else None,
+            "algorithmic_valu_frac": (round(algo_ops / VALU_PEAK_LANE_OPS / (dom_avg_us * 1e-6), 4)
+                                      if algo_ops else None),
+            "lane_op_economy": round(algo_ops / valu, 4) if (algo_ops and valu) else None,
         },
     }
     clip.close()

@@ -1389,7 +1389,15 @@ struct TapAcc {
 // rescue * bump is +0 (bump is finite), wx + 0 == wx and the weight is
 // exactly sim * wx (a zero weight may change sign, which no accumulator
 // sees), so the bump kernel and the projected offsets are not computed.
-template <bool RADIAL, int SUP, bool WIN, bool FAST, bool GUIDED, bool NORESCUE = false>
+// Inactive taps of the 6-tap axis records carry an exact zero base weight.
+// With finite samples every similarity and bump value is finite, so an
+// inactive tap's weight sim * (wx + r * bump) is an exact (signed) zero
+// when its rescue factor r is zero as well, which leaves every accumulator
+// unchanged (no accumulator is ever -0.0).  NORESCUE (separable) thus needs
+// no weight masks; ROWSON (separable, every vertical tap of the wave
+// active) takes the rescue factor per column, zero on inactive columns, in
+// place of a mask per tap.
+template <bool RADIAL, int SUP, bool WIN, bool FAST, bool GUIDED, bool NORESCUE = false, bool ROWSON = false>
 __device__ __forceinline__ void guided_taps(const GuidedArgs &A, const PixAxis<SUP> &px, const PixAxis<SUP> &py,
                                             int ox, int oy, const TapIn &t, const float4 *win, int wx0, int wy0,
                                             TapAcc &acc) {
@@ -1414,6 +1422,11 @@ __device__ __forceinline__ void guided_taps(const GuidedArgs &A, const PixAxis<S
     }
     constexpr int NH = SUP ? SUP : 1;
     float dxn[NH], dxt[NH];
+    float resc[NH];
+    if constexpr (ROWSON) {
+        _Pragma("unroll") for (int i = 0; i < NH; ++i)
+            resc[i] = qx.on(i) ? t.rescue : 0.0f;
+    }
     if constexpr (SUP != 0 && !NORESCUE) {
         _Pragma("unroll") for (int i = 0; i < SUP; ++i) {
             const float dx = ((float)(t.tx0 + i) - t.scx) * gp.frw;
@@ -1463,16 +1476,19 @@ __device__ __forceinline__ void guided_taps(const GuidedArgs &A, const PixAxis<S
                 const float sim_raw = FAST ? rcp_nr(ys) : recip_ge1(ys);
                 const float sim = 1.0f - t.guide_fade * (1.0f - sim_raw);
                 const float bump = FAST ? rcp_nr(yb) : recip_ge1(yb);
-                w = sim * (wx + t.rescue * bump);
+                w = sim * (wx + (ROWSON ? resc[i] : t.rescue) * bump);
             } else {
                 w = wx;
             }
-            if (SUP)
+            constexpr bool kMaskFree = ROWSON || (NORESCUE && !RADIAL);
+            if (SUP && !kMaskFree)
                 w = on ? w : 0.0f;
             acc.u = dacc(acc.u, w, q.y);
             acc.v = dacc(acc.v, w, q.z);
-            acc.wsum += w;
-            acc.sabs += fabsf(w);
+            // |double(w)| == double(|w|): one conversion serves both sums
+            const double wd = (double)w;
+            acc.wsum += wd;
+            acc.sabs += __builtin_fabs(wd);
         }
         // one row of taps in flight at a time keeps the register footprint
         // of the unrolled loop small
@@ -1481,10 +1497,16 @@ __device__ __forceinline__ void guided_taps(const GuidedArgs &A, const PixAxis<S
     }
 }
 
-template <bool RADIAL, bool TEMPORAL, int SEL, bool CEDGE, int SUP, bool WIN = false>
+// HULL_IN: the caller passes the chroma hull of the pixel's active taps
+// (hull_in = {min U, max U, min V, max V}), which the plain base of the tile
+// has formed over the same taps; pass 1 then only forms the luma range.
+// With finite samples the minimum and maximum of a set do not depend on the
+// order they are taken in, so the hull is the one pass 1 would form.
+template <bool RADIAL, bool TEMPORAL, int SEL, bool CEDGE, int SUP, bool WIN = false, bool HULL_IN = false>
 __device__ __attribute__((always_inline)) void guided_pixel(const GuidedArgs &A, int ox, int oy, uint32_t k,
                                                             const float4 *win = nullptr, int wx0 = 0,
-                                                            int wy0 = 0, float2 *res = nullptr) {
+                                                            int wy0 = 0, float2 *res = nullptr,
+                                                            float4 hull_in = float4{}) {
     const GuideParams &gp = A.gp;
     const int cw = A.cw, ch = A.ch, sw = A.sw, sh = A.sh;
     const bool guided = gp.strength > 0.0f;
@@ -1554,15 +1576,23 @@ __device__ __attribute__((always_inline)) void guided_pixel(const GuidedArgs &A,
                                  : A.luv[row + clampi(tx0 + i, 0, cw - 1)];
             const float m = fmx(max_dl, fabsf(q.x - L0));
             max_dl = on ? m : max_dl;
-            const float a = q.y, b = q.z;
-            const float nmu = fmn(mnu, a), nxu = fmx(mxu, a), nmv = fmn(mnv, b), nxv2 = fmx(mxv, b);
-            mnu = on ? nmu : mnu;
-            mxu = on ? nxu : mxu;
-            mnv = on ? nmv : mnv;
-            mxv = on ? nxv2 : mxv;
+            if constexpr (!HULL_IN) {
+                const float a = q.y, b = q.z;
+                const float nmu = fmn(mnu, a), nxu = fmx(mxu, a), nmv = fmn(mnv, b), nxv2 = fmx(mxv, b);
+                mnu = on ? nmu : mnu;
+                mxu = on ? nxu : mxu;
+                mnv = on ? nmv : mnv;
+                mxv = on ? nxv2 : mxv;
+            }
         }
         if (SUP)
             __builtin_amdgcn_sched_barrier(0);
+    }
+    if constexpr (HULL_IN) {
+        mnu = hull_in.x;
+        mxu = hull_in.y;
+        mnv = hull_in.z;
+        mxv = hull_in.w;
     }
     if (!guided)
         max_dl = 0.0f;
@@ -1681,8 +1711,18 @@ __device__ __attribute__((always_inline)) void guided_pixel(const GuidedArgs &A,
         guided_taps<RADIAL, SUP, WIN, true, false>(A, px, py, ox, oy, ti, win, wx0, wy0, acc);
     else if (gp.finite && __all(fast) && __all(rescue == 0.0f))
         guided_taps<RADIAL, SUP, WIN, true, true, true>(A, px, py, ox, oy, ti, win, wx0, wy0, acc);
-    else if (__all(fast))
-        guided_taps<RADIAL, SUP, WIN, true, true>(A, px, py, ox, oy, ti, win, wx0, wy0, acc);
+    else if (__all(fast)) {
+        // every vertical tap active for the whole wave (4:2:0 with centred
+        // vertical siting, away from the borders): no weight masks
+        if constexpr (SUP == 6 && !RADIAL && WIN) {
+            if (gp.finite && __all(py.bits == 0x3fu))
+                guided_taps<RADIAL, SUP, WIN, true, true, false, true>(A, px, py, ox, oy, ti, win, wx0, wy0, acc);
+            else
+                guided_taps<RADIAL, SUP, WIN, true, true>(A, px, py, ox, oy, ti, win, wx0, wy0, acc);
+        } else {
+            guided_taps<RADIAL, SUP, WIN, true, true>(A, px, py, ox, oy, ti, win, wx0, wy0, acc);
+        }
+    }
     else  // rare: exact reciprocals, weights from the tables, compact loop
         guided_taps<RADIAL, 0, WIN, false, true>(A, pix_axis<0>(A.ax, ox), pix_axis<0>(A.ay, oy), ox, oy, ti,
                                                  win, wx0, wy0, acc);
@@ -1886,9 +1926,12 @@ __device__ __forceinline__ void plain_v_col(const float2 *val, const float4 *hul
 // OUTT: 0 = results to out_u/out_v as floats; otherwise the tile also
 // applies back-projection (A.bp_on) and writes the output planes itself
 // (1 = 8-bit, 2 = 16-bit, 3 = float samples).
-template <bool TEMPORAL, int SEL, bool PLAIN, int OUTT = 0>
+// PH (with PLAIN, finite samples only): the guided pass takes the chroma
+// hull of each pixel from the plain base's vertical pass.
+template <bool TEMPORAL, int SEL, bool PLAIN, int OUTT = 0, bool PH = false>
 __global__ __launch_bounds__(kBlock, TEMPORAL ? 4 : (PLAIN ? 5 : 6)) void k_guided_tiles(GuidedArgs A, int tiles_x,
                                                                                        int tiles) {
+    static_assert(!PH || PLAIN, "the plain hull comes from the plain base in the tile");
     // fused output needs every inside pixel's result in the tile: the plain
     // base in the tiles (Recon), or a dense temporal pass (TRecon, no mask)
     static_assert(OUTT == 0 || (SEL == SEL_NONE && PLAIN != TEMPORAL), "fused output needs every pixel's result");
@@ -1906,6 +1949,7 @@ __global__ __launch_bounds__(kBlock, TEMPORAL ? 4 : (PLAIN ? 5 : 6)) void k_guid
     const int ox = tx * kTile + (lane & (kTile - 1)), oy = ty * kTile + (lane / kTile);
     const bool inside = ox < ow && oy < oh;
     float2 res = make_float2(0.0f, 0.0f);  // this pixel's result (fused output)
+    float4 phull = make_float4(1e30f, -1e30f, 1e30f, -1e30f);
     bool act = inside;
     if (act && A.mbits) {
         // nearest mask sample (buildSparseWorkset, maps.cpp:342-360); the
@@ -1963,6 +2007,7 @@ __global__ __launch_bounds__(kBlock, TEMPORAL ? 4 : (PLAIN ? 5 : 6)) void k_guid
                 plain_v_col<true>(&hval[wave][t0][c8], &hhull[wave][t0][c8], pr, vu, vv, mnu, mxu, mnv, mxv);
             else
                 plain_v_col<false>(&hval[wave][t0][c8], &hhull[wave][t0][c8], pr, vu, vv, mnu, mxu, mnv, mxv);
+            phull = make_float4(mnu, mxu, mnv, mxv);
             const double wsum = (double)pc.wsum * pr.wsum;
             const double asum = (double)pc.wabs * pr.wabs;
             if (wsum > 1e-6 && wsum >= 0.05 * asum) {
@@ -1988,8 +2033,8 @@ __global__ __launch_bounds__(kBlock, TEMPORAL ? 4 : (PLAIN ? 5 : 6)) void k_guid
             return;
     }
     if (act)
-        guided_pixel<false, TEMPORAL, SEL, false, 6, true>(A, ox, oy, 0u, wb, x_lo, y_lo,
-                                                           OUTT != 0 ? &res : nullptr);
+        guided_pixel<false, TEMPORAL, SEL, false, 6, true, PH>(A, ox, oy, 0u, wb, x_lo, y_lo,
+                                                               OUTT != 0 ? &res : nullptr, phull);
     if constexpr (OUTT != 0) {
         // back-projection (backProject, algos.cpp:663-677): same-size 4:2:0
         // with an even output, so the 2x2 block of chroma sample (ox/2,
@@ -3941,7 +3986,13 @@ hipError_t launch_guided(const float *u, const float *v, int cw, int ch, const f
     if (guided_tiled(tiles_fit, gp.radial, gp.cedge, ax, ay, sel.mode)) {
         const int tiles_x = (ax.n + kTile - 1) / kTile, tiles = tiles_x * ((ay.n + kTile - 1) / kTile);
         const dim3 g((unsigned)((tiles + kBlock / 64 - 1) / (kBlock / 64)));
-#define LGCR_TILES(T, S_, P) hipLaunchKernelGGL((k_guided_tiles<T, S_, P>), g, dim3(kBlock), 0, s, A, tiles_x, tiles)
+#define LGCR_TILES(T, S_, P)                                                                                      \
+    do {                                                                                                        \
+        if (P && gp.finite)                                                                                     \
+            hipLaunchKernelGGL((k_guided_tiles<T, S_, P, 0, P>), g, dim3(kBlock), 0, s, A, tiles_x, tiles);     \
+        else                                                                                                    \
+            hipLaunchKernelGGL((k_guided_tiles<T, S_, P, 0, false>), g, dim3(kBlock), 0, s, A, tiles_x, tiles); \
+    } while (0)
         if (fused_out) {
             // the caller checked: no selector, and either the plain base in
             // the tiles without temporal taps (bp_on only for same-size 4:2:0
@@ -3954,17 +4005,19 @@ hipError_t launch_guided(const float *u, const float *v, int cw, int ch, const f
             A.po_v = fused_out[1];
             A.bp = bp;
             A.bp_on = bp_on;
-#define LGCR_FUSED(T, P)                                                                                          \
+#define LGCR_FUSED(T, P, H)                                                                                       \
     if (out_type == RAW_U8)                                                                                     \
-        hipLaunchKernelGGL((k_guided_tiles<T, SEL_NONE, P, 1>), g, dim3(kBlock), 0, s, A, tiles_x, tiles);      \
+        hipLaunchKernelGGL((k_guided_tiles<T, SEL_NONE, P, 1, H>), g, dim3(kBlock), 0, s, A, tiles_x, tiles);   \
     else if (out_type == RAW_U16)                                                                               \
-        hipLaunchKernelGGL((k_guided_tiles<T, SEL_NONE, P, 2>), g, dim3(kBlock), 0, s, A, tiles_x, tiles);      \
+        hipLaunchKernelGGL((k_guided_tiles<T, SEL_NONE, P, 2, H>), g, dim3(kBlock), 0, s, A, tiles_x, tiles);   \
     else                                                                                                        \
-        hipLaunchKernelGGL((k_guided_tiles<T, SEL_NONE, P, 3>), g, dim3(kBlock), 0, s, A, tiles_x, tiles);
-            if (recon) {
-                LGCR_FUSED(false, true)
+        hipLaunchKernelGGL((k_guided_tiles<T, SEL_NONE, P, 3, H>), g, dim3(kBlock), 0, s, A, tiles_x, tiles);
+            if (recon && gp.finite) {
+                LGCR_FUSED(false, true, true)
+            } else if (recon) {
+                LGCR_FUSED(false, true, false)
             } else {
-                LGCR_FUSED(true, false)
+                LGCR_FUSED(true, false, false)
             }
 #undef LGCR_FUSED
             return hipGetLastError();

@@ -45,6 +45,9 @@ STAGE_NAMES = [
     "DownsampleGuide", "DownsampleCandidateScore", "DownsampleOutput", "BackProject", "ConvertOutput",
 ]
 
+STAGE_BUILD_BASE_CHROMA = STAGE_NAMES.index("BuildBaseChroma")
+STAGE_APPLY_GUIDED_CORRECTION = STAGE_NAMES.index("ApplyGuidedCorrection")
+
 
 class Params(C.Structure):
     _fields_ = [
