@@ -20,6 +20,12 @@ __device__ __forceinline__ float fmn(float a, float b) { return b < a ? b : a; }
 __device__ __forceinline__ float fmx(float a, float b) { return a < b ? b : a; }
 __device__ __forceinline__ float fclamp(float v, float lo, float hi) { return v < lo ? lo : (hi < v ? hi : v); }
 __device__ __forceinline__ double dclamp(double v, double lo, double hi) { return v < lo ? lo : (hi < v ? hi : v); }
+// Maximum of finite non-negative floats (no -0.0): such floats order like
+// their bit patterns read as unsigned integers.
+__device__ __forceinline__ float nn_max(float a, float b) {
+    const uint32_t x = __float_as_uint(a), y = __float_as_uint(b);
+    return __uint_as_float(x < y ? y : x);
+}
 
 // Planes hold fewer than 2^31 samples (checked at clip creation), so sample
 // offsets are 32-bit.
@@ -1574,8 +1580,14 @@ __device__ __attribute__((always_inline)) void guided_pixel(const GuidedArgs &A,
                 continue;
             const float4 q = WIN ? win[(ty0 + j - wy0) * kTStride + (tx0 + i - wx0)]
                                  : A.luv[row + clampi(tx0 + i, 0, cw - 1)];
-            const float m = fmx(max_dl, fabsf(q.x - L0));
-            max_dl = on ? m : max_dl;
+            if constexpr (HULL_IN) {
+                // integer input: |q.x - L0| is finite and >= +0, max_dl
+                // starts at +0
+                max_dl = nn_max(max_dl, on ? fabsf(q.x - L0) : 0.0f);
+            } else {
+                const float m = fmx(max_dl, fabsf(q.x - L0));
+                max_dl = on ? m : max_dl;
+            }
             if constexpr (!HULL_IN) {
                 const float a = q.y, b = q.z;
                 const float nmu = fmn(mnu, a), nxu = fmx(mxu, a), nmv = fmn(mnv, b), nxv2 = fmx(mxv, b);
@@ -1923,6 +1935,65 @@ __device__ __forceinline__ void plain_v_col(const float2 *val, const float4 *hul
     }
 }
 
+// Samples of integer input (luma k * s, chroma k * s - 1/2: finite, never
+// -0.0): the active taps of a 6-tap axis record form one run [lo, hi];
+// reading tap clamp(i, lo, hi) in place of tap i repeats active samples,
+// which changes no minimum or maximum, and the weight of an inactive tap is
+// an exact zero, so the sums are unchanged too.  The hulls are then plain
+// minimum / maximum chains over the six reads: for finite samples without
+// -0.0 the minimum of a set is the same whichever way it is formed, so the
+// three-operand VALU minimum / maximum give the ordered compares' result.
+__device__ __forceinline__ float min3f(float a, float b, float c) {
+    float r;
+    __asm__("v_min3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+__device__ __forceinline__ float max3f(float a, float b, float c) {
+    float r;
+    __asm__("v_max3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+__device__ __forceinline__ void tap_run(uint32_t bits, int off[6], int scale) {
+    const int lo = bits ? __builtin_ctz(bits) : 0;
+    const int hi = bits ? 31 - __builtin_clz(bits) : 5;
+    _Pragma("unroll") for (int i = 0; i < 6; ++i)
+        off[i] = min(max(i, lo), hi) * scale;
+}
+
+__device__ __forceinline__ void plain_h_row_fin(const float4 *row, const int off[6], const PixAxis<6> &pc,
+                                                float2 &val, float4 &hull) {
+    double vu = 0.0, vv = 0.0;
+    float a[6], b[6];
+    _Pragma("unroll") for (int i = 0; i < 6; ++i) {
+        const float4 q = row[off[i]];
+        a[i] = q.y;
+        b[i] = q.z;
+        vu = dacc(vu, pc.w[i], a[i]);
+        vv = dacc(vv, pc.w[i], b[i]);
+    }
+    val = make_float2((float)vu, (float)vv);
+    hull = make_float4(min3f(min3f(a[0], a[1], a[2]), a[3], min3f(a[4], a[5], a[5])),
+                       max3f(max3f(a[0], a[1], a[2]), a[3], max3f(a[4], a[5], a[5])),
+                       min3f(min3f(b[0], b[1], b[2]), b[3], min3f(b[4], b[5], b[5])),
+                       max3f(max3f(b[0], b[1], b[2]), b[3], max3f(b[4], b[5], b[5])));
+}
+
+__device__ __forceinline__ void plain_v_col_fin(const float2 *val, const float4 *hull, const int off[6],
+                                                const PixAxis<6> &pr, double &vu, double &vv, float &mnu,
+                                                float &mxu, float &mnv, float &mxv) {
+    float4 h[6];
+    _Pragma("unroll") for (int j = 0; j < 6; ++j) {
+        const float2 v = val[off[j]];
+        h[j] = hull[off[j]];
+        vu = dacc(vu, pr.w[j], v.x);
+        vv = dacc(vv, pr.w[j], v.y);
+    }
+    mnu = min3f(min3f(h[0].x, h[1].x, h[2].x), h[3].x, min3f(h[4].x, h[5].x, h[5].x));
+    mxu = max3f(max3f(h[0].y, h[1].y, h[2].y), h[3].y, max3f(h[4].y, h[5].y, h[5].y));
+    mnv = min3f(min3f(h[0].z, h[1].z, h[2].z), h[3].z, min3f(h[4].z, h[5].z, h[5].z));
+    mxv = max3f(max3f(h[0].w, h[1].w, h[2].w), h[3].w, max3f(h[4].w, h[5].w, h[5].w));
+}
+
 // OUTT: 0 = results to out_u/out_v as floats; otherwise the tile also
 // applies back-projection (A.bp_on) and writes the output planes itself
 // (1 = 8-bit, 2 = 16-bit, 3 = float samples).
@@ -1984,7 +2055,13 @@ __global__ __launch_bounds__(kBlock, TEMPORAL ? 4 : (PLAIN ? 5 : 6)) void k_guid
         const int col0 = pc.start - x_lo;
         // inactive taps carry zero weight in the axis records; when every
         // lane's six taps are active (the interior) the hulls need no masks
-        if (__all(pc.bits == 0x3fu)) {
+        if constexpr (PH) {
+            int hx[6];
+            tap_run(pc.bits, hx, 1);
+            _Pragma("unroll") for (int k = 0; k < 2; ++k)
+                plain_h_row_fin(wb + ((lane >> 3) + 8 * k) * kTStride + col0, hx, pc,
+                                hval[wave][(lane >> 3) + 8 * k][c8], hhull[wave][(lane >> 3) + 8 * k][c8]);
+        } else if (__all(pc.bits == 0x3fu)) {
             _Pragma("unroll") for (int k = 0; k < 2; ++k)
                 plain_h_row<true>(wb + ((lane >> 3) + 8 * k) * kTStride + col0, pc, hval[wave][(lane >> 3) + 8 * k][c8],
                                   hhull[wave][(lane >> 3) + 8 * k][c8]);
@@ -2003,10 +2080,15 @@ __global__ __launch_bounds__(kBlock, TEMPORAL ? 4 : (PLAIN ? 5 : 6)) void k_guid
             const int t0 = pr.start - y_lo;
             double vu = 0.0, vv = 0.0;
             float mnu = 1e30f, mxu = -1e30f, mnv = 1e30f, mxv = -1e30f;
-            if (vall)
+            if constexpr (PH) {
+                int hy[6];
+                tap_run(pr.bits, hy, kTile);
+                plain_v_col_fin(&hval[wave][t0][c8], &hhull[wave][t0][c8], hy, pr, vu, vv, mnu, mxu, mnv, mxv);
+            } else if (vall) {
                 plain_v_col<true>(&hval[wave][t0][c8], &hhull[wave][t0][c8], pr, vu, vv, mnu, mxu, mnv, mxv);
-            else
+            } else {
                 plain_v_col<false>(&hval[wave][t0][c8], &hhull[wave][t0][c8], pr, vu, vv, mnu, mxu, mnv, mxv);
+            }
             phull = make_float4(mnu, mxu, mnv, mxv);
             const double wsum = (double)pc.wsum * pr.wsum;
             const double asum = (double)pc.wabs * pr.wabs;
