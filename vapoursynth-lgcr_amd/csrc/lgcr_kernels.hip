@@ -20,6 +20,23 @@ __device__ __forceinline__ float fmn(float a, float b) { return b < a ? b : a; }
 __device__ __forceinline__ float fmx(float a, float b) { return a < b ? b : a; }
 __device__ __forceinline__ float fclamp(float v, float lo, float hi) { return v < lo ? lo : (hi < v ? hi : v); }
 __device__ __forceinline__ double dclamp(double v, double lo, double hi) { return v < lo ? lo : (hi < v ? hi : v); }
+// max(m, |a|, |b|) for m >= +0 and finite a, b: |a| and |b| are finite and
+// non-negative, so the VALU maximum equals the ordered compares' result.
+__device__ __forceinline__ float max3abs(float m, float a, float b) {
+    float r;
+    __asm__("v_max3_f32 %0, %1, |%2|, |%3|" : "=v"(r) : "v"(m), "v"(a), "v"(b));
+    return r;
+}
+
+// First and last active tap of a 6-tap axis record (the active taps form
+// one run [lo, hi]) and the offsets of taps clamp(i, lo, hi) * scale.
+__device__ __forceinline__ void tap_run(uint32_t bits, int off[6], int scale) {
+    const int lo = bits ? __builtin_ctz(bits) : 0;
+    const int hi = bits ? 31 - __builtin_clz(bits) : 5;
+    _Pragma("unroll") for (int i = 0; i < 6; ++i)
+        off[i] = min(max(i, lo), hi) * scale;
+}
+
 // Maximum of finite non-negative floats (no -0.0): such floats order like
 // their bit patterns read as unsigned integers.
 __device__ __forceinline__ float nn_max(float a, float b) {
@@ -1569,7 +1586,31 @@ __device__ __attribute__((always_inline)) void guided_pixel(const GuidedArgs &A,
     const int SX = SUP ? SUP : A.ax.sup, SY = SUP ? SUP : A.ay.sup;  // taps per axis
     float max_dl = 0.0f;
     float mnu = 1e30f, mxu = -1e30f, mnv = 1e30f, mxv = -1e30f;
+    bool range_done = false;
+    if constexpr (HULL_IN && SUP == 6 && WIN) {
+        // every vertical tap of the wave active: the range over the active
+        // taps is the maximum over the clamped column run (a repeated
+        // active tap changes no maximum), read at fixed row offsets
+        if (__all(py.bits == 0x3fu)) {
+            int cx[6];
+            tap_run(px.bits, cx, 1);
+            const float4 *b0 = win + (ty0 - wy0) * kTStride + (tx0 - wx0);
+            float m = 0.0f;
+            _Pragma("unroll") for (int j = 0; j < 6; ++j) {
+                float d[6];
+                _Pragma("unroll") for (int i = 0; i < 6; ++i)
+                    d[i] = b0[cx[i] + j * kTStride].x - L0;
+                m = max3abs(m, d[0], d[1]);
+                m = max3abs(m, d[2], d[3]);
+                m = max3abs(m, d[4], d[5]);
+            }
+            max_dl = m;
+            range_done = true;
+        }
+    }
     _Pragma("unroll") for (int j = 0; j < SY; ++j) {
+        if (range_done)
+            break;
         const bool onj = py.on(j);
         if (!SUP && !onj)
             continue;
@@ -1953,13 +1994,6 @@ __device__ __forceinline__ float max3f(float a, float b, float c) {
     __asm__("v_max3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
     return r;
 }
-__device__ __forceinline__ void tap_run(uint32_t bits, int off[6], int scale) {
-    const int lo = bits ? __builtin_ctz(bits) : 0;
-    const int hi = bits ? 31 - __builtin_clz(bits) : 5;
-    _Pragma("unroll") for (int i = 0; i < 6; ++i)
-        off[i] = min(max(i, lo), hi) * scale;
-}
-
 __device__ __forceinline__ void plain_h_row_fin(const float4 *row, const int off[6], const PixAxis<6> &pc,
                                                 float2 &val, float4 &hull) {
     double vu = 0.0, vv = 0.0;
