@@ -803,73 +803,12 @@ __device__ __forceinline__ void sobel_lds(const float (*p)[SW], int xm, int xc, 
     gy = (bl + 2.0f * b + br - tl - 2.0f * t - tr) * 0.125f;
 }
 
-template <int kGateTW, int kGateTH>
-__global__ __launch_bounds__(kBlock) void k_gate_fused(const float *__restrict__ lc, const float *__restrict__ u,
-                                                       const float *__restrict__ v, int w, int h,
-                                                       const uint8_t *cmask, const uint32_t *count, int ncount,
-                                                       float *gate) {
-    static_assert(kGateTW * kGateTH == kBlock, "one thread per tile sample");
-    constexpr int kGateGW = kGateTW + 2 * kGateHalo, kGateGH = kGateTH + 2 * kGateHalo;
-    constexpr int kGateSW = kGateGW + 2, kGateSH = kGateGH + 2;
-    __shared__ float src[3][kGateSH][kGateSW];
-    // gradients per sample, interleaved so a bilinear corner is one 16-byte
-    // and one 8-byte read: {gx, gy} of lc and U, then of V
-    __shared__ float4 gyu[kGateGH][kGateGW];
-    __shared__ float2 gvv[kGateGH][kGateGW];
-    __shared__ int s_dense;
-    unsigned bx, by;
-    xcd_block2(bx, by);
-    const int tx0 = (int)bx * kGateTW, ty0 = (int)by * kGateTH;
-    const int gx0 = tx0 - kGateHalo, gy0 = ty0 - kGateHalo, sx0 = gx0 - 1, sy0 = gy0 - 1;
-    const int x = tx0 + (int)(threadIdx.x % kGateTW), y = ty0 + (int)(threadIdx.x / kGateTW);
-    const size_t n = (size_t)w * h;
-    // dense unless at most half the chroma samples are marked (the marked
-    // count is the sum of count[0..ncount))
-    if (threadIdx.x < 64) {
-        uint32_t c = 0u;
-        if (cmask) {
-            _Pragma("unroll 8") for (int i = threadIdx.x; i < ncount; i += 64)
-                c += count[i];
-        }
-        for (int o = 32; o > 0; o >>= 1)
-            c += __shfl_xor(c, o, 64);
-        if (threadIdx.x == 0)
-            s_dense = !cmask || (uint64_t)c * 2 >= (uint64_t)n;
-    }
-    __syncthreads();
-    const bool dense = s_dense != 0;
-    const bool inside = x < w && y < h;
-    const bool need = inside && (dense || cmask[(size_t)y * w + x] != 0);
-    if (!__syncthreads_or(need)) {
-        if (inside)
-            gate[(size_t)y * w + x] = 0.0f;
-        return;
-    }
-    for (int e = threadIdx.x; e < kGateSH * kGateSW; e += kBlock) {
-        const int r = e / kGateSW, c = e - r * kGateSW;
-        const size_t k = (size_t)clampi(sy0 + r, 0, h - 1) * w + clampi(sx0 + c, 0, w - 1);
-        src[0][r][c] = lc[k];
-        src[1][r][c] = u[k];
-        src[2][r][c] = v[k];
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < kGateGH * kGateGW; e += kBlock) {
-        const int r = e / kGateGW, c = e - r * kGateGW;
-        const int P = clampi(gx0 + c, 0, w - 1), Q = clampi(gy0 + r, 0, h - 1);
-        const int xm = clampi(P - 1, 0, w - 1) - sx0, xc = P - sx0, xp = clampi(P + 1, 0, w - 1) - sx0;
-        const int ym = clampi(Q - 1, 0, h - 1) - sy0, yc = Q - sy0, yp = clampi(Q + 1, 0, h - 1) - sy0;
-        float g[6];
-        _Pragma("unroll") for (int q = 0; q < 3; ++q)
-            sobel_lds<kGateSW>(src[q], xm, xc, xp, ym, yc, yp, g[2 * q], g[2 * q + 1]);
-        gyu[r][c] = make_float4(g[0], g[1], g[2], g[3]);
-        gvv[r][c] = make_float2(g[4], g[5]);
-    }
-    __syncthreads();
-    if (!need) {
-        if (inside)
-            gate[(size_t)y * w + x] = 0.0f;
-        return;
-    }
+// The mutual gate of one chroma sample (x, y) from the block's LDS gradients
+// (buildMutualGate's processPixel, maps.cpp:853-905).
+template <int kGateGW, int kGateGH>
+__device__ __forceinline__ void gate_sample(int x, int y, int w, int h, int gx0, int gy0,
+                                            const float4 (*gyu)[kGateGW], const float2 (*gvv)[kGateGW],
+                                            float *gate) {
     float *out = gate + (size_t)y * w + x;
     // LDS column / row of a clamped image coordinate (bounded for any input)
     auto cx = [&](int X) { return clampi(clampi(X, 0, w - 1) - gx0, 0, kGateGW - 1); };
@@ -931,6 +870,95 @@ __global__ __launch_bounds__(kBlock) void k_gate_fused(const float *__restrict__
     const float tp = fclamp((1.0f - dcent) / 0.5f, 0.0f, 1.0f);
     *out = (tw * tw * (3.0f - 2.0f * tw)) * (tp * tp * (3.0f - 2.0f * tp));
 }
+
+template <int kGateTW, int kGateTH>
+__global__ __launch_bounds__(kBlock) void k_gate_fused(const float *__restrict__ lc, const float *__restrict__ u,
+                                                       const float *__restrict__ v, int w, int h,
+                                                       const uint8_t *cmask, const uint32_t *count, int ncount,
+                                                       float *gate) {
+    static_assert(kGateTW * kGateTH == kBlock, "one thread per tile sample");
+    constexpr int kGateGW = kGateTW + 2 * kGateHalo, kGateGH = kGateTH + 2 * kGateHalo;
+    constexpr int kGateSW = kGateGW + 2, kGateSH = kGateGH + 2;
+    __shared__ float src[3][kGateSH][kGateSW];
+    // gradients per sample, interleaved so a bilinear corner is one 16-byte
+    // and one 8-byte read: {gx, gy} of lc and U, then of V
+    __shared__ float4 gyu[kGateGH][kGateGW];
+    __shared__ float2 gvv[kGateGH][kGateGW];
+    __shared__ int s_dense;
+    unsigned bx, by;
+    xcd_block2(bx, by);
+    const int tx0 = (int)bx * kGateTW, ty0 = (int)by * kGateTH;
+    const int gx0 = tx0 - kGateHalo, gy0 = ty0 - kGateHalo, sx0 = gx0 - 1, sy0 = gy0 - 1;
+    const int x = tx0 + (int)(threadIdx.x % kGateTW), y = ty0 + (int)(threadIdx.x / kGateTW);
+    const size_t n = (size_t)w * h;
+    // dense unless at most half the chroma samples are marked (the marked
+    // count is the sum of count[0..ncount))
+    if (threadIdx.x < 64) {
+        uint32_t c = 0u;
+        if (cmask) {
+            _Pragma("unroll 8") for (int i = threadIdx.x; i < ncount; i += 64)
+                c += count[i];
+        }
+        for (int o = 32; o > 0; o >>= 1)
+            c += __shfl_xor(c, o, 64);
+        if (threadIdx.x == 0)
+            s_dense = !cmask || (uint64_t)c * 2 >= (uint64_t)n;
+    }
+    __syncthreads();
+    const bool dense = s_dense != 0;
+    const bool inside = x < w && y < h;
+    const bool need = inside && (dense || cmask[(size_t)y * w + x] != 0);
+    if (!__syncthreads_or(need)) {
+        if (inside)
+            gate[(size_t)y * w + x] = 0.0f;
+        return;
+    }
+    for (int e = threadIdx.x; e < kGateSH * kGateSW; e += kBlock) {
+        const int r = e / kGateSW, c = e - r * kGateSW;
+        const size_t k = (size_t)clampi(sy0 + r, 0, h - 1) * w + clampi(sx0 + c, 0, w - 1);
+        src[0][r][c] = lc[k];
+        src[1][r][c] = u[k];
+        src[2][r][c] = v[k];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kGateGH * kGateGW; e += kBlock) {
+        const int r = e / kGateGW, c = e - r * kGateGW;
+        const int P = clampi(gx0 + c, 0, w - 1), Q = clampi(gy0 + r, 0, h - 1);
+        const int xm = clampi(P - 1, 0, w - 1) - sx0, xc = P - sx0, xp = clampi(P + 1, 0, w - 1) - sx0;
+        const int ym = clampi(Q - 1, 0, h - 1) - sy0, yc = Q - sy0, yp = clampi(Q + 1, 0, h - 1) - sy0;
+        float g[6];
+        _Pragma("unroll") for (int q = 0; q < 3; ++q)
+            sobel_lds<kGateSW>(src[q], xm, xc, xp, ym, yc, yp, g[2 * q], g[2 * q + 1]);
+        gyu[r][c] = make_float4(g[0], g[1], g[2], g[3]);
+        gvv[r][c] = make_float2(g[4], g[5]);
+    }
+    // the block's needed samples, gathered in row-major order into the
+    // first lanes, so the gate below runs on full waves
+    __shared__ uint16_t s_list[kBlock];
+    __shared__ int s_wcount[kBlock / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint64_t nb = __ballot(need);
+    if (lane == 0)
+        s_wcount[wv] = __popcll(nb);
+    if (inside && !need)
+        gate[(size_t)y * w + x] = 0.0f;
+    __syncthreads();
+    int base = 0, n_need = 0;
+    _Pragma("unroll") for (int k = 0; k < kBlock / 64; ++k) {
+        base += k < wv ? s_wcount[k] : 0;
+        n_need += s_wcount[k];
+    }
+    if (need)
+        s_list[base + __popcll(nb & ((1ull << lane) - 1ull))] = (uint16_t)threadIdx.x;
+    __syncthreads();
+    if ((int)threadIdx.x >= n_need)
+        return;
+    {
+        const int p = s_list[threadIdx.x];
+        gate_sample<kGateGW, kGateGH>(tx0 + p % kGateTW, ty0 + p / kGateTW, w, h, gx0, gy0, gyu, gvv, gate);
+    }
+}
+
 
 // Per-output-pixel axis data: first tap, positions, bilinear indices, the
 // tap weights and active-tap flags.  6-tap axes read everything from one
