@@ -287,7 +287,7 @@ lgcr_status lgcr_clip_set_graphs(lgcr_clip *clip, int enable);
 /* Tile shapes of the kernels whose best shape depends on the frame size.
  * No counterpart in the reference (its CPU stages have no tiles). */
 typedef struct lgcr_tuning {
-    int tensor_tile_rows; /* Sobel tensor tiles of 64 x {4, 8, 16} rows */
+    int tensor_tile_rows; /* Sobel tensor strips of 64 x {8, 16, 32} rows */
     int gate_tile_cols;   /* mutual-gate tiles of {16x16, 32x8, 64x4} (by columns) */
 } lgcr_tuning;
 lgcr_status lgcr_clip_get_tuning(const lgcr_clip *clip, lgcr_tuning *tuning);

@@ -12,7 +12,7 @@ from parity import compare_planes, frame
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = list(itertools.product((4, 8, 16), (16, 32, 64)))
+SHAPES = list(itertools.product((8, 16, 32), (16, 32, 64)))
 CASES = [
     # (id, width, height, ss_w, ss_h, bits, params)
     ("p8-420-odd", 98, 46, 1, 1, 8, {}),
@@ -45,7 +45,7 @@ def test_trecon_shapes_match_oracle(device):
     ref = oracle.trecon_frame(oracle.trecon_params(), frames[1], [frames[0], frames[2]], bits=10)
     clip = abi.Clip(device, abi.trecon_params(), abi.make_format(np.uint16, 96, 64, bits=10), temporal=True)
     try:
-        for rows, cols in ((4, 64), (16, 16)):
+        for rows, cols in ((8, 64), (32, 16)):
             clip.set_tuning(rows, cols)
             got = clip.trecon_host(frames[1], [frames[0], frames[2]])
             for i in range(3):
@@ -62,7 +62,7 @@ def test_invalid_shape_is_rejected(device):
             clip.set_tuning(5, 32)
         with pytest.raises(abi.LgcrError):
             clip.set_tuning(8, 48)
-        assert clip.tuning() == (8, 32)
+        assert clip.tuning() == (16, 32)
     finally:
         clip.close()
 

@@ -366,93 +366,153 @@ __global__ __launch_bounds__(kBlock) void k_bp_pack(float *ou, float *ov, int ow
 // ---------------------------------------------------------------------------
 // structure tensor + fused trust seed (sobelTensor3x3, maps.cpp:64-168)
 //
-// One 64xTH output tile per block of 64x4 threads (each thread steps down
-// the tile by 4 rows; TH is tuned per clip, 8 by default).  LDS holds the luma tile with a 2-pixel halo, the
-// Sobel gradients (1-pixel halo) evaluated at clamped centres, and the
-// vertical 3-row sums, so every border rule of the reference falls out of
-// the clamped centres; tiles whose halo lies inside the image skip the
-// clamping.
+// One wave per 64 x TH strip (TH tuned per clip, 16 by default).  Lane L
+// owns column x0 + L and walks down the strip: the three luma rows of the
+// Sobel stencil (at clamped centres and taps, the reference's border rule)
+// and the last three rows of gradient products stay in registers, so each
+// luma row is read once and each product formed once; the column sums of
+// the 3x3 box come from the neighbouring lanes (DPP wave shifts), and lanes
+// 0 and 63 also form the strip's halo columns x0 - 1 and x0 + 64.  Every
+// sum keeps the reference's operand order.
 // ---------------------------------------------------------------------------
 
-constexpr int TW = 64, TR = 4;
+constexpr int kSobelW = 64;
+
+// v of the lane to the left (wave_shr:1) / right (wave_shl:1); lanes 0 and
+// 63 get `edge` instead
+__device__ __forceinline__ float from_left(float v, float edge, int lane) {
+    const float l = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+    return lane == 0 ? edge : l;
+}
+__device__ __forceinline__ float from_right(float v, float edge, int lane) {
+    const float r = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+    return lane == kSobelW - 1 ? edge : r;
+}
+
+// One strip.  EDGE: the strip touches the image border, where the stencil
+// rows clamp and the box sums of column 0 / w - 1 repeat the edge column;
+// otherwise every stencil row is inside and the strip simply rolls down one
+// luma row per step.  Its own instantiation keeps the border logic out of
+// the interior strips.  The row loop is unrolled, so the rolling rows and
+// products rotate through registers without copies.  Luma is read through
+// a buffer resource: a wave-uniform row offset plus the per-lane column
+// offsets, which are fixed for the strip.
+template <int TH, bool EDGE>
+__device__ __forceinline__ void sobel_strip(const float *__restrict__ y, int w, int h, float4 *yt, uint64_t *seed,
+                                            int w64, float thr, int x0, int y0, int word) {
+    const int lane = threadIdx.x;
+    const int x = x0 + lane;
+    // clamped stencil columns (byte offsets) of the own column and of the
+    // halo column (x0 - 1 for lane 0, x0 + 64 for lane 63; unused elsewhere)
+    const int cx = clampi(x, 0, w - 1), hc = clampi(lane == 0 ? x0 - 1 : x0 + kSobelW, 0, w - 1);
+    const uint32_t oxl = 4u * clampi(cx - 1, 0, w - 1), oxc = 4u * cx, oxr = 4u * clampi(cx + 1, 0, w - 1);
+    const uint32_t ohl = 4u * clampi(hc - 1, 0, w - 1), ohc = 4u * hc, ohr = 4u * clampi(hc + 1, 0, w - 1);
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(y), 0, (int)(4u * (uint32_t)w * (uint32_t)h), 0x00020000);
+    struct Row {
+        float l, c, r, hl, hc, hr;
+    };
+    auto load_row = [&](int r) {
+        const int ro = (int)(4u * (uint32_t)r * (uint32_t)w);
+        auto ld = [&](uint32_t off) { return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsrc, off, ro, 0)); };
+        return Row{ld(oxl), ld(oxc), ld(oxr), ld(ohl), ld(ohc), ld(ohr)};
+    };
+    // rows (iu, ic, id) = (clamp(cy - 1), cy, clamp(cy + 1)) of the current
+    // gradient centre cy
+    int iu = -1, ic = -1, id = -1;
+    Row ru{}, rc{}, rd{};
+    if constexpr (!EDGE) {
+        rc = load_row(y0 - 2);
+        rd = load_row(y0 - 1);
+    }
+    // gradient products of the last three gradient rows ([0] the oldest)
+    float pxx[3] = {}, pxy[3] = {}, pyy[3] = {}, hxx[3] = {}, hxy[3] = {}, hyy[3] = {};
+    float yc_prev = 0.0f;  // luma at (x, previous gradient centre)
+    const float ninth = 1.0f / 9.0f;
+    _Pragma("unroll") for (int k = y0 - 1, step = 0; step < TH + 2; ++k, ++step) {
+        const float y_out = yc_prev;  // luma of output row k - 1
+        if constexpr (EDGE) {
+            const int cy = clampi(k, 0, h - 1);
+            const int yu = clampi(cy - 1, 0, h - 1), yd = clampi(cy + 1, 0, h - 1);
+            if (yu == ic && cy == id) {
+                ru = rc;
+                rc = rd;
+                rd = load_row(yd);
+            } else if (!(yu == iu && cy == ic && yd == id)) {
+                ru = load_row(yu);
+                rc = load_row(cy);
+                rd = load_row(yd);
+            }
+            iu = yu;
+            ic = cy;
+            id = yd;
+        } else {
+            ru = rc;
+            rc = rd;
+            rd = load_row(k + 1);
+        }
+        yc_prev = rc.c;
+        // Sobel / 8 at the clamped centre
+        const float gx = (ru.r + 2.0f * rc.r + rd.r - ru.l - 2.0f * rc.l - rd.l) * 0.125f;
+        const float gy = (rd.l + 2.0f * rd.c + rd.r - ru.l - 2.0f * ru.c - ru.r) * 0.125f;
+        const float hgx = (ru.hr + 2.0f * rc.hr + rd.hr - ru.hl - 2.0f * rc.hl - rd.hl) * 0.125f;
+        const float hgy = (rd.hl + 2.0f * rd.hc + rd.hr - ru.hl - 2.0f * ru.hc - ru.hr) * 0.125f;
+        pxx[0] = pxx[1]; pxx[1] = pxx[2]; pxx[2] = gx * gx;
+        pxy[0] = pxy[1]; pxy[1] = pxy[2]; pxy[2] = gx * gy;
+        pyy[0] = pyy[1]; pyy[1] = pyy[2]; pyy[2] = gy * gy;
+        hxx[0] = hxx[1]; hxx[1] = hxx[2]; hxx[2] = hgx * hgx;
+        hxy[0] = hxy[1]; hxy[1] = hxy[2]; hxy[2] = hgx * hgy;
+        hyy[0] = hyy[1]; hyy[1] = hyy[2]; hyy[2] = hgy * hgy;
+        if (step < 2)
+            continue;
+        const int r = k - 1;  // output row: gradient rows r - 1, r, r + 1
+        if (EDGE && r >= h)
+            break;  // uniform
+        // vertical 3-sums of the products at this column and the halo
+        const float vxx = pxx[0] + pxx[1] + pxx[2], vxy = pxy[0] + pxy[1] + pxy[2], vyy = pyy[0] + pyy[1] + pyy[2];
+        const float wxx = hxx[0] + hxx[1] + hxx[2], wxy = hxy[0] + hxy[1] + hxy[2], wyy = hyy[0] + hyy[1] + hyy[2];
+        const float lxx = from_left(vxx, wxx, lane), lxy = from_left(vxy, wxy, lane), lyy = from_left(vyy, wyy, lane);
+        const float rxx = from_right(vxx, wxx, lane), rxy = from_right(vxy, wxy, lane),
+                    ryy = from_right(vyy, wyy, lane);
+        float oxx = ninth * ((lxx + vxx) + rxx);
+        float oxy = ninth * ((lxy + vxy) + rxy);
+        float oyy = ninth * ((lyy + vyy) + ryy);
+        if constexpr (EDGE) {
+            if (x == 0) {
+                // the box repeats the edge column
+                const float nxx = w > 1 ? rxx : vxx, nxy = w > 1 ? rxy : vxy, nyy = w > 1 ? ryy : vyy;
+                oxx = (2.0f * vxx + nxx) / 9.0f;
+                oxy = (2.0f * vxy + nxy) / 9.0f;
+                oyy = (2.0f * vyy + nyy) / 9.0f;
+            } else if (x == w - 1) {
+                oxx = (lxx + 2.0f * vxx) / 9.0f;
+                oxy = (lxy + 2.0f * vxy) / 9.0f;
+                oyy = (lyy + 2.0f * vyy) / 9.0f;
+            }
+        }
+        const bool inside = x < w;
+        // trust seed (fused in buildGuideMaps, maps.cpp:252-276) as one bit
+        // per pixel: the wave's ballot is the row's 64-pixel word
+        const uint64_t bits = __ballot(inside && oxx + oyy > thr);
+        if (seed && lane == 0)
+            seed[(size_t)r * w64 + word] = bits;
+        if (inside)
+            yt[(uint32_t)r * (uint32_t)w + (uint32_t)x] = make_float4(y_out, oxx, oxy, oyy);
+    }
+}
 
 template <int TH>
-__global__ __launch_bounds__(TW * TR) void k_sobel_tensor(const float *__restrict__ y, int w, int h,
-                                                         float4 *yt, uint64_t *seed, int w64, float thr) {
-    __shared__ float sy[TH + 4][TW + 4];
-    __shared__ float gx[TH + 2][TW + 2], gy[TH + 2][TW + 2];
-    __shared__ float vxx[TH][TW + 2], vxy[TH][TW + 2], vyy[TH][TW + 2];
+__global__ __launch_bounds__(kSobelW) void k_sobel_tensor(const float *__restrict__ y, int w, int h, float4 *yt,
+                                                         uint64_t *seed, int w64, float thr) {
     unsigned bx, by;
     xcd_block2(bx, by);
-    const int x0 = bx * TW, y0 = by * TH;
-    const int tx = threadIdx.x, ty = threadIdx.y;
-    const int tid = ty * TW + tx;
-    for (int i = tid; i < (TH + 4) * (TW + 4); i += TW * TR) {
-        const int ly = i / (TW + 4), lx = i % (TW + 4);
-        sy[ly][lx] = y[(uint32_t)(clampi(y0 - 2 + ly, 0, h - 1) * w + clampi(x0 - 2 + lx, 0, w - 1))];
-    }
-    __syncthreads();
-    const bool interior = x0 >= 2 && x0 + TW + 1 < w && y0 >= 2 && y0 + TH + 1 < h;
-    for (int i = tid; i < (TH + 2) * (TW + 2); i += TW * TR) {
-        const int ly = i / (TW + 2), lx = i % (TW + 2);
-        int xl, xc, xr, yu, yc, yd;
-        if (interior) {
-            xl = lx; xc = lx + 1; xr = lx + 2;
-            yu = ly; yc = ly + 1; yd = ly + 2;
-        } else {
-            const int cx = clampi(x0 - 1 + lx, 0, w - 1), cy = clampi(y0 - 1 + ly, 0, h - 1);
-            // tap coordinates clamped to the image, then made tile-relative
-            xl = clampi(cx - 1, 0, w - 1) - (x0 - 2); xc = cx - (x0 - 2); xr = clampi(cx + 1, 0, w - 1) - (x0 - 2);
-            yu = clampi(cy - 1, 0, h - 1) - (y0 - 2); yc = cy - (y0 - 2); yd = clampi(cy + 1, 0, h - 1) - (y0 - 2);
-        }
-        const float tl = sy[yu][xl], t = sy[yu][xc], tr = sy[yu][xr];
-        const float l = sy[yc][xl], r = sy[yc][xr];
-        const float bl = sy[yd][xl], b = sy[yd][xc], br = sy[yd][xr];
-        gx[ly][lx] = (tr + 2.0f * r + br - tl - 2.0f * l - bl) * 0.125f;
-        gy[ly][lx] = (bl + 2.0f * b + br - tl - 2.0f * t - tr) * 0.125f;
-    }
-    __syncthreads();
-    for (int i = tid; i < TH * (TW + 2); i += TW * TR) {
-        const int ly = i / (TW + 2), lx = i % (TW + 2);
-        const float a0 = gx[ly][lx], a1 = gx[ly + 1][lx], a2 = gx[ly + 2][lx];
-        const float b0 = gy[ly][lx], b1 = gy[ly + 1][lx], b2 = gy[ly + 2][lx];
-        vxx[ly][lx] = a0 * a0 + a1 * a1 + a2 * a2;
-        vxy[ly][lx] = a0 * b0 + a1 * b1 + a2 * b2;
-        vyy[ly][lx] = b0 * b0 + b1 * b1 + b2 * b2;
-    }
-    __syncthreads();
-    const int x = x0 + tx;
-    const int c = tx + 1;  // column of x in the V arrays
-    for (int ly = ty; ly < TH; ly += TR) {
-        const int r = y0 + ly;
-        const bool inside = x < w && r < h;
-        float oxx = 0.0f, oxy = 0.0f, oyy = 0.0f;
-        if (!inside) {
-        } else if (x == 0) {
-            const int c1 = (w > 1 ? 1 : 0) - (x0 - 1);
-            oxx = (2.0f * vxx[ly][c] + vxx[ly][c1]) / 9.0f;
-            oxy = (2.0f * vxy[ly][c] + vxy[ly][c1]) / 9.0f;
-            oyy = (2.0f * vyy[ly][c] + vyy[ly][c1]) / 9.0f;
-        } else if (x == w - 1) {
-            oxx = (vxx[ly][c - 1] + 2.0f * vxx[ly][c]) / 9.0f;
-            oxy = (vxy[ly][c - 1] + 2.0f * vxy[ly][c]) / 9.0f;
-            oyy = (vyy[ly][c - 1] + 2.0f * vyy[ly][c]) / 9.0f;
-        } else {
-            const float ninth = 1.0f / 9.0f;
-            oxx = ninth * ((vxx[ly][c - 1] + vxx[ly][c]) + vxx[ly][c + 1]);
-            oxy = ninth * ((vxy[ly][c - 1] + vxy[ly][c]) + vxy[ly][c + 1]);
-            oyy = ninth * ((vyy[ly][c - 1] + vyy[ly][c]) + vyy[ly][c + 1]);
-        }
-        // trust seed (fused in buildGuideMaps, maps.cpp:252-276) as one bit
-        // per pixel: a wave is one 64-pixel tile row, its ballot one word
-        const uint64_t bits = __ballot(inside && oxx + oyy > thr);
-        if (r >= h)
-            break;
-        if (seed && tx == 0)
-            seed[(size_t)r * w64 + bx] = bits;
-        if (inside)
-            yt[(uint32_t)r * (uint32_t)w + (uint32_t)x] = make_float4(sy[ly + 2][tx + 2], oxx, oxy, oyy);
-    }
+    const int x0 = (int)bx * kSobelW, y0 = (int)by * TH;
+    // interior: stencil rows y0 - 2 .. y0 + TH + 1 inside, neither border
+    // column in the strip
+    if (x0 == 0 || x0 + kSobelW >= w || y0 < 2 || y0 + TH + 1 > h - 1)
+        sobel_strip<TH, true>(y, w, h, yt, seed, w64, thr, x0, y0, (int)bx);
+    else
+        sobel_strip<TH, false>(y, w, h, yt, seed, w64, thr, x0, y0, (int)bx);
 }
 
 // ---------------------------------------------------------------------------
@@ -4012,15 +4072,16 @@ hipError_t launch_from_float(const RawOut *planes, int count, int type, hipStrea
 
 hipError_t launch_sobel_tensor(const float *y, int w, int h, float4 *yt, uint64_t *seed, float threshold,
                                hipStream_t s, int tile_rows) {
-#define LGCR_SOBEL(TH_)                                                                                    \
-    hipLaunchKernelGGL(k_sobel_tensor<TH_>, dim3((unsigned)((w + TW - 1) / TW), (unsigned)((h + TH_ - 1) / TH_)), \
-                       dim3(TW, TR), 0, s, y, w, h, yt, seed, (w + 63) / 64, threshold)
-    if (tile_rows == 4)
-        LGCR_SOBEL(4);
-    else if (tile_rows == 16)
-        LGCR_SOBEL(16);
-    else
+#define LGCR_SOBEL(TH_)                                                                                        \
+    hipLaunchKernelGGL(k_sobel_tensor<TH_>,                                                                     \
+                       dim3((unsigned)((w + kSobelW - 1) / kSobelW), (unsigned)((h + TH_ - 1) / TH_)),          \
+                       dim3(kSobelW), 0, s, y, w, h, yt, seed, (w + 63) / 64, threshold)
+    if (tile_rows == 8)
         LGCR_SOBEL(8);
+    else if (tile_rows == 32)
+        LGCR_SOBEL(32);
+    else
+        LGCR_SOBEL(16);
 #undef LGCR_SOBEL
     return hipGetLastError();
 }

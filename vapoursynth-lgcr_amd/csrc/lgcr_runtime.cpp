@@ -835,7 +835,7 @@ struct lgcr_clip {
     bool same = true;
     bool capture = false;
     std::atomic<bool> graphs{false};  // replay captured frame graphs for device-resident Recon frames
-    lgcr_tuning tune{8, 32};          // tile shapes (lgcr_clip_autotune)
+    lgcr_tuning tune{16, 32};         // tile shapes (lgcr_clip_autotune)
     bool temporal = false;  // TRecon clip (neighbour buffers)
     bool sharpen = false;   // Sharpen clip
     bool downsample = false;  // Downsample clip
@@ -1352,7 +1352,7 @@ void tuned_lookup(lgcr_clip *c) {
 }
 
 bool tuning_valid(const lgcr_tuning &t) {
-    return (t.tensor_tile_rows == 4 || t.tensor_tile_rows == 8 || t.tensor_tile_rows == 16) &&
+    return (t.tensor_tile_rows == 8 || t.tensor_tile_rows == 16 || t.tensor_tile_rows == 32) &&
            (t.gate_tile_cols == 16 || t.gate_tile_cols == 32 || t.gate_tile_cols == 64);
 }
 
@@ -3282,7 +3282,7 @@ lgcr_status lgcr_clip_set_tuning(lgcr_clip *c, const lgcr_tuning *t) {
     if (!c || !t)
         return fail(LGCR_INVALID_ARGUMENT, "lgcr_clip_set_tuning: NULL argument");
     if (!tuning_valid(*t))
-        return fail(LGCR_INVALID_ARGUMENT, "lgcr_clip_set_tuning: tensor tile rows must be 4, 8 or 16 and gate "
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_clip_set_tuning: tensor tile rows must be 8, 16 or 32 and gate "
                                            "tile columns 16, 32 or 64");
     c->tune = *t;
     return LGCR_OK;
@@ -3381,7 +3381,7 @@ lgcr_status lgcr_clip_autotune(lgcr_clip *c, const lgcr_frame *frame) {
         // the tensor pass rewrites this frame's tensor and seed with the same values
         const double thr = p.algo == 4 ? 0.25 * p.sigma : p.sigma;
         float best_ms = 1e30f;
-        for (int rows : {4, 8, 16}) {
+        for (int rows : {8, 16, 32}) {
             float ms = 0.0f;
             HIP_TRY(time_it([&] { return launch_sobel_tensor(w->y, c->sw, c->sh, w->yt, p.sparse ? w->seed : nullptr,
                                                             float(thr * thr), s, rows); }, ms));
