@@ -946,7 +946,10 @@ __global__ __launch_bounds__(kBlock) void k_gate_fused(const float *__restrict__
     __shared__ float2 gvv[kGateGH][kGateGW];
     __shared__ int s_dense;
     unsigned bx, by;
-    xcd_block2(bx, by);
+    // hardware (round-robin) block order: the work per block follows the
+    // chroma mask, and round-robin spreads it evenly over the XCDs
+    bx = blockIdx.x;
+    by = blockIdx.y;
     const int tx0 = (int)bx * kGateTW, ty0 = (int)by * kGateTH;
     const int gx0 = tx0 - kGateHalo, gy0 = ty0 - kGateHalo, sx0 = gx0 - 1, sy0 = gy0 - 1;
     const int x = tx0 + (int)(threadIdx.x % kGateTW), y = ty0 + (int)(threadIdx.x / kGateTW);
