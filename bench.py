@@ -257,15 +257,17 @@ WORKLOADS = (
 )
 
 
-def workloads(args):
+def workloads(args, only=None):
     """fps, per-stage device time and the dominant stage's HBM fraction for
-    each entry of WORKLOADS."""
+    each entry of WORKLOADS (only: that entry alone)."""
     import numpy as np
     import torch
     from lgcr_amd import abi, synth
     dev = abi.Device(0)
     out = {}
     for name, desc, W, H, ssw, ssh, bits, ow, oh, kw, noise, n in WORKLOADS:
+        if only is not None and name != only:
+            continue
         bps = 4 if bits == 32 else (1 if bits <= 8 else 2)
         np_dt = {1: np.uint8, 2: np.uint16, 4: np.float32}[bps]
         tdt = {1: torch.uint8, 2: torch.int16, 4: torch.float32}[bps]
@@ -376,12 +378,28 @@ def main():
     ap.add_argument("--no-host-path", action="store_true")
     ap.add_argument("--graphs", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--section", default=None,
+                    help="run one section alone and print its JSON (for profiling): downsample, trecon, serving, "
+                         "or a workload name")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(Recon is data-parallel over frames: see the sharding notes)", file=sys.stderr)
         sys.exit(2)
+
+    if args.section:
+        names = [w[0] for w in WORKLOADS]
+        if args.section in ("downsample", "trecon", "serving"):
+            out = {"downsample": downsample, "trecon": trecon, "serving": serving}[args.section](args)
+        elif args.section in names:
+            out = workloads(args, only=args.section)
+        else:
+            print(f"bench.py: unknown section {args.section!r} (downsample, trecon, serving, {', '.join(names)})",
+                  file=sys.stderr)
+            sys.exit(2)
+        print(json.dumps({"section": args.section, **out}))
+        return
 
     import numpy as np
     import torch
