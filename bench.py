@@ -327,7 +327,8 @@ def serving(args):
     resident = sum(c.src_bytes + c.out_bytes for c in clips)
     rates = [float(r) for r in str(args.serve_rate).split(",") if r]
     out = {"clips": len(specs), "frames_per_clip": args.serve_frames, "offered_fps": rates,
-           "poll_us": args.serve_poll_us, "max_inflight": args.serve_inflight, "graphs": args.serve_graphs,
+           "poll_us": args.serve_poll_us, "max_inflight": args.serve_inflight, "max_work": args.serve_max_work,
+           "graphs": args.serve_graphs,
            "mix": {k: sum(1 for s in specs if s.kind == k) for k in ("recon", "trecon")},
            "resident_frames_gb": round(resident / 2**30, 3)}
     loads = [(f"offered{r:g}", serve.poisson_requests(specs, r, seed=0)) for r in rates]
@@ -338,7 +339,7 @@ def serving(args):
     for name, reqs in loads:
         for mode in ("continuous", "per_frame"):
             st = serve.serve(clips, reqs, mode=mode, poll_sleep=args.serve_poll_us * 1e-6,
-                             max_inflight=args.serve_inflight)
+                             max_inflight=args.serve_inflight, max_work=args.serve_max_work)
             out[f"{name}_{mode}"] = {"fps": round(st["fps"], 1), "p50_ms": round(st["p50_ms"], 3),
                                      "p99_ms": round(st["p99_ms"], 3)}
             out["peak_hbm_gb"] = round(max(out.get("peak_hbm_gb", 0.0),
@@ -375,6 +376,8 @@ def main():
     ap.add_argument("--serve-poll-us", type=float, default=20.0)
     ap.add_argument("--serve-inflight", type=int, default=4)
     ap.add_argument("--serve-graphs", action="store_true")
+    ap.add_argument("--serve-max-work", type=float, default=None,
+                    help="continuous mode: relative work allowed in flight (1080p Recon = 1) unless idle")
     ap.add_argument("--no-host-path", action="store_true")
     ap.add_argument("--graphs", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
