@@ -1407,20 +1407,49 @@ __global__ __launch_bounds__(kBlock) void k_plain_radial_tab(const float *__rest
     const int tx0 = ax.start[ox], ty0 = ay.start[oy];
     double au = 0.0, av = 0.0;
     float mnu = 1e30f, mxu = -1e30f, mnv = 1e30f, mxv = -1e30f;
-    for (int j = 0; j < SY; ++j) {
-        if (amy[j] == 0.0f)
-            continue;
-        const size_t row = (size_t)clampi(ty0 + j, 0, ch - 1) * cw;
-        _Pragma("unroll") for (int i = 0; i < SX; ++i) {
-            if (amx[i] == 0.0f)
+    if constexpr (SUP == 6) {
+        // the pair's weights and the tap activity (wave-uniform) read up
+        // front, so no tap waits on a scalar load
+        float wr[36];
+        _Pragma("unroll") for (int t = 0; t < 36; ++t)
+            wr[t] = base[t];
+        uint32_t mx = 0, my = 0;
+        _Pragma("unroll") for (int t = 0; t < 6; ++t) {
+            mx |= (amx[t] != 0.0f ? 1u : 0u) << t;
+            my |= (amy[t] != 0.0f ? 1u : 0u) << t;
+        }
+        _Pragma("unroll") for (int j = 0; j < 6; ++j) {
+            if (!((my >> j) & 1u))
                 continue;
-            const float w = base[j * SX + i];
-            const size_t q = row + clampi(tx0 + i, 0, cw - 1);
-            const float a = u[q], b = v[q];
-            au = dacc(au, w, a);
-            av = dacc(av, w, b);
-            mnu = fmn(mnu, a); mxu = fmx(mxu, a);
-            mnv = fmn(mnv, b); mxv = fmx(mxv, b);
+            const size_t row = (size_t)clampi(ty0 + j, 0, ch - 1) * cw;
+            _Pragma("unroll") for (int i = 0; i < 6; ++i) {
+                if (!((mx >> i) & 1u))
+                    continue;
+                const float w = wr[j * 6 + i];
+                const size_t q = row + clampi(tx0 + i, 0, cw - 1);
+                const float a = u[q], b = v[q];
+                au = dacc(au, w, a);
+                av = dacc(av, w, b);
+                mnu = fmn(mnu, a); mxu = fmx(mxu, a);
+                mnv = fmn(mnv, b); mxv = fmx(mxv, b);
+            }
+        }
+    } else {
+        for (int j = 0; j < SY; ++j) {
+            if (amy[j] == 0.0f)
+                continue;
+            const size_t row = (size_t)clampi(ty0 + j, 0, ch - 1) * cw;
+            _Pragma("unroll") for (int i = 0; i < SX; ++i) {
+                if (amx[i] == 0.0f)
+                    continue;
+                const float w = base[j * SX + i];
+                const size_t q = row + clampi(tx0 + i, 0, cw - 1);
+                const float a = u[q], b = v[q];
+                au = dacc(au, w, a);
+                av = dacc(av, w, b);
+                mnu = fmn(mnu, a); mxu = fmx(mxu, a);
+                mnv = fmn(mnv, b); mxv = fmx(mxv, b);
+            }
         }
     }
     const double ws = rd.wsa[2 * pair], sa = rd.wsa[2 * pair + 1];
@@ -3060,10 +3089,22 @@ __global__ __launch_bounds__(kBlock) void k_resample_radial_tab(const float *__r
     const double *__restrict__ wt = T.w + (size_t)pair * S * S;
     const int x0 = T.x0[ox], y0 = T.y0[oy];
     double acc = 0.0;
-    for (int j = 0; j < S; ++j) {
-        const float *row = src + (size_t)clampi(y0 + j, 0, sh - 1) * sw;
-        _Pragma("unroll") for (int i = 0; i < S; ++i)
-            acc = __builtin_fma(wt[j * S + i], (double)row[clampi(x0 + i, 0, sw - 1)], acc);
+    if constexpr (SUP == 6) {
+        // the pair's weights (wave-uniform) read up front
+        double wr[36];
+        _Pragma("unroll") for (int t = 0; t < 36; ++t)
+            wr[t] = wt[t];
+        _Pragma("unroll") for (int j = 0; j < 6; ++j) {
+            const float *row = src + (size_t)clampi(y0 + j, 0, sh - 1) * sw;
+            _Pragma("unroll") for (int i = 0; i < 6; ++i)
+                acc = __builtin_fma(wr[j * 6 + i], (double)row[clampi(x0 + i, 0, sw - 1)], acc);
+        }
+    } else {
+        for (int j = 0; j < S; ++j) {
+            const float *row = src + (size_t)clampi(y0 + j, 0, sh - 1) * sw;
+            _Pragma("unroll") for (int i = 0; i < S; ++i)
+                acc = __builtin_fma(wt[j * S + i], (double)row[clampi(x0 + i, 0, sw - 1)], acc);
+        }
     }
     const double wsum = T.wsum[pair];
     dst[(size_t)oy * dw + ox] = (float)(wsum > 1e-6 ? acc / wsum : 0.0);
