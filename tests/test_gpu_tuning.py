@@ -1,5 +1,5 @@
 """Tile-shape tuning: every candidate shape of the tunable kernels (Sobel
-tensor 64x{4,8,16}, mutual gate 16x16 / 32x8 / 64x4) gives the oracle's
+tensor 64x{8,16,32}, mutual gate 16x16 / 32x8 / 64x4) gives the oracle's
 frames, the autotuner picks one of them and remembers it per shape key."""
 import itertools
 
