@@ -315,6 +315,57 @@ def workloads(args, only=None):
     return out
 
 
+def batched_launch(args):
+    """Multi-frame fused tile launches (lgcr_clip_set_batching) against one
+    launch per frame, on device-resident frames: the 640x360 YUV420P8
+    workload with Recon defaults and the 1080p P10 benchmark workload.
+    Per frame: eight and sixteen workspaces; batched: sixteen workspaces,
+    groups of four frames on four streams, as one wave per tile over a
+    (tiles, frames) grid and as a persistent grid."""
+    import numpy as np
+    import torch
+    from lgcr_amd import abi, synth
+    dev = abi.Device(0)
+    out = {}
+    cases = (("640x360_420p8", "Recon 640x360 YUV420P8 -> YUV444P8, defaults (algo 2, lanczos3, strength 0.8)",
+              640, 360, 8, {}, 480),
+             ("1080p420p10", "Recon 1920x1080 YUV420P10 -> YUV444P10, algo 2, lanczos3, strength 0.8, bp 0.5",
+              1920, 1080, 10, {"bp": 0.5}, 96))
+    for name, desc, W, H, bits, kw, n in cases:
+        bps = 1 if bits <= 8 else 2
+        np_dt = np.uint8 if bps == 1 else np.uint16
+        tdt = torch.uint8 if bps == 1 else torch.int16
+        frames = synth.clip(n, W, H, bits=bits, seed=0, scene_per_frame=True, threads=min(16, os.cpu_count() or 1))
+        din = [[torch.from_numpy(p.view(np.int16) if bps == 2 else p).to("cuda") for p in f] for f in frames]
+        dout = [[torch.empty((H, W), dtype=tdt, device="cuda") for _ in range(2)] for _ in frames]
+        fin = [[(t.data_ptr(), t.stride(0) * bps) for t in f] for f in din]
+        fout = [[(0, 0)] + [(t.data_ptr(), t.stride(0) * bps) for t in o] for o in dout]
+        fmt = abi.make_format(np_dt, W, H, bits=None if bits == 8 else bits)
+        params = abi.default_params(**kw)
+        res = {"workload": desc, "frames": n}
+        for tag, inflight, batching, persistent in (("per_frame_8", 8, 0, False), ("per_frame_16", 16, 0, False),
+                                                   ("batched_4x4_grid", 16, 4, False),
+                                                   ("batched_4x4_persistent", 16, 4, True)):
+            os.environ["LGCR_BATCH_PERSISTENT"] = "1" if persistent else "0"
+            clip = abi.Clip(dev, params, fmt, max_inflight=inflight)
+            os.environ.pop("LGCR_BATCH_PERSISTENT")
+            clip.set_batching(batching)
+            clip.recon_batch(fin, fout)
+            torch.cuda.synchronize()
+            steps = 3
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                clip.recon_batch(fin, fout)
+            torch.cuda.synchronize()
+            res[f"{tag}_fps"] = round(steps * n / (time.perf_counter() - t0), 1)
+            clip.close()
+        out[name] = res
+        del din, dout, fin, fout
+        torch.cuda.empty_cache()
+    dev.close()
+    return out
+
+
 def serving(args):
     """Clips of mixed resolution / format / filter with Poisson frame
     arrivals: continuous batching vs one blocking call per frame, at the
@@ -373,6 +424,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-downsample", action="store_true")
     ap.add_argument("--no-trecon", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--serve-poll-us", type=float, default=20.0)
     ap.add_argument("--serve-inflight", type=int, default=4)
     ap.add_argument("--serve-graphs", action="store_true")
@@ -380,6 +432,8 @@ def main():
                     help="continuous mode: relative work allowed in flight (1080p Recon = 1) unless idle")
     ap.add_argument("--no-host-path", action="store_true")
     ap.add_argument("--graphs", action="store_true")
+    ap.add_argument("--batching", type=int, default=0,
+                    help="frames per multi-frame fused tile launch (0: one launch per frame)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--section", default=None,
                     help="run one section alone and print its JSON (for profiling): downsample, trecon, serving, "
@@ -393,12 +447,14 @@ def main():
 
     if args.section:
         names = [w[0] for w in WORKLOADS]
-        if args.section in ("downsample", "trecon", "serving"):
-            out = {"downsample": downsample, "trecon": trecon, "serving": serving}[args.section](args)
+        if args.section in ("downsample", "trecon", "serving", "batched"):
+            out = {"downsample": downsample, "trecon": trecon, "serving": serving,
+                   "batched": batched_launch}[args.section](args)
         elif args.section in names:
             out = workloads(args, only=args.section)
         else:
-            print(f"bench.py: unknown section {args.section!r} (downsample, trecon, serving, {', '.join(names)})",
+            print(f"bench.py: unknown section {args.section!r} (downsample, trecon, serving, batched, "
+                  f"{', '.join(names)})",
                   file=sys.stderr)
             sys.exit(2)
         print(json.dumps({"section": args.section, **out}))
@@ -430,6 +486,7 @@ def main():
     clip = abi.Clip(dev, params, abi.make_format(np_dt, W, H, bits=None if bits == 32 else bits),
                     max_inflight=args.inflight)
     clip.set_graphs(args.graphs)
+    clip.set_batching(args.batching)
     # tile shapes of the tunable kernels for this frame size (autotuner)
     tuning = clip.autotune_device(fin[0]) if not args.no_autotune else clip.tuning()
     for _ in range(args.warmup):
@@ -489,6 +546,7 @@ def main():
         "warmup": args.warmup,
         "ms_per_step": round(elapsed / args.steps * 1000.0, 3),
         "frame_graphs": args.graphs,
+        "batching": args.batching,
         "tuning": {"tensor_tile_rows": tuning[0], "gate_tile_cols": tuning[1], "autotuned": not args.no_autotune},
         "stage_pass": {"inflight": 1, "fps": round(n / (t3 - t2), 2)},
         "higher_is_better": True,
@@ -537,6 +595,9 @@ def main():
 
     if not args.no_workloads:
         result["workloads"] = workloads(args)
+
+    if not args.no_batched:
+        result["batched_launch"] = batched_launch(args)
 
     if not args.no_serve:
         result["serving"] = serving(args)

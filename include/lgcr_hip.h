@@ -284,6 +284,22 @@ lgcr_status lgcr_clip_stage_times(lgcr_clip *clip, double *us, uint64_t *launche
  * on the host). */
 lgcr_status lgcr_clip_set_graphs(lgcr_clip *clip, int enable);
 
+/* Multi-frame launches for lgcr_recon_batch: with frames_per_launch > 0 the
+ * frames of a batch call go in groups of up to that many (at most the
+ * clip's workspace count, at most 64), spread over up to four streams; each
+ * frame runs its stages up to the mutual gate on its own workspace, then ONE
+ * launch computes the fused tile pass (plain base, guided correction,
+ * back-projection, output packing) over the tiles of the whole group.
+ * Results are identical to frame-by-frame launches.  Pays for frames too
+ * small to fill the device on their own (640x360); at 1080p frame-by-frame
+ * launches on several streams are as fast or faster.  0 (the default) turns
+ * it off.  Applies to device-resident frames without stage timing, stage
+ * capture or frame graphs; frames whose guided pass is not the fused tile
+ * pass complete on their group's stream.  No counterpart in the reference
+ * (its frames run one at a time on the host). */
+lgcr_status lgcr_clip_set_batching(lgcr_clip *clip, int frames_per_launch);
+int lgcr_clip_get_batching(const lgcr_clip *clip);
+
 /* Tile shapes of the kernels whose best shape depends on the frame size.
  * No counterpart in the reference (its CPU stages have no tiles). */
 typedef struct lgcr_tuning {

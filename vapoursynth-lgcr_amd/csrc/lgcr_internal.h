@@ -246,7 +246,34 @@ hipError_t launch_guided(const float *u, const float *v, int cw, int ch, const f
                          // non-null: the tiled pass also back-projects
                          // (bp_on) and writes these two output planes of
                          // out_type itself (out_u/out_v are not written)
-                         const RawOut *fused_out = nullptr, int out_type = 0, float bp = 0.0f, int bp_on = 0);
+                         const RawOut *fused_out = nullptr, int out_type = 0, float bp = 0.0f, int bp_on = 0,
+                         // non-null and the pass is the fused tile pass:
+                         // nothing is launched, the frame's arguments go here
+                         // for launch_guided_batch (tiles > 0 when deferred)
+                         struct GuidedDeferred *defer = nullptr);
+// A frame's fused tile pass held back for a multi-frame launch: the kernel
+// arguments (opaque here), the kernel variant and the frame's tile count.
+constexpr size_t kGuidedDeferBytes = 1024;
+struct GuidedDeferred {
+    alignas(16) unsigned char args[kGuidedDeferBytes];
+    int variant = -1;
+    int tiles = 0;
+};
+// Device and pinned host tables of one multi-frame launch.
+constexpr int kBatchMaxFrames = 64;
+// lanes (streams) of a clip's multi-frame launches: the hardware queues a
+// process gets by default
+constexpr int kBatchLanes = 4;
+size_t guided_batch_table_bytes();
+// One launch of the fused tile pass over the deferred frames items[0..n)
+// (all of one variant, n <= kBatchMaxFrames): the tables go through the
+// pinned host staging `host` into `dev` (both guided_batch_table_bytes()
+// long, the staging not rewritten until this launch's work has finished),
+// then either one wave per tile over a (tiles, frames) grid, or (persistent)
+// a grid of the device's resident workgroups strides over the tiles of all
+// frames.
+hipError_t launch_guided_batch(const GuidedDeferred *items, int n, void *dev, void *host, bool persistent,
+                               hipStream_t s);
 // whether every 8x8 output tile's chroma footprint (6 taps per axis) fits
 // the LDS window of the tiled guided pass
 int guided_tiles_fit(const int *xs, int ow, const int *ys, int oh);

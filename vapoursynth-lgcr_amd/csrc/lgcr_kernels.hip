@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstring>
 
 namespace lgcr_hip {
 namespace {
@@ -1501,6 +1502,16 @@ struct GuidedArgs {
     int bp_on;
 };
 
+// One frame of a multi-frame tile launch: its arguments and tile grid.
+struct BatchFrame {
+    GuidedArgs A;
+    int tiles_x, tiles;
+};
+// kernel variant of a fused multi-frame tile launch: temporal taps (TRecon,
+// no plain base), plain hull, output sample type (1 = 8-bit, 2 = 16-bit, 3 =
+// float)
+constexpr int batch_variant(bool temporal, bool ph, int outt) { return (temporal ? 1 : 0) | (ph ? 2 : 0) | (outt << 2); }
+
 // True when the float selector keeps compressed per-active arrays: a sparse
 // workset that is not in dense fallback (plugin.cpp:455-456).
 __device__ __forceinline__ bool sel_compressed(const uint32_t *row_off, const uint32_t *indices, int ow, int oh) {
@@ -2153,22 +2164,15 @@ __device__ __forceinline__ void plain_v_col_fin(const float2 *val, const float4 
 // (1 = 8-bit, 2 = 16-bit, 3 = float samples).
 // PH (with PLAIN, finite samples only): the guided pass takes the chroma
 // hull of each pixel from the plain base's vertical pass.
-template <bool TEMPORAL, int SEL, bool PLAIN, int OUTT = 0, bool PH = false>
-__global__ __launch_bounds__(kBlock, TEMPORAL ? 4 : (PLAIN ? 5 : 6)) void k_guided_tiles(GuidedArgs A, int tiles_x,
-                                                                                       int tiles) {
+// One wave computes tile t (wave-uniform) of the frame described by A, using
+// its own LDS window wb and plain-base rows hv/hh.
+template <bool TEMPORAL, int SEL, bool PLAIN, int OUTT, bool PH>
+__device__ __attribute__((always_inline)) void guided_tile(const GuidedArgs &A, int t, int tiles_x, float4 *wb,
+                                                           float2 (*hv)[kTile], float4 (*hh)[kTile], int lane) {
     static_assert(!PH || PLAIN, "the plain hull comes from the plain base in the tile");
     // fused output needs every inside pixel's result in the tile: the plain
     // base in the tiles (Recon), or a dense temporal pass (TRecon, no mask)
     static_assert(OUTT == 0 || (SEL == SEL_NONE && PLAIN != TEMPORAL), "fused output needs every pixel's result");
-    __shared__ float4 wins[kBlock / 64][kTWin * kTStride];
-    __shared__ float2 hval[PLAIN ? kBlock / 64 : 1][kTWin][kTile];
-    __shared__ float4 hhull[PLAIN ? kBlock / 64 : 1][kTWin][kTile];
-    const int lane = threadIdx.x & 63;
-    // wave-uniform tile index, held in scalar registers
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int t = blockIdx.x * (kBlock / 64) + wave;
-    if (t >= tiles)
-        return;  // whole wave
     const int ow = A.ax.n, oh = A.ay.n, cw = A.cw, ch = A.ch;
     const int tx = t % tiles_x, ty = t / tiles_x;
     const int ox = tx * kTile + (lane & (kTile - 1)), oy = ty * kTile + (lane / kTile);
@@ -2194,7 +2198,6 @@ __global__ __launch_bounds__(kBlock, TEMPORAL ? 4 : (PLAIN ? 5 : 6)) void k_guid
     }
     // the whole kTWin x kTWin window from (x_lo, y_lo); the host checked that
     // every tile's taps lie inside it (guided_tiles_fit)
-    float4 *wb = wins[wave];
     _Pragma("unroll") for (int e = lane; e < kTWin * kTWin; e += 64) {
         const int r = e / kTWin, c = e % kTWin;
         wb[r * kTStride + c] = A.luv[(uint32_t)(clampi(y_lo + r, 0, ch - 1) * cw + clampi(x_lo + c, 0, cw - 1))];
@@ -2214,15 +2217,15 @@ __global__ __launch_bounds__(kBlock, TEMPORAL ? 4 : (PLAIN ? 5 : 6)) void k_guid
             tap_run(pc.bits, hx, 1);
             _Pragma("unroll") for (int k = 0; k < 2; ++k)
                 plain_h_row_fin(wb + ((lane >> 3) + 8 * k) * kTStride + col0, hx, pc,
-                                hval[wave][(lane >> 3) + 8 * k][c8], hhull[wave][(lane >> 3) + 8 * k][c8]);
+                                hv[(lane >> 3) + 8 * k][c8], hh[(lane >> 3) + 8 * k][c8]);
         } else if (__all(pc.bits == 0x3fu)) {
             _Pragma("unroll") for (int k = 0; k < 2; ++k)
-                plain_h_row<true>(wb + ((lane >> 3) + 8 * k) * kTStride + col0, pc, hval[wave][(lane >> 3) + 8 * k][c8],
-                                  hhull[wave][(lane >> 3) + 8 * k][c8]);
+                plain_h_row<true>(wb + ((lane >> 3) + 8 * k) * kTStride + col0, pc, hv[(lane >> 3) + 8 * k][c8],
+                                  hh[(lane >> 3) + 8 * k][c8]);
         } else {
             _Pragma("unroll") for (int k = 0; k < 2; ++k)
                 plain_h_row<false>(wb + ((lane >> 3) + 8 * k) * kTStride + col0, pc,
-                                   hval[wave][(lane >> 3) + 8 * k][c8], hhull[wave][(lane >> 3) + 8 * k][c8]);
+                                   hv[(lane >> 3) + 8 * k][c8], hh[(lane >> 3) + 8 * k][c8]);
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         __builtin_amdgcn_wave_barrier();
@@ -2237,11 +2240,11 @@ __global__ __launch_bounds__(kBlock, TEMPORAL ? 4 : (PLAIN ? 5 : 6)) void k_guid
             if constexpr (PH) {
                 int hy[6];
                 tap_run(pr.bits, hy, kTile);
-                plain_v_col_fin(&hval[wave][t0][c8], &hhull[wave][t0][c8], hy, pr, vu, vv, mnu, mxu, mnv, mxv);
+                plain_v_col_fin(&hv[t0][c8], &hh[t0][c8], hy, pr, vu, vv, mnu, mxu, mnv, mxv);
             } else if (vall) {
-                plain_v_col<true>(&hval[wave][t0][c8], &hhull[wave][t0][c8], pr, vu, vv, mnu, mxu, mnv, mxv);
+                plain_v_col<true>(&hv[t0][c8], &hh[t0][c8], pr, vu, vv, mnu, mxu, mnv, mxv);
             } else {
-                plain_v_col<false>(&hval[wave][t0][c8], &hhull[wave][t0][c8], pr, vu, vv, mnu, mxu, mnv, mxv);
+                plain_v_col<false>(&hv[t0][c8], &hh[t0][c8], pr, vu, vv, mnu, mxu, mnv, mxv);
             }
             phull = make_float4(mnu, mxu, mnv, mxv);
             const double wsum = (double)pc.wsum * pr.wsum;
@@ -2303,6 +2306,73 @@ __global__ __launch_bounds__(kBlock, TEMPORAL ? 4 : (PLAIN ? 5 : 6)) void k_guid
                 reinterpret_cast<float *>(bv + sv * oy)[ox] = fv;
             }
         }
+    }
+}
+
+template <bool TEMPORAL, int SEL, bool PLAIN, int OUTT = 0, bool PH = false>
+__global__ __launch_bounds__(kBlock, TEMPORAL ? 4 : (PLAIN ? 5 : 6)) void k_guided_tiles(GuidedArgs A, int tiles_x,
+                                                                                       int tiles) {
+    __shared__ float4 wins[kBlock / 64][kTWin * kTStride];
+    __shared__ float2 hval[PLAIN ? kBlock / 64 : 1][kTWin][kTile];
+    __shared__ float4 hhull[PLAIN ? kBlock / 64 : 1][kTWin][kTile];
+    const int lane = threadIdx.x & 63;
+    // wave-uniform tile index, held in scalar registers
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int t = blockIdx.x * (kBlock / 64) + wave;
+    if (t >= tiles)
+        return;  // whole wave
+    const int hw = PLAIN ? wave : 0;
+    guided_tile<TEMPORAL, SEL, PLAIN, OUTT, PH>(A, t, tiles_x, wins[wave], hval[hw], hhull[hw], lane);
+}
+
+// The fused tile pass over the tiles of many frames in one launch.
+// stride == 0: one wave per tile, frame blockIdx.y, its tiles over
+// blockIdx.x, the hardware handing out the workgroups as CUs free up.
+// Persistent form (stride > 0): frame f's tiles are positions first[f] ..
+// first[f + 1] - 1 (first[] ascending, first[0] = 0, at most 64 frames), the
+// grid is what the device holds at once and wave g computes positions g,
+// g + stride, g + 2 stride, ... (stride = the grid's wave count), so
+// neighbouring positions run on the waves of one workgroup and the frames'
+// tiles spread evenly over the CUs and XCDs; every wave leaves after its
+// last position.
+template <bool TEMPORAL, int SEL, bool PLAIN, int OUTT = 0, bool PH = false>
+__global__ __launch_bounds__(kBlock, TEMPORAL ? 4 : (PLAIN ? 5 : 6)) void k_guided_tiles_batch(
+    const BatchFrame *__restrict__ frames_in, const uint32_t *__restrict__ first, int nframes, uint32_t total,
+    uint32_t stride) {
+    // the frame table is read-only for the whole launch: reading it through
+    // the constant address space keeps its loads scalar and the plane
+    // pointers in it known to be global
+    const __attribute__((address_space(4))) BatchFrame *frames =
+        (const __attribute__((address_space(4))) BatchFrame *)frames_in;
+    __shared__ float4 wins[kBlock / 64][kTWin * kTStride];
+    __shared__ float2 hval[PLAIN ? kBlock / 64 : 1][kTWin][kTile];
+    __shared__ float4 hhull[PLAIN ? kBlock / 64 : 1][kTWin][kTile];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int hw = PLAIN ? wave : 0;
+    // lane f holds frame f's first position (lanes past the batch: never
+    // below any position)
+    if (stride == 0) {
+        // one wave per tile: frame blockIdx.y, its tiles over blockIdx.x
+        const int f = blockIdx.y;
+        const int t = blockIdx.x * (kBlock / 64) + wave;
+        if (t >= frames[f].tiles)
+            return;
+        const GuidedArgs &A = *(const GuidedArgs *)&frames[f].A;
+        guided_tile<TEMPORAL, SEL, PLAIN, OUTT, PH>(A, t, frames[f].tiles_x, wins[wave], hval[hw], hhull[hw], lane);
+        return;
+    }
+    const uint32_t lfirst = lane < nframes ? first[lane] : 0xffffffffu;
+    for (uint32_t t = blockIdx.x * (kBlock / 64) + wave; t < total; t += stride) {
+        const int f = __builtin_amdgcn_readfirstlane((int)__popcll(__ballot(lfirst <= t)) - 1);
+        const uint32_t f0 = __builtin_amdgcn_readfirstlane(__shfl(lfirst, f));
+        // the previous tile's LDS reads have completed before its window is
+        // overwritten
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        const GuidedArgs &A = *(const GuidedArgs *)&frames[f].A;
+        guided_tile<TEMPORAL, SEL, PLAIN, OUTT, PH>(A, (int)(t - f0), frames[f].tiles_x, wins[wave], hval[hw],
+                                                    hhull[hw], lane);
     }
 }
 
@@ -4229,7 +4299,7 @@ hipError_t launch_guided(const float *u, const float *v, int cw, int ch, const f
                          const uint32_t *row_offsets, const uint32_t *indices, float *out_u, float *out_v,
                          const SelectorDev &sel, const TemporalDev &tmp, hipStream_t s, const uint64_t *mask_bits,
                          int mask_w, int mask_h, int tiles_fit, int plain_in_tiles, const RawOut *fused_out,
-                         int out_type, float bp, int bp_on) {
+                         int out_type, float bp, int bp_on, GuidedDeferred *defer) {
     GuidedArgs A{u, v, cw, ch, y, sw, sh, yt, luv, lc, ms, ax, ay, rd, gp, row_offsets, indices, out_u, out_v,
                  sel, tmp, indices ? mask_bits : nullptr, mask_w, mask_h, (mask_w + 63) / 64};
     if (guided_tiled(tiles_fit, gp.radial, gp.cedge, ax, ay, sel.mode)) {
@@ -4254,6 +4324,14 @@ hipError_t launch_guided(const float *u, const float *v, int cw, int ch, const f
             A.po_v = fused_out[1];
             A.bp = bp;
             A.bp_on = bp_on;
+            if (defer) {
+                const bool ph = recon && gp.finite;
+                BatchFrame bf{A, tiles_x, tiles};
+                memcpy(defer->args, &bf, sizeof bf);
+                defer->variant = batch_variant(!recon, ph, out_type == RAW_U8 ? 1 : out_type == RAW_U16 ? 2 : 3);
+                defer->tiles = tiles;
+                return hipSuccess;
+            }
 #define LGCR_FUSED(T, P, H)                                                                                       \
     if (out_type == RAW_U8)                                                                                     \
         hipLaunchKernelGGL((k_guided_tiles<T, SEL_NONE, P, 1, H>), g, dim3(kBlock), 0, s, A, tiles_x, tiles);   \
@@ -4317,6 +4395,85 @@ hipError_t launch_guided(const float *u, const float *v, int cw, int ch, const f
 #undef LGCR_GUIDED_SEL
 #undef LGCR_GUIDED
     return hipErrorInvalidValue;
+}
+
+size_t guided_batch_table_bytes() {
+    return sizeof(BatchFrame) * kBatchMaxFrames + sizeof(uint32_t) * kBatchMaxFrames;
+}
+
+template <bool T, bool P, int O, bool H>
+static hipError_t batch_launch_one(const BatchFrame *frames, const uint32_t *first, int n, uint32_t total,
+                                   int max_tiles, bool persistent, hipStream_t s) {
+    auto kern = k_guided_tiles_batch<T, SEL_NONE, P, O, H>;
+    // resident workgroups of the device, once per variant (one device type)
+    struct Fill {
+        hipError_t err;
+        int blocks;
+    };
+    static const Fill fill = [&]() {
+        int dev = 0, b = 0, c = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess)
+            e = hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e == hipSuccess)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kern, kBlock, 0);
+        return Fill{e, std::max(1, b) * std::max(1, c)};
+    }();
+    if (fill.err != hipSuccess)
+        return fill.err;
+    if (!persistent) {
+        hipLaunchKernelGGL(kern, dim3((unsigned)((max_tiles + kBlock / 64 - 1) / (kBlock / 64)), (unsigned)n),
+                           dim3(kBlock), 0, s, frames, first, n, total, 0u);
+        return hipGetLastError();
+    }
+    const uint32_t need = (total + kBlock / 64 - 1) / (kBlock / 64);
+    const uint32_t blocks = std::min<uint32_t>(need, (uint32_t)fill.blocks);
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), 0, s, frames, first, n, total, blocks * (kBlock / 64));
+    return hipGetLastError();
+}
+
+hipError_t launch_guided_batch(const GuidedDeferred *items, int n, void *dev, void *host, bool persistent,
+                               hipStream_t s) {
+    static_assert(sizeof(BatchFrame) <= kGuidedDeferBytes, "deferred frame storage too small");
+    if (n <= 0)
+        return hipSuccess;
+    if (n > kBatchMaxFrames)
+        return hipErrorInvalidValue;
+    const int variant = items[0].variant;
+    BatchFrame *hf = static_cast<BatchFrame *>(host);
+    uint32_t *hfirst = reinterpret_cast<uint32_t *>(hf + kBatchMaxFrames);
+    uint32_t total = 0;
+    int max_tiles = 0;
+    for (int k = 0; k < n; ++k) {
+        if (items[k].variant != variant || items[k].tiles <= 0)
+            return hipErrorInvalidValue;
+        max_tiles = std::max(max_tiles, items[k].tiles);
+        memcpy(&hf[k], items[k].args, sizeof(BatchFrame));
+        hfirst[k] = total;
+        total += (uint32_t)items[k].tiles;
+    }
+    BatchFrame *df = static_cast<BatchFrame *>(dev);
+    uint32_t *dfirst = reinterpret_cast<uint32_t *>(df + kBatchMaxFrames);
+    hipError_t e = hipMemcpyAsync(df, hf, sizeof(BatchFrame) * n, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dfirst, hfirst, sizeof(uint32_t) * n, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess)
+        return e;
+#define LGCR_BATCH(T, P, O, H) \
+    case batch_variant(T, H, O): return batch_launch_one<T, P, O, H>(df, dfirst, n, total, max_tiles, persistent, s);
+    switch (variant) {
+        LGCR_BATCH(false, true, 1, true)
+        LGCR_BATCH(false, true, 2, true)
+        LGCR_BATCH(false, true, 3, true)
+        LGCR_BATCH(false, true, 1, false)
+        LGCR_BATCH(false, true, 2, false)
+        LGCR_BATCH(false, true, 3, false)
+        LGCR_BATCH(true, false, 1, false)
+        LGCR_BATCH(true, false, 2, false)
+        LGCR_BATCH(true, false, 3, false)
+    default: return hipErrorInvalidValue;
+    }
+#undef LGCR_BATCH
 }
 
 hipError_t launch_plain_radial(const float *u, const float *v, int cw, int ch, const AxisDev &ax,

@@ -871,6 +871,25 @@ struct lgcr_clip {
     // a thread that holds a slot reads spaces[slot] without the lock.
     std::vector<std::unique_ptr<Workspace>> spaces;
     std::vector<int> idle;
+
+    // multi-frame fused tile launches of lgcr_recon_batch
+    // (lgcr_clip_set_batching): frames per launch (0 = off), the lanes'
+    // streams and two table slots per lane, used in turn; batch_mutex keeps
+    // one batch call at a time on the tables
+    std::atomic<int> batch_frames{0};
+    std::mutex batch_mutex;
+    hipStream_t lane[kBatchLanes] = {};
+    struct BatchSlot {
+        void *dev = nullptr;    // frame table + first positions + queue word
+        void *host = nullptr;   // pinned staging of the tables
+        hipEvent_t done = nullptr;
+        bool used = false;
+    } bslot[2 * kBatchLanes];
+    size_t batch_bytes = 0;     // device bytes of the slots (reserved)
+    // one wave per tile over a grid the hardware hands out (measured faster
+    // than the persistent grid on the 1080p and 640x360 benchmark frames);
+    // LGCR_BATCH_PERSISTENT=1 at clip creation selects the persistent grid
+    bool batch_persistent = false;
 };
 
 namespace {
@@ -1461,10 +1480,11 @@ hipError_t trust_workset(lgcr_clip *c, Workspace *w, const Geometry &g, bool til
 
 lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const void *const in[3],
                           const ptrdiff_t in_stride[3], void *const out[3], const ptrdiff_t out_stride[3],
-                          bool timed, IoTable *io = nullptr) {
+                          bool timed, IoTable *io = nullptr, GuidedDeferred *defer = nullptr,
+                          hipStream_t stream = nullptr) {
     INJECT_LAUNCH_FAULT();
     const lgcr_params &p = c->p;
-    hipStream_t s = w->stream;
+    hipStream_t s = stream ? stream : w->stream;
     auto mark = [&](int stage) -> hipError_t {
         if (!timed)
             return hipSuccess;
@@ -1631,7 +1651,8 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
         HIP_TRY(launch_guided(w->u, w->v, c->cw, c->ch, w->y, c->sw, c->sh, w->yt, w->luv, w->lc,
                               gate, g.ax, g.ay, g.rd, gp, w->row_off, idx, w->out_u, w->out_v, sel,
                               TemporalDev{}, s, w->mask, c->sw, c->sh, g.tiles_fit, plain_in_tiles,
-                              fused_out ? fo : nullptr, c->raw_type, float(p.bp), bp_same ? 1 : 0));
+                              fused_out ? fo : nullptr, c->raw_type, float(p.bp), bp_same ? 1 : 0,
+                              fused_out && !timed && !io ? defer : nullptr));
         HIP_TRY(mark(LGCR_STAGE_APPLY_GUIDED_CORRECTION));
         if (algo4 && is_float) {
             HIP_TRY(launch_roi_mark(w->row_off, idx, c->ow, c->oh, sel, g.ax, g.ay, w->roi, c->cw, c->ch,
@@ -2090,6 +2111,8 @@ static lgcr_status create_clip(lgcr_device *dev, const lgcr_params *params, cons
         return fail(LGCR_UNSUPPORTED, "LGCR HIP backend: planes of 2^31 or more samples are not supported");
     if (const char *cap = std::getenv("LGCR_CAPTURE"))
         c->capture = cap[0] && cap[0] != '0';
+    if (const char *bp = std::getenv("LGCR_BATCH_PERSISTENT"))
+        c->batch_persistent = bp[0] && bp[0] != '0';
     tuned_lookup(c.get());
     HIP_TRY(hipSetDevice(dev->ordinal));
     if (!c->same && !p.radial) {
@@ -2319,6 +2342,21 @@ void lgcr_clip_destroy(lgcr_clip *c) {
     hipSetDevice(c->dev->ordinal);
     for (auto &w : c->spaces)
         destroy_workspace(c, w.get());
+    for (hipStream_t &l : c->lane)
+        if (l) {
+            hipStreamSynchronize(l);
+            hipStreamDestroy(l);
+        }
+    for (auto &b : c->bslot) {
+        if (b.dev)
+            hipFree(b.dev);
+        if (b.host)
+            hipHostFree(b.host);
+        if (b.done)
+            hipEventDestroy(b.done);
+    }
+    if (c->batch_bytes)
+        c->dev->unreserve(c->batch_bytes);
     for (auto &kv : c->geos)
         hipFree(kv.second->dev);
     if (c->lut_dev)
@@ -2503,6 +2541,98 @@ lgcr_status lgcr_recon_frame(lgcr_clip *c, const lgcr_frame *in, double shift_x,
     return LGCR_OK;
 }
 
+namespace {
+
+// lgcr_recon_batch with multi-frame tile launches.  The frames go in groups
+// of up to batch_frames, and the groups go round-robin to up to kBatchLanes
+// lanes, each a stream with its own workspaces and table slots: a group's
+// frames run their stages up to the mutual gate one after another on the
+// lane's stream, then ONE launch computes the fused tile pass (plain base,
+// guided correction, back-projection and packing) of the whole group.  A
+// lane's work is ordered on its own stream (no waits between streams, which
+// would hold up the hardware queues the streams share), and the lanes run
+// concurrently, so one lane's early stages overlap another lane's tile
+// launch.  Frames whose pass is not the fused tile pass (algo 4, bm, algo 6,
+// tiles that do not fit) complete on the lane's stream.
+lgcr_status recon_batch_fused(lgcr_clip *c, const Geometry &g, const std::vector<int> &slots, int n,
+                              const lgcr_frame *in, lgcr_frame *out) {
+    std::lock_guard<std::mutex> lock(c->batch_mutex);
+    const int ns = (int)slots.size();
+    const int G = std::max(1, std::min({c->batch_frames.load(), ns, kBatchMaxFrames}));
+    const int lanes = std::max(1, std::min(kBatchLanes, ns / G));
+    std::vector<GuidedDeferred> items(G);
+    lgcr_status result = LGCR_OK;
+    for (int j = 0, k0 = 0; k0 < n && result == LGCR_OK; ++j, k0 += G) {
+        const int cnt = std::min(G, n - k0), lane = j % lanes;
+        hipStream_t s = c->lane[lane];
+        lgcr_clip::BatchSlot &slot = c->bslot[lane * 2 + (j / lanes) % 2];
+        if (slot.used)
+            HIP_TRY(hipEventSynchronize(slot.done));  // its staging is rewritten below
+        int nd = 0;
+        for (int i = 0; i < cnt && result == LGCR_OK; ++i) {
+            const int k = k0 + i;
+            Workspace *w = c->spaces[slots[lane * G + i]].get();
+            items[nd].tiles = 0;
+            void *dout[3] = {out[k].data[0], out[k].data[1], out[k].data[2]};
+            result = enqueue_frame(c, w, g, in[k].data, in[k].stride, dout, out[k].stride, false, nullptr,
+                                   &items[nd], s);
+            if (result == LGCR_OK && c->same && dout[0] && dout[0] != in[k].data[0])
+                HIP_TRY(hipMemcpy2DAsync(dout[0], out[k].stride[0], in[k].data[0], in[k].stride[0],
+                                         size_t(c->sw) * c->bps, c->sh, hipMemcpyDeviceToDevice, s));
+            if (items[nd].tiles > 0)
+                ++nd;
+        }
+        if (nd == 0)
+            continue;
+        HIP_TRY(launch_guided_batch(items.data(), nd, slot.dev, slot.host, c->batch_persistent, s));
+        HIP_TRY(hipEventRecord(slot.done, s));
+        slot.used = true;
+    }
+    for (int l = 0; l < lanes; ++l) {
+        const hipError_t e = hipStreamSynchronize(c->lane[l]);
+        if (e != hipSuccess && result == LGCR_OK)
+            result = fail(LGCR_BACKEND_FAILURE, "hipStreamSynchronize failed: %s", hipGetErrorString(e));
+    }
+    return result;
+}
+
+}  // namespace
+
+lgcr_status lgcr_clip_set_batching(lgcr_clip *c, int frames_per_launch) {
+    if (!c || frames_per_launch < 0 || frames_per_launch > kBatchMaxFrames)
+        return fail(LGCR_INVALID_ARGUMENT, "lgcr_clip_set_batching: frames per launch must be 0..%d", kBatchMaxFrames);
+    if (c->temporal || c->sharpen || c->downsample)
+        return fail(LGCR_INVALID_CONTEXT, "lgcr_clip_set_batching: not a Recon clip");
+    std::lock_guard<std::mutex> lock(c->batch_mutex);
+    if (frames_per_launch > 0 && !c->lane[kBatchLanes - 1]) {
+        // a failure part-way leaves what was made for the next call (freed
+        // by lgcr_clip_destroy)
+        const size_t table = guided_batch_table_bytes(), dev_bytes = table;
+        if (c->batch_bytes == 0) {
+            if (!c->dev->reserve(2 * kBatchLanes * dev_bytes))
+                return fail(LGCR_REJECTED, "batch tables of %zu bytes exceed the device HBM budget",
+                            2 * kBatchLanes * dev_bytes);
+            c->batch_bytes = 2 * kBatchLanes * dev_bytes;
+        }
+        HIP_TRY(hipSetDevice(c->dev->ordinal));
+        for (auto &b : c->bslot) {
+            if (!b.dev)
+                HIP_TRY(hipMalloc(&b.dev, dev_bytes));
+            if (!b.host)
+                HIP_TRY(hipHostMalloc(&b.host, table, hipHostMallocDefault));
+            if (!b.done)
+                HIP_TRY(hipEventCreateWithFlags(&b.done, hipEventDisableTiming));
+        }
+        for (hipStream_t &l : c->lane)
+            if (!l)
+                HIP_TRY(hipStreamCreateWithFlags(&l, hipStreamNonBlocking));
+    }
+    c->batch_frames = frames_per_launch;
+    return LGCR_OK;
+}
+
+int lgcr_clip_get_batching(const lgcr_clip *c) { return c ? c->batch_frames.load() : 0; }
+
 lgcr_status lgcr_recon_batch(lgcr_clip *c, int n, const lgcr_frame *in, lgcr_frame *out) {
     if (!c || n < 0 || (n && (!in || !out)))
         return fail(LGCR_INVALID_ARGUMENT, "lgcr_recon_batch: bad arguments");
@@ -2556,7 +2686,10 @@ lgcr_status lgcr_recon_batch(lgcr_clip *c, int n, const lgcr_frame *in, lgcr_fra
         if (!idx.empty())
             result = enqueue_frames_graph(c, c->spaces[slots[j]].get(), *g, in, out, idx.data(), (int)idx.size());
     }
-    for (int k = 0; !graphs && k < n && result == LGCR_OK; ++k) {
+    const bool fused = !graphs && !timed && !c->capture && c->batch_frames.load() > 0;
+    if (fused)
+        result = recon_batch_fused(c, *g, slots, n, in, out);
+    for (int k = 0; !graphs && !fused && k < n && result == LGCR_OK; ++k) {
         Workspace *w = c->spaces[slots[k % slots.size()]].get();
         void *dout[3] = {out[k].data[0], out[k].data[1], out[k].data[2]};
         result = enqueue_frame(c, w, *g, in[k].data, in[k].stride, dout, out[k].stride, timed);

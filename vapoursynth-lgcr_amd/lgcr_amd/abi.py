@@ -34,7 +34,7 @@ SYMBOLS = [
     "lgcr_clip_stage_times", "lgcr_clip_fetch", "lgcr_params_default_trecon", "lgcr_trecon_create",
     "lgcr_trecon_frame", "lgcr_sharpen_params_default", "lgcr_sharpen_create", "lgcr_sharpen_frame",
     "lgcr_recon_submit", "lgcr_trecon_submit", "lgcr_ticket_query", "lgcr_ticket_wait",
-    "lgcr_clip_set_graphs", "lgcr_clip_get_tuning", "lgcr_clip_set_tuning", "lgcr_clip_autotune",
+    "lgcr_clip_set_graphs", "lgcr_clip_set_batching", "lgcr_clip_get_batching", "lgcr_clip_get_tuning", "lgcr_clip_set_tuning", "lgcr_clip_autotune",
     "lgcr_downsample_params_default", "lgcr_downsample_create", "lgcr_downsample_frame",
     "lgcr_ticket_wait_any", "lgcr_downsample_frame_noalloc", "lgcr_check_rcp",
 ]
@@ -143,6 +143,9 @@ def lib():
                                         C.POINTER(Frame), C.POINTER(Stats)]
         L.lgcr_clip_set_timing.argtypes = [vp, C.c_int]
         L.lgcr_clip_set_graphs.argtypes = [vp, C.c_int]
+        L.lgcr_clip_set_batching.argtypes = [vp, C.c_int]
+        L.lgcr_clip_get_batching.argtypes = [vp]
+        L.lgcr_clip_get_batching.restype = C.c_int
         L.lgcr_clip_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
         L.lgcr_clip_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
         L.lgcr_clip_autotune.argtypes = [vp, C.POINTER(Frame)]
@@ -354,6 +357,13 @@ class Clip:
 
     def set_graphs(self, enable=True):
         check(lib().lgcr_clip_set_graphs(self.handle, int(enable)))
+
+    def set_batching(self, frames_per_launch):
+        """recon_batch: fused tile pass of up to this many frames per launch (0 = off)."""
+        check(lib().lgcr_clip_set_batching(self.handle, int(frames_per_launch)))
+
+    def batching(self):
+        return lib().lgcr_clip_get_batching(self.handle)
 
     def tuning(self):
         t = Tuning()
