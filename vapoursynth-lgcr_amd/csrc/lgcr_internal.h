@@ -40,17 +40,26 @@ struct WeightsDev {
     const float *w = nullptr;
 };
 
+// Output columns per block of the class-table radial kernels.
+constexpr int kSegCols = 256;
+
 // Radial (jinc) base-weight phase table (RadialWeightTable,
 // src/lgcr.h:793-811).  valid == 0 means the kernels evaluate the LUT per tap.
 struct RadialDev {
     int valid = 0, sup_x = 0, sup_y = 0, nx = 0, ny = 0;
     const uint16_t *xphase = nullptr, *yphase = nullptr;
     const float *w = nullptr;
+    const double *wd = nullptr;  // w as doubles
     // output columns of each x phase (xoff[p] .. xoff[p + 1]) and, per phase
     // pair, the sum and absolute sum of the active weights in (j, i) order
     const int *xs = nullptr, *xoff = nullptr;
     const double *wsa = nullptr;
     int max_count = 0;
+    // waves per segment of kSegCols output columns: segw[segoff[s] ..
+    // segoff[s + 1]) = {phase, first xs entry, end xs entry, 0}
+    const int *segoff = nullptr;
+    const int4 *segw = nullptr;
+    int nseg = 0;
     const float *lut = nullptr;
     int lut_n = 0;
     float lut_scale = 1024.0f;
@@ -68,6 +77,10 @@ struct RadialResampleDev {
     const int *x0 = nullptr, *y0 = nullptr, *xs = nullptr, *xoff = nullptr;
     const uint8_t *ycls = nullptr;
     const double *w = nullptr, *wsum = nullptr;
+    // waves per segment of kSegCols output columns (as in RadialDev)
+    const int *segoff = nullptr;
+    const int4 *segw = nullptr;
+    int nseg = 0;
 };
 
 // Scalar parameters the kernels read, already narrowed exactly as the

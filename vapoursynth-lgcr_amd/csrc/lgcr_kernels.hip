@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <climits>
 #include <cstring>
+#include <type_traits>
 
 namespace lgcr_hip {
 namespace {
@@ -1408,49 +1409,20 @@ __global__ __launch_bounds__(kBlock) void k_plain_radial_tab(const float *__rest
     const int tx0 = ax.start[ox], ty0 = ay.start[oy];
     double au = 0.0, av = 0.0;
     float mnu = 1e30f, mxu = -1e30f, mnv = 1e30f, mxv = -1e30f;
-    if constexpr (SUP == 6) {
-        // the pair's weights and the tap activity (wave-uniform) read up
-        // front, so no tap waits on a scalar load
-        float wr[36];
-        _Pragma("unroll") for (int t = 0; t < 36; ++t)
-            wr[t] = base[t];
-        uint32_t mx = 0, my = 0;
-        _Pragma("unroll") for (int t = 0; t < 6; ++t) {
-            mx |= (amx[t] != 0.0f ? 1u : 0u) << t;
-            my |= (amy[t] != 0.0f ? 1u : 0u) << t;
-        }
-        _Pragma("unroll") for (int j = 0; j < 6; ++j) {
-            if (!((my >> j) & 1u))
+    for (int j = 0; j < SY; ++j) {
+        if (amy[j] == 0.0f)
+            continue;
+        const size_t row = (size_t)clampi(ty0 + j, 0, ch - 1) * cw;
+        _Pragma("unroll") for (int i = 0; i < SX; ++i) {
+            if (amx[i] == 0.0f)
                 continue;
-            const size_t row = (size_t)clampi(ty0 + j, 0, ch - 1) * cw;
-            _Pragma("unroll") for (int i = 0; i < 6; ++i) {
-                if (!((mx >> i) & 1u))
-                    continue;
-                const float w = wr[j * 6 + i];
-                const size_t q = row + clampi(tx0 + i, 0, cw - 1);
-                const float a = u[q], b = v[q];
-                au = dacc(au, w, a);
-                av = dacc(av, w, b);
-                mnu = fmn(mnu, a); mxu = fmx(mxu, a);
-                mnv = fmn(mnv, b); mxv = fmx(mxv, b);
-            }
-        }
-    } else {
-        for (int j = 0; j < SY; ++j) {
-            if (amy[j] == 0.0f)
-                continue;
-            const size_t row = (size_t)clampi(ty0 + j, 0, ch - 1) * cw;
-            _Pragma("unroll") for (int i = 0; i < SX; ++i) {
-                if (amx[i] == 0.0f)
-                    continue;
-                const float w = base[j * SX + i];
-                const size_t q = row + clampi(tx0 + i, 0, cw - 1);
-                const float a = u[q], b = v[q];
-                au = dacc(au, w, a);
-                av = dacc(av, w, b);
-                mnu = fmn(mnu, a); mxu = fmx(mxu, a);
-                mnv = fmn(mnv, b); mxv = fmx(mxv, b);
-            }
+            const float w = base[j * SX + i];
+            const size_t q = row + clampi(tx0 + i, 0, cw - 1);
+            const float a = u[q], b = v[q];
+            au = dacc(au, w, a);
+            av = dacc(av, w, b);
+            mnu = fmn(mnu, a); mxu = fmx(mxu, a);
+            mnv = fmn(mnv, b); mxv = fmx(mxv, b);
         }
     }
     const double ws = rd.wsa[2 * pair], sa = rd.wsa[2 * pair + 1];
@@ -1470,6 +1442,162 @@ __global__ __launch_bounds__(kBlock) void k_plain_radial_tab(const float *__rest
     const size_t o = (size_t)oy * ax.n + ox;
     out_u[o] = (float)vu;
     out_v[o] = (float)vv;
+}
+
+// The radial plain base with 6x6 taps.  A block computes R consecutive
+// output rows of one segment of kSegCols output columns: each of its waves
+// takes the columns of one x phase inside the segment (rd.segw), so the
+// phase pair's weights and tap activity are wave-uniform, and each thread
+// computes its column in the R rows by walking the union of their source
+// rows once.  A source row's six samples are loaded once and feed every
+// output row whose taps include it (at 2x or more upscaling R rows share
+// most of their source rows), and the row's hull over the active columns is
+// formed once for all of them.  The results go through LDS, so the block
+// writes whole rows of the segment (a phase's columns alone are strided,
+// and partial-line writes cost a read of the rest of the line).  Each output
+// accumulates its active taps in the reference's (j, i) order; its hull is
+// the running minimum/maximum over the rows' minima/maxima in j order, which
+// picks the same sample as the running hull over the taps in (j, i) order
+// (ties keep the earlier sample, NaN never replaces a bound), so the results
+// are those of the per-tap order.
+constexpr int kRadialRows = 4;
+template <int R>
+__global__ __launch_bounds__(kBlock) void k_plain_radial_rows(const float *__restrict__ u,
+                                                              const float *__restrict__ v, int cw, int ch,
+                                                              AxisDev ax, AxisDev ay, RadialDev rd, float *out_u,
+                                                              float *out_v, int clamp_hull, float margin) {
+    __shared__ float2 tile[R][kSegCols];
+    const int sg = blockIdx.x, oy0 = blockIdx.y * R;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int nrow = min(R, ay.n - oy0);
+    int ty[R], ypair[R];
+    uint32_t my[R];
+    int rlo = INT_MAX, rhi = INT_MIN;
+    _Pragma("unroll") for (int s = 0; s < R; ++s) {
+        const int oy = min(oy0 + s, ay.n - 1);
+        ty[s] = ay.start[oy];
+        ypair[s] = (int)rd.yphase[oy] * rd.nx;
+        const float *__restrict__ amy = ay.am + (size_t)oy * 6;
+        my[s] = 0;
+        _Pragma("unroll") for (int t = 0; t < 6; ++t)
+            my[s] |= (amy[t] != 0.0f ? 1u : 0u) << t;
+        if (s < nrow) {
+            rlo = min(rlo, ty[s]);
+            rhi = max(rhi, ty[s] + 5);
+        }
+    }
+    for (int e = rd.segoff[sg] + wave; e < rd.segoff[sg + 1]; e += kBlock / 64) {
+        const int4 en = rd.segw[e];
+        const int cls = en.x, k = en.y + lane;
+        if (k >= en.z)
+            continue;
+        const int ox = rd.xs[k];
+        // tap activity is part of the phase: the class's first column has it
+        const float *__restrict__ amx = ax.am + (size_t)rd.xs[rd.xoff[cls]] * 6;
+        uint32_t mx = 0;
+        _Pragma("unroll") for (int t = 0; t < 6; ++t)
+            mx |= (amx[t] != 0.0f ? 1u : 0u) << t;
+        const int tx0 = ax.start[ox];
+        int col[6];
+        _Pragma("unroll") for (int i = 0; i < 6; ++i)
+            col[i] = clampi(tx0 + i, 0, cw - 1);
+        double au[R], av[R];
+        float mnu[R], mxu[R], mnv[R], mxv[R];
+        _Pragma("unroll") for (int s = 0; s < R; ++s) {
+            au[s] = av[s] = 0.0;
+            mnu[s] = mnv[s] = 1e30f;
+            mxu[s] = mxv[s] = -1e30f;
+        }
+        // source rows are loaded one ahead of their use; the row loop is
+        // compiled once with every column tap active (the interior phases:
+        // straight-line code) and once with the activity mask
+        auto rows = [&](auto all6_tag) {
+            constexpr bool ALL6 = decltype(all6_tag)::value;
+            float na[6], nb[6];
+            {
+                const size_t row = (size_t)clampi(rlo, 0, ch - 1) * cw;
+                _Pragma("unroll") for (int i = 0; i < 6; ++i) {
+                    na[i] = u[row + col[i]];
+                    nb[i] = v[row + col[i]];
+                }
+            }
+            for (int r = rlo; r <= rhi; ++r) {
+                float a[6], b[6];
+                _Pragma("unroll") for (int i = 0; i < 6; ++i) {
+                    a[i] = na[i];
+                    b[i] = nb[i];
+                }
+                if (r < rhi) {
+                    const size_t row = (size_t)clampi(r + 1, 0, ch - 1) * cw;
+                    _Pragma("unroll") for (int i = 0; i < 6; ++i) {
+                        na[i] = u[row + col[i]];
+                        nb[i] = v[row + col[i]];
+                    }
+                }
+                // the samples in double (exact) once for every output row
+                double da[6], db[6];
+                float rmnu = 1e30f, rmxu = -1e30f, rmnv = 1e30f, rmxv = -1e30f;
+                _Pragma("unroll") for (int i = 0; i < 6; ++i) {
+                    da[i] = (double)a[i];
+                    db[i] = (double)b[i];
+                    if (ALL6 || ((mx >> i) & 1u)) {
+                        rmnu = fmn(rmnu, a[i]); rmxu = fmx(rmxu, a[i]);
+                        rmnv = fmn(rmnv, b[i]); rmxv = fmx(rmxv, b[i]);
+                    }
+                }
+                _Pragma("unroll") for (int s = 0; s < R; ++s) {
+                    const int j = r - ty[s];
+                    if (s >= nrow || j < 0 || j > 5 || !((my[s] >> j) & 1u))
+                        continue;  // wave-uniform
+                    // the pair's weights as doubles (the float weights,
+                    // exactly), wave-uniform
+                    const double *__restrict__ wr = rd.wd + (size_t)(ypair[s] + cls) * 36 + j * 6;
+                    _Pragma("unroll") for (int i = 0; i < 6; ++i) {
+                        if (!ALL6 && !((mx >> i) & 1u))
+                            continue;
+                        au[s] = __builtin_fma(wr[i], da[i], au[s]);
+                        av[s] = __builtin_fma(wr[i], db[i], av[s]);
+                    }
+                    mnu[s] = fmn(mnu[s], rmnu); mxu[s] = fmx(mxu[s], rmxu);
+                    mnv[s] = fmn(mnv[s], rmnv); mxv[s] = fmx(mxv[s], rmxv);
+                }
+            }
+        };
+        if (mx == 0x3fu)  // wave-uniform
+            rows(std::true_type{});
+        else
+            rows(std::false_type{});
+        const float scx = ax.pos[ox];
+        _Pragma("unroll") for (int s = 0; s < R; ++s) {
+            if (s >= nrow)
+                break;
+            const int pair = ypair[s] + cls;
+            const double ws = rd.wsa[2 * pair], sa = rd.wsa[2 * pair + 1];
+            double vu, vv;
+            if (ws > 1e-6 && ws >= 0.05 * sa) {
+                vu = au[s] / ws;
+                vv = av[s] / ws;
+            } else {
+                const float scy = ay.pos[oy0 + s];
+                vu = bilinear(u, cw, ch, scx, scy);
+                vv = bilinear(v, cw, ch, scx, scy);
+            }
+            if (clamp_hull) {
+                vu = dclamp(vu, (double)(mnu[s] - margin), (double)(mxu[s] + margin));
+                vv = dclamp(vv, (double)(mnv[s] - margin), (double)(mxv[s] + margin));
+            }
+            tile[s][ox - sg * kSegCols] = make_float2((float)vu, (float)vv);
+        }
+    }
+    __syncthreads();
+    const int x = sg * kSegCols + (int)threadIdx.x;
+    if (x < ax.n)
+        for (int s = 0; s < nrow; ++s) {
+            const size_t o = (size_t)(oy0 + s) * ax.n + x;
+            const float2 t = tile[s][threadIdx.x];
+            out_u[o] = t.x;
+            out_v[o] = t.y;
+        }
 }
 
 // ---------------------------------------------------------------------------
@@ -3180,6 +3308,73 @@ __global__ __launch_bounds__(kBlock) void k_resample_radial_tab(const float *__r
     dst[(size_t)oy * dw + ox] = (float)(wsum > 1e-6 ? acc / wsum : 0.0);
 }
 
+// The class-table radial resample with 6x6 taps.  A block computes R
+// consecutive output rows of one segment of kSegCols output columns, each
+// wave the columns of one x class inside it (T.segw): the union of the R
+// rows' source rows is walked once, each source row's six samples loaded
+// once for every output row whose taps include it, and each output
+// accumulates in the reference's (j, i) order.  The results go through LDS so
+// the block writes whole rows of the segment.
+template <int R>
+__global__ __launch_bounds__(kBlock) void k_resample_radial_rows(const float *__restrict__ src, int sw, int sh,
+                                                                 float *__restrict__ dst, int dw, int dh,
+                                                                 RadialResampleDev T) {
+    __shared__ float tile[R][kSegCols];
+    const int sg = blockIdx.x, oy0 = blockIdx.y * R;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int nrow = min(R, dh - oy0);
+    int y0[R], ypair[R];
+    int rlo = INT_MAX, rhi = INT_MIN;
+    _Pragma("unroll") for (int s = 0; s < R; ++s) {
+        const int oy = min(oy0 + s, dh - 1);
+        y0[s] = T.y0[oy];
+        ypair[s] = (int)T.ycls[oy] * T.ncx;
+        if (s < nrow) {
+            rlo = min(rlo, y0[s]);
+            rhi = max(rhi, y0[s] + 5);
+        }
+    }
+    for (int e = T.segoff[sg] + wave; e < T.segoff[sg + 1]; e += kBlock / 64) {
+        const int4 en = T.segw[e];
+        const int cls = en.x, k = en.y + lane;
+        if (k >= en.z)
+            continue;
+        const int ox = T.xs[k];
+        const int x0 = T.x0[ox];
+        int col[6];
+        _Pragma("unroll") for (int i = 0; i < 6; ++i)
+            col[i] = clampi(x0 + i, 0, sw - 1);
+        double acc[R];
+        _Pragma("unroll") for (int s = 0; s < R; ++s)
+            acc[s] = 0.0;
+        for (int r = rlo; r <= rhi; ++r) {
+            const float *row = src + (size_t)clampi(r, 0, sh - 1) * sw;
+            double a[6];
+            _Pragma("unroll") for (int i = 0; i < 6; ++i)
+                a[i] = (double)row[col[i]];
+            _Pragma("unroll") for (int s = 0; s < R; ++s) {
+                const int j = r - y0[s];
+                if (s >= nrow || j < 0 || j > 5)
+                    continue;  // wave-uniform
+                const double *__restrict__ w = T.w + (size_t)(ypair[s] + cls) * 36 + j * 6;
+                _Pragma("unroll") for (int i = 0; i < 6; ++i)
+                    acc[s] = __builtin_fma(w[i], a[i], acc[s]);
+            }
+        }
+        _Pragma("unroll") for (int s = 0; s < R; ++s) {
+            if (s >= nrow)
+                break;
+            const double wsum = T.wsum[ypair[s] + cls];
+            tile[s][ox - sg * kSegCols] = (float)(wsum > 1e-6 ? acc[s] / wsum : 0.0);
+        }
+    }
+    __syncthreads();
+    const int x = sg * kSegCols + (int)threadIdx.x;
+    if (x < dw)
+        for (int s = 0; s < nrow; ++s)
+            dst[(size_t)(oy0 + s) * dw + x] = tile[s][threadIdx.x];
+}
+
 // ---------------------------------------------------------------------------
 // collaborative chroma filter (collaborativeChromaFilter, bm.cpp:250-442).
 // Pass 1: one 64-lane block per anchor patch matches up to 4 patches, runs
@@ -4481,10 +4676,13 @@ hipError_t launch_plain_radial(const float *u, const float *v, int cw, int ch, c
                                float margin, hipStream_t s) {
     if (rd.valid && rd.max_count > 0) {
         const dim3 g((unsigned)((rd.max_count + kBlock - 1) / kBlock), (unsigned)ay.n, (unsigned)rd.nx);
-        if (ax.sup == 6 && ay.sup == 6)
-            hipLaunchKernelGGL(k_plain_radial_tab<6>, g, dim3(kBlock), 0, s, u, v, cw, ch, ax, ay, rd, out_u, out_v,
+        if (ax.sup == 6 && ay.sup == 6) {
+            constexpr int R = kRadialRows;
+            static_assert(kSegCols == kBlock, "one output column per thread in the segment write");
+            const dim3 gr((unsigned)rd.nseg, (unsigned)((ay.n + R - 1) / R));
+            hipLaunchKernelGGL(k_plain_radial_rows<R>, gr, dim3(kBlock), 0, s, u, v, cw, ch, ax, ay, rd, out_u, out_v,
                                clamp_hull, margin);
-        else
+        } else
             hipLaunchKernelGGL(k_plain_radial_tab<0>, g, dim3(kBlock), 0, s, u, v, cw, ch, ax, ay, rd, out_u, out_v,
                                clamp_hull, margin);
         return hipGetLastError();
@@ -4650,9 +4848,11 @@ hipError_t launch_resample_v(const float *src, int sw, int sh, float *dst, const
 hipError_t launch_resample_radial_tab(const float *src, int sw, int sh, float *dst, int dw, int dh,
                                       const RadialResampleDev &t, hipStream_t s) {
     const dim3 g((unsigned)((t.max_count + kBlock - 1) / kBlock), (unsigned)dh, (unsigned)t.ncx);
-    if (t.sup == 6)
-        hipLaunchKernelGGL(k_resample_radial_tab<6>, g, dim3(kBlock), 0, s, src, sw, sh, dst, dw, t);
-    else
+    if (t.sup == 6) {
+        constexpr int R = kRadialRows;
+        const dim3 gr((unsigned)t.nseg, (unsigned)((dh + R - 1) / R));
+        hipLaunchKernelGGL(k_resample_radial_rows<R>, gr, dim3(kBlock), 0, s, src, sw, sh, dst, dw, dh, t);
+    } else
         hipLaunchKernelGGL(k_resample_radial_tab<0>, g, dim3(kBlock), 0, s, src, sw, sh, dst, dw, t);
     return hipGetLastError();
 }

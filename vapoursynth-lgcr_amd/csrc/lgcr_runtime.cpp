@@ -391,7 +391,9 @@ struct HostRadial {
     int nx = 0, ny = 0, max_count = 0;
     std::vector<uint16_t> xphase, yphase;
     std::vector<float> w;
+    std::vector<double> wd;      // w as doubles
     std::vector<int> xs, xoff;   // output columns of each x phase
+    std::vector<int> segoff, segw;  // waves of each column segment (segment_waves)
     std::vector<double> wsa;     // per phase pair: sum and absolute sum of the active weights
 };
 
@@ -424,6 +426,37 @@ int collect_phases(const HostAxis &a, std::vector<uint16_t> &index, std::vector<
     return int(bits.size());
 }
 
+// The waves of a class-major column list (xs, xoff: the output columns of
+// each class, ascending within a class), per segment of kSegCols output
+// columns: for every segment, each class's run of columns inside it cut into
+// waves of up to 64, as {class, first xs entry, end xs entry, 0}.  segoff[s]
+// .. segoff[s + 1] are segment s's waves.  A block computes one segment with
+// every wave on one class and writes the segment's outputs contiguously.
+void segment_waves(const std::vector<int> &xs, const std::vector<int> &xoff, int n, std::vector<int> &segoff,
+                   std::vector<int> &ent) {
+    const int ncls = int(xoff.size()) - 1, nseg = (n + kSegCols - 1) / kSegCols;
+    std::vector<int> at(xoff.begin(), xoff.end() - 1);
+    segoff.assign(size_t(nseg) + 1, 0);
+    ent.clear();
+    for (int sg = 0; sg < nseg; ++sg) {
+        const int end_col = std::min(n, (sg + 1) * kSegCols);
+        for (int q = 0; q < ncls; ++q) {
+            const int b = at[size_t(q)];
+            int e = b;
+            while (e < xoff[size_t(q) + 1] && xs[size_t(e)] < end_col)
+                ++e;
+            for (int k = b; k < e; k += 64) {
+                ent.push_back(q);
+                ent.push_back(k);
+                ent.push_back(std::min(e, k + 64));
+                ent.push_back(0);
+            }
+            at[size_t(q)] = e;
+        }
+        segoff[size_t(sg) + 1] = int(ent.size() / 4);
+    }
+}
+
 float lut_lookup(const std::vector<float> &lut, float idx) {
     const int n = int(lut.size());
     const int i0 = std::clamp(int(idx), 0, n - 2);
@@ -440,6 +473,7 @@ float lut_lookup(const std::vector<float> &lut, float idx) {
 struct HostRadialResample {
     int ncx = 0, ncy = 0, sup = 0, max_count = 0;
     std::vector<int> x0, y0, xs, xoff;
+    std::vector<int> segoff, segw;  // waves of each column segment (segment_waves)
     std::vector<uint8_t> ycls;
     std::vector<double> w, wsum;
 };
@@ -496,6 +530,7 @@ bool build_radial_resample(int sw, int sh, int dw, int dh, double support, const
     std::vector<int> fill(r.xoff.begin(), r.xoff.end() - 1);
     for (int o = 0; o < dw; ++o)
         r.xs[size_t(fill[size_t(xc[size_t(o)])]++)] = o;
+    segment_waves(r.xs, r.xoff, dw, r.segoff, r.segw);
     r.w.resize(size_t(r.ncy) * r.ncx * sup * sup);
     r.wsum.resize(size_t(r.ncy) * r.ncx);
     for (int a = 0; a < r.ncy; ++a)
@@ -558,6 +593,8 @@ HostRadial build_radial(const HostAxis &ax, const HostAxis &ay, int src_w, int s
     std::vector<int> fill(r.xoff.begin(), r.xoff.end() - 1);
     for (size_t o = 0; o < r.xphase.size(); ++o)
         r.xs[size_t(fill[r.xphase[o]]++)] = int(o);
+    segment_waves(r.xs, r.xoff, ax.n, r.segoff, r.segw);
+    r.wd.assign(r.w.begin(), r.w.end());
     r.wsa.assign(2 * pairs, 0.0);
     for (int py = 0; py < ny; ++py)
         for (int px = 0; px < nx; ++px) {
@@ -946,6 +983,7 @@ lgcr_status get_geometry(lgcr_clip *c, double sx, double sy, Geometry **out) {
         hr = build_radial(ax, ay, c->cw, c->ch, c->lut);
     const size_t orx = blob.add(hr.xphase), ory = blob.add(hr.yphase), orw = blob.add(hr.w);
     const size_t orxs = blob.add(hr.xs), orxo = blob.add(hr.xoff), orwsa = blob.add(hr.wsa);
+    const size_t orso = blob.add(hr.segoff), orsw = blob.add(hr.segw), orwd = blob.add(hr.wd);
     const bool a6 = c->p.algo == 6 && c->p.strength > 0.0;
     HostWeights ycw[4];
     size_t oyc[4][2] = {};
@@ -979,6 +1017,7 @@ lgcr_status get_geometry(lgcr_clip *c, double sx, double sy, Geometry **out) {
     }
     const size_t osrx = blob.add(shr.xphase), osry = blob.add(shr.yphase), osrw = blob.add(shr.w);
     const size_t osrxs = blob.add(shr.xs), osrxo = blob.add(shr.xoff), osrwsa = blob.add(shr.wsa);
+    const size_t osrso = blob.add(shr.segoff), osrsw = blob.add(shr.segw), osrwd = blob.add(shr.wd);
     std::vector<TempTap> ttx, tty;
     std::vector<float> tsin;
     if (c->temporal) {
@@ -1058,6 +1097,10 @@ lgcr_status get_geometry(lgcr_clip *c, double sx, double sy, Geometry **out) {
     g->rd.xoff = reinterpret_cast<const int *>(base + orxo);
     g->rd.wsa = reinterpret_cast<const double *>(base + orwsa);
     g->rd.max_count = hr.max_count;
+    g->rd.wd = reinterpret_cast<const double *>(base + orwd);
+    g->rd.segoff = reinterpret_cast<const int *>(base + orso);
+    g->rd.segw = reinterpret_cast<const int4 *>(base + orsw);
+    g->rd.nseg = hr.segoff.empty() ? 0 : int(hr.segoff.size()) - 1;
     g->rd.lut = static_cast<const float *>(c->lut_dev);
     g->rd.lut_n = c->lut_n;
     g->rd.lut_scale = 1024.0f;
@@ -1103,6 +1146,10 @@ lgcr_status get_geometry(lgcr_clip *c, double sx, double sy, Geometry **out) {
         g->srd.xoff = reinterpret_cast<const int *>(base + osrxo);
         g->srd.wsa = reinterpret_cast<const double *>(base + osrwsa);
         g->srd.max_count = shr.max_count;
+        g->srd.wd = reinterpret_cast<const double *>(base + osrwd);
+        g->srd.segoff = reinterpret_cast<const int *>(base + osrso);
+        g->srd.segw = reinterpret_cast<const int4 *>(base + osrsw);
+        g->srd.nseg = shr.segoff.empty() ? 0 : int(shr.segoff.size()) - 1;
         g->srd.inv_wx = float(std::min(1.0, double(c->sw) / c->cw));
         g->srd.inv_wy = float(std::min(1.0, double(c->sh) / c->ch));
     } else {
@@ -2150,7 +2197,10 @@ static lgcr_status create_clip(lgcr_device *dev, const lgcr_params *params, cons
             const size_t a_xs = place(rr.xs.size() * 4), a_xo = place(rr.xoff.size() * 4);
             const size_t a_yc = place(rr.ycls.size()), a_w = place(rr.w.size() * 8);
             const size_t a_ws = place(rr.wsum.size() * 8);
+            const size_t a_so = place(rr.segoff.size() * 4), a_sw = place(rr.segw.size() * 4);
             std::vector<uint8_t> blob(off);
+            std::memcpy(&blob[a_so], rr.segoff.data(), rr.segoff.size() * 4);
+            std::memcpy(&blob[a_sw], rr.segw.data(), rr.segw.size() * 4);
             std::memcpy(&blob[a_x0], rr.x0.data(), rr.x0.size() * 4);
             std::memcpy(&blob[a_y0], rr.y0.data(), rr.y0.size() * 4);
             std::memcpy(&blob[a_xs], rr.xs.data(), rr.xs.size() * 4);
@@ -2174,6 +2224,9 @@ static lgcr_status create_clip(lgcr_device *dev, const lgcr_params *params, cons
             t.ycls = b + a_yc;
             t.w = reinterpret_cast<const double *>(b + a_w);
             t.wsum = reinterpret_cast<const double *>(b + a_ws);
+            t.segoff = reinterpret_cast<const int *>(b + a_so);
+            t.segw = reinterpret_cast<const int4 *>(b + a_sw);
+            t.nseg = int(rr.segoff.size()) - 1;
         }
     }
     Geometry *g = nullptr;
