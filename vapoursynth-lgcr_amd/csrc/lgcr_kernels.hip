@@ -1116,6 +1116,31 @@ __global__ __launch_bounds__(kBlock) void k_plain_h(const float *__restrict__ u,
             mnv = on ? nmv : mnv;
             mxv = on ? nxv : mxv;
         }
+    } else if constexpr (SUP > 6) {
+        // fixed tap count from the plain tables: every load issued up
+        // front, inactive taps leave the sums and hull untouched (selects)
+        const int start = ax.start[ox];
+        const float *wt = ax.w + (size_t)ox * SUP;
+        const float *am = ax.am + (size_t)ox * SUP;
+        float a[SUP], b[SUP], w[SUP];
+        bool on[SUP];
+        _Pragma("unroll") for (int i = 0; i < SUP; ++i) {
+            const int x = clampi(start + i, 0, cw - 1);
+            a[i] = su[x];
+            b[i] = sv[x];
+            w[i] = wt[i];
+            on[i] = am[i] != 0.0f;
+        }
+        _Pragma("unroll") for (int i = 0; i < SUP; ++i) {
+            const double nu = dacc(vu, w[i], a[i]), nv = dacc(vv, w[i], b[i]);
+            vu = on[i] ? nu : vu;
+            vv = on[i] ? nv : vv;
+            const float nmu = fmn(mnu, a[i]), nxu = fmx(mxu, a[i]), nmv = fmn(mnv, b[i]), nxv = fmx(mxv, b[i]);
+            mnu = on[i] ? nmu : mnu;
+            mxu = on[i] ? nxu : mxu;
+            mnv = on[i] ? nmv : mnv;
+            mxv = on[i] ? nxv : mxv;
+        }
     } else {
         const int start = ax.start[ox];
         const float *wt = ax.w + (size_t)ox * ax.sup;
@@ -1175,6 +1200,22 @@ __global__ __launch_bounds__(kBlock) void k_plain_v(const float *__restrict__ u,
             mxu = on ? nxu : mxu;
             mnv = on ? nmv : mnv;
             mxv = on ? nxv : mxv;
+        }
+    } else if constexpr (SUP > 6) {
+        float2 val[SUP];
+        float4 hull[SUP];
+        _Pragma("unroll") for (int j = 0; j < SUP; ++j) {
+            const size_t k = (size_t)clampi(start + j, 0, ch - 1) * ow + ox;
+            val[j] = hv[k];
+            hull[j] = hh[k];
+        }
+        _Pragma("unroll") for (int j = 0; j < SUP; ++j) {
+            if (am[j] == 0.0f)   // uniform per block row
+                continue;
+            vu = dacc(vu, wt[j], val[j].x);
+            vv = dacc(vv, wt[j], val[j].y);
+            mnu = fmn(mnu, hull[j].x); mxu = fmx(mxu, hull[j].y);
+            mnv = fmn(mnv, hull[j].z); mxv = fmx(mxv, hull[j].w);
         }
     } else {
         for (int j = 0; j < ay.sup; ++j) {
@@ -3218,27 +3259,48 @@ __global__ __launch_bounds__(kBlock) void k_back_project(float *ou, float *ov, i
 // kernels.cpp:276-334)
 // ---------------------------------------------------------------------------
 
+// SUP > 0: that many taps, every load issued before the sums; 0: t.sup.
+template <int SUP>
 __global__ __launch_bounds__(kBlock) void k_resample_h(const float *src, int sw, int sh, float *dst, WeightsDev t) {
     const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
     if (x >= t.n)
         return;
-    const float *wp = t.w + (size_t)x * t.sup;
+    const float *wp = t.w + (size_t)x * (SUP ? SUP : t.sup);
+    const float *row = src + (size_t)y * sw;
     const int s = t.start[x];
     double acc = 0.0;
-    for (int j = 0; j < t.sup; ++j)
-        acc = dacc(acc, wp[j], src[(size_t)y * sw + clampi(s + j, 0, sw - 1)]);
+    if constexpr (SUP > 0) {
+        float w[SUP], a[SUP];
+        _Pragma("unroll") for (int j = 0; j < SUP; ++j) {
+            w[j] = wp[j];
+            a[j] = row[clampi(s + j, 0, sw - 1)];
+        }
+        _Pragma("unroll") for (int j = 0; j < SUP; ++j)
+            acc = dacc(acc, w[j], a[j]);
+    } else {
+        for (int j = 0; j < t.sup; ++j)
+            acc = dacc(acc, wp[j], row[clampi(s + j, 0, sw - 1)]);
+    }
     dst[(size_t)y * t.n + x] = (float)acc;
 }
-
+template <int SUP>
 __global__ __launch_bounds__(kBlock) void k_resample_v(const float *src, int sw, int sh, float *dst, WeightsDev t) {
     const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
     if (x >= sw)
         return;
-    const float *wp = t.w + (size_t)y * t.sup;
+    const float *wp = t.w + (size_t)y * (SUP ? SUP : t.sup);
     const int s = t.start[y];
     double acc = 0.0;
-    for (int j = 0; j < t.sup; ++j)
-        acc = dacc(acc, wp[j], src[(size_t)clampi(s + j, 0, sh - 1) * sw + x]);
+    if constexpr (SUP > 0) {
+        float a[SUP];
+        _Pragma("unroll") for (int j = 0; j < SUP; ++j)
+            a[j] = src[(size_t)clampi(s + j, 0, sh - 1) * sw + x];
+        _Pragma("unroll") for (int j = 0; j < SUP; ++j)
+            acc = dacc(acc, wp[j], a[j]);
+    } else {
+        for (int j = 0; j < t.sup; ++j)
+            acc = dacc(acc, wp[j], src[(size_t)clampi(s + j, 0, sh - 1) * sw + x]);
+    }
     dst[(size_t)y * sw + x] = (float)acc;
 }
 
@@ -4470,10 +4532,15 @@ hipError_t launch_plain_separable(const float *u, const float *v, int cw, int ch
     }
     if (ax.sup == 6 && ax.rec)
         hipLaunchKernelGGL(k_plain_h<6>, rows_grid(ax.n, ch), dim3(kBlock), 0, s, u, v, cw, ch, ax, hbuf);
+    else if (ax.sup == 8)
+        hipLaunchKernelGGL(k_plain_h<8>, rows_grid(ax.n, ch), dim3(kBlock), 0, s, u, v, cw, ch, ax, hbuf);
     else
         hipLaunchKernelGGL(k_plain_h<0>, rows_grid(ax.n, ch), dim3(kBlock), 0, s, u, v, cw, ch, ax, hbuf);
     if (ay.sup == 6)
         hipLaunchKernelGGL(k_plain_v<6>, rows_grid(ax.n, ay.n), dim3(kBlock), 0, s, u, v, cw, ch, ax, ay, hbuf,
+                           out_u, out_v, clamp_hull, margin);
+    else if (ay.sup == 8)
+        hipLaunchKernelGGL(k_plain_v<8>, rows_grid(ax.n, ay.n), dim3(kBlock), 0, s, u, v, cw, ch, ax, ay, hbuf,
                            out_u, out_v, clamp_hull, margin);
     else
         hipLaunchKernelGGL(k_plain_v<0>, rows_grid(ax.n, ay.n), dim3(kBlock), 0, s, u, v, cw, ch, ax, ay, hbuf,
@@ -4836,12 +4903,28 @@ hipError_t launch_back_project(float *out_u, float *out_v, int ow, int oh, const
 }
 
 hipError_t launch_resample_h(const float *src, int sw, int sh, float *dst, const WeightsDev &t, hipStream_t s) {
-    hipLaunchKernelGGL(k_resample_h, rows_grid(t.n, sh), dim3(kBlock), 0, s, src, sw, sh, dst, t);
+    const dim3 g = rows_grid(t.n, sh);
+    if (t.sup == 4)
+        hipLaunchKernelGGL(k_resample_h<4>, g, dim3(kBlock), 0, s, src, sw, sh, dst, t);
+    else if (t.sup == 6)
+        hipLaunchKernelGGL(k_resample_h<6>, g, dim3(kBlock), 0, s, src, sw, sh, dst, t);
+    else if (t.sup == 8)
+        hipLaunchKernelGGL(k_resample_h<8>, g, dim3(kBlock), 0, s, src, sw, sh, dst, t);
+    else
+        hipLaunchKernelGGL(k_resample_h<0>, g, dim3(kBlock), 0, s, src, sw, sh, dst, t);
     return hipGetLastError();
 }
 
 hipError_t launch_resample_v(const float *src, int sw, int sh, float *dst, const WeightsDev &t, hipStream_t s) {
-    hipLaunchKernelGGL(k_resample_v, rows_grid(sw, t.n), dim3(kBlock), 0, s, src, sw, sh, dst, t);
+    const dim3 g = rows_grid(sw, t.n);
+    if (t.sup == 4)
+        hipLaunchKernelGGL(k_resample_v<4>, g, dim3(kBlock), 0, s, src, sw, sh, dst, t);
+    else if (t.sup == 6)
+        hipLaunchKernelGGL(k_resample_v<6>, g, dim3(kBlock), 0, s, src, sw, sh, dst, t);
+    else if (t.sup == 8)
+        hipLaunchKernelGGL(k_resample_v<8>, g, dim3(kBlock), 0, s, src, sw, sh, dst, t);
+    else
+        hipLaunchKernelGGL(k_resample_v<0>, g, dim3(kBlock), 0, s, src, sw, sh, dst, t);
     return hipGetLastError();
 }
 
