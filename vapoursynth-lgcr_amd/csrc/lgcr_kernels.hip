@@ -3740,42 +3740,64 @@ struct SharpenArgs {
     int clamp_hull;
 };
 
+// sharpenPlane (algos.cpp:595-659): a block computes a 64x4 tile; the
+// tile's plane and guide samples with a 2-sample apron are staged in LDS at
+// clamped coordinates, so every lookup equals the reference's clamped read,
+// and the 24 spatial weights (the same for every pixel) are formed once per
+// block.  Both reciprocals have arguments >= 1, where recip_ge1 equals the
+// IEEE quotient.  Taps in the reference's (j, i) order.
+constexpr int kShW = 64, kShH = 4, kShAW = kShW + 4, kShAH = kShH + 4;
 __global__ __launch_bounds__(kBlock) void k_sharpen(SharpenArgs A) {
-    const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
-    if (x >= A.w)
-        return;
+    __shared__ float ts[kShAH][kShAW + 1];
+    __shared__ float tg[kShAH][kShAW + 1];
+    __shared__ float spw[25];
     const int w = A.w, h = A.h;
+    const int x0 = blockIdx.x * kShW, y0 = blockIdx.y * kShH;
     const float *src = A.src;
     const float *gd = A.lc ? A.lc : src;
-    const float c0 = at(src, w, h, x, y);
+    for (int e = threadIdx.x; e < kShAH * kShAW; e += kBlock) {
+        const int r = e / kShAW, c = e % kShAW;
+        const size_t q = (size_t)clampi(y0 - 2 + r, 0, h - 1) * w + clampi(x0 - 2 + c, 0, w - 1);
+        ts[r][c] = src[q];
+        tg[r][c] = gd[q];
+    }
+    if (threadIdx.x < 25) {
+        const int j = (int)threadIdx.x / 5 - 2, i = (int)threadIdx.x % 5 - 2;
+        spw[threadIdx.x] = recip_ge1(1.0f + (float)(i * i + j * j) * A.inv_g2);
+    }
+    __syncthreads();
+    const int tx = threadIdx.x % kShW, ty = threadIdx.x / kShW;
+    const int x = x0 + tx, y = y0 + ty;
+    if (x >= w || y >= h)
+        return;
+    const float c0 = ts[ty + 2][tx + 2];
     const float l0 = A.lc ? bilinear(A.gy, A.gw, A.gh, (x + 0.5) * A.rw - 0.5 + A.sx, (y + 0.5) * A.rh - 0.5 + A.sy)
                           : c0;
     float max_dl = 0.0f, lo = 1e30f, hi = -1e30f;
-    for (int j = -2; j <= 2; ++j)
-        for (int i = -2; i <= 2; ++i) {
-            const float s = at(src, w, h, x + i, y + j);
-            max_dl = fmx(max_dl, fabsf(at(gd, w, h, x + i, y + j) - l0));
-            lo = fmn(lo, s);
-            hi = fmx(hi, s);
+    for (int j = 0; j < 5; ++j)
+        for (int i = 0; i < 5; ++i) {
+            const float sv = ts[ty + j][tx + i];
+            max_dl = fmx(max_dl, fabsf(tg[ty + j][tx + i] - l0));
+            lo = fmn(lo, sv);
+            hi = fmx(hi, sv);
         }
     // median of the four neighbour slopes: mean of the middle two
-    const float a = fabsf(at(src, w, h, x + 1, y) - c0), b = fabsf(at(src, w, h, x - 1, y) - c0);
-    const float c = fabsf(at(src, w, h, x, y + 1) - c0), d = fabsf(at(src, w, h, x, y - 1) - c0);
+    const float a = fabsf(ts[ty + 2][tx + 3] - c0), b = fabsf(ts[ty + 2][tx + 1] - c0);
+    const float c = fabsf(ts[ty + 3][tx + 2] - c0), d = fabsf(ts[ty + 1][tx + 2] - c0);
     const float a1 = fmn(a, b), b1 = fmx(a, b), c1 = fmn(c, d), d1 = fmx(c, d);
     const float m1 = fmx(a1, c1), m2 = fmn(b1, d1);
     const float med = (fmn(m1, m2) + fmx(m1, m2)) * 0.5f;
     const float sig = fmx(fmx(A.sigma, A.sratio * max_dl), 3.0f * med);
     const float inv_s2 = 1.0f / (sig * sig);
     float hp = 0.0f, ws = 0.0f;
-    for (int j = -2; j <= 2; ++j)
-        for (int i = -2; i <= 2; ++i) {
-            if (i == 0 && j == 0)
+    for (int j = 0; j < 5; ++j)
+        for (int i = 0; i < 5; ++i) {
+            if (i == 2 && j == 2)
                 continue;
-            const float dl = at(gd, w, h, x + i, y + j) - l0;
-            const float sim = 1.0f / (1.0f + dl * dl * inv_s2);
-            const float sp = 1.0f / (1.0f + (float)(i * i + j * j) * A.inv_g2);
-            const float n = sp * sim;
-            hp += n * (c0 - at(src, w, h, x + i, y + j));
+            const float dl = tg[ty + j][tx + i] - l0;
+            const float sim = recip_ge1(1.0f + dl * dl * inv_s2);
+            const float n = spw[j * 5 + i] * sim;
+            hp += n * (c0 - ts[ty + j][tx + i]);
             ws += n;
         }
     float out = c0;
@@ -4956,7 +4978,9 @@ hipError_t launch_sharpen(const float *src, float *dst, int w, int h, const floa
                           hipStream_t s) {
     SharpenArgs A{src, dst, w, h, lc, gy, gw, gh, rw, rh, sx, sy, p.inv_g2, p.sigma, p.sratio, p.alpha, p.margin,
                   p.clamp_hull};
-    hipLaunchKernelGGL(k_sharpen, rows_grid(w, h), dim3(kBlock), 0, s, A);
+    static_assert(kShW * kShH == kBlock, "one pixel per thread");
+    hipLaunchKernelGGL(k_sharpen, dim3((unsigned)((w + kShW - 1) / kShW), (unsigned)((h + kShH - 1) / kShH)),
+                       dim3(kBlock), 0, s, A);
     return hipGetLastError();
 }
 
