@@ -40,8 +40,12 @@ struct WeightsDev {
     const float *w = nullptr;
 };
 
-// Output columns per block of the class-table radial kernels.
+// Output columns per block of the class-table radial kernels, output rows
+// per thread, and the largest source window (rows x columns) a block of the
+// radial plain base stages in LDS.
 constexpr int kSegCols = 256;
+constexpr int kRadialRows = 4;
+constexpr int kRadTileRows = 10, kRadTileCols = 264;
 
 // Radial (jinc) base-weight phase table (RadialWeightTable,
 // src/lgcr.h:793-811).  valid == 0 means the kernels evaluate the LUT per tap.
@@ -60,6 +64,7 @@ struct RadialDev {
     const int *segoff = nullptr;
     const int4 *segw = nullptr;
     int nseg = 0;
+    int tile_fit = 0;  // every block's source window fits kRadTileRows x kRadTileCols
     const float *lut = nullptr;
     int lut_n = 0;
     float lut_scale = 1024.0f;

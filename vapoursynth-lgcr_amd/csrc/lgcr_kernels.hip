@@ -1501,13 +1501,17 @@ __global__ __launch_bounds__(kBlock) void k_plain_radial_tab(const float *__rest
 // picks the same sample as the running hull over the taps in (j, i) order
 // (ties keep the earlier sample, NaN never replaces a bound), so the results
 // are those of the per-tap order.
-constexpr int kRadialRows = 4;
-template <int R>
+// TILE (rd.tile_fit): the block's source rows and columns are first staged
+// in LDS at clamped coordinates (every read then equals the clamped global
+// read), so the taps read LDS instead of global memory.
+template <int R, bool TILE>
 __global__ __launch_bounds__(kBlock) void k_plain_radial_rows(const float *__restrict__ u,
                                                               const float *__restrict__ v, int cw, int ch,
                                                               AxisDev ax, AxisDev ay, RadialDev rd, float *out_u,
                                                               float *out_v, int clamp_hull, float margin) {
+    static_assert(R == kRadialRows, "the host checks the source window for kRadialRows rows");
     __shared__ float2 tile[R][kSegCols];
+    __shared__ float2 srct[TILE ? kRadTileRows : 1][TILE ? kRadTileCols : 1];
     const int sg = blockIdx.x, oy0 = blockIdx.y * R;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int nrow = min(R, ay.n - oy0);
@@ -1526,6 +1530,18 @@ __global__ __launch_bounds__(kBlock) void k_plain_radial_rows(const float *__res
             rlo = min(rlo, ty[s]);
             rhi = max(rhi, ty[s] + 5);
         }
+    }
+    // source columns of the segment (chroma starts rise with the output column)
+    const int c_lo = ax.start[sg * kSegCols];
+    if constexpr (TILE) {
+        const int c_hi = ax.start[min(sg * kSegCols + kSegCols - 1, ax.n - 1)] + 5;
+        const int tw = c_hi - c_lo + 1, th = rhi - rlo + 1;  // at most kRadTileCols x kRadTileRows (host check)
+        for (int e = threadIdx.x; e < tw * th; e += kBlock) {
+            const int r = e / tw, c = e - r * tw;
+            const size_t q = (size_t)clampi(rlo + r, 0, ch - 1) * cw + clampi(c_lo + c, 0, cw - 1);
+            srct[r][c] = make_float2(u[q], v[q]);
+        }
+        __syncthreads();
     }
     for (int e = rd.segoff[sg] + wave; e < rd.segoff[sg + 1]; e += kBlock / 64) {
         const int4 en = rd.segw[e];
@@ -1555,24 +1571,34 @@ __global__ __launch_bounds__(kBlock) void k_plain_radial_rows(const float *__res
         auto rows = [&](auto all6_tag) {
             constexpr bool ALL6 = decltype(all6_tag)::value;
             float na[6], nb[6];
-            {
+            if constexpr (!TILE) {
                 const size_t row = (size_t)clampi(rlo, 0, ch - 1) * cw;
                 _Pragma("unroll") for (int i = 0; i < 6; ++i) {
                     na[i] = u[row + col[i]];
                     nb[i] = v[row + col[i]];
                 }
             }
+            const int co = tx0 - c_lo;
             for (int r = rlo; r <= rhi; ++r) {
                 float a[6], b[6];
-                _Pragma("unroll") for (int i = 0; i < 6; ++i) {
-                    a[i] = na[i];
-                    b[i] = nb[i];
-                }
-                if (r < rhi) {
-                    const size_t row = (size_t)clampi(r + 1, 0, ch - 1) * cw;
+                if constexpr (TILE) {
+                    const float2 *tr = &srct[r - rlo][co];
                     _Pragma("unroll") for (int i = 0; i < 6; ++i) {
-                        na[i] = u[row + col[i]];
-                        nb[i] = v[row + col[i]];
+                        const float2 t = tr[i];
+                        a[i] = t.x;
+                        b[i] = t.y;
+                    }
+                } else {
+                    _Pragma("unroll") for (int i = 0; i < 6; ++i) {
+                        a[i] = na[i];
+                        b[i] = nb[i];
+                    }
+                    if (r < rhi) {
+                        const size_t row = (size_t)clampi(r + 1, 0, ch - 1) * cw;
+                        _Pragma("unroll") for (int i = 0; i < 6; ++i) {
+                            na[i] = u[row + col[i]];
+                            nb[i] = v[row + col[i]];
+                        }
                     }
                 }
                 // the samples in double (exact) once for every output row
@@ -4769,8 +4795,12 @@ hipError_t launch_plain_radial(const float *u, const float *v, int cw, int ch, c
             constexpr int R = kRadialRows;
             static_assert(kSegCols == kBlock, "one output column per thread in the segment write");
             const dim3 gr((unsigned)rd.nseg, (unsigned)((ay.n + R - 1) / R));
-            hipLaunchKernelGGL(k_plain_radial_rows<R>, gr, dim3(kBlock), 0, s, u, v, cw, ch, ax, ay, rd, out_u, out_v,
-                               clamp_hull, margin);
+            if (rd.tile_fit)
+                hipLaunchKernelGGL((k_plain_radial_rows<R, true>), gr, dim3(kBlock), 0, s, u, v, cw, ch, ax, ay, rd,
+                                   out_u, out_v, clamp_hull, margin);
+            else
+                hipLaunchKernelGGL((k_plain_radial_rows<R, false>), gr, dim3(kBlock), 0, s, u, v, cw, ch, ax, ay, rd,
+                                   out_u, out_v, clamp_hull, margin);
         } else
             hipLaunchKernelGGL(k_plain_radial_tab<0>, g, dim3(kBlock), 0, s, u, v, cw, ch, ax, ay, rd, out_u, out_v,
                                clamp_hull, margin);

@@ -394,6 +394,7 @@ struct HostRadial {
     std::vector<double> wd;      // w as doubles
     std::vector<int> xs, xoff;   // output columns of each x phase
     std::vector<int> segoff, segw;  // waves of each column segment (segment_waves)
+    bool tile_fit = false;          // every block's source window fits the LDS tile
     std::vector<double> wsa;     // per phase pair: sum and absolute sum of the active weights
 };
 
@@ -595,6 +596,25 @@ HostRadial build_radial(const HostAxis &ax, const HostAxis &ay, int src_w, int s
         r.xs[size_t(fill[r.xphase[o]]++)] = int(o);
     segment_waves(r.xs, r.xoff, ax.n, r.segoff, r.segw);
     r.wd.assign(r.w.begin(), r.w.end());
+    // source window of each (segment, row group) block of the plain pass:
+    // starts rise with the output position, so the first and last entries
+    // of a segment / row group bound it
+    r.tile_fit = ax.sup == 6 && ay.sup == 6;
+    for (int c0 = 0; r.tile_fit && c0 < ax.n; c0 += kSegCols) {
+        const int c1 = std::min(c0 + kSegCols, ax.n) - 1;
+        r.tile_fit = ax.start[size_t(c1)] >= ax.start[size_t(c0)] &&
+                     ax.start[size_t(c1)] + 6 - ax.start[size_t(c0)] <= kRadTileCols;
+        for (int c = c0 + 1; r.tile_fit && c <= c1; ++c)
+            r.tile_fit = ax.start[size_t(c)] >= ax.start[size_t(c - 1)];
+    }
+    for (int y0 = 0; r.tile_fit && y0 < ay.n; y0 += kRadialRows) {
+        int lo = INT_MAX, hi = INT_MIN;
+        for (int y = y0; y < std::min(y0 + kRadialRows, ay.n); ++y) {
+            lo = std::min(lo, ay.start[size_t(y)]);
+            hi = std::max(hi, ay.start[size_t(y)] + 5);
+        }
+        r.tile_fit = hi - lo + 1 <= kRadTileRows;
+    }
     r.wsa.assign(2 * pairs, 0.0);
     for (int py = 0; py < ny; ++py)
         for (int px = 0; px < nx; ++px) {
@@ -1098,6 +1118,7 @@ lgcr_status get_geometry(lgcr_clip *c, double sx, double sy, Geometry **out) {
     g->rd.wsa = reinterpret_cast<const double *>(base + orwsa);
     g->rd.max_count = hr.max_count;
     g->rd.wd = reinterpret_cast<const double *>(base + orwd);
+    g->rd.tile_fit = hr.tile_fit ? 1 : 0;
     g->rd.segoff = reinterpret_cast<const int *>(base + orso);
     g->rd.segw = reinterpret_cast<const int4 *>(base + orsw);
     g->rd.nseg = hr.segoff.empty() ? 0 : int(hr.segoff.size()) - 1;
@@ -1147,6 +1168,7 @@ lgcr_status get_geometry(lgcr_clip *c, double sx, double sy, Geometry **out) {
         g->srd.wsa = reinterpret_cast<const double *>(base + osrwsa);
         g->srd.max_count = shr.max_count;
         g->srd.wd = reinterpret_cast<const double *>(base + osrwd);
+        g->srd.tile_fit = shr.tile_fit ? 1 : 0;
         g->srd.segoff = reinterpret_cast<const int *>(base + osrso);
         g->srd.segw = reinterpret_cast<const int4 *>(base + osrsw);
         g->srd.nseg = shr.segoff.empty() ? 0 : int(shr.segoff.size()) - 1;
