@@ -3695,24 +3695,33 @@ __global__ __launch_bounds__(64) void k_bm_group(BmArgs A) {
         A.gw[a] = 1.0f / (1.0f + retained);
 }
 
+// One wave per 8x8 pixel tile (a block covers 32x8 pixels).  A pixel's
+// matches come from anchors starting within [p - 15, p + 8] on each axis
+// (members lie within 8 of their anchor, clamped inward); the wave walks the
+// anchors of its whole tile, [x0 - 15, x0 + 15] x [y0 - 15, y0 + 15], with
+// wave-uniform bounds, and each lane keeps the members that cover its pixel,
+// so every pixel sums its contributions in the reference's (anchor row,
+// anchor column, member) order.
+constexpr int kBmAggTiles = kBlock / 64;
 __global__ __launch_bounds__(kBlock) void k_bm_aggregate(BmArgs A) {
-    const int x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
-    if (x >= A.w)
-        return;
-    // anchors whose matches can cover (x, y): start within [p - 15, p + 8]
-    const int ix0 = max(0, (x - 15) / 8 - 1), iy0 = max(0, (y - 15) / 8 - 1);
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int x0 = ((int)blockIdx.x * kBmAggTiles + wave) * 8, y0 = (int)blockIdx.y * 8;
+    if (x0 >= A.w)
+        return;  // whole wave
+    const int x = x0 + (lane & 7), y = y0 + (lane >> 3);
+    const int ix0 = max(0, (x0 - 15) / 8 - 1), iy0 = max(0, (y0 - 15) / 8 - 1);
     float su = 0.0f, sv = 0.0f, sw = 0.0f;
     for (int iy = iy0; iy < A.ny; ++iy) {
         const int ay = bm_start(iy, A.ny, A.h);
-        if (ay > y + 8)
+        if (ay > y0 + 15)
             break;
-        if (ay < y - 15)
+        if (ay < y0 - 15)
             continue;
         for (int ix = ix0; ix < A.nx; ++ix) {
             const int ax = bm_start(ix, A.nx, A.w);
-            if (ax > x + 8)
+            if (ax > x0 + 15)
                 break;
-            if (ax < x - 15)
+            if (ax < x0 - 15)
                 continue;
             const int a = iy * A.nx + ix;
             const int gc = A.gc[a];
@@ -3732,6 +3741,8 @@ __global__ __launch_bounds__(kBlock) void k_bm_aggregate(BmArgs A) {
             }
         }
     }
+    if (x >= A.w || y >= A.h)
+        return;
     const size_t q = (size_t)y * A.w + x;
     if (sw > 0.0f) {
         const float s0 = A.u[q], s1 = A.v[q];
@@ -5074,7 +5085,8 @@ hipError_t launch_collaborative(const float *guide, int gw, int gh, float *u, fl
         A.g = guide_tmp;
     }
     hipLaunchKernelGGL(k_bm_group, dim3((unsigned)na), dim3(64), 0, s, A);
-    hipLaunchKernelGGL(k_bm_aggregate, rows_grid(w, h), dim3(kBlock), 0, s, A);
+    hipLaunchKernelGGL(k_bm_aggregate, dim3((unsigned)((w + 8 * kBmAggTiles - 1) / (8 * kBmAggTiles)), (unsigned)((h + 7) / 8)),
+                       dim3(kBlock), 0, s, A);
     return hipGetLastError();
 }
 
