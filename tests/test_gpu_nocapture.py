@@ -38,6 +38,11 @@ CASES = [
     ("p8-420-bp", 96, 64, 1, 1, 8, {"bp": 1.0}),
     ("ps-420-bp", 96, 64, 1, 1, 32, {"bp": 0.5}),
     ("p16-420-bp-ragged", 100, 58, 1, 1, 16, {"bp": 0.7}),
+    # algo 4 with integer output: the selector is final in the tiles, which
+    # back-project and pack the output themselves
+    ("p10-420-algo4-bp", 160, 96, 1, 1, 10, {"algo": 4, "bp": 0.5}),
+    ("p8-420-algo4-bp-ragged", 100, 58, 1, 1, 8, {"algo": 4, "bp": 0.5}),
+    ("p16-422-algo4", 96, 64, 1, 0, 16, {"algo": 4}),
     ("p8-422-bp", 96, 64, 1, 0, 8, {"bp": 0.5}),
 ]
 
@@ -85,6 +90,20 @@ def test_bench_frame_no_capture(no_capture):
         _, st = clip.recon_host(y, u, v, stats=True)
         assert st.active_output_pixels == len(dbg["indices"])
         assert st.active_chroma_pixels == int(dbg["chroma_mask"].sum())
+    finally:
+        clip.close()
+
+
+def test_algo4_frame_no_capture(no_capture):
+    """1920x1080 YUV420P10, algo 4, bp 0.5 (the algo-4 benchmark workload):
+    selector, back-projection and packing in the tiles, bit-exact."""
+    y, u, v = frame(1920, 1080, bits=10, seed=3, polygons=40)
+    o_out, _, g_out, clip = run_both(y, u, v, bits=10, device=no_capture, algo=4, bp=0.5)
+    try:
+        for i, name in enumerate("YUV"):
+            d = np.abs(o_out[i].astype(np.int64) - g_out[i].astype(np.int64))
+            print(f"1080p algo 4 {name}: max_abs={d.max()}")
+            assert d.max() == 0
     finally:
         clip.close()
 

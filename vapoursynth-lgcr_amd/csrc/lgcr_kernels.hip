@@ -2209,11 +2209,13 @@ __device__ __attribute__((always_inline)) void guided_pixel(const GuidedArgs &A,
             }
             return;
         }
+        // the plain base of the pixel: in `res` when the tile packs the
+        // output itself, else in the output planes
+        const float bu = res ? res->x : A.out_u[o], bv = res ? res->y : A.out_v[o];
         if (w2 >= 1e-4f || w3 >= 1e-4f) {
             const size_t nc = (size_t)cw * ch;
             const int cx = px.cb_i0, cy = py.cb_i0;
             const float cxf = px.cb_f, cyf = py.cb_f;
-            const float bu = A.out_u[o], bv = A.out_v[o];
             const float au = bilinear_fast(S.lgf, cw, ch, cx, cxf, cy, cyf);
             const float bbu = bilinear_fast(S.lgf + nc, cw, ch, cx, cxf, cy, cyf);
             const float cu = bilinear_fast(S.lgf + 2 * nc, cw, ch, cx, cxf, cy, cyf);
@@ -2223,8 +2225,8 @@ __device__ __attribute__((always_inline)) void guided_pixel(const GuidedArgs &A,
             vu = bu + w2 * ((float)vu - bu) + w3 * cu * (au * L0 + bbu - bu);
             vv = bv + w2 * ((float)vv - bv) + w3 * cv * (av * L0 + bbv - bv);
         } else {
-            vu = A.out_u[o];
-            vv = A.out_v[o];
+            vu = bu;
+            vv = bv;
         }
     }
     if (res) {
@@ -2367,7 +2369,7 @@ __device__ __attribute__((always_inline)) void guided_tile(const GuidedArgs &A, 
     static_assert(!PH || PLAIN, "the plain hull comes from the plain base in the tile");
     // fused output needs every inside pixel's result in the tile: the plain
     // base in the tiles (Recon), or a dense temporal pass (TRecon, no mask)
-    static_assert(OUTT == 0 || (SEL == SEL_NONE && PLAIN != TEMPORAL), "fused output needs every pixel's result");
+    static_assert(OUTT == 0 || (SEL != SEL_FLOAT && PLAIN != TEMPORAL), "fused output needs every pixel's result");
     const int ow = A.ax.n, oh = A.ay.n, cw = A.cw, ch = A.ch;
     const int tx = t % tiles_x, ty = t / tiles_x;
     const int ox = tx * kTile + (lane & (kTile - 1)), oy = ty * kTile + (lane / kTile);
@@ -4638,14 +4640,14 @@ hipError_t launch_guided(const float *u, const float *v, int cw, int ch, const f
             // the tiles without temporal taps (bp_on only for same-size 4:2:0
             // with even sizes) or a dense temporal pass without bp
             const bool recon = plain_in_tiles && tmp.n == 0 && !(bp_on && ((ax.n | ay.n) & 1));
-            const bool trecon = !plain_in_tiles && tmp.n > 0 && !A.mbits && !bp_on;
-            if (sel.mode != SEL_NONE || !(recon || trecon))
+            const bool trecon = !plain_in_tiles && tmp.n > 0 && !A.mbits && !bp_on && sel.mode == SEL_NONE;
+            if (sel.mode == SEL_FLOAT || !(recon || trecon))
                 return hipErrorInvalidValue;
             A.po_u = fused_out[0];
             A.po_v = fused_out[1];
             A.bp = bp;
             A.bp_on = bp_on;
-            if (defer) {
+            if (defer && sel.mode == SEL_NONE) {
                 const bool ph = recon && gp.finite;
                 BatchFrame bf{A, tiles_x, tiles};
                 memcpy(defer->args, &bf, sizeof bf);
@@ -4653,19 +4655,27 @@ hipError_t launch_guided(const float *u, const float *v, int cw, int ch, const f
                 defer->tiles = tiles;
                 return hipSuccess;
             }
-#define LGCR_FUSED(T, P, H)                                                                                       \
+#define LGCR_FUSED(T, S_, P, H)                                                                                   \
     if (out_type == RAW_U8)                                                                                     \
-        hipLaunchKernelGGL((k_guided_tiles<T, SEL_NONE, P, 1, H>), g, dim3(kBlock), 0, s, A, tiles_x, tiles);   \
+        hipLaunchKernelGGL((k_guided_tiles<T, S_, P, 1, H>), g, dim3(kBlock), 0, s, A, tiles_x, tiles);         \
     else if (out_type == RAW_U16)                                                                               \
-        hipLaunchKernelGGL((k_guided_tiles<T, SEL_NONE, P, 2, H>), g, dim3(kBlock), 0, s, A, tiles_x, tiles);   \
+        hipLaunchKernelGGL((k_guided_tiles<T, S_, P, 2, H>), g, dim3(kBlock), 0, s, A, tiles_x, tiles);         \
     else                                                                                                        \
-        hipLaunchKernelGGL((k_guided_tiles<T, SEL_NONE, P, 3, H>), g, dim3(kBlock), 0, s, A, tiles_x, tiles);
-            if (recon && gp.finite) {
-                LGCR_FUSED(false, true, true)
+        hipLaunchKernelGGL((k_guided_tiles<T, S_, P, 3, H>), g, dim3(kBlock), 0, s, A, tiles_x, tiles);
+            if (sel.mode == SEL_MAPS) {
+                // algo 4 with integer output: the selector of every active
+                // pixel is final in the tile (its LGF maps are built before)
+                if (gp.finite) {
+                    LGCR_FUSED(false, SEL_MAPS, true, true)
+                } else {
+                    LGCR_FUSED(false, SEL_MAPS, true, false)
+                }
+            } else if (recon && gp.finite) {
+                LGCR_FUSED(false, SEL_NONE, true, true)
             } else if (recon) {
-                LGCR_FUSED(false, true, false)
+                LGCR_FUSED(false, SEL_NONE, true, false)
             } else {
-                LGCR_FUSED(true, false, false)
+                LGCR_FUSED(true, SEL_NONE, false, false)
             }
 #undef LGCR_FUSED
             return hipGetLastError();

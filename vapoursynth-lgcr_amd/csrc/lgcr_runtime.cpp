@@ -1711,8 +1711,9 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
         }
         // the tiles also back-project and pack the output when every pixel's
         // result is final there (no selector, bm or detail pass after it)
-        fused_out = plain_in_tiles && !algo4 && !p.bm &&
-                    guided_tiled(g.tiles_fit, gp.radial, gp.cedge, g.ax, g.ay, SEL_NONE);
+        // (algo 4 with integer output too: its selector is final in the tile)
+        fused_out = plain_in_tiles && (!algo4 || sel.mode == SEL_MAPS) && !p.bm &&
+                    guided_tiled(g.tiles_fit, gp.radial, gp.cedge, g.ax, g.ay, sel.mode);
         const RawOut fo[2] = {
             {nullptr, out[1], out_stride[1], c->ow, c->oh, float(oscale), out_offset, 0, hi, io, 1},
             {nullptr, out[2], out_stride[2], c->ow, c->oh, float(oscale), out_offset, 0, hi, io, 2},
@@ -1721,7 +1722,7 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
                               gate, g.ax, g.ay, g.rd, gp, w->row_off, idx, w->out_u, w->out_v, sel,
                               TemporalDev{}, s, w->mask, c->sw, c->sh, g.tiles_fit, plain_in_tiles,
                               fused_out ? fo : nullptr, c->raw_type, float(p.bp), bp_same ? 1 : 0,
-                              fused_out && !timed && !io ? defer : nullptr));
+                              fused_out && !timed && !io && !algo4 ? defer : nullptr));
         HIP_TRY(mark(LGCR_STAGE_APPLY_GUIDED_CORRECTION));
         if (algo4 && is_float) {
             HIP_TRY(launch_roi_mark(w->row_off, idx, c->ow, c->oh, sel, g.ax, g.ay, w->roi, c->cw, c->ch,
