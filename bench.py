@@ -316,12 +316,15 @@ def workloads(args, only=None):
 
 
 def batched_launch(args):
-    """Multi-frame fused tile launches (lgcr_clip_set_batching) against one
-    launch per frame, on device-resident frames: the 640x360 YUV420P8
-    workload with Recon defaults and the 1080p P10 benchmark workload.
-    Per frame: eight and sixteen workspaces; batched: sixteen workspaces,
-    groups of four frames on four streams, as one wave per tile over a
-    (tiles, frames) grid and as a persistent grid."""
+    """Multi-frame launches (lgcr_clip_set_batching: every stage as one
+    launch per group of frames) against one launch per frame, on
+    device-resident frames: the 640x360 YUV420P8 workload with Recon
+    defaults and the 1080p P10 benchmark workload.  Per frame: eight and
+    sixteen workspaces, and sixteen workspaces replaying one captured graph
+    per frame; batched: sixteen workspaces, groups of four frames on four
+    streams, the tile pass as one wave per tile over a (tiles, frames) grid
+    and as a persistent grid.  Each mode runs one warm-up and three timed
+    passes over the frames."""
     import numpy as np
     import torch
     from lgcr_amd import abi, synth
@@ -332,6 +335,8 @@ def batched_launch(args):
              ("1080p420p10", "Recon 1920x1080 YUV420P10 -> YUV444P10, algo 2, lanczos3, strength 0.8, bp 0.5",
               1920, 1080, 10, {"bp": 0.5}, 96))
     for name, desc, W, H, bits, kw, n in cases:
+        if args.batched_case and name != args.batched_case:
+            continue
         bps = 1 if bits <= 8 else 2
         np_dt = np.uint8 if bps == 1 else np.uint16
         tdt = torch.uint8 if bps == 1 else torch.int16
@@ -343,13 +348,18 @@ def batched_launch(args):
         fmt = abi.make_format(np_dt, W, H, bits=None if bits == 8 else bits)
         params = abi.default_params(**kw)
         res = {"workload": desc, "frames": n}
-        for tag, inflight, batching, persistent in (("per_frame_8", 8, 0, False), ("per_frame_16", 16, 0, False),
-                                                   ("batched_4x4_grid", 16, 4, False),
-                                                   ("batched_4x4_persistent", 16, 4, True)):
+        for tag, inflight, batching, persistent, graphs in (("per_frame_8", 8, 0, False, False),
+                                                           ("per_frame_16", 16, 0, False, False),
+                                                           ("graphs_16", 16, 0, False, True),
+                                                           ("batched_4x4_grid", 16, 4, False, False),
+                                                           ("batched_4x4_persistent", 16, 4, True, False)):
+            if args.batched_mode and tag != args.batched_mode:
+                continue
             os.environ["LGCR_BATCH_PERSISTENT"] = "1" if persistent else "0"
             clip = abi.Clip(dev, params, fmt, max_inflight=inflight)
             os.environ.pop("LGCR_BATCH_PERSISTENT")
             clip.set_batching(batching)
+            clip.set_graphs(graphs)
             clip.recon_batch(fin, fout)
             torch.cuda.synchronize()
             steps = 3
@@ -425,6 +435,10 @@ def main():
     ap.add_argument("--no-downsample", action="store_true")
     ap.add_argument("--no-trecon", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-case", default=None, help="multi-frame section: one workload (640x360_420p8, 1080p420p10)")
+    ap.add_argument("--batched-mode", default=None,
+                    help="multi-frame section: one launch mode (per_frame_8, per_frame_16, graphs_16, "
+                         "batched_4x4_grid, batched_4x4_persistent)")
     ap.add_argument("--serve-poll-us", type=float, default=20.0)
     ap.add_argument("--serve-inflight", type=int, default=4)
     ap.add_argument("--serve-graphs", action="store_true")

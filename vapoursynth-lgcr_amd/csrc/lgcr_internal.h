@@ -207,17 +207,28 @@ struct DsArgs {
 // baseline kernel: kDsBaseTileW x kDsBaseTileH outputs per block; the
 // tile's source columns (start of its first output to start of its last +
 // support) and source rows must span at most kDsBaseSegW and kDsBaseSegH
+// Group launches of the stages before the tile pass: one launch serves up
+// to kGroupMax frames (blockIdx.z = frame).  The frames' workspaces share
+// one layout, so a buffer of frame k lies d[k] bytes from the same buffer of
+// frame 0 (d[0] = 0).
+constexpr int kGroupMax = 8;
+struct GroupOff {
+    long long d[kGroupMax];
+};
+
 constexpr int kDsBaseTileW = 32, kDsBaseTileH = 8, kDsBaseSegW = 80, kDsBaseSegH = 32;
 
 // Kernel launchers (lgcr_kernels.hip).  All are asynchronous on `s`.
 hipError_t launch_check_rcp(uint32_t lo, uint32_t hi, unsigned long long *out, hipStream_t s);
 hipError_t launch_to_float(const RawPlane *planes, int count, int type, hipStream_t s);
+// the conversion of up to kGroupMax frames (planes 3k .. 3k + 2: frame k)
+hipError_t launch_to_float_group(const RawPlane *planes, int count, int type, hipStream_t s);
 hipError_t launch_from_float(const RawOut *planes, int count, int type, hipStream_t s);
 // structure tensor written interleaved with the luma: yt = {Y, jxx, jxy, jyy}
 // seed: one bit per pixel, rows of (w + 63) / 64 words
 // tile_rows: 4, 8 or 16 (tile 64 x tile_rows)
 hipError_t launch_sobel_tensor(const float *y, int w, int h, float4 *yt, uint64_t *seed, float threshold,
-                               hipStream_t s, int tile_rows = 8);
+                               hipStream_t s, int tile_rows = 8, const GroupOff *go = nullptr, int nf = 1);
 // trust mask (dilation of the seed bits by radius) and the sparse workset:
 // per-output-row counts, row offsets, row-major indices, chroma mask and
 // its population in counters[0]
@@ -229,12 +240,12 @@ hipError_t launch_trust_workset(const uint64_t *seed, uint64_t *tmp, uint64_t *m
 hipError_t launch_lc_map(const float *y, int w, int h, float *lc, int cw, int ch,
                          const int *xb, const int *xe, const int *yb, const int *ye,
                          hipStream_t s, const float *u = nullptr, const float *v = nullptr,
-                         float4 *luv = nullptr);
+                         float4 *luv = nullptr, const GroupOff *go = nullptr, int nf = 1);
 // count / ncount: the marked chroma samples are count[0] + ... + count[ncount - 1]
 // tile_cols: 16, 32 or 64 (tiles 16x16, 32x8, 64x4)
 hipError_t launch_mutual_gate(const float *lc, const float *u, const float *v, int w, int h,
                               const uint8_t *chroma_mask, const uint32_t *count, int ncount, float *gate,
-                              hipStream_t s, int tile_cols = 32);
+                              hipStream_t s, int tile_cols = 32, const GroupOff *go = nullptr, int nf = 1);
 hipError_t launch_plain_separable(const float *u, const float *v, int cw, int ch,
                                   const AxisDev &ax, const AxisDev &ay, float *hbuf,
                                   float *out_u, float *out_v, int clamp_hull, float margin,
@@ -311,7 +322,8 @@ constexpr int kCounterWords = kCntChromaWords + 2 * kMaxBands;
 size_t trust_fused_lds(int w, int radius, int rows, int cw);
 hipError_t launch_trust_mask_tiles(const uint64_t *seed, uint64_t *mask, int w, int h, int radius, const int4 *band,
                                    int nbands, int cband, int max_rows, uint8_t *chroma_mask, int cw, int ch,
-                                   const int2 *xr, const int2 *yr, uint32_t *counters, hipStream_t s);
+                                   const int2 *xr, const int2 *yr, uint32_t *counters, hipStream_t s,
+                                   const GroupOff *go = nullptr, int nf = 1);
 // TRecon motion search and chroma consistency (algos.cpp:683-807)
 hipError_t launch_block_match(const float *cur, const float *nbr, int w, int h, int block,
                               int search, float tsad, int16_t *mvx, int16_t *mvy, float *conf,

@@ -216,12 +216,9 @@ __global__ void k_io_advance(IoTable *io) {
 // four at a time so several loads are in flight per lane.
 constexpr int kConvUnroll = 4;
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_to_float(InBatch b) {
-    const RawPlane &P = b.p[blockIdx.z];
-    const int y = blockIdx.y;
-    if (y >= P.h)
-        return;
+// One row of one plane.  COPY: float input, copied bit for bit.
+template <typename T, bool COPY>
+__device__ __forceinline__ void to_float_row(const RawPlane &P, int y) {
     ptrdiff_t stride;
     const char *base = plane_in(P, stride);
     const T *row = reinterpret_cast<const T *>(base + stride * y);
@@ -238,34 +235,50 @@ __global__ __launch_bounds__(kBlock) void k_to_float(InBatch b) {
         for (int i = 0; i < kConvUnroll; ++i) {
             const int x = x0 + i * kBlock;
             if (x < P.w)
-                dst[x] = (float)((double)v[i] * scale + offset);
+                dst[x] = COPY ? (float)v[i] : (float)((double)v[i] * scale + offset);
         }
     }
 }
 
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_to_float(InBatch b) {
+    const RawPlane &P = b.p[blockIdx.z];
+    if ((int)blockIdx.y < P.h)
+        to_float_row<T, false>(P, blockIdx.y);
+}
+
 __global__ __launch_bounds__(kBlock) void k_copy_f32(InBatch b) {
     const RawPlane &P = b.p[blockIdx.z];
-    const int y = blockIdx.y;
-    if (y >= P.h)
-        return;
-    ptrdiff_t stride;
-    const char *base = plane_in(P, stride);
-    const float *row = reinterpret_cast<const float *>(base + stride * y);
-    float *dst = P.dst + (size_t)y * P.w;
-    for (int x0 = threadIdx.x; x0 < P.w; x0 += kBlock * kConvUnroll) {
-        float v[kConvUnroll];
-#pragma unroll
-        for (int i = 0; i < kConvUnroll; ++i) {
-            const int x = x0 + i * kBlock;
-            v[i] = x < P.w ? row[x] : 0.0f;
-        }
-#pragma unroll
-        for (int i = 0; i < kConvUnroll; ++i) {
-            const int x = x0 + i * kBlock;
-            if (x < P.w)
-                dst[x] = v[i];
-        }
-    }
+    if ((int)blockIdx.y < P.h)
+        to_float_row<float, true>(P, blockIdx.y);
+}
+
+// The conversion of a group of frames in one launch: planes 3k .. 3k + 2
+// are frame k's Y, U and V.
+struct InBatchG { RawPlane p[3 * kGroupMax]; };
+
+template <typename T, bool COPY>
+__global__ __launch_bounds__(kBlock) void k_to_float_group(InBatchG b) {
+    const RawPlane &P = b.p[blockIdx.z];
+    if ((int)blockIdx.y < P.h)
+        to_float_row<T, COPY>(P, blockIdx.y);
+}
+
+// Group launches of the per-frame stages: blockIdx.z is the frame of the
+// group, and every frame's workspace has the same layout, so each buffer of
+// frame z lies go.d[z] bytes from the same buffer of frame 0 (d[0] = 0; a
+// frame-by-frame launch passes zeros and one grid layer).  Null pointers
+// stay null.
+__device__ __forceinline__ long long group_off(const GroupOff &go) {
+    const unsigned z = blockIdx.z;
+    long long d = 0;
+    _Pragma("unroll") for (int i = 1; i < kGroupMax; ++i)
+        d = z == (unsigned)i ? go.d[i] : d;
+    return d;
+}
+template <class P>
+__device__ __forceinline__ P *gshift(P *p, long long d) {
+    return p ? (P *)((uintptr_t)p + (uintptr_t)d) : p;
 }
 
 template <typename T> __device__ __forceinline__ T pack_sample(float s, const RawOut &P) {
@@ -505,7 +518,11 @@ __device__ __forceinline__ void sobel_strip(const float *__restrict__ y, int w, 
 
 template <int TH>
 __global__ __launch_bounds__(kSobelW) void k_sobel_tensor(const float *__restrict__ y, int w, int h, float4 *yt,
-                                                         uint64_t *seed, int w64, float thr) {
+                                                         uint64_t *seed, int w64, float thr, GroupOff go) {
+    const long long gd = group_off(go);
+    y = gshift(y, gd);
+    yt = gshift(yt, gd);
+    seed = gshift(seed, gd);
     unsigned bx, by;
     xcd_block2(bx, by);
     const int x0 = (int)bx * kSobelW, y0 = (int)by * TH;
@@ -524,7 +541,14 @@ __global__ __launch_bounds__(kSobelW) void k_sobel_tensor(const float *__restric
 
 __global__ __launch_bounds__(kBlock) void k_lc_map(const float *__restrict__ y, int w, float *lc, int cw,
                                                    int ch, const int *xb, const int *xe, const int *yb,
-                                                   const int *ye, const float *u, const float *v, float4 *luv) {
+                                                   const int *ye, const float *u, const float *v, float4 *luv,
+                                                   GroupOff go) {
+    const long long gd = group_off(go);
+    y = gshift(y, gd);
+    lc = gshift(lc, gd);
+    u = gshift(u, gd);
+    v = gshift(v, gd);
+    luv = gshift(luv, gd);
     const int cx = blockIdx.x * kBlock + threadIdx.x, cy = blockIdx.y;
     if (cx >= cw || cy >= ch)
         return;
@@ -640,7 +664,12 @@ __global__ __launch_bounds__(64) void k_dilate_vbits(const uint64_t *in, uint64_
 __global__ __launch_bounds__(kBlock) void k_trust_fused(const uint64_t *seed, uint64_t *mask, int w, int w64, int h,
                                                         int r, const int4 *band, int cband, int cw, int ch,
                                                         const int2 *xr, const int2 *yr, uint8_t *cmask,
-                                                        uint32_t *counters) {
+                                                        uint32_t *counters, GroupOff go) {
+    const long long gd = group_off(go);
+    seed = gshift(seed, gd);
+    mask = gshift(mask, gd);
+    cmask = gshift(cmask, gd);
+    counters = gshift(counters, gd);
     extern __shared__ uint64_t tf_lds[];
     __shared__ int scratch[kBlock / 64];
     __shared__ int2 ys_band[kMaxCband];
@@ -937,8 +966,15 @@ template <int kGateTW, int kGateTH>
 __global__ __launch_bounds__(kBlock) void k_gate_fused(const float *__restrict__ lc, const float *__restrict__ u,
                                                        const float *__restrict__ v, int w, int h,
                                                        const uint8_t *cmask, const uint32_t *count, int ncount,
-                                                       float *gate) {
+                                                       float *gate, GroupOff go) {
     static_assert(kGateTW * kGateTH == kBlock, "one thread per tile sample");
+    const long long gd = group_off(go);
+    lc = gshift(lc, gd);
+    u = gshift(u, gd);
+    v = gshift(v, gd);
+    cmask = gshift(cmask, gd);
+    count = gshift(count, gd);
+    gate = gshift(gate, gd);
     constexpr int kGateGW = kGateTW + 2 * kGateHalo, kGateGH = kGateTH + 2 * kGateHalo;
     constexpr int kGateSW = kGateGW + 2, kGateSH = kGateGH + 2;
     __shared__ float src[3][kGateSH][kGateSW];
@@ -4484,6 +4520,25 @@ hipError_t launch_to_float(const RawPlane *planes, int count, int type, hipStrea
     return hipGetLastError();
 }
 
+hipError_t launch_to_float_group(const RawPlane *planes, int count, int type, hipStream_t s) {
+    if (count < 1 || count > 3 * kGroupMax)
+        return hipErrorInvalidValue;
+    InBatchG b{};
+    int mh = 0;
+    for (int i = 0; i < count; ++i) {
+        b.p[i] = planes[i];
+        mh = max(mh, planes[i].h);
+    }
+    const dim3 g(1, (unsigned)mh, (unsigned)count);
+    if (type == RAW_U8)
+        hipLaunchKernelGGL((k_to_float_group<uint8_t, false>), g, dim3(kBlock), 0, s, b);
+    else if (type == RAW_U16)
+        hipLaunchKernelGGL((k_to_float_group<uint16_t, false>), g, dim3(kBlock), 0, s, b);
+    else
+        hipLaunchKernelGGL((k_to_float_group<float, true>), g, dim3(kBlock), 0, s, b);
+    return hipGetLastError();
+}
+
 hipError_t launch_from_float(const RawOut *planes, int count, int type, hipStream_t s) {
     OutBatch b{};
     int mw = 0, mh = 0;
@@ -4503,11 +4558,12 @@ hipError_t launch_from_float(const RawOut *planes, int count, int type, hipStrea
 }
 
 hipError_t launch_sobel_tensor(const float *y, int w, int h, float4 *yt, uint64_t *seed, float threshold,
-                               hipStream_t s, int tile_rows) {
+                               hipStream_t s, int tile_rows, const GroupOff *go, int nf) {
+    const GroupOff g0 = go ? *go : GroupOff{};
 #define LGCR_SOBEL(TH_)                                                                                        \
     hipLaunchKernelGGL(k_sobel_tensor<TH_>,                                                                     \
-                       dim3((unsigned)((w + kSobelW - 1) / kSobelW), (unsigned)((h + TH_ - 1) / TH_)),          \
-                       dim3(kSobelW), 0, s, y, w, h, yt, seed, (w + 63) / 64, threshold)
+                       dim3((unsigned)((w + kSobelW - 1) / kSobelW), (unsigned)((h + TH_ - 1) / TH_), (unsigned)nf), \
+                       dim3(kSobelW), 0, s, y, w, h, yt, seed, (w + 63) / 64, threshold, g0)
     if (tile_rows == 8)
         LGCR_SOBEL(8);
     else if (tile_rows == 32)
@@ -4520,10 +4576,10 @@ hipError_t launch_sobel_tensor(const float *y, int w, int h, float4 *yt, uint64_
 
 hipError_t launch_lc_map(const float *y, int w, int h, float *lc, int cw, int ch, const int *xb,
                          const int *xe, const int *yb, const int *ye, hipStream_t s, const float *u, const float *v,
-                         float4 *luv) {
+                         float4 *luv, const GroupOff *go, int nf) {
     (void)h;
-    hipLaunchKernelGGL(k_lc_map, rows_grid(cw, ch), dim3(kBlock), 0, s, y, w, lc, cw, ch, xb, xe, yb, ye, u, v,
-                       luv);
+    hipLaunchKernelGGL(k_lc_map, rows_grid(cw, ch, nf), dim3(kBlock), 0, s, y, w, lc, cw, ch, xb, xe, yb, ye, u, v,
+                       luv, go ? *go : GroupOff{});
     return hipGetLastError();
 }
 
@@ -4555,10 +4611,12 @@ size_t trust_fused_lds(int w, int radius, int rows, int cw) {
 
 hipError_t launch_trust_mask_tiles(const uint64_t *seed, uint64_t *mask, int w, int h, int radius, const int4 *band,
                                    int nbands, int cband, int max_rows, uint8_t *chroma_mask, int cw, int ch,
-                                   const int2 *xr, const int2 *yr, uint32_t *counters, hipStream_t s) {
+                                   const int2 *xr, const int2 *yr, uint32_t *counters, hipStream_t s,
+                                   const GroupOff *go, int nf) {
     const int w64 = (w + 63) / 64;
-    hipLaunchKernelGGL(k_trust_fused, dim3((unsigned)nbands), dim3(kBlock), trust_fused_lds(w, radius, max_rows, cw), s,
-                       seed, mask, w, w64, h, radius, band, cband, cw, ch, xr, yr, chroma_mask, counters);
+    hipLaunchKernelGGL(k_trust_fused, dim3((unsigned)nbands, 1, (unsigned)nf), dim3(kBlock),
+                       trust_fused_lds(w, radius, max_rows, cw), s, seed, mask, w, w64, h, radius, band, cband, cw,
+                       ch, xr, yr, chroma_mask, counters, go ? *go : GroupOff{});
     return hipGetLastError();
 }
 
@@ -4568,10 +4626,12 @@ bool guided_tiled(int tiles_fit, int radial, int cedge, const AxisDev &ax, const
 
 hipError_t launch_mutual_gate(const float *lc, const float *u, const float *v, int w, int h,
                               const uint8_t *chroma_mask, const uint32_t *count, int ncount, float *gate,
-                              hipStream_t s, int tile_cols) {
+                              hipStream_t s, int tile_cols, const GroupOff *go, int nf) {
+    const GroupOff g0 = go ? *go : GroupOff{};
 #define LGCR_GATE(TW_, TH_)                                                                                        \
-    hipLaunchKernelGGL((k_gate_fused<TW_, TH_>), dim3((unsigned)((w + TW_ - 1) / TW_), (unsigned)((h + TH_ - 1) / TH_)), \
-                       dim3(kBlock), 0, s, lc, u, v, w, h, chroma_mask, count, ncount, gate)
+    hipLaunchKernelGGL((k_gate_fused<TW_, TH_>),                                                               \
+                       dim3((unsigned)((w + TW_ - 1) / TW_), (unsigned)((h + TH_ - 1) / TH_), (unsigned)nf),   \
+                       dim3(kBlock), 0, s, lc, u, v, w, h, chroma_mask, count, ncount, gate, g0)
     if (tile_cols == 64)
         LGCR_GATE(64, 4);
     else if (tile_cols == 16)
@@ -4771,8 +4831,10 @@ hipError_t launch_guided_batch(const GuidedDeferred *items, int n, void *dev, vo
     if (n > kBatchMaxFrames)
         return hipErrorInvalidValue;
     const int variant = items[0].variant;
-    BatchFrame *hf = static_cast<BatchFrame *>(host);
-    uint32_t *hfirst = reinterpret_cast<uint32_t *>(hf + kBatchMaxFrames);
+    // table layout: first[kBatchMaxFrames], then the frames (one upload)
+    static_assert(sizeof(uint32_t) * kBatchMaxFrames % alignof(BatchFrame) == 0, "frames after first[] aligned");
+    uint32_t *hfirst = static_cast<uint32_t *>(host);
+    BatchFrame *hf = reinterpret_cast<BatchFrame *>(hfirst + kBatchMaxFrames);
     uint32_t total = 0;
     int max_tiles = 0;
     for (int k = 0; k < n; ++k) {
@@ -4783,11 +4845,10 @@ hipError_t launch_guided_batch(const GuidedDeferred *items, int n, void *dev, vo
         hfirst[k] = total;
         total += (uint32_t)items[k].tiles;
     }
-    BatchFrame *df = static_cast<BatchFrame *>(dev);
-    uint32_t *dfirst = reinterpret_cast<uint32_t *>(df + kBatchMaxFrames);
-    hipError_t e = hipMemcpyAsync(df, hf, sizeof(BatchFrame) * n, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(dfirst, hfirst, sizeof(uint32_t) * n, hipMemcpyHostToDevice, s);
+    uint32_t *dfirst = static_cast<uint32_t *>(dev);
+    BatchFrame *df = reinterpret_cast<BatchFrame *>(dfirst + kBatchMaxFrames);
+    const hipError_t e = hipMemcpyAsync(dfirst, hfirst, sizeof(uint32_t) * kBatchMaxFrames + sizeof(BatchFrame) * n,
+                                        hipMemcpyHostToDevice, s);
     if (e != hipSuccess)
         return e;
 #define LGCR_BATCH(T, P, O, H) \

@@ -1547,10 +1547,61 @@ hipError_t trust_workset(lgcr_clip *c, Workspace *w, const Geometry &g, bool til
                                 w->cmask, c->cw, c->ch, g.ax, g.ay, w->counters, s);
 }
 
+// The three input planes of a Recon frame and their conversion to float
+// (planeToFloat, lgcr.h:986-1013) into the workspace.
+void input_planes(const lgcr_clip *c, Workspace *w, const void *const in[3], const ptrdiff_t in_stride[3],
+                  IoTable *io, RawPlane rp[3]) {
+    const bool is_float = c->f.sample_type == 1;
+    const double yscale = is_float ? 1.0 : 1.0 / double((1 << c->f.bits_per_sample) - 1);
+    const double coff = is_float ? 0.0 : -0.5;
+    rp[0] = {in[0], in_stride[0], c->sw, c->sh, w->y, yscale, 0.0, io, 0};
+    rp[1] = {in[1], in_stride[1], c->cw, c->ch, w->u, yscale, coff, io, 1};
+    rp[2] = {in[2], in_stride[2], c->cw, c->ch, w->v, yscale, coff, io, 2};
+}
+
+// Whether a frame of the clip takes the path whose stages before the tile
+// pass can run as group launches (enqueue_group_pre): output at the input
+// size, guided algo 2 with the sparse trust mask built by the fused band
+// kernel, no bm, and the fused tile pass writing the output.
+bool group_pre_ok(const lgcr_clip *c, const Geometry &g) {
+    const lgcr_params &p = c->p;
+    return c->same && p.strength > 0.0 && p.algo == 2 && p.sparse && !p.bm && !c->capture && g.tb_ok &&
+           c->dil < 64 && guided_tiled(g.tiles_fit, p.radial, p.cedge, g.ax, g.ay, SEL_NONE);
+}
+
+// ConvertInput, BuildGuideMaps, BuildTrustMask and BuildMutualGate
+// (plugin.cpp:210-365) of `cnt` frames (at most kGroupMax) as one launch
+// per stage on `s`, in the order and with the arguments the frame-by-frame
+// path uses; the frames then continue with enqueue_frame(pre_done).
+lgcr_status enqueue_group_pre(lgcr_clip *c, const Geometry &g, Workspace *const *ws, const lgcr_frame *in, int cnt,
+                              hipStream_t s) {
+    INJECT_LAUNCH_FAULT();
+    const lgcr_params &p = c->p;
+    GroupOff go{};
+    RawPlane rp[3 * kGroupMax];
+    for (int i = 0; i < cnt; ++i) {
+        go.d[i] = (long long)(reinterpret_cast<uintptr_t>(ws[i]->base) - reinterpret_cast<uintptr_t>(ws[0]->base));
+        input_planes(c, ws[i], in[i].data, in[i].stride, nullptr, rp + 3 * i);
+    }
+    HIP_TRY(launch_to_float_group(rp, 3 * cnt, c->raw_type, s));
+    Workspace *w = ws[0];
+    const double thr = p.sigma;
+    HIP_TRY(launch_sobel_tensor(w->y, c->sw, c->sh, w->yt, w->seed, float(thr * thr), s, c->tune.tensor_tile_rows,
+                                &go, cnt));
+    HIP_TRY(launch_lc_map(w->y, c->sw, c->sh, w->lc, c->cw, c->ch, g.xb, g.xe, g.yb, g.ye, s, w->u, w->v, w->luv,
+                          &go, cnt));
+    HIP_TRY(launch_trust_mask_tiles(w->seed, w->mask, c->sw, c->sh, c->dil, g.tb, g.tb_n, g.tb_c, g.tb_rows,
+                                    w->cmask, c->cw, c->ch, g.cm_x, g.cm_y, w->counters, s, &go, cnt));
+    if (p.ms > 0.0)
+        HIP_TRY(launch_mutual_gate(w->lc, w->u, w->v, c->cw, c->ch, w->cmask, w->counters + kCntChromaWords,
+                                   g.tb_n, w->ms, s, c->tune.gate_tile_cols, &go, cnt));
+    return LGCR_OK;
+}
+
 lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const void *const in[3],
                           const ptrdiff_t in_stride[3], void *const out[3], const ptrdiff_t out_stride[3],
                           bool timed, IoTable *io = nullptr, GuidedDeferred *defer = nullptr,
-                          hipStream_t stream = nullptr) {
+                          hipStream_t stream = nullptr, bool pre_done = false) {
     INJECT_LAUNCH_FAULT();
     const lgcr_params &p = c->p;
     hipStream_t s = stream ? stream : w->stream;
@@ -1570,18 +1621,15 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
         return hipEventRecord(ev, s);
     };
     const bool is_float = c->f.sample_type == 1;
-    const double yscale = is_float ? 1.0 : 1.0 / double((1 << c->f.bits_per_sample) - 1);
-    const double coff = is_float ? 0.0 : -0.5;
 
     HIP_TRY(mark(-1));
-    // ConvertInput (plugin.cpp:210-253)
-    RawPlane rp[3] = {
-        {in[0], in_stride[0], c->sw, c->sh, w->y, yscale, 0.0, io, 0},
-        {in[1], in_stride[1], c->cw, c->ch, w->u, yscale, coff, io, 1},
-        {in[2], in_stride[2], c->cw, c->ch, w->v, yscale, coff, io, 2},
-    };
-    HIP_TRY(launch_to_float(rp, 3, c->raw_type, s));
-    HIP_TRY(mark(LGCR_STAGE_CONVERT_INPUT));
+    if (!pre_done) {
+        // ConvertInput (plugin.cpp:210-253)
+        RawPlane rp[3];
+        input_planes(c, w, in, in_stride, io, rp);
+        HIP_TRY(launch_to_float(rp, 3, c->raw_type, s));
+        HIP_TRY(mark(LGCR_STAGE_CONVERT_INPUT));
+    }
 
     const double oscale = is_float ? 1.0 : double((1 << c->f.bits_per_sample) - 1);
     const int hi = is_float ? 0 : (1 << c->f.bits_per_sample) - 1;
@@ -1611,7 +1659,12 @@ lgcr_status enqueue_frame(lgcr_clip *c, Workspace *w, const Geometry &g, const v
     const bool algo4 = p.algo == 4 && guided;
     const int sel_mode = algo4 ? (is_float ? SEL_FLOAT : SEL_MAPS) : SEL_NONE;
     const bool tiled = guided && p.algo != 6 && guided_tiled(g.tiles_fit, p.radial, p.cedge, g.ax, g.ay, sel_mode);
-    if (guided) {
+    if (pre_done) {
+        // the group launches built the maps, the trust mask by the band
+        // kernel (no index list) and the gate (enqueue_group_pre)
+        w->has_ws = true;
+        w->ws_indices = false;
+    } else if (guided) {
         // BuildGuideMaps (plugin.cpp:324-334, maps.cpp:252-276)
         const double thr = p.algo == 4 ? 0.25 * p.sigma : p.sigma;
         HIP_TRY(launch_sobel_tensor(w->y, c->sw, c->sh, w->yt, p.sparse ? w->seed : nullptr,
@@ -2624,7 +2677,9 @@ namespace {
 // lanes, each a stream with its own workspaces and table slots: a group's
 // frames run their stages up to the mutual gate one after another on the
 // lane's stream, then ONE launch computes the fused tile pass (plain base,
-// guided correction, back-projection and packing) of the whole group.  A
+// guided correction, back-projection and packing) of the whole group.  On
+// the fused tile path (group_pre_ok) the earlier stages are group launches
+// as well (one launch per stage for up to kGroupMax frames).  A
 // lane's work is ordered on its own stream (no waits between streams, which
 // would hold up the hardware queues the streams share), and the lanes run
 // concurrently, so one lane's early stages overlap another lane's tile
@@ -2645,13 +2700,21 @@ lgcr_status recon_batch_fused(lgcr_clip *c, const Geometry &g, const std::vector
         if (slot.used)
             HIP_TRY(hipEventSynchronize(slot.done));  // its staging is rewritten below
         int nd = 0;
+        Workspace *gw[kBatchMaxFrames];
+        for (int i = 0; i < cnt; ++i)
+            gw[i] = c->spaces[slots[lane * G + i]].get();
+        // the stages before the tile pass as group launches too, when the
+        // frames take the fused tile path
+        const bool gpre = group_pre_ok(c, g);
+        for (int i0 = 0; gpre && i0 < cnt && result == LGCR_OK; i0 += kGroupMax)
+            result = enqueue_group_pre(c, g, gw + i0, in + k0 + i0, std::min(kGroupMax, cnt - i0), s);
         for (int i = 0; i < cnt && result == LGCR_OK; ++i) {
             const int k = k0 + i;
-            Workspace *w = c->spaces[slots[lane * G + i]].get();
+            Workspace *w = gw[i];
             items[nd].tiles = 0;
             void *dout[3] = {out[k].data[0], out[k].data[1], out[k].data[2]};
             result = enqueue_frame(c, w, g, in[k].data, in[k].stride, dout, out[k].stride, false, nullptr,
-                                   &items[nd], s);
+                                   &items[nd], s, gpre);
             if (result == LGCR_OK && c->same && dout[0] && dout[0] != in[k].data[0])
                 HIP_TRY(hipMemcpy2DAsync(dout[0], out[k].stride[0], in[k].data[0], in[k].stride[0],
                                          size_t(c->sw) * c->bps, c->sh, hipMemcpyDeviceToDevice, s));
