@@ -276,9 +276,16 @@ __device__ __forceinline__ long long group_off(const GroupOff &go) {
         d = z == (unsigned)i ? go.d[i] : d;
     return d;
 }
+// byte arithmetic on the pointer itself (no round trip through an integer,
+// which would hide that it addresses global memory and turn its loads and
+// stores into flat ones)
+template <class P>
+__device__ __forceinline__ const P *gshift(const P *p, long long d) {
+    return p ? reinterpret_cast<const P *>(reinterpret_cast<const char *>(p) + d) : p;
+}
 template <class P>
 __device__ __forceinline__ P *gshift(P *p, long long d) {
-    return p ? (P *)((uintptr_t)p + (uintptr_t)d) : p;
+    return p ? reinterpret_cast<P *>(reinterpret_cast<char *>(p) + d) : p;
 }
 
 template <typename T> __device__ __forceinline__ T pack_sample(float s, const RawOut &P) {
