@@ -287,6 +287,12 @@ template <class P>
 __device__ __forceinline__ P *gshift(P *p, long long d) {
     return p ? reinterpret_cast<P *>(reinterpret_cast<char *>(p) + d) : p;
 }
+// A device pointer read from memory (a table entry) as a global-memory
+// pointer, so its loads are global rather than flat ones.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *gmem(const T *p) {
+    return (const __attribute__((address_space(1))) T *)p;
+}
 
 template <typename T> __device__ __forceinline__ T pack_sample(float s, const RawOut &P) {
     const int v = (int)(s * P.scale + P.offset + 0.5f);
@@ -2158,10 +2164,10 @@ __device__ __attribute__((always_inline)) void guided_pixel(const GuidedArgs &A,
             const NbrDev nb = T.nbrs[q];
             const int bxx = min(nb.bw - 1, lxi / nb.block), byy = min(nb.bh - 1, lyi / nb.block);
             const size_t bi = (size_t)byy * nb.bw + bxx;
-            const float tconf = nb.conf[bi];
+            const float tconf = gmem(nb.conf)[bi];
             if (tconf < 0.05f)
                 continue;
-            const int ivx = nb.mvx[bi], ivy = nb.mvy[bi];
+            const int ivx = gmem(nb.mvx)[bi], ivy = gmem(nb.mvy)[bi];
             const float mx = (float)ivx, my = (float)ivy;
             // target sample, residual and kernel weight per (output column,
             // vector x) and (output row, vector y), tabulated per clip
@@ -2174,7 +2180,7 @@ __device__ __attribute__((always_inline)) void guided_pixel(const GuidedArgs &A,
             const bool stat = (fabsf(mx) + fabsf(my) <= 1.0f) && tconf > 0.5f;
             float dL;
             if (stat) {
-                dL = nb.lc[(size_t)tny * cw + tnx] - L0;
+                dL = gmem(nb.lc)[(size_t)tny * cw + tnx] - L0;
             } else {
                 const double lx = (tnx + 0.5) * T.rw - 0.5 + T.sx - mx;
                 const double ly = (tny + 0.5) * T.rh - 0.5 + T.sy - my;
@@ -2203,7 +2209,7 @@ __device__ __attribute__((always_inline)) void guided_pixel(const GuidedArgs &A,
             }
             const float w = tconf * sim * kw * nw;
             const size_t tk = (size_t)tny * cw + tnx;
-            const float nu = nb.u[tk], nv = nb.v[tk];
+            const float nu = gmem(nb.u)[tk], nv = gmem(nb.v)[tk];
             acc_u = dacc(acc_u, w, nu);
             acc_v = dacc(acc_v, w, nv);
             wsum += w;
