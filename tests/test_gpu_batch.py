@@ -2,7 +2,9 @@
 tile pass (plain base, guided correction, back-projection, packing) of a
 group of frames in ONE launch (one wave per tile over a (tiles, frames)
 grid, or a persistent grid whose waves stride over the tiles of every
-frame).  Every frame must equal the frame-by-frame launches
+frame), and on that path the earlier stages (conversion, Sobel tensor,
+footprint luma, trust mask, mutual gate) as one launch per stage for up to
+eight frames.  Every frame must equal the frame-by-frame launches
 bit for bit, and the oracle; groups of 1..8 frames, a batch that does not
 fill its last group, ragged frames, every fused output type, and frames
 whose guided pass is not the fused tile pass (those complete on their own
@@ -26,6 +28,10 @@ CASES = [
     ("ps-420-g4", 160, 96, 1, 1, 32, {"bp": 0.5}, 8, 6, 4),
     ("p10-422-g2", 160, 96, 1, 0, 10, {}, 4, 4, 2),
     ("p10-420-dense-g2", 160, 96, 1, 1, 10, {"sparse": 0}, 4, 4, 2),
+    # groups past eight frames: the earlier stages go in launches of eight
+    ("p10-420-bp-g12", 160, 96, 1, 1, 10, {"bp": 0.5}, 12, 14, 12),
+    # no mutual gate: the group launches stop at the trust mask
+    ("p8-420-ms0-g3", 198, 118, 1, 1, 8, {"ms": 0.0}, 6, 7, 3),
     # not the fused tile pass: algo 4 and bm finish on their own streams
     ("p10-420-algo4-g2", 160, 96, 1, 1, 10, {"algo": 4}, 4, 3, 2),
     ("p8-420-bm-g2", 160, 96, 1, 1, 8, {"bm": 1}, 4, 3, 2),
