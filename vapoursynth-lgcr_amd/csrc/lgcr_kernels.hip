@@ -529,13 +529,15 @@ __device__ __forceinline__ void sobel_strip(const float *__restrict__ y, int w, 
     }
 }
 
-template <int TH>
+template <int TH, bool GROUP>
 __global__ __launch_bounds__(kSobelW) void k_sobel_tensor(const float *__restrict__ y, int w, int h, float4 *yt,
                                                          uint64_t *seed, int w64, float thr, GroupOff go) {
-    const long long gd = group_off(go);
-    y = gshift(y, gd);
-    yt = gshift(yt, gd);
-    seed = gshift(seed, gd);
+    if constexpr (GROUP) {
+        const long long gd = group_off(go);
+        y = gshift(y, gd);
+        yt = gshift(yt, gd);
+        seed = gshift(seed, gd);
+    }
     unsigned bx, by;
     xcd_block2(bx, by);
     const int x0 = (int)bx * kSobelW, y0 = (int)by * TH;
@@ -552,16 +554,19 @@ __global__ __launch_bounds__(kSobelW) void k_sobel_tensor(const float *__restric
 // the interleaved chroma guide {lc, U, V, 0} the guided pass gathers from
 // ---------------------------------------------------------------------------
 
+template <bool GROUP>
 __global__ __launch_bounds__(kBlock) void k_lc_map(const float *__restrict__ y, int w, float *lc, int cw,
                                                    int ch, const int *xb, const int *xe, const int *yb,
                                                    const int *ye, const float *u, const float *v, float4 *luv,
                                                    GroupOff go) {
-    const long long gd = group_off(go);
-    y = gshift(y, gd);
-    lc = gshift(lc, gd);
-    u = gshift(u, gd);
-    v = gshift(v, gd);
-    luv = gshift(luv, gd);
+    if constexpr (GROUP) {
+        const long long gd = group_off(go);
+        y = gshift(y, gd);
+        lc = gshift(lc, gd);
+        u = gshift(u, gd);
+        v = gshift(v, gd);
+        luv = gshift(luv, gd);
+    }
     const int cx = blockIdx.x * kBlock + threadIdx.x, cy = blockIdx.y;
     if (cx >= cw || cy >= ch)
         return;
@@ -674,15 +679,18 @@ __global__ __launch_bounds__(64) void k_dilate_vbits(const uint64_t *in, uint64_
 // Horizontal dilation for radius < 64: a word and its two neighbours hold
 // every bit within the radius, so the OR of the 2r+1 shifted rows is formed
 // in registers.
+template <bool GROUP>
 __global__ __launch_bounds__(kBlock) void k_trust_fused(const uint64_t *seed, uint64_t *mask, int w, int w64, int h,
                                                         int r, const int4 *band, int cband, int cw, int ch,
                                                         const int2 *xr, const int2 *yr, uint8_t *cmask,
                                                         uint32_t *counters, GroupOff go) {
-    const long long gd = group_off(go);
-    seed = gshift(seed, gd);
-    mask = gshift(mask, gd);
-    cmask = gshift(cmask, gd);
-    counters = gshift(counters, gd);
+    if constexpr (GROUP) {
+        const long long gd = group_off(go);
+        seed = gshift(seed, gd);
+        mask = gshift(mask, gd);
+        cmask = gshift(cmask, gd);
+        counters = gshift(counters, gd);
+    }
     extern __shared__ uint64_t tf_lds[];
     __shared__ int scratch[kBlock / 64];
     __shared__ int2 ys_band[kMaxCband];
@@ -975,19 +983,21 @@ __device__ __forceinline__ void gate_sample(int x, int y, int w, int h, int gx0,
     *out = (tw * tw * (3.0f - 2.0f * tw)) * (tp * tp * (3.0f - 2.0f * tp));
 }
 
-template <int kGateTW, int kGateTH>
+template <int kGateTW, int kGateTH, bool GROUP>
 __global__ __launch_bounds__(kBlock) void k_gate_fused(const float *__restrict__ lc, const float *__restrict__ u,
                                                        const float *__restrict__ v, int w, int h,
                                                        const uint8_t *cmask, const uint32_t *count, int ncount,
                                                        float *gate, GroupOff go) {
     static_assert(kGateTW * kGateTH == kBlock, "one thread per tile sample");
-    const long long gd = group_off(go);
-    lc = gshift(lc, gd);
-    u = gshift(u, gd);
-    v = gshift(v, gd);
-    cmask = gshift(cmask, gd);
-    count = gshift(count, gd);
-    gate = gshift(gate, gd);
+    if constexpr (GROUP) {
+        const long long gd = group_off(go);
+        lc = gshift(lc, gd);
+        u = gshift(u, gd);
+        v = gshift(v, gd);
+        cmask = gshift(cmask, gd);
+        count = gshift(count, gd);
+        gate = gshift(gate, gd);
+    }
     constexpr int kGateGW = kGateTW + 2 * kGateHalo, kGateGH = kGateTH + 2 * kGateHalo;
     constexpr int kGateSW = kGateGW + 2, kGateSH = kGateGH + 2;
     __shared__ float src[3][kGateSH][kGateSW];
@@ -4573,17 +4583,25 @@ hipError_t launch_from_float(const RawOut *planes, int count, int type, hipStrea
 hipError_t launch_sobel_tensor(const float *y, int w, int h, float4 *yt, uint64_t *seed, float threshold,
                                hipStream_t s, int tile_rows, const GroupOff *go, int nf) {
     const GroupOff g0 = go ? *go : GroupOff{};
-#define LGCR_SOBEL(TH_)                                                                                        \
-    hipLaunchKernelGGL(k_sobel_tensor<TH_>,                                                                     \
+#define LGCR_SOBEL_G(TH_, G_)                                                                                  \
+    hipLaunchKernelGGL((k_sobel_tensor<TH_, G_>),                                                              \
                        dim3((unsigned)((w + kSobelW - 1) / kSobelW), (unsigned)((h + TH_ - 1) / TH_), (unsigned)nf), \
                        dim3(kSobelW), 0, s, y, w, h, yt, seed, (w + 63) / 64, threshold, g0)
+#define LGCR_SOBEL(TH_)                 \
+    {                                   \
+        if (go)                         \
+            LGCR_SOBEL_G(TH_, true);    \
+        else                            \
+            LGCR_SOBEL_G(TH_, false);   \
+    }
     if (tile_rows == 8)
-        LGCR_SOBEL(8);
+        LGCR_SOBEL(8)
     else if (tile_rows == 32)
-        LGCR_SOBEL(32);
+        LGCR_SOBEL(32)
     else
-        LGCR_SOBEL(16);
+        LGCR_SOBEL(16)
 #undef LGCR_SOBEL
+#undef LGCR_SOBEL_G
     return hipGetLastError();
 }
 
@@ -4591,8 +4609,12 @@ hipError_t launch_lc_map(const float *y, int w, int h, float *lc, int cw, int ch
                          const int *xe, const int *yb, const int *ye, hipStream_t s, const float *u, const float *v,
                          float4 *luv, const GroupOff *go, int nf) {
     (void)h;
-    hipLaunchKernelGGL(k_lc_map, rows_grid(cw, ch, nf), dim3(kBlock), 0, s, y, w, lc, cw, ch, xb, xe, yb, ye, u, v,
-                       luv, go ? *go : GroupOff{});
+    if (go)
+        hipLaunchKernelGGL(k_lc_map<true>, rows_grid(cw, ch, nf), dim3(kBlock), 0, s, y, w, lc, cw, ch, xb, xe, yb,
+                           ye, u, v, luv, *go);
+    else
+        hipLaunchKernelGGL(k_lc_map<false>, rows_grid(cw, ch), dim3(kBlock), 0, s, y, w, lc, cw, ch, xb, xe, yb, ye,
+                           u, v, luv, GroupOff{});
     return hipGetLastError();
 }
 
@@ -4627,9 +4649,13 @@ hipError_t launch_trust_mask_tiles(const uint64_t *seed, uint64_t *mask, int w, 
                                    const int2 *xr, const int2 *yr, uint32_t *counters, hipStream_t s,
                                    const GroupOff *go, int nf) {
     const int w64 = (w + 63) / 64;
-    hipLaunchKernelGGL(k_trust_fused, dim3((unsigned)nbands, 1, (unsigned)nf), dim3(kBlock),
-                       trust_fused_lds(w, radius, max_rows, cw), s, seed, mask, w, w64, h, radius, band, cband, cw,
-                       ch, xr, yr, chroma_mask, counters, go ? *go : GroupOff{});
+    const size_t lds = trust_fused_lds(w, radius, max_rows, cw);
+    if (go)
+        hipLaunchKernelGGL(k_trust_fused<true>, dim3((unsigned)nbands, 1, (unsigned)nf), dim3(kBlock), lds, s, seed,
+                           mask, w, w64, h, radius, band, cband, cw, ch, xr, yr, chroma_mask, counters, *go);
+    else
+        hipLaunchKernelGGL(k_trust_fused<false>, dim3((unsigned)nbands), dim3(kBlock), lds, s, seed, mask, w, w64, h,
+                           radius, band, cband, cw, ch, xr, yr, chroma_mask, counters, GroupOff{});
     return hipGetLastError();
 }
 
@@ -4641,17 +4667,25 @@ hipError_t launch_mutual_gate(const float *lc, const float *u, const float *v, i
                               const uint8_t *chroma_mask, const uint32_t *count, int ncount, float *gate,
                               hipStream_t s, int tile_cols, const GroupOff *go, int nf) {
     const GroupOff g0 = go ? *go : GroupOff{};
-#define LGCR_GATE(TW_, TH_)                                                                                        \
-    hipLaunchKernelGGL((k_gate_fused<TW_, TH_>),                                                               \
+#define LGCR_GATE_G(TW_, TH_, G_)                                                                              \
+    hipLaunchKernelGGL((k_gate_fused<TW_, TH_, G_>),                                                           \
                        dim3((unsigned)((w + TW_ - 1) / TW_), (unsigned)((h + TH_ - 1) / TH_), (unsigned)nf),   \
                        dim3(kBlock), 0, s, lc, u, v, w, h, chroma_mask, count, ncount, gate, g0)
+#define LGCR_GATE(TW_, TH_)               \
+    {                                     \
+        if (go)                           \
+            LGCR_GATE_G(TW_, TH_, true);  \
+        else                              \
+            LGCR_GATE_G(TW_, TH_, false); \
+    }
     if (tile_cols == 64)
-        LGCR_GATE(64, 4);
+        LGCR_GATE(64, 4)
     else if (tile_cols == 16)
-        LGCR_GATE(16, 16);
+        LGCR_GATE(16, 16)
     else
-        LGCR_GATE(32, 8);
+        LGCR_GATE(32, 8)
 #undef LGCR_GATE
+#undef LGCR_GATE_G
     return hipGetLastError();
 }
 
