@@ -1968,24 +1968,37 @@ lgcr_status enqueue_trecon(lgcr_clip *c, Workspace *w, const Geometry &g, const 
     const double yscale = is_float ? 1.0 : 1.0 / double((1 << c->f.bits_per_sample) - 1);
     const double coff = is_float ? 0.0 : -0.5;
     HIP_TRY(mark(-1));
-    RawPlane rp[3] = {
-        {in[0], in_stride[0], c->sw, c->sh, w->y, yscale, 0.0},
-        {in[1], in_stride[1], c->cw, c->ch, w->u, yscale, coff},
-        {in[2], in_stride[2], c->cw, c->ch, w->v, yscale, coff},
-    };
-    HIP_TRY(launch_to_float(rp, 3, c->raw_type, s));
-    HIP_TRY(mark(LGCR_STAGE_CONVERT_INPUT));
     const bool guided = p.strength > 0.0;
     const int nn = guided ? n_nbrs : 0;
+    // the centre frame's planes, then each neighbour's: converted in one
+    // launch when they fit one (they do for up to seven neighbours)
+    RawPlane rp[3 * kGroupMax];
+    rp[0] = {in[0], in_stride[0], c->sw, c->sh, w->y, yscale, 0.0};
+    rp[1] = {in[1], in_stride[1], c->cw, c->ch, w->u, yscale, coff};
+    rp[2] = {in[2], in_stride[2], c->cw, c->ch, w->v, yscale, coff};
+    const bool one_launch = 1 + nn <= kGroupMax;
+    for (int k = 0; one_launch && k < nn; ++k) {
+        auto &q = w->nb[k];
+        rp[3 + 3 * k] = {nbr_data[3 * k], nbr_stride[3 * k], c->sw, c->sh, q.y, yscale, 0.0};
+        rp[4 + 3 * k] = {nbr_data[3 * k + 1], nbr_stride[3 * k + 1], c->cw, c->ch, q.u, yscale, coff};
+        rp[5 + 3 * k] = {nbr_data[3 * k + 2], nbr_stride[3 * k + 2], c->cw, c->ch, q.v, yscale, coff};
+    }
+    if (one_launch && nn > 0)
+        HIP_TRY(launch_to_float_group(rp, 3 * (1 + nn), c->raw_type, s));
+    else
+        HIP_TRY(launch_to_float(rp, 3, c->raw_type, s));
+    HIP_TRY(mark(LGCR_STAGE_CONVERT_INPUT));
     const int bw = (c->sw + 15) / 16, bh = (c->sh + 15) / 16;
     for (int k = 0; k < nn; ++k) {
         auto &q = w->nb[k];
-        RawPlane np[3] = {
-            {nbr_data[3 * k], nbr_stride[3 * k], c->sw, c->sh, q.y, yscale, 0.0},
-            {nbr_data[3 * k + 1], nbr_stride[3 * k + 1], c->cw, c->ch, q.u, yscale, coff},
-            {nbr_data[3 * k + 2], nbr_stride[3 * k + 2], c->cw, c->ch, q.v, yscale, coff},
-        };
-        HIP_TRY(launch_to_float(np, 3, c->raw_type, s));
+        if (!one_launch) {
+            RawPlane np[3] = {
+                {nbr_data[3 * k], nbr_stride[3 * k], c->sw, c->sh, q.y, yscale, 0.0},
+                {nbr_data[3 * k + 1], nbr_stride[3 * k + 1], c->cw, c->ch, q.u, yscale, coff},
+                {nbr_data[3 * k + 2], nbr_stride[3 * k + 2], c->cw, c->ch, q.v, yscale, coff},
+            };
+            HIP_TRY(launch_to_float(np, 3, c->raw_type, s));
+        }
         HIP_TRY(launch_block_match(w->y, q.y, c->sw, c->sh, 16, p.tsearch, float(p.tsad), q.mv, q.mv + size_t(bw) * bh,
                                    q.conf, s));
         HIP_TRY(launch_chroma_consistency(w->u, w->v, q.u, q.v, c->cw, c->ch, c->sw, c->sh, 16, q.mv,
