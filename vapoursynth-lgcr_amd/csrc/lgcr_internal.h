@@ -327,11 +327,12 @@ hipError_t launch_trust_mask_tiles(const uint64_t *seed, uint64_t *mask, int w, 
 // TRecon motion search and chroma consistency (algos.cpp:683-807)
 hipError_t launch_block_match(const float *cur, const float *nbr, int w, int h, int block,
                               int search, float tsad, int16_t *mvx, int16_t *mvy, float *conf,
-                              hipStream_t s);
+                              hipStream_t s, const GroupOff *go = nullptr, int nf = 1);
 hipError_t launch_chroma_consistency(const float *cu, const float *cv, const float *nu,
                                      const float *nv, int cw, int ch, int luma_w, int luma_h,
                                      int block, const int16_t *mvx, const int16_t *mvy, int bw,
-                                     int bh, float *conf, hipStream_t s);
+                                     int bh, float *conf, hipStream_t s, const GroupOff *go = nullptr,
+                                     int nf = 1);
 // algo 4 (algos.cpp:241-593)
 hipError_t launch_point_luma(const float *y, int w, int h, float *ls, int cw, int ch,
                              const int *xi, const float *xf, const int *yi, const float *yf,

@@ -2967,9 +2967,18 @@ __global__ __launch_bounds__(kBlock) void k_detail_transfer(WorkList L, const fl
 
 constexpr int kMeThreads = 256;
 
+template <bool GROUP>
 __global__ __launch_bounds__(kMeThreads) void k_block_match(const float *__restrict__ cur, const float *__restrict__ nbr,
                                                             int w, int h, int block, int search, float tsad,
-                                                            int16_t *mvx, int16_t *mvy, float *conf) {
+                                                            int16_t *mvx, int16_t *mvy, float *conf, GroupOff go) {
+    if constexpr (GROUP) {
+        // blockIdx.z: the neighbour; the centre frame is shared
+        const long long gd = group_off(go);
+        nbr = gshift(nbr, gd);
+        mvx = gshift(mvx, gd);
+        mvy = gshift(mvy, gd);
+        conf = gshift(conf, gd);
+    }
     __shared__ float sc[32 * 32];
     __shared__ float ksad[kMeThreads];
     __shared__ int kidx[kMeThreads];
@@ -3106,11 +3115,18 @@ __device__ __forceinline__ void me_sad_fixed(const float *sc, const float *win, 
     }
 }
 
-template <int WAVES>
+template <int WAVES, bool GROUP>
 __global__ __launch_bounds__(64 * WAVES) void k_block_match_win(const float *__restrict__ cur,
                                                                 const float *__restrict__ nbr, int w, int h, int block,
                                                                 int search, float tsad, int16_t *mvx, int16_t *mvy,
-                                                                float *conf) {
+                                                                float *conf, GroupOff go) {
+    if constexpr (GROUP) {
+        const long long gd = group_off(go);
+        nbr = gshift(nbr, gd);
+        mvx = gshift(mvx, gd);
+        mvy = gshift(mvy, gd);
+        conf = gshift(conf, gd);
+    }
     extern __shared__ float me_lds[];
     __shared__ float s_sad0[WAVES], s_best[WAVES];
     __shared__ int s_bidx[WAVES];
@@ -3263,10 +3279,20 @@ constexpr int kCcMax = 1024;
 
 constexpr int kCcThreads = 64;  // one wave per block
 
+template <bool GROUP>
 __global__ __launch_bounds__(kCcThreads) void k_chroma_consistency(const float *cu, const float *cv, const float *nu,
                                                                const float *nv, int cw, int ch, int luma_w,
                                                                int luma_h, int block, const int16_t *mvx,
-                                                               const int16_t *mvy, int bw, float *conf) {
+                                                               const int16_t *mvy, int bw, float *conf, GroupOff go) {
+    if constexpr (GROUP) {
+        // blockIdx.z: the neighbour; the centre frame is shared
+        const long long gd = group_off(go);
+        nu = gshift(nu, gd);
+        nv = gshift(nv, gd);
+        mvx = gshift(mvx, gd);
+        mvy = gshift(mvy, gd);
+        conf = gshift(conf, gd);
+    }
     __shared__ float delta[kCcMax];
     __shared__ float lo_val, mid_val;
     const int bi = blockIdx.x;
@@ -5026,30 +5052,44 @@ hipError_t launch_detail_transfer(const uint32_t *row_offsets, const uint32_t *i
 }
 
 hipError_t launch_block_match(const float *cur, const float *nbr, int w, int h, int block, int search, float tsad,
-                              int16_t *mvx, int16_t *mvy, float *conf, hipStream_t s) {
+                              int16_t *mvx, int16_t *mvy, float *conf, hipStream_t s, const GroupOff *go, int nf) {
     if (block > 64)
         return hipErrorInvalidValue;  // LDS block cache holds 32x32 subsampled samples
     const int bw = (w + block - 1) / block, bh = (h + block - 1) / block;
     const int half = (block + 1) / 2, ws = block + 2 * search;
     const size_t lds = sizeof(float) * ((size_t)half * half + (size_t)ws * ws);
     const unsigned nb = (unsigned)(bw * bh);
-    if (4 * lds <= (size_t)kMeLdsMax)
-        hipLaunchKernelGGL(k_block_match_win<4>, dim3((nb + 3) / 4), dim3(256), 4 * lds, s, cur, nbr, w, h, block,
-                           search, tsad, mvx, mvy, conf);
-    else if (lds <= (size_t)kMeLdsMax)
-        hipLaunchKernelGGL(k_block_match_win<1>, dim3(nb), dim3(64), lds, s, cur, nbr, w, h, block, search, tsad,
-                           mvx, mvy, conf);
+    const GroupOff g0 = go ? *go : GroupOff{};
+#define LGCR_ME(G_)                                                                                              \
+    {                                                                                                            \
+        if (4 * lds <= (size_t)kMeLdsMax)                                                                        \
+            hipLaunchKernelGGL((k_block_match_win<4, G_>), dim3((nb + 3) / 4, 1, (unsigned)nf), dim3(256), 4 * lds, \
+                               s, cur, nbr, w, h, block, search, tsad, mvx, mvy, conf, g0);                       \
+        else if (lds <= (size_t)kMeLdsMax)                                                                       \
+            hipLaunchKernelGGL((k_block_match_win<1, G_>), dim3(nb, 1, (unsigned)nf), dim3(64), lds, s, cur, nbr, w, \
+                               h, block, search, tsad, mvx, mvy, conf, g0);                                      \
+        else                                                                                                     \
+            hipLaunchKernelGGL((k_block_match<G_>), dim3(nb, 1, (unsigned)nf), dim3(kMeThreads), 0, s, cur, nbr, w, \
+                               h, block, search, tsad, mvx, mvy, conf, g0);                                      \
+    }
+    if (go)
+        LGCR_ME(true)
     else
-        hipLaunchKernelGGL(k_block_match, dim3((unsigned)(bw * bh)), dim3(kMeThreads), 0, s, cur, nbr, w, h, block,
-                           search, tsad, mvx, mvy, conf);
+        LGCR_ME(false)
+#undef LGCR_ME
     return hipGetLastError();
 }
 
 hipError_t launch_chroma_consistency(const float *cu, const float *cv, const float *nu, const float *nv, int cw,
                                      int ch, int luma_w, int luma_h, int block, const int16_t *mvx,
-                                     const int16_t *mvy, int bw, int bh, float *conf, hipStream_t s) {
-    hipLaunchKernelGGL(k_chroma_consistency, dim3((unsigned)(bw * bh)), dim3(kCcThreads), 0, s, cu, cv, nu, nv, cw, ch,
-                       luma_w, luma_h, block, mvx, mvy, bw, conf);
+                                     const int16_t *mvy, int bw, int bh, float *conf, hipStream_t s, const GroupOff *go,
+                                     int nf) {
+    if (go)
+        hipLaunchKernelGGL(k_chroma_consistency<true>, dim3((unsigned)(bw * bh), 1, (unsigned)nf), dim3(kCcThreads), 0,
+                           s, cu, cv, nu, nv, cw, ch, luma_w, luma_h, block, mvx, mvy, bw, conf, *go);
+    else
+        hipLaunchKernelGGL(k_chroma_consistency<false>, dim3((unsigned)(bw * bh)), dim3(kCcThreads), 0, s, cu, cv, nu,
+                           nv, cw, ch, luma_w, luma_h, block, mvx, mvy, bw, conf, GroupOff{});
     return hipGetLastError();
 }
 

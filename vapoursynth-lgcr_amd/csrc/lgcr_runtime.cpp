@@ -1989,7 +1989,35 @@ lgcr_status enqueue_trecon(lgcr_clip *c, Workspace *w, const Geometry &g, const 
         HIP_TRY(launch_to_float(rp, 3, c->raw_type, s));
     HIP_TRY(mark(LGCR_STAGE_CONVERT_INPUT));
     const int bw = (c->sw + 15) / 16, bh = (c->sh + 15) / 16;
-    for (int k = 0; k < nn; ++k) {
+    // The neighbours' buffers are carved one neighbour after another with
+    // the same sizes, so neighbour k's copy of each lies the same distance
+    // from neighbour 0's: then the motion search, the chroma check and the
+    // footprint luma of every neighbour are one launch each.
+    GroupOff ngo{};
+    bool nb_group = one_launch && nn > 1;
+    for (int k = 1; nb_group && k < nn; ++k) {
+        const auto &q0 = w->nb[0], &q = w->nb[k];
+        auto dist = [](const void *a, const void *b) {
+            return (long long)(reinterpret_cast<uintptr_t>(a) - reinterpret_cast<uintptr_t>(b));
+        };
+        ngo.d[k] = dist(q.y, q0.y);
+        nb_group = dist(q.u, q0.u) == ngo.d[k] && dist(q.v, q0.v) == ngo.d[k] && dist(q.lc, q0.lc) == ngo.d[k] &&
+                   dist(q.conf, q0.conf) == ngo.d[k] && dist(q.mv, q0.mv) == ngo.d[k];
+    }
+    if (nb_group) {
+        const auto &q = w->nb[0];
+        HIP_TRY(launch_block_match(w->y, q.y, c->sw, c->sh, 16, p.tsearch, float(p.tsad), q.mv, q.mv + size_t(bw) * bh,
+                                   q.conf, s, &ngo, nn));
+        HIP_TRY(launch_chroma_consistency(w->u, w->v, q.u, q.v, c->cw, c->ch, c->sw, c->sh, 16, q.mv,
+                                          q.mv + size_t(bw) * bh, bw, bh, q.conf, s, &ngo, nn));
+        HIP_TRY(launch_lc_map(q.y, c->sw, c->sh, q.lc, c->cw, c->ch, g.xb, g.xe, g.yb, g.ye, s, nullptr, nullptr,
+                              nullptr, &ngo, nn));
+        for (int k = 0; k < nn; ++k) {
+            const auto &r = w->nb[k];
+            w->nbr_host[k] = NbrDev{r.u, r.v, r.lc, r.mv, r.mv + size_t(bw) * bh, r.conf, bw, bh, 16};
+        }
+    }
+    for (int k = 0; !nb_group && k < nn; ++k) {
         auto &q = w->nb[k];
         if (!one_launch) {
             RawPlane np[3] = {
