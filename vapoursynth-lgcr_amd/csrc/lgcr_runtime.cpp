@@ -855,6 +855,7 @@ struct Workspace {
     std::vector<Nbr> nb;
     NbrDev *nbr_dev = nullptr;
     std::vector<NbrDev> nbr_host;
+    std::vector<NbrDev> nbr_on_dev;  // what nbr_dev holds (its entries depend only on this workspace)
     void *nbr_raw = nullptr;  // staging of host neighbour frames
     // algo 6
     float *yct = nullptr, *yc1 = nullptr, *yc2 = nullptr, *aff = nullptr, *rec = nullptr, *rec2 = nullptr,
@@ -2034,8 +2035,12 @@ lgcr_status enqueue_trecon(lgcr_clip *c, Workspace *w, const Geometry &g, const 
         HIP_TRY(launch_lc_map(q.y, c->sw, c->sh, q.lc, c->cw, c->ch, g.xb, g.xe, g.yb, g.ye, s));
         w->nbr_host[k] = NbrDev{q.u, q.v, q.lc, q.mv, q.mv + size_t(bw) * bh, q.conf, bw, bh, 16};
     }
-    if (nn > 0)
-        HIP_TRY(hipMemcpyAsync(w->nbr_dev, w->nbr_host.data(), sizeof(NbrDev) * nn, hipMemcpyHostToDevice, s));
+    // the table reaches the device once per workspace unless it changes
+    if (nn > 0 && ((int)w->nbr_on_dev.size() < nn ||
+                   std::memcmp(w->nbr_on_dev.data(), w->nbr_host.data(), sizeof(NbrDev) * nn) != 0)) {
+            HIP_TRY(hipMemcpyAsync(w->nbr_dev, w->nbr_host.data(), sizeof(NbrDev) * nn, hipMemcpyHostToDevice, s));
+            w->nbr_on_dev.assign(w->nbr_host.begin(), w->nbr_host.begin() + nn);
+        }
     // the guided pass gathers U/V (and lc) from the interleaved plane even
     // at strength 0
     HIP_TRY(launch_lc_map(w->y, c->sw, c->sh, w->lc, c->cw, c->ch, g.xb, g.xe, g.yb, g.ye, s, w->u, w->v, w->luv));
