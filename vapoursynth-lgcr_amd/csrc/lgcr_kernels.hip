@@ -186,6 +186,13 @@ __device__ Dir principal_dir(float jxx, float jxy, float jyy) {
 struct InBatch { RawPlane p[3]; };
 struct OutBatch { RawOut p[3]; };
 
+// A device pointer read from memory (a table entry) as a global-memory
+// pointer, so its loads are global rather than flat ones.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *gmem(const T *p) {
+    return (const __attribute__((address_space(1))) T *)p;
+}
+
 // plane address and stride of a conversion: by value, or from the frame
 // table of a captured frame graph
 __device__ __forceinline__ const char *plane_in(const RawPlane &P, ptrdiff_t &stride) {
@@ -216,7 +223,16 @@ __global__ void k_io_advance(IoTable *io) {
 // four at a time so several loads are in flight per lane.
 constexpr int kConvUnroll = 4;
 
-// One row of one plane.  COPY: float input, copied bit for bit.
+// Four samples of one type, loaded together.
+template <typename T>
+struct alignas(4 * sizeof(T)) Quad {
+    T v[4];
+};
+
+// One row of one plane.  COPY: float input, copied bit for bit.  Rows whose
+// source and destination are aligned for it move four samples per lane and
+// load (the tail and unaligned rows go one sample at a time); the value of
+// every sample is the same expression either way.
 template <typename T, bool COPY>
 __device__ __forceinline__ void to_float_row(const RawPlane &P, int y) {
     ptrdiff_t stride;
@@ -224,18 +240,37 @@ __device__ __forceinline__ void to_float_row(const RawPlane &P, int y) {
     const T *row = reinterpret_cast<const T *>(base + stride * y);
     float *dst = P.dst + (size_t)y * P.w;
     const double scale = P.scale, offset = P.offset;
-    for (int x0 = threadIdx.x; x0 < P.w; x0 += kBlock * kConvUnroll) {
+    auto cv = [&](T v) { return COPY ? (float)v : (float)((double)v * scale + offset); };
+    int x_tail = 0;
+    if ((reinterpret_cast<uintptr_t>(row) % sizeof(Quad<T>)) == 0 && (reinterpret_cast<uintptr_t>(dst) % 16) == 0) {
+        const int n4 = P.w / 4;
+        const Quad<T> *src4 = reinterpret_cast<const Quad<T> *>(row);
+        float4 *dst4 = reinterpret_cast<float4 *>(dst);
+        for (int i0 = threadIdx.x; i0 < n4; i0 += kBlock * 2) {
+            Quad<T> q[2];
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+                if (i0 + k * kBlock < n4)
+                    q[k] = src4[i0 + k * kBlock];
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+                if (i0 + k * kBlock < n4)
+                    dst4[i0 + k * kBlock] = make_float4(cv(q[k].v[0]), cv(q[k].v[1]), cv(q[k].v[2]), cv(q[k].v[3]));
+        }
+        x_tail = 4 * n4;
+    }
+    for (int x0 = x_tail + (int)threadIdx.x; x0 < P.w; x0 += kBlock * kConvUnroll) {
         T v[kConvUnroll];
 #pragma unroll
         for (int i = 0; i < kConvUnroll; ++i) {
             const int x = x0 + i * kBlock;
-            v[i] = x < P.w ? row[x] : T(0);
+            v[i] = x < P.w ? gmem(row)[x] : T(0);
         }
 #pragma unroll
         for (int i = 0; i < kConvUnroll; ++i) {
             const int x = x0 + i * kBlock;
             if (x < P.w)
-                dst[x] = COPY ? (float)v[i] : (float)((double)v[i] * scale + offset);
+                dst[x] = cv(v[i]);
         }
     }
 }
@@ -286,12 +321,6 @@ __device__ __forceinline__ const P *gshift(const P *p, long long d) {
 template <class P>
 __device__ __forceinline__ P *gshift(P *p, long long d) {
     return p ? reinterpret_cast<P *>(reinterpret_cast<char *>(p) + d) : p;
-}
-// A device pointer read from memory (a table entry) as a global-memory
-// pointer, so its loads are global rather than flat ones.
-template <class T>
-__device__ __forceinline__ const __attribute__((address_space(1))) T *gmem(const T *p) {
-    return (const __attribute__((address_space(1))) T *)p;
 }
 
 template <typename T> __device__ __forceinline__ T pack_sample(float s, const RawOut &P) {
