@@ -328,17 +328,46 @@ template <typename T> __device__ __forceinline__ T pack_sample(float s, const Ra
     return (T)clampi(v, P.lo, P.hi);
 }
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_from_float(OutBatch b) {
-    const RawOut &P = b.p[blockIdx.z];
-    const int y = blockIdx.y;
-    if (y >= P.h)
-        return;
+// One output row of one plane.  COPY: float output, copied bit for bit;
+// otherwise packed (floatToPlane).  Aligned rows move four samples per lane
+// load and store, the tail and unaligned rows one sample at a time.
+template <typename T, bool COPY>
+__device__ __forceinline__ void from_float_row(const RawOut &P, int y) {
     ptrdiff_t stride;
     char *base = plane_out(P, stride);
     T *row = reinterpret_cast<T *>(base + stride * y);
     const float *src = P.src + (size_t)y * P.w;
-    for (int x0 = threadIdx.x; x0 < P.w; x0 += kBlock * kConvUnroll) {
+    auto pk = [&](float v) -> T {
+        if constexpr (COPY)
+            return v;
+        else
+            return pack_sample<T>(v, P);
+    };
+    int x_tail = 0;
+    if ((reinterpret_cast<uintptr_t>(row) % sizeof(Quad<T>)) == 0 && (reinterpret_cast<uintptr_t>(src) % 16) == 0) {
+        const int n4 = P.w / 4;
+        const float4 *src4 = reinterpret_cast<const float4 *>(src);
+        Quad<T> *dst4 = reinterpret_cast<Quad<T> *>(row);
+        for (int i0 = threadIdx.x; i0 < n4; i0 += kBlock * 2) {
+            float4 q[2];
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+                if (i0 + k * kBlock < n4)
+                    q[k] = src4[i0 + k * kBlock];
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+                if (i0 + k * kBlock < n4) {
+                    Quad<T> o;
+                    o.v[0] = pk(q[k].x);
+                    o.v[1] = pk(q[k].y);
+                    o.v[2] = pk(q[k].z);
+                    o.v[3] = pk(q[k].w);
+                    dst4[i0 + k * kBlock] = o;
+                }
+        }
+        x_tail = 4 * n4;
+    }
+    for (int x0 = x_tail + (int)threadIdx.x; x0 < P.w; x0 += kBlock * kConvUnroll) {
         float v[kConvUnroll];
 #pragma unroll
         for (int i = 0; i < kConvUnroll; ++i) {
@@ -349,34 +378,22 @@ __global__ __launch_bounds__(kBlock) void k_from_float(OutBatch b) {
         for (int i = 0; i < kConvUnroll; ++i) {
             const int x = x0 + i * kBlock;
             if (x < P.w)
-                row[x] = pack_sample<T>(v[i], P);
+                row[x] = pk(v[i]);
         }
     }
 }
 
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_from_float(OutBatch b) {
+    const RawOut &P = b.p[blockIdx.z];
+    if ((int)blockIdx.y < P.h)
+        from_float_row<T, false>(P, blockIdx.y);
+}
+
 __global__ __launch_bounds__(kBlock) void k_from_float_f32(OutBatch b) {
     const RawOut &P = b.p[blockIdx.z];
-    const int y = blockIdx.y;
-    if (y >= P.h)
-        return;
-    ptrdiff_t stride;
-    char *base = plane_out(P, stride);
-    float *row = reinterpret_cast<float *>(base + stride * y);
-    const float *src = P.src + (size_t)y * P.w;
-    for (int x0 = threadIdx.x; x0 < P.w; x0 += kBlock * kConvUnroll) {
-        float v[kConvUnroll];
-#pragma unroll
-        for (int i = 0; i < kConvUnroll; ++i) {
-            const int x = x0 + i * kBlock;
-            v[i] = x < P.w ? src[x] : 0.0f;
-        }
-#pragma unroll
-        for (int i = 0; i < kConvUnroll; ++i) {
-            const int x = x0 + i * kBlock;
-            if (x < P.w)
-                row[x] = v[i];
-        }
-    }
+    if ((int)blockIdx.y < P.h)
+        from_float_row<float, true>(P, blockIdx.y);
 }
 
 // Back-projection fused with output packing (backProject, algos.cpp:663-677,
